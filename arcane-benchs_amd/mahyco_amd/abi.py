@@ -1,0 +1,309 @@
+"""ctypes mirror of include/mahyco_gpu.h.
+
+The structures here match the C structures field for field; `Library` binds
+every entry point of a shared library that implements the ABI under a given
+symbol prefix, and `Simulation` owns one context plus numpy views of the
+state.  Nothing in this module computes physics: every number comes from the
+C-ABI library it wraps.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+MAX_ENV = 8
+MAX_BC = 16
+
+OK, STOP = 0, 1
+E_ARG, E_DT_MIN, E_HIP, E_STATE, E_UNSUPPORTED, E_ALLOC = -1, -2, -3, -4, -5, -6
+ERRORS = {E_ARG: "bad argument", E_DT_MIN: "dt below deltat-min", E_HIP: "HIP error",
+          E_STATE: "call out of order", E_UNSUPPORTED: "unsupported option",
+          E_ALLOC: "device allocation failed"}
+
+BC_VX, BC_VY, BC_VZ = 0, 1, 2
+SURF = {"XMIN": 0, "XMAX": 1, "YMIN": 2, "YMAX": 3, "ZMIN": 4, "ZMAX": 5}
+EOS_PERFECT_GAS, EOS_STIFFENED_GAS = 0, 1
+LC_RACINE_CUBIQUE_VOLUME, LC_FACES_OPPOSEES = 0, 1
+PREC_FP64, PREC_FP32_BULK = 0, 1
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("dim", C.c_int32), ("n", C.c_int32 * 3),
+                ("origin", C.c_double * 3), ("length", C.c_double * 3)]
+
+
+class BC(C.Structure):
+    _fields_ = [("surface", C.c_int32), ("type", C.c_int32), ("value", C.c_double)]
+
+
+class EnvDesc(C.Structure):
+    _fields_ = [("eos", C.c_int32), ("adiabatic_cst", C.c_double),
+                ("limit_tension", C.c_double)]
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("cfl", C.c_double), ("threshold", C.c_double), ("deltat_init", C.c_double),
+        ("deltat_min", C.c_double), ("deltat_max", C.c_double), ("final_time", C.c_double),
+        ("with_projection", C.c_int32), ("pseudo_centree", C.c_int32),
+        ("schema_csts", C.c_int32), ("with_newton", C.c_int32), ("sans_lagrange", C.c_int32),
+        ("longueur_caracteristique", C.c_int32),
+        ("nb_bc", C.c_int32), ("bc", BC * MAX_BC),
+        ("nb_env", C.c_int32), ("env", EnvDesc * MAX_ENV),
+        ("remap_threshold", C.c_double), ("is_euler_scheme", C.c_int32),
+        ("ordre_projection", C.c_int32), ("projection_limiteur_id", C.c_int32),
+        ("projection_pente_borne", C.c_int32), ("conservation_energie_totale", C.c_int32),
+        ("visc_quad_coeff", C.c_double), ("precision", C.c_int32),
+        ("reserved", C.c_int32 * 7),
+    ]
+
+
+_DP = C.POINTER(C.c_double)
+_IP = C.POINTER(C.c_int32)
+
+STATE_CELL = ["density", "internal_energy", "pressure", "sound_speed", "pseudo_viscosity",
+              "cell_mass", "fracvol", "mass_fraction"]
+STATE_ENV = [f + "_env" for f in STATE_CELL]
+
+
+class State(C.Structure):
+    _fields_ = ([(f, _DP) for f in STATE_CELL] + [("env_mask", _IP)] +
+                [(f, _DP) for f in STATE_ENV] +
+                [("velocity", _DP), ("node_mass", _DP), ("node_coord", _DP),
+                 ("dual_grad_phi", _DP), ("caracteristic_length", _DP),
+                 ("global_deltat", C.c_double), ("old_deltat", C.c_double),
+                 ("global_time", C.c_double), ("sens_projection", C.c_int32),
+                 ("global_iteration", C.c_int32)])
+
+
+ENTRY_POINTS = ["begin_iteration", "save_values_at_n", "update_velocity",
+                "apply_boundary_condition", "update_position", "compute_geometric_values",
+                "update_density", "compute_artificial_viscosity",
+                "update_energy_and_pressure", "remap"]
+
+
+class MahycoError(RuntimeError):
+    def __init__(self, code, msg=""):
+        super().__init__(f"{ERRORS.get(code, code)} ({code}): {msg}")
+        self.code = code
+
+
+class Library:
+    """Binds a shared library implementing the mahyco_gpu.h ABI under `prefix`."""
+
+    def __init__(self, path, prefix="mhg_"):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"C-ABI library not built: {path}")
+        self.path = path
+        self.prefix = prefix
+        self.lib = C.CDLL(path)
+        p = prefix
+        L = self.lib
+        vp = C.c_void_p
+
+        def fn(name, res, args):
+            f = getattr(L, p + name)
+            f.restype = res
+            f.argtypes = args
+            return f
+
+        self.abi_version = fn("abi_version", C.c_int, [])
+        self.default_options = fn("default_options", None, [C.POINTER(Options)])
+        self.create = fn("create", C.c_int, [C.POINTER(MeshDesc), C.POINTER(Options), C.POINTER(vp)])
+        self.destroy = fn("destroy", C.c_int, [vp])
+        self.last_error = fn("last_error", C.c_char_p, [vp])
+        self.sizes = fn("sizes", C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)])
+        self.init_sod = fn("init_sod", C.c_int, [vp, C.c_int32])
+        self.init_triple_point = fn("init_triple_point", C.c_int, [vp, _DP, _DP])
+        self.init_cells = fn("init_cells", C.c_int, [vp, _IP, _DP, _DP, _DP])
+        self.upload_state = fn("upload_state", C.c_int, [vp, C.POINTER(State)])
+        self.download_state = fn("download_state", C.c_int, [vp, C.POINTER(State)])
+        for e in ENTRY_POINTS:
+            setattr(self, e, fn(e, C.c_int, [vp]))
+        self.compute_delta_t = fn("compute_delta_t", C.c_int, [vp, _DP, _IP])
+        self.step = fn("step", C.c_int, [vp, _IP])
+        self.run = fn("run", C.c_int, [vp, C.c_int32, _IP])
+        self.conservation = fn("conservation", C.c_int, [vp, _DP])
+        for g in ["remap_get_ordre_projection", "remap_has_projection_pente_borne",
+                  "remap_has_conservation_energie_totale", "remap_is_euler"]:
+            setattr(self, g, fn(g, C.c_int, [vp]))
+        self.optional = {}
+        for name, res, args in [
+                ("run_async", C.c_int, [vp, C.c_int32]),
+                ("synchronize", C.c_int, [vp]),
+                ("kernel_times", C.c_int, [vp, C.c_int32, C.POINTER(C.c_char_p), _DP, _IP, C.c_int32]),
+                ("set_threads", C.c_int, [C.c_int32])]:
+            if hasattr(L, p + name):
+                self.optional[name] = fn(name, res, args)
+
+    def options(self, **kw):
+        o = Options()
+        self.default_options(C.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        return o
+
+
+def set_envs(opt, envs):
+    """envs: list of (eos, gamma, limit_tension)."""
+    opt.nb_env = len(envs)
+    for i, (eos, g, t) in enumerate(envs):
+        opt.env[i].eos = eos
+        opt.env[i].adiabatic_cst = g
+        opt.env[i].limit_tension = t
+    return opt
+
+
+def set_bcs(opt, bcs):
+    """bcs: list of (surface name, type, value)."""
+    opt.nb_bc = len(bcs)
+    for i, (s, t, v) in enumerate(bcs):
+        opt.bc[i].surface = SURF[s]
+        opt.bc[i].type = t
+        opt.bc[i].value = v
+    return opt
+
+
+def reflecting_walls(dim):
+    b = [("XMIN", BC_VX, 0.), ("XMAX", BC_VX, 0.), ("YMIN", BC_VY, 0.), ("YMAX", BC_VY, 0.)]
+    if dim == 3:
+        b += [("ZMIN", BC_VZ, 0.), ("ZMAX", BC_VZ, 0.)]
+    return b
+
+
+def mesh_desc(dim, n, origin, length):
+    m = MeshDesc()
+    m.dim = dim
+    for d in range(3):
+        m.n[d] = n[d] if d < len(n) else 1
+        m.origin[d] = origin[d] if d < len(origin) else 0.
+        m.length[d] = length[d] if d < len(length) else 1.
+    return m
+
+
+class Simulation:
+    """One context of a C-ABI library (GPU product library or, in tests, the oracle)."""
+
+    def __init__(self, library, mesh, options):
+        self.L = library
+        self.mesh = mesh
+        self.opt = options
+        self.dim = mesh.dim
+        self.nenv = options.nb_env
+        ctx = C.c_void_p()
+        r = self.L.create(C.byref(mesh), C.byref(options), C.byref(ctx))
+        if r != OK:
+            raise MahycoError(r, "create")
+        self.ctx = ctx
+        nc, nn = C.c_int64(), C.c_int64()
+        self.L.sizes(self.ctx, C.byref(nc), C.byref(nn))
+        self.nc, self.nn = nc.value, nn.value
+
+    def close(self):
+        if self.ctx:
+            self.L.destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, r, what):
+        if r < 0:
+            msg = self.L.last_error(self.ctx)
+            raise MahycoError(r, f"{what}: {msg.decode() if msg else ''}")
+        return r
+
+    # -- init --
+    def init_sod(self, cas_test=3):
+        return self._check(self.L.init_sod(self.ctx, cas_test), "init_sod")
+
+    def init_triple_point(self, rho=(1.0, 1.0, 0.125), p=(1.0, 0.1, 0.1)):
+        r = np.ascontiguousarray(rho, dtype=np.float64)
+        q = np.ascontiguousarray(p, dtype=np.float64)
+        return self._check(self.L.init_triple_point(self.ctx, r.ctypes.data_as(_DP),
+                                                    q.ctypes.data_as(_DP)), "init_triple_point")
+
+    def init_cells(self, env_of_cell, density, pressure, velocity=None):
+        e = np.ascontiguousarray(env_of_cell, dtype=np.int32).ravel()
+        d = np.ascontiguousarray(density, dtype=np.float64).ravel()
+        p = np.ascontiguousarray(pressure, dtype=np.float64).ravel()
+        assert e.size == self.nc and d.size == self.nc and p.size == self.nc
+        vp = None
+        if velocity is not None:
+            v = np.ascontiguousarray(velocity, dtype=np.float64).reshape(self.dim, self.nn)
+            vp = v.ctypes.data_as(_DP)
+        return self._check(self.L.init_cells(self.ctx, e.ctypes.data_as(_IP), d.ctypes.data_as(_DP),
+                                             p.ctypes.data_as(_DP), vp), "init_cells")
+
+    # -- state --
+    def alloc_state(self):
+        nc, nn, ne, dim = self.nc, self.nn, self.nenv, self.dim
+        a = {f: np.zeros(nc) for f in STATE_CELL}
+        a.update({f: np.zeros((ne, nc)) for f in STATE_ENV})
+        a["env_mask"] = np.zeros(nc, dtype=np.int32)
+        a["velocity"] = np.zeros((dim, nn))
+        a["node_mass"] = np.zeros(nn)
+        a["node_coord"] = np.zeros((dim, nn))
+        a["dual_grad_phi"] = np.zeros((dim, nn))
+        a["caracteristic_length"] = np.zeros(nc)
+        return a
+
+    @staticmethod
+    def _fill_struct(arrays, scal=None):
+        st = State()
+        for name, _ in State._fields_:
+            if name in arrays and arrays[name] is not None:
+                arr = arrays[name]
+                assert arr.flags.c_contiguous
+                ptype = _IP if arr.dtype == np.int32 else _DP
+                setattr(st, name, arr.ctypes.data_as(ptype))
+        if scal:
+            for k, v in scal.items():
+                setattr(st, k, v)
+        return st
+
+    def download(self, fields=None):
+        a = self.alloc_state()
+        if fields is not None:
+            a = {k: v for k, v in a.items() if k in fields}
+        st = self._fill_struct(a)
+        self._check(self.L.download_state(self.ctx, C.byref(st)), "download_state")
+        for k in ["global_deltat", "old_deltat", "global_time", "sens_projection",
+                  "global_iteration"]:
+            a[k] = getattr(st, k)
+        return a
+
+    def upload(self, a):
+        arrays = {k: np.ascontiguousarray(v) for k, v in a.items() if isinstance(v, np.ndarray)}
+        scal = {k: a[k] for k in ["global_deltat", "old_deltat", "global_time",
+                                  "sens_projection", "global_iteration"] if k in a}
+        st = self._fill_struct(arrays, scal)
+        return self._check(self.L.upload_state(self.ctx, C.byref(st)), "upload_state")
+
+    # -- loop --
+    def entry(self, name):
+        return self._check(getattr(self.L, name)(self.ctx), name)
+
+    def compute_delta_t(self):
+        dt = C.c_double()
+        stop = C.c_int32()
+        self._check(self.L.compute_delta_t(self.ctx, C.byref(dt), C.byref(stop)), "compute_delta_t")
+        return dt.value, bool(stop.value)
+
+    def step(self):
+        stop = C.c_int32()
+        r = self._check(self.L.step(self.ctx, C.byref(stop)), "step")
+        return bool(stop.value) or r == STOP
+
+    def run(self, max_steps):
+        done = C.c_int32()
+        self._check(self.L.run(self.ctx, max_steps, C.byref(done)), "run")
+        return done.value
+
+    def conservation(self):
+        out = np.zeros(3)
+        self._check(self.L.conservation(self.ctx, out.ctypes.data_as(_DP)), "conservation")
+        return out

@@ -1,0 +1,217 @@
+/*
+ * mahyco_gpu.h -- C-ABI of the MI355X (gfx950) MaHyCo Lagrange + Euler-ADI
+ * remap time step.
+ *
+ * Everything here is plain C: pointers, sizes, int status codes.  No HIP or
+ * torch type crosses the boundary.  A context owns all device memory; the
+ * caller owns every host array it passes and the library never keeps a host
+ * pointer after a call returns.  A context is not thread-safe.  All calls are
+ * host-synchronous unless their name ends in _async.
+ *
+ * Reference interfaces replaced (paths relative to the reference root):
+ *   - the MahycoModule entry points of the compute loop
+ *       mahyco/src/Mahyco.config:30-41, declared in mahyco/src/Mahyco.axl:150-168
+ *       and mahyco/src/MahycoModule.h:145-381;
+ *   - the IRemap service (mahyco/src/Remap/IRemap.h:32-65) with its RemapADI
+ *     implementation (mahyco/src/Remap/RemapADIService.cc:19-60);
+ *   - the IEquationOfState services (mahyco/src/eos/IEquationOfState.h:32-65):
+ *     PerfectGas (mahyco/src/eos/perfectgas/PerfectGasEOSService.cc:20-137) and
+ *     StiffenedGas (mahyco/src/eos/stiffenedgas/StiffenedGasEOSService.cc:10-59);
+ *   - the option surface of Mahyco.axl:170-340, RemapADI.axl:136-178,
+ *     PerfectGasEOS.axl:99-105.
+ *
+ * Error behaviour: the reference calls exit(1) when dt falls below deltat-min
+ * (mahyco/src/MahycoModule.cc:2152-2159) and stopComputeLoop(true) at the end
+ * of the run (MahycoModule.cc:2177-2183).  Here these become the return codes
+ * MHG_E_DT_MIN and MHG_STOP.
+ */
+#ifndef MAHYCO_GPU_H
+#define MAHYCO_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MHG_ABI_VERSION 1
+#define MHG_MAX_ENV 8
+#define MHG_MAX_BC 16
+
+/* Status codes. */
+enum {
+  MHG_OK = 0,
+  MHG_STOP = 1,           /* end of run reached (stopComputeLoop)           */
+  MHG_E_ARG = -1,         /* bad argument / inconsistent sizes              */
+  MHG_E_DT_MIN = -2,      /* dt < deltat-min (reference exits with code 1)  */
+  MHG_E_HIP = -3,         /* HIP runtime error                              */
+  MHG_E_STATE = -4,       /* call out of order (e.g. step before init)      */
+  MHG_E_UNSUPPORTED = -5, /* option value outside the GPU path              */
+  MHG_E_ALLOC = -6        /* device allocation failed                       */
+};
+
+/* Cartesian mesh as built by Arcane's cartesian generator
+ * (mahyco/data/Data.1.arc:13-24): n[d] cells of uniform size length[d]/n[d]
+ * from origin[d].  n[2] and the z entries are ignored when dim == 2. */
+typedef struct {
+  int32_t dim;
+  int32_t n[3];
+  double origin[3];
+  double length[3];
+} mhg_mesh_desc;
+
+/* Boundary conditions (Mahyco.axl:312-329, TypesMahyco.h:11-17).  The face
+ * groups are the ones PrepareFaceGroup builds (MahycoAnnexe.cc:183-251). */
+enum { MHG_BC_VX = 0, MHG_BC_VY = 1, MHG_BC_VZ = 2 };
+enum {
+  MHG_SURF_XMIN = 0, MHG_SURF_XMAX = 1, MHG_SURF_YMIN = 2,
+  MHG_SURF_YMAX = 3, MHG_SURF_ZMIN = 4, MHG_SURF_ZMAX = 5
+};
+typedef struct {
+  int32_t surface;
+  int32_t type;
+  double value;
+} mhg_bc;
+
+/* Equation of state of one environment (<eos-model name=...>). */
+enum { MHG_EOS_PERFECT_GAS = 0, MHG_EOS_STIFFENED_GAS = 1 };
+typedef struct {
+  int32_t eos;
+  double adiabatic_cst;  /* PerfectGasEOS.axl:101 default 1.4 */
+  double limit_tension;  /* PerfectGasEOS.axl:104 default 0.0 */
+} mhg_env_desc;
+
+/* Characteristic length (Mahyco.axl "longueur-caracteristique"). */
+enum { MHG_LC_RACINE_CUBIQUE_VOLUME = 0, MHG_LC_FACES_OPPOSEES = 1 };
+
+/* Limiters (TypesMahyco.h:65-74). */
+enum { MHG_LIM_MINMOD = 0, MHG_LIM_SUPERBEE = 1, MHG_LIM_VANLEER = 2, MHG_LIM_MINMODG = 3 };
+
+/* Arithmetic of the bulk fields. */
+enum { MHG_PREC_FP64 = 0, MHG_PREC_FP32_BULK = 1 };
+
+typedef struct {
+  /* Mahyco.axl */
+  double cfl;               /* default 0.3   */
+  double threshold;         /* default 1e-10 */
+  double deltat_init;
+  double deltat_min;
+  double deltat_max;
+  double final_time;
+  int32_t with_projection;  /* default 1 */
+  int32_t pseudo_centree;   /* default 0 */
+  int32_t schema_csts;      /* must be 0 (host-only path in the reference) */
+  int32_t with_newton;      /* must be 0 (host-only path in the reference) */
+  int32_t sans_lagrange;    /* must be 0 */
+  int32_t longueur_caracteristique;
+  int32_t nb_bc;
+  mhg_bc bc[MHG_MAX_BC];
+  int32_t nb_env;
+  mhg_env_desc env[MHG_MAX_ENV];
+  /* RemapADI.axl */
+  double remap_threshold;             /* default 1e-10 */
+  int32_t is_euler_scheme;            /* must be 1     */
+  int32_t ordre_projection;           /* 1 or 2        */
+  int32_t projection_limiteur_id;     /* < MHG_LIM_MINMODG */
+  int32_t projection_pente_borne;     /* must be 0     */
+  int32_t conservation_energie_totale;/* must be 0     */
+  /* Extensions (no counterpart in the reference). */
+  double visc_quad_coeff;   /* < 0: reference value (gamma+1)/2 */
+  int32_t precision;        /* MHG_PREC_FP64 (the drop-in) or MHG_PREC_FP32_BULK */
+  int32_t reserved[7];
+} mhg_options;
+
+/* Full per-step state, structure-of-arrays, caller-owned host memory.
+ * Any pointer may be NULL (that field is skipped).  Sizes: N_c cells,
+ * N_n nodes (see mhg_sizes).  "env" arrays are [nb_env][N_c] and only
+ * meaningful where the cell holds two or more environments; for a cell with
+ * one environment the environment value is the cell value (Arcane's pure-cell
+ * aliasing, mahyco/src/accenv/MultiEnvUtils.h:303-440). */
+typedef struct {
+  double *density, *internal_energy, *pressure, *sound_speed;
+  double *pseudo_viscosity, *cell_mass;
+  double *fracvol, *mass_fraction;
+  int32_t *env_mask;                 /* bit k <=> cell holds environment k */
+  double *density_env, *internal_energy_env, *pressure_env, *sound_speed_env;
+  double *pseudo_viscosity_env, *cell_mass_env, *fracvol_env, *mass_fraction_env;
+  double *velocity;                  /* [dim][N_n] */
+  double *node_mass;                 /* [N_n]      */
+  double *node_coord;                /* [dim][N_n] coordinates at the end of the last step */
+  double *dual_grad_phi;             /* [dim][N_n] persistent limited nodal gradient      */
+  double *caracteristic_length;      /* [N_c] L of the last Lagrange phase (output only)  */
+  double global_deltat, old_deltat, global_time;
+  int32_t sens_projection, global_iteration;
+} mhg_state;
+
+typedef struct mhg_ctx mhg_ctx;
+
+/* ---- life cycle ------------------------------------------------------- */
+int mhg_abi_version(void);
+void mhg_default_options(mhg_options* opt);
+int mhg_create(const mhg_mesh_desc* mesh, const mhg_options* opt, mhg_ctx** out);
+int mhg_destroy(mhg_ctx* ctx);
+const char* mhg_last_error(const mhg_ctx* ctx);
+int mhg_sizes(const mhg_ctx* ctx, int64_t* n_cells, int64_t* n_nodes);
+
+/* ---- initialisation (hydroStartInit + ComputeCellMass + ComputeNodeMass,
+ *      mahyco/src/MahycoModule.cc:48-266) ------------------------------- */
+/* SODService (mahyco/src/casTest/SOD/SODService.cc:3-135): cas_test 3/4/5
+ * mono-material tube along x/y/z, 13/14/15 two-material tube. */
+int mhg_init_sod(mhg_ctx* ctx, int32_t cas_test);
+/* Three-environment triple point (TypesMahyco.h:26 names it; the reference
+ * has no service for it): x<1: env 0; x>1,y<1.5: env 1; x>1,y>1.5: env 2. */
+int mhg_init_triple_point(mhg_ctx* ctx, const double rho[3], const double p[3]);
+/* Generic pure-cell initial state: env index, density and pressure per cell,
+ * velocity [dim][N_n] (NULL = at rest). */
+int mhg_init_cells(mhg_ctx* ctx, const int32_t* env_of_cell, const double* density,
+                   const double* pressure, const double* velocity);
+
+int mhg_upload_state(mhg_ctx* ctx, const mhg_state* st);
+int mhg_download_state(mhg_ctx* ctx, mhg_state* st);
+
+/* ---- compute loop, one call per Mahyco.config entry point ------------ */
+/* Arcane's time-loop manager increments time by dt and the iteration count
+ * before the compute-loop entry points of an iteration; a standalone caller
+ * does it with this call (an Arcane shim would not call it). */
+int mhg_begin_iteration(mhg_ctx* ctx);
+int mhg_save_values_at_n(mhg_ctx* ctx);
+int mhg_update_velocity(mhg_ctx* ctx);
+int mhg_apply_boundary_condition(mhg_ctx* ctx);
+int mhg_update_position(mhg_ctx* ctx);
+int mhg_compute_geometric_values(mhg_ctx* ctx);
+int mhg_update_density(mhg_ctx* ctx);
+int mhg_compute_artificial_viscosity(mhg_ctx* ctx);
+int mhg_update_energy_and_pressure(mhg_ctx* ctx);
+int mhg_remap(mhg_ctx* ctx);
+int mhg_compute_delta_t(mhg_ctx* ctx, double* new_dt, int32_t* stop);
+
+/* ---- IRemap (mahyco/src/Remap/IRemap.h:46-64) ------------------------- */
+int mhg_remap_get_ordre_projection(const mhg_ctx* ctx);
+int mhg_remap_has_projection_pente_borne(const mhg_ctx* ctx);
+int mhg_remap_has_conservation_energie_totale(const mhg_ctx* ctx);
+int mhg_remap_is_euler(const mhg_ctx* ctx);
+
+/* ---- fused fast path ---------------------------------------------------- */
+/* One full iteration (all entry points of Mahyco.config:30-41), fused. */
+int mhg_step(mhg_ctx* ctx, int32_t* stop);
+/* Up to max_steps fused iterations with dt kept on the device; stops early
+ * at the end of the run.  *steps_done receives the number executed. */
+int mhg_run(mhg_ctx* ctx, int32_t max_steps, int32_t* steps_done);
+/* Enqueue n fused iterations on the context's stream without host sync. */
+int mhg_run_async(mhg_ctx* ctx, int32_t n_steps);
+int mhg_synchronize(mhg_ctx* ctx);
+
+/* ---- diagnostics ---------------------------------------------------- */
+/* Compensated global sums: total mass, total internal energy
+ * (sum m*e), total kinetic energy (sum 1/2 m_n |u|^2). */
+int mhg_conservation(mhg_ctx* ctx, double out[3]);
+/* Kernel timing: average duration (ms) of each fused kernel over the
+ * launches recorded since the last reset, measured with HIP events on the
+ * context's stream.  names/ms arrays of length cap; returns count. */
+int mhg_kernel_times(mhg_ctx* ctx, int32_t enable, const char** names, double* ms,
+                     int32_t* launches, int32_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MAHYCO_GPU_H */

@@ -1,0 +1,78 @@
+/*
+ * mahyco_oracle.h -- CPU FP64 oracle of the MaHyCo time step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker the GPU path is
+ * compared against; only tests/, __graft_entry__.smoke() and bench.py's CPU
+ * baseline leg may load it.  It is never part of the product path.
+ *
+ * It exports the same interface as include/mahyco_gpu.h with the prefix mho_
+ * instead of mhg_, using the same option/state/mesh structures, so a harness
+ * can swap the oracle and the GPU library.  In addition it exports a few
+ * single-function probes (mho_probe_*) used by the closed-form unit tests.
+ */
+#ifndef MAHYCO_ORACLE_H
+#define MAHYCO_ORACLE_H
+
+#include "../include/mahyco_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mho_ctx mho_ctx;
+
+int mho_abi_version(void);
+void mho_default_options(mhg_options* opt);
+int mho_create(const mhg_mesh_desc* mesh, const mhg_options* opt, mho_ctx** out);
+int mho_destroy(mho_ctx* ctx);
+const char* mho_last_error(const mho_ctx* ctx);
+int mho_sizes(const mho_ctx* ctx, int64_t* n_cells, int64_t* n_nodes);
+int mho_init_sod(mho_ctx* ctx, int32_t cas_test);
+int mho_init_triple_point(mho_ctx* ctx, const double rho[3], const double p[3]);
+int mho_init_cells(mho_ctx* ctx, const int32_t* env_of_cell, const double* density,
+                   const double* pressure, const double* velocity);
+int mho_upload_state(mho_ctx* ctx, const mhg_state* st);
+/* Arcane's time-loop manager: time += dt, iteration += 1 before the
+ * compute-loop entry points of an iteration. */
+int mho_begin_iteration(mho_ctx* ctx);
+int mho_download_state(mho_ctx* ctx, mhg_state* st);
+int mho_save_values_at_n(mho_ctx* ctx);
+int mho_update_velocity(mho_ctx* ctx);
+int mho_apply_boundary_condition(mho_ctx* ctx);
+int mho_update_position(mho_ctx* ctx);
+int mho_compute_geometric_values(mho_ctx* ctx);
+int mho_update_density(mho_ctx* ctx);
+int mho_compute_artificial_viscosity(mho_ctx* ctx);
+int mho_update_energy_and_pressure(mho_ctx* ctx);
+int mho_remap(mho_ctx* ctx);
+int mho_compute_delta_t(mho_ctx* ctx, double* new_dt, int32_t* stop);
+int mho_remap_get_ordre_projection(const mho_ctx* ctx);
+int mho_remap_has_projection_pente_borne(const mho_ctx* ctx);
+int mho_remap_has_conservation_energie_totale(const mho_ctx* ctx);
+int mho_remap_is_euler(const mho_ctx* ctx);
+int mho_step(mho_ctx* ctx, int32_t* stop);
+int mho_run(mho_ctx* ctx, int32_t max_steps, int32_t* steps_done);
+int mho_conservation(mho_ctx* ctx, double out[3]);
+/* Number of OpenMP threads the oracle uses (0 = leave the runtime default). */
+int mho_set_threads(int32_t n);
+
+/* ---- probes for closed-form tests -------------------------------------- */
+/* Corner normals of one cell (MahycoModule.cc:1093-1112 in 2D,
+ * computeCQs MahycoModule.cc:2195-2258 in 3D).  coord: [nb_node][3],
+ * out_cqs: [nb_node][3]; returns the volume (1/dim) sum x_i . C_i. */
+double mho_probe_cqs(int32_t dim, const double* coord, double* out_cqs);
+/* Limiters of RemapADIService.h:260-286 (id 0 minmod, 1 superbee, 2 vanLeer,
+ * other: order 1). */
+double mho_probe_limiter(int32_t id, double r);
+/* compute_eint (MahycoModule.cc:1630-1654). */
+double mho_probe_eint(int32_t pseudo_centree, double gamma, double q_n, double q,
+                      double rho_n, double rho, double p, double e_n);
+/* Pseudo-viscosity formula (MahycoModule.cc:517-528). */
+double mho_probe_viscosity(double gamma, double tau, double L, double c, double divu);
+/* Perfect-gas EOS (PerfectGasEOSService.cc:38-45): out[0]=p, out[1]=c, out[2]=dpde. */
+void mho_probe_eos_pg(double gamma, double rho, double e, double* out3);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
