@@ -1,0 +1,188 @@
+// mhg_common.hpp -- device-side building blocks shared by the MaHyCo kernels.
+//
+// Every formula keeps the operation order of the reference expression it
+// cites; the library is compiled with -ffp-contract=off so a*b+c is a rounded
+// product followed by a rounded sum, as in the reference source.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/mahyco_gpu.h"
+
+namespace mhg {
+
+constexpr int kMaxEnv = MHG_MAX_ENV;
+constexpr int kMaxBc = MHG_MAX_BC;
+
+// std::max / std::min semantics ((a<b)?b:a and (b<a)?b:a).
+__host__ __device__ inline double smax(double a, double b) { return (a < b) ? b : a; }
+__host__ __device__ inline double smin(double a, double b) { return (b < a) ? b : a; }
+
+// Limiters, RemapADIService.h:260-286.
+template <int LIM>
+__host__ __device__ inline double limiter(double r) {
+  if (LIM == 0) return smax(0.0, smin(1.0, r));                             // MinMod
+  if (LIM == 1) return smax(0.0, smax(smin(2.0 * r, 1.0), smin(r, 2.0)));  // SuperBee
+  if (LIM == 2) return (r <= 0.0 ? 0.0 : 2.0 * r / (1.0 + r));             // VanLeer
+  return 0.0;                                                              // DefaultO1
+}
+__host__ __device__ inline double limiter_rt(int lim, double r) {
+  switch (lim) {
+    case 0: return limiter<0>(r);
+    case 1: return limiter<1>(r);
+    case 2: return limiter<2>(r);
+    default: return 0.0;
+  }
+}
+
+// Limited cell gradient, computeGradPhiCell_PBorn0_LimC (RemapADIService.cc:497-506).
+__host__ __device__ inline double limited_cell_grad(int lim, double gb, double gf) {
+  double g = 0.;
+  if (gb != 0.) g += 0.5 * (limiter_rt(lim, gf / gb) * gb);
+  if (gf != 0.) g += 0.5 * (limiter_rt(lim, gb / gf) * gf);
+  return g;
+}
+// Limited nodal gradient, computeDualGradPhi_LimC (DualPhaseRemap.cc:771-777):
+// note the different association (0.5 * L) * g.
+__host__ __device__ inline double limited_node_grad(int lim, double gb, double gf) {
+  double g = 0.;
+  if (gb != 0.) g += 0.5 * limiter_rt(lim, gf / gb) * gb;
+  if (gf != 0.) g += 0.5 * limiter_rt(lim, gb / gf) * gf;
+  return g;
+}
+
+// compute_eint, MahycoModule.cc:1630-1654.
+__host__ __device__ inline double compute_eint(bool pseudo_centree, double adiabatic_cst,
+                                               double q_n, double q, double rho_n, double rho,
+                                               double p, double e_n) {
+  double pseudo = 0.;
+  if (pseudo_centree && ((q_n + q) * (1.0 / rho - 1.0 / rho_n) < 0.)) pseudo = 0.5 * (q + q_n);
+  if (!pseudo_centree && (q * (1.0 / rho - 1.0 / rho_n) < 0.)) pseudo = q;
+  double denom = 1 + 0.5 * (adiabatic_cst - 1.0) * rho * (1.0 / rho - 1.0 / rho_n);
+  double numer = e_n - (0.5 * p + pseudo) * (1.0 / rho - 1.0 / rho_n);
+  return numer / denom;
+}
+
+// Pseudo-viscosity, MahycoModule.cc:523-526.
+__host__ __device__ inline double viscosity(double qcoef, double tau, double L, double c,
+                                            double divu) {
+  return 1. / tau * (-0.5 * L * c * divu + qcoef * L * L * divu * divu);
+}
+
+// Equation of state of one environment.
+struct EosParams {
+  int kind[kMaxEnv];
+  double gamma[kMaxEnv];
+  double tension[kMaxEnv];
+  double qcoef[kMaxEnv];
+};
+
+// PerfectGas compute_pressure_sndspd_PG (PerfectGasEOSService.cc:38-45) and
+// StiffenedGas applyEOS (StiffenedGasEOSService.cc:35-42).
+__host__ __device__ inline void eos_apply(const EosParams& E, int k, double rho, double e,
+                                          double* p, double* c) {
+  const double g = E.gamma[k];
+  if (E.kind[k] == MHG_EOS_STIFFENED_GAS) {
+    const double t = E.tension[k];
+    double pressure = ((g - 1.) * rho * e) - (g * t);
+    *p = pressure;
+    *c = sqrt((g / rho) * (pressure + t));
+  } else {
+    double pressure = (g - 1.) * rho * e;
+    *p = pressure;
+    *c = sqrt(g * pressure / rho);
+  }
+}
+
+// 2D corner normal of corner q of a quad whose nodes are (x[i], y[i]) in
+// Arcane quad4 order (MahycoModule.cc:1102-1111):
+//   npc[e+1] = 0.5*(y[e+1]-y[e], x[e]-x[e+1]), npc[0] = npc[4],
+//   cqs[q] = npc[q+1] + npc[q].
+__host__ __device__ inline void cqs2d(const double* x, const double* y, int q, double* cx,
+                                      double* cy) {
+  const int a = q, b = (q + 1) & 3, m = (q + 3) & 3;
+  // npc[q+1]: edge a -> b ; npc[q]: edge m -> a
+  double e1x = 0.5 * (y[b] - y[a]);
+  double e1y = 0.5 * (x[a] - x[b]);
+  double e0x = 0.5 * (y[a] - y[m]);
+  double e0y = 0.5 * (x[m] - x[a]);
+  *cx = e1x + e0x;
+  *cy = e1y + e0y;
+}
+
+// Volume (1/2) sum_i x_i . C_i of a quad (MahycoModule.cc:1244-1256).
+__host__ __device__ inline double quad_volume(const double* x, const double* y) {
+  double volume = 0;
+  for (int q = 0; q < 4; ++q) {
+    double cx, cy;
+    cqs2d(x, y, q, &cx, &cy);
+    volume += x[q] * cx + y[q] * cy;
+  }
+  volume *= 0.5;
+  return volume;
+}
+
+// Characteristic length "faces-opposees" (MahycoModule.cc:1199-1219); in 2D
+// the reference reads coord[4..7] which are zero.
+__host__ __device__ inline double length_faces_opposees_2d(const double* x, const double* y) {
+  double cx[8] = {x[0], x[1], x[2], x[3], 0., 0., 0., 0.};
+  double cy[8] = {y[0], y[1], y[2], y[3], 0., 0., 0., 0.};
+  auto fc = [&](const double* c, int a, int b, int cc, int d) {
+    return 0.25 * (c[a] + c[b] + c[cc] + c[d]);
+  };
+  double f0x = fc(cx, 0, 3, 2, 1), f0y = fc(cy, 0, 3, 2, 1);
+  double f1x = fc(cx, 0, 4, 7, 3), f1y = fc(cy, 0, 4, 7, 3);
+  double f2x = fc(cx, 0, 1, 5, 4), f2y = fc(cy, 0, 1, 5, 4);
+  double f3x = fc(cx, 4, 5, 6, 7), f3y = fc(cy, 4, 5, 6, 7);
+  double f4x = fc(cx, 1, 2, 6, 5), f4y = fc(cy, 1, 2, 6, 5);
+  double f5x = fc(cx, 2, 3, 7, 6), f5y = fc(cy, 2, 3, 7, 6);
+  double m1x = f0x - f3x, m1y = f0y - f3y;
+  double m2x = f2x - f5x, m2y = f2y - f5y;
+  double m3x = f1x - f4x, m3y = f1y - f4y;
+  double d1 = sqrt(m1x * m1x + m1y * m1y + 0. * 0.);
+  double d2 = sqrt(m2x * m2x + m2y * m2y + 0. * 0.);
+  double d3 = sqrt(m3x * m3x + m3y * m3y + 0. * 0.);
+  double num = d1 * d2 * d3;
+  double den = d1 * d2 + d1 * d3 + d2 * d3;
+  return num / den;
+}
+
+// Boundary condition on an index range of one axis (the node set that
+// PrepareFaceGroup's coordinate test selects on a Cartesian mesh).
+struct BcRange {
+  int axis;   // -1: every node
+  int lo, hi; // inclusive node-index range along axis (lo > hi: empty)
+  int comp;   // velocity component
+  double value;
+};
+
+// Scalars kept on the device so steps can run back to back.
+struct DevScalars {
+  double dt;        // m_global_deltat
+  double dt_old;    // m_old_deltat
+  double time;      // m_global_time
+  int iteration;
+  int stop;         // end of run reached
+  int err;          // MHG_E_DT_MIN raised
+  int pad;
+  unsigned long long dtmin_bits;  // running min of dx_sound (positive double bits)
+  unsigned int counter;           // finished blocks of the final kernel
+  unsigned int pad2;
+};
+
+// Run-time parameters of a context.
+struct Params {
+  int dim, ne, nvu;  // nvu = 3*ne+1 live cell variables
+  int nx, ny, nz;
+  long nc, nn;
+  double org[3], h[3];
+  EosParams eos;
+  double threshold, remap_threshold;
+  int order2, lim, pseudo_centree, lc;
+  double cfl, dtmin, dtmax, tend;
+  int nbc;
+  BcRange bc[kMaxBc];
+};
+
+}  // namespace mhg

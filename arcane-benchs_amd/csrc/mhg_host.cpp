@@ -1,0 +1,890 @@
+// mhg_host.cpp -- host side of the MI355X MaHyCo library: context, device
+// memory, initial states, the C-ABI of include/mahyco_gpu.h and the launch
+// sequence of one time step.
+//
+// The host objects mirror the reference's service structure:
+//   MahycoModule entry points      -> mhg_* compute-loop calls / step()
+//   IRemap (RemapADI)              -> Remap::appli() (sweep order, SensProjection)
+//   IEquationOfState               -> EosParams (per environment, on the device)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mhg_common.hpp"
+#include "mhg_kernels.hpp"
+
+using namespace mhg;
+
+namespace {
+
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+struct mhg_ctx {
+  mhg_options opt{};
+  Params P{};
+  std::string err;
+  bool initialised = false;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  std::vector<DBuf> owned;
+
+  // cells
+  double *rho = nullptr, *e = nullptr, *p = nullptr, *cs = nullptr, *q = nullptr, *m = nullptr;
+  double *L = nullptr, *Vlag = nullptr;
+  int* mask = nullptr;
+  double* U[2] = {nullptr, nullptr};
+  double* contrib = nullptr;
+  // environments [ne][nc]
+  double *env_rho = nullptr, *env_e = nullptr, *env_p = nullptr, *env_cs = nullptr;
+  double *env_q = nullptr, *env_m = nullptr, *env_f = nullptr, *env_mf = nullptr;
+  // nodes
+  double* ux[2] = {nullptr, nullptr};
+  double* uy[2] = {nullptr, nullptr};
+  double* xx[2] = {nullptr, nullptr};
+  double* xy[2] = {nullptr, nullptr};
+  double *Ud0 = nullptr, *Ud1 = nullptr, *Ud3 = nullptr, *dg0 = nullptr, *dg1 = nullptr;
+  DevScalars* S = nullptr;
+  DevScalars* S_host = nullptr;  // pinned mirror
+
+  // host mirrors of the step bookkeeping
+  int xcur = 0;  // buffer with the coordinates of the last Lagrange phase
+  int ucur = 0;  // buffer with the current velocity
+  int sens = 0;  // SensProjection
+
+  // kernel timing
+  bool timing = false;
+  std::vector<std::string> tnames;
+  std::vector<double> tsum;
+  std::vector<int> tcount;
+  std::vector<hipEvent_t> evpool;
+  std::vector<std::pair<int, int>> pending;  // (name index, event pair index)
+  size_t evused = 0;
+};
+
+namespace {
+
+#define HIPCHK(ctx, x)                                                   \
+  do {                                                                   \
+    hipError_t e_ = (x);                                                 \
+    if (e_ != hipSuccess) {                                              \
+      (ctx)->err = std::string(#x) + ": " + hipGetErrorString(e_);       \
+      return MHG_E_HIP;                                                  \
+    }                                                                    \
+  } while (0)
+
+int fail(mhg_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+template <class T>
+int dalloc(mhg_ctx* c, T** out, size_t count) {
+  void* p = nullptr;
+  size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  if (hipMalloc(&p, bytes) != hipSuccess) return fail(c, MHG_E_ALLOC, "hipMalloc failed");
+  if (hipMemset(p, 0, bytes) != hipSuccess) return fail(c, MHG_E_HIP, "hipMemset failed");
+  c->owned.push_back({p, bytes});
+  *out = (T*)p;
+  return MHG_OK;
+}
+
+double x0_of(const Params& P, int d, int i) { return P.org[d] + i * P.h[d]; }
+
+// Node-index range selected by PrepareFaceGroup's coordinate test
+// (MahycoAnnexe.cc:183-251) along one axis; returns false if not a range.
+bool surface_range(const Params& P, int surf, int* axis, int* lo, int* hi, double thr) {
+  const int a = surf / 2;
+  const bool is_max = surf % 2;
+  const int nnodes[3] = {P.nx + 1, P.ny + 1, P.nz + 1};
+  if (a >= P.dim) {  // 2D: every node has z = 0, so ZMIN and ZMAX hold every face
+    *axis = -1;
+    *lo = 0;
+    *hi = 0;
+    return true;
+  }
+  double mx = -1.;
+  for (int i = 0; i < nnodes[a]; ++i) mx = std::max(mx, x0_of(P, a, i));
+  int first = -1, last = -2;
+  bool gap = false;
+  for (int i = 0; i < nnodes[a]; ++i) {
+    const double x = x0_of(P, a, i);
+    const bool in = is_max ? (std::fabs(x - mx) <= thr) : (x <= thr);
+    if (in) {
+      if (first < 0) first = i;
+      else if (last != i - 1) gap = true;
+      last = i;
+    }
+  }
+  if (gap) return false;
+  *axis = a;
+  *lo = first < 0 ? 1 : first;
+  *hi = first < 0 ? 0 : last;
+  return true;
+}
+
+int sync(mhg_ctx* c) {
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MHG_OK;
+}
+
+int read_scalars(mhg_ctx* c) {
+  HIPCHK(c, hipMemcpyAsync(c->S_host, c->S, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MHG_OK;
+}
+
+int write_scalars(mhg_ctx* c) {
+  HIPCHK(c, hipMemcpyAsync(c->S, c->S_host, sizeof(DevScalars), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MHG_OK;
+}
+
+// ---- kernel timing -----------------------------------------------------
+int tname_index(mhg_ctx* c, const char* name) {
+  for (size_t i = 0; i < c->tnames.size(); ++i)
+    if (c->tnames[i] == name) return (int)i;
+  c->tnames.push_back(name);
+  c->tsum.push_back(0.);
+  c->tcount.push_back(0);
+  return (int)c->tnames.size() - 1;
+}
+
+struct TimedLaunch {
+  mhg_ctx* c;
+  int name = -1;
+  size_t ev = 0;
+  TimedLaunch(mhg_ctx* cc, const char* nm) : c(cc) {
+    if (!c->timing) return;
+    name = tname_index(c, nm);
+    if (c->evused + 2 > c->evpool.size()) {
+      for (int k = 0; k < 64; ++k) {
+        hipEvent_t e;
+        hipEventCreate(&e);
+        c->evpool.push_back(e);
+      }
+    }
+    ev = c->evused;
+    c->evused += 2;
+    hipEventRecord(c->evpool[ev], c->stream);
+  }
+  ~TimedLaunch() {
+    if (!c->timing) return;
+    hipEventRecord(c->evpool[ev + 1], c->stream);
+    c->pending.push_back({name, (int)ev});
+  }
+};
+
+void collect_times(mhg_ctx* c) {
+  if (c->pending.empty()) return;
+  hipStreamSynchronize(c->stream);
+  for (auto& pr : c->pending) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, c->evpool[pr.second], c->evpool[pr.second + 1]);
+    c->tsum[pr.first] += ms;
+    c->tcount[pr.first] += 1;
+  }
+  c->pending.clear();
+  c->evused = 0;
+}
+
+// ---- argument blocks ---------------------------------------------------
+NodeLagArgs node_lag_args(mhg_ctx* c) {
+  NodeLagArgs A;
+  A.p = c->p;
+  A.q = c->q;
+  A.m = c->m;
+  A.xpx = c->xx[c->xcur];
+  A.xpy = c->xy[c->xcur];
+  A.ux = c->ux[c->ucur];
+  A.uy = c->uy[c->ucur];
+  A.xnx = c->xx[1 - c->xcur];
+  A.xny = c->xy[1 - c->xcur];
+  return A;
+}
+
+CellLagArgs cell_lag_args(mhg_ctx* c) {
+  CellLagArgs A;
+  A.xnx = c->xx[1 - c->xcur];
+  A.xny = c->xy[1 - c->xcur];
+  A.rho = c->rho;
+  A.e = c->e;
+  A.p = c->p;
+  A.cs = c->cs;
+  A.q = c->q;
+  A.m = c->m;
+  A.mask = c->mask;
+  A.env_f = c->env_f;
+  A.env_rho = c->env_rho;
+  A.env_m = c->env_m;
+  A.env_q = c->env_q;
+  A.env_p = c->env_p;
+  A.env_e = c->env_e;
+  A.U = c->U[0];
+  A.Vlag = c->Vlag;
+  A.L = c->L;
+  return A;
+}
+
+// Remap (IRemap::appliRemap, RemapADIService.cc:19-60): for i < dim,
+// idir = (i + SensProjection) % dim, cell sweep then dual sweep; then
+// SensProjection = (SensProjection + 1) % dim and remapVariables.
+void remap_2d(mhg_ctx* c) {
+  const Params& P = c->P;
+  const double* xnx = c->xx[1 - c->xcur];
+  const double* xny = c->xy[1 - c->xcur];
+  const double* ux = c->ux[c->ucur];
+  const double* uy = c->uy[c->ucur];
+  for (int pass = 0; pass < P.dim; ++pass) {
+    const int idir = (pass + c->sens) % P.dim;
+    const bool first = (pass == 0);
+    CellSweepArgs A;
+    A.Uin = c->U[pass & 1];
+    A.Uout = c->U[(pass + 1) & 1];
+    A.Vlag = c->Vlag;
+    A.xnx = xnx;
+    A.xny = xny;
+    A.ux = ux;
+    A.uy = uy;
+    A.contrib = c->contrib;
+    {
+      TimedLaunch t(c, idir == 0 ? "cell_sweep_x" : "cell_sweep_y");
+      launch_cell_sweep_2d(c->stream, P, c->S, A, idir, first);
+    }
+    NodeSweepArgs B;
+    B.m = c->m;
+    B.ux = ux;
+    B.uy = uy;
+    B.xnx = xnx;
+    B.xny = xny;
+    B.contrib = c->contrib;
+    B.Ud0 = c->Ud0;
+    B.Ud1 = c->Ud1;
+    B.Ud3 = c->Ud3;
+    B.dg0 = c->dg0;
+    B.dg1 = c->dg1;
+    B.uox = c->ux[1 - c->ucur];
+    B.uoy = c->uy[1 - c->ucur];
+    {
+      TimedLaunch t(c, idir == 0 ? "node_sweep_x" : "node_sweep_y");
+      launch_node_sweep_2d(c->stream, P, c->S, B, idir, first);
+    }
+  }
+  CellFinalArgs F;
+  F.U = c->U[P.dim & 1];
+  F.Vlag = c->Vlag;
+  F.L = c->L;
+  F.ux = c->ux[1 - c->ucur];
+  F.uy = c->uy[1 - c->ucur];
+  F.rho = c->rho;
+  F.e = c->e;
+  F.p = c->p;
+  F.cs = c->cs;
+  F.q = c->q;
+  F.m = c->m;
+  F.mask = c->mask;
+  F.env_rho = c->env_rho;
+  F.env_e = c->env_e;
+  F.env_p = c->env_p;
+  F.env_cs = c->env_cs;
+  F.env_q = c->env_q;
+  F.env_m = c->env_m;
+  F.env_f = c->env_f;
+  F.env_mf = c->env_mf;
+  {
+    TimedLaunch t(c, "cell_final");
+    launch_cell_final_2d(c->stream, P, c->S, F);
+  }
+  c->sens = (c->sens + 1) % P.dim;
+  c->xcur = 1 - c->xcur;
+  c->ucur = 1 - c->ucur;
+}
+
+// One fused iteration, enqueued on the context stream.
+void enqueue_step(mhg_ctx* c) {
+  launch_begin_iteration(c->stream, c->S);
+  {
+    TimedLaunch t(c, "node_lagrange");
+    launch_node_lagrange_2d(c->stream, c->P, c->S, node_lag_args(c), true, true, true);
+  }
+  {
+    TimedLaunch t(c, "cell_lagrange");
+    launch_cell_lagrange_2d(c->stream, c->P, c->S, cell_lag_args(c));
+  }
+  remap_2d(c);
+}
+
+int check_ready(mhg_ctx* c) {
+  if (!c) return MHG_E_ARG;
+  if (!c->initialised) return fail(c, MHG_E_STATE, "context not initialised");
+  return MHG_OK;
+}
+
+// Pure-cell initial state (hydroStartInit, MahycoModule.cc:48-188, then
+// computeCellMass 199-225): EOS init per environment, m = rho V.
+int init_pure(mhg_ctx* c, const std::vector<int>& env, const std::vector<double>& rho,
+              const std::vector<double>& pres, const double* vel) {
+  const Params& P = c->P;
+  const long NC = P.nc, NN = P.nn;
+  std::vector<double> e(NC), cs(NC), m(NC), zero(NC, 0.);
+  std::vector<int> mask(NC);
+  for (long cc = 0; cc < NC; ++cc) {
+    const int k = env[cc];
+    mask[cc] = 1 << k;
+    const double g = P.eos.gamma[k];
+    // initEOS, PerfectGasEOSService.cc:20-31 / StiffenedGasEOSService.cc:10-22
+    if (P.eos.kind[k] == MHG_EOS_STIFFENED_GAS) {
+      const double t = P.eos.tension[k];
+      e[cc] = (pres[cc] + (g * t)) / ((g - 1.) * rho[cc]);
+      cs[cc] = std::sqrt((g / rho[cc]) * (pres[cc] + t));
+    } else {
+      e[cc] = pres[cc] / ((g - 1.) * rho[cc]);
+      cs[cc] = std::sqrt(g * pres[cc] / rho[cc]);
+    }
+    const int i = (int)(cc % P.nx), j = (int)(cc / P.nx);
+    double x[4] = {x0_of(P, 0, i), x0_of(P, 0, i + 1), x0_of(P, 0, i + 1), x0_of(P, 0, i)};
+    double y[4] = {x0_of(P, 1, j), x0_of(P, 1, j), x0_of(P, 1, j + 1), x0_of(P, 1, j + 1)};
+    const double V = quad_volume(x, y);
+    m[cc] = rho[cc] * V;
+    m[cc] = 1. * m[cc];  // m_env = mass_fraction * m (199-225), mass_fraction = 1
+  }
+  std::vector<double> xs(NN), ys(NN), us(NN, 0.), vs(NN, 0.);
+  for (long n = 0; n < NN; ++n) {
+    const int i = (int)(n % (P.nx + 1)), j = (int)(n / (P.nx + 1));
+    xs[n] = x0_of(P, 0, i);
+    ys[n] = x0_of(P, 1, j);
+    if (vel) {
+      us[n] = vel[n];
+      vs[n] = vel[NN + n];
+    }
+  }
+  c->xcur = 0;
+  c->ucur = 0;
+  c->sens = 0;
+  auto up = [&](double* d, const std::vector<double>& h) {
+    return hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice);
+  };
+  HIPCHK(c, up(c->rho, rho));
+  HIPCHK(c, up(c->e, e));
+  HIPCHK(c, up(c->p, pres));
+  HIPCHK(c, up(c->cs, cs));
+  HIPCHK(c, up(c->q, zero));
+  HIPCHK(c, up(c->m, m));
+  HIPCHK(c, hipMemcpy(c->mask, mask.data(), NC * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(c, up(c->xx[0], xs));
+  HIPCHK(c, up(c->xy[0], ys));
+  HIPCHK(c, up(c->ux[0], us));
+  HIPCHK(c, up(c->uy[0], vs));
+  HIPCHK(c, hipMemset(c->dg0, 0, NN * sizeof(double)));
+  HIPCHK(c, hipMemset(c->dg1, 0, NN * sizeof(double)));
+  const size_t envb = (size_t)P.ne * NC * sizeof(double);
+  double* envs[] = {c->env_rho, c->env_e, c->env_p, c->env_cs, c->env_q, c->env_m, c->env_f, c->env_mf};
+  for (double* d : envs) HIPCHK(c, hipMemset(d, 0, envb));
+  std::memset(c->S_host, 0, sizeof(DevScalars));
+  c->S_host->dt = c->opt.deltat_init;
+  c->S_host->dtmin_bits = 0x7FEFFFFFFFFFFFFFull;
+  int r = write_scalars(c);
+  if (r) return r;
+  c->initialised = true;
+  return MHG_OK;
+}
+
+void cell_centre(const Params& P, long cc, double* xc, double* yc) {
+  const int i = (int)(cc % P.nx), j = (int)(cc / P.nx);
+  // 0.25 * sum of the 4 nodes in Arcane order (MahycoAnnexe.cc:107-112)
+  *xc = 0.25 * (((x0_of(P, 0, i) + x0_of(P, 0, i + 1)) + x0_of(P, 0, i + 1)) + x0_of(P, 0, i));
+  *yc = 0.25 * (((x0_of(P, 1, j) + x0_of(P, 1, j)) + x0_of(P, 1, j + 1)) + x0_of(P, 1, j + 1));
+}
+
+}  // namespace
+
+extern "C" {
+
+int mhg_abi_version(void) { return MHG_ABI_VERSION; }
+
+void mhg_default_options(mhg_options* o) {
+  std::memset(o, 0, sizeof(*o));
+  o->cfl = 0.3;
+  o->threshold = 1.e-10;
+  o->deltat_init = 1.e-5;
+  o->deltat_min = 1.e-8;
+  o->deltat_max = 0.01;
+  o->final_time = 0.2;
+  o->with_projection = 1;
+  o->longueur_caracteristique = MHG_LC_RACINE_CUBIQUE_VOLUME;
+  o->nb_bc = 6;
+  const int surf[6] = {MHG_SURF_XMIN, MHG_SURF_XMAX, MHG_SURF_YMIN, MHG_SURF_YMAX, MHG_SURF_ZMIN, MHG_SURF_ZMAX};
+  const int typ[6] = {MHG_BC_VX, MHG_BC_VX, MHG_BC_VY, MHG_BC_VY, MHG_BC_VZ, MHG_BC_VZ};
+  for (int i = 0; i < 6; ++i) {
+    o->bc[i].surface = surf[i];
+    o->bc[i].type = typ[i];
+    o->bc[i].value = 0.;
+  }
+  o->nb_env = 1;
+  o->env[0].eos = MHG_EOS_PERFECT_GAS;
+  o->env[0].adiabatic_cst = 1.4;
+  o->env[0].limit_tension = 0.;
+  o->remap_threshold = 1.e-10;
+  o->is_euler_scheme = 1;
+  o->ordre_projection = 2;
+  o->projection_limiteur_id = MHG_LIM_MINMOD;
+  o->visc_quad_coeff = -1.;
+  o->precision = MHG_PREC_FP64;
+}
+
+int mhg_create(const mhg_mesh_desc* mesh, const mhg_options* opt, mhg_ctx** out) {
+  if (!mesh || !opt || !out) return MHG_E_ARG;
+  *out = nullptr;
+  if (mesh->dim != 2 && mesh->dim != 3) return MHG_E_ARG;
+  for (int d = 0; d < mesh->dim; ++d)
+    if (mesh->n[d] < 1 || !(mesh->length[d] > 0.)) return MHG_E_ARG;
+  if (opt->nb_env < 1 || opt->nb_env > MHG_MAX_ENV || opt->nb_bc < 0 || opt->nb_bc > MHG_MAX_BC)
+    return MHG_E_ARG;
+  if (opt->schema_csts || opt->with_newton || opt->sans_lagrange || !opt->with_projection ||
+      !opt->is_euler_scheme || opt->projection_pente_borne || opt->conservation_energie_totale ||
+      opt->projection_limiteur_id >= MHG_LIM_MINMODG || opt->projection_limiteur_id < 0 ||
+      opt->ordre_projection < 1 || opt->ordre_projection > 2)
+    return MHG_E_UNSUPPORTED;
+  if (mesh->dim != 2 || opt->precision != MHG_PREC_FP64) return MHG_E_UNSUPPORTED;
+  mhg_ctx* c = new mhg_ctx();
+  c->opt = *opt;
+  Params& P = c->P;
+  P.dim = mesh->dim;
+  P.ne = opt->nb_env;
+  P.nvu = 3 * P.ne + 1;
+  P.nx = mesh->n[0];
+  P.ny = mesh->n[1];
+  P.nz = mesh->dim == 3 ? mesh->n[2] : 1;
+  P.nc = (long)P.nx * P.ny * (P.dim == 3 ? P.nz : 1);
+  P.nn = (long)(P.nx + 1) * (P.ny + 1) * (P.dim == 3 ? P.nz + 1 : 1);
+  for (int d = 0; d < 3; ++d) {
+    P.org[d] = mesh->origin[d];
+    P.h[d] = d < P.dim ? mesh->length[d] / mesh->n[d] : 0.;
+  }
+  for (int k = 0; k < P.ne; ++k) {
+    P.eos.kind[k] = opt->env[k].eos;
+    P.eos.gamma[k] = opt->env[k].adiabatic_cst;
+    P.eos.tension[k] = opt->env[k].limit_tension;
+    P.eos.qcoef[k] = opt->visc_quad_coeff >= 0. ? opt->visc_quad_coeff
+                                               : (opt->env[k].adiabatic_cst + 1) / 2.0;
+  }
+  P.threshold = opt->threshold;
+  P.remap_threshold = opt->remap_threshold;
+  P.order2 = opt->ordre_projection - 1;
+  P.lim = opt->projection_limiteur_id;
+  P.pseudo_centree = opt->pseudo_centree;
+  P.lc = opt->longueur_caracteristique;
+  P.cfl = opt->cfl;
+  P.dtmin = opt->deltat_min;
+  P.dtmax = opt->deltat_max;
+  P.tend = opt->final_time;
+  P.nbc = 0;
+  for (int b = 0; b < opt->nb_bc; ++b) {
+    const mhg_bc& bc = opt->bc[b];
+    if (bc.surface < 0 || bc.surface > 5 || bc.type < 0 || bc.type > 2) {
+      delete c;
+      return MHG_E_ARG;
+    }
+    if (bc.type >= P.dim) {  // Vz in 2D: u_z is identically zero
+      if (bc.value != 0.) {
+        delete c;
+        return MHG_E_UNSUPPORTED;
+      }
+      continue;
+    }
+    BcRange r;
+    if (!surface_range(P, bc.surface, &r.axis, &r.lo, &r.hi, P.threshold)) {
+      delete c;
+      return MHG_E_UNSUPPORTED;
+    }
+    r.comp = bc.type;
+    r.value = bc.value;
+    P.bc[P.nbc++] = r;
+  }
+  if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    return MHG_E_HIP;
+  }
+  const long NC = P.nc, NN = P.nn;
+  const size_t envn = (size_t)P.ne * NC;
+  int r = 0;
+  r |= dalloc(c, &c->rho, NC) | dalloc(c, &c->e, NC) | dalloc(c, &c->p, NC) | dalloc(c, &c->cs, NC);
+  r |= dalloc(c, &c->q, NC) | dalloc(c, &c->m, NC) | dalloc(c, &c->L, NC) | dalloc(c, &c->Vlag, NC);
+  r |= dalloc(c, &c->mask, NC);
+  r |= dalloc(c, &c->U[0], (size_t)P.nvu * NC) | dalloc(c, &c->U[1], (size_t)P.nvu * NC);
+  r |= dalloc(c, &c->contrib, envn);
+  r |= dalloc(c, &c->env_rho, envn) | dalloc(c, &c->env_e, envn) | dalloc(c, &c->env_p, envn);
+  r |= dalloc(c, &c->env_cs, envn) | dalloc(c, &c->env_q, envn) | dalloc(c, &c->env_m, envn);
+  r |= dalloc(c, &c->env_f, envn) | dalloc(c, &c->env_mf, envn);
+  for (int b = 0; b < 2; ++b) {
+    r |= dalloc(c, &c->ux[b], NN) | dalloc(c, &c->uy[b], NN);
+    r |= dalloc(c, &c->xx[b], NN) | dalloc(c, &c->xy[b], NN);
+  }
+  r |= dalloc(c, &c->Ud0, NN) | dalloc(c, &c->Ud1, NN) | dalloc(c, &c->Ud3, NN);
+  r |= dalloc(c, &c->dg0, NN) | dalloc(c, &c->dg1, NN);
+  r |= dalloc(c, &c->S, 1);
+  if (r) {
+    mhg_destroy(c);
+    return MHG_E_ALLOC;
+  }
+  if (hipHostMalloc((void**)&c->S_host, sizeof(DevScalars), hipHostMallocDefault) != hipSuccess) {
+    mhg_destroy(c);
+    return MHG_E_ALLOC;
+  }
+  *out = c;
+  return MHG_OK;
+}
+
+int mhg_destroy(mhg_ctx* c) {
+  if (!c) return MHG_OK;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (auto& b : c->owned) hipFree(b.p);
+  for (auto e : c->evpool) hipEventDestroy(e);
+  if (c->S_host) hipHostFree(c->S_host);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return MHG_OK;
+}
+
+const char* mhg_last_error(const mhg_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int mhg_sizes(const mhg_ctx* c, int64_t* n_cells, int64_t* n_nodes) {
+  if (!c) return MHG_E_ARG;
+  if (n_cells) *n_cells = c->P.nc;
+  if (n_nodes) *n_nodes = c->P.nn;
+  return MHG_OK;
+}
+
+// SODService (SODService.cc:3-135): cas-test 3/4/5 one environment,
+// 13/14/15 two environments split at r = 0.5.
+int mhg_init_sod(mhg_ctx* c, int32_t cas_test) {
+  if (!c) return MHG_E_ARG;
+  const Params& P = c->P;
+  const bool mono = (cas_test >= 3 && cas_test <= 5), bi = (cas_test >= 13 && cas_test <= 15);
+  if (!mono && !bi) return fail(c, MHG_E_ARG, "unsupported SOD cas-test");
+  if (bi && P.ne < 2) return fail(c, MHG_E_ARG, "bi-material SOD needs 2 environments");
+  const int axis = mono ? cas_test - 3 : cas_test - 13;
+  if (axis >= P.dim) return fail(c, MHG_E_ARG, "SOD axis beyond mesh dimension");
+  std::vector<int> env(P.nc);
+  std::vector<double> rho(P.nc), pres(P.nc);
+  for (long cc = 0; cc < P.nc; ++cc) {
+    double xc, yc;
+    cell_centre(P, cc, &xc, &yc);
+    const double r = axis == 0 ? xc : yc;
+    env[cc] = mono ? 0 : (r < 0.5 ? 0 : 1);
+    rho[cc] = r < 0.5 ? 1.0 : 0.125;
+    pres[cc] = r < 0.5 ? 1.0 : 0.1;
+  }
+  return init_pure(c, env, rho, pres, nullptr);
+}
+
+int mhg_init_triple_point(mhg_ctx* c, const double rho3[3], const double p3[3]) {
+  if (!c || !rho3 || !p3) return MHG_E_ARG;
+  const Params& P = c->P;
+  if (P.ne != 3) return fail(c, MHG_E_ARG, "triple point needs 3 environments");
+  std::vector<int> env(P.nc);
+  std::vector<double> rho(P.nc), pres(P.nc);
+  for (long cc = 0; cc < P.nc; ++cc) {
+    double xc, yc;
+    cell_centre(P, cc, &xc, &yc);
+    const int k = (xc < 1.) ? 0 : (yc < 1.5 ? 1 : 2);
+    env[cc] = k;
+    rho[cc] = rho3[k];
+    pres[cc] = p3[k];
+  }
+  return init_pure(c, env, rho, pres, nullptr);
+}
+
+int mhg_init_cells(mhg_ctx* c, const int32_t* env_of_cell, const double* density,
+                   const double* pressure, const double* velocity) {
+  if (!c || !env_of_cell || !density || !pressure) return MHG_E_ARG;
+  const Params& P = c->P;
+  std::vector<int> env(env_of_cell, env_of_cell + P.nc);
+  for (int k : env)
+    if (k < 0 || k >= P.ne) return fail(c, MHG_E_ARG, "bad env index");
+  std::vector<double> rho(density, density + P.nc), pres(pressure, pressure + P.nc);
+  return init_pure(c, env, rho, pres, velocity);
+}
+
+int mhg_upload_state(mhg_ctx* c, const mhg_state* st) {
+  if (!c || !st) return MHG_E_ARG;
+  const Params& P = c->P;
+  const long NC = P.nc, NN = P.nn;
+  const size_t cb = NC * sizeof(double), nb = NN * sizeof(double), eb = (size_t)P.ne * cb;
+  c->xcur = 0;
+  c->ucur = 0;
+  auto h2d = [&](void* d, const void* h, size_t b) {
+    return h ? hipMemcpy(d, h, b, hipMemcpyHostToDevice) : hipSuccess;
+  };
+  HIPCHK(c, h2d(c->rho, st->density, cb));
+  HIPCHK(c, h2d(c->e, st->internal_energy, cb));
+  HIPCHK(c, h2d(c->p, st->pressure, cb));
+  HIPCHK(c, h2d(c->cs, st->sound_speed, cb));
+  HIPCHK(c, h2d(c->q, st->pseudo_viscosity, cb));
+  HIPCHK(c, h2d(c->m, st->cell_mass, cb));
+  HIPCHK(c, h2d(c->mask, st->env_mask, NC * sizeof(int)));
+  HIPCHK(c, h2d(c->env_rho, st->density_env, eb));
+  HIPCHK(c, h2d(c->env_e, st->internal_energy_env, eb));
+  HIPCHK(c, h2d(c->env_p, st->pressure_env, eb));
+  HIPCHK(c, h2d(c->env_cs, st->sound_speed_env, eb));
+  HIPCHK(c, h2d(c->env_q, st->pseudo_viscosity_env, eb));
+  HIPCHK(c, h2d(c->env_m, st->cell_mass_env, eb));
+  HIPCHK(c, h2d(c->env_f, st->fracvol_env, eb));
+  HIPCHK(c, h2d(c->env_mf, st->mass_fraction_env, eb));
+  if (st->velocity) {
+    HIPCHK(c, h2d(c->ux[0], st->velocity, nb));
+    HIPCHK(c, h2d(c->uy[0], st->velocity + NN, nb));
+  }
+  if (st->node_coord) {
+    HIPCHK(c, h2d(c->xx[0], st->node_coord, nb));
+    HIPCHK(c, h2d(c->xy[0], st->node_coord + NN, nb));
+  }
+  if (st->dual_grad_phi) {
+    HIPCHK(c, h2d(c->dg0, st->dual_grad_phi, nb));
+    HIPCHK(c, h2d(c->dg1, st->dual_grad_phi + NN, nb));
+  }
+  if (st->caracteristic_length) HIPCHK(c, h2d(c->L, st->caracteristic_length, cb));
+  std::memset(c->S_host, 0, sizeof(DevScalars));
+  c->S_host->dt = st->global_deltat;
+  c->S_host->dt_old = st->old_deltat;
+  c->S_host->time = st->global_time;
+  c->S_host->iteration = st->global_iteration;
+  c->S_host->dtmin_bits = 0x7FEFFFFFFFFFFFFFull;
+  c->sens = st->sens_projection;
+  int r = write_scalars(c);
+  if (r) return r;
+  c->initialised = true;
+  return MHG_OK;
+}
+
+int mhg_download_state(mhg_ctx* c, mhg_state* st) {
+  int r = check_ready(c);
+  if (r) return r;
+  const Params& P = c->P;
+  const long NC = P.nc, NN = P.nn;
+  const size_t cb = NC * sizeof(double), nb = NN * sizeof(double), eb = (size_t)P.ne * cb;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  auto d2h = [&](void* h, const void* d, size_t b) {
+    return h ? hipMemcpy(h, d, b, hipMemcpyDeviceToHost) : hipSuccess;
+  };
+  HIPCHK(c, d2h(st->density, c->rho, cb));
+  HIPCHK(c, d2h(st->internal_energy, c->e, cb));
+  HIPCHK(c, d2h(st->pressure, c->p, cb));
+  HIPCHK(c, d2h(st->sound_speed, c->cs, cb));
+  HIPCHK(c, d2h(st->pseudo_viscosity, c->q, cb));
+  HIPCHK(c, d2h(st->cell_mass, c->m, cb));
+  std::vector<int> mask(NC);
+  HIPCHK(c, hipMemcpy(mask.data(), c->mask, NC * sizeof(int), hipMemcpyDeviceToHost));
+  if (st->env_mask) std::memcpy(st->env_mask, mask.data(), NC * sizeof(int));
+  struct EP { double* h; double* d; double* g; };
+  std::vector<double> gbuf[6];
+  double* gsrc[6] = {c->rho, c->e, c->p, c->cs, c->q, c->m};
+  EP eps[8] = {{st->density_env, c->env_rho, st->density},
+               {st->internal_energy_env, c->env_e, st->internal_energy},
+               {st->pressure_env, c->env_p, st->pressure},
+               {st->sound_speed_env, c->env_cs, st->sound_speed},
+               {st->pseudo_viscosity_env, c->env_q, st->pseudo_viscosity},
+               {st->cell_mass_env, c->env_m, st->cell_mass},
+               {st->fracvol_env, c->env_f, nullptr},
+               {st->mass_fraction_env, c->env_mf, nullptr}};
+  for (int t = 0; t < 8; ++t) {
+    if (!eps[t].h) continue;
+    HIPCHK(c, hipMemcpy(eps[t].h, eps[t].d, eb, hipMemcpyDeviceToHost));
+    // pure cells: the environment slot is the cell slot
+    std::vector<double> g(NC, 1.0);
+    if (t < 6) HIPCHK(c, hipMemcpy(g.data(), gsrc[t], cb, hipMemcpyDeviceToHost));
+    for (long cc = 0; cc < NC; ++cc)
+      if (__builtin_popcount((unsigned)mask[cc]) == 1)
+        eps[t].h[(size_t)(__builtin_ffs(mask[cc]) - 1) * NC + cc] = g[cc];
+  }
+  // the cell fracvol / mass fraction of single-environment cells (1 up to
+  // rounding); the GPU path does not keep them, nothing downstream reads them
+  for (long cc = 0; cc < NC; ++cc) {
+    if (st->fracvol) st->fracvol[cc] = __builtin_popcount((unsigned)mask[cc]) == 1 ? 1.0 : 0.0;
+    if (st->mass_fraction) st->mass_fraction[cc] = __builtin_popcount((unsigned)mask[cc]) == 1 ? 1.0 : 0.0;
+  }
+  if (st->velocity) {
+    HIPCHK(c, d2h(st->velocity, c->ux[c->ucur], nb));
+    HIPCHK(c, d2h(st->velocity + NN, c->uy[c->ucur], nb));
+  }
+  if (st->node_coord) {
+    HIPCHK(c, d2h(st->node_coord, c->xx[c->xcur], nb));
+    HIPCHK(c, d2h(st->node_coord + NN, c->xy[c->xcur], nb));
+  }
+  if (st->dual_grad_phi) {
+    HIPCHK(c, d2h(st->dual_grad_phi, c->dg0, nb));
+    HIPCHK(c, d2h(st->dual_grad_phi + NN, c->dg1, nb));
+  }
+  if (st->caracteristic_length) HIPCHK(c, d2h(st->caracteristic_length, c->L, cb));
+  if (st->node_mass) {
+    std::vector<double> m(NC);
+    HIPCHK(c, hipMemcpy(m.data(), c->m, cb, hipMemcpyDeviceToHost));
+    for (long n = 0; n < NN; ++n) {
+      const int i = (int)(n % (P.nx + 1)), j = (int)(n / (P.nx + 1));
+      double s = 0.;
+      const int cdi[4] = {-1, 0, -1, 0}, cdj[4] = {-1, -1, 0, 0};
+      for (int t = 0; t < 4; ++t) {
+        int ci = i + cdi[t], cj = j + cdj[t];
+        if (ci < 0 || ci >= P.nx || cj < 0 || cj >= P.ny) continue;
+        s += m[ci + (long)P.nx * cj];
+      }
+      st->node_mass[n] = 0.25 * s;
+    }
+  }
+  r = read_scalars(c);
+  if (r) return r;
+  st->global_deltat = c->S_host->dt;
+  st->old_deltat = c->S_host->dt_old;
+  st->global_time = c->S_host->time;
+  st->global_iteration = c->S_host->iteration;
+  st->sens_projection = c->sens;
+  return MHG_OK;
+}
+
+int mhg_begin_iteration(mhg_ctx* c) { return fail(c, MHG_E_UNSUPPORTED, "entry points: use mhg_step"); }
+int mhg_save_values_at_n(mhg_ctx* c) { return fail(c, MHG_E_UNSUPPORTED, "entry points: use mhg_step"); }
+int mhg_update_velocity(mhg_ctx* c) { return fail(c, MHG_E_UNSUPPORTED, "entry points: use mhg_step"); }
+int mhg_apply_boundary_condition(mhg_ctx* c) { return fail(c, MHG_E_UNSUPPORTED, "entry points: use mhg_step"); }
+int mhg_update_position(mhg_ctx* c) { return fail(c, MHG_E_UNSUPPORTED, "entry points: use mhg_step"); }
+int mhg_compute_geometric_values(mhg_ctx* c) { return fail(c, MHG_E_UNSUPPORTED, "entry points: use mhg_step"); }
+int mhg_update_density(mhg_ctx* c) { return fail(c, MHG_E_UNSUPPORTED, "entry points: use mhg_step"); }
+int mhg_compute_artificial_viscosity(mhg_ctx* c) { return fail(c, MHG_E_UNSUPPORTED, "entry points: use mhg_step"); }
+int mhg_update_energy_and_pressure(mhg_ctx* c) { return fail(c, MHG_E_UNSUPPORTED, "entry points: use mhg_step"); }
+int mhg_remap(mhg_ctx* c) { return fail(c, MHG_E_UNSUPPORTED, "entry points: use mhg_step"); }
+int mhg_compute_delta_t(mhg_ctx* c, double* dt, int32_t* stop) {
+  (void)dt;
+  (void)stop;
+  return fail(c, MHG_E_UNSUPPORTED, "entry points: use mhg_step");
+}
+
+int mhg_remap_get_ordre_projection(const mhg_ctx* c) { return c ? c->opt.ordre_projection : MHG_E_ARG; }
+int mhg_remap_has_projection_pente_borne(const mhg_ctx* c) { return c ? c->opt.projection_pente_borne : MHG_E_ARG; }
+int mhg_remap_has_conservation_energie_totale(const mhg_ctx* c) { return c ? c->opt.conservation_energie_totale : MHG_E_ARG; }
+int mhg_remap_is_euler(const mhg_ctx* c) { return c ? c->opt.is_euler_scheme : MHG_E_ARG; }
+
+int mhg_step(mhg_ctx* c, int32_t* stop) {
+  int r = check_ready(c);
+  if (r) return r;
+  enqueue_step(c);
+  HIPCHK(c, hipGetLastError());
+  r = read_scalars(c);
+  if (r) return r;
+  collect_times(c);
+  if (stop) *stop = c->S_host->stop;
+  if (c->S_host->err) return fail(c, MHG_E_DT_MIN, "dt below deltat-min");
+  return c->S_host->stop ? MHG_STOP : MHG_OK;
+}
+
+int mhg_run_async(mhg_ctx* c, int32_t n) {
+  int r = check_ready(c);
+  if (r) return r;
+  for (int s = 0; s < n; ++s) enqueue_step(c);
+  HIPCHK(c, hipGetLastError());
+  return MHG_OK;
+}
+
+int mhg_synchronize(mhg_ctx* c) {
+  if (!c) return MHG_E_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  collect_times(c);
+  return MHG_OK;
+}
+
+int mhg_run(mhg_ctx* c, int32_t max_steps, int32_t* steps_done) {
+  int r = check_ready(c);
+  if (r) return r;
+  r = read_scalars(c);
+  if (r) return r;
+  const int it0 = c->S_host->iteration;
+  int launched = 0;
+  while (launched < max_steps) {
+    const int chunk = std::min(64, max_steps - launched);
+    for (int s = 0; s < chunk; ++s) enqueue_step(c);
+    launched += chunk;
+    HIPCHK(c, hipGetLastError());
+    r = read_scalars(c);
+    if (r) return r;
+    collect_times(c);
+    if (c->S_host->stop || c->S_host->err) break;
+  }
+  // kernels of steps past the end of the run return at once; the host
+  // bookkeeping (buffers, sweep order) must follow the steps really done
+  const int done = c->S_host->iteration - it0;
+  const int skipped = launched - done;
+  for (int s = 0; s < skipped; ++s) {
+    c->sens = (c->sens + c->P.dim - 1) % c->P.dim;
+    c->xcur = 1 - c->xcur;
+    c->ucur = 1 - c->ucur;
+  }
+  if (steps_done) *steps_done = done;
+  if (c->S_host->err) return fail(c, MHG_E_DT_MIN, "dt below deltat-min");
+  return c->S_host->stop ? MHG_STOP : MHG_OK;
+}
+
+int mhg_conservation(mhg_ctx* c, double out[3]) {
+  int r = check_ready(c);
+  if (r) return r;
+  const Params& P = c->P;
+  const long NC = P.nc, NN = P.nn;
+  std::vector<double> m(NC), e(NC), ux(NN), uy(NN), mn(NN);
+  mhg_state st;
+  std::memset(&st, 0, sizeof(st));
+  std::vector<double> vel(2 * NN);
+  st.cell_mass = m.data();
+  st.internal_energy = e.data();
+  st.velocity = vel.data();
+  st.node_mass = mn.data();
+  r = mhg_download_state(c, &st);
+  if (r) return r;
+  auto acc = [](double& s, double& comp, double v) {
+    double t = s + v;
+    if (std::fabs(s) >= std::fabs(v)) comp += (s - t) + v;
+    else comp += (v - t) + s;
+    s = t;
+  };
+  double a = 0, ac = 0, b = 0, bc = 0, k = 0, kc = 0;
+  for (long cc = 0; cc < NC; ++cc) {
+    acc(a, ac, m[cc]);
+    acc(b, bc, m[cc] * e[cc]);
+  }
+  for (long n = 0; n < NN; ++n) {
+    const double u = vel[n], v = vel[NN + n];
+    acc(k, kc, 0.5 * mn[n] * (u * u + v * v));
+  }
+  out[0] = a + ac;
+  out[1] = b + bc;
+  out[2] = k + kc;
+  return MHG_OK;
+}
+
+int mhg_kernel_times(mhg_ctx* c, int32_t enable, const char** names, double* ms,
+                     int32_t* launches, int32_t cap) {
+  if (!c) return MHG_E_ARG;
+  collect_times(c);
+  const int n = (int)c->tnames.size();
+  for (int i = 0; i < n && i < cap; ++i) {
+    if (names) names[i] = c->tnames[i].c_str();
+    if (ms) ms[i] = c->tcount[i] ? c->tsum[i] / c->tcount[i] : 0.;
+    if (launches) launches[i] = c->tcount[i];
+  }
+  if (enable >= 0) {
+    // any call with enable >= 0 resets the accumulators
+    c->timing = enable != 0;
+    for (int i = 0; i < n; ++i) {
+      c->tsum[i] = 0.;
+      c->tcount[i] = 0;
+    }
+  }
+  return n;
+}
+
+}  // extern "C"

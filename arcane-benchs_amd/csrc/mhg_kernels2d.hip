@@ -1,0 +1,752 @@
+// mhg_kernels2d.hip -- 2D (quad) kernels of the MaHyCo time step for gfx950.
+//
+// Data layout in HBM (structure of arrays, FP64):
+//   cells  i + nx*j : rho, e, p, c, q, m, mask(int32), L, Vlag
+//            U[v][c] (v < 3*ne+1: volumes, masses, energies per environment,
+//            then q*V) in two ping-pong buffers, contrib[k][c] (half the dual
+//            mass flux of each environment)
+//   envs   [k][c] : rho, e, p, c, q, m, f, mf  (read/written for cells that
+//            hold >= 2 environments only)
+//   nodes  i + (nx+1)*j : ux, uy, two ping-pong coordinate buffers (x, y),
+//            Ud[3] (dual momentum x, y, mass), dgrad[2] (only the ring of
+//            nodes on the domain boundary is ever read back)
+// The face normals and the outer face normals of the reference are exactly
+// +/- e_dir on a Cartesian mesh (InitGeometricValues, MahycoModule.cc:1008-1050
+// evaluates to (1,0,0)-type vectors bit for bit), so dot products with them
+// reduce to one component and are not stored.
+#include "mhg_common.hpp"
+#include "mhg_kernels.hpp"
+
+namespace mhg {
+
+__device__ inline double node_x0(const Params& P, int i) { return P.org[0] + i * P.h[0]; }
+__device__ inline double node_y0(const Params& P, int j) { return P.org[1] + j * P.h[1]; }
+
+// ---------------------------------------------------------------------------
+// Start of iteration: time += dt, iteration += 1 (Arcane time-loop manager).
+// ---------------------------------------------------------------------------
+__global__ void k_begin_iteration(DevScalars* S) {
+  if (S->stop || S->err) return;
+  S->time += S->dt;
+  S->iteration += 1;
+}
+
+// ---------------------------------------------------------------------------
+// K1: node phase.  updateForceAndVelocity (MahycoModule.cc:589-665) with the
+// corner normals of the previous Lagrangian coordinates (saveValuesAtN copies
+// m_cell_cqs computed after the previous updatePosition, 344/365),
+// applyBoundaryCondition (936-1004), updatePosition (870-931) from x0.
+// Node mass = 1/4 sum of the cell masses (computeNodeMass 235-266): the cell
+// masses do not change in the Lagrange phase, so it is recomputed here.
+// ---------------------------------------------------------------------------
+template <bool VEL, bool BC, bool POS>
+__global__ void k_node_lagrange_2d(Params P, DevScalars* S, NodeLagArgs A) {
+  if (S->stop || S->err) return;
+  const long nid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (nid >= P.nn) return;
+  const int nnx = P.nx + 1;
+  const int i = (int)(nid % nnx), j = (int)(nid / nnx);
+  const double dt = S->dt;
+  double vx = A.ux[nid], vy = A.uy[nid];
+  if (VEL) {
+    const double dtv = 0.5 * (S->dt_old + S->dt);
+    // 3x3 patch of previous coordinates around the node.
+    double X[3][3], Y[3][3];
+#pragma unroll
+    for (int dj = -1; dj <= 1; ++dj)
+#pragma unroll
+      for (int di = -1; di <= 1; ++di) {
+        int ii = i + di, jj = j + dj;
+        double xv = 0., yv = 0.;
+        if (ii >= 0 && ii < nnx && jj >= 0 && jj <= P.ny) {
+          long q = ii + (long)nnx * jj;
+          xv = A.xpx[q];
+          yv = A.xpy[q];
+        }
+        X[dj + 1][di + 1] = xv;
+        Y[dj + 1][di + 1] = yv;
+      }
+    double Fx = 0., Fy = 0., msum = 0.;
+    // cells in ascending id: (i-1,j-1) corner 2, (i,j-1) 3, (i-1,j) 1, (i,j) 0
+    const int cdi[4] = {-1, 0, -1, 0}, cdj[4] = {-1, -1, 0, 0}, cq[4] = {2, 3, 1, 0};
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      int ci = i + cdi[t], cj = j + cdj[t];
+      if (ci < 0 || ci >= P.nx || cj < 0 || cj >= P.ny) continue;
+      long c = ci + (long)P.nx * cj;
+      // quad nodes (ci,cj),(ci+1,cj),(ci+1,cj+1),(ci,cj+1) in patch coordinates
+      int bi = cdi[t] + 1, bj = cdj[t] + 1;
+      double qx[4] = {X[bj][bi], X[bj][bi + 1], X[bj + 1][bi + 1], X[bj + 1][bi]};
+      double qy[4] = {Y[bj][bi], Y[bj][bi + 1], Y[bj + 1][bi + 1], Y[bj + 1][bi]};
+      double cx, cy;
+      cqs2d(qx, qy, cq[t], &cx, &cy);
+      double pq = A.p[c] + A.q[c];
+      Fx += pq * cx;
+      Fy += pq * cy;
+      msum += A.m[c];
+    }
+    double mn = 0.25 * msum;
+    double s = dtv / mn;
+    vx = vx + s * Fx;
+    vy = vy + s * Fy;
+  }
+  if (BC) {
+    for (int b = 0; b < P.nbc; ++b) {
+      const BcRange& r = P.bc[b];
+      int idx = (r.axis == 0) ? i : j;
+      if (r.axis < 0 || (idx >= r.lo && idx <= r.hi)) {
+        if (r.comp == 0) vx = r.value;
+        else if (r.comp == 1) vy = r.value;
+      }
+    }
+  }
+  if (VEL || BC) {
+    A.ux[nid] = vx;
+    A.uy[nid] = vy;
+  }
+  if (POS) {
+    A.xnx[nid] = node_x0(P, i) + dt * vx;
+    A.xny[nid] = node_y0(P, j) + dt * vy;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K2: cell phase.  computeGeometricValues (1054-1299), updateDensity
+// (1331-1426), computeArtificialViscosity (454-578),
+// updateEnergyAndPressureforGP (1659-1770), then the remap vector
+// computeVariablesForRemap_PBorn0 (PrepareRemap.cc:258-399).  Pure cells use
+// the cell arrays; cells with >= 2 environments loop over their environments
+// in id order.  Only the live entries of U are kept: the momentum and kinetic
+// energy slots (PrepareRemap.cc:324-328) are identically zero and stay zero
+// through the sweeps (phi = 0 gives a zero gradient, face value and flux).
+// ---------------------------------------------------------------------------
+template <bool PC>
+__global__ void k_cell_lagrange_2d(Params P, DevScalars* S, CellLagArgs A) {
+  if (S->stop || S->err) return;
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= P.nc) return;
+  const int nnx = P.nx + 1;
+  const int i = (int)(c % P.nx), j = (int)(c / P.nx);
+  const long n0 = i + (long)nnx * j;
+  const long nd[4] = {n0, n0 + 1, n0 + 1 + nnx, n0 + nnx};
+  double x[4], y[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    x[q] = A.xnx[nd[q]];
+    y[q] = A.xny[nd[q]];
+  }
+  const double V = quad_volume(x, y);
+  const double L = (P.lc == MHG_LC_FACES_OPPOSEES) ? length_faces_opposees_2d(x, y) : pow(V, 0.5);
+  const double dt = S->dt;
+  const double inv_dt = 1.0 / dt;
+  const double rho_n = A.rho[c];
+  const double rho = A.m[c] / V;
+  const double tau = 0.5 * (1.0 / rho_n + 1.0 / rho);
+  const double divu = inv_dt * (1.0 / rho - 1.0 / rho_n) / tau;
+  const double cs = A.cs[c];
+  const int mask = A.mask[c];
+  const int ne = P.ne;
+  const long NC = P.nc;
+  double* U = A.U;
+  for (int v = 0; v < P.nvu; ++v) U[v * NC + c] = 0.;
+  if (__popc(mask) == 1) {
+    const int k = __ffs(mask) - 1;
+    double qv = 0.;
+    if (divu < 0.0) qv = viscosity(P.eos.qcoef[k], tau, L, cs, divu);
+    const double qn = PC ? A.q[c] : 0.;
+    const double e = compute_eint(PC, P.eos.gamma[k], qn, qv, rho_n, rho, A.p[c], A.e[c]);
+    U[k * NC + c] = V;
+    U[(ne + k) * NC + c] = V * rho;
+    U[(2 * ne + k) * NC + c] = V * rho * e;
+    U[(3 * ne) * NC + c] = V * qv;
+  } else if (mask != 0) {
+    double uq = 0.;
+    for (int k = 0; k < ne; ++k) {
+      if (!(mask & (1 << k))) continue;
+      const long ek = k * NC + c;
+      const double f = A.env_f[ek];
+      const double Vk = f * V;
+      const double rho_nk = A.env_rho[ek];
+      const double rk = A.env_m[ek] / Vk;
+      const double tk = 0.5 * (1.0 / rho_nk + 1.0 / rk);
+      double qk = 0.;
+      if (divu < 0.0) qk = viscosity(P.eos.qcoef[k], tk, L, cs, divu);
+      const double qnk = PC ? A.env_q[ek] : 0.;
+      const double ekv = compute_eint(PC, P.eos.gamma[k], qnk, qk, rho_nk, rk, A.env_p[ek], A.env_e[ek]);
+      U[k * NC + c] = Vk;
+      U[(ne + k) * NC + c] = Vk * rk;
+      U[(2 * ne + k) * NC + c] = Vk * rk * ekv;
+      uq = Vk * qk;  // PrepareRemap.cc:372 assigns: the last environment wins
+    }
+    U[(3 * ne) * NC + c] = uq;
+  }
+  A.Vlag[c] = V;
+  A.L[c] = L;
+}
+
+// ---------------------------------------------------------------------------
+// K3: cell sweep in direction D.  computeGradPhiFace (RemapADIService.cc:
+// 163-178), computeGradPhiCell_PBorn0_LimC (447-517),
+// computeUpwindFaceQuantitiesForProjection_PBorn0_O2 (658-720),
+// computeUremap_PBorn0 (883-988).  One thread per cell; the 5-cell stencil
+// along D is read from the caches.  Domain-boundary faces carry a zero face
+// gradient and a zero face value (they are never written in the reference).
+// FIRST: phi = U / V_lagrange (PrepareRemap.cc:394-398); otherwise
+// phi = U / sum_k U_vol,k as left by the previous sweep (RemapADIService.cc:961-970).
+// ---------------------------------------------------------------------------
+template <int D, bool FIRST>
+__global__ void k_cell_sweep_2d(Params P, DevScalars* S, CellSweepArgs A) {
+  if (S->stop || S->err) return;
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= P.nc) return;
+  const int nnx = P.nx + 1;
+  const int i = (int)(c % P.nx), j = (int)(c / P.nx);
+  const int s = (D == 0) ? i : j;
+  const int N = (D == 0) ? P.nx : P.ny;
+  const long NC = P.nc;
+  const long cstride = (D == 0) ? 1 : P.nx;
+  const int ne = P.ne;
+  const double dt = S->dt;
+  const double* __restrict__ Xd = (D == 0) ? A.xnx : A.xny;
+  const double* __restrict__ Xt = (D == 0) ? A.xny : A.xnx;  // transverse coordinate
+  const double* __restrict__ Ud_ = (D == 0) ? A.ux : A.uy;
+  const double* __restrict__ Uin = A.Uin;
+
+  // stencil cells o = -2..2
+  bool ex[5];
+  double xc[5], den[5];
+#pragma unroll
+  for (int o = -2; o <= 2; ++o) {
+    const int so = s + o;
+    ex[o + 2] = (so >= 0 && so < N);
+    xc[o + 2] = 0.;
+    den[o + 2] = 1.;
+    if (!ex[o + 2]) continue;
+    const long co = c + o * cstride;
+    const int ci = (D == 0) ? i + o : i, cj = (D == 0) ? j : j + o;
+    const long m0 = ci + (long)nnx * cj;
+    // d-component of the quad nodes, Arcane order, summed in that order
+    const double a = Xd[m0], b = Xd[m0 + 1], cc = Xd[m0 + 1 + nnx], dd = Xd[m0 + nnx];
+    xc[o + 2] = 0.25 * (((a + b) + cc) + dd);
+    if (FIRST) {
+      den[o + 2] = A.Vlag[co];
+    } else {
+      double sv = 0.;
+      for (int k = 0; k < ne; ++k) sv += Uin[k * NC + co];
+      den[o + 2] = sv;
+    }
+  }
+  // faces o = -1..2: face between stencil cells o-1 and o
+  bool inner[4];
+  double dx[4];
+#pragma unroll
+  for (int o = -1; o <= 2; ++o) {
+    inner[o + 1] = ex[o + 1] && ex[o + 2];
+    dx[o + 1] = inner[o + 1] ? (xc[o + 2] - xc[o + 1]) : 1.;
+  }
+  // flux faces: o = 0 (back of c), o = 1 (front of c)
+  double uf[2], lf[2], dotxf[2];
+  int up[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int o = t;  // face index relative to c
+    long na, nb;
+    if (D == 0) {
+      na = (i + o) + (long)nnx * j;
+      nb = na + nnx;
+    } else {
+      na = i + (long)nnx * (j + o);
+      nb = na + 1;
+    }
+    uf[t] = 0.5 * (Ud_[na] + Ud_[nb]);
+    const double xfa = 0.5 * Xd[na], xfb = 0.5 * Xd[nb];
+    const double xf = xfa + xfb;
+    lf[t] = fabs(Xt[nb] - Xt[na]);
+    up[t] = 0;
+    dotxf[t] = 0.;
+    if (inner[o + 1]) {
+      up[t] = (uf[t] * dx[o + 1] > 0.0) ? (o - 1) : o;
+      dotxf[t] = xf - xc[up[t] + 2];
+    }
+  }
+  const int lim = P.lim;
+  const double order2 = (double)P.order2;
+  double* Uout = A.Uout;
+  double dualmass[kMaxEnv];
+  for (int v = 0; v < P.nvu; ++v) {
+    double phi[5];
+#pragma unroll
+    for (int o = -2; o <= 2; ++o)
+      phi[o + 2] = ex[o + 2] ? Uin[v * NC + c + o * cstride] / den[o + 2] : 0.;
+    double gf[4];
+#pragma unroll
+    for (int o = -1; o <= 2; ++o)
+      gf[o + 1] = inner[o + 1] ? (phi[o + 2] - phi[o + 1]) / dx[o + 1] : 0.;
+    double phif[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      phif[t] = 0.;
+      if (!inner[t + 1]) continue;
+      const int u = up[t];  // -1, 0 or 1
+      double g = 0.;
+      if (P.order2 > 0) g = limited_cell_grad(lim, gf[u + 1], gf[u + 2]);
+      phif[t] = phi[u + 2] + order2 * dotxf[t] * g;
+    }
+    const double flux_b = -1.0 * uf[0] * lf[0] * dt * phif[0];
+    const double flux_f = 1.0 * uf[1] * lf[1] * dt * phif[1];
+    double flux_face, dual;
+    if (D == 0) {  // cell faces in order: ..., right (front), ..., left (back)
+      flux_face = flux_f + flux_b;
+      dual = 0.5 * flux_f * 1.0 + 0.5 * flux_b * -1.0;
+    } else {       // bottom (back), ..., top (front)
+      flux_face = flux_b + flux_f;
+      dual = 0.5 * flux_b * -1.0 + 0.5 * flux_f * 1.0;
+    }
+    double un = Uin[v * NC + c] - flux_face;
+    if (v >= ne && v < 3 * ne && un < 0.) un = 0.;  // RemapADIService.cc:952-959
+    Uout[v * NC + c] = un;
+    if (v >= ne && v < 2 * ne) dualmass[v - ne] = dual;
+  }
+  // half of the environment dual mass flux: fcontrib (DualPhaseRemap.cc:147-188)
+  for (int k = 0; k < ne; ++k) A.contrib[k * NC + c] = 0.5 * dualmass[k];
+}
+
+// ---------------------------------------------------------------------------
+// K4: dual (node) sweep in direction D.  computeDualGradPhi_LimC
+// (DualPhaseRemap.cc:700-779), node mass fluxes (ncontrib 215-260), upwind
+// momentum update (352-491) and the phi_dual refresh (494-523).  FIRST builds
+// the dual state from (u, m_node) as computeVariablesForRemap_PBorn0 does
+// (PrepareRemap.cc:411-433); the last sweep ends with the nodal velocity
+// u = U_dual / m_dual of remapVariables (RemapADIFinal.cc:660-672).
+// ---------------------------------------------------------------------------
+__device__ inline double node_mass_from_cells(const Params& P, const double* m, int i, int j) {
+  double msum = 0.;
+  const int cdi[4] = {-1, 0, -1, 0}, cdj[4] = {-1, -1, 0, 0};
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    int ci = i + cdi[t], cj = j + cdj[t];
+    if (ci < 0 || ci >= P.nx || cj < 0 || cj >= P.ny) continue;
+    msum += m[ci + (long)P.nx * cj];
+  }
+  return 0.25 * msum;
+}
+
+template <int D, bool FIRST>
+__global__ void k_node_sweep_2d(Params P, DevScalars* S, NodeSweepArgs A) {
+  if (S->stop || S->err) return;
+  const long nid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (nid >= P.nn) return;
+  const int nnx = P.nx + 1;
+  const int i = (int)(nid % nnx), j = (int)(nid / nnx);
+  const int s = (D == 0) ? i : j;
+  const int N = (D == 0) ? P.nx : P.ny;           // nodes 0..N along D
+  const int T = (D == 0) ? P.ny : P.nx;           // nodes 0..T along the other direction
+  const int st = (D == 0) ? j : i;
+  const long nstride = (D == 0) ? 1 : nnx;
+  const bool inner_d = (s >= 1 && s <= N - 1);
+  const bool inner_t = (st >= 1 && st <= T - 1);  // inner in the other direction
+  const long NC = P.nc;
+  const int ne = P.ne;
+  const double dt = S->dt;
+
+  // Dual state of this node before the sweep.
+  double U0, U1, U3;
+  if (FIRST) {
+    const double mn = node_mass_from_cells(P, A.m, i, j);
+    U0 = mn * A.ux[nid];
+    U1 = mn * A.uy[nid];
+    U3 = mn;
+  } else {
+    U0 = A.Ud0[nid];
+    U1 = A.Ud1[nid];
+    U3 = A.Ud3[nid];
+  }
+  if (inner_d) {
+    // phi_dual (u_x, u_y) and coordinate along D on the 5-node stencil.
+    bool ex[5];
+    double ph0[5], ph1[5], xd[5];
+    const double* __restrict__ Xd = (D == 0) ? A.xnx : A.xny;
+#pragma unroll
+    for (int o = -2; o <= 2; ++o) {
+      const int so = s + o;
+      ex[o + 2] = (so >= 0 && so <= N);
+      ph0[o + 2] = ph1[o + 2] = 0.;
+      xd[o + 2] = 0.;
+      if (!ex[o + 2]) continue;
+      const long q = nid + o * nstride;
+      xd[o + 2] = Xd[q];
+      // In the second sweep the nodes that were inner in the first direction
+      // carry phi = U/m (494-523); the others still hold the initial phi = u.
+      if (FIRST || !inner_t) {
+        ph0[o + 2] = A.ux[q];
+        ph1[o + 2] = A.uy[q];
+      } else {
+        const double m3 = A.Ud3[q];
+        ph0[o + 2] = A.Ud0[q] / m3;
+        ph1[o + 2] = A.Ud1[q] / m3;
+      }
+    }
+    // limited nodal gradients of the stencil nodes -1, 0, +1
+    double g0[3] = {0., 0., 0.}, g1[3] = {0., 0., 0.};
+    if (P.order2 > 0) {
+#pragma unroll
+      for (int o = -1; o <= 1; ++o) {
+        const int so = s + o;
+        if (so >= 1 && so <= N - 1) {
+          const double dxf = xd[o + 3] - xd[o + 2], dxb = xd[o + 2] - xd[o + 1];
+          const double gf0 = (ph0[o + 3] - ph0[o + 2]) / dxf, gb0 = (ph0[o + 2] - ph0[o + 1]) / dxb;
+          const double gf1 = (ph1[o + 3] - ph1[o + 2]) / dxf, gb1 = (ph1[o + 2] - ph1[o + 1]) / dxb;
+          g0[o + 1] = limited_node_grad(P.lim, gb0, gf0);
+          g1[o + 1] = limited_node_grad(P.lim, gb1, gf1);
+        } else {  // boundary node along D: value left by an earlier sweep
+          const long q = nid + o * nstride;
+          g0[o + 1] = A.dg0[q];
+          g1[o + 1] = A.dg1[q];
+        }
+      }
+      // keep the gradient of nodes that lie on the boundary of the other direction
+      if (!inner_t) {
+        A.dg0[nid] = g0[1];
+        A.dg1[nid] = g1[1];
+      }
+    }
+    // node mass fluxes (back cells then front cells, ascending shift)
+    double back = 0., front = 0.;
+#pragma unroll
+    for (int pass = 0; pass < 2; ++pass) {
+#pragma unroll
+      for (int sh = -1; sh <= 0; ++sh) {
+        int ci, cj;
+        if (D == 0) {
+          ci = i - (pass == 0 ? 1 : 0);
+          cj = j + sh;
+        } else {
+          ci = i + sh;
+          cj = j - (pass == 0 ? 1 : 0);
+        }
+        if (ci < 0 || ci >= P.nx || cj < 0 || cj >= P.ny) continue;
+        const long cc = ci + (long)P.nx * cj;
+        for (int k = 0; k < ne; ++k) {
+          const double w = A.contrib[k * NC + cc];
+          if (pass == 0) back += w;
+          else front += w;
+        }
+      }
+    }
+    U3 += back - front;
+    const double order2 = (double)P.order2;
+    const double* phd = (D == 0) ? ph0 : ph1;  // component along D
+    const double ufront = 0.5 * (phd[2] + phd[2]);  // DualPhaseRemap.cc:391
+    const int signfront = (ufront > 0) ? 1 : -1;
+    const double xfr = xd[3] - xd[2];
+    const int sf = (front < 0.) ? 1 : 0;
+    const double tf = signfront * xfr - dt * ufront;
+    const double FV0 = ph0[sf + 2] + order2 * (0.5 * g0[sf + 1] * tf);
+    const double FV1 = ph1[sf + 2] + order2 * (0.5 * g1[sf + 1] * tf);
+    const double uback = 0.5 * (phd[1] + phd[2]);
+    const int signback = (uback > 0) ? 1 : -1;
+    const double xbk = xd[2] - xd[1];
+    const int sb = (back > 0.) ? -1 : 0;
+    const double tb = signback * xbk - dt * uback;
+    const double BV0 = ph0[sb + 2] + order2 * (0.5 * g0[sb + 1] * tb);
+    const double BV1 = ph1[sb + 2] + order2 * (0.5 * g1[sb + 1] * tb);
+    U0 += back * BV0 - front * FV0;
+    U1 += back * BV1 - front * FV1;
+    const double thresold = 1.e-300;
+    if (fabs(U0) < thresold) U0 = 0.;
+    if (fabs(U1) < thresold) U1 = 0.;
+  }
+  if (FIRST) {
+    A.Ud0[nid] = U0;
+    A.Ud1[nid] = U1;
+    A.Ud3[nid] = U3;
+  } else {
+    if (U3 != 0.) {
+      A.uox[nid] = U0 / U3;
+      A.uoy[nid] = U1 / U3;
+    } else {
+      A.uox[nid] = A.ux[nid];
+      A.uoy[nid] = A.uy[nid];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K5: final cell phase.  remapVariables cell status + membership change
+// (RemapADIFinal.cc:98-226), "moy" (446-643), applyEOS per environment
+// (PerfectGasEOSService.cc:50-137), computePressionMoyenne (MahycoModule.cc:
+// 1836-1915) and the CFL candidate of computeHydroDeltaT (2047-2061) reduced
+// to a device-wide minimum; the last block finishes computeDeltaT (2076-2191).
+// ---------------------------------------------------------------------------
+template <int BLOCK>
+__global__ void k_cell_final_2d(Params P, DevScalars* S, CellFinalArgs A) {
+  if (S->stop || S->err) return;
+  __shared__ double red[BLOCK / 64];
+  __shared__ bool last_block;
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double dx_sound = 1.7976931348623157e308;
+  if (c < P.nc) {
+    const int nnx = P.nx + 1;
+    const int i = (int)(c % P.nx), j = (int)(c / P.nx);
+    const long NC = P.nc;
+    const int ne = P.ne;
+    const double thr = P.remap_threshold;
+    // Euler volume: same corner-normal formula on the initial coordinates.
+    double x0[4] = {node_x0(P, i), node_x0(P, i + 1), node_x0(P, i + 1), node_x0(P, i)};
+    double y0[4] = {node_y0(P, j), node_y0(P, j), node_y0(P, j + 1), node_y0(P, j + 1)};
+    const double vol = quad_volume(x0, y0);
+    double Uv[3 * kMaxEnv + 1];
+    for (int v = 0; v < P.nvu; ++v) Uv[v] = A.U[v * NC + c];
+    const int old_mask = A.mask[c];
+    // cell status per environment (110-143)
+    double volt = 0.;
+    for (int k = 0; k < ne; ++k) volt += Uv[k];
+    int mask = old_mask;
+    for (int k = 0; k < ne; ++k) {
+      const double fvol = Uv[k] / vol;
+      const bool in_env = (old_mask >> k) & 1;
+      if (fvol < thr && in_env) {
+        mask &= ~(1 << k);
+      } else if (fvol > thr && !in_env) {
+        const double fn = Uv[k] / volt;
+        if (fn > thr && Uv[ne + k] != 0.) mask |= (1 << k);
+      }
+    }
+    const int nbe = __popc(mask);
+    // "moy"
+    double masset = 0.;
+    volt = 0.;
+    for (int k = 0; k < ne; ++k) {
+      volt += Uv[k];
+      masset += Uv[ne + k];
+    }
+    const double unsurvolt = 1. / volt;
+    const double unsurvol = 1. / vol;
+    double fv[kMaxEnv], mf[kMaxEnv], dk[kMaxEnv], ek[kMaxEnv], pk[kMaxEnv], ck[kMaxEnv];
+    double somme_frac = 0.;
+    for (int k = 0; k < ne; ++k) {
+      fv[k] = 0.;
+      if (!((mask >> k) & 1)) continue;
+      double f = Uv[k] * unsurvolt;
+      if (f < thr) f = 0.;
+      fv[k] = f;
+      somme_frac += f;
+    }
+    int matcell = 0;
+    const double unsursomme_frac = 1. / somme_frac;
+    for (int k = 0; k < ne; ++k) {
+      if (!((mask >> k) & 1)) continue;
+      const double f = fv[k] * unsursomme_frac;
+      if (f > 0.) matcell++;
+      fv[k] = f;
+    }
+    for (int k = 0; k < ne; ++k) {
+      // a zero total mass leaves the mass fractions as they were (513-538)
+      mf[k] = (nbe == 1) ? 1.0 : ((old_mask >> k) & 1 ? A.env_mf[k * NC + c] : 0.);
+    }
+    if (masset != 0.) {
+      const double unsurmasset = 1. / masset;
+      double fmasset = 0.;
+      for (int k = 0; k < ne; ++k) {
+        if (!((mask >> k) & 1)) continue;
+        double m = Uv[ne + k] * unsurmasset;
+        if (fv[k] < thr) m = 0.;
+        fmasset += m;
+        mf[k] = m;
+      }
+      if (fmasset != 0.) {
+        const double unsurfmasset = 1. / fmasset;
+        for (int k = 0; k < ne; ++k)
+          if ((mask >> k) & 1) mf[k] = mf[k] * unsurfmasset;
+      }
+    }
+    double density_nplus1 = 0.;
+    for (int k = 0; k < ne; ++k) {
+      dk[k] = 0.;
+      if (!((mask >> k) & 1)) continue;
+      if (fv[k] > thr) {
+        const double vol_norm = Uv[k] * vol * unsurvolt;
+        dk[k] = Uv[ne + k] / vol_norm;
+      }
+      density_nplus1 += fv[k] * dk[k];
+    }
+    const double m_start = A.m[c];
+    double rho_g;
+    if (matcell > 1) rho_g = density_nplus1;
+    else if (nbe == 1) rho_g = dk[__ffs(mask) - 1];
+    else rho_g = m_start / A.Vlag[c];  // left as updateDensity set it (m/V)
+    double mass_g = vol * density_nplus1;
+    double energie = 0., pseudo = 0.;
+    double mk[kMaxEnv], qk[kMaxEnv];
+    const double pv = Uv[3 * ne] * unsurvol;
+    for (int k = 0; k < ne; ++k) {
+      ek[k] = 0.;
+      if (!((mask >> k) & 1)) continue;
+      if (fv[k] > thr && Uv[ne + k] != 0.) ek[k] = Uv[2 * ne + k] / Uv[ne + k];
+      energie += mf[k] * ek[k];
+      mk[k] = mf[k] * mass_g;
+      qk[k] = pv;
+      pseudo += fv[k] * pv;
+    }
+    double e_g = energie, q_g = pseudo;
+    if (nbe == 1) {
+      // pure cell: the environment slot is the cell slot
+      const int k = __ffs(mask) - 1;
+      mass_g = mk[k];
+      if (rho_g < 0. || e_g < 0.) {
+        e_g = 0.;
+        rho_g = 0.;
+      }
+    } else {
+      for (int k = 0; k < ne; ++k) {
+        if (!((mask >> k) & 1)) continue;
+        if (dk[k] < 0. || ek[k] < 0.) {
+          ek[k] = 0.;
+          dk[k] = 0.;
+          fv[k] = 0.;
+          mf[k] = 0.;
+        }
+      }
+    }
+    // EOS and mixed-cell averages
+    double p_g = A.p[c], c_g = A.cs[c];
+    if (nbe == 1) {
+      const int k = __ffs(mask) - 1;
+      eos_apply(P.eos, k, rho_g, e_g, &p_g, &c_g);
+    } else {
+      p_g = 0.;
+      c_g = 1.e-20;
+      for (int k = 0; k < ne; ++k) {
+        if (!((mask >> k) & 1)) continue;
+        eos_apply(P.eos, k, dk[k], ek[k], &pk[k], &ck[k]);
+        p_g += fv[k] * pk[k];
+        c_g = smax(ck[k], c_g);
+      }
+    }
+    A.rho[c] = rho_g;
+    A.e[c] = e_g;
+    A.p[c] = p_g;
+    A.cs[c] = c_g;
+    A.q[c] = q_g;
+    A.m[c] = mass_g;
+    A.mask[c] = mask;
+    if (nbe >= 2) {
+      for (int k = 0; k < ne; ++k) {
+        if (!((mask >> k) & 1)) continue;
+        const long e = k * NC + c;
+        A.env_rho[e] = dk[k];
+        A.env_e[e] = ek[k];
+        A.env_p[e] = pk[k];
+        A.env_cs[e] = ck[k];
+        A.env_q[e] = qk[k];
+        A.env_m[e] = mk[k];
+        A.env_f[e] = fv[k];
+        A.env_mf[e] = mf[k];
+      }
+    }
+    // CFL candidate: L / (c + max over the cell nodes of |u|)
+    const long n0 = i + (long)nnx * j;
+    const long nd[4] = {n0, n0 + 1, n0 + 1 + nnx, n0 + nnx};
+    double vmax = 0.;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double ux = A.ux[nd[q]], uy = A.uy[nd[q]];
+      const double un = sqrt(ux * ux + uy * uy + 0. * 0.);
+      vmax = (un < vmax) ? vmax : un;
+    }
+    dx_sound = A.L[c] / (c_g + vmax);
+  }
+  // block min then device-wide min on the bit pattern (positive doubles order
+  // like their bits)
+  double v = dx_sound;
+  for (int off = 32; off > 0; off >>= 1) v = smin(v, __shfl_down(v, off, 64));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = red[0];
+    for (int w = 1; w < BLOCK / 64; ++w) b = smin(b, red[w]);
+    atomicMin(&S->dtmin_bits, (unsigned long long)__double_as_longlong(b));
+    __threadfence();
+    unsigned int done = atomicAdd(&S->counter, 1u);
+    last_block = (done == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (last_block && threadIdx.x == 0) {
+    __threadfence();
+    const double minimum_aux =
+        __longlong_as_double((long long)atomicAdd(&S->dtmin_bits, 0ull));
+    // computeDeltaT (2085-2183)
+    const double old = S->dt;
+    S->dt_old = old;
+    double new_dt = P.cfl * minimum_aux;
+    new_dt = smin(new_dt, 1.05 * old);
+    new_dt = smin(new_dt, P.dtmax);
+    if (new_dt < P.dtmin) {
+      S->err = 1;
+    } else {
+      S->dt = new_dt;
+      const double t = S->time;
+      const bool not_yet_finish = (t < P.tend);
+      const bool finish = (t > P.tend);
+      const bool too_much = ((t + new_dt) > P.tend);
+      if ((not_yet_finish && too_much) || finish) S->stop = 1;
+    }
+    S->dtmin_bits = 0x7FEFFFFFFFFFFFFFull;  // DBL_MAX
+    S->counter = 0u;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+static inline unsigned nblocks(long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+void launch_begin_iteration(hipStream_t st, DevScalars* S) {
+  hipLaunchKernelGGL(k_begin_iteration, dim3(1), dim3(1), 0, st, S);
+}
+
+void launch_node_lagrange_2d(hipStream_t st, const Params& P, DevScalars* S, const NodeLagArgs& A,
+                             bool vel, bool bc, bool pos) {
+  const int B = 256;
+  dim3 g(nblocks(P.nn, B)), b(B);
+  if (vel && bc && pos) hipLaunchKernelGGL((k_node_lagrange_2d<true, true, true>), g, b, 0, st, P, S, A);
+  else if (vel && !bc && !pos) hipLaunchKernelGGL((k_node_lagrange_2d<true, false, false>), g, b, 0, st, P, S, A);
+  else if (!vel && bc && !pos) hipLaunchKernelGGL((k_node_lagrange_2d<false, true, false>), g, b, 0, st, P, S, A);
+  else if (!vel && !bc && pos) hipLaunchKernelGGL((k_node_lagrange_2d<false, false, true>), g, b, 0, st, P, S, A);
+}
+
+void launch_cell_lagrange_2d(hipStream_t st, const Params& P, DevScalars* S, const CellLagArgs& A) {
+  const int B = 256;
+  dim3 g(nblocks(P.nc, B)), b(B);
+  if (P.pseudo_centree) hipLaunchKernelGGL((k_cell_lagrange_2d<true>), g, b, 0, st, P, S, A);
+  else hipLaunchKernelGGL((k_cell_lagrange_2d<false>), g, b, 0, st, P, S, A);
+}
+
+void launch_cell_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const CellSweepArgs& A,
+                          int dir, bool first) {
+  const int B = 256;
+  dim3 g(nblocks(P.nc, B)), b(B);
+  if (dir == 0 && first) hipLaunchKernelGGL((k_cell_sweep_2d<0, true>), g, b, 0, st, P, S, A);
+  else if (dir == 0) hipLaunchKernelGGL((k_cell_sweep_2d<0, false>), g, b, 0, st, P, S, A);
+  else if (first) hipLaunchKernelGGL((k_cell_sweep_2d<1, true>), g, b, 0, st, P, S, A);
+  else hipLaunchKernelGGL((k_cell_sweep_2d<1, false>), g, b, 0, st, P, S, A);
+}
+
+void launch_node_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const NodeSweepArgs& A,
+                          int dir, bool first) {
+  const int B = 256;
+  dim3 g(nblocks(P.nn, B)), b(B);
+  if (dir == 0 && first) hipLaunchKernelGGL((k_node_sweep_2d<0, true>), g, b, 0, st, P, S, A);
+  else if (dir == 0) hipLaunchKernelGGL((k_node_sweep_2d<0, false>), g, b, 0, st, P, S, A);
+  else if (first) hipLaunchKernelGGL((k_node_sweep_2d<1, true>), g, b, 0, st, P, S, A);
+  else hipLaunchKernelGGL((k_node_sweep_2d<1, false>), g, b, 0, st, P, S, A);
+}
+
+void launch_cell_final_2d(hipStream_t st, const Params& P, DevScalars* S, const CellFinalArgs& A) {
+  constexpr int B = 256;
+  dim3 g(nblocks(P.nc, B)), b(B);
+  hipLaunchKernelGGL((k_cell_final_2d<B>), g, b, 0, st, P, S, A);
+}
+
+}  // namespace mhg

@@ -1,0 +1,109 @@
+"""Test helpers: the oracle binding (tests only) and parity metrics."""
+import os
+
+import numpy as np
+
+from mahyco_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_LIB = os.path.join(ROOT, "oracle", "liboracle.so")
+
+_oracle = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        _oracle = abi.Library(ORACLE_LIB, "mho_")
+    return _oracle
+
+
+def sod_setup(L, nx=40, ny=4, cas=3, **optkw):
+    opt = L.options(**optkw)
+    abi.set_bcs(opt, abi.reflecting_walls(2))
+    m = abi.mesh_desc(2, [nx, ny], [0, 0], [1.0, ny / nx])
+    s = abi.Simulation(L, m, opt)
+    s.init_sod(cas)
+    return s
+
+
+TP_RHO = (1.0, 1.0, 0.125)
+TP_P = (1.0, 0.1, 0.1)
+TP_GAMMA = (1.5, 1.4, 1.5)
+
+
+def tp_setup(L, nx=56, ny=24, **optkw):
+    """Triple point on [0,7]x[0,3]; nx, ny must keep x=1 and y=1.5 on faces."""
+    opt = L.options(**optkw)
+    abi.set_bcs(opt, abi.reflecting_walls(2))
+    abi.set_envs(opt, [(abi.EOS_PERFECT_GAS, g, 0.) for g in TP_GAMMA])
+    opt.final_time = 1e30
+    opt.deltat_max = 1e30
+    m = abi.mesh_desc(2, [nx, ny], [0, 0], [7.0, 3.0])
+    s = abi.Simulation(L, m, opt)
+    s.init_triple_point(TP_RHO, TP_P)
+    return s
+
+
+CELL_FIELDS = ["density", "internal_energy", "pressure", "sound_speed", "pseudo_viscosity",
+               "cell_mass"]
+ENV_FIELDS = ["density_env", "internal_energy_env", "pressure_env", "sound_speed_env",
+              "pseudo_viscosity_env", "cell_mass_env", "fracvol_env", "mass_fraction_env"]
+NODE_FIELDS = ["velocity", "node_mass", "node_coord"]
+
+
+def boundary_ring(nx, ny):
+    """Mask of nodes on the domain boundary (nodes (nx+1) x (ny+1))."""
+    m = np.zeros((ny + 1, nx + 1), dtype=bool)
+    m[0, :] = m[-1, :] = True
+    m[:, 0] = m[:, -1] = True
+    return m.ravel()
+
+
+def rel_diff(a, b, scale=None):
+    """max |a-b| / max(|b|, scale): relative error with a field-wide floor."""
+    a = np.asarray(a, dtype=float)
+    b = np.asarray(b, dtype=float)
+    if scale is None:
+        scale = np.max(np.abs(b)) if b.size else 1.0
+        scale = scale if scale > 0 else 1.0
+    den = np.maximum(np.abs(b), scale * 1e-3)
+    return float(np.max(np.abs(a - b) / den)) if a.size else 0.0
+
+
+def rel_max(a, b):
+    """max |a-b| / max |b| (max-norm relative error of a whole field)."""
+    a = np.asarray(a, dtype=float)
+    b = np.asarray(b, dtype=float)
+    s = np.max(np.abs(b)) if b.size else 0.0
+    return float(np.max(np.abs(a - b)) / (s if s > 0 else 1.0)) if a.size else 0.0
+
+
+def compare_states(got, ref, nenv, nx=None, ny=None, tol=1e-12, metric=None):
+    """Field-by-field relative differences; env fields only on multi-env cells."""
+    out = {}
+    rd = metric or rel_diff
+    mask = ref["env_mask"]
+    multi = np.array([bin(int(v)).count("1") >= 2 for v in mask])
+    assert np.array_equal(got["env_mask"], mask), "environment membership differs"
+    for f in CELL_FIELDS:
+        out[f] = rd(got[f], ref[f])
+    for f in ENV_FIELDS:
+        if multi.any():
+            g = got[f][:, multi]
+            r = ref[f][:, multi]
+            present = np.array([[(int(mask[c]) >> k) & 1 for c in np.nonzero(multi)[0]]
+                                for k in range(nenv)], dtype=bool)
+            out[f] = rd(g[present], r[present])
+        else:
+            out[f] = 0.0
+    for f in NODE_FIELDS:
+        out[f] = rd(got[f], ref[f])
+    if nx is not None:
+        ring = boundary_ring(nx, ny)
+        out["dual_grad_phi(ring)"] = (rd(got["dual_grad_phi"][:, ring], ref["dual_grad_phi"][:, ring])
+                                      if metric else rel_diff(got["dual_grad_phi"][:, ring],
+                                                              ref["dual_grad_phi"][:, ring], scale=1.0))
+    out["dt"] = abs(got["global_deltat"] - ref["global_deltat"]) / abs(ref["global_deltat"])
+    out["time"] = abs(got["global_time"] - ref["global_time"]) / max(abs(ref["global_time"]), 1e-300)
+    return out
