@@ -1,0 +1,48 @@
+"""Algorithmic HBM bytes of the time step (compulsory traffic, 8-byte words).
+
+Per-zone word counts of the reference step over pure cells with nodes ~ cells
+(2D, n_m environments):
+  L-node  (velocity, boundary conditions, position)        13
+  L-cell  (geometry, density, viscosity, energy, U build)  3 n_m + 9
+  cell sweep, per direction                                6 n_m + 7
+  node (dual) sweep, per direction                         10
+  final cells (moy, EOS, pressure average)                 3 n_m + 7
+  final nodes (velocity from the dual, node mass)          7
+  dt                                                       1
+  total                                                    71 + 18 n_m
+giving 1000 B per zone-cycle on the 3-material triple point.  The kernels of
+the library map onto these rows; the final-node words are carried by the last
+node sweep (which writes the velocity) and the dt word by the final cell
+kernel.
+"""
+
+PEAK_HBM_GBS = 8000.0  # MI355X HBM3E peak, GB/s
+
+
+def words_per_zone(n_env, dim=2):
+    if dim != 2:
+        raise NotImplementedError
+    return 71 + 18 * n_env
+
+
+def bytes_per_zone_cycle(n_env, dim=2, word=8):
+    return words_per_zone(n_env, dim) * word
+
+
+def kernel_bytes(name, n_cells, n_nodes, n_env, word=8):
+    """Algorithmic bytes of one launch of a library kernel."""
+    w = {
+        "node_lagrange": 13 * n_nodes,
+        "cell_lagrange": (3 * n_env + 9) * n_cells,
+        "cell_sweep_x": (6 * n_env + 7) * n_cells,
+        "cell_sweep_y": (6 * n_env + 7) * n_cells,
+        "node_sweep_x": 10 * n_nodes,
+        "node_sweep_y": 10 * n_nodes,
+        "cell_final": (3 * n_env + 7 + 1) * n_cells,
+    }.get(name)
+    return None if w is None else w * word
+
+
+def last_node_sweep_extra(n_nodes, word=8):
+    """The last node sweep also carries the final-node words (velocity, mass)."""
+    return 7 * n_nodes * word

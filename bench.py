@@ -1,0 +1,149 @@
+"""Benchmark: zone-cycles/s of the FP64 MaHyCo time step on the 1792x768
+three-material triple point, on one MI355X.
+
+A step is one full Lagrange + Euler-ADI remap iteration over the whole mesh
+with all data resident in HBM.  W warmup steps run first, then the state is
+re-initialised and exactly K steps are timed (K=500 is the configuration's
+run), bracketed by device synchronisations.  Per-kernel durations come from
+HIP events recorded on the library's stream around every launch of the timed
+region.  The CPU baseline is the C++ oracle (a port of the reference
+arithmetic) timed on this host on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "arcane-benchs_amd"))
+
+import numpy as np  # noqa: E402
+
+from mahyco_amd import abi, cases, load_gpu, roofline  # noqa: E402
+
+NX, NY, NENV = 1792, 768, 3
+
+
+def kernel_times(sim):
+    kt = sim.L.optional["kernel_times"]
+    import ctypes as C
+    cap = 32
+    names = (C.c_char_p * cap)()
+    ms = (C.c_double * cap)()
+    cnt = (C.c_int32 * cap)()
+    n = kt(sim.ctx, -1, names, ms, cnt, cap)
+    return {names[i].decode(): (ms[i], cnt[i]) for i in range(n)}
+
+
+def set_timing(sim, on):
+    sim.L.optional["kernel_times"](sim.ctx, 1 if on else 0, None, None, None, 0)
+
+
+def cpu_baseline(gpu_density_after, steps):
+    """Oracle on the same triple point for `steps` steps, all host threads."""
+    from importlib import util
+    spec = util.spec_from_file_location("oracle_lib", os.path.join(ROOT, "tests", "helpers.py"))
+    H = util.module_from_spec(spec)
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    spec.loader.exec_module(H)
+    L = H.oracle()
+    threads = L.optional["set_threads"](0)
+    o = cases.triple_point(L, NX, NY)
+    t0 = time.perf_counter()
+    o.run(steps)
+    t1 = time.perf_counter()
+    rho = o.download(["density"])["density"]
+    l1 = float(np.sum(np.abs(gpu_density_after - rho)) / np.sum(np.abs(rho)))
+    return {"value": NX * NY * steps / (t1 - t0), "unit": "zone-cycles/s", "cores": threads,
+            "kind": "port",
+            "sample": f"C++ oracle (OpenMP, {threads} threads) on the same 1792x768 triple point, "
+                      f"steps 1..{steps}, {t1 - t0:.1f} s"}, l1
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-events", action="store_true", help="do not record per-kernel events")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {args.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:
+        sync = lambda: None  # noqa: E731
+
+    gpu = load_gpu()
+    sim = cases.triple_point(gpu, NX, NY)
+    if args.warmup:
+        sim.L.optional["run_async"](sim.ctx, args.warmup)
+        sim.L.optional["synchronize"](sim.ctx)
+    sim.init_triple_point(cases.TP_RHO, cases.TP_P)
+    set_timing(sim, not args.no_events)
+    sync()
+    sim.L.optional["synchronize"](sim.ctx)
+    t0 = time.perf_counter()
+    r = sim.L.optional["run_async"](sim.ctx, args.steps)
+    assert r == 0, sim.L.last_error(sim.ctx)
+    sim.L.optional["synchronize"](sim.ctx)
+    sync()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    kt = kernel_times(sim) if not args.no_events else {}
+    set_timing(sim, False)
+    st = sim.download(["density", "cell_mass"])
+    assert st["global_iteration"] == args.steps, st["global_iteration"]
+    assert np.all(np.isfinite(st["density"]))
+    nc, nn = sim.nc, sim.nn
+    value = nc * args.steps / elapsed
+
+    rl = None
+    if kt:
+        tot = {k: v[0] * v[1] for k, v in kt.items()}
+        dom = max(tot, key=tot.get)
+        avg_ms, launches = kt[dom]
+        b = roofline.kernel_bytes(dom, nc, nn, NENV)
+        ach = b / (avg_ms * 1e-3) / 1e9
+        rl = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1),
+              "peak": roofline.PEAK_HBM_GBS, "unit": "GB/s",
+              "frac": round(ach / roofline.PEAK_HBM_GBS, 4), "traffic": None,
+              "avg_ms": round(avg_ms, 5), "launches": launches,
+              "bytes_per_launch": b}
+    step_frac = value * roofline.bytes_per_zone_cycle(NENV) / (roofline.PEAK_HBM_GBS * 1e9)
+
+    cpu, l1 = None, None
+    if not args.no_cpu_baseline:
+        g2 = cases.triple_point(gpu, NX, NY)
+        g2.run(args.cpu_steps)
+        rho_g = g2.download(["density"])["density"]
+        g2.close()
+        cpu, l1 = cpu_baseline(rho_g, args.cpu_steps)
+
+    out = {
+        "metric": "zone-cycles/s on 1792x768 triple-point; L1 density err vs FP64 oracle",
+        "value": value, "unit": "zone-cycles/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "triple point 1792x768, 3 perfect-gas materials, plain FP64 "
+                               "Lagrange + Euler-ADI remap (order 2, minmod), 500 steps",
+                   "cells": nc, "nodes": nn, "steps": args.steps, "parallelism": "none (1 GPU)"},
+        "l1_density_err_vs_oracle": (None if l1 is None else
+                                     {"value": l1, "after_steps": args.cpu_steps}),
+        "step_roofline_frac": round(step_frac, 4),
+        "kernel_ms": {k: round(v[0], 5) for k, v in kt.items()},
+        "roofline": rl,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

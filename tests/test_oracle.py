@@ -1,0 +1,228 @@
+"""The CPU oracle pinned by closed forms, invariants and the exact Riemann
+solution (the reference ships no golden vectors for MaHyCo: its ctest cases
+run 50 iterations with no comparison, CMakeLists.txt:100-101)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import helpers as H
+from mahyco_amd import abi, cases, riemann
+
+
+@pytest.fixture(scope="module")
+def L():
+    return H.oracle()
+
+
+def _probe(L):
+    lib = L.lib
+    lib.mho_probe_limiter.restype = C.c_double
+    lib.mho_probe_limiter.argtypes = [C.c_int32, C.c_double]
+    lib.mho_probe_eint.restype = C.c_double
+    lib.mho_probe_eint.argtypes = [C.c_int32] + [C.c_double] * 7
+    lib.mho_probe_viscosity.restype = C.c_double
+    lib.mho_probe_viscosity.argtypes = [C.c_double] * 5
+    lib.mho_probe_cqs.restype = C.c_double
+    lib.mho_probe_cqs.argtypes = [C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    lib.mho_probe_eos_pg.restype = None
+    lib.mho_probe_eos_pg.argtypes = [C.c_double] * 3 + [C.POINTER(C.c_double)]
+    return lib
+
+
+def test_limiters_closed_form(L):
+    lib = _probe(L)
+    rs = [-1.0, 0.0, 0.5, 1.0, 2.0, 3.0]
+    exp = {0: [0, 0, 0.5, 1, 1, 1],            # minmod max(0, min(1, r))
+           1: [0, 0, 1, 1, 2, 2],              # superbee
+           2: [0, 0, 2 / 3, 1, 4 / 3, 1.5],    # van Leer 2r/(1+r)
+           7: [0] * 6}                         # order 1
+    for lid, vals in exp.items():
+        got = [lib.mho_probe_limiter(lid, r) for r in rs]
+        assert np.allclose(got, vals, rtol=0, atol=1e-15), (lid, got)
+
+
+def test_energy_update_solves_midpoint_equation(L):
+    """compute_eint returns the e that solves e - e_n = -(p_n/2 + q + p(e)/2) d(1/rho)
+    with p(e) = (gamma-1) rho e (the implicit perfect-gas VNR update)."""
+    lib = _probe(L)
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        g = rng.uniform(1.1, 1.9)
+        rn, r = rng.uniform(0.1, 10, 2)
+        pn, en = rng.uniform(0.01, 10), rng.uniform(0.1, 10)
+        q = rng.uniform(0, 1)
+        e = lib.mho_probe_eint(0, g, 0.0, q, rn, r, pn, en)
+        d = 1 / r - 1 / rn
+        qq = q if q * d < 0 else 0.0
+        resid = e - en + (0.5 * pn + qq + 0.5 * (g - 1) * r * e) * d
+        assert abs(resid) < 1e-13 * max(1.0, abs(e)), resid
+    # centred pseudo: (q_n + q)/2 when (q_n + q) d < 0
+    e = lib.mho_probe_eint(1, 1.4, 0.3, 0.1, 1.0, 2.0, 1.0, 2.5)
+    d = 0.5 - 1.0
+    assert abs(e - (2.5 - (0.5 + 0.2) * d) / (1 + 0.2 * 2.0 * d)) < 1e-14
+
+
+def test_viscosity_and_eos(L):
+    lib = _probe(L)
+    # q = (1/tau) (-1/2 L c div + (g+1)/2 L^2 div^2)
+    g, tau, Lc, c, div = 1.4, 0.5, 0.01, 1.2, -3.0
+    exp = (1 / tau) * (-0.5 * Lc * c * div + 1.2 * Lc * Lc * div * div)
+    assert abs(lib.mho_probe_viscosity(g, tau, Lc, c, div) - exp) < 1e-16
+    out = (C.c_double * 3)()
+    lib.mho_probe_eos_pg(1.4, 1.0, 2.5, out)
+    assert abs(out[0] - 1.0) < 1e-15 and abs(out[1] - np.sqrt(1.4)) < 1e-15
+    assert abs(out[2] - 0.4) < 1e-15
+
+
+def _cqs(lib, dim, coord):
+    coord = np.ascontiguousarray(coord, dtype=np.float64)
+    out = np.zeros_like(coord)
+    v = lib.mho_probe_cqs(dim, coord.ctypes.data_as(C.POINTER(C.c_double)),
+                          out.ctypes.data_as(C.POINTER(C.c_double)))
+    return v, out
+
+
+def _hex_volume_divergence(c):
+    """Volume of a hexahedron whose faces are split into 4 triangles around the
+    face centre, by the divergence theorem (independent of the corner normals)."""
+    faces = [(0, 3, 2, 1), (0, 4, 7, 3), (0, 1, 5, 4), (4, 5, 6, 7), (1, 2, 6, 5), (2, 3, 7, 6)]
+    vol = 0.0
+    for f in faces:
+        fc = c[list(f)].mean(axis=0)
+        for a in range(4):
+            p, q = c[f[a]], c[f[(a + 1) % 4]]
+            vol += np.dot(fc, np.cross(p - fc, q - fc)) / 6.0
+    return vol  # hexa8 faces (MahycoModule.cc:1080-1087) are outward-oriented
+
+
+def test_corner_normals_2d(L):
+    lib = _probe(L)
+    rng = np.random.default_rng(1)
+    for _ in range(100):
+        base = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]], float)
+        c = base * rng.uniform(0.5, 2, 3) + rng.normal(0, 0.1, (4, 3))
+        c[:, 2] = 0
+        v, q = _cqs(lib, 2, c)
+        x, y = c[:, 0], c[:, 1]
+        shoelace = 0.5 * np.sum(x * np.roll(y, -1) - np.roll(x, -1) * y)
+        assert abs(v - shoelace) < 1e-14 * abs(shoelace)
+        assert np.abs(q.sum(axis=0)).max() < 1e-14
+        v2, q2 = _cqs(lib, 2, c + np.array([3.0, -2.0, 0.0]))
+        assert np.allclose(q2, q, atol=1e-14) and abs(v2 - v) < 1e-13
+    # axis-aligned rectangle dx x dy: C_0 = (-dy/2, -dx/2)
+    v, q = _cqs(lib, 2, [[0, 0, 0], [2, 0, 0], [2, 3, 0], [0, 3, 0]])
+    assert v == 6.0 and np.allclose(q[0], [-1.5, -1.0, 0]) and np.allclose(q[2], [1.5, 1.0, 0])
+
+
+def test_corner_normals_3d(L):
+    lib = _probe(L)
+    rng = np.random.default_rng(2)
+    cube = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                     [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], float)
+    v, q = _cqs(lib, 3, cube)
+    assert abs(v - 1.0) < 1e-15 and np.allclose(q[0], [-0.25, -0.25, -0.25])
+    for _ in range(50):
+        c = cube * rng.uniform(0.5, 2, 3) + rng.normal(0, 0.08, (8, 3))
+        v, q = _cqs(lib, 3, c)
+        ref = _hex_volume_divergence(c)
+        assert abs(v - ref) < 1e-13 * abs(ref), (v, ref)
+        assert np.abs(q.sum(axis=0)).max() < 1e-14
+
+
+def test_sod_exact_riemann(L):
+    """Whole method vs the exact solution (Toro ch. 4): L1(rho) at t_end and
+    first-order-in-shock convergence; rows identical along y."""
+    errs = []
+    for nx in [100, 200, 400]:
+        s = cases.sod(L, nx, 4)
+        s.run(100000)
+        st = s.download()
+        rho = st["density"].reshape(4, nx)
+        assert np.abs(rho - rho[0]).max() < 1e-11
+        xc = (np.arange(nx) + 0.5) / nx
+        rex, _, _ = riemann.sample(xc, st["global_time"], 0.5, (1, 0, 1), (0.125, 0, 0.1), 1.4)
+        errs.append(np.mean(np.abs(rho[0] - rex)))
+        assert st["global_time"] <= 0.2 and st["global_time"] + st["global_deltat"] > 0.2
+    print("Sod L1(rho)", errs)
+    assert errs[2] < 0.0025
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert np.all(rates > 0.6), rates
+
+
+def test_sod_star_state():
+    p, u = riemann.star_state(1, 0, 1, 0.125, 0, 0.1, 1.4)
+    assert abs(p - 0.30313) < 1e-5 and abs(u - 0.92745) < 1e-5  # Toro table 4.2, test 1
+
+
+def _rest(L, length, steps):
+    opt = L.options()
+    abi.set_bcs(opt, abi.reflecting_walls(2))
+    abi.set_envs(opt, [(abi.EOS_PERFECT_GAS, 1.4, 0.), (abi.EOS_PERFECT_GAS, 1.6, 0.)])
+    s = abi.Simulation(L, abi.mesh_desc(2, [12, 8], [0, 0], length), opt)
+    env = np.zeros(96, dtype=np.int32)
+    env[48:] = 1
+    rho = np.where(env == 0, 1.0, 2.0)
+    s.init_cells(env, rho, np.full(96, 0.7))  # pressure equilibrium across the interface
+    a = s.download()
+    s.run(steps)
+    return a, s.download()
+
+
+def test_rest_state_is_stationary(L):
+    # dyadic cell sizes: every corner-normal difference is exact, forces cancel exactly
+    a, b = _rest(L, [1.5, 1.0], 30)
+    for f in ["density", "internal_energy", "cell_mass"]:
+        assert np.array_equal(a[f], b[f]), f
+    # p = (gamma-1) rho e after the first EOS call: the init's p -> e -> p round
+    # trip differs by an ulp between the two materials, which leaves a force of
+    # that size on the interface nodes
+    assert np.abs(a["pressure"] - b["pressure"]).max() <= 2.3e-16
+    assert np.abs(b["velocity"]).max() < 1e-17
+    # dx = 0.1: cancellation up to rounding of the coordinates
+    a, b = _rest(L, [1.2, 0.8], 30)
+    for f in ["density", "internal_energy", "pressure", "cell_mass"]:
+        assert np.abs(a[f] - b[f]).max() < 1e-14 * np.abs(a[f]).max(), f
+    assert np.abs(b["velocity"]).max() < 1e-16
+
+
+def test_mass_conservation_sod(L):
+    s = cases.sod(L, 100, 4)
+    c0 = s.conservation()
+    s.run(100000)
+    c1 = s.conservation()
+    assert abs(c1[0] - c0[0]) / c0[0] < 1e-14
+
+
+def test_upload_download_roundtrip_is_exact(L):
+    o = H.tp_setup(L, 56, 24)
+    o.run(120)
+    st = o.download()
+    assert sum(bin(int(v)).count("1") >= 2 for v in st["env_mask"]) > 0  # mixed cells exist
+    o2 = H.tp_setup(L, 56, 24)
+    o2.upload(st)
+    o.step()
+    o2.step()
+    d = H.compare_states(o2.download(), o.download(), 3, 56, 24)
+    assert max(d.values()) == 0.0
+
+
+def test_dt_min_abort_and_stop(L):
+    s = cases.sod(L, 20, 2, deltat_min=1.0)
+    with pytest.raises(abi.MahycoError) as ei:
+        s.step()
+    assert ei.value.code == abi.E_DT_MIN
+    s = cases.sod(L, 20, 2, final_time=1e-4)
+    n = s.run(1000)
+    st = s.download()
+    assert st["global_time"] <= 1e-4 < st["global_time"] + st["global_deltat"]
+    assert n == st["global_iteration"]
+
+
+def test_unsupported_options_rejected(L):
+    m = abi.mesh_desc(2, [4, 4], [0, 0], [1, 1])
+    for kw in [dict(schema_csts=1), dict(projection_pente_borne=1), dict(projection_limiteur_id=3),
+               dict(ordre_projection=3), dict(conservation_energie_totale=1)]:
+        with pytest.raises(abi.MahycoError) as ei:
+            abi.Simulation(L, m, L.options(**kw))
+        assert ei.value.code == abi.E_UNSUPPORTED
