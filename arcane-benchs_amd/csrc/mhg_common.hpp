@@ -166,9 +166,6 @@ struct DevScalars {
   int stop;         // end of run reached
   int err;          // MHG_E_DT_MIN raised
   int pad;
-  unsigned long long dtmin_bits;  // running min of dx_sound (positive double bits)
-  unsigned int counter;           // finished blocks of the final kernel
-  unsigned int pad2;
 };
 
 // Run-time parameters of a context.

@@ -53,6 +53,7 @@ struct mhg_ctx {
   double* xx[2] = {nullptr, nullptr};
   double* xy[2] = {nullptr, nullptr};
   double *Ud0 = nullptr, *Ud1 = nullptr, *Ud3 = nullptr, *dg0 = nullptr, *dg1 = nullptr;
+  double* block_min = nullptr;
   DevScalars* S = nullptr;
   DevScalars* S_host = nullptr;  // pinned mirror
 
@@ -300,6 +301,7 @@ void remap_2d(mhg_ctx* c) {
   F.env_m = c->env_m;
   F.env_f = c->env_f;
   F.env_mf = c->env_mf;
+  F.block_min = c->block_min;
   {
     TimedLaunch t(c, "cell_final");
     launch_cell_final_2d(c->stream, P, c->S, F);
@@ -391,7 +393,6 @@ int init_pure(mhg_ctx* c, const std::vector<int>& env, const std::vector<double>
   for (double* d : envs) HIPCHK(c, hipMemset(d, 0, envb));
   std::memset(c->S_host, 0, sizeof(DevScalars));
   c->S_host->dt = c->opt.deltat_init;
-  c->S_host->dtmin_bits = 0x7FEFFFFFFFFFFFFFull;
   int r = write_scalars(c);
   if (r) return r;
   c->initialised = true;
@@ -531,6 +532,7 @@ int mhg_create(const mhg_mesh_desc* mesh, const mhg_options* opt, mhg_ctx** out)
   }
   r |= dalloc(c, &c->Ud0, NN) | dalloc(c, &c->Ud1, NN) | dalloc(c, &c->Ud3, NN);
   r |= dalloc(c, &c->dg0, NN) | dalloc(c, &c->dg1, NN);
+  r |= dalloc(c, &c->block_min, cell_final_blocks(P));
   r |= dalloc(c, &c->S, 1);
   if (r) {
     mhg_destroy(c);
@@ -658,7 +660,6 @@ int mhg_upload_state(mhg_ctx* c, const mhg_state* st) {
   c->S_host->dt_old = st->old_deltat;
   c->S_host->time = st->global_time;
   c->S_host->iteration = st->global_iteration;
-  c->S_host->dtmin_bits = 0x7FEFFFFFFFFFFFFFull;
   c->sens = st->sens_projection;
   int r = write_scalars(c);
   if (r) return r;

@@ -38,8 +38,10 @@ struct CellFinalArgs {
   double *rho, *e, *p, *cs, *q, *m;
   int* mask;
   double *env_rho, *env_e, *env_p, *env_cs, *env_q, *env_m, *env_f, *env_mf;
+  double* block_min;  // one CFL minimum per block of k_cell_final_2d
 };
 
+int cell_final_blocks(const Params& P);
 void launch_begin_iteration(hipStream_t st, DevScalars* S);
 void launch_node_lagrange_2d(hipStream_t st, const Params& P, DevScalars* S, const NodeLagArgs& A,
                              bool vel, bool bc, bool pos);

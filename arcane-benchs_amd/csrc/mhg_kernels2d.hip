@@ -111,6 +111,12 @@ __global__ void k_node_lagrange_2d(Params P, DevScalars* S, NodeLagArgs A) {
 }
 
 // ---------------------------------------------------------------------------
+// Environment bound.  Every cell kernel is compiled for a maximum number of
+// environments NEM (1, 2, 3, 4 or 8) so that per-environment values live in
+// registers; the run-time count ne <= NEM guards loads and stores.
+// ---------------------------------------------------------------------------
+
+// ---------------------------------------------------------------------------
 // K2: cell phase.  computeGeometricValues (1054-1299), updateDensity
 // (1331-1426), computeArtificialViscosity (454-578),
 // updateEnergyAndPressureforGP (1659-1770), then the remap vector
@@ -120,8 +126,8 @@ __global__ void k_node_lagrange_2d(Params P, DevScalars* S, NodeLagArgs A) {
 // energy slots (PrepareRemap.cc:324-328) are identically zero and stay zero
 // through the sweeps (phi = 0 gives a zero gradient, face value and flux).
 // ---------------------------------------------------------------------------
-template <bool PC>
-__global__ void k_cell_lagrange_2d(Params P, DevScalars* S, CellLagArgs A) {
+template <bool PC, int NEM>
+__global__ __launch_bounds__(256) void k_cell_lagrange_2d(Params P, DevScalars* S, CellLagArgs A) {
   if (S->stop || S->err) return;
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= P.nc) return;
@@ -137,8 +143,7 @@ __global__ void k_cell_lagrange_2d(Params P, DevScalars* S, CellLagArgs A) {
   }
   const double V = quad_volume(x, y);
   const double L = (P.lc == MHG_LC_FACES_OPPOSEES) ? length_faces_opposees_2d(x, y) : pow(V, 0.5);
-  const double dt = S->dt;
-  const double inv_dt = 1.0 / dt;
+  const double inv_dt = 1.0 / S->dt;
   const double rho_n = A.rho[c];
   const double rho = A.m[c] / V;
   const double tau = 0.5 * (1.0 / rho_n + 1.0 / rho);
@@ -147,21 +152,27 @@ __global__ void k_cell_lagrange_2d(Params P, DevScalars* S, CellLagArgs A) {
   const int mask = A.mask[c];
   const int ne = P.ne;
   const long NC = P.nc;
-  double* U = A.U;
-  for (int v = 0; v < P.nvu; ++v) U[v * NC + c] = 0.;
+  double uv[NEM], um[NEM], ue[NEM];
+  double uq = 0.;
+#pragma unroll
+  for (int k = 0; k < NEM; ++k) uv[k] = um[k] = ue[k] = 0.;
   if (__popc(mask) == 1) {
-    const int k = __ffs(mask) - 1;
+    const int kk = __ffs(mask) - 1;
     double qv = 0.;
-    if (divu < 0.0) qv = viscosity(P.eos.qcoef[k], tau, L, cs, divu);
+    if (divu < 0.0) qv = viscosity(P.eos.qcoef[kk], tau, L, cs, divu);
     const double qn = PC ? A.q[c] : 0.;
-    const double e = compute_eint(PC, P.eos.gamma[k], qn, qv, rho_n, rho, A.p[c], A.e[c]);
-    U[k * NC + c] = V;
-    U[(ne + k) * NC + c] = V * rho;
-    U[(2 * ne + k) * NC + c] = V * rho * e;
-    U[(3 * ne) * NC + c] = V * qv;
+    const double e = compute_eint(PC, P.eos.gamma[kk], qn, qv, rho_n, rho, A.p[c], A.e[c]);
+#pragma unroll
+    for (int k = 0; k < NEM; ++k)
+      if (k == kk) {
+        uv[k] = V;
+        um[k] = V * rho;
+        ue[k] = V * rho * e;
+      }
+    uq = V * qv;
   } else if (mask != 0) {
-    double uq = 0.;
-    for (int k = 0; k < ne; ++k) {
+#pragma unroll
+    for (int k = 0; k < NEM; ++k) {
       if (!(mask & (1 << k))) continue;
       const long ek = k * NC + c;
       const double f = A.env_f[ek];
@@ -173,142 +184,500 @@ __global__ void k_cell_lagrange_2d(Params P, DevScalars* S, CellLagArgs A) {
       if (divu < 0.0) qk = viscosity(P.eos.qcoef[k], tk, L, cs, divu);
       const double qnk = PC ? A.env_q[ek] : 0.;
       const double ekv = compute_eint(PC, P.eos.gamma[k], qnk, qk, rho_nk, rk, A.env_p[ek], A.env_e[ek]);
-      U[k * NC + c] = Vk;
-      U[(ne + k) * NC + c] = Vk * rk;
-      U[(2 * ne + k) * NC + c] = Vk * rk * ekv;
+      uv[k] = Vk;
+      um[k] = Vk * rk;
+      ue[k] = Vk * rk * ekv;
       uq = Vk * qk;  // PrepareRemap.cc:372 assigns: the last environment wins
     }
-    U[(3 * ne) * NC + c] = uq;
   }
+  double* U = A.U;
+#pragma unroll
+  for (int k = 0; k < NEM; ++k) {
+    if (k >= ne) break;
+    U[k * NC + c] = uv[k];
+    U[(ne + k) * NC + c] = um[k];
+    U[(2 * ne + k) * NC + c] = ue[k];
+  }
+  U[(3 * ne) * NC + c] = uq;
   A.Vlag[c] = V;
   A.L[c] = L;
 }
 
 // ---------------------------------------------------------------------------
-// K3: cell sweep in direction D.  computeGradPhiFace (RemapADIService.cc:
-// 163-178), computeGradPhiCell_PBorn0_LimC (447-517),
-// computeUpwindFaceQuantitiesForProjection_PBorn0_O2 (658-720),
-// computeUremap_PBorn0 (883-988).  One thread per cell; the 5-cell stencil
-// along D is read from the caches.  Domain-boundary faces carry a zero face
-// gradient and a zero face value (they are never written in the reference).
-// FIRST: phi = U / V_lagrange (PrepareRemap.cc:394-398); otherwise
-// phi = U / sum_k U_vol,k as left by the previous sweep (RemapADIService.cc:961-970).
+// Sweep arithmetic shared by both directions, for one variable of one cell:
+// computeGradPhiFace (RemapADIService.cc:171-177), the limited cell gradient
+// (497-506), the upwind face value (707-714) and the flux update (909-936).
 // ---------------------------------------------------------------------------
-template <int D, bool FIRST>
-__global__ void k_cell_sweep_2d(Params P, DevScalars* S, CellSweepArgs A) {
+__device__ inline double face_grad(bool inner, double phi, double phim, double dx) {
+  return inner ? (phi - phim) / dx : 0.;
+}
+
+// x sweep: one wavefront holds 64 consecutive cells of a row (60 outputs and
+// two halo cells on each side); stencil neighbours move between lanes.
+template <int NEM, bool FIRST>
+__global__ __launch_bounds__(256) void k_cell_sweep_x_2d(Params P, DevScalars* S, CellSweepArgs A) {
   if (S->stop || S->err) return;
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= P.nc) return;
+  const int lane = threadIdx.x & 63;
+  const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int nseg = (P.nx + 59) / 60;
+  const int j = (int)(wave / nseg);
+  if (j >= P.ny) return;  // uniform across the wave
+  const int seg = (int)(wave % nseg);
+  const int i = seg * 60 - 2 + lane;
+  const bool ex = (i >= 0 && i < P.nx);
+  const bool out = ex && lane >= 2 && lane <= 61;
   const int nnx = P.nx + 1;
-  const int i = (int)(c % P.nx), j = (int)(c / P.nx);
-  const int s = (D == 0) ? i : j;
-  const int N = (D == 0) ? P.nx : P.ny;
   const long NC = P.nc;
-  const long cstride = (D == 0) ? 1 : P.nx;
   const int ne = P.ne;
   const double dt = S->dt;
-  const double* __restrict__ Xd = (D == 0) ? A.xnx : A.xny;
-  const double* __restrict__ Xt = (D == 0) ? A.xny : A.xnx;  // transverse coordinate
-  const double* __restrict__ Ud_ = (D == 0) ? A.ux : A.uy;
+  const double order2 = (double)P.order2;
+  const long c = ex ? i + (long)P.nx * j : 0;
   const double* __restrict__ Uin = A.Uin;
-
-  // stencil cells o = -2..2
-  bool ex[5];
-  double xc[5], den[5];
-#pragma unroll
-  for (int o = -2; o <= 2; ++o) {
-    const int so = s + o;
-    ex[o + 2] = (so >= 0 && so < N);
-    xc[o + 2] = 0.;
-    den[o + 2] = 1.;
-    if (!ex[o + 2]) continue;
-    const long co = c + o * cstride;
-    const int ci = (D == 0) ? i + o : i, cj = (D == 0) ? j : j + o;
-    const long m0 = ci + (long)nnx * cj;
-    // d-component of the quad nodes, Arcane order, summed in that order
-    const double a = Xd[m0], b = Xd[m0 + 1], cc = Xd[m0 + 1 + nnx], dd = Xd[m0 + nnx];
-    xc[o + 2] = 0.25 * (((a + b) + cc) + dd);
+  // cell centre (x component) and phi denominator
+  double xc = 0., den = 1.;
+  if (ex) {
+    const long m0 = i + (long)nnx * j;
+    const double a = A.xnx[m0], b = A.xnx[m0 + 1], cc = A.xnx[m0 + 1 + nnx], dd = A.xnx[m0 + nnx];
+    xc = 0.25 * (((a + b) + cc) + dd);
     if (FIRST) {
-      den[o + 2] = A.Vlag[co];
+      den = A.Vlag[c];
     } else {
       double sv = 0.;
-      for (int k = 0; k < ne; ++k) sv += Uin[k * NC + co];
-      den[o + 2] = sv;
-    }
-  }
-  // faces o = -1..2: face between stencil cells o-1 and o
-  bool inner[4];
-  double dx[4];
 #pragma unroll
-  for (int o = -1; o <= 2; ++o) {
-    inner[o + 1] = ex[o + 1] && ex[o + 2];
-    dx[o + 1] = inner[o + 1] ? (xc[o + 2] - xc[o + 1]) : 1.;
-  }
-  // flux faces: o = 0 (back of c), o = 1 (front of c)
-  double uf[2], lf[2], dotxf[2];
-  int up[2];
-#pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    const int o = t;  // face index relative to c
-    long na, nb;
-    if (D == 0) {
-      na = (i + o) + (long)nnx * j;
-      nb = na + nnx;
-    } else {
-      na = i + (long)nnx * (j + o);
-      nb = na + 1;
-    }
-    uf[t] = 0.5 * (Ud_[na] + Ud_[nb]);
-    const double xfa = 0.5 * Xd[na], xfb = 0.5 * Xd[nb];
-    const double xf = xfa + xfb;
-    lf[t] = fabs(Xt[nb] - Xt[na]);
-    up[t] = 0;
-    dotxf[t] = 0.;
-    if (inner[o + 1]) {
-      up[t] = (uf[t] * dx[o + 1] > 0.0) ? (o - 1) : o;
-      dotxf[t] = xf - xc[up[t] + 2];
+      for (int k = 0; k < NEM; ++k)
+        if (k < ne) sv += Uin[k * NC + c];
+      den = sv;
     }
   }
+  // back face of cell i = x-face at node column i, nodes (i,j), (i,j+1)
+  double uf = 0., lf = 0., xf = 0.;
+  if (i >= 0 && i <= P.nx) {
+    const long na = i + (long)nnx * j, nb = na + nnx;
+    uf = 0.5 * (A.ux[na] + A.ux[nb]);
+    xf = 0.5 * A.xnx[na] + 0.5 * A.xnx[nb];
+    lf = fabs(A.xny[nb] - A.xny[na]);
+  }
+  const bool inner = ex && i >= 1;
+  const double xcm = __shfl_up(xc, 1, 64);
+  const double dx = inner ? xc - xcm : 1.;
+  const bool upm = inner && (uf * dx > 0.0);
+  const double dotxf = inner ? (upm ? xf - xcm : xf - xc) : 0.;
+  const double uf_f = __shfl_down(uf, 1, 64), lf_f = __shfl_down(lf, 1, 64);
   const int lim = P.lim;
-  const double order2 = (double)P.order2;
-  double* Uout = A.Uout;
-  double dualmass[kMaxEnv];
-  for (int v = 0; v < P.nvu; ++v) {
-    double phi[5];
+  const bool o2 = P.order2 > 0;
+  auto sweep_var = [&](int v, double& unew, double& dual) {
+    const double u0 = ex ? Uin[v * NC + c] : 0.;
+    const double phi = ex ? u0 / den : 0.;
+    const double phim = __shfl_up(phi, 1, 64);
+    const double gf = face_grad(inner, phi, phim, dx);
+    const double gfp = __shfl_down(gf, 1, 64);
+    const double g = o2 ? limited_cell_grad(lim, gf, gfp) : 0.;
+    const double gm = __shfl_up(g, 1, 64);
+    const double phif = inner ? (upm ? phim + order2 * dotxf * gm : phi + order2 * dotxf * g) : 0.;
+    const double phifp = __shfl_down(phif, 1, 64);
+    const double flux_b = -1.0 * uf * lf * dt * phif;
+    const double flux_f = 1.0 * uf_f * lf_f * dt * phifp;
+    // cell faces in order: bottom, right (front), top, left (back)
+    unew = u0 - (flux_f + flux_b);
+    dual = 0.5 * flux_f * 1.0 + 0.5 * flux_b * -1.0;
+  };
+  double* __restrict__ Uout = A.Uout;
 #pragma unroll
-    for (int o = -2; o <= 2; ++o)
-      phi[o + 2] = ex[o + 2] ? Uin[v * NC + c + o * cstride] / den[o + 2] : 0.;
-    double gf[4];
-#pragma unroll
-    for (int o = -1; o <= 2; ++o)
-      gf[o + 1] = inner[o + 1] ? (phi[o + 2] - phi[o + 1]) / dx[o + 1] : 0.;
-    double phif[2];
-#pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      phif[t] = 0.;
-      if (!inner[t + 1]) continue;
-      const int u = up[t];  // -1, 0 or 1
-      double g = 0.;
-      if (P.order2 > 0) g = limited_cell_grad(lim, gf[u + 1], gf[u + 2]);
-      phif[t] = phi[u + 2] + order2 * dotxf[t] * g;
-    }
-    const double flux_b = -1.0 * uf[0] * lf[0] * dt * phif[0];
-    const double flux_f = 1.0 * uf[1] * lf[1] * dt * phif[1];
-    double flux_face, dual;
-    if (D == 0) {  // cell faces in order: ..., right (front), ..., left (back)
-      flux_face = flux_f + flux_b;
-      dual = 0.5 * flux_f * 1.0 + 0.5 * flux_b * -1.0;
-    } else {       // bottom (back), ..., top (front)
-      flux_face = flux_b + flux_f;
-      dual = 0.5 * flux_b * -1.0 + 0.5 * flux_f * 1.0;
-    }
-    double un = Uin[v * NC + c] - flux_face;
-    if (v >= ne && v < 3 * ne && un < 0.) un = 0.;  // RemapADIService.cc:952-959
-    Uout[v * NC + c] = un;
-    if (v >= ne && v < 2 * ne) dualmass[v - ne] = dual;
+  for (int k = 0; k < NEM; ++k) {
+    if (k >= ne) break;
+    double un, du;
+    sweep_var(k, un, du);
+    if (out) Uout[k * NC + c] = un;
   }
-  // half of the environment dual mass flux: fcontrib (DualPhaseRemap.cc:147-188)
-  for (int k = 0; k < ne; ++k) A.contrib[k * NC + c] = 0.5 * dualmass[k];
+#pragma unroll
+  for (int k = 0; k < NEM; ++k) {
+    if (k >= ne) break;
+    double un, du;
+    sweep_var(ne + k, un, du);
+    if (un < 0.) un = 0.;  // RemapADIService.cc:953-955
+    if (out) {
+      Uout[(ne + k) * NC + c] = un;
+      A.contrib[k * NC + c] = 0.5 * du;  // fcontrib (DualPhaseRemap.cc:168-185)
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NEM; ++k) {
+    if (k >= ne) break;
+    double un, du;
+    sweep_var(2 * ne + k, un, du);
+    if (un < 0.) un = 0.;  // RemapADIService.cc:956-958
+    if (out) Uout[(2 * ne + k) * NC + c] = un;
+  }
+  {
+    double un, du;
+    sweep_var(3 * ne, un, du);
+    if (out) Uout[(3 * ne) * NC + c] = un;
+  }
+}
+
+// y sweep: each lane owns one column and walks a strip of rows, keeping a
+// two-cell window of phi, face gradients, limited gradients and face values
+// in registers.  Iteration r loads cell r, forms the gradient of face r, the
+// limited gradient of cell r-1, the face value of face r-1 and updates cell
+// r-2.  Face r lies between cells r-1 and r (node row r).  The variables of U
+// are independent in a sweep (only the denominator sum_k U_vol,k couples
+// them), so a thread carries VG of them and the variable group is a grid
+// dimension: small register windows, many waves in flight.
+template <bool FIRST, int STRIP, int VG>
+__global__ __launch_bounds__(64) void k_cell_sweep_y_2d(Params P, DevScalars* S, CellSweepArgs A) {
+  if (S->stop || S->err) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.nx) return;
+  const int j0 = blockIdx.y * STRIP;
+  const int j1 = min(j0 + STRIP, P.ny);
+  const int v0 = blockIdx.z * VG;
+  const int nnx = P.nx + 1;
+  const long NC = P.nc;
+  const int ne = P.ne;
+  const int nvu = P.nvu;
+  const double dt = S->dt;
+  const double order2 = (double)P.order2;
+  const int lim = P.lim;
+  const bool o2 = P.order2 > 0;
+  const double* __restrict__ Uin = A.Uin;
+  double* __restrict__ Uout = A.Uout;
+  double ph1[VG], ph2[VG], gf1[VG], g2[VG], pf2[VG], U1[VG], U2[VG];
+#pragma unroll
+  for (int s = 0; s < VG; ++s) ph1[s] = ph2[s] = gf1[s] = g2[s] = pf2[s] = U1[s] = U2[s] = 0.;
+  double xc1 = 0., xc2 = 0.;                      // centres of cells r-1, r-2
+  double uf1 = 0., lf1 = 0., xf1 = 0., dx1 = 1.;  // face r-1
+  double uf2 = 0., lf2 = 0.;                      // face r-2
+  bool in1 = false;                               // face r-1 inner
+  for (int r = j0 - 2; r <= j1 + 1; ++r) {
+    const bool rv = (r >= 0 && r < P.ny);
+    const long c = rv ? i + (long)P.nx * r : 0;
+    double xc = 0., den = 1.;
+    if (rv) {
+      const long m0 = i + (long)nnx * r;
+      const double a = A.xny[m0], b = A.xny[m0 + 1], cc = A.xny[m0 + 1 + nnx], dd = A.xny[m0 + nnx];
+      xc = 0.25 * (((a + b) + cc) + dd);
+      if (FIRST) {
+        den = A.Vlag[c];
+      } else {
+        double sv = 0.;
+        for (int k = 0; k < ne; ++k) sv += Uin[k * NC + c];
+        den = sv;
+      }
+    }
+    double uf = 0., lf = 0., xf = 0.;
+    if (r >= 0 && r <= P.ny) {  // face r: nodes (i,r), (i+1,r)
+      const long na = i + (long)nnx * r, nb = na + 1;
+      uf = 0.5 * (A.uy[na] + A.uy[nb]);
+      xf = 0.5 * A.xny[na] + 0.5 * A.xny[nb];
+      lf = fabs(A.xnx[nb] - A.xnx[na]);
+    }
+    const bool inner = rv && r >= 1;
+    const double dx = inner ? xc - xc1 : 1.;
+    const bool upm1 = in1 && (uf1 * dx1 > 0.0);
+    const double dot1 = in1 ? (upm1 ? xf1 - xc2 : xf1 - xc1) : 0.;
+    const int out_row = r - 2;
+    const bool out = (out_row >= j0 && out_row < j1);
+    const long co = out ? i + (long)P.nx * out_row : 0;
+#pragma unroll
+    for (int s = 0; s < VG; ++s) {
+      const int v = v0 + s;
+      if (v >= nvu) break;
+      const double ur = rv ? Uin[v * NC + c] : 0.;
+      const double phi = rv ? ur / den : 0.;
+      const double gf = face_grad(inner, phi, ph1[s], dx);
+      const double g1 = o2 ? limited_cell_grad(lim, gf1[s], gf) : 0.;  // cell r-1
+      const double pf1 = in1 ? (upm1 ? ph2[s] + order2 * dot1 * g2[s] : ph1[s] + order2 * dot1 * g1) : 0.;
+      if (out) {
+        const double flux_b = -1.0 * uf2 * lf2 * dt * pf2[s];
+        const double flux_f = 1.0 * uf1 * lf1 * dt * pf1;
+        // cell faces in order: bottom (back), right, top (front), left
+        double un = U2[s] - (flux_b + flux_f);
+        if (v >= ne && v < 3 * ne && un < 0.) un = 0.;
+        Uout[v * NC + co] = un;
+        if (v >= ne && v < 2 * ne) {
+          const double dual = 0.5 * flux_b * -1.0 + 0.5 * flux_f * 1.0;
+          A.contrib[(v - ne) * NC + co] = 0.5 * dual;
+        }
+      }
+      ph2[s] = ph1[s];
+      ph1[s] = phi;
+      gf1[s] = gf;
+      g2[s] = g1;
+      pf2[s] = pf1;
+      U2[s] = U1[s];
+      U1[s] = ur;
+    }
+    xc2 = xc1;
+    xc1 = xc;
+    uf2 = uf1;
+    lf2 = lf1;
+    uf1 = uf;
+    lf1 = lf;
+    xf1 = xf;
+    dx1 = dx;
+    in1 = inner;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K5: final cell phase.  remapVariables cell status + membership change
+// (RemapADIFinal.cc:98-226), "moy" (446-643), applyEOS per environment
+// (PerfectGasEOSService.cc:50-137), computePressionMoyenne (MahycoModule.cc:
+// 1836-1915) and the CFL candidate of computeHydroDeltaT (2047-2061) reduced
+// to one minimum per block; k_dt_finalize finishes computeDeltaT (2076-2191).
+// ---------------------------------------------------------------------------
+template <int BLOCK, int NEM>
+__global__ __launch_bounds__(BLOCK) void k_cell_final_2d(Params P, DevScalars* S, CellFinalArgs A) {
+  if (S->stop || S->err) return;
+  __shared__ double red[BLOCK / 64];
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double dx_sound = 1.7976931348623157e308;
+  if (c < P.nc) {
+    const int nnx = P.nx + 1;
+    const int i = (int)(c % P.nx), j = (int)(c / P.nx);
+    const long NC = P.nc;
+    const int ne = P.ne;
+    const double thr = P.remap_threshold;
+    // Euler volume: the corner-normal formula on the initial coordinates.
+    double x0[4] = {node_x0(P, i), node_x0(P, i + 1), node_x0(P, i + 1), node_x0(P, i)};
+    double y0[4] = {node_y0(P, j), node_y0(P, j), node_y0(P, j + 1), node_y0(P, j + 1)};
+    const double vol = quad_volume(x0, y0);
+    double Uvol[NEM], Umas[NEM], Uene[NEM];
+#pragma unroll
+    for (int k = 0; k < NEM; ++k) {
+      Uvol[k] = Umas[k] = Uene[k] = 0.;
+      if (k < ne) {
+        Uvol[k] = A.U[k * NC + c];
+        Umas[k] = A.U[(ne + k) * NC + c];
+        Uene[k] = A.U[(2 * ne + k) * NC + c];
+      }
+    }
+    const double Uq = A.U[(3 * ne) * NC + c];
+    const int old_mask = A.mask[c];
+    // cell status per environment (110-143)
+    double volt = 0.;
+#pragma unroll
+    for (int k = 0; k < NEM; ++k)
+      if (k < ne) volt += Uvol[k];
+    int mask = old_mask;
+#pragma unroll
+    for (int k = 0; k < NEM; ++k) {
+      if (k >= ne) break;
+      const double fvol = Uvol[k] / vol;
+      const bool in_env = (old_mask >> k) & 1;
+      if (fvol < thr && in_env) {
+        mask &= ~(1 << k);
+      } else if (fvol > thr && !in_env) {
+        const double fn = Uvol[k] / volt;
+        if (fn > thr && Umas[k] != 0.) mask |= (1 << k);
+      }
+    }
+    const int nbe = __popc(mask);
+    // "moy"
+    double masset = 0.;
+    volt = 0.;
+#pragma unroll
+    for (int k = 0; k < NEM; ++k)
+      if (k < ne) {
+        volt += Uvol[k];
+        masset += Umas[k];
+      }
+    const double unsurvolt = 1. / volt;
+    const double unsurvol = 1. / vol;
+    double fv[NEM], mf[NEM], dk[NEM], ek[NEM], pk[NEM], ck[NEM];
+    double somme_frac = 0.;
+#pragma unroll
+    for (int k = 0; k < NEM; ++k) {
+      fv[k] = 0.;
+      pk[k] = ck[k] = 0.;
+      if (!((mask >> k) & 1)) continue;
+      double f = Uvol[k] * unsurvolt;
+      if (f < thr) f = 0.;
+      fv[k] = f;
+      somme_frac += f;
+    }
+    int matcell = 0;
+    const double unsursomme_frac = 1. / somme_frac;
+#pragma unroll
+    for (int k = 0; k < NEM; ++k) {
+      if (!((mask >> k) & 1)) continue;
+      const double f = fv[k] * unsursomme_frac;
+      if (f > 0.) matcell++;
+      fv[k] = f;
+    }
+#pragma unroll
+    for (int k = 0; k < NEM; ++k) {
+      // a zero total mass leaves the mass fractions as they were (513-538)
+      mf[k] = 0.;
+      if (nbe == 1) mf[k] = 1.0;
+      else if (k < ne && ((old_mask >> k) & 1) && ((mask >> k) & 1)) mf[k] = A.env_mf[k * NC + c];
+    }
+    if (masset != 0.) {
+      const double unsurmasset = 1. / masset;
+      double fmasset = 0.;
+#pragma unroll
+      for (int k = 0; k < NEM; ++k) {
+        if (!((mask >> k) & 1)) continue;
+        double m = Umas[k] * unsurmasset;
+        if (fv[k] < thr) m = 0.;
+        fmasset += m;
+        mf[k] = m;
+      }
+      if (fmasset != 0.) {
+        const double unsurfmasset = 1. / fmasset;
+#pragma unroll
+        for (int k = 0; k < NEM; ++k)
+          if ((mask >> k) & 1) mf[k] = mf[k] * unsurfmasset;
+      }
+    }
+    double density_nplus1 = 0.;
+    int kpure = 0;
+#pragma unroll
+    for (int k = 0; k < NEM; ++k) {
+      dk[k] = 0.;
+      if (!((mask >> k) & 1)) continue;
+      kpure = k;
+      if (fv[k] > thr) {
+        const double vol_norm = Uvol[k] * vol * unsurvolt;
+        dk[k] = Umas[k] / vol_norm;
+      }
+      density_nplus1 += fv[k] * dk[k];
+    }
+    double rho_g;
+    if (matcell > 1) rho_g = density_nplus1;
+    else if (nbe == 1) {
+      rho_g = 0.;
+#pragma unroll
+      for (int k = 0; k < NEM; ++k)
+        if (k == kpure) rho_g = dk[k];
+    } else {
+      rho_g = A.m[c] / A.Vlag[c];  // left as updateDensity set it (m/V)
+    }
+    double mass_g = vol * density_nplus1;
+    double energie = 0., pseudo = 0.;
+    const double pv = Uq * unsurvol;
+    double mk[NEM];
+#pragma unroll
+    for (int k = 0; k < NEM; ++k) {
+      ek[k] = 0.;
+      mk[k] = 0.;
+      if (!((mask >> k) & 1)) continue;
+      if (fv[k] > thr && Umas[k] != 0.) ek[k] = Uene[k] / Umas[k];
+      energie += mf[k] * ek[k];
+      mk[k] = mf[k] * mass_g;
+      pseudo += fv[k] * pv;
+    }
+    double e_g = energie, q_g = pseudo;
+    double p_g = 0., c_g = 1.e-20;
+    if (nbe == 1) {
+      // pure cell: the environment slot is the cell slot
+#pragma unroll
+      for (int k = 0; k < NEM; ++k)
+        if (k == kpure) mass_g = mk[k];
+      if (rho_g < 0. || e_g < 0.) {
+        e_g = 0.;
+        rho_g = 0.;
+      }
+#pragma unroll
+      for (int k = 0; k < NEM; ++k)
+        if (k == kpure) eos_apply(P.eos, k, rho_g, e_g, &p_g, &c_g);
+    } else {
+#pragma unroll
+      for (int k = 0; k < NEM; ++k) {
+        if (!((mask >> k) & 1)) continue;
+        if (dk[k] < 0. || ek[k] < 0.) {
+          ek[k] = 0.;
+          dk[k] = 0.;
+          fv[k] = 0.;
+          mf[k] = 0.;
+        }
+      }
+      // EOS per environment, then computePressionMoyenne
+#pragma unroll
+      for (int k = 0; k < NEM; ++k) {
+        if (!((mask >> k) & 1)) continue;
+        eos_apply(P.eos, k, dk[k], ek[k], &pk[k], &ck[k]);
+        p_g += fv[k] * pk[k];
+        c_g = smax(ck[k], c_g);
+      }
+    }
+    A.rho[c] = rho_g;
+    A.e[c] = e_g;
+    A.p[c] = p_g;
+    A.cs[c] = c_g;
+    A.q[c] = q_g;
+    A.m[c] = mass_g;
+    A.mask[c] = mask;
+    if (nbe >= 2) {
+#pragma unroll
+      for (int k = 0; k < NEM; ++k) {
+        if (!((mask >> k) & 1)) continue;
+        const long e = k * NC + c;
+        A.env_rho[e] = dk[k];
+        A.env_e[e] = ek[k];
+        A.env_p[e] = pk[k];
+        A.env_cs[e] = ck[k];
+        A.env_q[e] = pv;
+        A.env_m[e] = mk[k];
+        A.env_f[e] = fv[k];
+        A.env_mf[e] = mf[k];
+      }
+    }
+    // CFL candidate: L / (c + max over the cell nodes of |u|)
+    const long n0 = i + (long)nnx * j;
+    const long nd[4] = {n0, n0 + 1, n0 + 1 + nnx, n0 + nnx};
+    double vmax = 0.;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double ux = A.ux[nd[q]], uy = A.uy[nd[q]];
+      const double un = sqrt(ux * ux + uy * uy + 0. * 0.);
+      vmax = (un < vmax) ? vmax : un;
+    }
+    // std::min(minimum_aux, dx_sound) fold: a NaN candidate is skipped
+    dx_sound = smin(dx_sound, A.L[c] / (c_g + vmax));
+  }
+  double v = dx_sound;
+  for (int off = 32; off > 0; off >>= 1) v = smin(v, __shfl_down(v, off, 64));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = red[0];
+    for (int w = 1; w < BLOCK / 64; ++w) b = smin(b, red[w]);
+    A.block_min[blockIdx.x] = b;
+  }
+}
+
+// computeDeltaT (MahycoModule.cc:2076-2191) on the block minima of
+// k_cell_final_2d: dt = min(cfl * min_c dx_sound, 1.05 dt_old, deltat-max),
+// deltat-min check, stop test with the time of this iteration.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_dt_finalize(Params P, DevScalars* S, const double* block_min, int nb) {
+  if (S->stop || S->err) return;
+  __shared__ double red[BLOCK / 64];
+  double v = 1.7976931348623157e308;
+  for (int b = threadIdx.x; b < nb; b += BLOCK) v = smin(v, block_min[b]);
+  for (int off = 32; off > 0; off >>= 1) v = smin(v, __shfl_down(v, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double minimum_aux = red[0];
+  for (int w = 1; w < BLOCK / 64; ++w) minimum_aux = smin(minimum_aux, red[w]);
+  const double old = S->dt;
+  S->dt_old = old;
+  double new_dt = P.cfl * minimum_aux;
+  new_dt = smin(new_dt, 1.05 * old);
+  new_dt = smin(new_dt, P.dtmax);
+  if (new_dt < P.dtmin) {
+    S->err = 1;
+    return;
+  }
+  S->dt = new_dt;
+  const double t = S->time;
+  const bool not_yet_finish = (t < P.tend);
+  const bool finish = (t > P.tend);
+  const bool too_much = ((t + new_dt) > P.tend);
+  if ((not_yet_finish && too_much) || finish) S->stop = 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -472,235 +841,18 @@ __global__ void k_node_sweep_2d(Params P, DevScalars* S, NodeSweepArgs A) {
 }
 
 // ---------------------------------------------------------------------------
-// K5: final cell phase.  remapVariables cell status + membership change
-// (RemapADIFinal.cc:98-226), "moy" (446-643), applyEOS per environment
-// (PerfectGasEOSService.cc:50-137), computePressionMoyenne (MahycoModule.cc:
-// 1836-1915) and the CFL candidate of computeHydroDeltaT (2047-2061) reduced
-// to a device-wide minimum; the last block finishes computeDeltaT (2076-2191).
-// ---------------------------------------------------------------------------
-template <int BLOCK>
-__global__ void k_cell_final_2d(Params P, DevScalars* S, CellFinalArgs A) {
-  if (S->stop || S->err) return;
-  __shared__ double red[BLOCK / 64];
-  __shared__ bool last_block;
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  double dx_sound = 1.7976931348623157e308;
-  if (c < P.nc) {
-    const int nnx = P.nx + 1;
-    const int i = (int)(c % P.nx), j = (int)(c / P.nx);
-    const long NC = P.nc;
-    const int ne = P.ne;
-    const double thr = P.remap_threshold;
-    // Euler volume: same corner-normal formula on the initial coordinates.
-    double x0[4] = {node_x0(P, i), node_x0(P, i + 1), node_x0(P, i + 1), node_x0(P, i)};
-    double y0[4] = {node_y0(P, j), node_y0(P, j), node_y0(P, j + 1), node_y0(P, j + 1)};
-    const double vol = quad_volume(x0, y0);
-    double Uv[3 * kMaxEnv + 1];
-    for (int v = 0; v < P.nvu; ++v) Uv[v] = A.U[v * NC + c];
-    const int old_mask = A.mask[c];
-    // cell status per environment (110-143)
-    double volt = 0.;
-    for (int k = 0; k < ne; ++k) volt += Uv[k];
-    int mask = old_mask;
-    for (int k = 0; k < ne; ++k) {
-      const double fvol = Uv[k] / vol;
-      const bool in_env = (old_mask >> k) & 1;
-      if (fvol < thr && in_env) {
-        mask &= ~(1 << k);
-      } else if (fvol > thr && !in_env) {
-        const double fn = Uv[k] / volt;
-        if (fn > thr && Uv[ne + k] != 0.) mask |= (1 << k);
-      }
-    }
-    const int nbe = __popc(mask);
-    // "moy"
-    double masset = 0.;
-    volt = 0.;
-    for (int k = 0; k < ne; ++k) {
-      volt += Uv[k];
-      masset += Uv[ne + k];
-    }
-    const double unsurvolt = 1. / volt;
-    const double unsurvol = 1. / vol;
-    double fv[kMaxEnv], mf[kMaxEnv], dk[kMaxEnv], ek[kMaxEnv], pk[kMaxEnv], ck[kMaxEnv];
-    double somme_frac = 0.;
-    for (int k = 0; k < ne; ++k) {
-      fv[k] = 0.;
-      if (!((mask >> k) & 1)) continue;
-      double f = Uv[k] * unsurvolt;
-      if (f < thr) f = 0.;
-      fv[k] = f;
-      somme_frac += f;
-    }
-    int matcell = 0;
-    const double unsursomme_frac = 1. / somme_frac;
-    for (int k = 0; k < ne; ++k) {
-      if (!((mask >> k) & 1)) continue;
-      const double f = fv[k] * unsursomme_frac;
-      if (f > 0.) matcell++;
-      fv[k] = f;
-    }
-    for (int k = 0; k < ne; ++k) {
-      // a zero total mass leaves the mass fractions as they were (513-538)
-      mf[k] = (nbe == 1) ? 1.0 : ((old_mask >> k) & 1 ? A.env_mf[k * NC + c] : 0.);
-    }
-    if (masset != 0.) {
-      const double unsurmasset = 1. / masset;
-      double fmasset = 0.;
-      for (int k = 0; k < ne; ++k) {
-        if (!((mask >> k) & 1)) continue;
-        double m = Uv[ne + k] * unsurmasset;
-        if (fv[k] < thr) m = 0.;
-        fmasset += m;
-        mf[k] = m;
-      }
-      if (fmasset != 0.) {
-        const double unsurfmasset = 1. / fmasset;
-        for (int k = 0; k < ne; ++k)
-          if ((mask >> k) & 1) mf[k] = mf[k] * unsurfmasset;
-      }
-    }
-    double density_nplus1 = 0.;
-    for (int k = 0; k < ne; ++k) {
-      dk[k] = 0.;
-      if (!((mask >> k) & 1)) continue;
-      if (fv[k] > thr) {
-        const double vol_norm = Uv[k] * vol * unsurvolt;
-        dk[k] = Uv[ne + k] / vol_norm;
-      }
-      density_nplus1 += fv[k] * dk[k];
-    }
-    const double m_start = A.m[c];
-    double rho_g;
-    if (matcell > 1) rho_g = density_nplus1;
-    else if (nbe == 1) rho_g = dk[__ffs(mask) - 1];
-    else rho_g = m_start / A.Vlag[c];  // left as updateDensity set it (m/V)
-    double mass_g = vol * density_nplus1;
-    double energie = 0., pseudo = 0.;
-    double mk[kMaxEnv], qk[kMaxEnv];
-    const double pv = Uv[3 * ne] * unsurvol;
-    for (int k = 0; k < ne; ++k) {
-      ek[k] = 0.;
-      if (!((mask >> k) & 1)) continue;
-      if (fv[k] > thr && Uv[ne + k] != 0.) ek[k] = Uv[2 * ne + k] / Uv[ne + k];
-      energie += mf[k] * ek[k];
-      mk[k] = mf[k] * mass_g;
-      qk[k] = pv;
-      pseudo += fv[k] * pv;
-    }
-    double e_g = energie, q_g = pseudo;
-    if (nbe == 1) {
-      // pure cell: the environment slot is the cell slot
-      const int k = __ffs(mask) - 1;
-      mass_g = mk[k];
-      if (rho_g < 0. || e_g < 0.) {
-        e_g = 0.;
-        rho_g = 0.;
-      }
-    } else {
-      for (int k = 0; k < ne; ++k) {
-        if (!((mask >> k) & 1)) continue;
-        if (dk[k] < 0. || ek[k] < 0.) {
-          ek[k] = 0.;
-          dk[k] = 0.;
-          fv[k] = 0.;
-          mf[k] = 0.;
-        }
-      }
-    }
-    // EOS and mixed-cell averages
-    double p_g = A.p[c], c_g = A.cs[c];
-    if (nbe == 1) {
-      const int k = __ffs(mask) - 1;
-      eos_apply(P.eos, k, rho_g, e_g, &p_g, &c_g);
-    } else {
-      p_g = 0.;
-      c_g = 1.e-20;
-      for (int k = 0; k < ne; ++k) {
-        if (!((mask >> k) & 1)) continue;
-        eos_apply(P.eos, k, dk[k], ek[k], &pk[k], &ck[k]);
-        p_g += fv[k] * pk[k];
-        c_g = smax(ck[k], c_g);
-      }
-    }
-    A.rho[c] = rho_g;
-    A.e[c] = e_g;
-    A.p[c] = p_g;
-    A.cs[c] = c_g;
-    A.q[c] = q_g;
-    A.m[c] = mass_g;
-    A.mask[c] = mask;
-    if (nbe >= 2) {
-      for (int k = 0; k < ne; ++k) {
-        if (!((mask >> k) & 1)) continue;
-        const long e = k * NC + c;
-        A.env_rho[e] = dk[k];
-        A.env_e[e] = ek[k];
-        A.env_p[e] = pk[k];
-        A.env_cs[e] = ck[k];
-        A.env_q[e] = qk[k];
-        A.env_m[e] = mk[k];
-        A.env_f[e] = fv[k];
-        A.env_mf[e] = mf[k];
-      }
-    }
-    // CFL candidate: L / (c + max over the cell nodes of |u|)
-    const long n0 = i + (long)nnx * j;
-    const long nd[4] = {n0, n0 + 1, n0 + 1 + nnx, n0 + nnx};
-    double vmax = 0.;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const double ux = A.ux[nd[q]], uy = A.uy[nd[q]];
-      const double un = sqrt(ux * ux + uy * uy + 0. * 0.);
-      vmax = (un < vmax) ? vmax : un;
-    }
-    dx_sound = A.L[c] / (c_g + vmax);
-  }
-  // block min then device-wide min on the bit pattern (positive doubles order
-  // like their bits)
-  double v = dx_sound;
-  for (int off = 32; off > 0; off >>= 1) v = smin(v, __shfl_down(v, off, 64));
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) red[wid] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double b = red[0];
-    for (int w = 1; w < BLOCK / 64; ++w) b = smin(b, red[w]);
-    atomicMin(&S->dtmin_bits, (unsigned long long)__double_as_longlong(b));
-    __threadfence();
-    unsigned int done = atomicAdd(&S->counter, 1u);
-    last_block = (done == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (last_block && threadIdx.x == 0) {
-    __threadfence();
-    const double minimum_aux =
-        __longlong_as_double((long long)atomicAdd(&S->dtmin_bits, 0ull));
-    // computeDeltaT (2085-2183)
-    const double old = S->dt;
-    S->dt_old = old;
-    double new_dt = P.cfl * minimum_aux;
-    new_dt = smin(new_dt, 1.05 * old);
-    new_dt = smin(new_dt, P.dtmax);
-    if (new_dt < P.dtmin) {
-      S->err = 1;
-    } else {
-      S->dt = new_dt;
-      const double t = S->time;
-      const bool not_yet_finish = (t < P.tend);
-      const bool finish = (t > P.tend);
-      const bool too_much = ((t + new_dt) > P.tend);
-      if ((not_yet_finish && too_much) || finish) S->stop = 1;
-    }
-    S->dtmin_bits = 0x7FEFFFFFFFFFFFFFull;  // DBL_MAX
-    S->counter = 0u;
-  }
-}
-
-// ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 static inline unsigned nblocks(long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+#define MHG_NEM_DISPATCH(ne, F) \
+  do {                          \
+    if ((ne) <= 1) F(1);        \
+    else if ((ne) <= 2) F(2);   \
+    else if ((ne) <= 3) F(3);   \
+    else if ((ne) <= 4) F(4);   \
+    else F(8);                  \
+  } while (0)
 
 void launch_begin_iteration(hipStream_t st, DevScalars* S) {
   hipLaunchKernelGGL(k_begin_iteration, dim3(1), dim3(1), 0, st, S);
@@ -719,18 +871,33 @@ void launch_node_lagrange_2d(hipStream_t st, const Params& P, DevScalars* S, con
 void launch_cell_lagrange_2d(hipStream_t st, const Params& P, DevScalars* S, const CellLagArgs& A) {
   const int B = 256;
   dim3 g(nblocks(P.nc, B)), b(B);
-  if (P.pseudo_centree) hipLaunchKernelGGL((k_cell_lagrange_2d<true>), g, b, 0, st, P, S, A);
-  else hipLaunchKernelGGL((k_cell_lagrange_2d<false>), g, b, 0, st, P, S, A);
+#define L_(N)                                                                                  \
+  if (P.pseudo_centree) hipLaunchKernelGGL((k_cell_lagrange_2d<true, N>), g, b, 0, st, P, S, A); \
+  else hipLaunchKernelGGL((k_cell_lagrange_2d<false, N>), g, b, 0, st, P, S, A)
+  MHG_NEM_DISPATCH(P.ne, L_);
+#undef L_
 }
+
+constexpr int kStripY = 32;
+constexpr int kVarGroupY = 2;
 
 void launch_cell_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const CellSweepArgs& A,
                           int dir, bool first) {
-  const int B = 256;
-  dim3 g(nblocks(P.nc, B)), b(B);
-  if (dir == 0 && first) hipLaunchKernelGGL((k_cell_sweep_2d<0, true>), g, b, 0, st, P, S, A);
-  else if (dir == 0) hipLaunchKernelGGL((k_cell_sweep_2d<0, false>), g, b, 0, st, P, S, A);
-  else if (first) hipLaunchKernelGGL((k_cell_sweep_2d<1, true>), g, b, 0, st, P, S, A);
-  else hipLaunchKernelGGL((k_cell_sweep_2d<1, false>), g, b, 0, st, P, S, A);
+  if (dir == 0) {
+    const int nseg = (P.nx + 59) / 60;
+    const long waves = (long)nseg * P.ny;
+    dim3 g(nblocks(waves * 64, 256)), b(256);
+#define X_(N)                                                                               \
+  if (first) hipLaunchKernelGGL((k_cell_sweep_x_2d<N, true>), g, b, 0, st, P, S, A);        \
+  else hipLaunchKernelGGL((k_cell_sweep_x_2d<N, false>), g, b, 0, st, P, S, A)
+    MHG_NEM_DISPATCH(P.ne, X_);
+#undef X_
+  } else {
+    dim3 g(nblocks(P.nx, 64), (P.ny + kStripY - 1) / kStripY, (P.nvu + kVarGroupY - 1) / kVarGroupY);
+    dim3 b(64);
+    if (first) hipLaunchKernelGGL((k_cell_sweep_y_2d<true, kStripY, kVarGroupY>), g, b, 0, st, P, S, A);
+    else hipLaunchKernelGGL((k_cell_sweep_y_2d<false, kStripY, kVarGroupY>), g, b, 0, st, P, S, A);
+  }
 }
 
 void launch_node_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const NodeSweepArgs& A,
@@ -743,10 +910,15 @@ void launch_node_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const 
   else hipLaunchKernelGGL((k_node_sweep_2d<1, false>), g, b, 0, st, P, S, A);
 }
 
+int cell_final_blocks(const Params& P) { return (int)nblocks(P.nc, 256); }
+
 void launch_cell_final_2d(hipStream_t st, const Params& P, DevScalars* S, const CellFinalArgs& A) {
   constexpr int B = 256;
   dim3 g(nblocks(P.nc, B)), b(B);
-  hipLaunchKernelGGL((k_cell_final_2d<B>), g, b, 0, st, P, S, A);
+#define F_(N) hipLaunchKernelGGL((k_cell_final_2d<B, N>), g, b, 0, st, P, S, A)
+  MHG_NEM_DISPATCH(P.ne, F_);
+#undef F_
+  hipLaunchKernelGGL((k_dt_finalize<1024>), dim3(1), dim3(1024), 0, st, P, S, A.block_min, (int)g.x);
 }
 
 }  // namespace mhg
