@@ -52,6 +52,44 @@ __host__ __device__ inline double limited_node_grad(int lim, double gb, double g
   return g;
 }
 
+// One term of the limited gradients, lim(a / b) * b, for the minmod limiter,
+// evaluated without a division where the quotient's only role is to be
+// clamped: a and b of opposite signs (or a = 0) give lim = 0 and a zero term,
+// |a| >= |b| with the same sign gives lim = 1 exactly.  Only 0 < a/b < 1 (and
+// any NaN) divides.  Returns false when the term is a signed zero, which
+// leaves the accumulated gradient (started at +0) unchanged.
+// CELL selects the association of the cell form 0.5 * (lim * b) or the nodal
+// form (0.5 * lim) * b.
+template <bool CELL>
+__host__ __device__ inline bool minmod_term(double a, double b, double* term) {
+  const bool opp = (a > 0. && b < 0.) || (a < 0. && b > 0.) || (a == 0.);
+  if (opp) return false;
+  const bool same = (a > 0. && b > 0.) || (a < 0. && b < 0.);
+  if (same && fabs(a) >= fabs(b)) {
+    *term = CELL ? 0.5 * (1.0 * b) : 0.5 * 1.0 * b;
+    return true;
+  }
+  const double l = smax(0.0, smin(1.0, a / b));
+  *term = CELL ? 0.5 * (l * b) : 0.5 * l * b;
+  return true;
+}
+
+// Limited gradients with the limiter fixed at compile time (LIM 0 = minmod
+// uses minmod_term; others evaluate the quotient as the reference does).
+template <int LIM, bool CELL>
+__host__ __device__ inline double limited_grad(double gb, double gf) {
+  double g = 0.;
+  if (LIM == 0) {
+    double t;
+    if (gb != 0. && minmod_term<CELL>(gf, gb, &t)) g += t;
+    if (gf != 0. && minmod_term<CELL>(gb, gf, &t)) g += t;
+    return g;
+  }
+  if (gb != 0.) g += CELL ? 0.5 * (limiter<LIM>(gf / gb) * gb) : 0.5 * limiter<LIM>(gf / gb) * gb;
+  if (gf != 0.) g += CELL ? 0.5 * (limiter<LIM>(gb / gf) * gf) : 0.5 * limiter<LIM>(gb / gf) * gf;
+  return g;
+}
+
 // compute_eint, MahycoModule.cc:1630-1654.
 __host__ __device__ inline double compute_eint(bool pseudo_centree, double adiabatic_cst,
                                                double q_n, double q, double rho_n, double rho,
@@ -156,6 +194,16 @@ struct BcRange {
   int comp;   // velocity component
   double value;
 };
+
+// XCD-aware block order.  The hardware hands block b to XCD b % 8; this maps
+// it to logical block (b % 8) * (n / 8) + b / 8 so that every XCD walks a
+// contiguous band of logical blocks and neighbouring tiles share its L2.
+__device__ inline int xcd_block(int b, int n) {
+  const int full = n & ~7;
+  if (b >= full) return b;
+  const int per = full >> 3;
+  return (b & 7) * per + (b >> 3);
+}
 
 // Scalars kept on the device so steps can run back to back.
 struct DevScalars {

@@ -61,6 +61,7 @@ struct mhg_ctx {
   int xcur = 0;  // buffer with the coordinates of the last Lagrange phase
   int ucur = 0;  // buffer with the current velocity
   int sens = 0;  // SensProjection
+  int yvariant = 0;  // launch variant of the y sweep (mhg_autotune)
 
   // kernel timing
   bool timing = false;
@@ -259,7 +260,7 @@ void remap_2d(mhg_ctx* c) {
     A.contrib = c->contrib;
     {
       TimedLaunch t(c, idir == 0 ? "cell_sweep_x" : "cell_sweep_y");
-      launch_cell_sweep_2d(c->stream, P, c->S, A, idir, first);
+      launch_cell_sweep_2d(c->stream, P, c->S, A, idir, first, c->yvariant);
     }
     NodeSweepArgs B;
     B.m = c->m;
@@ -864,6 +865,71 @@ int mhg_conservation(mhg_ctx* c, double out[3]) {
   out[0] = a + ac;
   out[1] = b + bc;
   out[2] = k + kc;
+  return MHG_OK;
+}
+
+// Autotuner: times every launch variant of the tunable kernels on this
+// context's mesh and keeps the fastest.  The candidate runs write only the
+// sweep output buffers and the dual-flux scratch, which hold no state between
+// steps, so the simulation state is unchanged.
+int mhg_autotune(mhg_ctx* c, int32_t reps, char* report, int32_t report_len) {
+  int r = check_ready(c);
+  if (r) return r;
+  if (reps < 1) reps = 1;
+  const Params& P = c->P;
+  CellSweepArgs A;
+  A.Uin = c->U[0];
+  A.Uout = c->U[1];
+  A.Vlag = c->Vlag;
+  A.xnx = c->xx[c->xcur];
+  A.xny = c->xy[c->xcur];
+  A.ux = c->ux[c->ucur];
+  A.uy = c->uy[c->ucur];
+  A.contrib = c->contrib;
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  std::string rep = "cell_sweep_y:";
+  int best = 0;
+  float best_ms = 1e30f;
+  for (int v = 0; v < cell_sweep_y_variants(); ++v) {
+    for (int w = 0; w < 2; ++w) launch_cell_sweep_2d(c->stream, P, c->S, A, 1, true, v);
+    HIPCHK(c, hipEventRecord(e0, c->stream));
+    for (int k = 0; k < reps; ++k) launch_cell_sweep_2d(c->stream, P, c->S, A, 1, true, v);
+    HIPCHK(c, hipEventRecord(e1, c->stream));
+    HIPCHK(c, hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+    ms /= reps;
+    char buf[96];
+    snprintf(buf, sizeof(buf), " %s=%.4f", cell_sweep_y_variant_name(v), ms);
+    rep += buf;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = v;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  c->yvariant = best;
+  rep += std::string(" -> ") + cell_sweep_y_variant_name(best);
+  if (report && report_len > 0) {
+    std::strncpy(report, rep.c_str(), report_len - 1);
+    report[report_len - 1] = 0;
+  }
+  return MHG_OK;
+}
+
+int32_t mhg_variant_count(const char* kernel) {
+  if (kernel && std::strcmp(kernel, "cell_sweep_y") == 0) return cell_sweep_y_variants();
+  return 0;
+}
+
+int mhg_select_variant(mhg_ctx* c, const char* kernel, int32_t variant) {
+  if (!c) return MHG_E_ARG;
+  const int n = mhg_variant_count(kernel);
+  if (variant < 0 || variant >= n) return fail(c, MHG_E_ARG, "select_variant: unknown kernel or variant");
+  c->yvariant = variant;
   return MHG_OK;
 }
 

@@ -47,7 +47,9 @@ void launch_node_lagrange_2d(hipStream_t st, const Params& P, DevScalars* S, con
                              bool vel, bool bc, bool pos);
 void launch_cell_lagrange_2d(hipStream_t st, const Params& P, DevScalars* S, const CellLagArgs& A);
 void launch_cell_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const CellSweepArgs& A,
-                          int dir, bool first);
+                          int dir, bool first, int variant);
+int cell_sweep_y_variants();
+const char* cell_sweep_y_variant_name(int v);
 void launch_node_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const NodeSweepArgs& A,
                           int dir, bool first);
 void launch_cell_final_2d(hipStream_t st, const Params& P, DevScalars* S, const CellFinalArgs& A);

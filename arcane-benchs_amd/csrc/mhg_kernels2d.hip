@@ -14,6 +14,8 @@
 // +/- e_dir on a Cartesian mesh (InitGeometricValues, MahycoModule.cc:1008-1050
 // evaluates to (1,0,0)-type vectors bit for bit), so dot products with them
 // reduce to one component and are not stored.
+#include <cstdio>
+
 #include "mhg_common.hpp"
 #include "mhg_kernels.hpp"
 
@@ -212,20 +214,35 @@ __device__ inline double face_grad(bool inner, double phi, double phim, double d
   return inner ? (phi - phim) / dx : 0.;
 }
 
-// x sweep: one wavefront holds 64 consecutive cells of a row (60 outputs and
-// two halo cells on each side); stencil neighbours move between lanes.
-template <int NEM, bool FIRST>
-__global__ __launch_bounds__(256) void k_cell_sweep_x_2d(Params P, DevScalars* S, CellSweepArgs A) {
+// Lane sweep: one wavefront holds 64 consecutive cells along the sweep
+// direction D (60 outputs and two halo cells on each side); stencil
+// neighbours move between lanes.  D = 0: a wave is a row segment.  D = 1: a
+// wave is a column segment and a block of 16 waves covers 16 adjacent
+// columns, so each cache line of a row is consumed by the whole block.
+template <int D, int NEM, bool FIRST, int LIM>
+__global__ __launch_bounds__(D == 0 ? 256 : 1024) void k_cell_sweep_lane_2d(Params P, DevScalars* S,
+                                                                          CellSweepArgs A) {
   if (S->stop || S->err) return;
   const int lane = threadIdx.x & 63;
-  const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int nseg = (P.nx + 59) / 60;
-  const int j = (int)(wave / nseg);
-  if (j >= P.ny) return;  // uniform across the wave
-  const int seg = (int)(wave % nseg);
-  const int i = seg * 60 - 2 + lane;
-  const bool ex = (i >= 0 && i < P.nx);
+  const int N = (D == 0) ? P.nx : P.ny;      // cells along D
+  const int T = (D == 0) ? P.ny : P.nx;      // cells across D
+  const int nseg = (N + 59) / 60;
+  int seg, line;                             // segment along D, index across D
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  if (D == 0) {
+    const long wave = ((long)blk * blockDim.x + threadIdx.x) >> 6;
+    line = (int)(wave / nseg);
+    seg = (int)(wave % nseg);
+  } else {
+    const int ncolblk = (T + 15) / 16;
+    line = (blk % ncolblk) * 16 + (threadIdx.x >> 6);
+    seg = blk / ncolblk;
+  }
+  if (line >= T) return;  // uniform across the wave
+  const int s = seg * 60 - 2 + lane;
+  const bool ex = (s >= 0 && s < N);
   const bool out = ex && lane >= 2 && lane <= 61;
+  const int i = (D == 0) ? s : line, j = (D == 0) ? line : s;
   const int nnx = P.nx + 1;
   const long NC = P.nc;
   const int ne = P.ne;
@@ -233,11 +250,14 @@ __global__ __launch_bounds__(256) void k_cell_sweep_x_2d(Params P, DevScalars* S
   const double order2 = (double)P.order2;
   const long c = ex ? i + (long)P.nx * j : 0;
   const double* __restrict__ Uin = A.Uin;
-  // cell centre (x component) and phi denominator
+  const double* __restrict__ Xd = (D == 0) ? A.xnx : A.xny;   // coordinate along D
+  const double* __restrict__ Xt = (D == 0) ? A.xny : A.xnx;   // transverse coordinate
+  const double* __restrict__ Ud = (D == 0) ? A.ux : A.uy;     // velocity along D
+  // cell centre (component D) and phi denominator
   double xc = 0., den = 1.;
   if (ex) {
     const long m0 = i + (long)nnx * j;
-    const double a = A.xnx[m0], b = A.xnx[m0 + 1], cc = A.xnx[m0 + 1 + nnx], dd = A.xnx[m0 + nnx];
+    const double a = Xd[m0], b = Xd[m0 + 1], cc = Xd[m0 + 1 + nnx], dd = Xd[m0 + nnx];
     xc = 0.25 * (((a + b) + cc) + dd);
     if (FIRST) {
       den = A.Vlag[c];
@@ -249,21 +269,21 @@ __global__ __launch_bounds__(256) void k_cell_sweep_x_2d(Params P, DevScalars* S
       den = sv;
     }
   }
-  // back face of cell i = x-face at node column i, nodes (i,j), (i,j+1)
+  // back face of the cell: the face at node index s along D, nodes
+  // (i,j),(i,j+1) for D = 0 and (i,j),(i+1,j) for D = 1
   double uf = 0., lf = 0., xf = 0.;
-  if (i >= 0 && i <= P.nx) {
-    const long na = i + (long)nnx * j, nb = na + nnx;
-    uf = 0.5 * (A.ux[na] + A.ux[nb]);
-    xf = 0.5 * A.xnx[na] + 0.5 * A.xnx[nb];
-    lf = fabs(A.xny[nb] - A.xny[na]);
+  if (s >= 0 && s <= N) {
+    const long na = i + (long)nnx * j, nb = (D == 0) ? na + nnx : na + 1;
+    uf = 0.5 * (Ud[na] + Ud[nb]);
+    xf = 0.5 * Xd[na] + 0.5 * Xd[nb];
+    lf = fabs(Xt[nb] - Xt[na]);
   }
-  const bool inner = ex && i >= 1;
+  const bool inner = ex && s >= 1;
   const double xcm = __shfl_up(xc, 1, 64);
   const double dx = inner ? xc - xcm : 1.;
   const bool upm = inner && (uf * dx > 0.0);
   const double dotxf = inner ? (upm ? xf - xcm : xf - xc) : 0.;
   const double uf_f = __shfl_down(uf, 1, 64), lf_f = __shfl_down(lf, 1, 64);
-  const int lim = P.lim;
   const bool o2 = P.order2 > 0;
   auto sweep_var = [&](int v, double& unew, double& dual) {
     const double u0 = ex ? Uin[v * NC + c] : 0.;
@@ -271,15 +291,19 @@ __global__ __launch_bounds__(256) void k_cell_sweep_x_2d(Params P, DevScalars* S
     const double phim = __shfl_up(phi, 1, 64);
     const double gf = face_grad(inner, phi, phim, dx);
     const double gfp = __shfl_down(gf, 1, 64);
-    const double g = o2 ? limited_cell_grad(lim, gf, gfp) : 0.;
+    const double g = o2 ? limited_grad<LIM, true>(gf, gfp) : 0.;
     const double gm = __shfl_up(g, 1, 64);
     const double phif = inner ? (upm ? phim + order2 * dotxf * gm : phi + order2 * dotxf * g) : 0.;
     const double phifp = __shfl_down(phif, 1, 64);
     const double flux_b = -1.0 * uf * lf * dt * phif;
     const double flux_f = 1.0 * uf_f * lf_f * dt * phifp;
-    // cell faces in order: bottom, right (front), top, left (back)
-    unew = u0 - (flux_f + flux_b);
-    dual = 0.5 * flux_f * 1.0 + 0.5 * flux_b * -1.0;
+    if (D == 0) {  // cell faces in order: bottom, right (front), top, left (back)
+      unew = u0 - (flux_f + flux_b);
+      dual = 0.5 * flux_f * 1.0 + 0.5 * flux_b * -1.0;
+    } else {       // bottom (back), right, top (front), left
+      unew = u0 - (flux_b + flux_f);
+      dual = 0.5 * flux_b * -1.0 + 0.5 * flux_f * 1.0;
+    }
   };
   double* __restrict__ Uout = A.Uout;
 #pragma unroll
@@ -317,27 +341,34 @@ __global__ __launch_bounds__(256) void k_cell_sweep_x_2d(Params P, DevScalars* S
 
 // y sweep: each lane owns one column and walks a strip of rows, keeping a
 // two-cell window of phi, face gradients, limited gradients and face values
-// in registers.  Iteration r loads cell r, forms the gradient of face r, the
+// in registers.  Row r loads cell r, forms the gradient of face r, the
 // limited gradient of cell r-1, the face value of face r-1 and updates cell
 // r-2.  Face r lies between cells r-1 and r (node row r).  The variables of U
 // are independent in a sweep (only the denominator sum_k U_vol,k couples
 // them), so a thread carries VG of them and the variable group is a grid
-// dimension: small register windows, many waves in flight.
-template <bool FIRST, int STRIP, int VG>
+// dimension.  Rows are processed in chunks of R: all loads of a chunk are
+// issued before its arithmetic so that R rows of memory traffic are in flight.
+template <bool FIRST, int STRIP, int VG, int R, int LIM>
 __global__ __launch_bounds__(64) void k_cell_sweep_y_2d(Params P, DevScalars* S, CellSweepArgs A) {
   if (S->stop || S->err) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  // logical tile: variable group fastest, then column block, then strip
+  const int ngroups = (P.nvu + VG - 1) / VG;
+  const int ncb = (P.nx + 63) / 64;
+  const int L = xcd_block(blockIdx.x, gridDim.x);
+  const int vg = L % ngroups;
+  const int t = L / ngroups;
+  const int i = (t % ncb) * 64 + threadIdx.x;
+  const int strip = t / ncb;
   if (i >= P.nx) return;
-  const int j0 = blockIdx.y * STRIP;
+  const int j0 = strip * STRIP;
   const int j1 = min(j0 + STRIP, P.ny);
-  const int v0 = blockIdx.z * VG;
+  const int v0 = vg * VG;
   const int nnx = P.nx + 1;
   const long NC = P.nc;
   const int ne = P.ne;
   const int nvu = P.nvu;
   const double dt = S->dt;
   const double order2 = (double)P.order2;
-  const int lim = P.lim;
   const bool o2 = P.order2 > 0;
   const double* __restrict__ Uin = A.Uin;
   double* __restrict__ Uout = A.Uout;
@@ -348,74 +379,101 @@ __global__ __launch_bounds__(64) void k_cell_sweep_y_2d(Params P, DevScalars* S,
   double uf1 = 0., lf1 = 0., xf1 = 0., dx1 = 1.;  // face r-1
   double uf2 = 0., lf2 = 0.;                      // face r-2
   bool in1 = false;                               // face r-1 inner
-  for (int r = j0 - 2; r <= j1 + 1; ++r) {
-    const bool rv = (r >= 0 && r < P.ny);
-    const long c = rv ? i + (long)P.nx * r : 0;
-    double xc = 0., den = 1.;
-    if (rv) {
-      const long m0 = i + (long)nnx * r;
-      const double a = A.xny[m0], b = A.xny[m0 + 1], cc = A.xny[m0 + 1 + nnx], dd = A.xny[m0 + nnx];
-      xc = 0.25 * (((a + b) + cc) + dd);
-      if (FIRST) {
-        den = A.Vlag[c];
-      } else {
-        double sv = 0.;
-        for (int k = 0; k < ne; ++k) sv += Uin[k * NC + c];
-        den = sv;
+  for (int r0 = j0 - 2; r0 <= j1 + 1; r0 += R) {
+    // ---- loads of the chunk: node rows r0..r0+R, cell rows r0..r0+R-1
+    double ya[R + 1], yb[R + 1], xa[R + 1], xb[R + 1], va[R + 1], vb[R + 1];
+#pragma unroll
+    for (int t = 0; t <= R; ++t) {
+      const int r = r0 + t;
+      ya[t] = yb[t] = xa[t] = xb[t] = va[t] = vb[t] = 0.;
+      if (r >= 0 && r <= P.ny) {
+        const long na = i + (long)nnx * r;
+        ya[t] = A.xny[na];
+        yb[t] = A.xny[na + 1];
+        xa[t] = A.xnx[na];
+        xb[t] = A.xnx[na + 1];
+        va[t] = A.uy[na];
+        vb[t] = A.uy[na + 1];
       }
     }
-    double uf = 0., lf = 0., xf = 0.;
-    if (r >= 0 && r <= P.ny) {  // face r: nodes (i,r), (i+1,r)
-      const long na = i + (long)nnx * r, nb = na + 1;
-      uf = 0.5 * (A.uy[na] + A.uy[nb]);
-      xf = 0.5 * A.xny[na] + 0.5 * A.xny[nb];
-      lf = fabs(A.xnx[nb] - A.xnx[na]);
-    }
-    const bool inner = rv && r >= 1;
-    const double dx = inner ? xc - xc1 : 1.;
-    const bool upm1 = in1 && (uf1 * dx1 > 0.0);
-    const double dot1 = in1 ? (upm1 ? xf1 - xc2 : xf1 - xc1) : 0.;
-    const int out_row = r - 2;
-    const bool out = (out_row >= j0 && out_row < j1);
-    const long co = out ? i + (long)P.nx * out_row : 0;
+    double den[R], Ur[R][VG];
 #pragma unroll
-    for (int s = 0; s < VG; ++s) {
-      const int v = v0 + s;
-      if (v >= nvu) break;
-      const double ur = rv ? Uin[v * NC + c] : 0.;
-      const double phi = rv ? ur / den : 0.;
-      const double gf = face_grad(inner, phi, ph1[s], dx);
-      const double g1 = o2 ? limited_cell_grad(lim, gf1[s], gf) : 0.;  // cell r-1
-      const double pf1 = in1 ? (upm1 ? ph2[s] + order2 * dot1 * g2[s] : ph1[s] + order2 * dot1 * g1) : 0.;
-      if (out) {
-        const double flux_b = -1.0 * uf2 * lf2 * dt * pf2[s];
-        const double flux_f = 1.0 * uf1 * lf1 * dt * pf1;
-        // cell faces in order: bottom (back), right, top (front), left
-        double un = U2[s] - (flux_b + flux_f);
-        if (v >= ne && v < 3 * ne && un < 0.) un = 0.;
-        Uout[v * NC + co] = un;
-        if (v >= ne && v < 2 * ne) {
-          const double dual = 0.5 * flux_b * -1.0 + 0.5 * flux_f * 1.0;
-          A.contrib[(v - ne) * NC + co] = 0.5 * dual;
+    for (int t = 0; t < R; ++t) {
+      const int r = r0 + t;
+      const bool rv = (r >= 0 && r < P.ny);
+      const long c = rv ? i + (long)P.nx * r : 0;
+      den[t] = 1.;
+      if (rv) {
+        if (FIRST) {
+          den[t] = A.Vlag[c];
+        } else {
+          double sv = 0.;
+          for (int k = 0; k < ne; ++k) sv += Uin[k * NC + c];
+          den[t] = sv;
         }
       }
-      ph2[s] = ph1[s];
-      ph1[s] = phi;
-      gf1[s] = gf;
-      g2[s] = g1;
-      pf2[s] = pf1;
-      U2[s] = U1[s];
-      U1[s] = ur;
+#pragma unroll
+      for (int s = 0; s < VG; ++s) Ur[t][s] = (rv && v0 + s < nvu) ? Uin[(v0 + s) * NC + c] : 0.;
     }
-    xc2 = xc1;
-    xc1 = xc;
-    uf2 = uf1;
-    lf2 = lf1;
-    uf1 = uf;
-    lf1 = lf;
-    xf1 = xf;
-    dx1 = dx;
-    in1 = inner;
+    // ---- arithmetic of the chunk, row by row
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+      const int r = r0 + t;
+      if (r > j1 + 1) break;
+      const bool rv = (r >= 0 && r < P.ny);
+      // cell centre: nodes (i,r),(i+1,r),(i+1,r+1),(i,r+1) in quad order
+      const double xc = rv ? 0.25 * (((ya[t] + yb[t]) + yb[t + 1]) + ya[t + 1]) : 0.;
+      // face r: nodes (i,r), (i+1,r)
+      const bool fv = (r >= 0 && r <= P.ny);
+      const double uf = fv ? 0.5 * (va[t] + vb[t]) : 0.;
+      const double xf = fv ? 0.5 * ya[t] + 0.5 * yb[t] : 0.;
+      const double lf = fv ? fabs(xb[t] - xa[t]) : 0.;
+      const bool inner = rv && r >= 1;
+      const double dx = inner ? xc - xc1 : 1.;
+      const bool upm1 = in1 && (uf1 * dx1 > 0.0);
+      const double dot1 = in1 ? (upm1 ? xf1 - xc2 : xf1 - xc1) : 0.;
+      const int out_row = r - 2;
+      const bool out = (out_row >= j0 && out_row < j1);
+      const long co = out ? i + (long)P.nx * out_row : 0;
+#pragma unroll
+      for (int s = 0; s < VG; ++s) {
+        const int v = v0 + s;
+        if (v >= nvu) break;
+        const double ur = Ur[t][s];
+        const double phi = rv ? ur / den[t] : 0.;
+        const double gf = face_grad(inner, phi, ph1[s], dx);
+        const double g1 = o2 ? limited_grad<LIM, true>(gf1[s], gf) : 0.;  // cell r-1
+        const double pf1 = in1 ? (upm1 ? ph2[s] + order2 * dot1 * g2[s] : ph1[s] + order2 * dot1 * g1) : 0.;
+        if (out) {
+          const double flux_b = -1.0 * uf2 * lf2 * dt * pf2[s];
+          const double flux_f = 1.0 * uf1 * lf1 * dt * pf1;
+          // cell faces in order: bottom (back), right, top (front), left
+          double un = U2[s] - (flux_b + flux_f);
+          if (v >= ne && v < 3 * ne && un < 0.) un = 0.;
+          Uout[v * NC + co] = un;
+          if (v >= ne && v < 2 * ne) {
+            const double dual = 0.5 * flux_b * -1.0 + 0.5 * flux_f * 1.0;
+            A.contrib[(v - ne) * NC + co] = 0.5 * dual;
+          }
+        }
+        ph2[s] = ph1[s];
+        ph1[s] = phi;
+        gf1[s] = gf;
+        g2[s] = g1;
+        pf2[s] = pf1;
+        U2[s] = U1[s];
+        U1[s] = ur;
+      }
+      xc2 = xc1;
+      xc1 = xc;
+      uf2 = uf1;
+      lf2 = lf1;
+      uf1 = uf;
+      lf1 = lf;
+      xf1 = xf;
+      dx1 = dx;
+      in1 = inner;
+    }
   }
 }
 
@@ -700,7 +758,7 @@ __device__ inline double node_mass_from_cells(const Params& P, const double* m, 
   return 0.25 * msum;
 }
 
-template <int D, bool FIRST>
+template <int D, bool FIRST, int LIM>
 __global__ void k_node_sweep_2d(Params P, DevScalars* S, NodeSweepArgs A) {
   if (S->stop || S->err) return;
   const long nid = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -765,8 +823,8 @@ __global__ void k_node_sweep_2d(Params P, DevScalars* S, NodeSweepArgs A) {
           const double dxf = xd[o + 3] - xd[o + 2], dxb = xd[o + 2] - xd[o + 1];
           const double gf0 = (ph0[o + 3] - ph0[o + 2]) / dxf, gb0 = (ph0[o + 2] - ph0[o + 1]) / dxb;
           const double gf1 = (ph1[o + 3] - ph1[o + 2]) / dxf, gb1 = (ph1[o + 2] - ph1[o + 1]) / dxb;
-          g0[o + 1] = limited_node_grad(P.lim, gb0, gf0);
-          g1[o + 1] = limited_node_grad(P.lim, gb1, gf1);
+          g0[o + 1] = limited_grad<LIM, false>(gb0, gf0);
+          g1[o + 1] = limited_grad<LIM, false>(gb1, gf1);
         } else {  // boundary node along D: value left by an earlier sweep
           const long q = nid + o * nstride;
           g0[o + 1] = A.dg0[q];
@@ -878,25 +936,69 @@ void launch_cell_lagrange_2d(hipStream_t st, const Params& P, DevScalars* S, con
 #undef L_
 }
 
-constexpr int kStripY = 32;
-constexpr int kVarGroupY = 2;
+// Launch variants of the y sweep (strip length, variables per thread, rows
+// per load chunk), chosen per mesh by the autotuner (mhg_autotune).
+struct YVariant {
+  int strip, vg, r;
+};
+static const YVariant kYVariants[] = {{32, 4, 2}, {8, 4, 2}, {16, 4, 4}, {8, 5, 2}};
+int cell_sweep_y_variants() { return (int)(sizeof(kYVariants) / sizeof(kYVariants[0])); }
+const char* cell_sweep_y_variant_name(int v) {
+  static char buf[16][32];
+  const YVariant& y = kYVariants[v];
+  snprintf(buf[v], 32, "strip%d_vg%d_r%d", y.strip, y.vg, y.r);
+  return buf[v];
+}
+
+#define MHG_LIM_DISPATCH(lim, F) \
+  do {                           \
+    if ((lim) == 0) F(0);        \
+    else if ((lim) == 1) F(1);   \
+    else F(2);                   \
+  } while (0)
+
+template <int STRIP, int VG, int R>
+static void launch_y(hipStream_t st, const Params& P, DevScalars* S, const CellSweepArgs& A, bool first) {
+  const long ntiles = (long)nblocks(P.nx, 64) * ((P.ny + STRIP - 1) / STRIP) * ((P.nvu + VG - 1) / VG);
+  dim3 g((unsigned)ntiles), b(64);
+#define YL_(LM)                                                                                   \
+  if (first) hipLaunchKernelGGL((k_cell_sweep_y_2d<true, STRIP, VG, R, LM>), g, b, 0, st, P, S, A); \
+  else hipLaunchKernelGGL((k_cell_sweep_y_2d<false, STRIP, VG, R, LM>), g, b, 0, st, P, S, A)
+  MHG_LIM_DISPATCH(P.lim, YL_);
+#undef YL_
+}
+
+template <int D, int NEM>
+static void launch_lane(hipStream_t st, const Params& P, DevScalars* S, const CellSweepArgs& A, bool first) {
+  dim3 g, b;
+  if (D == 0) {
+    const long waves = (long)((P.nx + 59) / 60) * P.ny;
+    g = dim3(nblocks(waves * 64, 256));
+    b = dim3(256);
+  } else {
+    g = dim3((unsigned)(((P.nx + 15) / 16) * ((P.ny + 59) / 60)));
+    b = dim3(1024);
+  }
+#define XL_(LM)                                                                                 \
+  if (first) hipLaunchKernelGGL((k_cell_sweep_lane_2d<D, NEM, true, LM>), g, b, 0, st, P, S, A); \
+  else hipLaunchKernelGGL((k_cell_sweep_lane_2d<D, NEM, false, LM>), g, b, 0, st, P, S, A)
+  MHG_LIM_DISPATCH(P.lim, XL_);
+#undef XL_
+}
 
 void launch_cell_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const CellSweepArgs& A,
-                          int dir, bool first) {
+                          int dir, bool first, int variant) {
   if (dir == 0) {
-    const int nseg = (P.nx + 59) / 60;
-    const long waves = (long)nseg * P.ny;
-    dim3 g(nblocks(waves * 64, 256)), b(256);
-#define X_(N)                                                                               \
-  if (first) hipLaunchKernelGGL((k_cell_sweep_x_2d<N, true>), g, b, 0, st, P, S, A);        \
-  else hipLaunchKernelGGL((k_cell_sweep_x_2d<N, false>), g, b, 0, st, P, S, A)
+#define X_(N) launch_lane<0, N>(st, P, S, A, first)
     MHG_NEM_DISPATCH(P.ne, X_);
 #undef X_
-  } else {
-    dim3 g(nblocks(P.nx, 64), (P.ny + kStripY - 1) / kStripY, (P.nvu + kVarGroupY - 1) / kVarGroupY);
-    dim3 b(64);
-    if (first) hipLaunchKernelGGL((k_cell_sweep_y_2d<true, kStripY, kVarGroupY>), g, b, 0, st, P, S, A);
-    else hipLaunchKernelGGL((k_cell_sweep_y_2d<false, kStripY, kVarGroupY>), g, b, 0, st, P, S, A);
+    return;
+  }
+  switch (variant) {
+    case 0: launch_y<32, 4, 2>(st, P, S, A, first); break;
+    case 1: launch_y<8, 4, 2>(st, P, S, A, first); break;
+    case 2: launch_y<16, 4, 4>(st, P, S, A, first); break;
+    default: launch_y<8, 5, 2>(st, P, S, A, first); break;
   }
 }
 
@@ -904,10 +1006,13 @@ void launch_node_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const 
                           int dir, bool first) {
   const int B = 256;
   dim3 g(nblocks(P.nn, B)), b(B);
-  if (dir == 0 && first) hipLaunchKernelGGL((k_node_sweep_2d<0, true>), g, b, 0, st, P, S, A);
-  else if (dir == 0) hipLaunchKernelGGL((k_node_sweep_2d<0, false>), g, b, 0, st, P, S, A);
-  else if (first) hipLaunchKernelGGL((k_node_sweep_2d<1, true>), g, b, 0, st, P, S, A);
-  else hipLaunchKernelGGL((k_node_sweep_2d<1, false>), g, b, 0, st, P, S, A);
+#define NL_(LM)                                                                                       \
+  if (dir == 0 && first) hipLaunchKernelGGL((k_node_sweep_2d<0, true, LM>), g, b, 0, st, P, S, A);      \
+  else if (dir == 0) hipLaunchKernelGGL((k_node_sweep_2d<0, false, LM>), g, b, 0, st, P, S, A);        \
+  else if (first) hipLaunchKernelGGL((k_node_sweep_2d<1, true, LM>), g, b, 0, st, P, S, A);            \
+  else hipLaunchKernelGGL((k_node_sweep_2d<1, false, LM>), g, b, 0, st, P, S, A)
+  MHG_LIM_DISPATCH(P.lim, NL_);
+#undef NL_
 }
 
 int cell_final_blocks(const Params& P) { return (int)nblocks(P.nc, 256); }

@@ -132,7 +132,10 @@ class Library:
                 ("run_async", C.c_int, [vp, C.c_int32]),
                 ("synchronize", C.c_int, [vp]),
                 ("kernel_times", C.c_int, [vp, C.c_int32, C.POINTER(C.c_char_p), _DP, _IP, C.c_int32]),
-                ("set_threads", C.c_int, [C.c_int32])]:
+                ("set_threads", C.c_int, [C.c_int32]),
+                ("autotune", C.c_int, [vp, C.c_int32, C.c_char_p, C.c_int32]),
+                ("variant_count", C.c_int32, [C.c_char_p]),
+                ("select_variant", C.c_int, [vp, C.c_char_p, C.c_int32])]:
             if hasattr(L, p + name):
                 self.optional[name] = fn(name, res, args)
 
@@ -302,6 +305,22 @@ class Simulation:
         done = C.c_int32()
         self._check(self.L.run(self.ctx, max_steps, C.byref(done)), "run")
         return done.value
+
+    def autotune(self, reps=10):
+        f = self.L.optional.get("autotune")
+        if f is None:
+            return ""
+        buf = C.create_string_buffer(4096)
+        self._check(f(self.ctx, reps, buf, 4096), "autotune")
+        return buf.value.decode()
+
+    def variant_count(self, kernel="cell_sweep_y"):
+        f = self.L.optional.get("variant_count")
+        return 0 if f is None else int(f(kernel.encode()))
+
+    def select_variant(self, variant, kernel="cell_sweep_y"):
+        self._check(self.L.optional["select_variant"](self.ctx, kernel.encode(), variant),
+                    "select_variant")
 
     def conservation(self):
         out = np.zeros(3)

@@ -82,6 +82,8 @@ def main():
 
     gpu = load_gpu()
     sim = cases.triple_point(gpu, NX, NY)
+    tuning = sim.autotune(10)
+    print("autotune:", tuning, file=sys.stderr)
     if args.warmup:
         sim.L.optional["run_async"](sim.ctx, args.warmup)
         sim.L.optional["synchronize"](sim.ctx)
@@ -134,7 +136,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "triple point 1792x768, 3 perfect-gas materials, plain FP64 "
                                "Lagrange + Euler-ADI remap (order 2, minmod), 500 steps",
-                   "cells": nc, "nodes": nn, "steps": args.steps, "parallelism": "none (1 GPU)"},
+                   "cells": nc, "nodes": nn, "steps": args.steps, "parallelism": "none (1 GPU)",
+                   "tuning": tuning.split("->")[-1].strip() if tuning else None},
         "l1_density_err_vs_oracle": (None if l1 is None else
                                      {"value": l1, "after_steps": args.cpu_steps}),
         "step_roofline_frac": round(step_frac, 4),

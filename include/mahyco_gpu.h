@@ -201,6 +201,18 @@ int mhg_run(mhg_ctx* ctx, int32_t max_steps, int32_t* steps_done);
 int mhg_run_async(mhg_ctx* ctx, int32_t n_steps);
 int mhg_synchronize(mhg_ctx* ctx);
 
+/* ---- tuning (extension, no counterpart in the reference) --------------- */
+/* Times every launch variant of the tunable kernels on this context's mesh
+ * (reps launches each) and keeps the fastest; the simulation state is not
+ * changed.  report (may be NULL) receives the per-variant timings. */
+int mhg_autotune(mhg_ctx* ctx, int32_t reps, char* report, int32_t report_len);
+/* Number of launch variants of a tunable kernel ("cell_sweep_y"), 0 for an
+ * unknown name. */
+int32_t mhg_variant_count(const char* kernel);
+/* Selects launch variant `variant` of a tunable kernel for later steps
+ * (MHG_E_ARG for an unknown kernel or an index out of range). */
+int mhg_select_variant(mhg_ctx* ctx, const char* kernel, int32_t variant);
+
 /* ---- diagnostics ---------------------------------------------------- */
 /* Compensated global sums: total mass, total internal energy
  * (sum m*e), total kinetic energy (sum 1/2 m_n |u|^2). */

@@ -9,7 +9,7 @@ Trajectories are compared in the max-norm relative metric |a-b|_inf / |b|_inf.""
 import numpy as np
 import pytest
 
-import helpers as H
+import helpers as H  # noqa: E402
 from mahyco_amd import load_gpu
 
 pytestmark = pytest.mark.gpu
@@ -97,3 +97,40 @@ def test_mass_conservation(gpu):
     eo = (o1[1] + o1[2] - o0[1] - o0[2]) / (o0[1] + o0[2])
     print("TP energy drift gpu", eg, "oracle", eo)
     assert abs(eg - eo) < 1e-12
+
+
+@pytest.mark.parametrize("kw", [dict(projection_limiteur_id=1), dict(projection_limiteur_id=2),
+                                dict(ordre_projection=1), dict(pseudo_centree=1),
+                                dict(longueur_caracteristique=1)])
+def test_triple_point_options_one_step(gpu, kw):
+    _one_step_parity(gpu, H.tp_setup, 40, 56, 24, **kw)
+
+
+def test_two_material_sod_one_step(gpu):
+    def bisod(L, nx, ny):
+        opt = L.options()
+        H.abi.set_bcs(opt, H.abi.reflecting_walls(2))
+        H.abi.set_envs(opt, [(H.abi.EOS_PERFECT_GAS, 1.4, 0.), (H.abi.EOS_PERFECT_GAS, 1.4, 0.)])
+        s = H.abi.Simulation(L, H.abi.mesh_desc(2, [nx, ny], [0, 0], [1.0, ny / nx]), opt)
+        s.init_sod(13)
+        return s
+    bisod.__name__ = "bisod"
+    _one_step_parity(gpu, bisod, 30, 40, 4)
+
+
+@pytest.mark.parametrize("variant", range(8))
+def test_cell_sweep_y_variants_one_step(gpu, variant):
+    """Every launch variant of the y cell sweep on a mesh that no strip, row
+    chunk or 60-cell segment divides (70 x 134)."""
+    sim = H.tp_setup(gpu, 70, 134)
+    n = sim.variant_count()
+    if variant >= n:
+        pytest.skip(f"{n} variants")
+
+    def tp_variant(L, nx, ny):
+        s = H.tp_setup(L, nx, ny)
+        if L is gpu:
+            s.select_variant(variant)
+        return s
+    tp_variant.__name__ = f"tp_variant{variant}"
+    _one_step_parity(gpu, tp_variant, 20, 70, 134)
