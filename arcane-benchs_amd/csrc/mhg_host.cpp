@@ -61,7 +61,9 @@ struct mhg_ctx {
   int xcur = 0;  // buffer with the coordinates of the last Lagrange phase
   int ucur = 0;  // buffer with the current velocity
   int sens = 0;  // SensProjection
-  int yvariant = 0;  // launch variant of the y sweep (mhg_autotune)
+  int yvariant = 0;   // launch variant of the y cell sweep (mhg_autotune)
+  int nxvariant = 1;  // launch variant of the x dual sweep
+  int nyvariant = 1;  // launch variant of the y dual sweep
 
   // kernel timing
   bool timing = false;
@@ -278,7 +280,7 @@ void remap_2d(mhg_ctx* c) {
     B.uoy = c->uy[1 - c->ucur];
     {
       TimedLaunch t(c, idir == 0 ? "node_sweep_x" : "node_sweep_y");
-      launch_node_sweep_2d(c->stream, P, c->S, B, idir, first);
+      launch_node_sweep_2d(c->stream, P, c->S, B, idir, first, idir == 0 ? c->nxvariant : c->nyvariant);
     }
   }
   CellFinalArgs F;
@@ -869,9 +871,32 @@ int mhg_conservation(mhg_ctx* c, double out[3]) {
 }
 
 // Autotuner: times every launch variant of the tunable kernels on this
-// context's mesh and keeps the fastest.  The candidate runs write only the
-// sweep output buffers and the dual-flux scratch, which hold no state between
-// steps, so the simulation state is unchanged.
+// context's mesh and keeps the fastest.  The candidates write the sweep output
+// buffers, the dual-flux scratch and the dual state, which hold no state
+// between steps, and the boundary dual gradients, which are saved before and
+// restored after; the simulation state is unchanged.
+namespace {
+const char* kTunable[] = {"cell_sweep_y", "node_sweep_x", "node_sweep_y"};
+int tunable_index(const char* kernel) {
+  if (!kernel) return -1;
+  for (int k = 0; k < 3; ++k)
+    if (std::strcmp(kernel, kTunable[k]) == 0) return k;
+  return -1;
+}
+int variant_count_of(int k) {
+  if (k == 0) return cell_sweep_y_variants();
+  if (k == 1) return node_sweep_variants(0);
+  if (k == 2) return node_sweep_variants(1);
+  return 0;
+}
+const char* variant_name_of(int k, int v) {
+  return k == 0 ? cell_sweep_y_variant_name(v) : node_sweep_variant_name(k - 1, v);
+}
+int* variant_slot(mhg_ctx* c, int k) {
+  return k == 0 ? &c->yvariant : (k == 1 ? &c->nxvariant : &c->nyvariant);
+}
+}  // namespace
+
 int mhg_autotune(mhg_ctx* c, int32_t reps, char* report, int32_t report_len) {
   int r = check_ready(c);
   if (r) return r;
@@ -886,33 +911,63 @@ int mhg_autotune(mhg_ctx* c, int32_t reps, char* report, int32_t report_len) {
   A.ux = c->ux[c->ucur];
   A.uy = c->uy[c->ucur];
   A.contrib = c->contrib;
+  NodeSweepArgs B;
+  B.m = c->m;
+  B.ux = A.ux;
+  B.uy = A.uy;
+  B.xnx = A.xnx;
+  B.xny = A.xny;
+  B.contrib = c->contrib;
+  B.Ud0 = c->Ud0;
+  B.Ud1 = c->Ud1;
+  B.Ud3 = c->Ud3;
+  B.dg0 = c->dg0;
+  B.dg1 = c->dg1;
+  B.uox = c->ux[1 - c->ucur];
+  B.uoy = c->uy[1 - c->ucur];
+  const size_t gbytes = sizeof(double) * (size_t)P.nn;
+  double* saved = nullptr;
+  if (hipMalloc(&saved, 2 * gbytes) != hipSuccess) return fail(c, MHG_E_ALLOC, "autotune: hipMalloc failed");
+  HIPCHK(c, hipMemcpyAsync(saved, c->dg0, gbytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(saved + P.nn, c->dg1, gbytes, hipMemcpyDeviceToDevice, c->stream));
   hipEvent_t e0, e1;
   HIPCHK(c, hipEventCreate(&e0));
   HIPCHK(c, hipEventCreate(&e1));
-  std::string rep = "cell_sweep_y:";
-  int best = 0;
-  float best_ms = 1e30f;
-  for (int v = 0; v < cell_sweep_y_variants(); ++v) {
-    for (int w = 0; w < 2; ++w) launch_cell_sweep_2d(c->stream, P, c->S, A, 1, true, v);
-    HIPCHK(c, hipEventRecord(e0, c->stream));
-    for (int k = 0; k < reps; ++k) launch_cell_sweep_2d(c->stream, P, c->S, A, 1, true, v);
-    HIPCHK(c, hipEventRecord(e1, c->stream));
-    HIPCHK(c, hipEventSynchronize(e1));
-    float ms = 0.f;
-    HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
-    ms /= reps;
-    char buf[96];
-    snprintf(buf, sizeof(buf), " %s=%.4f", cell_sweep_y_variant_name(v), ms);
-    rep += buf;
-    if (ms < best_ms) {
-      best_ms = ms;
-      best = v;
+  std::string rep;
+  for (int k = 0; k < 3; ++k) {
+    auto launch = [&](int v) {
+      if (k == 0) launch_cell_sweep_2d(c->stream, P, c->S, A, 1, true, v);
+      else launch_node_sweep_2d(c->stream, P, c->S, B, k - 1, true, v);
+    };
+    rep += std::string(k ? " " : "") + kTunable[k] + ":";
+    int best = 0;
+    float best_ms = 1e30f;
+    for (int v = 0; v < variant_count_of(k); ++v) {
+      for (int w = 0; w < 2; ++w) launch(v);
+      HIPCHK(c, hipEventRecord(e0, c->stream));
+      for (int n = 0; n < reps; ++n) launch(v);
+      HIPCHK(c, hipEventRecord(e1, c->stream));
+      HIPCHK(c, hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+      ms /= reps;
+      char buf[96];
+      snprintf(buf, sizeof(buf), " %s=%.4f", variant_name_of(k, v), ms);
+      rep += buf;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = v;
+      }
     }
+    *variant_slot(c, k) = best;
+    rep += std::string(" -> ") + variant_name_of(k, best) + ";";
   }
+  HIPCHK(c, hipMemcpyAsync(c->dg0, saved, gbytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->dg1, saved + P.nn, gbytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  hipFree(saved);
   hipEventDestroy(e0);
   hipEventDestroy(e1);
-  c->yvariant = best;
-  rep += std::string(" -> ") + cell_sweep_y_variant_name(best);
   if (report && report_len > 0) {
     std::strncpy(report, rep.c_str(), report_len - 1);
     report[report_len - 1] = 0;
@@ -920,16 +975,14 @@ int mhg_autotune(mhg_ctx* c, int32_t reps, char* report, int32_t report_len) {
   return MHG_OK;
 }
 
-int32_t mhg_variant_count(const char* kernel) {
-  if (kernel && std::strcmp(kernel, "cell_sweep_y") == 0) return cell_sweep_y_variants();
-  return 0;
-}
+int32_t mhg_variant_count(const char* kernel) { return variant_count_of(tunable_index(kernel)); }
 
 int mhg_select_variant(mhg_ctx* c, const char* kernel, int32_t variant) {
   if (!c) return MHG_E_ARG;
-  const int n = mhg_variant_count(kernel);
-  if (variant < 0 || variant >= n) return fail(c, MHG_E_ARG, "select_variant: unknown kernel or variant");
-  c->yvariant = variant;
+  const int k = tunable_index(kernel);
+  if (k < 0 || variant < 0 || variant >= variant_count_of(k))
+    return fail(c, MHG_E_ARG, "select_variant: unknown kernel or variant");
+  *variant_slot(c, k) = variant;
   return MHG_OK;
 }
 

@@ -51,7 +51,9 @@ void launch_cell_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const 
 int cell_sweep_y_variants();
 const char* cell_sweep_y_variant_name(int v);
 void launch_node_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const NodeSweepArgs& A,
-                          int dir, bool first);
+                          int dir, bool first, int variant);
+int node_sweep_variants(int dir);
+const char* node_sweep_variant_name(int dir, int v);
 void launch_cell_final_2d(hipStream_t st, const Params& P, DevScalars* S, const CellFinalArgs& A);
 
 }  // namespace mhg

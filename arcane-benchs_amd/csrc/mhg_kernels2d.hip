@@ -348,6 +348,112 @@ __global__ __launch_bounds__(D == 0 ? 256 : 1024) void k_cell_sweep_lane_2d(Para
 // them), so a thread carries VG of them and the variable group is a grid
 // dimension.  Rows are processed in chunks of R: all loads of a chunk are
 // issued before its arithmetic so that R rows of memory traffic are in flight.
+// y sweep on LDS tiles: a block of 16 wavefronts owns 16 columns (one
+// 128-byte line of a row) and 64 rows (60 outputs, two halo rows on each
+// side).  Global loads and stores are row-major and coalesced; the tile is
+// transposed through LDS so that each wavefront holds one column with one row
+// per lane and runs the lane-exchange arithmetic of the x sweep.
+constexpr int kTileW = 16, kTileR = 64, kTileP = kTileW + 1;
+
+template <int NEM, bool FIRST, int LIM>
+__global__ __launch_bounds__(1024) void k_cell_sweep_ytile_2d(Params P, DevScalars* S, CellSweepArgs A) {
+  __shared__ double lds[5][kTileR][kTileP];
+  if (S->stop || S->err) return;  // uniform over the grid
+  const int nx = P.nx, N = P.ny, nnx = nx + 1;
+  const int ncolblk = (nx + kTileW - 1) / kTileW;
+  const int L = xcd_block(blockIdx.x, gridDim.x);
+  const int i0 = (L % ncolblk) * kTileW;
+  const int s0 = (L / ncolblk) * 60 - 2;  // cell row of tile row 0
+  const int t = threadIdx.x;
+  const int tr = t >> 4, tc = t & 15;     // row-major position (loads, stores)
+  const int w = t >> 6, lane = t & 63;    // transposed position: column w, row lane
+  const long NC = P.nc;
+  const int ne = P.ne, nvu = P.nvu;
+  const double dt = S->dt;
+  const double order2 = (double)P.order2;
+  const bool o2 = P.order2 > 0;
+  const double* __restrict__ Uin = A.Uin;
+  // row-major cell of this thread
+  const int gi = i0 + tc, gs = s0 + tr;
+  const bool gex = gi < nx && gs >= 0 && gs < N;
+  const bool gout = gex && tr >= 2 && tr <= 61;
+  const long gc = gex ? gi + (long)nx * gs : 0;
+  {  // geometry of the tile, row-major
+    double xc = 0., den = 1., uf = 0., xf = 0., lf = 0.;
+    if (gex) {
+      const long m0 = gi + (long)nnx * gs;
+      const double a = A.xny[m0], b = A.xny[m0 + 1], cc = A.xny[m0 + 1 + nnx], dd = A.xny[m0 + nnx];
+      xc = 0.25 * (((a + b) + cc) + dd);
+      if (FIRST) {
+        den = A.Vlag[gc];
+      } else {
+        double sv = 0.;
+#pragma unroll
+        for (int k = 0; k < NEM; ++k)
+          if (k < ne) sv += Uin[k * NC + gc];
+        den = sv;
+      }
+    }
+    if (gi < nx && gs >= 0 && gs <= N) {  // back face: nodes (i, s), (i+1, s)
+      const long na = gi + (long)nnx * gs, nb = na + 1;
+      uf = 0.5 * (A.uy[na] + A.uy[nb]);
+      xf = 0.5 * A.xny[na] + 0.5 * A.xny[nb];
+      lf = fabs(A.xnx[nb] - A.xnx[na]);
+    }
+    lds[0][tr][tc] = xc;
+    lds[1][tr][tc] = den;
+    lds[2][tr][tc] = uf;
+    lds[3][tr][tc] = xf;
+    lds[4][tr][tc] = lf;
+  }
+  __syncthreads();
+  const int s = s0 + lane;
+  const bool ex = (i0 + w < nx) && s >= 0 && s < N;
+  const bool inner = ex && s >= 1;
+  const double xc = lds[0][lane][w], den = lds[1][lane][w];
+  const double uf = lds[2][lane][w], xf = lds[3][lane][w], lf = lds[4][lane][w];
+  const double xcm = __shfl_up(xc, 1, 64);
+  const double dx = inner ? xc - xcm : 1.;
+  const bool upm = inner && (uf * dx > 0.0);
+  const double dotxf = inner ? (upm ? xf - xcm : xf - xc) : 0.;
+  const double uf_f = __shfl_down(uf, 1, 64), lf_f = __shfl_down(lf, 1, 64);
+  __syncthreads();  // geometry tiles are reused as staging tiles below
+  double(*stin)[kTileP] = lds[0];
+  double(*stu)[kTileP] = lds[1];
+  double(*stc)[kTileP] = lds[2];
+  double nxt = gex ? Uin[gc] : 0.;
+  for (int v = 0; v < nvu; ++v) {
+    stin[tr][tc] = nxt;
+    __syncthreads();
+    if (v + 1 < nvu) nxt = gex ? Uin[(v + 1) * NC + gc] : 0.;
+    const double u0 = stin[lane][w];
+    const double phi = ex ? u0 / den : 0.;
+    const double phim = __shfl_up(phi, 1, 64);
+    const double gf = face_grad(inner, phi, phim, dx);
+    const double gfp = __shfl_down(gf, 1, 64);
+    const double g = o2 ? limited_grad<LIM, true>(gf, gfp) : 0.;
+    const double gm = __shfl_up(g, 1, 64);
+    const double phif = inner ? (upm ? phim + order2 * dotxf * gm : phi + order2 * dotxf * g) : 0.;
+    const double phifp = __shfl_down(phif, 1, 64);
+    const double flux_b = -1.0 * uf * lf * dt * phif;
+    const double flux_f = 1.0 * uf_f * lf_f * dt * phifp;
+    // cell faces in order: bottom (back), right, top (front), left
+    double un = u0 - (flux_b + flux_f);
+    if (v >= ne && v < 3 * ne && un < 0.) un = 0.;  // RemapADIService.cc:953-958
+    stu[lane][w] = un;
+    const bool mass = (v >= ne && v < 2 * ne);
+    if (mass) {
+      const double dual = 0.5 * flux_b * -1.0 + 0.5 * flux_f * 1.0;
+      stc[lane][w] = 0.5 * dual;  // fcontrib (DualPhaseRemap.cc:168-185)
+    }
+    __syncthreads();
+    if (gout) {
+      A.Uout[v * NC + gc] = stu[tr][tc];
+      if (mass) A.contrib[(v - ne) * NC + gc] = stc[tr][tc];
+    }
+  }
+}
+
 template <bool FIRST, int STRIP, int VG, int R, int LIM>
 __global__ __launch_bounds__(64) void k_cell_sweep_y_2d(Params P, DevScalars* S, CellSweepArgs A) {
   if (S->stop || S->err) return;
@@ -898,6 +1004,288 @@ __global__ void k_node_sweep_2d(Params P, DevScalars* S, NodeSweepArgs A) {
   }
 }
 
+// Dual sweep along x with lane exchange: a wavefront holds 64 consecutive
+// nodes of a row (60 outputs, two halo nodes on each side).  Every node forms
+// its phi_dual and its limited gradient once and reads its neighbours' values
+// across lanes.  front(i) sums the cells (i, j-1), (i, j) in the same order as
+// back(i+1) does, so back is the front of the previous lane.
+template <bool FIRST, int NEM, int LIM>
+__global__ __launch_bounds__(256) void k_node_sweep_x_2d(Params P, DevScalars* S, NodeSweepArgs A) {
+  if (S->stop || S->err) return;
+  const int lane = threadIdx.x & 63;
+  const int N = P.nx;
+  const int nseg = (N + 1 + 59) / 60;
+  const long wave = ((long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
+  const int j = (int)(wave / nseg);
+  if (j > P.ny) return;  // uniform across the wave
+  const int seg = (int)(wave % nseg);
+  const int i = seg * 60 - 2 + lane;
+  const bool ex = (i >= 0 && i <= N);
+  const bool out = ex && lane >= 2 && lane <= 61;
+  const long nid = ex ? i + (long)(N + 1) * j : 0;
+  const bool inner_t = (j >= 1 && j <= P.ny - 1);
+  const bool inner_d = (i >= 1 && i <= N - 1);
+  const long NC = P.nc;
+  const int ne = P.ne;
+  const double dt = S->dt;
+  double U0 = 0., U1 = 0., U3 = 0., xd = 0., p0 = 0., p1 = 0.;
+  if (ex) {
+    xd = A.xnx[nid];
+    if (FIRST) {
+      const double mn = node_mass_from_cells(P, A.m, i, j);
+      p0 = A.ux[nid];
+      p1 = A.uy[nid];
+      U0 = mn * p0;
+      U1 = mn * p1;
+      U3 = mn;
+    } else {
+      U0 = A.Ud0[nid];
+      U1 = A.Ud1[nid];
+      U3 = A.Ud3[nid];
+      // nodes inner in the first direction carry phi = U/m (DualPhaseRemap.cc:494-523)
+      if (inner_t) {
+        p0 = U0 / U3;
+        p1 = U1 / U3;
+      } else {
+        p0 = A.ux[nid];
+        p1 = A.uy[nid];
+      }
+    }
+  }
+  // front dual mass flux: cells (i, j-1), (i, j), environments in order
+  double front = 0.;
+  if (ex && i < P.nx) {
+#pragma unroll
+    for (int sh = -1; sh <= 0; ++sh) {
+      const int cj = j + sh;
+      if (cj < 0 || cj >= P.ny) continue;
+      const long cc = i + (long)P.nx * cj;
+#pragma unroll
+      for (int k = 0; k < NEM; ++k) {
+        if (k >= ne) break;
+        front += A.contrib[k * NC + cc];
+      }
+    }
+  }
+  const double xdm = __shfl_up(xd, 1, 64), xdp = __shfl_down(xd, 1, 64);
+  const double p0m = __shfl_up(p0, 1, 64), p0p = __shfl_down(p0, 1, 64);
+  const double p1m = __shfl_up(p1, 1, 64), p1p = __shfl_down(p1, 1, 64);
+  const double back = __shfl_up(front, 1, 64);
+  // limited nodal gradient (computeDualGradPhi_LimC, DualPhaseRemap.cc:700-779)
+  double g0 = 0., g1 = 0.;
+  if (P.order2 > 0 && ex) {
+    if (inner_d) {
+      const double dxf = xdp - xd, dxb = xd - xdm;
+      const double gf0 = (p0p - p0) / dxf, gb0 = (p0 - p0m) / dxb;
+      const double gf1 = (p1p - p1) / dxf, gb1 = (p1 - p1m) / dxb;
+      g0 = limited_grad<LIM, false>(gb0, gf0);
+      g1 = limited_grad<LIM, false>(gb1, gf1);
+      if (!inner_t && out) {  // kept for the sweep in the other direction
+        A.dg0[nid] = g0;
+        A.dg1[nid] = g1;
+      }
+    } else {  // boundary node along x: gradient left by an earlier sweep
+      g0 = A.dg0[nid];
+      g1 = A.dg1[nid];
+    }
+  }
+  const double g0m = __shfl_up(g0, 1, 64), g0p = __shfl_down(g0, 1, 64);
+  const double g1m = __shfl_up(g1, 1, 64), g1p = __shfl_down(g1, 1, 64);
+  if (!out) return;
+  if (inner_d) {
+    U3 += back - front;
+    const double order2 = (double)P.order2;
+    const double ufront = 0.5 * (p0 + p0);  // DualPhaseRemap.cc:391
+    const int signfront = (ufront > 0) ? 1 : -1;
+    const double xfr = xdp - xd;
+    const bool sf = (front < 0.);
+    const double tf = signfront * xfr - dt * ufront;
+    const double FV0 = (sf ? p0p : p0) + order2 * (0.5 * (sf ? g0p : g0) * tf);
+    const double FV1 = (sf ? p1p : p1) + order2 * (0.5 * (sf ? g1p : g1) * tf);
+    const double uback = 0.5 * (p0m + p0);
+    const int signback = (uback > 0) ? 1 : -1;
+    const double xbk = xd - xdm;
+    const bool sb = (back > 0.);
+    const double tb = signback * xbk - dt * uback;
+    const double BV0 = (sb ? p0m : p0) + order2 * (0.5 * (sb ? g0m : g0) * tb);
+    const double BV1 = (sb ? p1m : p1) + order2 * (0.5 * (sb ? g1m : g1) * tb);
+    U0 += back * BV0 - front * FV0;
+    U1 += back * BV1 - front * FV1;
+    const double thresold = 1.e-300;
+    if (fabs(U0) < thresold) U0 = 0.;
+    if (fabs(U1) < thresold) U1 = 0.;
+  }
+  if (FIRST) {
+    A.Ud0[nid] = U0;
+    A.Ud1[nid] = U1;
+    A.Ud3[nid] = U3;
+  } else if (U3 != 0.) {
+    A.uox[nid] = U0 / U3;
+    A.uoy[nid] = U1 / U3;
+  } else {
+    A.uox[nid] = A.ux[nid];
+    A.uoy[nid] = A.uy[nid];
+  }
+}
+
+// Dual sweep along y: each lane owns one node column and walks a strip of
+// STRIP node rows (two halo rows above and below).  Row r's phi_dual,
+// coordinate and front flux enter a four-row register window, its gradient
+// g(r-1) is formed once, and node r-2 is finished.  front(j) sums the cells
+// (i-1, j), (i, j) in the order back(j+1) does.
+template <bool FIRST, int NEM, int LIM, int STRIP>
+__global__ __launch_bounds__(64) void k_node_sweep_y_2d(Params P, DevScalars* S, NodeSweepArgs A) {
+  if (S->stop || S->err) return;
+  const int nnx = P.nx + 1;
+  const int ncb = (nnx + 63) / 64;
+  const int L = xcd_block(blockIdx.x, gridDim.x);
+  const int i = (L % ncb) * 64 + threadIdx.x;
+  if (i > P.nx) return;
+  const int N = P.ny;
+  const int j0 = (L / ncb) * STRIP;
+  const int j1 = min(j0 + STRIP, N + 1);  // output node rows [j0, j1)
+  const bool inner_t = (i >= 1 && i <= P.nx - 1);
+  const long NC = P.nc;
+  const int ne = P.ne;
+  const double dt = S->dt;
+  const bool o2 = P.order2 > 0;
+  const double order2 = (double)P.order2;
+  // windows: [0] row r-3, [1] r-2, [2] r-1, [3] r
+  double xw[4] = {0., 0., 0., 0.}, p0w[4] = {0., 0., 0., 0.}, p1w[4] = {0., 0., 0., 0.};
+  double fw[4] = {0., 0., 0., 0.};
+  double U0w[3] = {0., 0., 0.}, U1w[3] = {0., 0., 0.}, U3w[3] = {0., 0., 0.};  // rows r-2, r-1, r
+  double g0w[3] = {0., 0., 0.}, g1w[3] = {0., 0., 0.};                         // g(r-3), g(r-2), g(r-1)
+  for (int r = j0 - 2; r <= j1 + 1; ++r) {
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      xw[t] = xw[t + 1];
+      p0w[t] = p0w[t + 1];
+      p1w[t] = p1w[t + 1];
+      fw[t] = fw[t + 1];
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      U0w[t] = U0w[t + 1];
+      U1w[t] = U1w[t + 1];
+      U3w[t] = U3w[t + 1];
+      g0w[t] = g0w[t + 1];
+      g1w[t] = g1w[t + 1];
+    }
+    double xd = 0., p0 = 0., p1 = 0., U0 = 0., U1 = 0., U3 = 0., f = 0.;
+    if (r >= 0 && r <= N) {
+      const long q = i + (long)nnx * r;
+      xd = A.xny[q];
+      if (FIRST) {
+        p0 = A.ux[q];
+        p1 = A.uy[q];
+      } else {
+        U0 = A.Ud0[q];
+        U1 = A.Ud1[q];
+        U3 = A.Ud3[q];
+        if (inner_t) {
+          p0 = U0 / U3;
+          p1 = U1 / U3;
+        } else {
+          p0 = A.ux[q];
+          p1 = A.uy[q];
+        }
+      }
+      if (r < N) {
+#pragma unroll
+        for (int sh = -1; sh <= 0; ++sh) {
+          const int ci = i + sh;
+          if (ci < 0 || ci >= P.nx) continue;
+          const long cc = ci + (long)P.nx * r;
+#pragma unroll
+          for (int k = 0; k < NEM; ++k) {
+            if (k >= ne) break;
+            f += A.contrib[k * NC + cc];
+          }
+        }
+      }
+    }
+    xw[3] = xd;
+    p0w[3] = p0;
+    p1w[3] = p1;
+    fw[3] = f;
+    U0w[2] = U0;
+    U1w[2] = U1;
+    U3w[2] = U3;
+    // gradient of node row r-1
+    const int rg = r - 1;
+    double g0 = 0., g1 = 0.;
+    if (o2 && rg >= j0 - 1 && rg >= 0 && rg <= N) {
+      if (rg >= 1 && rg <= N - 1) {
+        const double dxf = xw[3] - xw[2], dxb = xw[2] - xw[1];
+        const double gf0 = (p0w[3] - p0w[2]) / dxf, gb0 = (p0w[2] - p0w[1]) / dxb;
+        const double gf1 = (p1w[3] - p1w[2]) / dxf, gb1 = (p1w[2] - p1w[1]) / dxb;
+        g0 = limited_grad<LIM, false>(gb0, gf0);
+        g1 = limited_grad<LIM, false>(gb1, gf1);
+        if (!inner_t && rg >= j0 && rg < j1) {
+          const long q = i + (long)nnx * rg;
+          A.dg0[q] = g0;
+          A.dg1[q] = g1;
+        }
+      } else {
+        const long q = i + (long)nnx * rg;
+        g0 = A.dg0[q];
+        g1 = A.dg1[q];
+      }
+    }
+    g0w[2] = g0;
+    g1w[2] = g1;
+    // finish node row o = r - 2
+    const int o = r - 2;
+    if (o < j0 || o >= j1) continue;
+    const long nid = i + (long)nnx * o;
+    double V0, V1, V3;
+    if (FIRST) {
+      const double mn = node_mass_from_cells(P, A.m, i, o);
+      V0 = mn * p0w[1];
+      V1 = mn * p1w[1];
+      V3 = mn;
+    } else {
+      V0 = U0w[0];
+      V1 = U1w[0];
+      V3 = U3w[0];
+    }
+    if (o >= 1 && o <= N - 1) {
+      const double back = fw[0], front = fw[1];
+      V3 += back - front;
+      const double ufront = 0.5 * (p1w[1] + p1w[1]);  // DualPhaseRemap.cc:391
+      const int signfront = (ufront > 0) ? 1 : -1;
+      const double xfr = xw[2] - xw[1];
+      const bool sf = (front < 0.);
+      const double tf = signfront * xfr - dt * ufront;
+      const double FV0 = (sf ? p0w[2] : p0w[1]) + order2 * (0.5 * (sf ? g0w[2] : g0w[1]) * tf);
+      const double FV1 = (sf ? p1w[2] : p1w[1]) + order2 * (0.5 * (sf ? g1w[2] : g1w[1]) * tf);
+      const double uback = 0.5 * (p1w[0] + p1w[1]);
+      const int signback = (uback > 0) ? 1 : -1;
+      const double xbk = xw[1] - xw[0];
+      const bool sb = (back > 0.);
+      const double tb = signback * xbk - dt * uback;
+      const double BV0 = (sb ? p0w[0] : p0w[1]) + order2 * (0.5 * (sb ? g0w[0] : g0w[1]) * tb);
+      const double BV1 = (sb ? p1w[0] : p1w[1]) + order2 * (0.5 * (sb ? g1w[0] : g1w[1]) * tb);
+      V0 += back * BV0 - front * FV0;
+      V1 += back * BV1 - front * FV1;
+      const double thresold = 1.e-300;
+      if (fabs(V0) < thresold) V0 = 0.;
+      if (fabs(V1) < thresold) V1 = 0.;
+    }
+    if (FIRST) {
+      A.Ud0[nid] = V0;
+      A.Ud1[nid] = V1;
+      A.Ud3[nid] = V3;
+    } else if (V3 != 0.) {
+      A.uox[nid] = V0 / V3;
+      A.uoy[nid] = V1 / V3;
+    } else {
+      A.uox[nid] = A.ux[nid];
+      A.uoy[nid] = A.uy[nid];
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
@@ -941,12 +1329,13 @@ void launch_cell_lagrange_2d(hipStream_t st, const Params& P, DevScalars* S, con
 struct YVariant {
   int strip, vg, r;
 };
-static const YVariant kYVariants[] = {{32, 4, 2}, {8, 4, 2}, {16, 4, 4}, {8, 5, 2}};
+static const YVariant kYVariants[] = {{32, 4, 2}, {8, 4, 2}, {16, 4, 4}, {8, 5, 2}, {0, 0, 0}};
 int cell_sweep_y_variants() { return (int)(sizeof(kYVariants) / sizeof(kYVariants[0])); }
 const char* cell_sweep_y_variant_name(int v) {
   static char buf[16][32];
   const YVariant& y = kYVariants[v];
-  snprintf(buf[v], 32, "strip%d_vg%d_r%d", y.strip, y.vg, y.r);
+  if (y.strip == 0) snprintf(buf[v], 32, "tile16x64");
+  else snprintf(buf[v], 32, "strip%d_vg%d_r%d", y.strip, y.vg, y.r);
   return buf[v];
 }
 
@@ -998,21 +1387,100 @@ void launch_cell_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const 
     case 0: launch_y<32, 4, 2>(st, P, S, A, first); break;
     case 1: launch_y<8, 4, 2>(st, P, S, A, first); break;
     case 2: launch_y<16, 4, 4>(st, P, S, A, first); break;
-    default: launch_y<8, 5, 2>(st, P, S, A, first); break;
+    case 3: launch_y<8, 5, 2>(st, P, S, A, first); break;
+    default: {
+      dim3 g((unsigned)(((P.nx + kTileW - 1) / kTileW) * ((P.ny + 59) / 60))), b(1024);
+#define YT_(N, LM)                                                                              \
+  if (first) hipLaunchKernelGGL((k_cell_sweep_ytile_2d<N, true, LM>), g, b, 0, st, P, S, A);    \
+  else hipLaunchKernelGGL((k_cell_sweep_ytile_2d<N, false, LM>), g, b, 0, st, P, S, A)
+#define YTL_(LM)                        \
+  do {                                  \
+    if (P.ne <= 1) YT_(1, LM);          \
+    else if (P.ne <= 2) YT_(2, LM);     \
+    else if (P.ne <= 3) YT_(3, LM);     \
+    else if (P.ne <= 4) YT_(4, LM);     \
+    else YT_(8, LM);                    \
+  } while (0)
+      MHG_LIM_DISPATCH(P.lim, YTL_);
+#undef YTL_
+#undef YT_
+    } break;
   }
 }
 
+// Launch variants of the dual sweeps: 0 = one thread per node re-forming the
+// 5-node stencil; x: 1 = lane exchange; y: 1, 2, 3 = column walkers of 8, 16
+// and 32 rows.
+int node_sweep_variants(int dir) { return dir == 0 ? 2 : 4; }
+const char* node_sweep_variant_name(int dir, int v) {
+  static const char* xn[] = {"stencil5", "lanes64"};
+  static const char* yn[] = {"stencil5", "walk8", "walk16", "walk32"};
+  if (v < 0 || v >= node_sweep_variants(dir)) return "?";
+  return dir == 0 ? xn[v] : yn[v];
+}
+
+template <int STRIP, int NEM, int LM>
+static void launch_node_y_walk(hipStream_t st, const Params& P, DevScalars* S, const NodeSweepArgs& A,
+                               bool first) {
+  const int ncb = (P.nx + 1 + 63) / 64;
+  const int nst = (P.ny + 1 + STRIP - 1) / STRIP;
+  dim3 g((unsigned)(ncb * nst)), b(64);
+  if (first) hipLaunchKernelGGL((k_node_sweep_y_2d<true, NEM, LM, STRIP>), g, b, 0, st, P, S, A);
+  else hipLaunchKernelGGL((k_node_sweep_y_2d<false, NEM, LM, STRIP>), g, b, 0, st, P, S, A);
+}
+
 void launch_node_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const NodeSweepArgs& A,
-                          int dir, bool first) {
-  const int B = 256;
-  dim3 g(nblocks(P.nn, B)), b(B);
+                          int dir, bool first, int variant) {
+  if (variant <= 0 || variant >= node_sweep_variants(dir)) {
+    const int B = 256;
+    dim3 g(nblocks(P.nn, B)), b(B);
 #define NL_(LM)                                                                                       \
   if (dir == 0 && first) hipLaunchKernelGGL((k_node_sweep_2d<0, true, LM>), g, b, 0, st, P, S, A);      \
   else if (dir == 0) hipLaunchKernelGGL((k_node_sweep_2d<0, false, LM>), g, b, 0, st, P, S, A);        \
   else if (first) hipLaunchKernelGGL((k_node_sweep_2d<1, true, LM>), g, b, 0, st, P, S, A);            \
   else hipLaunchKernelGGL((k_node_sweep_2d<1, false, LM>), g, b, 0, st, P, S, A)
-  MHG_LIM_DISPATCH(P.lim, NL_);
+    MHG_LIM_DISPATCH(P.lim, NL_);
 #undef NL_
+    return;
+  }
+  if (dir == 0) {
+    const long waves = (long)((P.nx + 1 + 59) / 60) * (P.ny + 1);
+    dim3 g(nblocks(waves * 64, 256)), b(256);
+#define NX_(N, LM)                                                                            \
+  if (first) hipLaunchKernelGGL((k_node_sweep_x_2d<true, N, LM>), g, b, 0, st, P, S, A); \
+  else hipLaunchKernelGGL((k_node_sweep_x_2d<false, N, LM>), g, b, 0, st, P, S, A)
+#define NXL_(LM)                 \
+  do {                           \
+    if (P.ne <= 1) NX_(1, LM);   \
+    else if (P.ne <= 2) NX_(2, LM); \
+    else if (P.ne <= 3) NX_(3, LM); \
+    else if (P.ne <= 4) NX_(4, LM); \
+    else NX_(8, LM);             \
+  } while (0)
+    MHG_LIM_DISPATCH(P.lim, NXL_);
+#undef NXL_
+#undef NX_
+    return;
+  }
+#define NY_(STR, N, LM) launch_node_y_walk<STR, N, LM>(st, P, S, A, first)
+#define NYN_(STR, LM)                  \
+  do {                                 \
+    if (P.ne <= 1) NY_(STR, 1, LM);      \
+    else if (P.ne <= 2) NY_(STR, 2, LM); \
+    else if (P.ne <= 3) NY_(STR, 3, LM); \
+    else if (P.ne <= 4) NY_(STR, 4, LM); \
+    else NY_(STR, 8, LM);                \
+  } while (0)
+#define NYL_(LM)                              \
+  do {                                        \
+    if (variant == 1) NYN_(8, LM);            \
+    else if (variant == 2) NYN_(16, LM);      \
+    else NYN_(32, LM);                        \
+  } while (0)
+  MHG_LIM_DISPATCH(P.lim, NYL_);
+#undef NYL_
+#undef NYN_
+#undef NY_
 }
 
 int cell_final_blocks(const Params& P) { return (int)nblocks(P.nc, 256); }

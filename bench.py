@@ -137,7 +137,8 @@ def main():
         "config": {"workload": "triple point 1792x768, 3 perfect-gas materials, plain FP64 "
                                "Lagrange + Euler-ADI remap (order 2, minmod), 500 steps",
                    "cells": nc, "nodes": nn, "steps": args.steps, "parallelism": "none (1 GPU)",
-                   "tuning": tuning.split("->")[-1].strip() if tuning else None},
+                   "tuning": {part.split(":")[0].strip(): part.split("->")[-1].strip()
+                              for part in tuning.split(";") if "->" in part} if tuning else None},
         "l1_density_err_vs_oracle": (None if l1 is None else
                                      {"value": l1, "after_steps": args.cpu_steps}),
         "step_roofline_frac": round(step_frac, 4),

@@ -206,8 +206,8 @@ int mhg_synchronize(mhg_ctx* ctx);
  * (reps launches each) and keeps the fastest; the simulation state is not
  * changed.  report (may be NULL) receives the per-variant timings. */
 int mhg_autotune(mhg_ctx* ctx, int32_t reps, char* report, int32_t report_len);
-/* Number of launch variants of a tunable kernel ("cell_sweep_y"), 0 for an
- * unknown name. */
+/* Number of launch variants of a tunable kernel ("cell_sweep_y",
+ * "node_sweep_x", "node_sweep_y"), 0 for an unknown name. */
 int32_t mhg_variant_count(const char* kernel);
 /* Selects launch variant `variant` of a tunable kernel for later steps
  * (MHG_E_ARG for an unknown kernel or an index out of range). */

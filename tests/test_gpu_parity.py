@@ -118,19 +118,22 @@ def test_two_material_sod_one_step(gpu):
     _one_step_parity(gpu, bisod, 30, 40, 4)
 
 
-@pytest.mark.parametrize("variant", range(8))
-def test_cell_sweep_y_variants_one_step(gpu, variant):
-    """Every launch variant of the y cell sweep on a mesh that no strip, row
-    chunk or 60-cell segment divides (70 x 134)."""
+@pytest.mark.parametrize("kernel", ["cell_sweep_y", "node_sweep_x", "node_sweep_y"])
+@pytest.mark.parametrize("variant", range(6))
+def test_launch_variants_one_step(gpu, kernel, variant):
+    """Every launch variant of the tunable kernels, on a mesh that no strip,
+    row chunk or 60-cell segment divides (70 x 134), from a state where both
+    sweep orders have run (odd and even warm-up)."""
     sim = H.tp_setup(gpu, 70, 134)
-    n = sim.variant_count()
+    n = sim.variant_count(kernel)
     if variant >= n:
-        pytest.skip(f"{n} variants")
+        pytest.skip(f"{kernel}: {n} variants")
 
     def tp_variant(L, nx, ny):
         s = H.tp_setup(L, nx, ny)
         if L is gpu:
-            s.select_variant(variant)
+            s.select_variant(variant, kernel)
         return s
-    tp_variant.__name__ = f"tp_variant{variant}"
-    _one_step_parity(gpu, tp_variant, 20, 70, 134)
+    tp_variant.__name__ = f"{kernel}_v{variant}"
+    for warm in (20, 21):
+        _one_step_parity(gpu, tp_variant, warm, 70, 134)
