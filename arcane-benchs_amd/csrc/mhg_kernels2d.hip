@@ -1128,6 +1128,158 @@ __global__ __launch_bounds__(256) void k_node_sweep_x_2d(Params P, DevScalars* S
   }
 }
 
+// Dual sweep along y on LDS tiles: a block of 16 wavefronts owns 16 node
+// columns and 64 node rows (60 outputs, two halo rows on each side).  The
+// row-major pass loads the nodes (coalesced) and forms phi_dual and the front
+// flux; the tile is transposed through LDS, each wavefront runs the lane
+// arithmetic of the x dual sweep on one column, and the increments of the
+// dual state travel back through LDS to be added and stored row-major.
+template <bool FIRST, int NEM, int LIM>
+__global__ __launch_bounds__(1024) void k_node_sweep_ytile_2d(Params P, DevScalars* S, NodeSweepArgs A) {
+  __shared__ double lds[4][kTileR][kTileP];
+  if (S->stop || S->err) return;  // uniform over the grid
+  const int nx = P.nx, N = P.ny, nnx = nx + 1;
+  const int ncolblk = (nnx + kTileW - 1) / kTileW;
+  const int L = xcd_block(blockIdx.x, gridDim.x);
+  const int i0 = (L % ncolblk) * kTileW;
+  const int s0 = (L / ncolblk) * 60 - 2;  // node row of tile row 0
+  const int t = threadIdx.x;
+  const int tr = t >> 4, tc = t & 15;
+  const int w = t >> 6, lane = t & 63;
+  const long NC = P.nc;
+  const int ne = P.ne;
+  const double dt = S->dt;
+  // ---- row-major pass
+  const int gi = i0 + tc, gs = s0 + tr;
+  const bool gex = gi <= nx && gs >= 0 && gs <= N;
+  const bool gout = gex && tr >= 2 && tr <= 61;
+  const long gq = gex ? gi + (long)nnx * gs : 0;
+  double U0 = 0., U1 = 0., U3 = 0.;
+  {
+    double xd = 0., p0 = 0., p1 = 0., f = 0.;
+    if (gex) {
+      const bool inner_t = (gi >= 1 && gi <= nx - 1);
+      xd = A.xny[gq];
+      if (FIRST) {
+        const double mn = node_mass_from_cells(P, A.m, gi, gs);
+        p0 = A.ux[gq];
+        p1 = A.uy[gq];
+        U0 = mn * p0;
+        U1 = mn * p1;
+        U3 = mn;
+      } else {
+        U0 = A.Ud0[gq];
+        U1 = A.Ud1[gq];
+        U3 = A.Ud3[gq];
+        if (inner_t) {  // DualPhaseRemap.cc:494-523
+          p0 = U0 / U3;
+          p1 = U1 / U3;
+        } else {
+          p0 = A.ux[gq];
+          p1 = A.uy[gq];
+        }
+      }
+      if (gs < N) {  // front flux: cells (i-1, s), (i, s), environments in order
+#pragma unroll
+        for (int sh = -1; sh <= 0; ++sh) {
+          const int ci = gi + sh;
+          if (ci < 0 || ci >= nx) continue;
+          const long cc = ci + (long)nx * gs;
+#pragma unroll
+          for (int k = 0; k < NEM; ++k) {
+            if (k >= ne) break;
+            f += A.contrib[k * NC + cc];
+          }
+        }
+      }
+    }
+    lds[0][tr][tc] = xd;
+    lds[1][tr][tc] = p0;
+    lds[2][tr][tc] = p1;
+    lds[3][tr][tc] = f;
+  }
+  __syncthreads();
+  // ---- transposed pass: column i, node row s
+  const int i = i0 + w, s = s0 + lane;
+  const bool ex = i <= nx && s >= 0 && s <= N;
+  const bool out = ex && lane >= 2 && lane <= 61;
+  const bool inner_t = (i >= 1 && i <= nx - 1);
+  const bool inner_d = (s >= 1 && s <= N - 1);
+  const double xd = lds[0][lane][w], p0 = lds[1][lane][w], p1 = lds[2][lane][w];
+  const double front = lds[3][lane][w];
+  const double xdm = __shfl_up(xd, 1, 64), xdp = __shfl_down(xd, 1, 64);
+  const double p0m = __shfl_up(p0, 1, 64), p0p = __shfl_down(p0, 1, 64);
+  const double p1m = __shfl_up(p1, 1, 64), p1p = __shfl_down(p1, 1, 64);
+  const double back = __shfl_up(front, 1, 64);
+  double g0 = 0., g1 = 0.;
+  if (P.order2 > 0 && ex) {
+    const long q = i + (long)nnx * s;
+    if (inner_d) {
+      const double dxf = xdp - xd, dxb = xd - xdm;
+      const double gf0 = (p0p - p0) / dxf, gb0 = (p0 - p0m) / dxb;
+      const double gf1 = (p1p - p1) / dxf, gb1 = (p1 - p1m) / dxb;
+      g0 = limited_grad<LIM, false>(gb0, gf0);
+      g1 = limited_grad<LIM, false>(gb1, gf1);
+      if (!inner_t && out) {  // kept for the sweep in the other direction
+        A.dg0[q] = g0;
+        A.dg1[q] = g1;
+      }
+    } else {  // boundary node along y: gradient left by an earlier sweep
+      g0 = A.dg0[q];
+      g1 = A.dg1[q];
+    }
+  }
+  const double g0m = __shfl_up(g0, 1, 64), g0p = __shfl_down(g0, 1, 64);
+  const double g1m = __shfl_up(g1, 1, 64), g1p = __shfl_down(g1, 1, 64);
+  double d0 = 0., d1 = 0., d3 = 0.;
+  if (out && inner_d) {
+    const double order2 = (double)P.order2;
+    d3 = back - front;
+    const double ufront = 0.5 * (p1 + p1);  // DualPhaseRemap.cc:391
+    const int signfront = (ufront > 0) ? 1 : -1;
+    const double xfr = xdp - xd;
+    const bool sf = (front < 0.);
+    const double tf = signfront * xfr - dt * ufront;
+    const double FV0 = (sf ? p0p : p0) + order2 * (0.5 * (sf ? g0p : g0) * tf);
+    const double FV1 = (sf ? p1p : p1) + order2 * (0.5 * (sf ? g1p : g1) * tf);
+    const double uback = 0.5 * (p1m + p1);
+    const int signback = (uback > 0) ? 1 : -1;
+    const double xbk = xd - xdm;
+    const bool sb = (back > 0.);
+    const double tb = signback * xbk - dt * uback;
+    const double BV0 = (sb ? p0m : p0) + order2 * (0.5 * (sb ? g0m : g0) * tb);
+    const double BV1 = (sb ? p1m : p1) + order2 * (0.5 * (sb ? g1m : g1) * tb);
+    d0 = back * BV0 - front * FV0;
+    d1 = back * BV1 - front * FV1;
+  }
+  __syncthreads();  // transposed reads done before the tiles are reused
+  lds[0][lane][w] = d0;
+  lds[1][lane][w] = d1;
+  lds[2][lane][w] = d3;
+  __syncthreads();
+  // ---- row-major: U += increments, store
+  if (!gout) return;
+  if (gs >= 1 && gs <= N - 1) {
+    U3 += lds[2][tr][tc];
+    U0 += lds[0][tr][tc];
+    U1 += lds[1][tr][tc];
+    const double thresold = 1.e-300;
+    if (fabs(U0) < thresold) U0 = 0.;
+    if (fabs(U1) < thresold) U1 = 0.;
+  }
+  if (FIRST) {
+    A.Ud0[gq] = U0;
+    A.Ud1[gq] = U1;
+    A.Ud3[gq] = U3;
+  } else if (U3 != 0.) {
+    A.uox[gq] = U0 / U3;
+    A.uoy[gq] = U1 / U3;
+  } else {
+    A.uox[gq] = A.ux[gq];
+    A.uoy[gq] = A.uy[gq];
+  }
+}
+
 // Dual sweep along y: each lane owns one node column and walks a strip of
 // STRIP node rows (two halo rows above and below).  Row r's phi_dual,
 // coordinate and front flux enter a four-row register window, its gradient
@@ -1410,11 +1562,11 @@ void launch_cell_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const 
 
 // Launch variants of the dual sweeps: 0 = one thread per node re-forming the
 // 5-node stencil; x: 1 = lane exchange; y: 1, 2, 3 = column walkers of 8, 16
-// and 32 rows.
-int node_sweep_variants(int dir) { return dir == 0 ? 2 : 4; }
+// and 32 rows, 4 = LDS tiles.
+int node_sweep_variants(int dir) { return dir == 0 ? 2 : 5; }
 const char* node_sweep_variant_name(int dir, int v) {
   static const char* xn[] = {"stencil5", "lanes64"};
-  static const char* yn[] = {"stencil5", "walk8", "walk16", "walk32"};
+  static const char* yn[] = {"stencil5", "walk8", "walk16", "walk32", "tile16x64"};
   if (v < 0 || v >= node_sweep_variants(dir)) return "?";
   return dir == 0 ? xn[v] : yn[v];
 }
@@ -1460,6 +1612,24 @@ void launch_node_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const 
     MHG_LIM_DISPATCH(P.lim, NXL_);
 #undef NXL_
 #undef NX_
+    return;
+  }
+  if (variant == 4) {
+    dim3 g((unsigned)(((P.nx + 1 + kTileW - 1) / kTileW) * ((P.ny + 1 + 59) / 60))), b(1024);
+#define NT_(N, LM)                                                                              \
+  if (first) hipLaunchKernelGGL((k_node_sweep_ytile_2d<true, N, LM>), g, b, 0, st, P, S, A);    \
+  else hipLaunchKernelGGL((k_node_sweep_ytile_2d<false, N, LM>), g, b, 0, st, P, S, A)
+#define NTL_(LM)                        \
+  do {                                  \
+    if (P.ne <= 1) NT_(1, LM);          \
+    else if (P.ne <= 2) NT_(2, LM);     \
+    else if (P.ne <= 3) NT_(3, LM);     \
+    else if (P.ne <= 4) NT_(4, LM);     \
+    else NT_(8, LM);                    \
+  } while (0)
+    MHG_LIM_DISPATCH(P.lim, NTL_);
+#undef NTL_
+#undef NT_
     return;
   }
 #define NY_(STR, N, LM) launch_node_y_walk<STR, N, LM>(st, P, S, A, first)
