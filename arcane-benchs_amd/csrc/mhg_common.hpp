@@ -195,6 +195,14 @@ struct BcRange {
   double value;
 };
 
+// Store of a value that this kernel writes once and a later kernel reads:
+// non-temporal, so that it streams out instead of staying dirty in the L2 of
+// the writing XCD (each kernel ends with a write-back of that L2).
+template <class T>
+__device__ inline void st_stream(T* p, T v) {
+  __builtin_nontemporal_store(v, p);
+}
+
 // XCD-aware block order.  The hardware hands block b to XCD b % 8; this maps
 // it to logical block (b % 8) * (n / 8) + b / 8 so that every XCD walks a
 // contiguous band of logical blocks and neighbouring tiles share its L2.

@@ -65,6 +65,14 @@ struct mhg_ctx {
   int nxvariant = 1;  // launch variant of the x dual sweep
   int nyvariant = 1;  // launch variant of the y dual sweep
 
+  // captured groups of fused iterations, one per host buffer state
+  struct StepGraph {
+    int xcur, ucur, sens;
+    hipGraphExec_t exec;
+  };
+  std::vector<StepGraph> graphs;
+  int use_graphs = 1;
+
   // kernel timing
   bool timing = false;
   std::vector<std::string> tnames;
@@ -307,16 +315,16 @@ void remap_2d(mhg_ctx* c) {
   F.block_min = c->block_min;
   {
     TimedLaunch t(c, "cell_final");
-    launch_cell_final_2d(c->stream, P, c->S, F);
+    launch_cell_final_2d(c->stream, P, c->S, F, true);
   }
   c->sens = (c->sens + 1) % P.dim;
   c->xcur = 1 - c->xcur;
   c->ucur = 1 - c->ucur;
 }
 
-// One fused iteration, enqueued on the context stream.
+// One fused iteration, enqueued on the context stream.  The time and iteration
+// advance of the iteration start is applied by the final dt kernel.
 void enqueue_step(mhg_ctx* c) {
-  launch_begin_iteration(c->stream, c->S);
   {
     TimedLaunch t(c, "node_lagrange");
     launch_node_lagrange_2d(c->stream, c->P, c->S, node_lag_args(c), true, true, true);
@@ -326,6 +334,44 @@ void enqueue_step(mhg_ctx* c) {
     launch_cell_lagrange_2d(c->stream, c->P, c->S, cell_lag_args(c));
   }
   remap_2d(c);
+}
+
+// Steps per captured graph: a multiple of the period of the host buffer state
+// (xcur and ucur alternate, sens cycles through dim), so a replay leaves that
+// state where it found it.
+constexpr int kGraphSteps = 12;
+
+void drop_graphs(mhg_ctx* c) {
+  for (auto& g : c->graphs) hipGraphExecDestroy(g.exec);
+  c->graphs.clear();
+}
+
+// Enqueues n fused iterations.  Whole groups of kGraphSteps replay a graph
+// captured once per buffer state (one launch for the group's kernels); the
+// rest, and every step while kernel timing is on, launch directly.
+int enqueue_steps(mhg_ctx* c, int n) {
+  while (c->use_graphs && !c->timing && n >= kGraphSteps) {
+    hipGraphExec_t exec = nullptr;
+    for (auto& g : c->graphs)
+      if (g.xcur == c->xcur && g.ucur == c->ucur && g.sens == c->sens) exec = g.exec;
+    if (!exec) {
+      mhg_ctx::StepGraph g{c->xcur, c->ucur, c->sens, nullptr};
+      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      for (int s = 0; s < kGraphSteps; ++s) enqueue_step(c);
+      hipGraph_t graph = nullptr;
+      HIPCHK(c, hipStreamEndCapture(c->stream, &graph));
+      const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+      hipGraphDestroy(graph);
+      HIPCHK(c, e);
+      c->graphs.push_back(g);
+      exec = g.exec;
+    }
+    HIPCHK(c, hipGraphLaunch(exec, c->stream));
+    n -= kGraphSteps;
+  }
+  for (int s = 0; s < n; ++s) enqueue_step(c);
+  HIPCHK(c, hipGetLastError());
+  return MHG_OK;
 }
 
 int check_ready(mhg_ctx* c) {
@@ -552,6 +598,7 @@ int mhg_create(const mhg_mesh_desc* mesh, const mhg_options* opt, mhg_ctx** out)
 int mhg_destroy(mhg_ctx* c) {
   if (!c) return MHG_OK;
   if (c->stream) hipStreamSynchronize(c->stream);
+  drop_graphs(c);
   for (auto& b : c->owned) hipFree(b.p);
   for (auto e : c->evpool) hipEventDestroy(e);
   if (c->S_host) hipHostFree(c->S_host);
@@ -791,9 +838,7 @@ int mhg_step(mhg_ctx* c, int32_t* stop) {
 int mhg_run_async(mhg_ctx* c, int32_t n) {
   int r = check_ready(c);
   if (r) return r;
-  for (int s = 0; s < n; ++s) enqueue_step(c);
-  HIPCHK(c, hipGetLastError());
-  return MHG_OK;
+  return enqueue_steps(c, n);
 }
 
 int mhg_synchronize(mhg_ctx* c) {
@@ -811,10 +856,10 @@ int mhg_run(mhg_ctx* c, int32_t max_steps, int32_t* steps_done) {
   const int it0 = c->S_host->iteration;
   int launched = 0;
   while (launched < max_steps) {
-    const int chunk = std::min(64, max_steps - launched);
-    for (int s = 0; s < chunk; ++s) enqueue_step(c);
+    const int chunk = std::min(5 * kGraphSteps, max_steps - launched);
+    r = enqueue_steps(c, chunk);
+    if (r) return r;
     launched += chunk;
-    HIPCHK(c, hipGetLastError());
     r = read_scalars(c);
     if (r) return r;
     collect_times(c);
@@ -876,10 +921,10 @@ int mhg_conservation(mhg_ctx* c, double out[3]) {
 // between steps, and the boundary dual gradients, which are saved before and
 // restored after; the simulation state is unchanged.
 namespace {
-const char* kTunable[] = {"cell_sweep_y", "node_sweep_x", "node_sweep_y"};
+const char* kTunable[] = {"cell_sweep_y", "node_sweep_x", "node_sweep_y", "step_graph"};
 int tunable_index(const char* kernel) {
   if (!kernel) return -1;
-  for (int k = 0; k < 3; ++k)
+  for (int k = 0; k < 4; ++k)
     if (std::strcmp(kernel, kTunable[k]) == 0) return k;
   return -1;
 }
@@ -887,12 +932,15 @@ int variant_count_of(int k) {
   if (k == 0) return cell_sweep_y_variants();
   if (k == 1) return node_sweep_variants(0);
   if (k == 2) return node_sweep_variants(1);
+  if (k == 3) return 2;
   return 0;
 }
 const char* variant_name_of(int k, int v) {
+  if (k == 3) return v ? "graph12" : "direct";
   return k == 0 ? cell_sweep_y_variant_name(v) : node_sweep_variant_name(k - 1, v);
 }
 int* variant_slot(mhg_ctx* c, int k) {
+  if (k == 3) return &c->use_graphs;
   return k == 0 ? &c->yvariant : (k == 1 ? &c->nxvariant : &c->nyvariant);
 }
 }  // namespace
@@ -960,6 +1008,7 @@ int mhg_autotune(mhg_ctx* c, int32_t reps, char* report, int32_t report_len) {
       }
     }
     *variant_slot(c, k) = best;
+    drop_graphs(c);
     rep += std::string(" -> ") + variant_name_of(k, best) + ";";
   }
   HIPCHK(c, hipMemcpyAsync(c->dg0, saved, gbytes, hipMemcpyDeviceToDevice, c->stream));
@@ -983,6 +1032,7 @@ int mhg_select_variant(mhg_ctx* c, const char* kernel, int32_t variant) {
   if (k < 0 || variant < 0 || variant >= variant_count_of(k))
     return fail(c, MHG_E_ARG, "select_variant: unknown kernel or variant");
   *variant_slot(c, k) = variant;
+  drop_graphs(c);  // captured graphs hold the previous launch variants
   return MHG_OK;
 }
 

@@ -54,6 +54,9 @@ void launch_node_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const 
                           int dir, bool first, int variant);
 int node_sweep_variants(int dir);
 const char* node_sweep_variant_name(int dir, int v);
-void launch_cell_final_2d(hipStream_t st, const Params& P, DevScalars* S, const CellFinalArgs& A);
+// advance: the final dt kernel also applies time += dt, iteration += 1 of the
+// iteration (fused step without a separate begin-iteration launch)
+void launch_cell_final_2d(hipStream_t st, const Params& P, DevScalars* S, const CellFinalArgs& A,
+                          bool advance);
 
 }  // namespace mhg

@@ -103,12 +103,12 @@ __global__ void k_node_lagrange_2d(Params P, DevScalars* S, NodeLagArgs A) {
     }
   }
   if (VEL || BC) {
-    A.ux[nid] = vx;
-    A.uy[nid] = vy;
+    st_stream(&A.ux[nid], vx);
+    st_stream(&A.uy[nid], vy);
   }
   if (POS) {
-    A.xnx[nid] = node_x0(P, i) + dt * vx;
-    A.xny[nid] = node_y0(P, j) + dt * vy;
+    st_stream(&A.xnx[nid], node_x0(P, i) + dt * vx);
+    st_stream(&A.xny[nid], node_y0(P, j) + dt * vy);
   }
 }
 
@@ -196,13 +196,13 @@ __global__ __launch_bounds__(256) void k_cell_lagrange_2d(Params P, DevScalars* 
 #pragma unroll
   for (int k = 0; k < NEM; ++k) {
     if (k >= ne) break;
-    U[k * NC + c] = uv[k];
-    U[(ne + k) * NC + c] = um[k];
-    U[(2 * ne + k) * NC + c] = ue[k];
+    st_stream(&U[k * NC + c], uv[k]);
+    st_stream(&U[(ne + k) * NC + c], um[k]);
+    st_stream(&U[(2 * ne + k) * NC + c], ue[k]);
   }
-  U[(3 * ne) * NC + c] = uq;
-  A.Vlag[c] = V;
-  A.L[c] = L;
+  st_stream(&U[(3 * ne) * NC + c], uq);
+  st_stream(&A.Vlag[c], V);
+  st_stream(&A.L[c], L);
 }
 
 // ---------------------------------------------------------------------------
@@ -311,7 +311,7 @@ __global__ __launch_bounds__(D == 0 ? 256 : 1024) void k_cell_sweep_lane_2d(Para
     if (k >= ne) break;
     double un, du;
     sweep_var(k, un, du);
-    if (out) Uout[k * NC + c] = un;
+    if (out) st_stream(&Uout[k * NC + c], un);
   }
 #pragma unroll
   for (int k = 0; k < NEM; ++k) {
@@ -320,8 +320,8 @@ __global__ __launch_bounds__(D == 0 ? 256 : 1024) void k_cell_sweep_lane_2d(Para
     sweep_var(ne + k, un, du);
     if (un < 0.) un = 0.;  // RemapADIService.cc:953-955
     if (out) {
-      Uout[(ne + k) * NC + c] = un;
-      A.contrib[k * NC + c] = 0.5 * du;  // fcontrib (DualPhaseRemap.cc:168-185)
+      st_stream(&Uout[(ne + k) * NC + c], un);
+      st_stream(&A.contrib[k * NC + c], 0.5 * du);  // fcontrib (DualPhaseRemap.cc:168-185)
     }
   }
 #pragma unroll
@@ -330,12 +330,12 @@ __global__ __launch_bounds__(D == 0 ? 256 : 1024) void k_cell_sweep_lane_2d(Para
     double un, du;
     sweep_var(2 * ne + k, un, du);
     if (un < 0.) un = 0.;  // RemapADIService.cc:956-958
-    if (out) Uout[(2 * ne + k) * NC + c] = un;
+    if (out) st_stream(&Uout[(2 * ne + k) * NC + c], un);
   }
   {
     double un, du;
     sweep_var(3 * ne, un, du);
-    if (out) Uout[(3 * ne) * NC + c] = un;
+    if (out) st_stream(&Uout[(3 * ne) * NC + c], un);
   }
 }
 
@@ -448,8 +448,8 @@ __global__ __launch_bounds__(1024) void k_cell_sweep_ytile_2d(Params P, DevScala
     }
     __syncthreads();
     if (gout) {
-      A.Uout[v * NC + gc] = stu[tr][tc];
-      if (mass) A.contrib[(v - ne) * NC + gc] = stc[tr][tc];
+      st_stream(&A.Uout[v * NC + gc], stu[tr][tc]);
+      if (mass) st_stream(&A.contrib[(v - ne) * NC + gc], stc[tr][tc]);
     }
   }
 }
@@ -556,10 +556,10 @@ __global__ __launch_bounds__(64) void k_cell_sweep_y_2d(Params P, DevScalars* S,
           // cell faces in order: bottom (back), right, top (front), left
           double un = U2[s] - (flux_b + flux_f);
           if (v >= ne && v < 3 * ne && un < 0.) un = 0.;
-          Uout[v * NC + co] = un;
+          st_stream(&Uout[v * NC + co], un);
           if (v >= ne && v < 2 * ne) {
             const double dual = 0.5 * flux_b * -1.0 + 0.5 * flux_f * 1.0;
-            A.contrib[(v - ne) * NC + co] = 0.5 * dual;
+            st_stream(&A.contrib[(v - ne) * NC + co], 0.5 * dual);
           }
         }
         ph2[s] = ph1[s];
@@ -765,13 +765,13 @@ __global__ __launch_bounds__(BLOCK) void k_cell_final_2d(Params P, DevScalars* S
         c_g = smax(ck[k], c_g);
       }
     }
-    A.rho[c] = rho_g;
-    A.e[c] = e_g;
-    A.p[c] = p_g;
-    A.cs[c] = c_g;
-    A.q[c] = q_g;
-    A.m[c] = mass_g;
-    A.mask[c] = mask;
+    st_stream(&A.rho[c], rho_g);
+    st_stream(&A.e[c], e_g);
+    st_stream(&A.p[c], p_g);
+    st_stream(&A.cs[c], c_g);
+    st_stream(&A.q[c], q_g);
+    st_stream(&A.m[c], mass_g);
+    st_stream(&A.mask[c], mask);
     if (nbe >= 2) {
 #pragma unroll
       for (int k = 0; k < NEM; ++k) {
@@ -814,8 +814,10 @@ __global__ __launch_bounds__(BLOCK) void k_cell_final_2d(Params P, DevScalars* S
 
 // computeDeltaT (MahycoModule.cc:2076-2191) on the block minima of
 // k_cell_final_2d: dt = min(cfl * min_c dx_sound, 1.05 dt_old, deltat-max),
-// deltat-min check, stop test with the time of this iteration.
-template <int BLOCK>
+// deltat-min check, stop test with the time of this iteration.  ADVANCE
+// first applies this iteration's time += dt, iteration += 1 (the work of
+// k_begin_iteration, which no kernel of the step between them reads).
+template <int BLOCK, bool ADVANCE>
 __global__ __launch_bounds__(BLOCK) void k_dt_finalize(Params P, DevScalars* S, const double* block_min, int nb) {
   if (S->stop || S->err) return;
   __shared__ double red[BLOCK / 64];
@@ -828,6 +830,10 @@ __global__ __launch_bounds__(BLOCK) void k_dt_finalize(Params P, DevScalars* S, 
   double minimum_aux = red[0];
   for (int w = 1; w < BLOCK / 64; ++w) minimum_aux = smin(minimum_aux, red[w]);
   const double old = S->dt;
+  if (ADVANCE) {
+    S->time += old;
+    S->iteration += 1;
+  }
   S->dt_old = old;
   double new_dt = P.cfl * minimum_aux;
   new_dt = smin(new_dt, 1.05 * old);
@@ -990,16 +996,16 @@ __global__ void k_node_sweep_2d(Params P, DevScalars* S, NodeSweepArgs A) {
     if (fabs(U1) < thresold) U1 = 0.;
   }
   if (FIRST) {
-    A.Ud0[nid] = U0;
-    A.Ud1[nid] = U1;
-    A.Ud3[nid] = U3;
+    st_stream(&A.Ud0[nid], U0);
+    st_stream(&A.Ud1[nid], U1);
+    st_stream(&A.Ud3[nid], U3);
   } else {
     if (U3 != 0.) {
-      A.uox[nid] = U0 / U3;
-      A.uoy[nid] = U1 / U3;
+      st_stream(&A.uox[nid], U0 / U3);
+      st_stream(&A.uoy[nid], U1 / U3);
     } else {
-      A.uox[nid] = A.ux[nid];
-      A.uoy[nid] = A.uy[nid];
+      st_stream(&A.uox[nid], A.ux[nid]);
+      st_stream(&A.uoy[nid], A.uy[nid]);
     }
   }
 }
@@ -1116,15 +1122,15 @@ __global__ __launch_bounds__(256) void k_node_sweep_x_2d(Params P, DevScalars* S
     if (fabs(U1) < thresold) U1 = 0.;
   }
   if (FIRST) {
-    A.Ud0[nid] = U0;
-    A.Ud1[nid] = U1;
-    A.Ud3[nid] = U3;
+    st_stream(&A.Ud0[nid], U0);
+    st_stream(&A.Ud1[nid], U1);
+    st_stream(&A.Ud3[nid], U3);
   } else if (U3 != 0.) {
-    A.uox[nid] = U0 / U3;
-    A.uoy[nid] = U1 / U3;
+    st_stream(&A.uox[nid], U0 / U3);
+    st_stream(&A.uoy[nid], U1 / U3);
   } else {
-    A.uox[nid] = A.ux[nid];
-    A.uoy[nid] = A.uy[nid];
+    st_stream(&A.uox[nid], A.ux[nid]);
+    st_stream(&A.uoy[nid], A.uy[nid]);
   }
 }
 
@@ -1268,15 +1274,15 @@ __global__ __launch_bounds__(1024) void k_node_sweep_ytile_2d(Params P, DevScala
     if (fabs(U1) < thresold) U1 = 0.;
   }
   if (FIRST) {
-    A.Ud0[gq] = U0;
-    A.Ud1[gq] = U1;
-    A.Ud3[gq] = U3;
+    st_stream(&A.Ud0[gq], U0);
+    st_stream(&A.Ud1[gq], U1);
+    st_stream(&A.Ud3[gq], U3);
   } else if (U3 != 0.) {
-    A.uox[gq] = U0 / U3;
-    A.uoy[gq] = U1 / U3;
+    st_stream(&A.uox[gq], U0 / U3);
+    st_stream(&A.uoy[gq], U1 / U3);
   } else {
-    A.uox[gq] = A.ux[gq];
-    A.uoy[gq] = A.uy[gq];
+    st_stream(&A.uox[gq], A.ux[gq]);
+    st_stream(&A.uoy[gq], A.uy[gq]);
   }
 }
 
@@ -1425,15 +1431,15 @@ __global__ __launch_bounds__(64) void k_node_sweep_y_2d(Params P, DevScalars* S,
       if (fabs(V1) < thresold) V1 = 0.;
     }
     if (FIRST) {
-      A.Ud0[nid] = V0;
-      A.Ud1[nid] = V1;
-      A.Ud3[nid] = V3;
+      st_stream(&A.Ud0[nid], V0);
+      st_stream(&A.Ud1[nid], V1);
+      st_stream(&A.Ud3[nid], V3);
     } else if (V3 != 0.) {
-      A.uox[nid] = V0 / V3;
-      A.uoy[nid] = V1 / V3;
+      st_stream(&A.uox[nid], V0 / V3);
+      st_stream(&A.uoy[nid], V1 / V3);
     } else {
-      A.uox[nid] = A.ux[nid];
-      A.uoy[nid] = A.uy[nid];
+      st_stream(&A.uox[nid], A.ux[nid]);
+      st_stream(&A.uoy[nid], A.uy[nid]);
     }
   }
 }
@@ -1655,13 +1661,15 @@ void launch_node_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const 
 
 int cell_final_blocks(const Params& P) { return (int)nblocks(P.nc, 256); }
 
-void launch_cell_final_2d(hipStream_t st, const Params& P, DevScalars* S, const CellFinalArgs& A) {
+void launch_cell_final_2d(hipStream_t st, const Params& P, DevScalars* S, const CellFinalArgs& A,
+                          bool advance) {
   constexpr int B = 256;
   dim3 g(nblocks(P.nc, B)), b(B);
 #define F_(N) hipLaunchKernelGGL((k_cell_final_2d<B, N>), g, b, 0, st, P, S, A)
   MHG_NEM_DISPATCH(P.ne, F_);
 #undef F_
-  hipLaunchKernelGGL((k_dt_finalize<1024>), dim3(1), dim3(1024), 0, st, P, S, A.block_min, (int)g.x);
+  if (advance) hipLaunchKernelGGL((k_dt_finalize<1024, true>), dim3(1), dim3(1024), 0, st, P, S, A.block_min, (int)g.x);
+  else hipLaunchKernelGGL((k_dt_finalize<1024, false>), dim3(1), dim3(1024), 0, st, P, S, A.block_min, (int)g.x);
 }
 
 }  // namespace mhg
