@@ -1,0 +1,111 @@
+"""Summaries of rocprofv3 output for the committed profiles.
+
+`stats`: per-kernel durations from a `--kernel-trace --stats` run.
+`traffic`: HBM bytes per dispatch from two `--pmc` runs (FETCH_SIZE, then
+WRITE_SIZE, KiB per dispatch).  Each counter is scaled by what it reads for a
+device-to-device copy of known size recorded in the same run (the largest
+`__amd_rocclr_copyBuffer` dispatch, `bench.py --copy-calibration MIB`).
+
+    python -m mahyco_amd.profiling stats <dir>/tp_kernel_stats.csv
+    python -m mahyco_amd.profiling traffic <fetch csv> <write csv> MIB [label=prefix ...]
+
+The optional label=prefix arguments name the launch variant that the step
+used for a kernel label (the autotuner's choice); the per-label table averages
+the dispatches of that variant.
+"""
+import collections
+import csv
+import sys
+
+# step kernel label (as timed by the library) -> kernel name prefixes
+STEP_KERNELS = {
+    "node_lagrange": ["k_node_lagrange_2d"],
+    "cell_lagrange": ["k_cell_lagrange_2d"],
+    "cell_sweep_x": ["k_cell_sweep_lane_2d<0"],
+    "cell_sweep_y": ["k_cell_sweep_ytile_2d", "k_cell_sweep_y_2d"],
+    "node_sweep_x": ["k_node_sweep_x_2d", "k_node_sweep_2d<0"],
+    "node_sweep_y": ["k_node_sweep_ytile_2d", "k_node_sweep_y_2d", "k_node_sweep_2d<1"],
+    "cell_final": ["k_cell_final_2d"],
+    "dt": ["k_dt_finalize"],
+}
+
+
+def short(name):
+    return name.split("(")[0].replace("void ", "").replace("mhg::", "")
+
+
+def label_of(kernel):
+    for lab, prefixes in STEP_KERNELS.items():
+        if any(kernel.startswith(p) for p in prefixes):
+            return lab
+    return ""
+
+
+def kernel_stats(path):
+    rows = []
+    for r in csv.DictReader(open(path)):
+        rows.append((short(r["Name"]), int(r["Calls"]), float(r["AverageNs"]) / 1e3,
+                     float(r["MinNs"]) / 1e3, float(r["MaxNs"]) / 1e3, float(r["Percentage"])))
+    return rows
+
+
+def _counter(path, name):
+    per = collections.defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        if r["Counter_Name"] == name:
+            per[short(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+    return per
+
+
+def traffic(fetch_csv, write_csv, calib_mib):
+    """Per kernel: mean FETCH_SIZE and WRITE_SIZE (KiB per dispatch), the copy
+    calibration factors and the scaled bytes per dispatch."""
+    f = _counter(fetch_csv, "FETCH_SIZE")
+    w = _counter(write_csv, "WRITE_SIZE")
+    calib = calib_mib * 1024.0  # KiB moved each way by the calibration copy
+    ff = calib / max(f["__amd_rocclr_copyBuffer"])
+    wf = calib / max(w["__amd_rocclr_copyBuffer"])
+    out = {}
+    for k in sorted(set(f) & set(w)):
+        if k.startswith("__amd") or k.startswith("at::"):
+            continue
+        fk = sum(f[k]) / len(f[k])
+        wk = sum(w[k]) / len(w[k])
+        out[k] = dict(n=len(f[k]), fetch_kib=fk, write_kib=wk,
+                      bytes=(fk * ff + wk * wf) * 1024.0)
+    return out, ff, wf
+
+
+def main(argv):
+    if argv[0] == "stats":
+        print("| kernel | label | calls | avg us | min us | max us | % |")
+        print("|---|---|---|---|---|---|---|")
+        for k, n, a, lo, hi, pct in kernel_stats(argv[1]):
+            print(f"| `{k}` | {label_of(k)} | {n} | {a:.2f} | {lo:.2f} | {hi:.2f} | {pct:.2f} |")
+        return 0
+    if argv[0] == "traffic":
+        out, ff, wf = traffic(argv[1], argv[2], float(argv[3]))
+        print(f"Copy calibration: FETCH_SIZE x {ff:.4f}, WRITE_SIZE x {wf:.4f}\n")
+        print("| kernel | label | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM bytes |")
+        print("|---|---|---|---|---|---|")
+        for k, v in out.items():
+            print(f"| `{k}` | {label_of(k)} | {v['n']} | {v['fetch_kib']:.0f} | "
+                  f"{v['write_kib']:.0f} | {v['bytes']:.0f} |")
+        tuned = dict(a.split("=", 1) for a in argv[4:])
+        print("\nStep kernels (launch variants used by the step; mean over their dispatches):\n")
+        print("| label | HBM bytes per launch |")
+        print("|---|---|")
+        for lab, prefixes in STEP_KERNELS.items():
+            pre = [tuned[lab]] if lab in tuned else prefixes
+            ks = [k for k in out if any(k.startswith(p) for p in pre)]
+            if not ks:
+                continue
+            b = sum(out[k]["bytes"] for k in ks) / len(ks)
+            print(f"| {lab} | {b:.0f} |")
+        return 0
+    print(__doc__)
+    return 2
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))

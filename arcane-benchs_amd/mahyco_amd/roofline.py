@@ -46,3 +46,26 @@ def kernel_bytes(name, n_cells, n_nodes, n_env, word=8):
 def last_node_sweep_extra(n_nodes, word=8):
     """The last node sweep also carries the final-node words (velocity, mass)."""
     return 7 * n_nodes * word
+
+
+TRAFFIC_PROFILE = "triple_point_1792x768_fp64_hbm_traffic.md"
+
+
+def measured_traffic(label, profiles_dir):
+    """HBM bytes per launch of a step kernel on the 1792x768 triple point, as
+    measured with rocprofv3 FETCH_SIZE / WRITE_SIZE counters (copy-calibrated)
+    and recorded in the committed traffic profile; None if absent."""
+    import os
+    path = os.path.join(profiles_dir, TRAFFIC_PROFILE)
+    if not os.path.exists(path):
+        return None
+    in_table = False
+    for line in open(path):
+        if line.startswith("| label | HBM bytes per launch |"):
+            in_table = True
+            continue
+        if in_table and line.startswith("| "):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if cells[0] == label:
+                return float(cells[1])
+    return None

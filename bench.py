@@ -69,6 +69,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-events", action="store_true", help="do not record per-kernel events")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device-to-device with torch: the reference kernel "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counts to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {args.gpus})",
@@ -80,6 +83,13 @@ def main():
     except Exception:
         sync = lambda: None  # noqa: E731
 
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        sync()
+        del a, b
     gpu = load_gpu()
     sim = cases.triple_point(gpu, NX, NY)
     tuning = sim.autotune(10)
@@ -128,7 +138,8 @@ def main():
         ach = b / (avg_ms * 1e-3) / 1e9
         rl = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1),
               "peak": roofline.PEAK_HBM_GBS, "unit": "GB/s",
-              "frac": round(ach / roofline.PEAK_HBM_GBS, 4), "traffic": None,
+              "frac": round(ach / roofline.PEAK_HBM_GBS, 4),
+              "traffic": roofline.measured_traffic(dom, os.path.join(ROOT, "profiles")),
               "avg_ms": round(avg_ms, 5), "launches": launches,
               "bytes_per_launch": b}
     step_frac = value * roofline.bytes_per_zone_cycle(NENV) / (roofline.PEAK_HBM_GBS * 1e9)
