@@ -65,6 +65,15 @@ struct mhg_ctx {
   int nxvariant = 1;  // launch variant of the x dual sweep
   int nyvariant = 1;  // launch variant of the y dual sweep
 
+  // per-entry-point path: the _n, tau and div u arrays (allocated on the
+  // first entry-point call) and which coordinates are current between calls
+  double *rho_n = nullptr, *e_n = nullptr, *q_n = nullptr, *tau = nullptr, *divu = nullptr;
+  double *env_rho_n = nullptr, *env_e_n = nullptr, *env_q_n = nullptr, *env_tau = nullptr;
+  bool ep_ready = false;
+  int coord_phase = 0;  // 0: xx[xcur] (last Lagrange phase); 1: x0 (after
+                        // saveValuesAtN); 2: xx[1 - xcur] (after updatePosition)
+  bool ep_remapped = false;  // k_cell_final_2d ran since the last dt
+
   // captured groups of fused iterations, one per host buffer state
   struct StepGraph {
     int xcur, ucur, sens;
@@ -250,7 +259,7 @@ CellLagArgs cell_lag_args(mhg_ctx* c) {
 // Remap (IRemap::appliRemap, RemapADIService.cc:19-60): for i < dim,
 // idir = (i + SensProjection) % dim, cell sweep then dual sweep; then
 // SensProjection = (SensProjection + 1) % dim and remapVariables.
-void remap_2d(mhg_ctx* c) {
+void remap_2d(mhg_ctx* c, int dt_mode) {
   const Params& P = c->P;
   const double* xnx = c->xx[1 - c->xcur];
   const double* xny = c->xy[1 - c->xcur];
@@ -315,7 +324,7 @@ void remap_2d(mhg_ctx* c) {
   F.block_min = c->block_min;
   {
     TimedLaunch t(c, "cell_final");
-    launch_cell_final_2d(c->stream, P, c->S, F, true);
+    launch_cell_final_2d(c->stream, P, c->S, F, dt_mode);
   }
   c->sens = (c->sens + 1) % P.dim;
   c->xcur = 1 - c->xcur;
@@ -333,7 +342,7 @@ void enqueue_step(mhg_ctx* c) {
     TimedLaunch t(c, "cell_lagrange");
     launch_cell_lagrange_2d(c->stream, c->P, c->S, cell_lag_args(c));
   }
-  remap_2d(c);
+  remap_2d(c, 2);
 }
 
 // Steps per captured graph: a multiple of the period of the host buffer state
@@ -380,6 +389,15 @@ int check_ready(mhg_ctx* c) {
   return MHG_OK;
 }
 
+// The fused step starts from a state between iterations.
+int check_fused(mhg_ctx* c) {
+  int r = check_ready(c);
+  if (r) return r;
+  if (c->coord_phase != 0)
+    return fail(c, MHG_E_STATE, "fused step inside an entry-point iteration: finish it with mhg_remap");
+  return MHG_OK;
+}
+
 // Pure-cell initial state (hydroStartInit, MahycoModule.cc:48-188, then
 // computeCellMass 199-225): EOS init per environment, m = rho V.
 int init_pure(mhg_ctx* c, const std::vector<int>& env, const std::vector<double>& rho,
@@ -420,6 +438,8 @@ int init_pure(mhg_ctx* c, const std::vector<int>& env, const std::vector<double>
   }
   c->xcur = 0;
   c->ucur = 0;
+  c->coord_phase = 0;
+  c->ep_remapped = false;
   c->sens = 0;
   auto up = [&](double* d, const std::vector<double>& h) {
     return hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice);
@@ -674,6 +694,8 @@ int mhg_upload_state(mhg_ctx* c, const mhg_state* st) {
   const size_t cb = NC * sizeof(double), nb = NN * sizeof(double), eb = (size_t)P.ne * cb;
   c->xcur = 0;
   c->ucur = 0;
+  c->coord_phase = 0;
+  c->ep_remapped = false;
   auto h2d = [&](void* d, const void* h, size_t b) {
     return h ? hipMemcpy(d, h, b, hipMemcpyHostToDevice) : hipSuccess;
   };
@@ -768,8 +790,16 @@ int mhg_download_state(mhg_ctx* c, mhg_state* st) {
     HIPCHK(c, d2h(st->velocity + NN, c->uy[c->ucur], nb));
   }
   if (st->node_coord) {
-    HIPCHK(c, d2h(st->node_coord, c->xx[c->xcur], nb));
-    HIPCHK(c, d2h(st->node_coord + NN, c->xy[c->xcur], nb));
+    if (c->coord_phase == 1) {  // Euler reset of saveValuesAtN: x = x0
+      for (long n = 0; n < NN; ++n) {
+        st->node_coord[n] = x0_of(P, 0, (int)(n % (P.nx + 1)));
+        st->node_coord[NN + n] = x0_of(P, 1, (int)(n / (P.nx + 1)));
+      }
+    } else {
+      const int b = c->coord_phase == 2 ? 1 - c->xcur : c->xcur;
+      HIPCHK(c, d2h(st->node_coord, c->xx[b], nb));
+      HIPCHK(c, d2h(st->node_coord + NN, c->xy[b], nb));
+    }
   }
   if (st->dual_grad_phi) {
     HIPCHK(c, d2h(st->dual_grad_phi, c->dg0, nb));
@@ -801,20 +831,148 @@ int mhg_download_state(mhg_ctx* c, mhg_state* st) {
   return MHG_OK;
 }
 
-int mhg_begin_iteration(mhg_ctx* c) { return fail(c, MHG_E_UNSUPPORTED, "entry points: use mhg_step"); }
-int mhg_save_values_at_n(mhg_ctx* c) { return fail(c, MHG_E_UNSUPPORTED, "entry points: use mhg_step"); }
-int mhg_update_velocity(mhg_ctx* c) { return fail(c, MHG_E_UNSUPPORTED, "entry points: use mhg_step"); }
-int mhg_apply_boundary_condition(mhg_ctx* c) { return fail(c, MHG_E_UNSUPPORTED, "entry points: use mhg_step"); }
-int mhg_update_position(mhg_ctx* c) { return fail(c, MHG_E_UNSUPPORTED, "entry points: use mhg_step"); }
-int mhg_compute_geometric_values(mhg_ctx* c) { return fail(c, MHG_E_UNSUPPORTED, "entry points: use mhg_step"); }
-int mhg_update_density(mhg_ctx* c) { return fail(c, MHG_E_UNSUPPORTED, "entry points: use mhg_step"); }
-int mhg_compute_artificial_viscosity(mhg_ctx* c) { return fail(c, MHG_E_UNSUPPORTED, "entry points: use mhg_step"); }
-int mhg_update_energy_and_pressure(mhg_ctx* c) { return fail(c, MHG_E_UNSUPPORTED, "entry points: use mhg_step"); }
-int mhg_remap(mhg_ctx* c) { return fail(c, MHG_E_UNSUPPORTED, "entry points: use mhg_step"); }
+// ---- per-entry-point path (Mahyco.config:30-41) ---------------------------
+// Each call enqueues the kernels of one reference entry point on the context
+// stream and waits for them, so the downloaded state between calls is the
+// state the reference holds between its entry points.
+namespace {
+
+int ep_prepare(mhg_ctx* c) {
+  int r = check_ready(c);
+  if (r) return r;
+  if (c->ep_ready) return MHG_OK;
+  const size_t NC = c->P.nc, envn = (size_t)c->P.ne * NC;
+  r |= dalloc(c, &c->rho_n, NC) | dalloc(c, &c->e_n, NC) | dalloc(c, &c->q_n, NC);
+  r |= dalloc(c, &c->tau, NC) | dalloc(c, &c->divu, NC);
+  r |= dalloc(c, &c->env_rho_n, envn) | dalloc(c, &c->env_e_n, envn) | dalloc(c, &c->env_q_n, envn);
+  r |= dalloc(c, &c->env_tau, envn);
+  if (r) return fail(c, MHG_E_ALLOC, "entry-point buffers: hipMalloc failed");
+  c->ep_ready = true;
+  return MHG_OK;
+}
+
+StageArgs stage_args(mhg_ctx* c) {
+  StageArgs A;
+  A.xnx = c->xx[1 - c->xcur];
+  A.xny = c->xy[1 - c->xcur];
+  A.rho = c->rho;
+  A.e = c->e;
+  A.p = c->p;
+  A.cs = c->cs;
+  A.q = c->q;
+  A.m = c->m;
+  A.L = c->L;
+  A.Vlag = c->Vlag;
+  A.mask = c->mask;
+  A.env_f = c->env_f;
+  A.env_mf = c->env_mf;
+  A.env_m = c->env_m;
+  A.env_p = c->env_p;
+  A.env_rho = c->env_rho;
+  A.env_e = c->env_e;
+  A.env_q = c->env_q;
+  A.rho_n = c->rho_n;
+  A.e_n = c->e_n;
+  A.q_n = c->q_n;
+  A.tau = c->tau;
+  A.divu = c->divu;
+  A.env_rho_n = c->env_rho_n;
+  A.env_e_n = c->env_e_n;
+  A.env_q_n = c->env_q_n;
+  A.env_tau = c->env_tau;
+  A.U = c->U[0];
+  return A;
+}
+
+int ep_finish(mhg_ctx* c) {
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MHG_OK;
+}
+
+int ep_stage(mhg_ctx* c, int stage) {
+  int r = ep_prepare(c);
+  if (r) return r;
+  launch_cell_stage_2d(c->stream, c->P, c->S, stage_args(c), stage);
+  return ep_finish(c);
+}
+
+int ep_node(mhg_ctx* c, bool vel, bool bc, bool pos) {
+  int r = ep_prepare(c);
+  if (r) return r;
+  launch_node_lagrange_2d(c->stream, c->P, c->S, node_lag_args(c), vel, bc, pos);
+  return ep_finish(c);
+}
+
+}  // namespace
+
+int mhg_begin_iteration(mhg_ctx* c) {
+  int r = ep_prepare(c);
+  if (r) return r;
+  launch_begin_iteration(c->stream, c->S);
+  return ep_finish(c);
+}
+
+// saveValuesAtN: rho_n, e_n, q_n <- rho, e, q; the velocity is updated in
+// place and the coordinates of the last Lagrange phase stay in their buffer
+// (the lagged corner normals are formed from them), so the Euler reset
+// x <- x0 only changes which coordinates are current.
+int mhg_save_values_at_n(mhg_ctx* c) {
+  int r = ep_stage(c, STAGE_SAVE);
+  if (r) return r;
+  c->coord_phase = 1;
+  return MHG_OK;
+}
+
+int mhg_update_velocity(mhg_ctx* c) { return ep_node(c, true, false, false); }
+int mhg_apply_boundary_condition(mhg_ctx* c) { return ep_node(c, false, true, false); }
+
+int mhg_update_position(mhg_ctx* c) {
+  if (c && c->coord_phase != 1) return fail(c, MHG_E_STATE, "update_position: call save_values_at_n first");
+  int r = ep_node(c, false, false, true);
+  if (r) return r;
+  c->coord_phase = 2;
+  return MHG_OK;
+}
+
+int mhg_compute_geometric_values(mhg_ctx* c) {
+  if (c && c->coord_phase != 2) return fail(c, MHG_E_STATE, "compute_geometric_values: call update_position first");
+  return ep_stage(c, STAGE_GEOMETRY);
+}
+int mhg_update_density(mhg_ctx* c) { return ep_stage(c, STAGE_DENSITY); }
+int mhg_compute_artificial_viscosity(mhg_ctx* c) { return ep_stage(c, STAGE_VISCOSITY); }
+int mhg_update_energy_and_pressure(mhg_ctx* c) { return ep_stage(c, STAGE_ENERGY); }
+
+// remap (PrepareRemap.cc:448-563): U from the state, the sweeps, the final
+// cell phase with the EOS and the pressure average; dt is left to
+// mhg_compute_delta_t.
+int mhg_remap(mhg_ctx* c) {
+  if (c && c->coord_phase != 2) return fail(c, MHG_E_STATE, "remap: the Lagrange phase of this iteration has not run");
+  int r = ep_prepare(c);
+  if (r) return r;
+  launch_cell_stage_2d(c->stream, c->P, c->S, stage_args(c), STAGE_BUILD_U);
+  remap_2d(c, 0);
+  r = ep_finish(c);
+  if (r) return r;
+  c->coord_phase = 0;
+  c->ep_remapped = true;
+  return MHG_OK;
+}
+
 int mhg_compute_delta_t(mhg_ctx* c, double* dt, int32_t* stop) {
-  (void)dt;
-  (void)stop;
-  return fail(c, MHG_E_UNSUPPORTED, "entry points: use mhg_step");
+  int r = ep_prepare(c);
+  if (r) return r;
+  if (!c->ep_remapped) return fail(c, MHG_E_STATE, "compute_delta_t: call remap first");
+  launch_dt_finalize(c->stream, c->P, c->S, c->block_min, false);
+  r = ep_finish(c);
+  if (r) return r;
+  c->ep_remapped = false;
+  r = read_scalars(c);
+  if (r) return r;
+  if (dt) *dt = c->S_host->dt;
+  if (stop) *stop = c->S_host->stop;
+  if (c->S_host->err) return fail(c, MHG_E_DT_MIN, "dt below deltat-min");
+  return MHG_OK;
 }
 
 int mhg_remap_get_ordre_projection(const mhg_ctx* c) { return c ? c->opt.ordre_projection : MHG_E_ARG; }
@@ -823,7 +981,7 @@ int mhg_remap_has_conservation_energie_totale(const mhg_ctx* c) { return c ? c->
 int mhg_remap_is_euler(const mhg_ctx* c) { return c ? c->opt.is_euler_scheme : MHG_E_ARG; }
 
 int mhg_step(mhg_ctx* c, int32_t* stop) {
-  int r = check_ready(c);
+  int r = check_fused(c);
   if (r) return r;
   enqueue_step(c);
   HIPCHK(c, hipGetLastError());
@@ -836,7 +994,7 @@ int mhg_step(mhg_ctx* c, int32_t* stop) {
 }
 
 int mhg_run_async(mhg_ctx* c, int32_t n) {
-  int r = check_ready(c);
+  int r = check_fused(c);
   if (r) return r;
   return enqueue_steps(c, n);
 }
@@ -849,7 +1007,7 @@ int mhg_synchronize(mhg_ctx* c) {
 }
 
 int mhg_run(mhg_ctx* c, int32_t max_steps, int32_t* steps_done) {
-  int r = check_ready(c);
+  int r = check_fused(c);
   if (r) return r;
   r = read_scalars(c);
   if (r) return r;

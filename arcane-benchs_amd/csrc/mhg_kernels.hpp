@@ -41,6 +41,30 @@ struct CellFinalArgs {
   double* block_min;  // one CFL minimum per block of k_cell_final_2d
 };
 
+// Per-entry-point cell stages (the unfused path): each reads and writes the
+// state arrays as the reference entry point of the same name does; the _n,
+// tau and div u arrays exist only for this path.
+struct StageArgs {
+  const double *xnx, *xny;  // Lagrangian coordinates of this iteration
+  double *rho, *e, *p, *cs, *q, *m, *L, *Vlag;
+  const int* mask;
+  const double *env_f, *env_mf, *env_m, *env_p;
+  double *env_rho, *env_e, *env_q;
+  double *rho_n, *e_n, *q_n, *tau, *divu;
+  double *env_rho_n, *env_e_n, *env_q_n, *env_tau;
+  double* U;
+};
+enum CellStage {
+  STAGE_SAVE = 0,       // saveValuesAtN (cell part)
+  STAGE_GEOMETRY = 1,   // computeGeometricValues: V, L
+  STAGE_DENSITY = 2,    // updateDensity: rho, tau, div u
+  STAGE_VISCOSITY = 3,  // computeArtificialViscosity: q
+  STAGE_ENERGY = 4,     // updateEnergyAndPressure: e
+  STAGE_BUILD_U = 5     // computeVariablesForRemap (cells)
+};
+void launch_cell_stage_2d(hipStream_t st, const Params& P, DevScalars* S, const StageArgs& A, int stage);
+void launch_dt_finalize(hipStream_t st, const Params& P, DevScalars* S, const double* block_min, bool advance);
+
 int cell_final_blocks(const Params& P);
 void launch_begin_iteration(hipStream_t st, DevScalars* S);
 void launch_node_lagrange_2d(hipStream_t st, const Params& P, DevScalars* S, const NodeLagArgs& A,
@@ -54,9 +78,10 @@ void launch_node_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const 
                           int dir, bool first, int variant);
 int node_sweep_variants(int dir);
 const char* node_sweep_variant_name(int dir, int v);
-// advance: the final dt kernel also applies time += dt, iteration += 1 of the
+// dt_mode 0: no dt kernel (the caller computes dt later); 1: the final dt
+// kernel follows; 2: it also applies time += dt, iteration += 1 of the
 // iteration (fused step without a separate begin-iteration launch)
 void launch_cell_final_2d(hipStream_t st, const Params& P, DevScalars* S, const CellFinalArgs& A,
-                          bool advance);
+                          int dt_mode);
 
 }  // namespace mhg

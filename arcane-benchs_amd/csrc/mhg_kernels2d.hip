@@ -206,6 +206,149 @@ __global__ __launch_bounds__(256) void k_cell_lagrange_2d(Params P, DevScalars* 
 }
 
 // ---------------------------------------------------------------------------
+// Per-entry-point cell stages.  The same formulas as k_cell_lagrange_2d, with
+// every intermediate kept in the state arrays between calls:
+//   SAVE       saveValuesAtN cell part (MahycoModule.cc:308-420): rho_n, e_n,
+//              q_n <- rho, e, q (global and mixed environments)
+//   GEOMETRY   computeGeometricValues (1054-1299): V, L from the coordinates
+//   DENSITY    updateDensity (1331-1426): rho = m/V, tau, div u; mixed
+//              rho_k = m_k/(f_k V), tau_k
+//   VISCOSITY  computeArtificialViscosity (454-578): q, q_k, q = sum f_k q_k
+//   ENERGY     updateEnergyAndPressureforGP (1659-1770): e, e_k,
+//              e = sum mf_k e_k (no EOS: with-projection)
+//   BUILD_U    computeVariablesForRemap_PBorn0 (PrepareRemap.cc:258-399),
+//              live entries of U
+// ---------------------------------------------------------------------------
+template <int STAGE, bool PC, int NEM>
+__global__ __launch_bounds__(256) void k_cell_stage_2d(Params P, DevScalars* S, StageArgs A) {
+  if (S->stop || S->err) return;
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= P.nc) return;
+  const int ne = P.ne;
+  const long NC = P.nc;
+  const int mask = A.mask[c];
+  const int nbe = __popc(mask);
+  const int kk = __ffs(mask) - 1;
+  if (STAGE == STAGE_SAVE) {
+    A.rho_n[c] = A.rho[c];
+    A.e_n[c] = A.e[c];
+    A.q_n[c] = A.q[c];
+    if (nbe >= 2) {
+#pragma unroll
+      for (int k = 0; k < NEM; ++k) {
+        if (!((mask >> k) & 1)) continue;
+        const long ek = k * NC + c;
+        A.env_rho_n[ek] = A.env_rho[ek];
+        A.env_e_n[ek] = A.env_e[ek];
+        A.env_q_n[ek] = A.env_q[ek];
+      }
+    }
+  } else if (STAGE == STAGE_GEOMETRY) {
+    const int nnx = P.nx + 1;
+    const int i = (int)(c % P.nx), j = (int)(c / P.nx);
+    const long n0 = i + (long)nnx * j;
+    const long nd[4] = {n0, n0 + 1, n0 + 1 + nnx, n0 + nnx};
+    double x[4], y[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      x[q] = A.xnx[nd[q]];
+      y[q] = A.xny[nd[q]];
+    }
+    const double V = quad_volume(x, y);
+    A.Vlag[c] = V;
+    A.L[c] = (P.lc == MHG_LC_FACES_OPPOSEES) ? length_faces_opposees_2d(x, y) : pow(V, 0.5);
+  } else if (STAGE == STAGE_DENSITY) {
+    const double inv_dt = 1.0 / S->dt;
+    const double V = A.Vlag[c];
+    const double rho_n = A.rho_n[c];
+    const double rho = A.m[c] / V;
+    const double tau = 0.5 * (1.0 / rho_n + 1.0 / rho);
+    A.rho[c] = rho;
+    A.tau[c] = tau;
+    A.divu[c] = inv_dt * (1.0 / rho - 1.0 / rho_n) / tau;
+    if (nbe >= 2) {
+#pragma unroll
+      for (int k = 0; k < NEM; ++k) {
+        if (!((mask >> k) & 1)) continue;
+        const long ek = k * NC + c;
+        const double rk = A.env_m[ek] / (A.env_f[ek] * V);
+        A.env_rho[ek] = rk;
+        A.env_tau[ek] = 0.5 * (1.0 / A.env_rho_n[ek] + 1.0 / rk);
+      }
+    }
+  } else if (STAGE == STAGE_VISCOSITY) {
+    const double divu = A.divu[c], L = A.L[c], cs = A.cs[c];
+    double q = 0.;
+    if (nbe == 1 && divu < 0.0) q = viscosity(P.eos.qcoef[kk], A.tau[c], L, cs, divu);
+    if (nbe >= 2) {
+#pragma unroll
+      for (int k = 0; k < NEM; ++k) {
+        if (!((mask >> k) & 1)) continue;
+        const long ek = k * NC + c;
+        double qk = 0.;
+        if (divu < 0.0) qk = viscosity(P.eos.qcoef[k], A.env_tau[ek], L, cs, divu);
+        A.env_q[ek] = qk;
+        q += qk * A.env_f[ek];
+      }
+    }
+    A.q[c] = q;
+  } else if (STAGE == STAGE_ENERGY) {
+    double e = 0.;
+    if (nbe == 1)
+      e = compute_eint(PC, P.eos.gamma[kk], A.q_n[c], A.q[c], A.rho_n[c], A.rho[c], A.p[c], A.e_n[c]);
+    if (nbe >= 2) {
+#pragma unroll
+      for (int k = 0; k < NEM; ++k) {
+        if (!((mask >> k) & 1)) continue;
+        const long ek = k * NC + c;
+        const double ev = compute_eint(PC, P.eos.gamma[k], A.env_q_n[ek], A.env_q[ek], A.env_rho_n[ek],
+                                       A.env_rho[ek], A.env_p[ek], A.env_e_n[ek]);
+        A.env_e[ek] = ev;
+        e += A.env_mf[ek] * ev;
+      }
+    }
+    A.e[c] = e;
+  } else if (STAGE == STAGE_BUILD_U) {
+    const double V = A.Vlag[c];
+    double uv[NEM], um[NEM], ue[NEM];
+    double uq = 0.;
+#pragma unroll
+    for (int k = 0; k < NEM; ++k) uv[k] = um[k] = ue[k] = 0.;
+    if (nbe == 1) {
+      const double rho = A.rho[c];
+#pragma unroll
+      for (int k = 0; k < NEM; ++k)
+        if (k == kk) {
+          uv[k] = V;
+          um[k] = V * rho;
+          ue[k] = V * rho * A.e[c];
+        }
+      uq = V * A.q[c];
+    } else if (nbe >= 2) {
+#pragma unroll
+      for (int k = 0; k < NEM; ++k) {
+        if (!((mask >> k) & 1)) continue;
+        const long ek = k * NC + c;
+        const double Vk = A.env_f[ek] * V;
+        const double rk = A.env_rho[ek];
+        uv[k] = Vk;
+        um[k] = Vk * rk;
+        ue[k] = Vk * rk * A.env_e[ek];
+        uq = Vk * A.env_q[ek];  // PrepareRemap.cc:372 assigns: the last environment wins
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NEM; ++k) {
+      if (k >= ne) break;
+      A.U[k * NC + c] = uv[k];
+      A.U[(ne + k) * NC + c] = um[k];
+      A.U[(2 * ne + k) * NC + c] = ue[k];
+    }
+    A.U[(3 * ne) * NC + c] = uq;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Sweep arithmetic shared by both directions, for one variable of one cell:
 // computeGradPhiFace (RemapADIService.cc:171-177), the limited cell gradient
 // (497-506), the upwind face value (707-714) and the flux update (909-936).
@@ -1661,15 +1804,45 @@ void launch_node_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const 
 
 int cell_final_blocks(const Params& P) { return (int)nblocks(P.nc, 256); }
 
+void launch_cell_stage_2d(hipStream_t st, const Params& P, DevScalars* S, const StageArgs& A, int stage) {
+  dim3 g(nblocks(P.nc, 256)), b(256);
+#define CS_(STG, N)                                                                                   \
+  if (P.pseudo_centree) hipLaunchKernelGGL((k_cell_stage_2d<STG, true, N>), g, b, 0, st, P, S, A);   \
+  else hipLaunchKernelGGL((k_cell_stage_2d<STG, false, N>), g, b, 0, st, P, S, A)
+#define CSN_(STG)                      \
+  do {                                 \
+    if (P.ne <= 1) CS_(STG, 1);        \
+    else if (P.ne <= 2) CS_(STG, 2);   \
+    else if (P.ne <= 3) CS_(STG, 3);   \
+    else if (P.ne <= 4) CS_(STG, 4);   \
+    else CS_(STG, 8);                  \
+  } while (0)
+  switch (stage) {
+    case STAGE_SAVE: CSN_(STAGE_SAVE); break;
+    case STAGE_GEOMETRY: CSN_(STAGE_GEOMETRY); break;
+    case STAGE_DENSITY: CSN_(STAGE_DENSITY); break;
+    case STAGE_VISCOSITY: CSN_(STAGE_VISCOSITY); break;
+    case STAGE_ENERGY: CSN_(STAGE_ENERGY); break;
+    default: CSN_(STAGE_BUILD_U); break;
+  }
+#undef CSN_
+#undef CS_
+}
+
+void launch_dt_finalize(hipStream_t st, const Params& P, DevScalars* S, const double* block_min, bool advance) {
+  const int nb = cell_final_blocks(P);
+  if (advance) hipLaunchKernelGGL((k_dt_finalize<1024, true>), dim3(1), dim3(1024), 0, st, P, S, block_min, nb);
+  else hipLaunchKernelGGL((k_dt_finalize<1024, false>), dim3(1), dim3(1024), 0, st, P, S, block_min, nb);
+}
+
 void launch_cell_final_2d(hipStream_t st, const Params& P, DevScalars* S, const CellFinalArgs& A,
-                          bool advance) {
+                          int dt_mode) {
   constexpr int B = 256;
   dim3 g(nblocks(P.nc, B)), b(B);
 #define F_(N) hipLaunchKernelGGL((k_cell_final_2d<B, N>), g, b, 0, st, P, S, A)
   MHG_NEM_DISPATCH(P.ne, F_);
 #undef F_
-  if (advance) hipLaunchKernelGGL((k_dt_finalize<1024, true>), dim3(1), dim3(1024), 0, st, P, S, A.block_min, (int)g.x);
-  else hipLaunchKernelGGL((k_dt_finalize<1024, false>), dim3(1), dim3(1024), 0, st, P, S, A.block_min, (int)g.x);
+  if (dt_mode > 0) launch_dt_finalize(st, P, S, A.block_min, dt_mode == 2);
 }
 
 }  // namespace mhg

@@ -150,3 +150,55 @@ def test_graph_replay_matches_direct_launches(gpu):
     da, db = a.download(), b.download()
     for k in da:
         assert np.array_equal(np.asarray(da[k]), np.asarray(db[k])), k
+
+
+ENTRY_SEQUENCE = ["begin_iteration", "save_values_at_n", "update_velocity",
+                  "apply_boundary_condition", "update_position", "compute_geometric_values",
+                  "update_density", "compute_artificial_viscosity", "update_energy_and_pressure",
+                  "remap"]
+
+
+@pytest.mark.parametrize("case", ["sod", "tp", "tp_pc"])
+def test_entry_points_match_oracle(gpu, case):
+    """The per-entry-point C-ABI (Mahyco.config:30-41 order): after every entry
+    point of two iterations the downloaded GPU state equals the oracle's, then
+    a fused step continues from the entry-point state."""
+    if case == "sod":
+        setup, nx, ny, ne, warm, kw = H.sod_setup, 40, 4, 1, 5, {}
+    else:
+        setup, nx, ny, ne, warm, kw = H.tp_setup, 56, 24, 3, 30, {}
+        if case == "tp_pc":
+            kw = dict(pseudo_centree=1)
+    o = setup(H.oracle(), nx, ny, **kw)
+    o.run(warm)
+    g = setup(gpu, nx, ny, **kw)
+    g.upload(o.download())
+    for it in range(2):
+        for name in ENTRY_SEQUENCE:
+            o.entry(name)
+            g.entry(name)
+            d = H.compare_states(g.download(), o.download(), ne, nx, ny)
+            bad = {k: v for k, v in d.items() if v > TOL}
+            assert not bad, (it, name, bad)
+        dto, stop_o = o.compute_delta_t()
+        dtg, stop_g = g.compute_delta_t()
+        assert abs(dtg - dto) <= TOL * dto and stop_o == stop_g, (dtg, dto)
+    o.step()
+    g.step()
+    d = H.compare_states(g.download(), o.download(), ne, nx, ny)
+    bad = {k: v for k, v in d.items() if v > TOL}
+    assert not bad, ("fused step after entry points", bad)
+
+
+def test_entry_point_order_errors(gpu):
+    """Entry points called out of order fail with MHG_E_STATE instead of
+    computing on stale buffers."""
+    g = H.sod_setup(gpu, 40, 4)
+    with pytest.raises(H.abi.MahycoError):
+        g.entry("update_position")
+    with pytest.raises(H.abi.MahycoError):
+        g.compute_delta_t()
+    g.entry("begin_iteration")
+    g.entry("save_values_at_n")
+    with pytest.raises(H.abi.MahycoError):
+        g.step()
