@@ -202,3 +202,24 @@ def test_entry_point_order_errors(gpu):
     g.entry("save_values_at_n")
     with pytest.raises(H.abi.MahycoError):
         g.step()
+
+
+def test_fp64_bit_identity_fixture(gpu):
+    """The FP64 path reproduces, bit for bit, the state that this library
+    wrote for the same runs when the fixture was made
+    (tests/golden/make_gpu_fp64_fixture.py): operation order is part of the
+    contract of the drop-in path."""
+    import os
+    import importlib.util
+    here = os.path.dirname(os.path.abspath(__file__))
+    path = os.path.join(here, "golden", "gpu_fp64_state.npz")
+    if not os.path.exists(path):
+        pytest.skip("fixture not generated")
+    spec = importlib.util.spec_from_file_location(
+        "mkfix", os.path.join(here, "golden", "make_gpu_fp64_fixture.py"))
+    mk = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mk)
+    ref = np.load(path)
+    got = mk.run_all(gpu)
+    for k in ref.files:
+        assert np.array_equal(ref[k], got[k]), k
