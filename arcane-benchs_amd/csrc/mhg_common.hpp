@@ -16,16 +16,25 @@ constexpr int kMaxEnv = MHG_MAX_ENV;
 constexpr int kMaxBc = MHG_MAX_BC;
 
 // std::max / std::min semantics ((a<b)?b:a and (b<a)?b:a).
-__host__ __device__ inline double smax(double a, double b) { return (a < b) ? b : a; }
-__host__ __device__ inline double smin(double a, double b) { return (b < a) ? b : a; }
+template <class T>
+__host__ __device__ inline T smax(T a, T b) { return (a < b) ? b : a; }
+template <class T>
+__host__ __device__ inline T smin(T a, T b) { return (b < a) ? b : a; }
+
+// Quotient in the arithmetic type: IEEE division in double (the drop-in keeps
+// the reference's rounding bit for bit), the hardware-reciprocal quotient
+// (within 2 ulp) in the float precision form, whose contract is an error
+// bound, not bit equality.
+__host__ __device__ inline double qdiv(double a, double b) { return a / b; }
+__device__ inline float qdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
 // Limiters, RemapADIService.h:260-286.
-template <int LIM>
-__host__ __device__ inline double limiter(double r) {
-  if (LIM == 0) return smax(0.0, smin(1.0, r));                             // MinMod
-  if (LIM == 1) return smax(0.0, smax(smin(2.0 * r, 1.0), smin(r, 2.0)));  // SuperBee
-  if (LIM == 2) return (r <= 0.0 ? 0.0 : 2.0 * r / (1.0 + r));             // VanLeer
-  return 0.0;                                                              // DefaultO1
+template <int LIM, class T = double>
+__host__ __device__ inline T limiter(T r) {
+  if (LIM == 0) return smax(T(0), smin(T(1), r));                            // MinMod
+  if (LIM == 1) return smax(T(0), smax(smin(T(2) * r, T(1)), smin(r, T(2))));  // SuperBee
+  if (LIM == 2) return (r <= T(0) ? T(0) : qdiv(T(2) * r, T(1) + r));       // VanLeer
+  return T(0);                                                               // DefaultO1
 }
 __host__ __device__ inline double limiter_rt(int lim, double r) {
   switch (lim) {
@@ -60,52 +69,53 @@ __host__ __device__ inline double limited_node_grad(int lim, double gb, double g
 // leaves the accumulated gradient (started at +0) unchanged.
 // CELL selects the association of the cell form 0.5 * (lim * b) or the nodal
 // form (0.5 * lim) * b.
-template <bool CELL>
-__host__ __device__ inline bool minmod_term(double a, double b, double* term) {
-  const bool opp = (a > 0. && b < 0.) || (a < 0. && b > 0.) || (a == 0.);
+template <bool CELL, class T>
+__host__ __device__ inline bool minmod_term(T a, T b, T* term) {
+  const bool opp = (a > T(0) && b < T(0)) || (a < T(0) && b > T(0)) || (a == T(0));
   if (opp) return false;
-  const bool same = (a > 0. && b > 0.) || (a < 0. && b < 0.);
+  const bool same = (a > T(0) && b > T(0)) || (a < T(0) && b < T(0));
   if (same && fabs(a) >= fabs(b)) {
-    *term = CELL ? 0.5 * (1.0 * b) : 0.5 * 1.0 * b;
+    *term = CELL ? T(0.5) * (T(1) * b) : T(0.5) * T(1) * b;
     return true;
   }
-  const double l = smax(0.0, smin(1.0, a / b));
-  *term = CELL ? 0.5 * (l * b) : 0.5 * l * b;
+  const T l = smax(T(0), smin(T(1), qdiv(a, b)));
+  *term = CELL ? T(0.5) * (l * b) : T(0.5) * l * b;
   return true;
 }
 
 // Limited gradients with the limiter fixed at compile time (LIM 0 = minmod
 // uses minmod_term; others evaluate the quotient as the reference does).
-template <int LIM, bool CELL>
-__host__ __device__ inline double limited_grad(double gb, double gf) {
-  double g = 0.;
+template <int LIM, bool CELL, class T>
+__host__ __device__ inline T limited_grad(T gb, T gf) {
+  T g = T(0);
   if (LIM == 0) {
-    double t;
-    if (gb != 0. && minmod_term<CELL>(gf, gb, &t)) g += t;
-    if (gf != 0. && minmod_term<CELL>(gb, gf, &t)) g += t;
+    T t;
+    if (gb != T(0) && minmod_term<CELL>(gf, gb, &t)) g += t;
+    if (gf != T(0) && minmod_term<CELL>(gb, gf, &t)) g += t;
     return g;
   }
-  if (gb != 0.) g += CELL ? 0.5 * (limiter<LIM>(gf / gb) * gb) : 0.5 * limiter<LIM>(gf / gb) * gb;
-  if (gf != 0.) g += CELL ? 0.5 * (limiter<LIM>(gb / gf) * gf) : 0.5 * limiter<LIM>(gb / gf) * gf;
+  if (gb != T(0)) g += CELL ? T(0.5) * (limiter<LIM>(qdiv(gf, gb)) * gb) : T(0.5) * limiter<LIM>(qdiv(gf, gb)) * gb;
+  if (gf != T(0)) g += CELL ? T(0.5) * (limiter<LIM>(qdiv(gb, gf)) * gf) : T(0.5) * limiter<LIM>(qdiv(gb, gf)) * gf;
   return g;
 }
 
 // compute_eint, MahycoModule.cc:1630-1654.
-__host__ __device__ inline double compute_eint(bool pseudo_centree, double adiabatic_cst,
-                                               double q_n, double q, double rho_n, double rho,
-                                               double p, double e_n) {
-  double pseudo = 0.;
-  if (pseudo_centree && ((q_n + q) * (1.0 / rho - 1.0 / rho_n) < 0.)) pseudo = 0.5 * (q + q_n);
-  if (!pseudo_centree && (q * (1.0 / rho - 1.0 / rho_n) < 0.)) pseudo = q;
-  double denom = 1 + 0.5 * (adiabatic_cst - 1.0) * rho * (1.0 / rho - 1.0 / rho_n);
-  double numer = e_n - (0.5 * p + pseudo) * (1.0 / rho - 1.0 / rho_n);
-  return numer / denom;
+template <class T>
+__host__ __device__ inline T compute_eint(bool pseudo_centree, T adiabatic_cst, T q_n, T q, T rho_n,
+                                          T rho, T p, T e_n) {
+  const T dv = qdiv(T(1), rho) - qdiv(T(1), rho_n);  // 1/rho - 1/rho_n
+  T pseudo = T(0);
+  if (pseudo_centree && ((q_n + q) * dv < T(0))) pseudo = T(0.5) * (q + q_n);
+  if (!pseudo_centree && (q * dv < T(0))) pseudo = q;
+  T denom = T(1) + T(0.5) * (adiabatic_cst - T(1)) * rho * dv;
+  T numer = e_n - (T(0.5) * p + pseudo) * dv;
+  return qdiv(numer, denom);
 }
 
 // Pseudo-viscosity, MahycoModule.cc:523-526.
-__host__ __device__ inline double viscosity(double qcoef, double tau, double L, double c,
-                                            double divu) {
-  return 1. / tau * (-0.5 * L * c * divu + qcoef * L * L * divu * divu);
+template <class T>
+__host__ __device__ inline T viscosity(T qcoef, T tau, T L, T c, T divu) {
+  return qdiv(T(1), tau) * (T(-0.5) * L * c * divu + qcoef * L * L * divu * divu);
 }
 
 // Equation of state of one environment.
@@ -118,18 +128,18 @@ struct EosParams {
 
 // PerfectGas compute_pressure_sndspd_PG (PerfectGasEOSService.cc:38-45) and
 // StiffenedGas applyEOS (StiffenedGasEOSService.cc:35-42).
-__host__ __device__ inline void eos_apply(const EosParams& E, int k, double rho, double e,
-                                          double* p, double* c) {
-  const double g = E.gamma[k];
+template <class T>
+__host__ __device__ inline void eos_apply(const EosParams& E, int k, T rho, T e, T* p, T* c) {
+  const T g = (T)E.gamma[k];
   if (E.kind[k] == MHG_EOS_STIFFENED_GAS) {
-    const double t = E.tension[k];
-    double pressure = ((g - 1.) * rho * e) - (g * t);
+    const T t = (T)E.tension[k];
+    T pressure = ((g - T(1)) * rho * e) - (g * t);
     *p = pressure;
-    *c = sqrt((g / rho) * (pressure + t));
+    *c = sqrt(qdiv(g, rho) * (pressure + t));
   } else {
-    double pressure = (g - 1.) * rho * e;
+    T pressure = (g - T(1)) * rho * e;
     *p = pressure;
-    *c = sqrt(g * pressure / rho);
+    *c = sqrt(qdiv(g * pressure, rho));
   }
 }
 

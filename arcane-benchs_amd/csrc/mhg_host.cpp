@@ -38,21 +38,24 @@ struct mhg_ctx {
   int device = 0;
   std::vector<DBuf> owned;
 
+  // Bulk fields hold `bulk`-byte words: double for the drop-in, float for the
+  // precision form (mhg_options.precision).  Coordinates stay double.
+  int bulk = 8;
   // cells
-  double *rho = nullptr, *e = nullptr, *p = nullptr, *cs = nullptr, *q = nullptr, *m = nullptr;
-  double *L = nullptr, *Vlag = nullptr;
+  void *rho = nullptr, *e = nullptr, *p = nullptr, *cs = nullptr, *q = nullptr, *m = nullptr;
+  void *L = nullptr, *Vlag = nullptr;
   int* mask = nullptr;
-  double* U[2] = {nullptr, nullptr};
-  double* contrib = nullptr;
+  void* U[2] = {nullptr, nullptr};
+  void* contrib = nullptr;
   // environments [ne][nc]
-  double *env_rho = nullptr, *env_e = nullptr, *env_p = nullptr, *env_cs = nullptr;
-  double *env_q = nullptr, *env_m = nullptr, *env_f = nullptr, *env_mf = nullptr;
+  void *env_rho = nullptr, *env_e = nullptr, *env_p = nullptr, *env_cs = nullptr;
+  void *env_q = nullptr, *env_m = nullptr, *env_f = nullptr, *env_mf = nullptr;
   // nodes
-  double* ux[2] = {nullptr, nullptr};
-  double* uy[2] = {nullptr, nullptr};
+  void* ux[2] = {nullptr, nullptr};
+  void* uy[2] = {nullptr, nullptr};
   double* xx[2] = {nullptr, nullptr};
   double* xy[2] = {nullptr, nullptr};
-  double *Ud0 = nullptr, *Ud1 = nullptr, *Ud3 = nullptr, *dg0 = nullptr, *dg1 = nullptr;
+  void *Ud0 = nullptr, *Ud1 = nullptr, *Ud3 = nullptr, *dg0 = nullptr, *dg1 = nullptr;
   double* block_min = nullptr;
   DevScalars* S = nullptr;
   DevScalars* S_host = nullptr;  // pinned mirror
@@ -117,6 +120,41 @@ int dalloc(mhg_ctx* c, T** out, size_t count) {
   c->owned.push_back({p, bytes});
   *out = (T*)p;
   return MHG_OK;
+}
+
+// Bulk array of `count` words of the context's storage type.
+int balloc(mhg_ctx* c, void** out, size_t count) {
+  void* p = nullptr;
+  size_t bytes = std::max<size_t>(count, 1) * (size_t)c->bulk;
+  if (hipMalloc(&p, bytes) != hipSuccess) return fail(c, MHG_E_ALLOC, "hipMalloc failed");
+  if (hipMemset(p, 0, bytes) != hipSuccess) return fail(c, MHG_E_HIP, "hipMemset failed");
+  c->owned.push_back({p, bytes});
+  *out = p;
+  return MHG_OK;
+}
+
+template <class R>
+R* bp(void* p) {
+  return static_cast<R*>(p);
+}
+
+// Host doubles to a device bulk array (rounded to float in the precision form).
+hipError_t put(mhg_ctx* c, void* d, const double* h, size_t n) {
+  if (!h) return hipSuccess;
+  if (c->bulk == 8) return hipMemcpy(d, h, n * sizeof(double), hipMemcpyHostToDevice);
+  std::vector<float> t(n);
+  for (size_t i = 0; i < n; ++i) t[i] = (float)h[i];
+  return hipMemcpy(d, t.data(), n * sizeof(float), hipMemcpyHostToDevice);
+}
+
+// A device bulk array to host doubles.
+hipError_t get(mhg_ctx* c, double* h, const void* d, size_t n) {
+  if (!h) return hipSuccess;
+  if (c->bulk == 8) return hipMemcpy(h, d, n * sizeof(double), hipMemcpyDeviceToHost);
+  std::vector<float> t(n);
+  hipError_t e = hipMemcpy(t.data(), d, n * sizeof(float), hipMemcpyDeviceToHost);
+  for (size_t i = 0; i < n; ++i) h[i] = t[i];
+  return e;
 }
 
 double x0_of(const Params& P, int d, int i) { return P.org[d] + i * P.h[d]; }
@@ -219,108 +257,111 @@ void collect_times(mhg_ctx* c) {
 }
 
 // ---- argument blocks ---------------------------------------------------
-NodeLagArgs node_lag_args(mhg_ctx* c) {
-  NodeLagArgs A;
-  A.p = c->p;
-  A.q = c->q;
-  A.m = c->m;
+template <class R>
+NodeLagArgsT<R> node_lag_args(mhg_ctx* c) {
+  NodeLagArgsT<R> A;
+  A.p = bp<R>(c->p);
+  A.q = bp<R>(c->q);
+  A.m = bp<R>(c->m);
   A.xpx = c->xx[c->xcur];
   A.xpy = c->xy[c->xcur];
-  A.ux = c->ux[c->ucur];
-  A.uy = c->uy[c->ucur];
+  A.ux = bp<R>(c->ux[c->ucur]);
+  A.uy = bp<R>(c->uy[c->ucur]);
   A.xnx = c->xx[1 - c->xcur];
   A.xny = c->xy[1 - c->xcur];
   return A;
 }
 
-CellLagArgs cell_lag_args(mhg_ctx* c) {
-  CellLagArgs A;
+template <class R>
+CellLagArgsT<R> cell_lag_args(mhg_ctx* c) {
+  CellLagArgsT<R> A;
   A.xnx = c->xx[1 - c->xcur];
   A.xny = c->xy[1 - c->xcur];
-  A.rho = c->rho;
-  A.e = c->e;
-  A.p = c->p;
-  A.cs = c->cs;
-  A.q = c->q;
-  A.m = c->m;
+  A.rho = bp<R>(c->rho);
+  A.e = bp<R>(c->e);
+  A.p = bp<R>(c->p);
+  A.cs = bp<R>(c->cs);
+  A.q = bp<R>(c->q);
+  A.m = bp<R>(c->m);
   A.mask = c->mask;
-  A.env_f = c->env_f;
-  A.env_rho = c->env_rho;
-  A.env_m = c->env_m;
-  A.env_q = c->env_q;
-  A.env_p = c->env_p;
-  A.env_e = c->env_e;
-  A.U = c->U[0];
-  A.Vlag = c->Vlag;
-  A.L = c->L;
+  A.env_f = bp<R>(c->env_f);
+  A.env_rho = bp<R>(c->env_rho);
+  A.env_m = bp<R>(c->env_m);
+  A.env_q = bp<R>(c->env_q);
+  A.env_p = bp<R>(c->env_p);
+  A.env_e = bp<R>(c->env_e);
+  A.U = bp<R>(c->U[0]);
+  A.Vlag = bp<R>(c->Vlag);
+  A.L = bp<R>(c->L);
   return A;
 }
 
 // Remap (IRemap::appliRemap, RemapADIService.cc:19-60): for i < dim,
 // idir = (i + SensProjection) % dim, cell sweep then dual sweep; then
 // SensProjection = (SensProjection + 1) % dim and remapVariables.
+template <class R>
 void remap_2d(mhg_ctx* c, int dt_mode) {
   const Params& P = c->P;
   const double* xnx = c->xx[1 - c->xcur];
   const double* xny = c->xy[1 - c->xcur];
-  const double* ux = c->ux[c->ucur];
-  const double* uy = c->uy[c->ucur];
+  const R* ux = bp<R>(c->ux[c->ucur]);
+  const R* uy = bp<R>(c->uy[c->ucur]);
   for (int pass = 0; pass < P.dim; ++pass) {
     const int idir = (pass + c->sens) % P.dim;
     const bool first = (pass == 0);
-    CellSweepArgs A;
-    A.Uin = c->U[pass & 1];
-    A.Uout = c->U[(pass + 1) & 1];
-    A.Vlag = c->Vlag;
+    CellSweepArgsT<R> A;
+    A.Uin = bp<R>(c->U[pass & 1]);
+    A.Uout = bp<R>(c->U[(pass + 1) & 1]);
+    A.Vlag = bp<R>(c->Vlag);
     A.xnx = xnx;
     A.xny = xny;
     A.ux = ux;
     A.uy = uy;
-    A.contrib = c->contrib;
+    A.contrib = bp<R>(c->contrib);
     {
       TimedLaunch t(c, idir == 0 ? "cell_sweep_x" : "cell_sweep_y");
       launch_cell_sweep_2d(c->stream, P, c->S, A, idir, first, c->yvariant);
     }
-    NodeSweepArgs B;
-    B.m = c->m;
+    NodeSweepArgsT<R> B;
+    B.m = bp<R>(c->m);
     B.ux = ux;
     B.uy = uy;
     B.xnx = xnx;
     B.xny = xny;
-    B.contrib = c->contrib;
-    B.Ud0 = c->Ud0;
-    B.Ud1 = c->Ud1;
-    B.Ud3 = c->Ud3;
-    B.dg0 = c->dg0;
-    B.dg1 = c->dg1;
-    B.uox = c->ux[1 - c->ucur];
-    B.uoy = c->uy[1 - c->ucur];
+    B.contrib = bp<R>(c->contrib);
+    B.Ud0 = bp<R>(c->Ud0);
+    B.Ud1 = bp<R>(c->Ud1);
+    B.Ud3 = bp<R>(c->Ud3);
+    B.dg0 = bp<R>(c->dg0);
+    B.dg1 = bp<R>(c->dg1);
+    B.uox = bp<R>(c->ux[1 - c->ucur]);
+    B.uoy = bp<R>(c->uy[1 - c->ucur]);
     {
       TimedLaunch t(c, idir == 0 ? "node_sweep_x" : "node_sweep_y");
       launch_node_sweep_2d(c->stream, P, c->S, B, idir, first, idir == 0 ? c->nxvariant : c->nyvariant);
     }
   }
-  CellFinalArgs F;
-  F.U = c->U[P.dim & 1];
-  F.Vlag = c->Vlag;
-  F.L = c->L;
-  F.ux = c->ux[1 - c->ucur];
-  F.uy = c->uy[1 - c->ucur];
-  F.rho = c->rho;
-  F.e = c->e;
-  F.p = c->p;
-  F.cs = c->cs;
-  F.q = c->q;
-  F.m = c->m;
+  CellFinalArgsT<R> F;
+  F.U = bp<R>(c->U[P.dim & 1]);
+  F.Vlag = bp<R>(c->Vlag);
+  F.L = bp<R>(c->L);
+  F.ux = bp<R>(c->ux[1 - c->ucur]);
+  F.uy = bp<R>(c->uy[1 - c->ucur]);
+  F.rho = bp<R>(c->rho);
+  F.e = bp<R>(c->e);
+  F.p = bp<R>(c->p);
+  F.cs = bp<R>(c->cs);
+  F.q = bp<R>(c->q);
+  F.m = bp<R>(c->m);
   F.mask = c->mask;
-  F.env_rho = c->env_rho;
-  F.env_e = c->env_e;
-  F.env_p = c->env_p;
-  F.env_cs = c->env_cs;
-  F.env_q = c->env_q;
-  F.env_m = c->env_m;
-  F.env_f = c->env_f;
-  F.env_mf = c->env_mf;
+  F.env_rho = bp<R>(c->env_rho);
+  F.env_e = bp<R>(c->env_e);
+  F.env_p = bp<R>(c->env_p);
+  F.env_cs = bp<R>(c->env_cs);
+  F.env_q = bp<R>(c->env_q);
+  F.env_m = bp<R>(c->env_m);
+  F.env_f = bp<R>(c->env_f);
+  F.env_mf = bp<R>(c->env_mf);
   F.block_min = c->block_min;
   {
     TimedLaunch t(c, "cell_final");
@@ -333,16 +374,22 @@ void remap_2d(mhg_ctx* c, int dt_mode) {
 
 // One fused iteration, enqueued on the context stream.  The time and iteration
 // advance of the iteration start is applied by the final dt kernel.
-void enqueue_step(mhg_ctx* c) {
+template <class R>
+void enqueue_step_t(mhg_ctx* c) {
   {
     TimedLaunch t(c, "node_lagrange");
-    launch_node_lagrange_2d(c->stream, c->P, c->S, node_lag_args(c), true, true, true);
+    launch_node_lagrange_2d(c->stream, c->P, c->S, node_lag_args<R>(c), true, true, true);
   }
   {
     TimedLaunch t(c, "cell_lagrange");
-    launch_cell_lagrange_2d(c->stream, c->P, c->S, cell_lag_args(c));
+    launch_cell_lagrange_2d(c->stream, c->P, c->S, cell_lag_args<R>(c));
   }
-  remap_2d(c, 2);
+  remap_2d<R>(c, 2);
+}
+
+void enqueue_step(mhg_ctx* c) {
+  if (c->bulk == 4) enqueue_step_t<float>(c);
+  else enqueue_step_t<double>(c);
 }
 
 // Steps per captured graph: a multiple of the period of the host buffer state
@@ -441,9 +488,7 @@ int init_pure(mhg_ctx* c, const std::vector<int>& env, const std::vector<double>
   c->coord_phase = 0;
   c->ep_remapped = false;
   c->sens = 0;
-  auto up = [&](double* d, const std::vector<double>& h) {
-    return hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice);
-  };
+  auto up = [&](void* d, const std::vector<double>& h) { return put(c, d, h.data(), h.size()); };
   HIPCHK(c, up(c->rho, rho));
   HIPCHK(c, up(c->e, e));
   HIPCHK(c, up(c->p, pres));
@@ -451,15 +496,15 @@ int init_pure(mhg_ctx* c, const std::vector<int>& env, const std::vector<double>
   HIPCHK(c, up(c->q, zero));
   HIPCHK(c, up(c->m, m));
   HIPCHK(c, hipMemcpy(c->mask, mask.data(), NC * sizeof(int), hipMemcpyHostToDevice));
-  HIPCHK(c, up(c->xx[0], xs));
-  HIPCHK(c, up(c->xy[0], ys));
+  HIPCHK(c, hipMemcpy(c->xx[0], xs.data(), NN * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->xy[0], ys.data(), NN * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(c, up(c->ux[0], us));
   HIPCHK(c, up(c->uy[0], vs));
-  HIPCHK(c, hipMemset(c->dg0, 0, NN * sizeof(double)));
-  HIPCHK(c, hipMemset(c->dg1, 0, NN * sizeof(double)));
-  const size_t envb = (size_t)P.ne * NC * sizeof(double);
-  double* envs[] = {c->env_rho, c->env_e, c->env_p, c->env_cs, c->env_q, c->env_m, c->env_f, c->env_mf};
-  for (double* d : envs) HIPCHK(c, hipMemset(d, 0, envb));
+  HIPCHK(c, hipMemset(c->dg0, 0, NN * (size_t)c->bulk));
+  HIPCHK(c, hipMemset(c->dg1, 0, NN * (size_t)c->bulk));
+  const size_t envb = (size_t)P.ne * NC * (size_t)c->bulk;
+  void* envs[] = {c->env_rho, c->env_e, c->env_p, c->env_cs, c->env_q, c->env_m, c->env_f, c->env_mf};
+  for (void* d : envs) HIPCHK(c, hipMemset(d, 0, envb));
   std::memset(c->S_host, 0, sizeof(DevScalars));
   c->S_host->dt = c->opt.deltat_init;
   int r = write_scalars(c);
@@ -524,7 +569,8 @@ int mhg_create(const mhg_mesh_desc* mesh, const mhg_options* opt, mhg_ctx** out)
       opt->projection_limiteur_id >= MHG_LIM_MINMODG || opt->projection_limiteur_id < 0 ||
       opt->ordre_projection < 1 || opt->ordre_projection > 2)
     return MHG_E_UNSUPPORTED;
-  if (mesh->dim != 2 || opt->precision != MHG_PREC_FP64) return MHG_E_UNSUPPORTED;
+  if (mesh->dim != 2 || (opt->precision != MHG_PREC_FP64 && opt->precision != MHG_PREC_FP32_BULK))
+    return MHG_E_UNSUPPORTED;
   mhg_ctx* c = new mhg_ctx();
   c->opt = *opt;
   Params& P = c->P;
@@ -587,20 +633,21 @@ int mhg_create(const mhg_mesh_desc* mesh, const mhg_options* opt, mhg_ctx** out)
   const long NC = P.nc, NN = P.nn;
   const size_t envn = (size_t)P.ne * NC;
   int r = 0;
-  r |= dalloc(c, &c->rho, NC) | dalloc(c, &c->e, NC) | dalloc(c, &c->p, NC) | dalloc(c, &c->cs, NC);
-  r |= dalloc(c, &c->q, NC) | dalloc(c, &c->m, NC) | dalloc(c, &c->L, NC) | dalloc(c, &c->Vlag, NC);
+  c->bulk = opt->precision == MHG_PREC_FP32_BULK ? 4 : 8;
+  r |= balloc(c, &c->rho, NC) | balloc(c, &c->e, NC) | balloc(c, &c->p, NC) | balloc(c, &c->cs, NC);
+  r |= balloc(c, &c->q, NC) | balloc(c, &c->m, NC) | balloc(c, &c->L, NC) | balloc(c, &c->Vlag, NC);
   r |= dalloc(c, &c->mask, NC);
-  r |= dalloc(c, &c->U[0], (size_t)P.nvu * NC) | dalloc(c, &c->U[1], (size_t)P.nvu * NC);
-  r |= dalloc(c, &c->contrib, envn);
-  r |= dalloc(c, &c->env_rho, envn) | dalloc(c, &c->env_e, envn) | dalloc(c, &c->env_p, envn);
-  r |= dalloc(c, &c->env_cs, envn) | dalloc(c, &c->env_q, envn) | dalloc(c, &c->env_m, envn);
-  r |= dalloc(c, &c->env_f, envn) | dalloc(c, &c->env_mf, envn);
+  r |= balloc(c, &c->U[0], (size_t)P.nvu * NC) | balloc(c, &c->U[1], (size_t)P.nvu * NC);
+  r |= balloc(c, &c->contrib, envn);
+  r |= balloc(c, &c->env_rho, envn) | balloc(c, &c->env_e, envn) | balloc(c, &c->env_p, envn);
+  r |= balloc(c, &c->env_cs, envn) | balloc(c, &c->env_q, envn) | balloc(c, &c->env_m, envn);
+  r |= balloc(c, &c->env_f, envn) | balloc(c, &c->env_mf, envn);
   for (int b = 0; b < 2; ++b) {
-    r |= dalloc(c, &c->ux[b], NN) | dalloc(c, &c->uy[b], NN);
+    r |= balloc(c, &c->ux[b], NN) | balloc(c, &c->uy[b], NN);
     r |= dalloc(c, &c->xx[b], NN) | dalloc(c, &c->xy[b], NN);
   }
-  r |= dalloc(c, &c->Ud0, NN) | dalloc(c, &c->Ud1, NN) | dalloc(c, &c->Ud3, NN);
-  r |= dalloc(c, &c->dg0, NN) | dalloc(c, &c->dg1, NN);
+  r |= balloc(c, &c->Ud0, NN) | balloc(c, &c->Ud1, NN) | balloc(c, &c->Ud3, NN);
+  r |= balloc(c, &c->dg0, NN) | balloc(c, &c->dg1, NN);
   r |= dalloc(c, &c->block_min, cell_final_blocks(P));
   r |= dalloc(c, &c->S, 1);
   if (r) {
@@ -691,7 +738,7 @@ int mhg_upload_state(mhg_ctx* c, const mhg_state* st) {
   if (!c || !st) return MHG_E_ARG;
   const Params& P = c->P;
   const long NC = P.nc, NN = P.nn;
-  const size_t cb = NC * sizeof(double), nb = NN * sizeof(double), eb = (size_t)P.ne * cb;
+  const size_t nb = NN * sizeof(double);
   c->xcur = 0;
   c->ucur = 0;
   c->coord_phase = 0;
@@ -699,34 +746,35 @@ int mhg_upload_state(mhg_ctx* c, const mhg_state* st) {
   auto h2d = [&](void* d, const void* h, size_t b) {
     return h ? hipMemcpy(d, h, b, hipMemcpyHostToDevice) : hipSuccess;
   };
-  HIPCHK(c, h2d(c->rho, st->density, cb));
-  HIPCHK(c, h2d(c->e, st->internal_energy, cb));
-  HIPCHK(c, h2d(c->p, st->pressure, cb));
-  HIPCHK(c, h2d(c->cs, st->sound_speed, cb));
-  HIPCHK(c, h2d(c->q, st->pseudo_viscosity, cb));
-  HIPCHK(c, h2d(c->m, st->cell_mass, cb));
+  const size_t en = (size_t)P.ne * NC;
+  HIPCHK(c, put(c, c->rho, st->density, NC));
+  HIPCHK(c, put(c, c->e, st->internal_energy, NC));
+  HIPCHK(c, put(c, c->p, st->pressure, NC));
+  HIPCHK(c, put(c, c->cs, st->sound_speed, NC));
+  HIPCHK(c, put(c, c->q, st->pseudo_viscosity, NC));
+  HIPCHK(c, put(c, c->m, st->cell_mass, NC));
   HIPCHK(c, h2d(c->mask, st->env_mask, NC * sizeof(int)));
-  HIPCHK(c, h2d(c->env_rho, st->density_env, eb));
-  HIPCHK(c, h2d(c->env_e, st->internal_energy_env, eb));
-  HIPCHK(c, h2d(c->env_p, st->pressure_env, eb));
-  HIPCHK(c, h2d(c->env_cs, st->sound_speed_env, eb));
-  HIPCHK(c, h2d(c->env_q, st->pseudo_viscosity_env, eb));
-  HIPCHK(c, h2d(c->env_m, st->cell_mass_env, eb));
-  HIPCHK(c, h2d(c->env_f, st->fracvol_env, eb));
-  HIPCHK(c, h2d(c->env_mf, st->mass_fraction_env, eb));
+  HIPCHK(c, put(c, c->env_rho, st->density_env, en));
+  HIPCHK(c, put(c, c->env_e, st->internal_energy_env, en));
+  HIPCHK(c, put(c, c->env_p, st->pressure_env, en));
+  HIPCHK(c, put(c, c->env_cs, st->sound_speed_env, en));
+  HIPCHK(c, put(c, c->env_q, st->pseudo_viscosity_env, en));
+  HIPCHK(c, put(c, c->env_m, st->cell_mass_env, en));
+  HIPCHK(c, put(c, c->env_f, st->fracvol_env, en));
+  HIPCHK(c, put(c, c->env_mf, st->mass_fraction_env, en));
   if (st->velocity) {
-    HIPCHK(c, h2d(c->ux[0], st->velocity, nb));
-    HIPCHK(c, h2d(c->uy[0], st->velocity + NN, nb));
+    HIPCHK(c, put(c, c->ux[0], st->velocity, NN));
+    HIPCHK(c, put(c, c->uy[0], st->velocity + NN, NN));
   }
   if (st->node_coord) {
     HIPCHK(c, h2d(c->xx[0], st->node_coord, nb));
     HIPCHK(c, h2d(c->xy[0], st->node_coord + NN, nb));
   }
   if (st->dual_grad_phi) {
-    HIPCHK(c, h2d(c->dg0, st->dual_grad_phi, nb));
-    HIPCHK(c, h2d(c->dg1, st->dual_grad_phi + NN, nb));
+    HIPCHK(c, put(c, c->dg0, st->dual_grad_phi, NN));
+    HIPCHK(c, put(c, c->dg1, st->dual_grad_phi + NN, NN));
   }
-  if (st->caracteristic_length) HIPCHK(c, h2d(c->L, st->caracteristic_length, cb));
+  if (st->caracteristic_length) HIPCHK(c, put(c, c->L, st->caracteristic_length, NC));
   std::memset(c->S_host, 0, sizeof(DevScalars));
   c->S_host->dt = st->global_deltat;
   c->S_host->dt_old = st->old_deltat;
@@ -744,23 +792,22 @@ int mhg_download_state(mhg_ctx* c, mhg_state* st) {
   if (r) return r;
   const Params& P = c->P;
   const long NC = P.nc, NN = P.nn;
-  const size_t cb = NC * sizeof(double), nb = NN * sizeof(double), eb = (size_t)P.ne * cb;
+  const size_t nb = NN * sizeof(double), en = (size_t)P.ne * NC;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   auto d2h = [&](void* h, const void* d, size_t b) {
     return h ? hipMemcpy(h, d, b, hipMemcpyDeviceToHost) : hipSuccess;
   };
-  HIPCHK(c, d2h(st->density, c->rho, cb));
-  HIPCHK(c, d2h(st->internal_energy, c->e, cb));
-  HIPCHK(c, d2h(st->pressure, c->p, cb));
-  HIPCHK(c, d2h(st->sound_speed, c->cs, cb));
-  HIPCHK(c, d2h(st->pseudo_viscosity, c->q, cb));
-  HIPCHK(c, d2h(st->cell_mass, c->m, cb));
+  HIPCHK(c, get(c, st->density, c->rho, NC));
+  HIPCHK(c, get(c, st->internal_energy, c->e, NC));
+  HIPCHK(c, get(c, st->pressure, c->p, NC));
+  HIPCHK(c, get(c, st->sound_speed, c->cs, NC));
+  HIPCHK(c, get(c, st->pseudo_viscosity, c->q, NC));
+  HIPCHK(c, get(c, st->cell_mass, c->m, NC));
   std::vector<int> mask(NC);
   HIPCHK(c, hipMemcpy(mask.data(), c->mask, NC * sizeof(int), hipMemcpyDeviceToHost));
   if (st->env_mask) std::memcpy(st->env_mask, mask.data(), NC * sizeof(int));
-  struct EP { double* h; double* d; double* g; };
-  std::vector<double> gbuf[6];
-  double* gsrc[6] = {c->rho, c->e, c->p, c->cs, c->q, c->m};
+  struct EP { double* h; void* d; double* g; };
+  void* gsrc[6] = {c->rho, c->e, c->p, c->cs, c->q, c->m};
   EP eps[8] = {{st->density_env, c->env_rho, st->density},
                {st->internal_energy_env, c->env_e, st->internal_energy},
                {st->pressure_env, c->env_p, st->pressure},
@@ -771,10 +818,10 @@ int mhg_download_state(mhg_ctx* c, mhg_state* st) {
                {st->mass_fraction_env, c->env_mf, nullptr}};
   for (int t = 0; t < 8; ++t) {
     if (!eps[t].h) continue;
-    HIPCHK(c, hipMemcpy(eps[t].h, eps[t].d, eb, hipMemcpyDeviceToHost));
+    HIPCHK(c, get(c, eps[t].h, eps[t].d, en));
     // pure cells: the environment slot is the cell slot
     std::vector<double> g(NC, 1.0);
-    if (t < 6) HIPCHK(c, hipMemcpy(g.data(), gsrc[t], cb, hipMemcpyDeviceToHost));
+    if (t < 6) HIPCHK(c, get(c, g.data(), gsrc[t], NC));
     for (long cc = 0; cc < NC; ++cc)
       if (__builtin_popcount((unsigned)mask[cc]) == 1)
         eps[t].h[(size_t)(__builtin_ffs(mask[cc]) - 1) * NC + cc] = g[cc];
@@ -786,8 +833,8 @@ int mhg_download_state(mhg_ctx* c, mhg_state* st) {
     if (st->mass_fraction) st->mass_fraction[cc] = __builtin_popcount((unsigned)mask[cc]) == 1 ? 1.0 : 0.0;
   }
   if (st->velocity) {
-    HIPCHK(c, d2h(st->velocity, c->ux[c->ucur], nb));
-    HIPCHK(c, d2h(st->velocity + NN, c->uy[c->ucur], nb));
+    HIPCHK(c, get(c, st->velocity, c->ux[c->ucur], NN));
+    HIPCHK(c, get(c, st->velocity + NN, c->uy[c->ucur], NN));
   }
   if (st->node_coord) {
     if (c->coord_phase == 1) {  // Euler reset of saveValuesAtN: x = x0
@@ -802,13 +849,13 @@ int mhg_download_state(mhg_ctx* c, mhg_state* st) {
     }
   }
   if (st->dual_grad_phi) {
-    HIPCHK(c, d2h(st->dual_grad_phi, c->dg0, nb));
-    HIPCHK(c, d2h(st->dual_grad_phi + NN, c->dg1, nb));
+    HIPCHK(c, get(c, st->dual_grad_phi, c->dg0, NN));
+    HIPCHK(c, get(c, st->dual_grad_phi + NN, c->dg1, NN));
   }
-  if (st->caracteristic_length) HIPCHK(c, d2h(st->caracteristic_length, c->L, cb));
+  if (st->caracteristic_length) HIPCHK(c, get(c, st->caracteristic_length, c->L, NC));
   if (st->node_mass) {
     std::vector<double> m(NC);
-    HIPCHK(c, hipMemcpy(m.data(), c->m, cb, hipMemcpyDeviceToHost));
+    HIPCHK(c, get(c, m.data(), c->m, NC));
     for (long n = 0; n < NN; ++n) {
       const int i = (int)(n % (P.nx + 1)), j = (int)(n / (P.nx + 1));
       double s = 0.;
@@ -840,6 +887,7 @@ namespace {
 int ep_prepare(mhg_ctx* c) {
   int r = check_ready(c);
   if (r) return r;
+  if (c->bulk != 8) return fail(c, MHG_E_UNSUPPORTED, "entry points: FP64 contexts only (use mhg_step)");
   if (c->ep_ready) return MHG_OK;
   const size_t NC = c->P.nc, envn = (size_t)c->P.ne * NC;
   r |= dalloc(c, &c->rho_n, NC) | dalloc(c, &c->e_n, NC) | dalloc(c, &c->q_n, NC);
@@ -855,22 +903,22 @@ StageArgs stage_args(mhg_ctx* c) {
   StageArgs A;
   A.xnx = c->xx[1 - c->xcur];
   A.xny = c->xy[1 - c->xcur];
-  A.rho = c->rho;
-  A.e = c->e;
-  A.p = c->p;
-  A.cs = c->cs;
-  A.q = c->q;
-  A.m = c->m;
-  A.L = c->L;
-  A.Vlag = c->Vlag;
+  A.rho = bp<double>(c->rho);
+  A.e = bp<double>(c->e);
+  A.p = bp<double>(c->p);
+  A.cs = bp<double>(c->cs);
+  A.q = bp<double>(c->q);
+  A.m = bp<double>(c->m);
+  A.L = bp<double>(c->L);
+  A.Vlag = bp<double>(c->Vlag);
   A.mask = c->mask;
-  A.env_f = c->env_f;
-  A.env_mf = c->env_mf;
-  A.env_m = c->env_m;
-  A.env_p = c->env_p;
-  A.env_rho = c->env_rho;
-  A.env_e = c->env_e;
-  A.env_q = c->env_q;
+  A.env_f = bp<double>(c->env_f);
+  A.env_mf = bp<double>(c->env_mf);
+  A.env_m = bp<double>(c->env_m);
+  A.env_p = bp<double>(c->env_p);
+  A.env_rho = bp<double>(c->env_rho);
+  A.env_e = bp<double>(c->env_e);
+  A.env_q = bp<double>(c->env_q);
   A.rho_n = c->rho_n;
   A.e_n = c->e_n;
   A.q_n = c->q_n;
@@ -880,7 +928,7 @@ StageArgs stage_args(mhg_ctx* c) {
   A.env_e_n = c->env_e_n;
   A.env_q_n = c->env_q_n;
   A.env_tau = c->env_tau;
-  A.U = c->U[0];
+  A.U = bp<double>(c->U[0]);
   return A;
 }
 
@@ -900,7 +948,7 @@ int ep_stage(mhg_ctx* c, int stage) {
 int ep_node(mhg_ctx* c, bool vel, bool bc, bool pos) {
   int r = ep_prepare(c);
   if (r) return r;
-  launch_node_lagrange_2d(c->stream, c->P, c->S, node_lag_args(c), vel, bc, pos);
+  launch_node_lagrange_2d(c->stream, c->P, c->S, node_lag_args<double>(c), vel, bc, pos);
   return ep_finish(c);
 }
 
@@ -951,7 +999,7 @@ int mhg_remap(mhg_ctx* c) {
   int r = ep_prepare(c);
   if (r) return r;
   launch_cell_stage_2d(c->stream, c->P, c->S, stage_args(c), STAGE_BUILD_U);
-  remap_2d(c, 0);
+  remap_2d<double>(c, 0);
   r = ep_finish(c);
   if (r) return r;
   c->coord_phase = 0;
@@ -1103,36 +1151,36 @@ int* variant_slot(mhg_ctx* c, int k) {
 }
 }  // namespace
 
-int mhg_autotune(mhg_ctx* c, int32_t reps, char* report, int32_t report_len) {
-  int r = check_ready(c);
-  if (r) return r;
-  if (reps < 1) reps = 1;
+extern "C++" {
+namespace {
+template <class R>
+int autotune_t(mhg_ctx* c, int32_t reps, char* report, int32_t report_len) {
   const Params& P = c->P;
-  CellSweepArgs A;
-  A.Uin = c->U[0];
-  A.Uout = c->U[1];
-  A.Vlag = c->Vlag;
+  CellSweepArgsT<R> A;
+  A.Uin = bp<R>(c->U[0]);
+  A.Uout = bp<R>(c->U[1]);
+  A.Vlag = bp<R>(c->Vlag);
   A.xnx = c->xx[c->xcur];
   A.xny = c->xy[c->xcur];
-  A.ux = c->ux[c->ucur];
-  A.uy = c->uy[c->ucur];
-  A.contrib = c->contrib;
-  NodeSweepArgs B;
-  B.m = c->m;
+  A.ux = bp<R>(c->ux[c->ucur]);
+  A.uy = bp<R>(c->uy[c->ucur]);
+  A.contrib = bp<R>(c->contrib);
+  NodeSweepArgsT<R> B;
+  B.m = bp<R>(c->m);
   B.ux = A.ux;
   B.uy = A.uy;
   B.xnx = A.xnx;
   B.xny = A.xny;
-  B.contrib = c->contrib;
-  B.Ud0 = c->Ud0;
-  B.Ud1 = c->Ud1;
-  B.Ud3 = c->Ud3;
-  B.dg0 = c->dg0;
-  B.dg1 = c->dg1;
-  B.uox = c->ux[1 - c->ucur];
-  B.uoy = c->uy[1 - c->ucur];
-  const size_t gbytes = sizeof(double) * (size_t)P.nn;
-  double* saved = nullptr;
+  B.contrib = bp<R>(c->contrib);
+  B.Ud0 = bp<R>(c->Ud0);
+  B.Ud1 = bp<R>(c->Ud1);
+  B.Ud3 = bp<R>(c->Ud3);
+  B.dg0 = bp<R>(c->dg0);
+  B.dg1 = bp<R>(c->dg1);
+  B.uox = bp<R>(c->ux[1 - c->ucur]);
+  B.uoy = bp<R>(c->uy[1 - c->ucur]);
+  const size_t gbytes = sizeof(R) * (size_t)P.nn;
+  R* saved = nullptr;
   if (hipMalloc(&saved, 2 * gbytes) != hipSuccess) return fail(c, MHG_E_ALLOC, "autotune: hipMalloc failed");
   HIPCHK(c, hipMemcpyAsync(saved, c->dg0, gbytes, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(saved + P.nn, c->dg1, gbytes, hipMemcpyDeviceToDevice, c->stream));
@@ -1180,6 +1228,16 @@ int mhg_autotune(mhg_ctx* c, int32_t reps, char* report, int32_t report_len) {
     report[report_len - 1] = 0;
   }
   return MHG_OK;
+}
+}  // namespace
+}  // extern "C++"
+
+int mhg_autotune(mhg_ctx* c, int32_t reps, char* report, int32_t report_len) {
+  int r = check_ready(c);
+  if (r) return r;
+  if (reps < 1) reps = 1;
+  return c->bulk == 4 ? autotune_t<float>(c, reps, report, report_len)
+                      : autotune_t<double>(c, reps, report, report_len);
 }
 
 int32_t mhg_variant_count(const char* kernel) { return variant_count_of(tunable_index(kernel)); }

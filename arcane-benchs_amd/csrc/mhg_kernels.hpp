@@ -5,41 +5,60 @@
 
 namespace mhg {
 
-struct NodeLagArgs {
-  const double *p, *q, *m;    // cell pressure, pseudo-viscosity, mass at the start of the step
+// Argument blocks of the fused step, templated on the storage type R of the
+// bulk fields (double: the drop-in; float: the precision form).  Node
+// coordinates, the dt scalars and the CFL block minima stay double in both.
+template <class R>
+struct NodeLagArgsT {
+  const R *p, *q, *m;         // cell pressure, pseudo-viscosity, mass at the start of the step
   const double *xpx, *xpy;    // node coordinates of the previous Lagrange phase
-  double *ux, *uy;            // node velocity (updated in place)
+  R *ux, *uy;                 // node velocity (updated in place)
   double *xnx, *xny;          // new Lagrangian coordinates
 };
 
-struct CellLagArgs {
+template <class R>
+struct CellLagArgsT {
   const double *xnx, *xny;
-  const double *rho, *e, *p, *cs, *q, *m;
+  const R *rho, *e, *p, *cs, *q, *m;
   const int* mask;
-  const double *env_f, *env_rho, *env_m, *env_q, *env_p, *env_e;
-  double *U, *Vlag, *L;
+  const R *env_f, *env_rho, *env_m, *env_q, *env_p, *env_e;
+  R *U, *Vlag, *L;
 };
 
-struct CellSweepArgs {
-  const double* Uin;
-  double* Uout;
-  const double *Vlag, *xnx, *xny, *ux, *uy;
-  double* contrib;
+template <class R>
+struct CellSweepArgsT {
+  const R* Uin;
+  R* Uout;
+  const R* Vlag;
+  const double *xnx, *xny;
+  const R *ux, *uy;
+  R* contrib;
 };
 
-struct NodeSweepArgs {
-  const double *m, *ux, *uy, *xnx, *xny, *contrib;
-  double *Ud0, *Ud1, *Ud3, *dg0, *dg1, *uox, *uoy;
+template <class R>
+struct NodeSweepArgsT {
+  const R* m;
+  const R *ux, *uy;
+  const double *xnx, *xny;
+  const R* contrib;
+  R *Ud0, *Ud1, *Ud3, *dg0, *dg1, *uox, *uoy;
 };
 
-struct CellFinalArgs {
-  const double* U;
-  const double *Vlag, *L, *ux, *uy;
-  double *rho, *e, *p, *cs, *q, *m;
+template <class R>
+struct CellFinalArgsT {
+  const R* U;
+  const R *Vlag, *L, *ux, *uy;
+  R *rho, *e, *p, *cs, *q, *m;
   int* mask;
-  double *env_rho, *env_e, *env_p, *env_cs, *env_q, *env_m, *env_f, *env_mf;
+  R *env_rho, *env_e, *env_p, *env_cs, *env_q, *env_m, *env_f, *env_mf;
   double* block_min;  // one CFL minimum per block of k_cell_final_2d
 };
+
+using NodeLagArgs = NodeLagArgsT<double>;
+using CellLagArgs = CellLagArgsT<double>;
+using CellSweepArgs = CellSweepArgsT<double>;
+using NodeSweepArgs = NodeSweepArgsT<double>;
+using CellFinalArgs = CellFinalArgsT<double>;
 
 // Per-entry-point cell stages (the unfused path): each reads and writes the
 // state arrays as the reference entry point of the same name does; the _n,
@@ -67,21 +86,28 @@ void launch_dt_finalize(hipStream_t st, const Params& P, DevScalars* S, const do
 
 int cell_final_blocks(const Params& P);
 void launch_begin_iteration(hipStream_t st, DevScalars* S);
-void launch_node_lagrange_2d(hipStream_t st, const Params& P, DevScalars* S, const NodeLagArgs& A,
+template <class R>
+void launch_node_lagrange_2d(hipStream_t st, const Params& P, DevScalars* S, const NodeLagArgsT<R>& A,
                              bool vel, bool bc, bool pos);
-void launch_cell_lagrange_2d(hipStream_t st, const Params& P, DevScalars* S, const CellLagArgs& A);
-void launch_cell_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const CellSweepArgs& A,
+template <class R>
+void launch_cell_lagrange_2d(hipStream_t st, const Params& P, DevScalars* S, const CellLagArgsT<R>& A);
+// Launch variants other than the LDS tile and lane forms exist for double only;
+// float contexts always run those two.
+template <class R>
+void launch_cell_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const CellSweepArgsT<R>& A,
                           int dir, bool first, int variant);
 int cell_sweep_y_variants();
 const char* cell_sweep_y_variant_name(int v);
-void launch_node_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const NodeSweepArgs& A,
+template <class R>
+void launch_node_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const NodeSweepArgsT<R>& A,
                           int dir, bool first, int variant);
 int node_sweep_variants(int dir);
 const char* node_sweep_variant_name(int dir, int v);
 // dt_mode 0: no dt kernel (the caller computes dt later); 1: the final dt
 // kernel follows; 2: it also applies time += dt, iteration += 1 of the
 // iteration (fused step without a separate begin-iteration launch)
-void launch_cell_final_2d(hipStream_t st, const Params& P, DevScalars* S, const CellFinalArgs& A,
+template <class R>
+void launch_cell_final_2d(hipStream_t st, const Params& P, DevScalars* S, const CellFinalArgsT<R>& A,
                           int dt_mode);
 
 }  // namespace mhg

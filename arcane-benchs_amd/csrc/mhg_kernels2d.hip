@@ -15,6 +15,7 @@
 // evaluates to (1,0,0)-type vectors bit for bit), so dot products with them
 // reduce to one component and are not stored.
 #include <cstdio>
+#include <type_traits>
 
 #include "mhg_common.hpp"
 #include "mhg_kernels.hpp"
@@ -41,8 +42,8 @@ __global__ void k_begin_iteration(DevScalars* S) {
 // Node mass = 1/4 sum of the cell masses (computeNodeMass 235-266): the cell
 // masses do not change in the Lagrange phase, so it is recomputed here.
 // ---------------------------------------------------------------------------
-template <bool VEL, bool BC, bool POS>
-__global__ void k_node_lagrange_2d(Params P, DevScalars* S, NodeLagArgs A) {
+template <class R, bool VEL, bool BC, bool POS>
+__global__ void k_node_lagrange_2d(Params P, DevScalars* S, NodeLagArgsT<R> A) {
   if (S->stop || S->err) return;
   const long nid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (nid >= P.nn) return;
@@ -82,10 +83,10 @@ __global__ void k_node_lagrange_2d(Params P, DevScalars* S, NodeLagArgs A) {
       double qy[4] = {Y[bj][bi], Y[bj][bi + 1], Y[bj + 1][bi + 1], Y[bj + 1][bi]};
       double cx, cy;
       cqs2d(qx, qy, cq[t], &cx, &cy);
-      double pq = A.p[c] + A.q[c];
+      double pq = (double)A.p[c] + (double)A.q[c];
       Fx += pq * cx;
       Fy += pq * cy;
-      msum += A.m[c];
+      msum += (double)A.m[c];
     }
     double mn = 0.25 * msum;
     double s = dtv / mn;
@@ -103,8 +104,8 @@ __global__ void k_node_lagrange_2d(Params P, DevScalars* S, NodeLagArgs A) {
     }
   }
   if (VEL || BC) {
-    st_stream(&A.ux[nid], vx);
-    st_stream(&A.uy[nid], vy);
+    st_stream(&A.ux[nid], (R)vx);
+    st_stream(&A.uy[nid], (R)vy);
   }
   if (POS) {
     st_stream(&A.xnx[nid], node_x0(P, i) + dt * vx);
@@ -128,8 +129,8 @@ __global__ void k_node_lagrange_2d(Params P, DevScalars* S, NodeLagArgs A) {
 // energy slots (PrepareRemap.cc:324-328) are identically zero and stay zero
 // through the sweeps (phi = 0 gives a zero gradient, face value and flux).
 // ---------------------------------------------------------------------------
-template <bool PC, int NEM>
-__global__ __launch_bounds__(256) void k_cell_lagrange_2d(Params P, DevScalars* S, CellLagArgs A) {
+template <class R, bool PC, int NEM>
+__global__ __launch_bounds__(256) void k_cell_lagrange_2d(Params P, DevScalars* S, CellLagArgsT<R> A) {
   if (S->stop || S->err) return;
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= P.nc) return;
@@ -143,27 +144,29 @@ __global__ __launch_bounds__(256) void k_cell_lagrange_2d(Params P, DevScalars* 
     x[q] = A.xnx[nd[q]];
     y[q] = A.xny[nd[q]];
   }
-  const double V = quad_volume(x, y);
-  const double L = (P.lc == MHG_LC_FACES_OPPOSEES) ? length_faces_opposees_2d(x, y) : pow(V, 0.5);
-  const double inv_dt = 1.0 / S->dt;
-  const double rho_n = A.rho[c];
-  const double rho = A.m[c] / V;
-  const double tau = 0.5 * (1.0 / rho_n + 1.0 / rho);
-  const double divu = inv_dt * (1.0 / rho - 1.0 / rho_n) / tau;
-  const double cs = A.cs[c];
+  // geometry from the coordinates in double, the rest in the storage type
+  const double Vd = quad_volume(x, y);
+  const double Ld = (P.lc == MHG_LC_FACES_OPPOSEES) ? length_faces_opposees_2d(x, y) : pow(Vd, 0.5);
+  const R V = (R)Vd, L = (R)Ld;
+  const R inv_dt = qdiv(R(1.0), (R)S->dt);
+  const R rho_n = A.rho[c];
+  const R rho = qdiv(A.m[c], V);
+  const R tau = R(0.5) * (qdiv(R(1.0), rho_n) + qdiv(R(1.0), rho));
+  const R divu = qdiv(inv_dt * (qdiv(R(1.0), rho) - qdiv(R(1.0), rho_n)), tau);
+  const R cs = A.cs[c];
   const int mask = A.mask[c];
   const int ne = P.ne;
   const long NC = P.nc;
-  double uv[NEM], um[NEM], ue[NEM];
-  double uq = 0.;
+  R uv[NEM], um[NEM], ue[NEM];
+  R uq = R(0);
 #pragma unroll
-  for (int k = 0; k < NEM; ++k) uv[k] = um[k] = ue[k] = 0.;
+  for (int k = 0; k < NEM; ++k) uv[k] = um[k] = ue[k] = R(0);
   if (__popc(mask) == 1) {
     const int kk = __ffs(mask) - 1;
-    double qv = 0.;
-    if (divu < 0.0) qv = viscosity(P.eos.qcoef[kk], tau, L, cs, divu);
-    const double qn = PC ? A.q[c] : 0.;
-    const double e = compute_eint(PC, P.eos.gamma[kk], qn, qv, rho_n, rho, A.p[c], A.e[c]);
+    R qv = R(0);
+    if (divu < R(0)) qv = viscosity((R)P.eos.qcoef[kk], tau, L, cs, divu);
+    const R qn = PC ? A.q[c] : R(0);
+    const R e = compute_eint(PC, (R)P.eos.gamma[kk], qn, qv, rho_n, rho, A.p[c], A.e[c]);
 #pragma unroll
     for (int k = 0; k < NEM; ++k)
       if (k == kk) {
@@ -177,22 +180,22 @@ __global__ __launch_bounds__(256) void k_cell_lagrange_2d(Params P, DevScalars* 
     for (int k = 0; k < NEM; ++k) {
       if (!(mask & (1 << k))) continue;
       const long ek = k * NC + c;
-      const double f = A.env_f[ek];
-      const double Vk = f * V;
-      const double rho_nk = A.env_rho[ek];
-      const double rk = A.env_m[ek] / Vk;
-      const double tk = 0.5 * (1.0 / rho_nk + 1.0 / rk);
-      double qk = 0.;
-      if (divu < 0.0) qk = viscosity(P.eos.qcoef[k], tk, L, cs, divu);
-      const double qnk = PC ? A.env_q[ek] : 0.;
-      const double ekv = compute_eint(PC, P.eos.gamma[k], qnk, qk, rho_nk, rk, A.env_p[ek], A.env_e[ek]);
+      const R f = A.env_f[ek];
+      const R Vk = f * V;
+      const R rho_nk = A.env_rho[ek];
+      const R rk = qdiv(A.env_m[ek], Vk);
+      const R tk = R(0.5) * (qdiv(R(1.0), rho_nk) + qdiv(R(1.0), rk));
+      R qk = R(0);
+      if (divu < R(0)) qk = viscosity((R)P.eos.qcoef[k], tk, L, cs, divu);
+      const R qnk = PC ? A.env_q[ek] : R(0);
+      const R ekv = compute_eint(PC, (R)P.eos.gamma[k], qnk, qk, rho_nk, rk, A.env_p[ek], A.env_e[ek]);
       uv[k] = Vk;
       um[k] = Vk * rk;
       ue[k] = Vk * rk * ekv;
       uq = Vk * qk;  // PrepareRemap.cc:372 assigns: the last environment wins
     }
   }
-  double* U = A.U;
+  R* U = A.U;
 #pragma unroll
   for (int k = 0; k < NEM; ++k) {
     if (k >= ne) break;
@@ -353,8 +356,9 @@ __global__ __launch_bounds__(256) void k_cell_stage_2d(Params P, DevScalars* S, 
 // computeGradPhiFace (RemapADIService.cc:171-177), the limited cell gradient
 // (497-506), the upwind face value (707-714) and the flux update (909-936).
 // ---------------------------------------------------------------------------
-__device__ inline double face_grad(bool inner, double phi, double phim, double dx) {
-  return inner ? (phi - phim) / dx : 0.;
+template <class T>
+__device__ inline T face_grad(bool inner, T phi, T phim, T dx) {
+  return inner ? qdiv(phi - phim, dx) : T(0);
 }
 
 // Lane sweep: one wavefront holds 64 consecutive cells along the sweep
@@ -362,9 +366,9 @@ __device__ inline double face_grad(bool inner, double phi, double phim, double d
 // neighbours move between lanes.  D = 0: a wave is a row segment.  D = 1: a
 // wave is a column segment and a block of 16 waves covers 16 adjacent
 // columns, so each cache line of a row is consumed by the whole block.
-template <int D, int NEM, bool FIRST, int LIM>
+template <class R, int D, int NEM, bool FIRST, int LIM>
 __global__ __launch_bounds__(D == 0 ? 256 : 1024) void k_cell_sweep_lane_2d(Params P, DevScalars* S,
-                                                                          CellSweepArgs A) {
+                                                                          CellSweepArgsT<R> A) {
   if (S->stop || S->err) return;
   const int lane = threadIdx.x & 63;
   const int N = (D == 0) ? P.nx : P.ny;      // cells along D
@@ -390,14 +394,15 @@ __global__ __launch_bounds__(D == 0 ? 256 : 1024) void k_cell_sweep_lane_2d(Para
   const long NC = P.nc;
   const int ne = P.ne;
   const double dt = S->dt;
-  const double order2 = (double)P.order2;
+  const R order2 = (R)P.order2;
   const long c = ex ? i + (long)P.nx * j : 0;
-  const double* __restrict__ Uin = A.Uin;
+  const R* __restrict__ Uin = A.Uin;
   const double* __restrict__ Xd = (D == 0) ? A.xnx : A.xny;   // coordinate along D
   const double* __restrict__ Xt = (D == 0) ? A.xny : A.xnx;   // transverse coordinate
-  const double* __restrict__ Ud = (D == 0) ? A.ux : A.uy;     // velocity along D
-  // cell centre (component D) and phi denominator
-  double xc = 0., den = 1.;
+  const R* __restrict__ Ud = (D == 0) ? A.ux : A.uy;          // velocity along D
+  // cell centre (component D, double) and phi denominator
+  double xc = 0.;
+  R den = R(1);
   if (ex) {
     const long m0 = i + (long)nnx * j;
     const double a = Xd[m0], b = Xd[m0 + 1], cc = Xd[m0 + 1 + nnx], dd = Xd[m0 + nnx];
@@ -405,7 +410,7 @@ __global__ __launch_bounds__(D == 0 ? 256 : 1024) void k_cell_sweep_lane_2d(Para
     if (FIRST) {
       den = A.Vlag[c];
     } else {
-      double sv = 0.;
+      R sv = R(0);
 #pragma unroll
       for (int k = 0; k < NEM; ++k)
         if (k < ne) sv += Uin[k * NC + c];
@@ -414,69 +419,74 @@ __global__ __launch_bounds__(D == 0 ? 256 : 1024) void k_cell_sweep_lane_2d(Para
   }
   // back face of the cell: the face at node index s along D, nodes
   // (i,j),(i,j+1) for D = 0 and (i,j),(i+1,j) for D = 1
-  double uf = 0., lf = 0., xf = 0.;
+  R uf = R(0);
+  double lf = 0., xf = 0.;
   if (s >= 0 && s <= N) {
     const long na = i + (long)nnx * j, nb = (D == 0) ? na + nnx : na + 1;
-    uf = 0.5 * (Ud[na] + Ud[nb]);
+    uf = R(0.5) * (Ud[na] + Ud[nb]);
     xf = 0.5 * Xd[na] + 0.5 * Xd[nb];
     lf = fabs(Xt[nb] - Xt[na]);
   }
   const bool inner = ex && s >= 1;
   const double xcm = __shfl_up(xc, 1, 64);
   const double dx = inner ? xc - xcm : 1.;
-  const bool upm = inner && (uf * dx > 0.0);
+  const bool upm = inner && ((double)uf * dx > 0.0);
   const double dotxf = inner ? (upm ? xf - xcm : xf - xc) : 0.;
-  const double uf_f = __shfl_down(uf, 1, 64), lf_f = __shfl_down(lf, 1, 64);
+  const R uf_f = __shfl_down(uf, 1, 64);
+  const double lf_f = __shfl_down(lf, 1, 64);
+  // face flux factors u_f L_f dt (double products, then the storage type)
+  const R fb = (R)(-1.0 * uf * lf * dt), ff = (R)(1.0 * uf_f * lf_f * dt);
+  const R dxr = (R)dx, dotr = (R)dotxf;
   const bool o2 = P.order2 > 0;
-  auto sweep_var = [&](int v, double& unew, double& dual) {
-    const double u0 = ex ? Uin[v * NC + c] : 0.;
-    const double phi = ex ? u0 / den : 0.;
-    const double phim = __shfl_up(phi, 1, 64);
-    const double gf = face_grad(inner, phi, phim, dx);
-    const double gfp = __shfl_down(gf, 1, 64);
-    const double g = o2 ? limited_grad<LIM, true>(gf, gfp) : 0.;
-    const double gm = __shfl_up(g, 1, 64);
-    const double phif = inner ? (upm ? phim + order2 * dotxf * gm : phi + order2 * dotxf * g) : 0.;
-    const double phifp = __shfl_down(phif, 1, 64);
-    const double flux_b = -1.0 * uf * lf * dt * phif;
-    const double flux_f = 1.0 * uf_f * lf_f * dt * phifp;
+  auto sweep_var = [&](int v, R& unew, R& dual) {
+    const R u0 = ex ? Uin[v * NC + c] : R(0);
+    const R phi = ex ? qdiv(u0, den) : R(0);
+    const R phim = __shfl_up(phi, 1, 64);
+    const R gf = face_grad(inner, phi, phim, dxr);
+    const R gfp = __shfl_down(gf, 1, 64);
+    const R g = o2 ? limited_grad<LIM, true>(gf, gfp) : R(0);
+    const R gm = __shfl_up(g, 1, 64);
+    const R phif = inner ? (upm ? phim + order2 * dotr * gm : phi + order2 * dotr * g) : R(0);
+    const R phifp = __shfl_down(phif, 1, 64);
+    const R flux_b = fb * phif;
+    const R flux_f = ff * phifp;
     if (D == 0) {  // cell faces in order: bottom, right (front), top, left (back)
       unew = u0 - (flux_f + flux_b);
-      dual = 0.5 * flux_f * 1.0 + 0.5 * flux_b * -1.0;
+      dual = R(0.5) * flux_f * R(1.0) + R(0.5) * flux_b * R(-1.0);
     } else {       // bottom (back), right, top (front), left
       unew = u0 - (flux_b + flux_f);
-      dual = 0.5 * flux_b * -1.0 + 0.5 * flux_f * 1.0;
+      dual = R(0.5) * flux_b * R(-1.0) + R(0.5) * flux_f * R(1.0);
     }
   };
-  double* __restrict__ Uout = A.Uout;
+  R* __restrict__ Uout = A.Uout;
 #pragma unroll
   for (int k = 0; k < NEM; ++k) {
     if (k >= ne) break;
-    double un, du;
+    R un, du;
     sweep_var(k, un, du);
     if (out) st_stream(&Uout[k * NC + c], un);
   }
 #pragma unroll
   for (int k = 0; k < NEM; ++k) {
     if (k >= ne) break;
-    double un, du;
+    R un, du;
     sweep_var(ne + k, un, du);
-    if (un < 0.) un = 0.;  // RemapADIService.cc:953-955
+    if (un < R(0)) un = R(0);  // RemapADIService.cc:953-955
     if (out) {
       st_stream(&Uout[(ne + k) * NC + c], un);
-      st_stream(&A.contrib[k * NC + c], 0.5 * du);  // fcontrib (DualPhaseRemap.cc:168-185)
+      st_stream(&A.contrib[k * NC + c], R(0.5) * du);  // fcontrib (DualPhaseRemap.cc:168-185)
     }
   }
 #pragma unroll
   for (int k = 0; k < NEM; ++k) {
     if (k >= ne) break;
-    double un, du;
+    R un, du;
     sweep_var(2 * ne + k, un, du);
-    if (un < 0.) un = 0.;  // RemapADIService.cc:956-958
+    if (un < R(0)) un = R(0);  // RemapADIService.cc:956-958
     if (out) st_stream(&Uout[(2 * ne + k) * NC + c], un);
   }
   {
-    double un, du;
+    R un, du;
     sweep_var(3 * ne, un, du);
     if (out) st_stream(&Uout[(3 * ne) * NC + c], un);
   }
@@ -498,9 +508,10 @@ __global__ __launch_bounds__(D == 0 ? 256 : 1024) void k_cell_sweep_lane_2d(Para
 // per lane and runs the lane-exchange arithmetic of the x sweep.
 constexpr int kTileW = 16, kTileR = 64, kTileP = kTileW + 1;
 
-template <int NEM, bool FIRST, int LIM>
-__global__ __launch_bounds__(1024) void k_cell_sweep_ytile_2d(Params P, DevScalars* S, CellSweepArgs A) {
-  __shared__ double lds[5][kTileR][kTileP];
+template <class R, int NEM, bool FIRST, int LIM>
+__global__ __launch_bounds__(1024) void k_cell_sweep_ytile_2d(Params P, DevScalars* S, CellSweepArgsT<R> A) {
+  __shared__ double lds[5][kTileR][kTileP];  // geometry
+  __shared__ R stg[3][kTileR][kTileP];       // staging of one variable: in, out, dual flux
   if (S->stop || S->err) return;  // uniform over the grid
   const int nx = P.nx, N = P.ny, nnx = nx + 1;
   const int ncolblk = (nx + kTileW - 1) / kTileW;
@@ -513,16 +524,17 @@ __global__ __launch_bounds__(1024) void k_cell_sweep_ytile_2d(Params P, DevScala
   const long NC = P.nc;
   const int ne = P.ne, nvu = P.nvu;
   const double dt = S->dt;
-  const double order2 = (double)P.order2;
+  const R order2 = (R)P.order2;
   const bool o2 = P.order2 > 0;
-  const double* __restrict__ Uin = A.Uin;
+  const R* __restrict__ Uin = A.Uin;
   // row-major cell of this thread
   const int gi = i0 + tc, gs = s0 + tr;
   const bool gex = gi < nx && gs >= 0 && gs < N;
   const bool gout = gex && tr >= 2 && tr <= 61;
   const long gc = gex ? gi + (long)nx * gs : 0;
-  {  // geometry of the tile, row-major
-    double xc = 0., den = 1., uf = 0., xf = 0., lf = 0.;
+  {  // geometry of the tile, row-major (stored in the double tiles exactly)
+    double xc = 0., xf = 0., lf = 0.;
+    R den = R(1), uf = R(0);
     if (gex) {
       const long m0 = gi + (long)nnx * gs;
       const double a = A.xny[m0], b = A.xny[m0 + 1], cc = A.xny[m0 + 1 + nnx], dd = A.xny[m0 + nnx];
@@ -530,7 +542,7 @@ __global__ __launch_bounds__(1024) void k_cell_sweep_ytile_2d(Params P, DevScala
       if (FIRST) {
         den = A.Vlag[gc];
       } else {
-        double sv = 0.;
+        R sv = R(0);
 #pragma unroll
         for (int k = 0; k < NEM; ++k)
           if (k < ne) sv += Uin[k * NC + gc];
@@ -539,7 +551,7 @@ __global__ __launch_bounds__(1024) void k_cell_sweep_ytile_2d(Params P, DevScala
     }
     if (gi < nx && gs >= 0 && gs <= N) {  // back face: nodes (i, s), (i+1, s)
       const long na = gi + (long)nnx * gs, nb = na + 1;
-      uf = 0.5 * (A.uy[na] + A.uy[nb]);
+      uf = R(0.5) * (A.uy[na] + A.uy[nb]);
       xf = 0.5 * A.xny[na] + 0.5 * A.xny[nb];
       lf = fabs(A.xnx[nb] - A.xnx[na]);
     }
@@ -553,41 +565,43 @@ __global__ __launch_bounds__(1024) void k_cell_sweep_ytile_2d(Params P, DevScala
   const int s = s0 + lane;
   const bool ex = (i0 + w < nx) && s >= 0 && s < N;
   const bool inner = ex && s >= 1;
-  const double xc = lds[0][lane][w], den = lds[1][lane][w];
-  const double uf = lds[2][lane][w], xf = lds[3][lane][w], lf = lds[4][lane][w];
+  const double xc = lds[0][lane][w], xf = lds[3][lane][w], lf = lds[4][lane][w];
+  const R den = (R)lds[1][lane][w], uf = (R)lds[2][lane][w];
   const double xcm = __shfl_up(xc, 1, 64);
   const double dx = inner ? xc - xcm : 1.;
-  const bool upm = inner && (uf * dx > 0.0);
+  const bool upm = inner && ((double)uf * dx > 0.0);
   const double dotxf = inner ? (upm ? xf - xcm : xf - xc) : 0.;
-  const double uf_f = __shfl_down(uf, 1, 64), lf_f = __shfl_down(lf, 1, 64);
-  __syncthreads();  // geometry tiles are reused as staging tiles below
-  double(*stin)[kTileP] = lds[0];
-  double(*stu)[kTileP] = lds[1];
-  double(*stc)[kTileP] = lds[2];
-  double nxt = gex ? Uin[gc] : 0.;
+  const R uf_f = __shfl_down(uf, 1, 64);
+  const double lf_f = __shfl_down(lf, 1, 64);
+  const R fb = (R)(-1.0 * uf * lf * dt), ff = (R)(1.0 * uf_f * lf_f * dt);
+  const R dxr = (R)dx, dotr = (R)dotxf;
+  R(*stin)[kTileP] = stg[0];
+  R(*stu)[kTileP] = stg[1];
+  R(*stc)[kTileP] = stg[2];
+  R nxt = gex ? Uin[gc] : R(0);
   for (int v = 0; v < nvu; ++v) {
     stin[tr][tc] = nxt;
     __syncthreads();
-    if (v + 1 < nvu) nxt = gex ? Uin[(v + 1) * NC + gc] : 0.;
-    const double u0 = stin[lane][w];
-    const double phi = ex ? u0 / den : 0.;
-    const double phim = __shfl_up(phi, 1, 64);
-    const double gf = face_grad(inner, phi, phim, dx);
-    const double gfp = __shfl_down(gf, 1, 64);
-    const double g = o2 ? limited_grad<LIM, true>(gf, gfp) : 0.;
-    const double gm = __shfl_up(g, 1, 64);
-    const double phif = inner ? (upm ? phim + order2 * dotxf * gm : phi + order2 * dotxf * g) : 0.;
-    const double phifp = __shfl_down(phif, 1, 64);
-    const double flux_b = -1.0 * uf * lf * dt * phif;
-    const double flux_f = 1.0 * uf_f * lf_f * dt * phifp;
+    if (v + 1 < nvu) nxt = gex ? Uin[(v + 1) * NC + gc] : R(0);
+    const R u0 = stin[lane][w];
+    const R phi = ex ? qdiv(u0, den) : R(0);
+    const R phim = __shfl_up(phi, 1, 64);
+    const R gf = face_grad(inner, phi, phim, dxr);
+    const R gfp = __shfl_down(gf, 1, 64);
+    const R g = o2 ? limited_grad<LIM, true>(gf, gfp) : R(0);
+    const R gm = __shfl_up(g, 1, 64);
+    const R phif = inner ? (upm ? phim + order2 * dotr * gm : phi + order2 * dotr * g) : R(0);
+    const R phifp = __shfl_down(phif, 1, 64);
+    const R flux_b = fb * phif;
+    const R flux_f = ff * phifp;
     // cell faces in order: bottom (back), right, top (front), left
-    double un = u0 - (flux_b + flux_f);
-    if (v >= ne && v < 3 * ne && un < 0.) un = 0.;  // RemapADIService.cc:953-958
+    R un = u0 - (flux_b + flux_f);
+    if (v >= ne && v < 3 * ne && un < R(0)) un = R(0);  // RemapADIService.cc:953-958
     stu[lane][w] = un;
     const bool mass = (v >= ne && v < 2 * ne);
     if (mass) {
-      const double dual = 0.5 * flux_b * -1.0 + 0.5 * flux_f * 1.0;
-      stc[lane][w] = 0.5 * dual;  // fcontrib (DualPhaseRemap.cc:168-185)
+      const R dual = R(0.5) * flux_b * R(-1.0) + R(0.5) * flux_f * R(1.0);
+      stc[lane][w] = R(0.5) * dual;  // fcontrib (DualPhaseRemap.cc:168-185)
     }
     __syncthreads();
     if (gout) {
@@ -733,8 +747,8 @@ __global__ __launch_bounds__(64) void k_cell_sweep_y_2d(Params P, DevScalars* S,
 // 1836-1915) and the CFL candidate of computeHydroDeltaT (2047-2061) reduced
 // to one minimum per block; k_dt_finalize finishes computeDeltaT (2076-2191).
 // ---------------------------------------------------------------------------
-template <int BLOCK, int NEM>
-__global__ __launch_bounds__(BLOCK) void k_cell_final_2d(Params P, DevScalars* S, CellFinalArgs A) {
+template <class R, int BLOCK, int NEM>
+__global__ __launch_bounds__(BLOCK) void k_cell_final_2d(Params P, DevScalars* S, CellFinalArgsT<R> A) {
   if (S->stop || S->err) return;
   __shared__ double red[BLOCK / 64];
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -744,25 +758,25 @@ __global__ __launch_bounds__(BLOCK) void k_cell_final_2d(Params P, DevScalars* S
     const int i = (int)(c % P.nx), j = (int)(c / P.nx);
     const long NC = P.nc;
     const int ne = P.ne;
-    const double thr = P.remap_threshold;
+    const R thr = (R)P.remap_threshold;
     // Euler volume: the corner-normal formula on the initial coordinates.
     double x0[4] = {node_x0(P, i), node_x0(P, i + 1), node_x0(P, i + 1), node_x0(P, i)};
     double y0[4] = {node_y0(P, j), node_y0(P, j), node_y0(P, j + 1), node_y0(P, j + 1)};
-    const double vol = quad_volume(x0, y0);
-    double Uvol[NEM], Umas[NEM], Uene[NEM];
+    const R vol = (R)quad_volume(x0, y0);
+    R Uvol[NEM], Umas[NEM], Uene[NEM];
 #pragma unroll
     for (int k = 0; k < NEM; ++k) {
-      Uvol[k] = Umas[k] = Uene[k] = 0.;
+      Uvol[k] = Umas[k] = Uene[k] = R(0);
       if (k < ne) {
         Uvol[k] = A.U[k * NC + c];
         Umas[k] = A.U[(ne + k) * NC + c];
         Uene[k] = A.U[(2 * ne + k) * NC + c];
       }
     }
-    const double Uq = A.U[(3 * ne) * NC + c];
+    const R Uq = A.U[(3 * ne) * NC + c];
     const int old_mask = A.mask[c];
     // cell status per environment (110-143)
-    double volt = 0.;
+    R volt = R(0);
 #pragma unroll
     for (int k = 0; k < NEM; ++k)
       if (k < ne) volt += Uvol[k];
@@ -770,120 +784,120 @@ __global__ __launch_bounds__(BLOCK) void k_cell_final_2d(Params P, DevScalars* S
 #pragma unroll
     for (int k = 0; k < NEM; ++k) {
       if (k >= ne) break;
-      const double fvol = Uvol[k] / vol;
+      const R fvol = qdiv(Uvol[k], vol);
       const bool in_env = (old_mask >> k) & 1;
       if (fvol < thr && in_env) {
         mask &= ~(1 << k);
       } else if (fvol > thr && !in_env) {
-        const double fn = Uvol[k] / volt;
-        if (fn > thr && Umas[k] != 0.) mask |= (1 << k);
+        const R fn = qdiv(Uvol[k], volt);
+        if (fn > thr && Umas[k] != R(0)) mask |= (1 << k);
       }
     }
     const int nbe = __popc(mask);
     // "moy"
-    double masset = 0.;
-    volt = 0.;
+    R masset = R(0);
+    volt = R(0);
 #pragma unroll
     for (int k = 0; k < NEM; ++k)
       if (k < ne) {
         volt += Uvol[k];
         masset += Umas[k];
       }
-    const double unsurvolt = 1. / volt;
-    const double unsurvol = 1. / vol;
-    double fv[NEM], mf[NEM], dk[NEM], ek[NEM], pk[NEM], ck[NEM];
-    double somme_frac = 0.;
+    const R unsurvolt = qdiv(R(1), volt);
+    const R unsurvol = qdiv(R(1), vol);
+    R fv[NEM], mf[NEM], dk[NEM], ek[NEM], pk[NEM], ck[NEM];
+    R somme_frac = R(0);
 #pragma unroll
     for (int k = 0; k < NEM; ++k) {
-      fv[k] = 0.;
-      pk[k] = ck[k] = 0.;
+      fv[k] = R(0);
+      pk[k] = ck[k] = R(0);
       if (!((mask >> k) & 1)) continue;
-      double f = Uvol[k] * unsurvolt;
-      if (f < thr) f = 0.;
+      R f = Uvol[k] * unsurvolt;
+      if (f < thr) f = R(0);
       fv[k] = f;
       somme_frac += f;
     }
     int matcell = 0;
-    const double unsursomme_frac = 1. / somme_frac;
+    const R unsursomme_frac = qdiv(R(1), somme_frac);
 #pragma unroll
     for (int k = 0; k < NEM; ++k) {
       if (!((mask >> k) & 1)) continue;
-      const double f = fv[k] * unsursomme_frac;
-      if (f > 0.) matcell++;
+      const R f = fv[k] * unsursomme_frac;
+      if (f > R(0)) matcell++;
       fv[k] = f;
     }
 #pragma unroll
     for (int k = 0; k < NEM; ++k) {
       // a zero total mass leaves the mass fractions as they were (513-538)
-      mf[k] = 0.;
-      if (nbe == 1) mf[k] = 1.0;
+      mf[k] = R(0);
+      if (nbe == 1) mf[k] = R(1.0);
       else if (k < ne && ((old_mask >> k) & 1) && ((mask >> k) & 1)) mf[k] = A.env_mf[k * NC + c];
     }
-    if (masset != 0.) {
-      const double unsurmasset = 1. / masset;
-      double fmasset = 0.;
+    if (masset != R(0)) {
+      const R unsurmasset = qdiv(R(1), masset);
+      R fmasset = R(0);
 #pragma unroll
       for (int k = 0; k < NEM; ++k) {
         if (!((mask >> k) & 1)) continue;
-        double m = Umas[k] * unsurmasset;
-        if (fv[k] < thr) m = 0.;
+        R m = Umas[k] * unsurmasset;
+        if (fv[k] < thr) m = R(0);
         fmasset += m;
         mf[k] = m;
       }
-      if (fmasset != 0.) {
-        const double unsurfmasset = 1. / fmasset;
+      if (fmasset != R(0)) {
+        const R unsurfmasset = qdiv(R(1), fmasset);
 #pragma unroll
         for (int k = 0; k < NEM; ++k)
           if ((mask >> k) & 1) mf[k] = mf[k] * unsurfmasset;
       }
     }
-    double density_nplus1 = 0.;
+    R density_nplus1 = R(0);
     int kpure = 0;
 #pragma unroll
     for (int k = 0; k < NEM; ++k) {
-      dk[k] = 0.;
+      dk[k] = R(0);
       if (!((mask >> k) & 1)) continue;
       kpure = k;
       if (fv[k] > thr) {
-        const double vol_norm = Uvol[k] * vol * unsurvolt;
-        dk[k] = Umas[k] / vol_norm;
+        const R vol_norm = Uvol[k] * vol * unsurvolt;
+        dk[k] = qdiv(Umas[k], vol_norm);
       }
       density_nplus1 += fv[k] * dk[k];
     }
-    double rho_g;
+    R rho_g;
     if (matcell > 1) rho_g = density_nplus1;
     else if (nbe == 1) {
-      rho_g = 0.;
+      rho_g = R(0);
 #pragma unroll
       for (int k = 0; k < NEM; ++k)
         if (k == kpure) rho_g = dk[k];
     } else {
-      rho_g = A.m[c] / A.Vlag[c];  // left as updateDensity set it (m/V)
+      rho_g = qdiv(A.m[c], A.Vlag[c]);  // left as updateDensity set it (m/V)
     }
-    double mass_g = vol * density_nplus1;
-    double energie = 0., pseudo = 0.;
-    const double pv = Uq * unsurvol;
-    double mk[NEM];
+    R mass_g = vol * density_nplus1;
+    R energie = R(0), pseudo = R(0);
+    const R pv = Uq * unsurvol;
+    R mk[NEM];
 #pragma unroll
     for (int k = 0; k < NEM; ++k) {
-      ek[k] = 0.;
-      mk[k] = 0.;
+      ek[k] = R(0);
+      mk[k] = R(0);
       if (!((mask >> k) & 1)) continue;
-      if (fv[k] > thr && Umas[k] != 0.) ek[k] = Uene[k] / Umas[k];
+      if (fv[k] > thr && Umas[k] != R(0)) ek[k] = qdiv(Uene[k], Umas[k]);
       energie += mf[k] * ek[k];
       mk[k] = mf[k] * mass_g;
       pseudo += fv[k] * pv;
     }
-    double e_g = energie, q_g = pseudo;
-    double p_g = 0., c_g = 1.e-20;
+    R e_g = energie, q_g = pseudo;
+    R p_g = R(0), c_g = R(1.e-20);
     if (nbe == 1) {
       // pure cell: the environment slot is the cell slot
 #pragma unroll
       for (int k = 0; k < NEM; ++k)
         if (k == kpure) mass_g = mk[k];
-      if (rho_g < 0. || e_g < 0.) {
-        e_g = 0.;
-        rho_g = 0.;
+      if (rho_g < R(0) || e_g < R(0)) {
+        e_g = R(0);
+        rho_g = R(0);
       }
 #pragma unroll
       for (int k = 0; k < NEM; ++k)
@@ -892,11 +906,11 @@ __global__ __launch_bounds__(BLOCK) void k_cell_final_2d(Params P, DevScalars* S
 #pragma unroll
       for (int k = 0; k < NEM; ++k) {
         if (!((mask >> k) & 1)) continue;
-        if (dk[k] < 0. || ek[k] < 0.) {
-          ek[k] = 0.;
-          dk[k] = 0.;
-          fv[k] = 0.;
-          mf[k] = 0.;
+        if (dk[k] < R(0) || ek[k] < R(0)) {
+          ek[k] = R(0);
+          dk[k] = R(0);
+          fv[k] = R(0);
+          mf[k] = R(0);
         }
       }
       // EOS per environment, then computePressionMoyenne
@@ -933,15 +947,15 @@ __global__ __launch_bounds__(BLOCK) void k_cell_final_2d(Params P, DevScalars* S
     // CFL candidate: L / (c + max over the cell nodes of |u|)
     const long n0 = i + (long)nnx * j;
     const long nd[4] = {n0, n0 + 1, n0 + 1 + nnx, n0 + nnx};
-    double vmax = 0.;
+    R vmax = R(0);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const double ux = A.ux[nd[q]], uy = A.uy[nd[q]];
-      const double un = sqrt(ux * ux + uy * uy + 0. * 0.);
+      const R ux = A.ux[nd[q]], uy = A.uy[nd[q]];
+      const R un = sqrt(ux * ux + uy * uy + R(0) * R(0));
       vmax = (un < vmax) ? vmax : un;
     }
     // std::min(minimum_aux, dx_sound) fold: a NaN candidate is skipped
-    dx_sound = smin(dx_sound, A.L[c] / (c_g + vmax));
+    dx_sound = smin(dx_sound, (double)qdiv(A.L[c], c_g + vmax));
   }
   double v = dx_sound;
   for (int off = 32; off > 0; off >>= 1) v = smin(v, __shfl_down(v, off, 64));
@@ -1001,8 +1015,9 @@ __global__ __launch_bounds__(BLOCK) void k_dt_finalize(Params P, DevScalars* S, 
 // (PrepareRemap.cc:411-433); the last sweep ends with the nodal velocity
 // u = U_dual / m_dual of remapVariables (RemapADIFinal.cc:660-672).
 // ---------------------------------------------------------------------------
-__device__ inline double node_mass_from_cells(const Params& P, const double* m, int i, int j) {
-  double msum = 0.;
+template <class R>
+__device__ inline R node_mass_from_cells(const Params& P, const R* m, int i, int j) {
+  R msum = R(0);
   const int cdi[4] = {-1, 0, -1, 0}, cdj[4] = {-1, -1, 0, 0};
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
@@ -1010,7 +1025,7 @@ __device__ inline double node_mass_from_cells(const Params& P, const double* m, 
     if (ci < 0 || ci >= P.nx || cj < 0 || cj >= P.ny) continue;
     msum += m[ci + (long)P.nx * cj];
   }
-  return 0.25 * msum;
+  return R(0.25) * msum;
 }
 
 template <int D, bool FIRST, int LIM>
@@ -1158,8 +1173,8 @@ __global__ void k_node_sweep_2d(Params P, DevScalars* S, NodeSweepArgs A) {
 // its phi_dual and its limited gradient once and reads its neighbours' values
 // across lanes.  front(i) sums the cells (i, j-1), (i, j) in the same order as
 // back(i+1) does, so back is the front of the previous lane.
-template <bool FIRST, int NEM, int LIM>
-__global__ __launch_bounds__(256) void k_node_sweep_x_2d(Params P, DevScalars* S, NodeSweepArgs A) {
+template <class R, bool FIRST, int NEM, int LIM>
+__global__ __launch_bounds__(256) void k_node_sweep_x_2d(Params P, DevScalars* S, NodeSweepArgsT<R> A) {
   if (S->stop || S->err) return;
   const int lane = threadIdx.x & 63;
   const int N = P.nx;
@@ -1177,11 +1192,12 @@ __global__ __launch_bounds__(256) void k_node_sweep_x_2d(Params P, DevScalars* S
   const long NC = P.nc;
   const int ne = P.ne;
   const double dt = S->dt;
-  double U0 = 0., U1 = 0., U3 = 0., xd = 0., p0 = 0., p1 = 0.;
+  R U0 = R(0), U1 = R(0), U3 = R(0), p0 = R(0), p1 = R(0);
+  double xd = 0.;
   if (ex) {
     xd = A.xnx[nid];
     if (FIRST) {
-      const double mn = node_mass_from_cells(P, A.m, i, j);
+      const R mn = node_mass_from_cells(P, A.m, i, j);
       p0 = A.ux[nid];
       p1 = A.uy[nid];
       U0 = mn * p0;
@@ -1193,8 +1209,8 @@ __global__ __launch_bounds__(256) void k_node_sweep_x_2d(Params P, DevScalars* S
       U3 = A.Ud3[nid];
       // nodes inner in the first direction carry phi = U/m (DualPhaseRemap.cc:494-523)
       if (inner_t) {
-        p0 = U0 / U3;
-        p1 = U1 / U3;
+        p0 = qdiv(U0, U3);
+        p1 = qdiv(U1, U3);
       } else {
         p0 = A.ux[nid];
         p1 = A.uy[nid];
@@ -1202,7 +1218,7 @@ __global__ __launch_bounds__(256) void k_node_sweep_x_2d(Params P, DevScalars* S
     }
   }
   // front dual mass flux: cells (i, j-1), (i, j), environments in order
-  double front = 0.;
+  R front = R(0);
   if (ex && i < P.nx) {
 #pragma unroll
     for (int sh = -1; sh <= 0; ++sh) {
@@ -1217,16 +1233,16 @@ __global__ __launch_bounds__(256) void k_node_sweep_x_2d(Params P, DevScalars* S
     }
   }
   const double xdm = __shfl_up(xd, 1, 64), xdp = __shfl_down(xd, 1, 64);
-  const double p0m = __shfl_up(p0, 1, 64), p0p = __shfl_down(p0, 1, 64);
-  const double p1m = __shfl_up(p1, 1, 64), p1p = __shfl_down(p1, 1, 64);
-  const double back = __shfl_up(front, 1, 64);
+  const R p0m = __shfl_up(p0, 1, 64), p0p = __shfl_down(p0, 1, 64);
+  const R p1m = __shfl_up(p1, 1, 64), p1p = __shfl_down(p1, 1, 64);
+  const R back = __shfl_up(front, 1, 64);
   // limited nodal gradient (computeDualGradPhi_LimC, DualPhaseRemap.cc:700-779)
-  double g0 = 0., g1 = 0.;
+  R g0 = R(0), g1 = R(0);
   if (P.order2 > 0 && ex) {
     if (inner_d) {
-      const double dxf = xdp - xd, dxb = xd - xdm;
-      const double gf0 = (p0p - p0) / dxf, gb0 = (p0 - p0m) / dxb;
-      const double gf1 = (p1p - p1) / dxf, gb1 = (p1 - p1m) / dxb;
+      const R dxf = (R)(xdp - xd), dxb = (R)(xd - xdm);
+      const R gf0 = qdiv(p0p - p0, dxf), gb0 = qdiv(p0 - p0m, dxb);
+      const R gf1 = qdiv(p1p - p1, dxf), gb1 = qdiv(p1 - p1m, dxb);
       g0 = limited_grad<LIM, false>(gb0, gf0);
       g1 = limited_grad<LIM, false>(gb1, gf1);
       if (!inner_t && out) {  // kept for the sweep in the other direction
@@ -1238,39 +1254,39 @@ __global__ __launch_bounds__(256) void k_node_sweep_x_2d(Params P, DevScalars* S
       g1 = A.dg1[nid];
     }
   }
-  const double g0m = __shfl_up(g0, 1, 64), g0p = __shfl_down(g0, 1, 64);
-  const double g1m = __shfl_up(g1, 1, 64), g1p = __shfl_down(g1, 1, 64);
+  const R g0m = __shfl_up(g0, 1, 64), g0p = __shfl_down(g0, 1, 64);
+  const R g1m = __shfl_up(g1, 1, 64), g1p = __shfl_down(g1, 1, 64);
   if (!out) return;
   if (inner_d) {
     U3 += back - front;
-    const double order2 = (double)P.order2;
-    const double ufront = 0.5 * (p0 + p0);  // DualPhaseRemap.cc:391
-    const int signfront = (ufront > 0) ? 1 : -1;
+    const R order2 = (R)P.order2;
+    const R ufront = R(0.5) * (p0 + p0);  // DualPhaseRemap.cc:391
+    const int signfront = (ufront > R(0)) ? 1 : -1;
     const double xfr = xdp - xd;
-    const bool sf = (front < 0.);
-    const double tf = signfront * xfr - dt * ufront;
-    const double FV0 = (sf ? p0p : p0) + order2 * (0.5 * (sf ? g0p : g0) * tf);
-    const double FV1 = (sf ? p1p : p1) + order2 * (0.5 * (sf ? g1p : g1) * tf);
-    const double uback = 0.5 * (p0m + p0);
-    const int signback = (uback > 0) ? 1 : -1;
+    const bool sf = (front < R(0));
+    const R tf = (R)(signfront * xfr - dt * ufront);
+    const R FV0 = (sf ? p0p : p0) + order2 * (R(0.5) * (sf ? g0p : g0) * tf);
+    const R FV1 = (sf ? p1p : p1) + order2 * (R(0.5) * (sf ? g1p : g1) * tf);
+    const R uback = R(0.5) * (p0m + p0);
+    const int signback = (uback > R(0)) ? 1 : -1;
     const double xbk = xd - xdm;
-    const bool sb = (back > 0.);
-    const double tb = signback * xbk - dt * uback;
-    const double BV0 = (sb ? p0m : p0) + order2 * (0.5 * (sb ? g0m : g0) * tb);
-    const double BV1 = (sb ? p1m : p1) + order2 * (0.5 * (sb ? g1m : g1) * tb);
+    const bool sb = (back > R(0));
+    const R tb = (R)(signback * xbk - dt * uback);
+    const R BV0 = (sb ? p0m : p0) + order2 * (R(0.5) * (sb ? g0m : g0) * tb);
+    const R BV1 = (sb ? p1m : p1) + order2 * (R(0.5) * (sb ? g1m : g1) * tb);
     U0 += back * BV0 - front * FV0;
     U1 += back * BV1 - front * FV1;
-    const double thresold = 1.e-300;
-    if (fabs(U0) < thresold) U0 = 0.;
-    if (fabs(U1) < thresold) U1 = 0.;
+    const R thresold = (R)1.e-300;  // 0 in float: the filter only acts in double
+    if (fabs(U0) < thresold) U0 = R(0);
+    if (fabs(U1) < thresold) U1 = R(0);
   }
   if (FIRST) {
     st_stream(&A.Ud0[nid], U0);
     st_stream(&A.Ud1[nid], U1);
     st_stream(&A.Ud3[nid], U3);
-  } else if (U3 != 0.) {
-    st_stream(&A.uox[nid], U0 / U3);
-    st_stream(&A.uoy[nid], U1 / U3);
+  } else if (U3 != R(0)) {
+    st_stream(&A.uox[nid], qdiv(U0, U3));
+    st_stream(&A.uoy[nid], qdiv(U1, U3));
   } else {
     st_stream(&A.uox[nid], A.ux[nid]);
     st_stream(&A.uoy[nid], A.uy[nid]);
@@ -1283,9 +1299,10 @@ __global__ __launch_bounds__(256) void k_node_sweep_x_2d(Params P, DevScalars* S
 // flux; the tile is transposed through LDS, each wavefront runs the lane
 // arithmetic of the x dual sweep on one column, and the increments of the
 // dual state travel back through LDS to be added and stored row-major.
-template <bool FIRST, int NEM, int LIM>
-__global__ __launch_bounds__(1024) void k_node_sweep_ytile_2d(Params P, DevScalars* S, NodeSweepArgs A) {
-  __shared__ double lds[4][kTileR][kTileP];
+template <class R, bool FIRST, int NEM, int LIM>
+__global__ __launch_bounds__(1024) void k_node_sweep_ytile_2d(Params P, DevScalars* S, NodeSweepArgsT<R> A) {
+  __shared__ double ldx[kTileR][kTileP];   // coordinate along y
+  __shared__ R lds[3][kTileR][kTileP];     // phi_dual x, y and front flux; then the increments
   if (S->stop || S->err) return;  // uniform over the grid
   const int nx = P.nx, N = P.ny, nnx = nx + 1;
   const int ncolblk = (nnx + kTileW - 1) / kTileW;
@@ -1303,14 +1320,15 @@ __global__ __launch_bounds__(1024) void k_node_sweep_ytile_2d(Params P, DevScala
   const bool gex = gi <= nx && gs >= 0 && gs <= N;
   const bool gout = gex && tr >= 2 && tr <= 61;
   const long gq = gex ? gi + (long)nnx * gs : 0;
-  double U0 = 0., U1 = 0., U3 = 0.;
+  R U0 = R(0), U1 = R(0), U3 = R(0);
   {
-    double xd = 0., p0 = 0., p1 = 0., f = 0.;
+    double xd = 0.;
+    R p0 = R(0), p1 = R(0), f = R(0);
     if (gex) {
       const bool inner_t = (gi >= 1 && gi <= nx - 1);
       xd = A.xny[gq];
       if (FIRST) {
-        const double mn = node_mass_from_cells(P, A.m, gi, gs);
+        const R mn = node_mass_from_cells(P, A.m, gi, gs);
         p0 = A.ux[gq];
         p1 = A.uy[gq];
         U0 = mn * p0;
@@ -1321,8 +1339,8 @@ __global__ __launch_bounds__(1024) void k_node_sweep_ytile_2d(Params P, DevScala
         U1 = A.Ud1[gq];
         U3 = A.Ud3[gq];
         if (inner_t) {  // DualPhaseRemap.cc:494-523
-          p0 = U0 / U3;
-          p1 = U1 / U3;
+          p0 = qdiv(U0, U3);
+          p1 = qdiv(U1, U3);
         } else {
           p0 = A.ux[gq];
           p1 = A.uy[gq];
@@ -1342,10 +1360,10 @@ __global__ __launch_bounds__(1024) void k_node_sweep_ytile_2d(Params P, DevScala
         }
       }
     }
-    lds[0][tr][tc] = xd;
-    lds[1][tr][tc] = p0;
-    lds[2][tr][tc] = p1;
-    lds[3][tr][tc] = f;
+    ldx[tr][tc] = xd;
+    lds[0][tr][tc] = p0;
+    lds[1][tr][tc] = p1;
+    lds[2][tr][tc] = f;
   }
   __syncthreads();
   // ---- transposed pass: column i, node row s
@@ -1354,19 +1372,20 @@ __global__ __launch_bounds__(1024) void k_node_sweep_ytile_2d(Params P, DevScala
   const bool out = ex && lane >= 2 && lane <= 61;
   const bool inner_t = (i >= 1 && i <= nx - 1);
   const bool inner_d = (s >= 1 && s <= N - 1);
-  const double xd = lds[0][lane][w], p0 = lds[1][lane][w], p1 = lds[2][lane][w];
-  const double front = lds[3][lane][w];
+  const double xd = ldx[lane][w];
+  const R p0 = lds[0][lane][w], p1 = lds[1][lane][w];
+  const R front = lds[2][lane][w];
   const double xdm = __shfl_up(xd, 1, 64), xdp = __shfl_down(xd, 1, 64);
-  const double p0m = __shfl_up(p0, 1, 64), p0p = __shfl_down(p0, 1, 64);
-  const double p1m = __shfl_up(p1, 1, 64), p1p = __shfl_down(p1, 1, 64);
-  const double back = __shfl_up(front, 1, 64);
-  double g0 = 0., g1 = 0.;
+  const R p0m = __shfl_up(p0, 1, 64), p0p = __shfl_down(p0, 1, 64);
+  const R p1m = __shfl_up(p1, 1, 64), p1p = __shfl_down(p1, 1, 64);
+  const R back = __shfl_up(front, 1, 64);
+  R g0 = R(0), g1 = R(0);
   if (P.order2 > 0 && ex) {
     const long q = i + (long)nnx * s;
     if (inner_d) {
-      const double dxf = xdp - xd, dxb = xd - xdm;
-      const double gf0 = (p0p - p0) / dxf, gb0 = (p0 - p0m) / dxb;
-      const double gf1 = (p1p - p1) / dxf, gb1 = (p1 - p1m) / dxb;
+      const R dxf = (R)(xdp - xd), dxb = (R)(xd - xdm);
+      const R gf0 = qdiv(p0p - p0, dxf), gb0 = qdiv(p0 - p0m, dxb);
+      const R gf1 = qdiv(p1p - p1, dxf), gb1 = qdiv(p1 - p1m, dxb);
       g0 = limited_grad<LIM, false>(gb0, gf0);
       g1 = limited_grad<LIM, false>(gb1, gf1);
       if (!inner_t && out) {  // kept for the sweep in the other direction
@@ -1378,26 +1397,26 @@ __global__ __launch_bounds__(1024) void k_node_sweep_ytile_2d(Params P, DevScala
       g1 = A.dg1[q];
     }
   }
-  const double g0m = __shfl_up(g0, 1, 64), g0p = __shfl_down(g0, 1, 64);
-  const double g1m = __shfl_up(g1, 1, 64), g1p = __shfl_down(g1, 1, 64);
-  double d0 = 0., d1 = 0., d3 = 0.;
+  const R g0m = __shfl_up(g0, 1, 64), g0p = __shfl_down(g0, 1, 64);
+  const R g1m = __shfl_up(g1, 1, 64), g1p = __shfl_down(g1, 1, 64);
+  R d0 = R(0), d1 = R(0), d3 = R(0);
   if (out && inner_d) {
-    const double order2 = (double)P.order2;
+    const R order2 = (R)P.order2;
     d3 = back - front;
-    const double ufront = 0.5 * (p1 + p1);  // DualPhaseRemap.cc:391
-    const int signfront = (ufront > 0) ? 1 : -1;
+    const R ufront = R(0.5) * (p1 + p1);  // DualPhaseRemap.cc:391
+    const int signfront = (ufront > R(0)) ? 1 : -1;
     const double xfr = xdp - xd;
-    const bool sf = (front < 0.);
-    const double tf = signfront * xfr - dt * ufront;
-    const double FV0 = (sf ? p0p : p0) + order2 * (0.5 * (sf ? g0p : g0) * tf);
-    const double FV1 = (sf ? p1p : p1) + order2 * (0.5 * (sf ? g1p : g1) * tf);
-    const double uback = 0.5 * (p1m + p1);
-    const int signback = (uback > 0) ? 1 : -1;
+    const bool sf = (front < R(0));
+    const R tf = (R)(signfront * xfr - dt * ufront);
+    const R FV0 = (sf ? p0p : p0) + order2 * (R(0.5) * (sf ? g0p : g0) * tf);
+    const R FV1 = (sf ? p1p : p1) + order2 * (R(0.5) * (sf ? g1p : g1) * tf);
+    const R uback = R(0.5) * (p1m + p1);
+    const int signback = (uback > R(0)) ? 1 : -1;
     const double xbk = xd - xdm;
-    const bool sb = (back > 0.);
-    const double tb = signback * xbk - dt * uback;
-    const double BV0 = (sb ? p0m : p0) + order2 * (0.5 * (sb ? g0m : g0) * tb);
-    const double BV1 = (sb ? p1m : p1) + order2 * (0.5 * (sb ? g1m : g1) * tb);
+    const bool sb = (back > R(0));
+    const R tb = (R)(signback * xbk - dt * uback);
+    const R BV0 = (sb ? p0m : p0) + order2 * (R(0.5) * (sb ? g0m : g0) * tb);
+    const R BV1 = (sb ? p1m : p1) + order2 * (R(0.5) * (sb ? g1m : g1) * tb);
     d0 = back * BV0 - front * FV0;
     d1 = back * BV1 - front * FV1;
   }
@@ -1412,17 +1431,17 @@ __global__ __launch_bounds__(1024) void k_node_sweep_ytile_2d(Params P, DevScala
     U3 += lds[2][tr][tc];
     U0 += lds[0][tr][tc];
     U1 += lds[1][tr][tc];
-    const double thresold = 1.e-300;
-    if (fabs(U0) < thresold) U0 = 0.;
-    if (fabs(U1) < thresold) U1 = 0.;
+    const R thresold = (R)1.e-300;  // 0 in float: the filter only acts in double
+    if (fabs(U0) < thresold) U0 = R(0);
+    if (fabs(U1) < thresold) U1 = R(0);
   }
   if (FIRST) {
     st_stream(&A.Ud0[gq], U0);
     st_stream(&A.Ud1[gq], U1);
     st_stream(&A.Ud3[gq], U3);
-  } else if (U3 != 0.) {
-    st_stream(&A.uox[gq], U0 / U3);
-    st_stream(&A.uoy[gq], U1 / U3);
+  } else if (U3 != R(0)) {
+    st_stream(&A.uox[gq], qdiv(U0, U3));
+    st_stream(&A.uoy[gq], qdiv(U1, U3));
   } else {
     st_stream(&A.uox[gq], A.ux[gq]);
     st_stream(&A.uoy[gq], A.uy[gq]);
@@ -1484,8 +1503,8 @@ __global__ __launch_bounds__(64) void k_node_sweep_y_2d(Params P, DevScalars* S,
         U1 = A.Ud1[q];
         U3 = A.Ud3[q];
         if (inner_t) {
-          p0 = U0 / U3;
-          p1 = U1 / U3;
+          p0 = qdiv(U0, U3);
+          p1 = qdiv(U1, U3);
         } else {
           p0 = A.ux[q];
           p1 = A.uy[q];
@@ -1605,28 +1624,31 @@ void launch_begin_iteration(hipStream_t st, DevScalars* S) {
   hipLaunchKernelGGL(k_begin_iteration, dim3(1), dim3(1), 0, st, S);
 }
 
-void launch_node_lagrange_2d(hipStream_t st, const Params& P, DevScalars* S, const NodeLagArgs& A,
+template <class R>
+void launch_node_lagrange_2d(hipStream_t st, const Params& P, DevScalars* S, const NodeLagArgsT<R>& A,
                              bool vel, bool bc, bool pos) {
   const int B = 256;
   dim3 g(nblocks(P.nn, B)), b(B);
-  if (vel && bc && pos) hipLaunchKernelGGL((k_node_lagrange_2d<true, true, true>), g, b, 0, st, P, S, A);
-  else if (vel && !bc && !pos) hipLaunchKernelGGL((k_node_lagrange_2d<true, false, false>), g, b, 0, st, P, S, A);
-  else if (!vel && bc && !pos) hipLaunchKernelGGL((k_node_lagrange_2d<false, true, false>), g, b, 0, st, P, S, A);
-  else if (!vel && !bc && pos) hipLaunchKernelGGL((k_node_lagrange_2d<false, false, true>), g, b, 0, st, P, S, A);
+  if (vel && bc && pos) hipLaunchKernelGGL((k_node_lagrange_2d<R, true, true, true>), g, b, 0, st, P, S, A);
+  else if (vel && !bc && !pos) hipLaunchKernelGGL((k_node_lagrange_2d<R, true, false, false>), g, b, 0, st, P, S, A);
+  else if (!vel && bc && !pos) hipLaunchKernelGGL((k_node_lagrange_2d<R, false, true, false>), g, b, 0, st, P, S, A);
+  else if (!vel && !bc && pos) hipLaunchKernelGGL((k_node_lagrange_2d<R, false, false, true>), g, b, 0, st, P, S, A);
 }
 
-void launch_cell_lagrange_2d(hipStream_t st, const Params& P, DevScalars* S, const CellLagArgs& A) {
+template <class R>
+void launch_cell_lagrange_2d(hipStream_t st, const Params& P, DevScalars* S, const CellLagArgsT<R>& A) {
   const int B = 256;
   dim3 g(nblocks(P.nc, B)), b(B);
-#define L_(N)                                                                                  \
-  if (P.pseudo_centree) hipLaunchKernelGGL((k_cell_lagrange_2d<true, N>), g, b, 0, st, P, S, A); \
-  else hipLaunchKernelGGL((k_cell_lagrange_2d<false, N>), g, b, 0, st, P, S, A)
+#define L_(N)                                                                                     \
+  if (P.pseudo_centree) hipLaunchKernelGGL((k_cell_lagrange_2d<R, true, N>), g, b, 0, st, P, S, A); \
+  else hipLaunchKernelGGL((k_cell_lagrange_2d<R, false, N>), g, b, 0, st, P, S, A)
   MHG_NEM_DISPATCH(P.ne, L_);
 #undef L_
 }
 
 // Launch variants of the y sweep (strip length, variables per thread, rows
-// per load chunk), chosen per mesh by the autotuner (mhg_autotune).
+// per load chunk of the column walker; strip 0 = LDS tiles), chosen per mesh
+// by the autotuner (mhg_autotune).
 struct YVariant {
   int strip, vg, r;
 };
@@ -1647,66 +1669,61 @@ const char* cell_sweep_y_variant_name(int v) {
     else F(2);                   \
   } while (0)
 
-template <int STRIP, int VG, int R>
+template <int STRIP, int VG, int RR>
 static void launch_y(hipStream_t st, const Params& P, DevScalars* S, const CellSweepArgs& A, bool first) {
   const long ntiles = (long)nblocks(P.nx, 64) * ((P.ny + STRIP - 1) / STRIP) * ((P.nvu + VG - 1) / VG);
   dim3 g((unsigned)ntiles), b(64);
-#define YL_(LM)                                                                                   \
-  if (first) hipLaunchKernelGGL((k_cell_sweep_y_2d<true, STRIP, VG, R, LM>), g, b, 0, st, P, S, A); \
-  else hipLaunchKernelGGL((k_cell_sweep_y_2d<false, STRIP, VG, R, LM>), g, b, 0, st, P, S, A)
+#define YL_(LM)                                                                                    \
+  if (first) hipLaunchKernelGGL((k_cell_sweep_y_2d<true, STRIP, VG, RR, LM>), g, b, 0, st, P, S, A); \
+  else hipLaunchKernelGGL((k_cell_sweep_y_2d<false, STRIP, VG, RR, LM>), g, b, 0, st, P, S, A)
   MHG_LIM_DISPATCH(P.lim, YL_);
 #undef YL_
 }
 
-template <int D, int NEM>
-static void launch_lane(hipStream_t st, const Params& P, DevScalars* S, const CellSweepArgs& A, bool first) {
-  dim3 g, b;
-  if (D == 0) {
-    const long waves = (long)((P.nx + 59) / 60) * P.ny;
-    g = dim3(nblocks(waves * 64, 256));
-    b = dim3(256);
-  } else {
-    g = dim3((unsigned)(((P.nx + 15) / 16) * ((P.ny + 59) / 60)));
-    b = dim3(1024);
-  }
-#define XL_(LM)                                                                                 \
-  if (first) hipLaunchKernelGGL((k_cell_sweep_lane_2d<D, NEM, true, LM>), g, b, 0, st, P, S, A); \
-  else hipLaunchKernelGGL((k_cell_sweep_lane_2d<D, NEM, false, LM>), g, b, 0, st, P, S, A)
+template <class R, int NEM>
+static void launch_lane_x(hipStream_t st, const Params& P, DevScalars* S, const CellSweepArgsT<R>& A,
+                          bool first) {
+  const long waves = (long)((P.nx + 59) / 60) * P.ny;
+  dim3 g(nblocks(waves * 64, 256)), b(256);
+#define XL_(LM)                                                                                    \
+  if (first) hipLaunchKernelGGL((k_cell_sweep_lane_2d<R, 0, NEM, true, LM>), g, b, 0, st, P, S, A); \
+  else hipLaunchKernelGGL((k_cell_sweep_lane_2d<R, 0, NEM, false, LM>), g, b, 0, st, P, S, A)
   MHG_LIM_DISPATCH(P.lim, XL_);
 #undef XL_
 }
 
-void launch_cell_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const CellSweepArgs& A,
+template <class R, int NEM>
+static void launch_tile_y(hipStream_t st, const Params& P, DevScalars* S, const CellSweepArgsT<R>& A,
+                          bool first) {
+  dim3 g((unsigned)(((P.nx + kTileW - 1) / kTileW) * ((P.ny + 59) / 60))), b(1024);
+#define YT_(LM)                                                                                  \
+  if (first) hipLaunchKernelGGL((k_cell_sweep_ytile_2d<R, NEM, true, LM>), g, b, 0, st, P, S, A); \
+  else hipLaunchKernelGGL((k_cell_sweep_ytile_2d<R, NEM, false, LM>), g, b, 0, st, P, S, A)
+  MHG_LIM_DISPATCH(P.lim, YT_);
+#undef YT_
+}
+
+template <class R>
+void launch_cell_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const CellSweepArgsT<R>& A,
                           int dir, bool first, int variant) {
   if (dir == 0) {
-#define X_(N) launch_lane<0, N>(st, P, S, A, first)
+#define X_(N) launch_lane_x<R, N>(st, P, S, A, first)
     MHG_NEM_DISPATCH(P.ne, X_);
 #undef X_
     return;
   }
-  switch (variant) {
-    case 0: launch_y<32, 4, 2>(st, P, S, A, first); break;
-    case 1: launch_y<8, 4, 2>(st, P, S, A, first); break;
-    case 2: launch_y<16, 4, 4>(st, P, S, A, first); break;
-    case 3: launch_y<8, 5, 2>(st, P, S, A, first); break;
-    default: {
-      dim3 g((unsigned)(((P.nx + kTileW - 1) / kTileW) * ((P.ny + 59) / 60))), b(1024);
-#define YT_(N, LM)                                                                              \
-  if (first) hipLaunchKernelGGL((k_cell_sweep_ytile_2d<N, true, LM>), g, b, 0, st, P, S, A);    \
-  else hipLaunchKernelGGL((k_cell_sweep_ytile_2d<N, false, LM>), g, b, 0, st, P, S, A)
-#define YTL_(LM)                        \
-  do {                                  \
-    if (P.ne <= 1) YT_(1, LM);          \
-    else if (P.ne <= 2) YT_(2, LM);     \
-    else if (P.ne <= 3) YT_(3, LM);     \
-    else if (P.ne <= 4) YT_(4, LM);     \
-    else YT_(8, LM);                    \
-  } while (0)
-      MHG_LIM_DISPATCH(P.lim, YTL_);
-#undef YTL_
-#undef YT_
-    } break;
+  if constexpr (std::is_same<R, double>::value) {
+    switch (variant) {
+      case 0: launch_y<32, 4, 2>(st, P, S, A, first); return;
+      case 1: launch_y<8, 4, 2>(st, P, S, A, first); return;
+      case 2: launch_y<16, 4, 4>(st, P, S, A, first); return;
+      case 3: launch_y<8, 5, 2>(st, P, S, A, first); return;
+      default: break;
+    }
   }
+#define T_(N) launch_tile_y<R, N>(st, P, S, A, first)
+  MHG_NEM_DISPATCH(P.ne, T_);
+#undef T_
 }
 
 // Launch variants of the dual sweeps: 0 = one thread per node re-forming the
@@ -1730,9 +1747,9 @@ static void launch_node_y_walk(hipStream_t st, const Params& P, DevScalars* S, c
   else hipLaunchKernelGGL((k_node_sweep_y_2d<false, NEM, LM, STRIP>), g, b, 0, st, P, S, A);
 }
 
-void launch_node_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const NodeSweepArgs& A,
-                          int dir, bool first, int variant) {
-  if (variant <= 0 || variant >= node_sweep_variants(dir)) {
+static void launch_node_fp64_variant(hipStream_t st, const Params& P, DevScalars* S, const NodeSweepArgs& A,
+                                     int dir, bool first, int variant) {
+  if (variant == 0) {
     const int B = 256;
     dim3 g(nblocks(P.nn, B)), b(B);
 #define NL_(LM)                                                                                       \
@@ -1742,43 +1759,6 @@ void launch_node_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const 
   else hipLaunchKernelGGL((k_node_sweep_2d<1, false, LM>), g, b, 0, st, P, S, A)
     MHG_LIM_DISPATCH(P.lim, NL_);
 #undef NL_
-    return;
-  }
-  if (dir == 0) {
-    const long waves = (long)((P.nx + 1 + 59) / 60) * (P.ny + 1);
-    dim3 g(nblocks(waves * 64, 256)), b(256);
-#define NX_(N, LM)                                                                            \
-  if (first) hipLaunchKernelGGL((k_node_sweep_x_2d<true, N, LM>), g, b, 0, st, P, S, A); \
-  else hipLaunchKernelGGL((k_node_sweep_x_2d<false, N, LM>), g, b, 0, st, P, S, A)
-#define NXL_(LM)                 \
-  do {                           \
-    if (P.ne <= 1) NX_(1, LM);   \
-    else if (P.ne <= 2) NX_(2, LM); \
-    else if (P.ne <= 3) NX_(3, LM); \
-    else if (P.ne <= 4) NX_(4, LM); \
-    else NX_(8, LM);             \
-  } while (0)
-    MHG_LIM_DISPATCH(P.lim, NXL_);
-#undef NXL_
-#undef NX_
-    return;
-  }
-  if (variant == 4) {
-    dim3 g((unsigned)(((P.nx + 1 + kTileW - 1) / kTileW) * ((P.ny + 1 + 59) / 60))), b(1024);
-#define NT_(N, LM)                                                                              \
-  if (first) hipLaunchKernelGGL((k_node_sweep_ytile_2d<true, N, LM>), g, b, 0, st, P, S, A);    \
-  else hipLaunchKernelGGL((k_node_sweep_ytile_2d<false, N, LM>), g, b, 0, st, P, S, A)
-#define NTL_(LM)                        \
-  do {                                  \
-    if (P.ne <= 1) NT_(1, LM);          \
-    else if (P.ne <= 2) NT_(2, LM);     \
-    else if (P.ne <= 3) NT_(3, LM);     \
-    else if (P.ne <= 4) NT_(4, LM);     \
-    else NT_(8, LM);                    \
-  } while (0)
-    MHG_LIM_DISPATCH(P.lim, NTL_);
-#undef NTL_
-#undef NT_
     return;
   }
 #define NY_(STR, N, LM) launch_node_y_walk<STR, N, LM>(st, P, S, A, first)
@@ -1800,6 +1780,52 @@ void launch_node_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const 
 #undef NYL_
 #undef NYN_
 #undef NY_
+}
+
+template <class R>
+void launch_node_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const NodeSweepArgsT<R>& A,
+                          int dir, bool first, int variant) {
+  if constexpr (std::is_same<R, double>::value) {
+    // the 5-node stencil (both directions) and the y column walkers
+    if (variant <= 0 || variant >= node_sweep_variants(dir) || (dir == 1 && variant <= 3)) {
+      launch_node_fp64_variant(st, P, S, A, dir, first, variant <= 0 || variant >= node_sweep_variants(dir) ? 0 : variant);
+      return;
+    }
+  }
+  if (dir == 0) {
+    const long waves = (long)((P.nx + 1 + 59) / 60) * (P.ny + 1);
+    dim3 g(nblocks(waves * 64, 256)), b(256);
+#define NX_(N, LM)                                                                               \
+  if (first) hipLaunchKernelGGL((k_node_sweep_x_2d<R, true, N, LM>), g, b, 0, st, P, S, A); \
+  else hipLaunchKernelGGL((k_node_sweep_x_2d<R, false, N, LM>), g, b, 0, st, P, S, A)
+#define NXL_(LM)                    \
+  do {                              \
+    if (P.ne <= 1) NX_(1, LM);      \
+    else if (P.ne <= 2) NX_(2, LM); \
+    else if (P.ne <= 3) NX_(3, LM); \
+    else if (P.ne <= 4) NX_(4, LM); \
+    else NX_(8, LM);                \
+  } while (0)
+    MHG_LIM_DISPATCH(P.lim, NXL_);
+#undef NXL_
+#undef NX_
+    return;
+  }
+  dim3 g((unsigned)(((P.nx + 1 + kTileW - 1) / kTileW) * ((P.ny + 1 + 59) / 60))), b(1024);
+#define NT_(N, LM)                                                                                 \
+  if (first) hipLaunchKernelGGL((k_node_sweep_ytile_2d<R, true, N, LM>), g, b, 0, st, P, S, A);    \
+  else hipLaunchKernelGGL((k_node_sweep_ytile_2d<R, false, N, LM>), g, b, 0, st, P, S, A)
+#define NTL_(LM)                    \
+  do {                              \
+    if (P.ne <= 1) NT_(1, LM);      \
+    else if (P.ne <= 2) NT_(2, LM); \
+    else if (P.ne <= 3) NT_(3, LM); \
+    else if (P.ne <= 4) NT_(4, LM); \
+    else NT_(8, LM);                \
+  } while (0)
+  MHG_LIM_DISPATCH(P.lim, NTL_);
+#undef NTL_
+#undef NT_
 }
 
 int cell_final_blocks(const Params& P) { return (int)nblocks(P.nc, 256); }
@@ -1835,14 +1861,31 @@ void launch_dt_finalize(hipStream_t st, const Params& P, DevScalars* S, const do
   else hipLaunchKernelGGL((k_dt_finalize<1024, false>), dim3(1), dim3(1024), 0, st, P, S, block_min, nb);
 }
 
-void launch_cell_final_2d(hipStream_t st, const Params& P, DevScalars* S, const CellFinalArgs& A,
+template <class R>
+void launch_cell_final_2d(hipStream_t st, const Params& P, DevScalars* S, const CellFinalArgsT<R>& A,
                           int dt_mode) {
   constexpr int B = 256;
   dim3 g(nblocks(P.nc, B)), b(B);
-#define F_(N) hipLaunchKernelGGL((k_cell_final_2d<B, N>), g, b, 0, st, P, S, A)
+#define F_(N) hipLaunchKernelGGL((k_cell_final_2d<R, B, N>), g, b, 0, st, P, S, A)
   MHG_NEM_DISPATCH(P.ne, F_);
 #undef F_
   if (dt_mode > 0) launch_dt_finalize(st, P, S, A.block_min, dt_mode == 2);
 }
+
+// The fused step for both storage types.
+#define MHG_INSTANTIATE(R)                                                                           \
+  template void launch_node_lagrange_2d<R>(hipStream_t, const Params&, DevScalars*,                  \
+                                           const NodeLagArgsT<R>&, bool, bool, bool);                \
+  template void launch_cell_lagrange_2d<R>(hipStream_t, const Params&, DevScalars*,                  \
+                                           const CellLagArgsT<R>&);                                  \
+  template void launch_cell_sweep_2d<R>(hipStream_t, const Params&, DevScalars*,                     \
+                                        const CellSweepArgsT<R>&, int, bool, int);                   \
+  template void launch_node_sweep_2d<R>(hipStream_t, const Params&, DevScalars*,                     \
+                                        const NodeSweepArgsT<R>&, int, bool, int);                   \
+  template void launch_cell_final_2d<R>(hipStream_t, const Params&, DevScalars*,                     \
+                                        const CellFinalArgsT<R>&, int);
+MHG_INSTANTIATE(double)
+MHG_INSTANTIATE(float)
+#undef MHG_INSTANTIATE
 
 }  // namespace mhg

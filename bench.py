@@ -61,6 +61,44 @@ def cpu_baseline(gpu_density_after, steps):
                       f"steps 1..{steps}, {t1 - t0:.1f} s"}, l1
 
 
+def timed_run(gpu, precision, args, sync):
+    """Autotune, W warm-up steps, re-initialise, then K steps timed (graph
+    replay); then, unless --no-events, the same K steps again launched one
+    kernel at a time between HIP events for the per-kernel durations."""
+    opt_kw = {"precision": precision}
+    sim = cases.triple_point(gpu, NX, NY, **opt_kw)
+    tuning = sim.autotune(10)
+    print("autotune:", tuning, file=sys.stderr)
+    if args.warmup:
+        sim.L.optional["run_async"](sim.ctx, args.warmup)
+        sim.L.optional["synchronize"](sim.ctx)
+    sim.init_triple_point(cases.TP_RHO, cases.TP_P)
+    set_timing(sim, False)
+    sync()
+    sim.L.optional["synchronize"](sim.ctx)
+    t0 = time.perf_counter()
+    r = sim.L.optional["run_async"](sim.ctx, args.steps)
+    assert r == 0, sim.L.last_error(sim.ctx)
+    sim.L.optional["synchronize"](sim.ctx)
+    sync()
+    t1 = time.perf_counter()
+    st = sim.download(["density", "cell_mass"])
+    assert st["global_iteration"] == args.steps, st["global_iteration"]
+    assert np.all(np.isfinite(st["density"]))
+    kt = {}
+    if not args.no_events:
+        sim.init_triple_point(cases.TP_RHO, cases.TP_P)
+        set_timing(sim, True)
+        r = sim.L.optional["run_async"](sim.ctx, args.steps)
+        assert r == 0, sim.L.last_error(sim.ctx)
+        sim.L.optional["synchronize"](sim.ctx)
+        kt = kernel_times(sim)
+        set_timing(sim, False)
+        st2 = sim.download(["density"])
+        assert np.array_equal(st2["density"], st["density"]), "graph and direct launches differ"
+    return {"sim": sim, "elapsed": t1 - t0, "tuning": tuning, "kt": kt, "density": st["density"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +107,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-events", action="store_true", help="do not record per-kernel events")
+    ap.add_argument("--no-precision-form", action="store_true",
+                    help="skip the FP32-bulk precision form run")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device with torch: the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counts to bytes")
@@ -91,41 +131,24 @@ def main():
         sync()
         del a, b
     gpu = load_gpu()
-    sim = cases.triple_point(gpu, NX, NY)
-    tuning = sim.autotune(10)
-    print("autotune:", tuning, file=sys.stderr)
-    if args.warmup:
-        sim.L.optional["run_async"](sim.ctx, args.warmup)
-        sim.L.optional["synchronize"](sim.ctx)
-    # timed region: K fused steps, replayed from captured HIP graphs
-    sim.init_triple_point(cases.TP_RHO, cases.TP_P)
-    set_timing(sim, False)
-    sync()
-    sim.L.optional["synchronize"](sim.ctx)
-    t0 = time.perf_counter()
-    r = sim.L.optional["run_async"](sim.ctx, args.steps)
-    assert r == 0, sim.L.last_error(sim.ctx)
-    sim.L.optional["synchronize"](sim.ctx)
-    sync()
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    st = sim.download(["density", "cell_mass"])
-    assert st["global_iteration"] == args.steps, st["global_iteration"]
-    assert np.all(np.isfinite(st["density"]))
-    rho_timed = st["density"]
-    # the same K steps again, launched one kernel at a time between HIP
-    # events on the library stream: per-kernel durations
-    kt = {}
-    if not args.no_events:
-        sim.init_triple_point(cases.TP_RHO, cases.TP_P)
-        set_timing(sim, True)
-        r = sim.L.optional["run_async"](sim.ctx, args.steps)
-        assert r == 0, sim.L.last_error(sim.ctx)
-        sim.L.optional["synchronize"](sim.ctx)
-        kt = kernel_times(sim)
-        set_timing(sim, False)
-        st2 = sim.download(["density"])
-        assert np.array_equal(st2["density"], rho_timed), "graph and direct launches differ"
+    res = timed_run(gpu, abi.PREC_FP64, args, sync)
+    sim, elapsed, tuning, kt = res["sim"], res["elapsed"], res["tuning"], res["kt"]
+    rho_timed = res["density"]
+    # precision form: FP32 bulk fields, FP64 coordinates, geometry and dt
+    p32 = None
+    if not args.no_precision_form:
+        r32 = timed_run(gpu, abi.PREC_FP32_BULK, args, sync)
+        v32 = sim.nc * args.steps / r32["elapsed"]
+        v64 = sim.nc * args.steps / elapsed
+        d32 = r32["density"]
+        p32 = {"value": v32, "unit": "zone-cycles/s", "ms_per_step": 1e3 * r32["elapsed"] / args.steps,
+               "speedup_vs_fp64": v32 / v64,
+               "l1_density_err_vs_fp64": float(np.sum(np.abs(d32 - rho_timed)) / np.sum(np.abs(rho_timed))),
+               "linf_density_err_vs_fp64": float(np.max(np.abs(d32 - rho_timed)) / np.max(np.abs(rho_timed))),
+               "reference": "the FP64 path's density after the same steps (it matches the oracle to 1e-10)",
+               "dtype": "f32 bulk; f64 coordinates, cell geometry, dt",
+               "kernel_ms": {k: round(v[0], 5) for k, v in r32["kt"].items()}}
+        r32["sim"].close()
     nc, nn = sim.nc, sim.nn
     value = nc * args.steps / elapsed
 
@@ -171,6 +194,7 @@ def main():
                             "of the same K steps launched one kernel at a time",
         "roofline": rl,
         "cpu_baseline": cpu,
+        "precision_form": p32,
     }
     print(json.dumps(out))
 

@@ -223,3 +223,55 @@ def test_fp64_bit_identity_fixture(gpu):
     got = mk.run_all(gpu)
     for k in ref.files:
         assert np.array_equal(ref[k], got[k]), k
+
+
+def _rel_l1(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    return float(np.sum(np.abs(a - b)) / np.sum(np.abs(b)))
+
+
+def _rel_linf(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def test_fp32_precision_form_triple_point(gpu):
+    """Precision form (FP32 bulk, FP64 coordinates, geometry and dt): relative
+    L1 density error vs the FP64 oracle within 1e-5 after 200 triple-point
+    steps, per-field L1/Linf printed; the same step count and mass within
+    FP32 round-off of the oracle's."""
+    nx, ny, steps = 112, 48, 200
+    o = H.tp_setup(H.oracle(), nx, ny)
+    g = H.tp_setup(gpu, nx, ny, precision=H.abi.PREC_FP32_BULK)
+    o.run(steps)
+    g.run(steps)
+    ref, got = o.download(), g.download()
+    errs = {f: (_rel_l1(got[f], ref[f]), _rel_linf(got[f], ref[f]))
+            for f in ["density", "internal_energy", "pressure", "velocity", "node_coord"]}
+    print("fp32 triple point", {k: f"L1 {a:.2e} Linf {b:.2e}" for k, (a, b) in errs.items()})
+    assert got["global_iteration"] == ref["global_iteration"] == steps
+    assert errs["density"][0] < 1e-5
+    assert abs(got["global_time"] - ref["global_time"]) / ref["global_time"] < 1e-5
+    m_o = np.sum(ref["cell_mass"])
+    m_g = np.sum(np.asarray(got["cell_mass"], dtype=np.float64))
+    assert abs(m_g - m_o) / m_o < 1e-5
+
+
+def test_fp32_precision_form_sod_exact(gpu):
+    """Precision form on Sod to t = 0.2: L1 density error vs the exact Riemann
+    solution within 1 % of the FP64 oracle's own error."""
+    from mahyco_amd import riemann
+    nx = 400
+    o = H.sod_setup(H.oracle(), nx, 4)
+    g = H.sod_setup(gpu, nx, 4, precision=H.abi.PREC_FP32_BULK)
+    o.run(100000)
+    g.run(100000)
+    ref, got = o.download(), g.download()
+    x = (np.arange(nx) + 0.5) / nx
+    exact, _, _ = riemann.sample(x, ref["global_time"], 0.5, (1, 0, 1), (0.125, 0, 0.1), 1.4)
+    row_o = np.asarray(ref["density"]).reshape(4, nx)[0]
+    row_g = np.asarray(got["density"]).reshape(4, nx)[0]
+    e_o = np.mean(np.abs(row_o - exact))
+    e_g = np.mean(np.abs(row_g - exact))
+    print("sod L1 vs exact: fp64 oracle", e_o, "fp32 gpu", e_g)
+    assert e_g < 1.01 * e_o
