@@ -83,10 +83,23 @@ __host__ __device__ inline bool minmod_term(T a, T b, T* term) {
   return true;
 }
 
-// Limited gradients with the limiter fixed at compile time (LIM 0 = minmod
-// uses minmod_term; others evaluate the quotient as the reference does).
-template <int LIM, bool CELL, class T>
+// Limited gradients with the limiter fixed at compile time.  BF (branch-free)
+// evaluates the reference formula for both terms and selects: each term is
+// taken only where its denominator is non-zero, exactly as the guarded sums
+// of the reference (a skipped term adds +0 to a sum that starts at +0).
+// Otherwise minmod uses minmod_term, which divides only where the quotient
+// is not clamped; the other limiters evaluate the quotient as the reference.
+template <int LIM, bool CELL, class T, bool BF = false>
 __host__ __device__ inline T limited_grad(T gb, T gf) {
+  if (BF) {
+    const T l1 = limiter<LIM>(qdiv(gf, gb)), l2 = limiter<LIM>(qdiv(gb, gf));
+    const T t1 = CELL ? T(0.5) * (l1 * gb) : T(0.5) * l1 * gb;
+    const T t2 = CELL ? T(0.5) * (l2 * gf) : T(0.5) * l2 * gf;
+    T g = T(0);
+    g = g + ((gb != T(0)) ? t1 : T(0));
+    g = g + ((gf != T(0)) ? t2 : T(0));
+    return g;
+  }
   T g = T(0);
   if (LIM == 0) {
     T t;

@@ -356,6 +356,12 @@ __global__ __launch_bounds__(256) void k_cell_stage_2d(Params P, DevScalars* S, 
 // computeGradPhiFace (RemapADIService.cc:171-177), the limited cell gradient
 // (497-506), the upwind face value (707-714) and the flux update (909-936).
 // ---------------------------------------------------------------------------
+// Limited gradients: branch-free in float (two hardware-reciprocal quotients
+// cost less than the divergent branches); in double the exact quotients are
+// dear, so minmod divides only where the quotient is not clamped.
+template <class R>
+constexpr bool kBranchFree = std::is_same<R, float>::value;
+
 template <class T>
 __device__ inline T face_grad(bool inner, T phi, T phim, T dx) {
   return inner ? qdiv(phi - phim, dx) : T(0);
@@ -438,13 +444,18 @@ __global__ __launch_bounds__(D == 0 ? 256 : 1024) void k_cell_sweep_lane_2d(Para
   const R fb = (R)(-1.0 * uf * lf * dt), ff = (R)(1.0 * uf_f * lf_f * dt);
   const R dxr = (R)dx, dotr = (R)dotxf;
   const bool o2 = P.order2 > 0;
-  auto sweep_var = [&](int v, R& unew, R& dual) {
-    const R u0 = ex ? Uin[v * NC + c] : R(0);
+  R pre[3][NEM];
+  const R preq = ex ? Uin[(3 * ne) * NC + c] : R(0);
+#pragma unroll
+  for (int g = 0; g < 3; ++g)
+#pragma unroll
+    for (int k = 0; k < NEM; ++k) pre[g][k] = (ex && k < ne) ? Uin[(g * ne + k) * NC + c] : R(0);
+  auto sweep_var = [&](R u0, R& unew, R& dual) {
     const R phi = ex ? qdiv(u0, den) : R(0);
     const R phim = __shfl_up(phi, 1, 64);
     const R gf = face_grad(inner, phi, phim, dxr);
     const R gfp = __shfl_down(gf, 1, 64);
-    const R g = o2 ? limited_grad<LIM, true>(gf, gfp) : R(0);
+    const R g = o2 ? limited_grad<LIM, true, R, kBranchFree<R>>(gf, gfp) : R(0);
     const R gm = __shfl_up(g, 1, 64);
     const R phif = inner ? (upm ? phim + order2 * dotr * gm : phi + order2 * dotr * g) : R(0);
     const R phifp = __shfl_down(phif, 1, 64);
@@ -463,14 +474,14 @@ __global__ __launch_bounds__(D == 0 ? 256 : 1024) void k_cell_sweep_lane_2d(Para
   for (int k = 0; k < NEM; ++k) {
     if (k >= ne) break;
     R un, du;
-    sweep_var(k, un, du);
+    sweep_var(pre[0][k], un, du);
     if (out) st_stream(&Uout[k * NC + c], un);
   }
 #pragma unroll
   for (int k = 0; k < NEM; ++k) {
     if (k >= ne) break;
     R un, du;
-    sweep_var(ne + k, un, du);
+    sweep_var(pre[1][k], un, du);
     if (un < R(0)) un = R(0);  // RemapADIService.cc:953-955
     if (out) {
       st_stream(&Uout[(ne + k) * NC + c], un);
@@ -481,13 +492,13 @@ __global__ __launch_bounds__(D == 0 ? 256 : 1024) void k_cell_sweep_lane_2d(Para
   for (int k = 0; k < NEM; ++k) {
     if (k >= ne) break;
     R un, du;
-    sweep_var(2 * ne + k, un, du);
+    sweep_var(pre[2][k], un, du);
     if (un < R(0)) un = R(0);  // RemapADIService.cc:956-958
     if (out) st_stream(&Uout[(2 * ne + k) * NC + c], un);
   }
   {
     R un, du;
-    sweep_var(3 * ne, un, du);
+    sweep_var(preq, un, du);
     if (out) st_stream(&Uout[(3 * ne) * NC + c], un);
   }
 }
@@ -522,7 +533,7 @@ __global__ __launch_bounds__(1024) void k_cell_sweep_ytile_2d(Params P, DevScala
   const int tr = t >> 4, tc = t & 15;     // row-major position (loads, stores)
   const int w = t >> 6, lane = t & 63;    // transposed position: column w, row lane
   const long NC = P.nc;
-  const int ne = P.ne, nvu = P.nvu;
+  const int ne = P.ne;
   const double dt = S->dt;
   const R order2 = (R)P.order2;
   const bool o2 = P.order2 > 0;
@@ -578,6 +589,7 @@ __global__ __launch_bounds__(1024) void k_cell_sweep_ytile_2d(Params P, DevScala
   R(*stin)[kTileP] = stg[0];
   R(*stu)[kTileP] = stg[1];
   R(*stc)[kTileP] = stg[2];
+  const int nvu = P.nvu;
   R nxt = gex ? Uin[gc] : R(0);
   for (int v = 0; v < nvu; ++v) {
     stin[tr][tc] = nxt;
@@ -588,7 +600,7 @@ __global__ __launch_bounds__(1024) void k_cell_sweep_ytile_2d(Params P, DevScala
     const R phim = __shfl_up(phi, 1, 64);
     const R gf = face_grad(inner, phi, phim, dxr);
     const R gfp = __shfl_down(gf, 1, 64);
-    const R g = o2 ? limited_grad<LIM, true>(gf, gfp) : R(0);
+    const R g = o2 ? limited_grad<LIM, true, R, kBranchFree<R>>(gf, gfp) : R(0);
     const R gm = __shfl_up(g, 1, 64);
     const R phif = inner ? (upm ? phim + order2 * dotr * gm : phi + order2 * dotr * g) : R(0);
     const R phifp = __shfl_down(phif, 1, 64);
@@ -1243,8 +1255,8 @@ __global__ __launch_bounds__(256) void k_node_sweep_x_2d(Params P, DevScalars* S
       const R dxf = (R)(xdp - xd), dxb = (R)(xd - xdm);
       const R gf0 = qdiv(p0p - p0, dxf), gb0 = qdiv(p0 - p0m, dxb);
       const R gf1 = qdiv(p1p - p1, dxf), gb1 = qdiv(p1 - p1m, dxb);
-      g0 = limited_grad<LIM, false>(gb0, gf0);
-      g1 = limited_grad<LIM, false>(gb1, gf1);
+      g0 = limited_grad<LIM, false, R, kBranchFree<R>>(gb0, gf0);
+      g1 = limited_grad<LIM, false, R, kBranchFree<R>>(gb1, gf1);
       if (!inner_t && out) {  // kept for the sweep in the other direction
         A.dg0[nid] = g0;
         A.dg1[nid] = g1;
@@ -1386,8 +1398,8 @@ __global__ __launch_bounds__(1024) void k_node_sweep_ytile_2d(Params P, DevScala
       const R dxf = (R)(xdp - xd), dxb = (R)(xd - xdm);
       const R gf0 = qdiv(p0p - p0, dxf), gb0 = qdiv(p0 - p0m, dxb);
       const R gf1 = qdiv(p1p - p1, dxf), gb1 = qdiv(p1 - p1m, dxb);
-      g0 = limited_grad<LIM, false>(gb0, gf0);
-      g1 = limited_grad<LIM, false>(gb1, gf1);
+      g0 = limited_grad<LIM, false, R, kBranchFree<R>>(gb0, gf0);
+      g1 = limited_grad<LIM, false, R, kBranchFree<R>>(gb1, gf1);
       if (!inner_t && out) {  // kept for the sweep in the other direction
         A.dg0[q] = g0;
         A.dg1[q] = g1;
