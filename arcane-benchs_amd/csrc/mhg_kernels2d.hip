@@ -521,8 +521,11 @@ constexpr int kTileW = 16, kTileR = 64, kTileP = kTileW + 1;
 
 template <class R, int NEM, bool FIRST, int LIM>
 __global__ __launch_bounds__(1024) void k_cell_sweep_ytile_2d(Params P, DevScalars* S, CellSweepArgsT<R> A) {
-  __shared__ double lds[5][kTileR][kTileP];  // geometry
-  __shared__ R stg[3][kTileR][kTileP];       // staging of one variable: in, out, dual flux
+  // geometry tiles [0..4]; once read, tiles [0..5] are reused as the staging
+  // tiles (in, out, dual flux) of a pair of variables
+  __shared__ double lds[6][kTileR][kTileP];
+  typedef R Tile[kTileR][kTileP];
+  Tile* stg = reinterpret_cast<Tile*>(&lds[0][0][0]);
   if (S->stop || S->err) return;  // uniform over the grid
   const int nx = P.nx, N = P.ny, nnx = nx + 1;
   const int ncolblk = (nx + kTileW - 1) / kTileW;
@@ -586,16 +589,8 @@ __global__ __launch_bounds__(1024) void k_cell_sweep_ytile_2d(Params P, DevScala
   const double lf_f = __shfl_down(lf, 1, 64);
   const R fb = (R)(-1.0 * uf * lf * dt), ff = (R)(1.0 * uf_f * lf_f * dt);
   const R dxr = (R)dx, dotr = (R)dotxf;
-  R(*stin)[kTileP] = stg[0];
-  R(*stu)[kTileP] = stg[1];
-  R(*stc)[kTileP] = stg[2];
-  const int nvu = P.nvu;
-  R nxt = gex ? Uin[gc] : R(0);
-  for (int v = 0; v < nvu; ++v) {
-    stin[tr][tc] = nxt;
-    __syncthreads();
-    if (v + 1 < nvu) nxt = gex ? Uin[(v + 1) * NC + gc] : R(0);
-    const R u0 = stin[lane][w];
+  // one variable on the transposed layout: new U and half the dual flux
+  auto sweep_var = [&](R u0, R& un, R& dual) {
     const R phi = ex ? qdiv(u0, den) : R(0);
     const R phim = __shfl_up(phi, 1, 64);
     const R gf = face_grad(inner, phi, phim, dxr);
@@ -607,18 +602,39 @@ __global__ __launch_bounds__(1024) void k_cell_sweep_ytile_2d(Params P, DevScala
     const R flux_b = fb * phif;
     const R flux_f = ff * phifp;
     // cell faces in order: bottom (back), right, top (front), left
-    R un = u0 - (flux_b + flux_f);
-    if (v >= ne && v < 3 * ne && un < R(0)) un = R(0);  // RemapADIService.cc:953-958
-    stu[lane][w] = un;
-    const bool mass = (v >= ne && v < 2 * ne);
-    if (mass) {
-      const R dual = R(0.5) * flux_b * R(-1.0) + R(0.5) * flux_f * R(1.0);
-      stc[lane][w] = R(0.5) * dual;  // fcontrib (DualPhaseRemap.cc:168-185)
-    }
+    un = u0 - (flux_b + flux_f);
+    dual = R(0.5) * flux_b * R(-1.0) + R(0.5) * flux_f * R(1.0);
+  };
+  __syncthreads();  // geometry reads done: the tiles become staging tiles
+  const int nvu = P.nvu;
+  // variables two at a time (a, b = a + 1), one barrier pair per round
+  R nxa = gex ? Uin[gc] : R(0);
+  R nxb = (gex && nvu > 1) ? Uin[NC + gc] : R(0);
+  for (int va = 0; va < nvu; va += 2) {
+    const int vb = va + 1;
+    const bool hb = vb < nvu;  // uniform
+    stg[0][tr][tc] = nxa;
+    stg[1][tr][tc] = nxb;
+    __syncthreads();
+    if (va + 2 < nvu) nxa = gex ? Uin[(va + 2) * NC + gc] : R(0);
+    if (va + 3 < nvu) nxb = gex ? Uin[(va + 3) * NC + gc] : R(0);
+    const R ua = stg[0][lane][w], ub = stg[1][lane][w];
+    R una, dua, unb, dub;
+    sweep_var(ua, una, dua);
+    sweep_var(ub, unb, dub);
+    if (va >= ne && va < 3 * ne && una < R(0)) una = R(0);  // RemapADIService.cc:953-958
+    if (vb >= ne && vb < 3 * ne && unb < R(0)) unb = R(0);
+    const bool ma = (va >= ne && va < 2 * ne), mb = hb && (vb >= ne && vb < 2 * ne);
+    stg[2][lane][w] = una;
+    stg[3][lane][w] = unb;
+    if (ma) stg[4][lane][w] = R(0.5) * dua;  // fcontrib (DualPhaseRemap.cc:168-185)
+    if (mb) stg[5][lane][w] = R(0.5) * dub;
     __syncthreads();
     if (gout) {
-      st_stream(&A.Uout[v * NC + gc], stu[tr][tc]);
-      if (mass) st_stream(&A.contrib[(v - ne) * NC + gc], stc[tr][tc]);
+      st_stream(&A.Uout[va * NC + gc], stg[2][tr][tc]);
+      if (hb) st_stream(&A.Uout[vb * NC + gc], stg[3][tr][tc]);
+      if (ma) st_stream(&A.contrib[(va - ne) * NC + gc], stg[4][tr][tc]);
+      if (mb) st_stream(&A.contrib[(vb - ne) * NC + gc], stg[5][tr][tc]);
     }
   }
 }
