@@ -307,8 +307,12 @@ struct mho_ctx {
       out[0] = nodeId(i, j, k); out[1] = nodeId(i, j + 1, k);
       out[2] = nodeId(i, j + 1, k + 1); out[3] = nodeId(i, j, k + 1);
     } else if (d == 1) {
-      out[0] = nodeId(i, j, k); out[1] = nodeId(i, j, k + 1);
-      out[2] = nodeId(i + 1, j, k + 1); out[3] = nodeId(i + 1, j, k);
+      // Arcane orders the nodes of a face from its smallest id towards the
+      // smaller neighbour: (i,j,k), (i+1,j,k), (i+1,j,k+1), (i,j,k+1); with
+      // InitGeometricValues' formula (MahycoModule.cc:1020-1023) every
+      // Cartesian face normal is then +e_dir
+      out[0] = nodeId(i, j, k); out[1] = nodeId(i + 1, j, k);
+      out[2] = nodeId(i + 1, j, k + 1); out[3] = nodeId(i, j, k + 1);
     } else {
       out[0] = nodeId(i, j, k); out[1] = nodeId(i + 1, j, k);
       out[2] = nodeId(i + 1, j + 1, k); out[3] = nodeId(i, j + 1, k);

@@ -107,3 +107,33 @@ def compare_states(got, ref, nenv, nx=None, ny=None, tol=1e-12, metric=None):
     out["dt"] = abs(got["global_deltat"] - ref["global_deltat"]) / abs(ref["global_deltat"])
     out["time"] = abs(got["global_time"] - ref["global_time"]) / max(abs(ref["global_time"]), 1e-300)
     return out
+
+
+def sod3d_setup(L, n_along=40, n_cross=4, axis=0, **optkw):
+    """Sod shock tube along `axis` of a 3D box (1 x w x w, w = n_cross/n_along),
+    left state for x_axis < 0.5: rho = 1, p = 1; right: rho = 0.125, p = 0.1."""
+    opt = L.options(**optkw)
+    abi.set_bcs(opt, abi.reflecting_walls(3))
+    n = [n_cross] * 3
+    n[axis] = n_along
+    length = [n_cross / n_along] * 3
+    length[axis] = 1.0
+    s = abi.Simulation(L, abi.mesh_desc(3, n, [0, 0, 0], length), opt)
+    idx = np.indices((n[2], n[1], n[0]))  # k, j, i
+    along = idx[2 - axis].reshape(-1)
+    left = along < n_along // 2
+    env = np.zeros(along.size, dtype=np.int32)
+    s.init_cells(env, np.where(left, 1.0, 0.125), np.where(left, 1.0, 0.1))
+    return s
+
+
+def line_along(field, n_along, n_cross, axis, a=0, b=0):
+    """Values along `axis` of a 3D cell field at cross indices (a, b)."""
+    n = [n_cross] * 3
+    n[axis] = n_along
+    f = np.asarray(field).reshape(n[2], n[1], n[0])
+    if axis == 0:
+        return f[b, a, :]
+    if axis == 1:
+        return f[b, :, a]
+    return f[:, b, a]
