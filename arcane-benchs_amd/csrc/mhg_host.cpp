@@ -56,6 +56,13 @@ struct mhg_ctx {
   double* xx[2] = {nullptr, nullptr};
   double* xy[2] = {nullptr, nullptr};
   void *Ud0 = nullptr, *Ud1 = nullptr, *Ud3 = nullptr, *dg0 = nullptr, *dg1 = nullptr;
+  // 3D: z components, a second dual state (the middle sweep reads one and
+  // writes the other), the Euler volumes of the hexahedra
+  double* xz[2] = {nullptr, nullptr};
+  void* uz[2] = {nullptr, nullptr};
+  void *Ud2 = nullptr, *dg2 = nullptr;
+  void* UdB[4] = {nullptr, nullptr, nullptr, nullptr};
+  double* Veuler = nullptr;
   double* block_min = nullptr;
   DevScalars* S = nullptr;
   DevScalars* S_host = nullptr;  // pinned mirror
@@ -387,7 +394,127 @@ void enqueue_step_t(mhg_ctx* c) {
   remap_2d<R>(c, 2);
 }
 
+// 3D remap: three cell sweeps and three dual sweeps; the dual state goes
+// velocity -> Ud -> UdB -> velocity.
+template <class R>
+void remap_3d(mhg_ctx* c, int dt_mode) {
+  const Params& P = c->P;
+  const double* xn[3] = {c->xx[1 - c->xcur], c->xy[1 - c->xcur], c->xz[1 - c->xcur]};
+  const R* u[3] = {bp<R>(c->ux[c->ucur]), bp<R>(c->uy[c->ucur]), bp<R>(c->uz[c->ucur])};
+  void* UdA[4] = {c->Ud0, c->Ud1, c->Ud2, c->Ud3};
+  int prevmask = 0;
+  const char* cname[3] = {"cell_sweep_x", "cell_sweep_y", "cell_sweep_z"};
+  const char* nname[3] = {"node_sweep_x", "node_sweep_y", "node_sweep_z"};
+  for (int pass = 0; pass < 3; ++pass) {
+    const int idir = (pass + c->sens) % 3;
+    const bool first = pass == 0, last = pass == 2;
+    CellSweep3ArgsT<R> A;
+    A.b.Uin = bp<R>(c->U[pass & 1]);
+    A.b.Uout = bp<R>(c->U[(pass + 1) & 1]);
+    A.b.Vlag = bp<R>(c->Vlag);
+    A.b.xnx = xn[0];
+    A.b.xny = xn[1];
+    A.xnz = xn[2];
+    A.b.ux = u[0];
+    A.b.uy = u[1];
+    A.uz = u[2];
+    A.b.contrib = bp<R>(c->contrib);
+    {
+      TimedLaunch t(c, cname[idir]);
+      launch_cell_sweep_3d(c->stream, P, c->S, A, idir, first);
+    }
+    NodeSweep3ArgsT<R> B;
+    B.m = bp<R>(c->m);
+    B.ux = u[0];
+    B.uy = u[1];
+    B.uz = u[2];
+    B.xnx = xn[0];
+    B.xny = xn[1];
+    B.xnz = xn[2];
+    B.contrib = bp<R>(c->contrib);
+    void** in = (pass == 2) ? c->UdB : UdA;
+    void** outb = (pass == 0) ? UdA : c->UdB;
+    B.Udi0 = bp<R>(in[0]);
+    B.Udi1 = bp<R>(in[1]);
+    B.Udi2 = bp<R>(in[2]);
+    B.Udi3 = bp<R>(in[3]);
+    B.Udo0 = bp<R>(outb[0]);
+    B.Udo1 = bp<R>(outb[1]);
+    B.Udo2 = bp<R>(outb[2]);
+    B.Udo3 = bp<R>(outb[3]);
+    B.dg0 = bp<R>(c->dg0);
+    B.dg1 = bp<R>(c->dg1);
+    B.dg2 = bp<R>(c->dg2);
+    B.uox = bp<R>(c->ux[1 - c->ucur]);
+    B.uoy = bp<R>(c->uy[1 - c->ucur]);
+    B.uoz = bp<R>(c->uz[1 - c->ucur]);
+    B.prevmask = prevmask;
+    {
+      TimedLaunch t(c, nname[idir]);
+      launch_node_sweep_3d(c->stream, P, c->S, B, idir, first, last);
+    }
+    prevmask |= 1 << idir;
+  }
+  CellFinal3ArgsT<R> F;
+  F.b.U = bp<R>(c->U[1]);
+  F.b.Vlag = bp<R>(c->Vlag);
+  F.b.L = bp<R>(c->L);
+  F.b.ux = bp<R>(c->ux[1 - c->ucur]);
+  F.b.uy = bp<R>(c->uy[1 - c->ucur]);
+  F.uz = bp<R>(c->uz[1 - c->ucur]);
+  F.b.rho = bp<R>(c->rho);
+  F.b.e = bp<R>(c->e);
+  F.b.p = bp<R>(c->p);
+  F.b.cs = bp<R>(c->cs);
+  F.b.q = bp<R>(c->q);
+  F.b.m = bp<R>(c->m);
+  F.b.mask = c->mask;
+  F.b.env_rho = bp<R>(c->env_rho);
+  F.b.env_e = bp<R>(c->env_e);
+  F.b.env_p = bp<R>(c->env_p);
+  F.b.env_cs = bp<R>(c->env_cs);
+  F.b.env_q = bp<R>(c->env_q);
+  F.b.env_m = bp<R>(c->env_m);
+  F.b.env_f = bp<R>(c->env_f);
+  F.b.env_mf = bp<R>(c->env_mf);
+  F.b.block_min = c->block_min;
+  F.Veuler = c->Veuler;
+  {
+    TimedLaunch t(c, "cell_final");
+    launch_cell_final_3d(c->stream, P, c->S, F, dt_mode);
+  }
+  c->sens = (c->sens + 1) % 3;
+  c->xcur = 1 - c->xcur;
+  c->ucur = 1 - c->ucur;
+}
+
+template <class R>
+void enqueue_step_3d(mhg_ctx* c) {
+  NodeLag3ArgsT<R> N;
+  N.b = node_lag_args<R>(c);
+  N.xpz = c->xz[c->xcur];
+  N.uz = bp<R>(c->uz[c->ucur]);
+  N.xnz = c->xz[1 - c->xcur];
+  {
+    TimedLaunch t(c, "node_lagrange");
+    launch_node_lagrange_3d(c->stream, c->P, c->S, N, true, true, true);
+  }
+  CellLag3ArgsT<R> L;
+  L.b = cell_lag_args<R>(c);
+  L.xnz = c->xz[1 - c->xcur];
+  {
+    TimedLaunch t(c, "cell_lagrange");
+    launch_cell_lagrange_3d(c->stream, c->P, c->S, L);
+  }
+  remap_3d<R>(c, 2);
+}
+
 void enqueue_step(mhg_ctx* c) {
+  if (c->P.dim == 3) {
+    if (c->bulk == 4) enqueue_step_3d<float>(c);
+    else enqueue_step_3d<double>(c);
+    return;
+  }
   if (c->bulk == 4) enqueue_step_t<float>(c);
   else enqueue_step_t<double>(c);
 }
@@ -447,6 +574,26 @@ int check_fused(mhg_ctx* c) {
 
 // Pure-cell initial state (hydroStartInit, MahycoModule.cc:48-188, then
 // computeCellMass 199-225): EOS init per environment, m = rho V.
+// Volume of cell cc on the initial mesh by the corner-normal formula
+// (the Euler volume the remap returns to).
+double euler_volume(const Params& P, long cc) {
+  if (P.dim == 2) {
+    const int i = (int)(cc % P.nx), j = (int)(cc / P.nx);
+    double x[4] = {x0_of(P, 0, i), x0_of(P, 0, i + 1), x0_of(P, 0, i + 1), x0_of(P, 0, i)};
+    double y[4] = {x0_of(P, 1, j), x0_of(P, 1, j), x0_of(P, 1, j + 1), x0_of(P, 1, j + 1)};
+    return quad_volume(x, y);
+  }
+  const int i = (int)(cc % P.nx), j = (int)((cc / P.nx) % P.ny), k = (int)(cc / ((long)P.nx * P.ny));
+  V3 cn[8];
+  for (int q = 0; q < 8; ++q) {
+    const int di = (q == 1 || q == 2 || q == 5 || q == 6) ? 1 : 0;
+    const int dj = (q == 2 || q == 3 || q == 6 || q == 7) ? 1 : 0;
+    const int dk = q >= 4 ? 1 : 0;
+    cn[q] = {x0_of(P, 0, i + di), x0_of(P, 1, j + dj), x0_of(P, 2, k + dk)};
+  }
+  return hex_volume(cn);
+}
+
 int init_pure(mhg_ctx* c, const std::vector<int>& env, const std::vector<double>& rho,
               const std::vector<double>& pres, const double* vel) {
   const Params& P = c->P;
@@ -466,21 +613,21 @@ int init_pure(mhg_ctx* c, const std::vector<int>& env, const std::vector<double>
       e[cc] = pres[cc] / ((g - 1.) * rho[cc]);
       cs[cc] = std::sqrt(g * pres[cc] / rho[cc]);
     }
-    const int i = (int)(cc % P.nx), j = (int)(cc / P.nx);
-    double x[4] = {x0_of(P, 0, i), x0_of(P, 0, i + 1), x0_of(P, 0, i + 1), x0_of(P, 0, i)};
-    double y[4] = {x0_of(P, 1, j), x0_of(P, 1, j), x0_of(P, 1, j + 1), x0_of(P, 1, j + 1)};
-    const double V = quad_volume(x, y);
+    const double V = euler_volume(P, cc);
     m[cc] = rho[cc] * V;
     m[cc] = 1. * m[cc];  // m_env = mass_fraction * m (199-225), mass_fraction = 1
   }
-  std::vector<double> xs(NN), ys(NN), us(NN, 0.), vs(NN, 0.);
+  std::vector<double> xs(NN), ys(NN), zs(NN, 0.), us(NN, 0.), vs(NN, 0.), ws(NN, 0.);
   for (long n = 0; n < NN; ++n) {
-    const int i = (int)(n % (P.nx + 1)), j = (int)(n / (P.nx + 1));
+    const int i = (int)(n % (P.nx + 1)), j = (int)((n / (P.nx + 1)) % (P.ny + 1));
+    const int k = (int)(n / ((long)(P.nx + 1) * (P.ny + 1)));
     xs[n] = x0_of(P, 0, i);
     ys[n] = x0_of(P, 1, j);
+    if (P.dim == 3) zs[n] = x0_of(P, 2, k);
     if (vel) {
       us[n] = vel[n];
       vs[n] = vel[NN + n];
+      if (P.dim == 3) ws[n] = vel[2 * NN + n];
     }
   }
   c->xcur = 0;
@@ -502,6 +649,11 @@ int init_pure(mhg_ctx* c, const std::vector<int>& env, const std::vector<double>
   HIPCHK(c, up(c->uy[0], vs));
   HIPCHK(c, hipMemset(c->dg0, 0, NN * (size_t)c->bulk));
   HIPCHK(c, hipMemset(c->dg1, 0, NN * (size_t)c->bulk));
+  if (P.dim == 3) {
+    HIPCHK(c, hipMemcpy(c->xz[0], zs.data(), NN * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, up(c->uz[0], ws));
+    HIPCHK(c, hipMemset(c->dg2, 0, NN * (size_t)c->bulk));
+  }
   const size_t envb = (size_t)P.ne * NC * (size_t)c->bulk;
   void* envs[] = {c->env_rho, c->env_e, c->env_p, c->env_cs, c->env_q, c->env_m, c->env_f, c->env_mf};
   for (void* d : envs) HIPCHK(c, hipMemset(d, 0, envb));
@@ -569,8 +721,7 @@ int mhg_create(const mhg_mesh_desc* mesh, const mhg_options* opt, mhg_ctx** out)
       opt->projection_limiteur_id >= MHG_LIM_MINMODG || opt->projection_limiteur_id < 0 ||
       opt->ordre_projection < 1 || opt->ordre_projection > 2)
     return MHG_E_UNSUPPORTED;
-  if (mesh->dim != 2 || (opt->precision != MHG_PREC_FP64 && opt->precision != MHG_PREC_FP32_BULK))
-    return MHG_E_UNSUPPORTED;
+  if (opt->precision != MHG_PREC_FP64 && opt->precision != MHG_PREC_FP32_BULK) return MHG_E_UNSUPPORTED;
   mhg_ctx* c = new mhg_ctx();
   c->opt = *opt;
   Params& P = c->P;
@@ -648,6 +799,18 @@ int mhg_create(const mhg_mesh_desc* mesh, const mhg_options* opt, mhg_ctx** out)
   }
   r |= balloc(c, &c->Ud0, NN) | balloc(c, &c->Ud1, NN) | balloc(c, &c->Ud3, NN);
   r |= balloc(c, &c->dg0, NN) | balloc(c, &c->dg1, NN);
+  if (P.dim == 3) {
+    for (int b = 0; b < 2; ++b) r |= dalloc(c, &c->xz[b], NN) | balloc(c, &c->uz[b], NN);
+    r |= balloc(c, &c->Ud2, NN) | balloc(c, &c->dg2, NN);
+    for (int q = 0; q < 4; ++q) r |= balloc(c, &c->UdB[q], NN);
+    r |= dalloc(c, &c->Veuler, NC);
+    if (!r) {
+      std::vector<double> ve(NC);
+      for (long cc = 0; cc < NC; ++cc) ve[cc] = euler_volume(P, cc);
+      if (hipMemcpy(c->Veuler, ve.data(), NC * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        r = MHG_E_HIP;
+    }
+  }
   r |= dalloc(c, &c->block_min, cell_final_blocks(P));
   r |= dalloc(c, &c->S, 1);
   if (r) {
@@ -692,6 +855,7 @@ int mhg_init_sod(mhg_ctx* c, int32_t cas_test) {
   if (!mono && !bi) return fail(c, MHG_E_ARG, "unsupported SOD cas-test");
   if (bi && P.ne < 2) return fail(c, MHG_E_ARG, "bi-material SOD needs 2 environments");
   const int axis = mono ? cas_test - 3 : cas_test - 13;
+  if (P.dim != 2) return fail(c, MHG_E_UNSUPPORTED, "3D initial states: use mhg_init_cells");
   if (axis >= P.dim) return fail(c, MHG_E_ARG, "SOD axis beyond mesh dimension");
   std::vector<int> env(P.nc);
   std::vector<double> rho(P.nc), pres(P.nc);
@@ -710,6 +874,7 @@ int mhg_init_triple_point(mhg_ctx* c, const double rho3[3], const double p3[3]) 
   if (!c || !rho3 || !p3) return MHG_E_ARG;
   const Params& P = c->P;
   if (P.ne != 3) return fail(c, MHG_E_ARG, "triple point needs 3 environments");
+  if (P.dim != 2) return fail(c, MHG_E_UNSUPPORTED, "3D initial states: use mhg_init_cells");
   std::vector<int> env(P.nc);
   std::vector<double> rho(P.nc), pres(P.nc);
   for (long cc = 0; cc < P.nc; ++cc) {
@@ -765,14 +930,17 @@ int mhg_upload_state(mhg_ctx* c, const mhg_state* st) {
   if (st->velocity) {
     HIPCHK(c, put(c, c->ux[0], st->velocity, NN));
     HIPCHK(c, put(c, c->uy[0], st->velocity + NN, NN));
+    if (P.dim == 3) HIPCHK(c, put(c, c->uz[0], st->velocity + 2 * NN, NN));
   }
   if (st->node_coord) {
     HIPCHK(c, h2d(c->xx[0], st->node_coord, nb));
     HIPCHK(c, h2d(c->xy[0], st->node_coord + NN, nb));
+    if (P.dim == 3) HIPCHK(c, h2d(c->xz[0], st->node_coord + 2 * NN, nb));
   }
   if (st->dual_grad_phi) {
     HIPCHK(c, put(c, c->dg0, st->dual_grad_phi, NN));
     HIPCHK(c, put(c, c->dg1, st->dual_grad_phi + NN, NN));
+    if (P.dim == 3) HIPCHK(c, put(c, c->dg2, st->dual_grad_phi + 2 * NN, NN));
   }
   if (st->caracteristic_length) HIPCHK(c, put(c, c->L, st->caracteristic_length, NC));
   std::memset(c->S_host, 0, sizeof(DevScalars));
@@ -835,6 +1003,7 @@ int mhg_download_state(mhg_ctx* c, mhg_state* st) {
   if (st->velocity) {
     HIPCHK(c, get(c, st->velocity, c->ux[c->ucur], NN));
     HIPCHK(c, get(c, st->velocity + NN, c->uy[c->ucur], NN));
+    if (P.dim == 3) HIPCHK(c, get(c, st->velocity + 2 * NN, c->uz[c->ucur], NN));
   }
   if (st->node_coord) {
     if (c->coord_phase == 1) {  // Euler reset of saveValuesAtN: x = x0
@@ -846,26 +1015,32 @@ int mhg_download_state(mhg_ctx* c, mhg_state* st) {
       const int b = c->coord_phase == 2 ? 1 - c->xcur : c->xcur;
       HIPCHK(c, d2h(st->node_coord, c->xx[b], nb));
       HIPCHK(c, d2h(st->node_coord + NN, c->xy[b], nb));
+      if (P.dim == 3) HIPCHK(c, d2h(st->node_coord + 2 * NN, c->xz[b], nb));
     }
   }
   if (st->dual_grad_phi) {
     HIPCHK(c, get(c, st->dual_grad_phi, c->dg0, NN));
     HIPCHK(c, get(c, st->dual_grad_phi + NN, c->dg1, NN));
+    if (P.dim == 3) HIPCHK(c, get(c, st->dual_grad_phi + 2 * NN, c->dg2, NN));
   }
   if (st->caracteristic_length) HIPCHK(c, get(c, st->caracteristic_length, c->L, NC));
   if (st->node_mass) {
     std::vector<double> m(NC);
     HIPCHK(c, get(c, m.data(), c->m, NC));
+    // computeNodeMass (MahycoModule.cc:235-266): cells in ascending id
+    const int nz = P.dim == 3 ? P.nz : 1;
     for (long n = 0; n < NN; ++n) {
-      const int i = (int)(n % (P.nx + 1)), j = (int)(n / (P.nx + 1));
+      const int i = (int)(n % (P.nx + 1)), j = (int)((n / (P.nx + 1)) % (P.ny + 1));
+      const int k = P.dim == 3 ? (int)(n / ((long)(P.nx + 1) * (P.ny + 1))) : 0;
       double s = 0.;
-      const int cdi[4] = {-1, 0, -1, 0}, cdj[4] = {-1, -1, 0, 0};
-      for (int t = 0; t < 4; ++t) {
-        int ci = i + cdi[t], cj = j + cdj[t];
-        if (ci < 0 || ci >= P.nx || cj < 0 || cj >= P.ny) continue;
-        s += m[ci + (long)P.nx * cj];
-      }
-      st->node_mass[n] = 0.25 * s;
+      for (int dk = (P.dim == 3 ? -1 : 0); dk <= 0; ++dk)
+        for (int dj = -1; dj <= 0; ++dj)
+          for (int di = -1; di <= 0; ++di) {
+            const int ci = i + di, cj = j + dj, ck = k + dk;
+            if (ci < 0 || ci >= P.nx || cj < 0 || cj >= P.ny || ck < 0 || ck >= nz) continue;
+            s += m[ci + (long)P.nx * (cj + (long)P.ny * ck)];
+          }
+      st->node_mass[n] = (P.dim == 3 ? 0.125 : 0.25) * s;
     }
   }
   r = read_scalars(c);
@@ -887,7 +1062,8 @@ namespace {
 int ep_prepare(mhg_ctx* c) {
   int r = check_ready(c);
   if (r) return r;
-  if (c->bulk != 8) return fail(c, MHG_E_UNSUPPORTED, "entry points: FP64 contexts only (use mhg_step)");
+  if (c->bulk != 8 || c->P.dim != 2)
+    return fail(c, MHG_E_UNSUPPORTED, "entry points: 2D FP64 contexts only (use mhg_step)");
   if (c->ep_ready) return MHG_OK;
   const size_t NC = c->P.nc, envn = (size_t)c->P.ne * NC;
   r |= dalloc(c, &c->rho_n, NC) | dalloc(c, &c->e_n, NC) | dalloc(c, &c->q_n, NC);
@@ -1093,7 +1269,7 @@ int mhg_conservation(mhg_ctx* c, double out[3]) {
   std::vector<double> m(NC), e(NC), ux(NN), uy(NN), mn(NN);
   mhg_state st;
   std::memset(&st, 0, sizeof(st));
-  std::vector<double> vel(2 * NN);
+  std::vector<double> vel((size_t)P.dim * NN);
   st.cell_mass = m.data();
   st.internal_energy = e.data();
   st.velocity = vel.data();
@@ -1112,8 +1288,8 @@ int mhg_conservation(mhg_ctx* c, double out[3]) {
     acc(b, bc, m[cc] * e[cc]);
   }
   for (long n = 0; n < NN; ++n) {
-    const double u = vel[n], v = vel[NN + n];
-    acc(k, kc, 0.5 * mn[n] * (u * u + v * v));
+    const double u = vel[n], v = vel[NN + n], w = P.dim == 3 ? vel[2 * NN + n] : 0.;
+    acc(k, kc, 0.5 * mn[n] * (u * u + v * v + w * w));
   }
   out[0] = a + ac;
   out[1] = b + bc;
@@ -1236,6 +1412,10 @@ int mhg_autotune(mhg_ctx* c, int32_t reps, char* report, int32_t report_len) {
   int r = check_ready(c);
   if (r) return r;
   if (reps < 1) reps = 1;
+  if (c->P.dim == 3) {  // one launch shape per 3D kernel
+    if (report && report_len > 0) std::strncpy(report, "3D: no launch variants", report_len - 1);
+    return MHG_OK;
+  }
   return c->bulk == 4 ? autotune_t<float>(c, reps, report, report_len)
                       : autotune_t<double>(c, reps, report, report_len);
 }

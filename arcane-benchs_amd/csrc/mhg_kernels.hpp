@@ -54,6 +54,49 @@ struct CellFinalArgsT {
   double* block_min;  // one CFL minimum per block of k_cell_final_2d
 };
 
+// 3D argument blocks: the 2D block (x and y components) plus the z
+// components; the dual sweeps read one dual state and write the other.
+template <class R>
+struct NodeLag3ArgsT {
+  NodeLagArgsT<R> b;
+  const double* xpz;
+  R* uz;
+  double* xnz;
+};
+
+template <class R>
+struct CellLag3ArgsT {
+  CellLagArgsT<R> b;
+  const double* xnz;
+};
+
+template <class R>
+struct CellSweep3ArgsT {
+  CellSweepArgsT<R> b;
+  const double* xnz;
+  const R* uz;
+};
+
+template <class R>
+struct NodeSweep3ArgsT {
+  const R* m;
+  const R *ux, *uy, *uz;
+  const double *xnx, *xny, *xnz;
+  const R* contrib;
+  const R *Udi0, *Udi1, *Udi2, *Udi3;  // dual state before the sweep
+  R *Udo0, *Udo1, *Udo2, *Udo3;        // dual state after it
+  R *dg0, *dg1, *dg2;                  // persistent nodal gradients
+  R *uox, *uoy, *uoz;                  // velocity out (last sweep)
+  int prevmask;                        // directions swept before this one
+};
+
+template <class R>
+struct CellFinal3ArgsT {
+  CellFinalArgsT<R> b;
+  const R* uz;
+  const double* Veuler;
+};
+
 using NodeLagArgs = NodeLagArgsT<double>;
 using CellLagArgs = CellLagArgsT<double>;
 using CellSweepArgs = CellSweepArgsT<double>;
@@ -109,5 +152,20 @@ const char* node_sweep_variant_name(int dir, int v);
 template <class R>
 void launch_cell_final_2d(hipStream_t st, const Params& P, DevScalars* S, const CellFinalArgsT<R>& A,
                           int dt_mode);
+
+// 3D fused step.
+template <class R>
+void launch_node_lagrange_3d(hipStream_t st, const Params& P, DevScalars* S, const NodeLag3ArgsT<R>& A, bool vel,
+                             bool bc, bool pos);
+template <class R>
+void launch_cell_lagrange_3d(hipStream_t st, const Params& P, DevScalars* S, const CellLag3ArgsT<R>& A);
+template <class R>
+void launch_cell_sweep_3d(hipStream_t st, const Params& P, DevScalars* S, const CellSweep3ArgsT<R>& A, int dir,
+                          bool first);
+template <class R>
+void launch_node_sweep_3d(hipStream_t st, const Params& P, DevScalars* S, const NodeSweep3ArgsT<R>& A, int dir,
+                          bool first, bool last);
+template <class R>
+void launch_cell_final_3d(hipStream_t st, const Params& P, DevScalars* S, const CellFinal3ArgsT<R>& A, int dt_mode);
 
 }  // namespace mhg

@@ -1,0 +1,870 @@
+// mhg_kernels3d.hip -- 3D (hexa8) kernels of the MaHyCo time step for gfx950.
+//
+// Same structure as the 2D path: node phase, cell phase, one cell sweep and
+// one dual sweep per direction (order (i + SensProjection) % 3), final cell
+// phase, dt.  Layout: cells i + nx*(j + ny*k), nodes i + (nx+1)*(j + (ny+1)*k),
+// bulk fields in the storage type R, coordinates and Euler volumes in double.
+// Every Cartesian face normal is +e_dir (InitGeometricValues,
+// MahycoModule.cc:1016-1025, with face nodes ordered from the smallest id), so
+// dot products with face normals reduce to one component.
+#include <type_traits>
+
+#include "mhg_common.hpp"
+#include "mhg_kernels.hpp"
+#include "mhg_kernels_impl.hpp"
+
+namespace mhg {
+
+__device__ inline long nid3(const Params& P, int i, int j, int k) {
+  return i + (long)(P.nx + 1) * (j + (long)(P.ny + 1) * k);
+}
+__device__ inline long cid3(const Params& P, int i, int j, int k) {
+  return i + (long)P.nx * (j + (long)P.ny * k);
+}
+__device__ inline double x0_3d(const Params& P, int d, int i) { return P.org[d] + i * P.h[d]; }
+
+// The 8 nodes of cell (ci, cj, ck) in hexa8 order.
+__device__ inline void hex_nodes(const Params& P, const double* X, const double* Y, const double* Z, int ci,
+                                 int cj, int ck, V3* c) {
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int di = (q == 1 || q == 2 || q == 5 || q == 6) ? 1 : 0;
+    const int dj = (q == 2 || q == 3 || q == 6 || q == 7) ? 1 : 0;
+    const int dk = q >= 4 ? 1 : 0;
+    const long n = nid3(P, ci + di, cj + dj, ck + dk);
+    c[q] = {X[n], Y[n], Z[n]};
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Node phase (MahycoModule.cc:589-665, 936-1004, 870-931): force from the 8
+// cells around the node in ascending id with the corner normals of the
+// previous Lagrangian coordinates, velocity, boundary conditions, position.
+// ---------------------------------------------------------------------------
+template <class R, bool VEL, bool BC, bool POS>
+__global__ __launch_bounds__(256) void k_node_lagrange_3d(Params P, DevScalars* S, NodeLag3ArgsT<R> A) {
+  if (S->stop || S->err) return;
+  const long nid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (nid >= P.nn) return;
+  const int nnx = P.nx + 1, nny = P.ny + 1;
+  const int i = (int)(nid % nnx), j = (int)((nid / nnx) % nny), k = (int)(nid / ((long)nnx * nny));
+  const double dt = S->dt;
+  double vx = A.b.ux[nid], vy = A.b.uy[nid], vz = A.uz[nid];
+  if (VEL) {
+    const double dtv = 0.5 * (S->dt_old + S->dt);
+    V3 F = {0., 0., 0.};
+    double msum = 0.;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int di = (t & 1) - 1, dj = ((t >> 1) & 1) - 1, dk = (t >> 2) - 1;
+      const int ci = i + di, cj = j + dj, ck = k + dk;
+      if (ci < 0 || ci >= P.nx || cj < 0 || cj >= P.ny || ck < 0 || ck >= P.nz) continue;
+      // the node's index in the cell (nodeCells order, hexa8 numbering)
+      const int q = ((dj == 0) ? (di == 0 ? 0 : 1) : (di == 0 ? 3 : 2)) + (dk == 0 ? 0 : 4);
+      V3 cn[8], fc[6];
+      hex_nodes(P, A.b.xpx, A.b.xpy, A.xpz, ci, cj, ck, cn);
+#pragma unroll
+      for (int f = 0; f < 6; ++f) fc[f] = hex_face_centre(cn, f);
+      const V3 C = hex_corner_normal(cn, fc, q);
+      const long c = cid3(P, ci, cj, ck);
+      const double pq = (double)A.b.p[c] + (double)A.b.q[c];
+      F = v3add(F, v3mul(pq, C));
+      msum += (double)A.b.m[c];
+    }
+    const double mn = 0.125 * msum;
+    const double s = dtv / mn;
+    vx = vx + s * F.x;
+    vy = vy + s * F.y;
+    vz = vz + s * F.z;
+  }
+  if (BC) {
+    for (int b = 0; b < P.nbc; ++b) {
+      const BcRange& r = P.bc[b];
+      const int idx = (r.axis == 0) ? i : (r.axis == 1 ? j : k);
+      if (r.axis < 0 || (idx >= r.lo && idx <= r.hi)) {
+        if (r.comp == 0) vx = r.value;
+        else if (r.comp == 1) vy = r.value;
+        else vz = r.value;
+      }
+    }
+  }
+  if (VEL || BC) {
+    st_stream(&A.b.ux[nid], (R)vx);
+    st_stream(&A.b.uy[nid], (R)vy);
+    st_stream(&A.uz[nid], (R)vz);
+  }
+  if (POS) {
+    st_stream(&A.b.xnx[nid], x0_3d(P, 0, i) + dt * vx);
+    st_stream(&A.b.xny[nid], x0_3d(P, 1, j) + dt * vy);
+    st_stream(&A.xnz[nid], x0_3d(P, 2, k) + dt * vz);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Cell phase: volume and length of the hexahedron (MahycoModule.cc:1054-1267),
+// then the shared density / viscosity / energy / remap-vector arithmetic.
+// ---------------------------------------------------------------------------
+template <class R, bool PC, int NEM>
+__global__ __launch_bounds__(256) void k_cell_lagrange_3d(Params P, DevScalars* S, CellLag3ArgsT<R> A) {
+  if (S->stop || S->err) return;
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= P.nc) return;
+  const int i = (int)(c % P.nx), j = (int)((c / P.nx) % P.ny), k = (int)(c / ((long)P.nx * P.ny));
+  V3 cn[8];
+  hex_nodes(P, A.b.xnx, A.b.xny, A.xnz, i, j, k, cn);
+  const double Vd = hex_volume(cn);
+  const double Ld = (P.lc == MHG_LC_FACES_OPPOSEES) ? hex_length_faces_opposees(cn) : pow(Vd, 1. / 3.);
+  lagrange_cell_bulk<R, PC, NEM>(P, S, A.b, c, Vd, Ld);
+}
+
+// ---------------------------------------------------------------------------
+// Cell sweeps.  Geometry of cell c along D: centre 0.125 * (sum of its 8
+// nodes) (cellCoordFromNodes), back face along D (node index s along D):
+// centre sum of 0.25 * x over its 4 nodes, velocity 0.25 * (sum of the 4
+// node velocities) . e_D, length |0.5 (x2 - x0) x (x3 - x1)|_D
+// (PrepareRemap.cc:15-123).  Fluxes of a cell: back face then front face
+// (hexa8 face order), contrib = 0.25 * dual flux (DualPhaseRemap.cc:147-188).
+// ---------------------------------------------------------------------------
+// Face nodes of the D-face with node index (i, j, k) in Arcane order.
+__device__ inline void face_nodes3(const Params& P, int D, int i, int j, int k, long* n) {
+  if (D == 0) {
+    n[0] = nid3(P, i, j, k); n[1] = nid3(P, i, j + 1, k);
+    n[2] = nid3(P, i, j + 1, k + 1); n[3] = nid3(P, i, j, k + 1);
+  } else if (D == 1) {
+    n[0] = nid3(P, i, j, k); n[1] = nid3(P, i + 1, j, k);
+    n[2] = nid3(P, i + 1, j, k + 1); n[3] = nid3(P, i, j, k + 1);
+  } else {
+    n[0] = nid3(P, i, j, k); n[1] = nid3(P, i + 1, j, k);
+    n[2] = nid3(P, i + 1, j + 1, k); n[3] = nid3(P, i, j + 1, k);
+  }
+}
+
+// Geometry of one cell of a sweep along D: centre component, and its back
+// face's centre component, velocity and length.
+template <class R, int D>
+__device__ inline void sweep_geometry3(const Params& P, const CellSweep3ArgsT<R>& A, int i, int j, int k,
+                                       bool cell_ok, bool face_ok, double* xc, double* xf, R* uf,
+                                       double* lf) {
+  const double* X[3] = {A.b.xnx, A.b.xny, A.xnz};
+  const R* U3[3] = {A.b.ux, A.b.uy, A.uz};
+  const double* Xd = X[D];
+  *xc = 0.;
+  if (cell_ok) {
+    double somme = 0.;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int di = (q == 1 || q == 2 || q == 5 || q == 6) ? 1 : 0;
+      const int dj = (q == 2 || q == 3 || q == 6 || q == 7) ? 1 : 0;
+      const int dk = q >= 4 ? 1 : 0;
+      somme += Xd[nid3(P, i + di, j + dj, k + dk)];
+    }
+    *xc = 0.125 * somme;
+  }
+  *xf = 0.;
+  *uf = R(0);
+  *lf = 0.;
+  if (face_ok) {
+    long n[4];
+    face_nodes3(P, D, i, j, k, n);
+    double fc = 0., vm = 0.;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      fc += 0.25 * Xd[n[q]];
+      vm += (double)U3[D][n[q]];
+    }
+    *xf = fc;
+    *uf = (R)(0.25 * vm);
+    const V3 x0 = {A.b.xnx[n[0]], A.b.xny[n[0]], A.xnz[n[0]]};
+    const V3 x1 = {A.b.xnx[n[1]], A.b.xny[n[1]], A.xnz[n[1]]};
+    const V3 x2 = {A.b.xnx[n[2]], A.b.xny[n[2]], A.xnz[n[2]]};
+    const V3 x3 = {A.b.xnx[n[3]], A.b.xny[n[3]], A.xnz[n[3]]};
+    const V3 t = v3mul(0.5, v3cross(v3sub(x2, x0), v3sub(x3, x1)));
+    *lf = fabs(D == 0 ? t.x : (D == 1 ? t.y : t.z));
+  }
+}
+
+// x sweep: a wavefront holds 64 consecutive cells of a row (j, k).
+template <class R, int NEM, bool FIRST, int LIM>
+__global__ __launch_bounds__(256) void k_cell_sweep_x_3d(Params P, DevScalars* S, CellSweep3ArgsT<R> A) {
+  if (S->stop || S->err) return;
+  const int lane = threadIdx.x & 63;
+  const int N = P.nx;
+  const int nseg = (N + 59) / 60;
+  const long wave = ((long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
+  const long line = wave / nseg;
+  if (line >= (long)P.ny * P.nz) return;  // uniform across the wave
+  const int seg = (int)(wave % nseg);
+  const int j = (int)(line % P.ny), k = (int)(line / P.ny);
+  const int i = seg * 60 - 2 + lane;
+  const bool ex = (i >= 0 && i < N);
+  const bool out = ex && lane >= 2 && lane <= 61;
+  const long NC = P.nc;
+  const int ne = P.ne;
+  const double dt = S->dt;
+  const R order2 = (R)P.order2;
+  const long c = ex ? cid3(P, i, j, k) : 0;
+  const R* __restrict__ Uin = A.b.Uin;
+  double xc, xf, lf;
+  R uf;
+  sweep_geometry3<R, 0>(P, A, i, j, k, ex, i >= 0 && i <= N, &xc, &xf, &uf, &lf);
+  R den = R(1);
+  R pre[3][NEM];
+  const R preq = ex ? Uin[(3 * ne) * NC + c] : R(0);
+#pragma unroll
+  for (int g = 0; g < 3; ++g)
+#pragma unroll
+    for (int q = 0; q < NEM; ++q) pre[g][q] = (ex && q < ne) ? Uin[(g * ne + q) * NC + c] : R(0);
+  if (ex) {
+    if (FIRST) {
+      den = A.b.Vlag[c];
+    } else {
+      R sv = R(0);
+#pragma unroll
+      for (int q = 0; q < NEM; ++q)
+        if (q < ne) sv += pre[0][q];
+      den = sv;
+    }
+  }
+  const bool inner = ex && i >= 1;
+  const double xcm = __shfl_up(xc, 1, 64);
+  const double dx = inner ? xc - xcm : 1.;
+  const bool upm = inner && ((double)uf * dx > 0.0);
+  const double dotxf = inner ? (upm ? xf - xcm : xf - xc) : 0.;
+  const R uf_f = __shfl_down(uf, 1, 64);
+  const double lf_f = __shfl_down(lf, 1, 64);
+  const R fb = (R)(-1.0 * uf * lf * dt), ff = (R)(1.0 * uf_f * lf_f * dt);
+  const R dxr = (R)dx, dotr = (R)dotxf;
+  const bool o2 = P.order2 > 0;
+  auto sweep_var = [&](R u0, R& un, R& dual) {
+    const R phi = ex ? qdiv(u0, den) : R(0);
+    const R phim = __shfl_up(phi, 1, 64);
+    const R gf = face_grad(inner, phi, phim, dxr);
+    const R gfp = __shfl_down(gf, 1, 64);
+    const R g = o2 ? limited_grad<LIM, true, R, kBranchFree<R>>(gf, gfp) : R(0);
+    const R gm = __shfl_up(g, 1, 64);
+    const R phif = inner ? (upm ? phim + order2 * dotr * gm : phi + order2 * dotr * g) : R(0);
+    const R phifp = __shfl_down(phif, 1, 64);
+    const R flux_b = fb * phif, flux_f = ff * phifp;
+    R fsum = R(0);  // faces in hexa8 order: back (x-) before front (x+)
+    fsum = fsum + flux_b;
+    fsum = fsum + flux_f;
+    un = u0 - fsum;
+    R d = R(0);
+    d = d + R(0.5) * flux_b * R(-1.0);
+    d = d + R(0.5) * flux_f * R(1.0);
+    dual = d;
+  };
+  R* __restrict__ Uout = A.b.Uout;
+#pragma unroll
+  for (int g = 0; g < 3; ++g)
+#pragma unroll
+    for (int q = 0; q < NEM; ++q) {
+      if (q >= ne) break;
+      R un, du;
+      sweep_var(pre[g][q], un, du);
+      if (g > 0 && un < R(0)) un = R(0);  // RemapADIService.cc:953-958
+      if (out) {
+        st_stream(&Uout[(g * ne + q) * NC + c], un);
+        if (g == 1) st_stream(&A.b.contrib[q * NC + c], R(0.25) * du);
+      }
+    }
+  {
+    R un, du;
+    sweep_var(preq, un, du);
+    if (out) st_stream(&Uout[(3 * ne) * NC + c], un);
+  }
+}
+
+// y and z sweeps on LDS tiles: a block of 16 wavefronts owns 16 columns (i)
+// and 64 rows along D (60 outputs) at one index of the third direction; the
+// tile is transposed through LDS so that a wavefront holds one column.
+template <class R, int D, int NEM, bool FIRST, int LIM>
+__global__ __launch_bounds__(1024) void k_cell_sweep_tile_3d(Params P, DevScalars* S, CellSweep3ArgsT<R> A) {
+  __shared__ double lds[6][kTileR][kTileP];
+  typedef R Tile[kTileR][kTileP];
+  Tile* stg = reinterpret_cast<Tile*>(&lds[0][0][0]);
+  if (S->stop || S->err) return;  // uniform over the grid
+  const int nx = P.nx;
+  const int N = (D == 1) ? P.ny : P.nz;      // cells along D
+  const int T = (D == 1) ? P.nz : P.ny;      // the third direction
+  const int ncolblk = (nx + kTileW - 1) / kTileW;
+  const int nseg = (N + 59) / 60;
+  const int L = xcd_block(blockIdx.x, gridDim.x);
+  const int i0 = (L % ncolblk) * kTileW;
+  const int s0 = ((L / ncolblk) % nseg) * 60 - 2;
+  const int o = L / (ncolblk * nseg);        // index along the third direction
+  if (o >= T) return;                        // uniform over the block
+  const int t = threadIdx.x;
+  const int tr = t >> 4, tc = t & 15;
+  const int w = t >> 6, lane = t & 63;
+  const long NC = P.nc;
+  const int ne = P.ne;
+  const double dt = S->dt;
+  const R order2 = (R)P.order2;
+  const bool o2 = P.order2 > 0;
+  const R* __restrict__ Uin = A.b.Uin;
+  auto cell_ijk = [&](int ii, int s, int* ci, int* cj, int* ck) {
+    *ci = ii;
+    *cj = (D == 1) ? s : o;
+    *ck = (D == 1) ? o : s;
+  };
+  const int gi = i0 + tc, gs = s0 + tr;
+  const bool gex = gi < nx && gs >= 0 && gs < N;
+  const bool gout = gex && tr >= 2 && tr <= 61;
+  int ci, cj, ck;
+  cell_ijk(gi, gs, &ci, &cj, &ck);
+  const long gc = gex ? cid3(P, ci, cj, ck) : 0;
+  {
+    double xc, xf, lf;
+    R uf, den = R(1);
+    sweep_geometry3<R, D>(P, A, ci, cj, ck, gex, gi < nx && gs >= 0 && gs <= N, &xc, &xf, &uf, &lf);
+    if (gex) {
+      if (FIRST) {
+        den = A.b.Vlag[gc];
+      } else {
+        R sv = R(0);
+#pragma unroll
+        for (int q = 0; q < NEM; ++q)
+          if (q < ne) sv += Uin[q * NC + gc];
+        den = sv;
+      }
+    }
+    lds[0][tr][tc] = xc;
+    lds[1][tr][tc] = (double)den;
+    lds[2][tr][tc] = (double)uf;
+    lds[3][tr][tc] = xf;
+    lds[4][tr][tc] = lf;
+  }
+  __syncthreads();
+  const int s = s0 + lane;
+  const bool ex = (i0 + w < nx) && s >= 0 && s < N;
+  const bool inner = ex && s >= 1;
+  const double xc = lds[0][lane][w], xf = lds[3][lane][w], lf = lds[4][lane][w];
+  const R den = (R)lds[1][lane][w], uf = (R)lds[2][lane][w];
+  const double xcm = __shfl_up(xc, 1, 64);
+  const double dx = inner ? xc - xcm : 1.;
+  const bool upm = inner && ((double)uf * dx > 0.0);
+  const double dotxf = inner ? (upm ? xf - xcm : xf - xc) : 0.;
+  const R uf_f = __shfl_down(uf, 1, 64);
+  const double lf_f = __shfl_down(lf, 1, 64);
+  const R fb = (R)(-1.0 * uf * lf * dt), ff = (R)(1.0 * uf_f * lf_f * dt);
+  const R dxr = (R)dx, dotr = (R)dotxf;
+  auto sweep_var = [&](R u0, R& un, R& dual) {
+    const R phi = ex ? qdiv(u0, den) : R(0);
+    const R phim = __shfl_up(phi, 1, 64);
+    const R gf = face_grad(inner, phi, phim, dxr);
+    const R gfp = __shfl_down(gf, 1, 64);
+    const R g = o2 ? limited_grad<LIM, true, R, kBranchFree<R>>(gf, gfp) : R(0);
+    const R gm = __shfl_up(g, 1, 64);
+    const R phif = inner ? (upm ? phim + order2 * dotr * gm : phi + order2 * dotr * g) : R(0);
+    const R phifp = __shfl_down(phif, 1, 64);
+    const R flux_b = fb * phif, flux_f = ff * phifp;
+    R fsum = R(0);  // faces in hexa8 order: back before front
+    fsum = fsum + flux_b;
+    fsum = fsum + flux_f;
+    un = u0 - fsum;
+    R d = R(0);
+    d = d + R(0.5) * flux_b * R(-1.0);
+    d = d + R(0.5) * flux_f * R(1.0);
+    dual = d;
+  };
+  __syncthreads();  // geometry reads done: the tiles become staging tiles
+  const int nvu = P.nvu;
+  R nxa = gex ? Uin[gc] : R(0);
+  R nxb = (gex && nvu > 1) ? Uin[NC + gc] : R(0);
+  for (int va = 0; va < nvu; va += 2) {
+    const int vb = va + 1;
+    const bool hb = vb < nvu;  // uniform
+    stg[0][tr][tc] = nxa;
+    stg[1][tr][tc] = nxb;
+    __syncthreads();
+    if (va + 2 < nvu) nxa = gex ? Uin[(va + 2) * NC + gc] : R(0);
+    if (va + 3 < nvu) nxb = gex ? Uin[(va + 3) * NC + gc] : R(0);
+    const R ua = stg[0][lane][w], ub = stg[1][lane][w];
+    R una, dua, unb, dub;
+    sweep_var(ua, una, dua);
+    sweep_var(ub, unb, dub);
+    if (va >= ne && va < 3 * ne && una < R(0)) una = R(0);  // RemapADIService.cc:953-958
+    if (vb >= ne && vb < 3 * ne && unb < R(0)) unb = R(0);
+    const bool ma = (va >= ne && va < 2 * ne), mb = hb && (vb >= ne && vb < 2 * ne);
+    stg[2][lane][w] = una;
+    stg[3][lane][w] = unb;
+    if (ma) stg[4][lane][w] = R(0.25) * dua;
+    if (mb) stg[5][lane][w] = R(0.25) * dub;
+    __syncthreads();
+    if (gout) {
+      st_stream(&A.b.Uout[va * NC + gc], stg[2][tr][tc]);
+      if (hb) st_stream(&A.b.Uout[vb * NC + gc], stg[3][tr][tc]);
+      if (ma) st_stream(&A.b.contrib[(va - ne) * NC + gc], stg[4][tr][tc]);
+      if (mb) st_stream(&A.b.contrib[(vb - ne) * NC + gc], stg[5][tr][tc]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Dual sweeps (DualPhaseRemap.cc:22-845).  phi_dual of a node is U/m if the
+// node was inner in a direction already swept in this remap (494-523), else
+// its velocity; the back flux of a node is the front flux of its neighbour
+// behind it; front(node) sums 0.25 * dual flux of the 4 cells ahead of it
+// (dir1 = D+1 outer, dir2 = D+2 inner, environments innermost).
+// ---------------------------------------------------------------------------
+template <class R>
+struct DualNode {
+  R U[4];    // dual state before the sweep
+  R p[3];    // phi_dual
+  double xd; // coordinate along D
+  R front;   // front mass flux
+};
+
+template <class R, int D, bool FIRST, int NEM>
+__device__ inline DualNode<R> dual_node_load(const Params& P, const NodeSweep3ArgsT<R>& A, int i, int j, int k,
+                                             bool ok) {
+  DualNode<R> d;
+  for (int q = 0; q < 4; ++q) d.U[q] = R(0);
+  for (int q = 0; q < 3; ++q) d.p[q] = R(0);
+  d.xd = 0.;
+  d.front = R(0);
+  if (!ok) return d;
+  const long nid = nid3(P, i, j, k);
+  const double* X[3] = {A.xnx, A.xny, A.xnz};
+  d.xd = X[D][nid];
+  const R u[3] = {A.ux[nid], A.uy[nid], A.uz[nid]};
+  if (FIRST) {
+    R msum = R(0);  // node mass 0.125 * sum of the 8 cells (computeNodeMass)
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int ci = i + (t & 1) - 1, cj = j + ((t >> 1) & 1) - 1, ck = k + (t >> 2) - 1;
+      if (ci < 0 || ci >= P.nx || cj < 0 || cj >= P.ny || ck < 0 || ck >= P.nz) continue;
+      msum += A.m[cid3(P, ci, cj, ck)];
+    }
+    const R mn = R(0.125) * msum;
+    for (int q = 0; q < 3; ++q) {
+      d.p[q] = u[q];
+      d.U[q] = mn * u[q];
+    }
+    d.U[3] = mn;
+  } else {
+    d.U[0] = A.Udi0[nid];
+    d.U[1] = A.Udi1[nid];
+    d.U[2] = A.Udi2[nid];
+    d.U[3] = A.Udi3[nid];
+    const int idx[3] = {i, j, k};
+    const int nn[3] = {P.nx, P.ny, P.nz};
+    bool inner_prev = false;
+    for (int e = 0; e < 3; ++e)
+      if (((A.prevmask >> e) & 1) && idx[e] >= 1 && idx[e] <= nn[e] - 1) inner_prev = true;
+    for (int q = 0; q < 3; ++q) d.p[q] = inner_prev ? qdiv(d.U[q], d.U[3]) : u[q];
+  }
+  // front cells: index D at the node, dir1 and dir2 shifted by -1 and 0
+  const int d1 = (D + 1) % 3, d2 = (D + 2) % 3;
+  const int nn3[3] = {P.nx, P.ny, P.nz};
+  const int base[3] = {i, j, k};
+  if (base[D] < nn3[D]) {
+    R f = R(0);
+#pragma unroll
+    for (int s1 = -1; s1 <= 0; ++s1)
+#pragma unroll
+      for (int s2 = -1; s2 <= 0; ++s2) {
+        int cc[3] = {i, j, k};
+        cc[d1] += s1;
+        cc[d2] += s2;
+        if (cc[d1] < 0 || cc[d1] >= nn3[d1] || cc[d2] < 0 || cc[d2] >= nn3[d2]) continue;
+        const long cell = cid3(P, cc[0], cc[1], cc[2]);
+#pragma unroll
+        for (int e = 0; e < NEM; ++e) {
+          if (e >= P.ne) break;
+          f += A.contrib[e * P.nc + cell];
+        }
+      }
+    d.front = f;
+  }
+  return d;
+}
+
+// Increments of the dual state of an inner node along D (352-491), given the
+// node and its two neighbours along D (phi, coordinate, gradient).
+template <class R, int D>
+__device__ inline void dual_increments(const Params& P, double dt, const R* pm, const R* p0, const R* pp, double xdm,
+                                       double xd, double xdp, const R* gm, const R* g0, const R* gp, R back, R front,
+                                       R* dU) {
+  const R order2 = (R)P.order2;
+  const R ufront = R(0.5) * (p0[D] + p0[D]);  // DualPhaseRemap.cc:391
+  const int signfront = (ufront > R(0)) ? 1 : -1;
+  const double xfr = xdp - xd;
+  const bool sf = (front < R(0));
+  const R tf = (R)(signfront * xfr - dt * ufront);
+  const R uback = R(0.5) * (pm[D] + p0[D]);
+  const int signback = (uback > R(0)) ? 1 : -1;
+  const double xbk = xd - xdm;
+  const bool sb = (back > R(0));
+  const R tb = (R)(signback * xbk - dt * uback);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const R FV = (sf ? pp[q] : p0[q]) + order2 * (R(0.5) * (sf ? gp[q] : g0[q]) * tf);
+    const R BV = (sb ? pm[q] : p0[q]) + order2 * (R(0.5) * (sb ? gm[q] : g0[q]) * tb);
+    dU[q] = back * BV - front * FV;
+  }
+  dU[3] = back - front;
+}
+
+template <class R, bool FIRST, bool LAST>
+__device__ inline void dual_store(const NodeSweep3ArgsT<R>& A, long nid, const R* U) {
+  if (!LAST) {
+    st_stream(&A.Udo0[nid], U[0]);
+    st_stream(&A.Udo1[nid], U[1]);
+    st_stream(&A.Udo2[nid], U[2]);
+    st_stream(&A.Udo3[nid], U[3]);
+  } else if (U[3] != R(0)) {  // RemapADIFinal.cc:660-672
+    st_stream(&A.uox[nid], qdiv(U[0], U[3]));
+    st_stream(&A.uoy[nid], qdiv(U[1], U[3]));
+    st_stream(&A.uoz[nid], qdiv(U[2], U[3]));
+  } else {
+    st_stream(&A.uox[nid], A.ux[nid]);
+    st_stream(&A.uoy[nid], A.uy[nid]);
+    st_stream(&A.uoz[nid], A.uz[nid]);
+  }
+}
+
+// True if the node lies on the domain boundary of a direction other than D:
+// its gradient along D is read back as a stale value by a later sweep.
+template <int D>
+__device__ inline bool on_other_boundary(const Params& P, int i, int j, int k) {
+  const int idx[3] = {i, j, k};
+  const int nn[3] = {P.nx, P.ny, P.nz};
+  bool b = false;
+#pragma unroll
+  for (int e = 0; e < 3; ++e)
+    if (e != D && (idx[e] == 0 || idx[e] == nn[e])) b = true;
+  return b;
+}
+
+template <class R, int D, bool FIRST, bool LAST, int NEM, int LIM>
+__global__ __launch_bounds__(256) void k_node_sweep_x_3d(Params P, DevScalars* S, NodeSweep3ArgsT<R> A) {
+  if (S->stop || S->err) return;
+  const int lane = threadIdx.x & 63;
+  const int N = P.nx;
+  const int nseg = (N + 1 + 59) / 60;
+  const long wave = ((long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
+  const long line = wave / nseg;
+  if (line >= (long)(P.ny + 1) * (P.nz + 1)) return;  // uniform across the wave
+  const int seg = (int)(wave % nseg);
+  const int j = (int)(line % (P.ny + 1)), k = (int)(line / (P.ny + 1));
+  const int i = seg * 60 - 2 + lane;
+  const bool ex = (i >= 0 && i <= N);
+  const bool out = ex && lane >= 2 && lane <= 61;
+  const bool inner_d = (i >= 1 && i <= N - 1);
+  const double dt = S->dt;
+  const DualNode<R> nd = dual_node_load<R, 0, FIRST, NEM>(P, A, i, j, k, ex);
+  const long nid = ex ? nid3(P, i, j, k) : 0;
+  R pm[3], pp[3], g0[3] = {R(0), R(0), R(0)}, gm[3], gp[3];
+  const double xdm = __shfl_up(nd.xd, 1, 64), xdp = __shfl_down(nd.xd, 1, 64);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    pm[q] = __shfl_up(nd.p[q], 1, 64);
+    pp[q] = __shfl_down(nd.p[q], 1, 64);
+  }
+  const R back = __shfl_up(nd.front, 1, 64);
+  R* dg[3] = {A.dg0, A.dg1, A.dg2};
+  if (P.order2 > 0 && ex) {
+    if (inner_d) {
+      const R dxf = (R)(xdp - nd.xd), dxb = (R)(nd.xd - xdm);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const R gf = qdiv(pp[q] - nd.p[q], dxf), gb = qdiv(nd.p[q] - pm[q], dxb);
+        g0[q] = limited_grad<LIM, false, R, kBranchFree<R>>(gb, gf);
+      }
+      if (out && on_other_boundary<0>(P, i, j, k))
+        for (int q = 0; q < 3; ++q) dg[q][nid] = g0[q];
+    } else {
+      for (int q = 0; q < 3; ++q) g0[q] = dg[q][nid];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    gm[q] = __shfl_up(g0[q], 1, 64);
+    gp[q] = __shfl_down(g0[q], 1, 64);
+  }
+  if (!out) return;
+  R U[4] = {nd.U[0], nd.U[1], nd.U[2], nd.U[3]};
+  if (inner_d) {
+    R dU[4];
+    dual_increments<R, 0>(P, dt, pm, nd.p, pp, xdm, nd.xd, xdp, gm, g0, gp, back, nd.front, dU);
+    U[3] += dU[3];
+    for (int q = 0; q < 3; ++q) U[q] += dU[q];
+    const R thresold = (R)1.e-300;  // 0 in float: the filter only acts in double
+    for (int q = 0; q < 3; ++q)
+      if (fabs(U[q]) < thresold) U[q] = R(0);
+  }
+  dual_store<R, FIRST, LAST>(A, nid, U);
+}
+
+// y and z dual sweeps on LDS tiles (columns i, rows along D).
+template <class R, int D, bool FIRST, bool LAST, int NEM, int LIM>
+__global__ __launch_bounds__(1024) void k_node_sweep_tile_3d(Params P, DevScalars* S, NodeSweep3ArgsT<R> A) {
+  __shared__ double ldx[kTileR][kTileP];
+  __shared__ R lds[4][kTileR][kTileP];
+  if (S->stop || S->err) return;
+  const int nx = P.nx, nnx = nx + 1;
+  const int N = (D == 1) ? P.ny : P.nz;  // nodes 0..N along D
+  const int T = (D == 1) ? P.nz : P.ny;  // nodes 0..T along the third direction
+  const int ncolblk = (nnx + kTileW - 1) / kTileW;
+  const int nseg = (N + 1 + 59) / 60;
+  const int L = xcd_block(blockIdx.x, gridDim.x);
+  const int i0 = (L % ncolblk) * kTileW;
+  const int s0 = ((L / ncolblk) % nseg) * 60 - 2;
+  const int o = L / (ncolblk * nseg);
+  if (o > T) return;  // uniform over the block
+  const int t = threadIdx.x;
+  const int tr = t >> 4, tc = t & 15;
+  const int w = t >> 6, lane = t & 63;
+  const double dt = S->dt;
+  auto node_ijk = [&](int ii, int s, int* ni, int* nj, int* nk) {
+    *ni = ii;
+    *nj = (D == 1) ? s : o;
+    *nk = (D == 1) ? o : s;
+  };
+  // ---- row-major: load the nodes
+  const int gi = i0 + tc, gs = s0 + tr;
+  const bool gex = gi <= nx && gs >= 0 && gs <= N;
+  const bool gout = gex && tr >= 2 && tr <= 61;
+  int ni, nj, nk;
+  node_ijk(gi, gs, &ni, &nj, &nk);
+  const DualNode<R> nd = dual_node_load<R, D, FIRST, NEM>(P, A, ni, nj, nk, gex);
+  const long gq = gex ? nid3(P, ni, nj, nk) : 0;
+  ldx[tr][tc] = nd.xd;
+  lds[0][tr][tc] = nd.p[0];
+  lds[1][tr][tc] = nd.p[1];
+  lds[2][tr][tc] = nd.p[2];
+  lds[3][tr][tc] = nd.front;
+  __syncthreads();
+  // ---- transposed: column i, row s
+  const int i = i0 + w, s = s0 + lane;
+  const bool ex = i <= nx && s >= 0 && s <= N;
+  const bool out = ex && lane >= 2 && lane <= 61;
+  const bool inner_d = (s >= 1 && s <= N - 1);
+  int ti, tj, tk;
+  node_ijk(i, s, &ti, &tj, &tk);
+  const double xd = ldx[lane][w];
+  R p0[3] = {lds[0][lane][w], lds[1][lane][w], lds[2][lane][w]};
+  const R front = lds[3][lane][w];
+  const double xdm = __shfl_up(xd, 1, 64), xdp = __shfl_down(xd, 1, 64);
+  R pm[3], pp[3], g0[3] = {R(0), R(0), R(0)}, gm[3], gp[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    pm[q] = __shfl_up(p0[q], 1, 64);
+    pp[q] = __shfl_down(p0[q], 1, 64);
+  }
+  const R back = __shfl_up(front, 1, 64);
+  R* dg[3] = {A.dg0, A.dg1, A.dg2};
+  if (P.order2 > 0 && ex) {
+    const long q0 = nid3(P, ti, tj, tk);
+    if (inner_d) {
+      const R dxf = (R)(xdp - xd), dxb = (R)(xd - xdm);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const R gf = qdiv(pp[q] - p0[q], dxf), gb = qdiv(p0[q] - pm[q], dxb);
+        g0[q] = limited_grad<LIM, false, R, kBranchFree<R>>(gb, gf);
+      }
+      if (out && on_other_boundary<D>(P, ti, tj, tk))
+        for (int q = 0; q < 3; ++q) dg[q][q0] = g0[q];
+    } else {
+      for (int q = 0; q < 3; ++q) g0[q] = dg[q][q0];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    gm[q] = __shfl_up(g0[q], 1, 64);
+    gp[q] = __shfl_down(g0[q], 1, 64);
+  }
+  R dU[4] = {R(0), R(0), R(0), R(0)};
+  if (out && inner_d) dual_increments<R, D>(P, dt, pm, p0, pp, xdm, xd, xdp, gm, g0, gp, back, front, dU);
+  __syncthreads();  // transposed reads done before the tiles are reused
+#pragma unroll
+  for (int q = 0; q < 4; ++q) lds[q][lane][w] = dU[q];
+  __syncthreads();
+  // ---- row-major: U += increments, store
+  if (!gout) return;
+  R U[4] = {nd.U[0], nd.U[1], nd.U[2], nd.U[3]};
+  if (gs >= 1 && gs <= N - 1) {
+    U[3] += lds[3][tr][tc];
+    for (int q = 0; q < 3; ++q) U[q] += lds[q][tr][tc];
+    const R thresold = (R)1.e-300;  // 0 in float: the filter only acts in double
+    for (int q = 0; q < 3; ++q)
+      if (fabs(U[q]) < thresold) U[q] = R(0);
+  }
+  dual_store<R, FIRST, LAST>(A, gq, U);
+}
+
+// ---------------------------------------------------------------------------
+// Final cell phase with the stored Euler volume; CFL over the 8 nodes.
+// ---------------------------------------------------------------------------
+template <class R, int BLOCK, int NEM>
+__global__ __launch_bounds__(BLOCK) void k_cell_final_3d(Params P, DevScalars* S, CellFinal3ArgsT<R> A) {
+  if (S->stop || S->err) return;
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double dx_sound = 1.7976931348623157e308;
+  if (c < P.nc) {
+    const int i = (int)(c % P.nx), j = (int)((c / P.nx) % P.ny), k = (int)(c / ((long)P.nx * P.ny));
+    const R c_g = final_cell<R, NEM>(P, A.b, c, A.Veuler[c]);
+    R vmax = R(0);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int di = (q == 1 || q == 2 || q == 5 || q == 6) ? 1 : 0;
+      const int dj = (q == 2 || q == 3 || q == 6 || q == 7) ? 1 : 0;
+      const int dk = q >= 4 ? 1 : 0;
+      const long n = nid3(P, i + di, j + dj, k + dk);
+      const R ux = A.b.ux[n], uy = A.b.uy[n], uz = A.uz[n];
+      const R un = sqrt(ux * ux + uy * uy + uz * uz);
+      vmax = (un < vmax) ? vmax : un;
+    }
+    dx_sound = smin(dx_sound, (double)qdiv(A.b.L[c], c_g + vmax));
+  }
+  block_min_store<BLOCK>(dx_sound, A.b.block_min);
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+static inline unsigned nblocks3(long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+#define MHG3_NEM(ne, F)         \
+  do {                          \
+    if ((ne) <= 1) F(1);        \
+    else if ((ne) <= 2) F(2);   \
+    else if ((ne) <= 3) F(3);   \
+    else if ((ne) <= 4) F(4);   \
+    else F(8);                  \
+  } while (0)
+#define MHG3_LIM(lim, F)       \
+  do {                         \
+    if ((lim) == 0) F(0);      \
+    else if ((lim) == 1) F(1); \
+    else F(2);                 \
+  } while (0)
+
+template <class R>
+void launch_node_lagrange_3d(hipStream_t st, const Params& P, DevScalars* S, const NodeLag3ArgsT<R>& A, bool vel,
+                             bool bc, bool pos) {
+  dim3 g(nblocks3(P.nn, 256)), b(256);
+  if (vel && bc && pos) hipLaunchKernelGGL((k_node_lagrange_3d<R, true, true, true>), g, b, 0, st, P, S, A);
+  else if (vel && !bc && !pos) hipLaunchKernelGGL((k_node_lagrange_3d<R, true, false, false>), g, b, 0, st, P, S, A);
+  else if (!vel && bc && !pos) hipLaunchKernelGGL((k_node_lagrange_3d<R, false, true, false>), g, b, 0, st, P, S, A);
+  else if (!vel && !bc && pos) hipLaunchKernelGGL((k_node_lagrange_3d<R, false, false, true>), g, b, 0, st, P, S, A);
+}
+
+template <class R>
+void launch_cell_lagrange_3d(hipStream_t st, const Params& P, DevScalars* S, const CellLag3ArgsT<R>& A) {
+  dim3 g(nblocks3(P.nc, 256)), b(256);
+#define L_(N)                                                                                     \
+  if (P.pseudo_centree) hipLaunchKernelGGL((k_cell_lagrange_3d<R, true, N>), g, b, 0, st, P, S, A); \
+  else hipLaunchKernelGGL((k_cell_lagrange_3d<R, false, N>), g, b, 0, st, P, S, A)
+  MHG3_NEM(P.ne, L_);
+#undef L_
+}
+
+template <class R, int D, int NEM, int LM>
+static void launch_sweep3(hipStream_t st, const Params& P, DevScalars* S, const CellSweep3ArgsT<R>& A, bool first) {
+  if (D == 0) {
+    const long waves = (long)((P.nx + 59) / 60) * P.ny * P.nz;
+    dim3 g(nblocks3(waves * 64, 256)), b(256);
+    if (first) hipLaunchKernelGGL((k_cell_sweep_x_3d<R, NEM, true, LM>), g, b, 0, st, P, S, A);
+    else hipLaunchKernelGGL((k_cell_sweep_x_3d<R, NEM, false, LM>), g, b, 0, st, P, S, A);
+    return;
+  }
+  constexpr int DD = D == 0 ? 1 : D;
+  const int N = (DD == 1) ? P.ny : P.nz, T = (DD == 1) ? P.nz : P.ny;
+  dim3 g((unsigned)(((P.nx + kTileW - 1) / kTileW) * ((N + 59) / 60) * T)), b(1024);
+  if (first) hipLaunchKernelGGL((k_cell_sweep_tile_3d<R, DD, NEM, true, LM>), g, b, 0, st, P, S, A);
+  else hipLaunchKernelGGL((k_cell_sweep_tile_3d<R, DD, NEM, false, LM>), g, b, 0, st, P, S, A);
+}
+
+template <class R>
+void launch_cell_sweep_3d(hipStream_t st, const Params& P, DevScalars* S, const CellSweep3ArgsT<R>& A, int dir,
+                          bool first) {
+#define SW_(N, LM)                                                  \
+  do {                                                              \
+    if (dir == 0) launch_sweep3<R, 0, N, LM>(st, P, S, A, first);   \
+    else if (dir == 1) launch_sweep3<R, 1, N, LM>(st, P, S, A, first); \
+    else launch_sweep3<R, 2, N, LM>(st, P, S, A, first);            \
+  } while (0)
+#define SWL_(LM)                     \
+  do {                               \
+    if (P.ne <= 1) SW_(1, LM);       \
+    else if (P.ne <= 2) SW_(2, LM);  \
+    else if (P.ne <= 3) SW_(3, LM);  \
+    else if (P.ne <= 4) SW_(4, LM);  \
+    else SW_(8, LM);                 \
+  } while (0)
+  MHG3_LIM(P.lim, SWL_);
+#undef SWL_
+#undef SW_
+}
+
+template <class R, int D, bool FIRST, bool LAST, int NEM, int LM>
+static void launch_dual3(hipStream_t st, const Params& P, DevScalars* S, const NodeSweep3ArgsT<R>& A) {
+  if (D == 0) {
+    const long waves = (long)((P.nx + 1 + 59) / 60) * (P.ny + 1) * (P.nz + 1);
+    dim3 g(nblocks3(waves * 64, 256)), b(256);
+    hipLaunchKernelGGL((k_node_sweep_x_3d<R, 0, FIRST, LAST, NEM, LM>), g, b, 0, st, P, S, A);
+    return;
+  }
+  constexpr int DD = D == 0 ? 1 : D;
+  const int N = (DD == 1) ? P.ny : P.nz, T = (DD == 1) ? P.nz : P.ny;
+  dim3 g((unsigned)(((P.nx + 1 + kTileW - 1) / kTileW) * ((N + 1 + 59) / 60) * (T + 1))), b(1024);
+  hipLaunchKernelGGL((k_node_sweep_tile_3d<R, DD, FIRST, LAST, NEM, LM>), g, b, 0, st, P, S, A);
+}
+
+template <class R>
+void launch_node_sweep_3d(hipStream_t st, const Params& P, DevScalars* S, const NodeSweep3ArgsT<R>& A, int dir,
+                          bool first, bool last) {
+#define DS_(D, N, LM)                                                                  \
+  do {                                                                                 \
+    if (first) launch_dual3<R, D, true, false, N, LM>(st, P, S, A);                    \
+    else if (last) launch_dual3<R, D, false, true, N, LM>(st, P, S, A);                \
+    else launch_dual3<R, D, false, false, N, LM>(st, P, S, A);                         \
+  } while (0)
+#define DSD_(N, LM)                   \
+  do {                                \
+    if (dir == 0) DS_(0, N, LM);      \
+    else if (dir == 1) DS_(1, N, LM); \
+    else DS_(2, N, LM);               \
+  } while (0)
+#define DSL_(LM)                      \
+  do {                                \
+    if (P.ne <= 1) DSD_(1, LM);       \
+    else if (P.ne <= 2) DSD_(2, LM);  \
+    else if (P.ne <= 3) DSD_(3, LM);  \
+    else if (P.ne <= 4) DSD_(4, LM);  \
+    else DSD_(8, LM);                 \
+  } while (0)
+  MHG3_LIM(P.lim, DSL_);
+#undef DSL_
+#undef DSD_
+#undef DS_
+}
+
+template <class R>
+void launch_cell_final_3d(hipStream_t st, const Params& P, DevScalars* S, const CellFinal3ArgsT<R>& A, int dt_mode) {
+  constexpr int B = 256;
+  dim3 g(nblocks3(P.nc, B)), b(B);
+#define F_(N) hipLaunchKernelGGL((k_cell_final_3d<R, B, N>), g, b, 0, st, P, S, A)
+  MHG3_NEM(P.ne, F_);
+#undef F_
+  if (dt_mode > 0) launch_dt_finalize(st, P, S, A.b.block_min, dt_mode == 2);
+}
+
+#define MHG3_INSTANTIATE(R)                                                                                   \
+  template void launch_node_lagrange_3d<R>(hipStream_t, const Params&, DevScalars*, const NodeLag3ArgsT<R>&, \
+                                           bool, bool, bool);                                                \
+  template void launch_cell_lagrange_3d<R>(hipStream_t, const Params&, DevScalars*, const CellLag3ArgsT<R>&); \
+  template void launch_cell_sweep_3d<R>(hipStream_t, const Params&, DevScalars*, const CellSweep3ArgsT<R>&,   \
+                                        int, bool);                                                          \
+  template void launch_node_sweep_3d<R>(hipStream_t, const Params&, DevScalars*, const NodeSweep3ArgsT<R>&,   \
+                                        int, bool, bool);                                                    \
+  template void launch_cell_final_3d<R>(hipStream_t, const Params&, DevScalars*, const CellFinal3ArgsT<R>&,   \
+                                        int);
+MHG3_INSTANTIATE(double)
+MHG3_INSTANTIATE(float)
+#undef MHG3_INSTANTIATE
+
+}  // namespace mhg

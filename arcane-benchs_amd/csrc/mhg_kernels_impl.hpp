@@ -1,0 +1,301 @@
+// mhg_kernels_impl.hpp -- device functions shared by the 2D and 3D kernels.
+#pragma once
+
+#include <type_traits>
+
+#include "mhg_common.hpp"
+#include "mhg_kernels.hpp"
+
+namespace mhg {
+
+// LDS tiles of the y (and z) sweeps: 16 columns, 64 rows, padded rows.
+constexpr int kTileW = 16, kTileR = 64, kTileP = kTileW + 1;
+
+// Limited gradients: branch-free in float (two hardware-reciprocal quotients
+// cost less than the divergent branches); in double the exact quotients are
+// dear, so minmod divides only where the quotient is not clamped.
+template <class R>
+constexpr bool kBranchFree = std::is_same<R, float>::value;
+
+template <class T>
+__device__ inline T face_grad(bool inner, T phi, T phim, T dx) {
+  return inner ? qdiv(phi - phim, dx) : T(0);
+}
+
+// Density, viscosity, energy and the remap vector of cell c from its
+// Lagrangian volume and characteristic length (both dimensions).
+template <class R, bool PC, int NEM>
+__device__ inline void lagrange_cell_bulk(const Params& P, const DevScalars* S, const CellLagArgsT<R>& A,
+                                          long c, double Vd, double Ld) {
+  // geometry computed in double, the rest in the storage type
+  const R V = (R)Vd, L = (R)Ld;
+  const R inv_dt = qdiv(R(1.0), (R)S->dt);
+  const R rho_n = A.rho[c];
+  const R rho = qdiv(A.m[c], V);
+  const R tau = R(0.5) * (qdiv(R(1.0), rho_n) + qdiv(R(1.0), rho));
+  const R divu = qdiv(inv_dt * (qdiv(R(1.0), rho) - qdiv(R(1.0), rho_n)), tau);
+  const R cs = A.cs[c];
+  const int mask = A.mask[c];
+  const int ne = P.ne;
+  const long NC = P.nc;
+  R uv[NEM], um[NEM], ue[NEM];
+  R uq = R(0);
+#pragma unroll
+  for (int k = 0; k < NEM; ++k) uv[k] = um[k] = ue[k] = R(0);
+  if (__popc(mask) == 1) {
+    const int kk = __ffs(mask) - 1;
+    R qv = R(0);
+    if (divu < R(0)) qv = viscosity((R)P.eos.qcoef[kk], tau, L, cs, divu);
+    const R qn = PC ? A.q[c] : R(0);
+    const R e = compute_eint(PC, (R)P.eos.gamma[kk], qn, qv, rho_n, rho, A.p[c], A.e[c]);
+#pragma unroll
+    for (int k = 0; k < NEM; ++k)
+      if (k == kk) {
+        uv[k] = V;
+        um[k] = V * rho;
+        ue[k] = V * rho * e;
+      }
+    uq = V * qv;
+  } else if (mask != 0) {
+#pragma unroll
+    for (int k = 0; k < NEM; ++k) {
+      if (!(mask & (1 << k))) continue;
+      const long ek = k * NC + c;
+      const R f = A.env_f[ek];
+      const R Vk = f * V;
+      const R rho_nk = A.env_rho[ek];
+      const R rk = qdiv(A.env_m[ek], Vk);
+      const R tk = R(0.5) * (qdiv(R(1.0), rho_nk) + qdiv(R(1.0), rk));
+      R qk = R(0);
+      if (divu < R(0)) qk = viscosity((R)P.eos.qcoef[k], tk, L, cs, divu);
+      const R qnk = PC ? A.env_q[ek] : R(0);
+      const R ekv = compute_eint(PC, (R)P.eos.gamma[k], qnk, qk, rho_nk, rk, A.env_p[ek], A.env_e[ek]);
+      uv[k] = Vk;
+      um[k] = Vk * rk;
+      ue[k] = Vk * rk * ekv;
+      uq = Vk * qk;  // PrepareRemap.cc:372 assigns: the last environment wins
+    }
+  }
+  R* U = A.U;
+#pragma unroll
+  for (int k = 0; k < NEM; ++k) {
+    if (k >= ne) break;
+    st_stream(&U[k * NC + c], uv[k]);
+    st_stream(&U[(ne + k) * NC + c], um[k]);
+    st_stream(&U[(2 * ne + k) * NC + c], ue[k]);
+  }
+  st_stream(&U[(3 * ne) * NC + c], uq);
+  st_stream(&A.Vlag[c], V);
+  st_stream(&A.L[c], L);
+}
+
+// Final cell phase of cell c with Euler volume vold: membership, "moy", EOS,
+// pressure average; returns the cell's sound speed for the CFL candidate.
+template <class R, int NEM>
+__device__ inline R final_cell(const Params& P, const CellFinalArgsT<R>& A, long c, double vold) {
+  {
+    const long NC = P.nc;
+    const int ne = P.ne;
+    const R thr = (R)P.remap_threshold;
+    const R vol = (R)vold;
+    R Uvol[NEM], Umas[NEM], Uene[NEM];
+#pragma unroll
+    for (int k = 0; k < NEM; ++k) {
+      Uvol[k] = Umas[k] = Uene[k] = R(0);
+      if (k < ne) {
+        Uvol[k] = A.U[k * NC + c];
+        Umas[k] = A.U[(ne + k) * NC + c];
+        Uene[k] = A.U[(2 * ne + k) * NC + c];
+      }
+    }
+    const R Uq = A.U[(3 * ne) * NC + c];
+    const int old_mask = A.mask[c];
+    // cell status per environment (110-143)
+    R volt = R(0);
+#pragma unroll
+    for (int k = 0; k < NEM; ++k)
+      if (k < ne) volt += Uvol[k];
+    int mask = old_mask;
+#pragma unroll
+    for (int k = 0; k < NEM; ++k) {
+      if (k >= ne) break;
+      const R fvol = qdiv(Uvol[k], vol);
+      const bool in_env = (old_mask >> k) & 1;
+      if (fvol < thr && in_env) {
+        mask &= ~(1 << k);
+      } else if (fvol > thr && !in_env) {
+        const R fn = qdiv(Uvol[k], volt);
+        if (fn > thr && Umas[k] != R(0)) mask |= (1 << k);
+      }
+    }
+    const int nbe = __popc(mask);
+    // "moy"
+    R masset = R(0);
+    volt = R(0);
+#pragma unroll
+    for (int k = 0; k < NEM; ++k)
+      if (k < ne) {
+        volt += Uvol[k];
+        masset += Umas[k];
+      }
+    const R unsurvolt = qdiv(R(1), volt);
+    const R unsurvol = qdiv(R(1), vol);
+    R fv[NEM], mf[NEM], dk[NEM], ek[NEM], pk[NEM], ck[NEM];
+    R somme_frac = R(0);
+#pragma unroll
+    for (int k = 0; k < NEM; ++k) {
+      fv[k] = R(0);
+      pk[k] = ck[k] = R(0);
+      if (!((mask >> k) & 1)) continue;
+      R f = Uvol[k] * unsurvolt;
+      if (f < thr) f = R(0);
+      fv[k] = f;
+      somme_frac += f;
+    }
+    int matcell = 0;
+    const R unsursomme_frac = qdiv(R(1), somme_frac);
+#pragma unroll
+    for (int k = 0; k < NEM; ++k) {
+      if (!((mask >> k) & 1)) continue;
+      const R f = fv[k] * unsursomme_frac;
+      if (f > R(0)) matcell++;
+      fv[k] = f;
+    }
+#pragma unroll
+    for (int k = 0; k < NEM; ++k) {
+      // a zero total mass leaves the mass fractions as they were (513-538)
+      mf[k] = R(0);
+      if (nbe == 1) mf[k] = R(1.0);
+      else if (k < ne && ((old_mask >> k) & 1) && ((mask >> k) & 1)) mf[k] = A.env_mf[k * NC + c];
+    }
+    if (masset != R(0)) {
+      const R unsurmasset = qdiv(R(1), masset);
+      R fmasset = R(0);
+#pragma unroll
+      for (int k = 0; k < NEM; ++k) {
+        if (!((mask >> k) & 1)) continue;
+        R m = Umas[k] * unsurmasset;
+        if (fv[k] < thr) m = R(0);
+        fmasset += m;
+        mf[k] = m;
+      }
+      if (fmasset != R(0)) {
+        const R unsurfmasset = qdiv(R(1), fmasset);
+#pragma unroll
+        for (int k = 0; k < NEM; ++k)
+          if ((mask >> k) & 1) mf[k] = mf[k] * unsurfmasset;
+      }
+    }
+    R density_nplus1 = R(0);
+    int kpure = 0;
+#pragma unroll
+    for (int k = 0; k < NEM; ++k) {
+      dk[k] = R(0);
+      if (!((mask >> k) & 1)) continue;
+      kpure = k;
+      if (fv[k] > thr) {
+        const R vol_norm = Uvol[k] * vol * unsurvolt;
+        dk[k] = qdiv(Umas[k], vol_norm);
+      }
+      density_nplus1 += fv[k] * dk[k];
+    }
+    R rho_g;
+    if (matcell > 1) rho_g = density_nplus1;
+    else if (nbe == 1) {
+      rho_g = R(0);
+#pragma unroll
+      for (int k = 0; k < NEM; ++k)
+        if (k == kpure) rho_g = dk[k];
+    } else {
+      rho_g = qdiv(A.m[c], A.Vlag[c]);  // left as updateDensity set it (m/V)
+    }
+    R mass_g = vol * density_nplus1;
+    R energie = R(0), pseudo = R(0);
+    const R pv = Uq * unsurvol;
+    R mk[NEM];
+#pragma unroll
+    for (int k = 0; k < NEM; ++k) {
+      ek[k] = R(0);
+      mk[k] = R(0);
+      if (!((mask >> k) & 1)) continue;
+      if (fv[k] > thr && Umas[k] != R(0)) ek[k] = qdiv(Uene[k], Umas[k]);
+      energie += mf[k] * ek[k];
+      mk[k] = mf[k] * mass_g;
+      pseudo += fv[k] * pv;
+    }
+    R e_g = energie, q_g = pseudo;
+    R p_g = R(0), c_g = R(1.e-20);
+    if (nbe == 1) {
+      // pure cell: the environment slot is the cell slot
+#pragma unroll
+      for (int k = 0; k < NEM; ++k)
+        if (k == kpure) mass_g = mk[k];
+      if (rho_g < R(0) || e_g < R(0)) {
+        e_g = R(0);
+        rho_g = R(0);
+      }
+#pragma unroll
+      for (int k = 0; k < NEM; ++k)
+        if (k == kpure) eos_apply(P.eos, k, rho_g, e_g, &p_g, &c_g);
+    } else {
+#pragma unroll
+      for (int k = 0; k < NEM; ++k) {
+        if (!((mask >> k) & 1)) continue;
+        if (dk[k] < R(0) || ek[k] < R(0)) {
+          ek[k] = R(0);
+          dk[k] = R(0);
+          fv[k] = R(0);
+          mf[k] = R(0);
+        }
+      }
+      // EOS per environment, then computePressionMoyenne
+#pragma unroll
+      for (int k = 0; k < NEM; ++k) {
+        if (!((mask >> k) & 1)) continue;
+        eos_apply(P.eos, k, dk[k], ek[k], &pk[k], &ck[k]);
+        p_g += fv[k] * pk[k];
+        c_g = smax(ck[k], c_g);
+      }
+    }
+    st_stream(&A.rho[c], rho_g);
+    st_stream(&A.e[c], e_g);
+    st_stream(&A.p[c], p_g);
+    st_stream(&A.cs[c], c_g);
+    st_stream(&A.q[c], q_g);
+    st_stream(&A.m[c], mass_g);
+    st_stream(&A.mask[c], mask);
+    if (nbe >= 2) {
+#pragma unroll
+      for (int k = 0; k < NEM; ++k) {
+        if (!((mask >> k) & 1)) continue;
+        const long e = k * NC + c;
+        A.env_rho[e] = dk[k];
+        A.env_e[e] = ek[k];
+        A.env_p[e] = pk[k];
+        A.env_cs[e] = ck[k];
+        A.env_q[e] = pv;
+        A.env_m[e] = mk[k];
+        A.env_f[e] = fv[k];
+        A.env_mf[e] = mf[k];
+      }
+    }
+    return c_g;
+  }
+}
+
+// Reduction of the CFL candidates of a block to one minimum.
+template <int BLOCK>
+__device__ inline void block_min_store(double v, double* block_min) {
+  __shared__ double red[BLOCK / 64];
+  for (int off = 32; off > 0; off >>= 1) v = smin(v, __shfl_down(v, off, 64));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = red[0];
+    for (int w = 1; w < BLOCK / 64; ++w) b = smin(b, red[w]);
+    block_min[blockIdx.x] = b;
+  }
+}
+
+}  // namespace mhg
