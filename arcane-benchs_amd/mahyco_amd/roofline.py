@@ -20,9 +20,9 @@ PEAK_HBM_GBS = 8000.0  # MI355X HBM3E peak, GB/s
 
 
 def words_per_zone(n_env, dim=2):
-    if dim != 2:
-        raise NotImplementedError
-    return 71 + 18 * n_env
+    """71 + 18 n_m in 2D; 108 + 24 n_m in 3D (three cell sweeps of 6 n_m + 9
+    words, three dual sweeps of 12, L-node 18, L-cell 3 n_m + 10)."""
+    return 71 + 18 * n_env if dim == 2 else 108 + 24 * n_env
 
 
 def bytes_per_zone_cycle(n_env, dim=2, word=8):

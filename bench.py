@@ -99,6 +99,49 @@ def timed_run(gpu, precision, args, sync):
     return {"sim": sim, "elapsed": t1 - t0, "tuning": tuning, "kt": kt, "density": st["density"]}
 
 
+def p3d_run(gpu, args, sync):
+    """The 3D random-field case in FP64 and in the FP32-bulk precision form:
+    throughput of each (graph-replayed steps) and per-field L1/Linf of the
+    precision form against the FP64 path after the same steps."""
+    n, steps = args.p3d_n, args.p3d_steps
+    t0 = time.perf_counter()
+    fields = cases.p3d_fields(n)
+    t_fields = time.perf_counter() - t0
+    out = {"config": f"{n}^3 unit cube, gamma 5/3, rho = exp(G) (corr. 8 cells, sigma 0.3), p = 1, "
+                     f"solenoidal u (rms 0.1), reflecting walls, {steps} steps, seed {cases.P3D_SEED}",
+           "fields_s": round(t_fields, 1)}
+    states = {}
+    for name, prec in [("fp64", abi.PREC_FP64), ("fp32", abi.PREC_FP32_BULK)]:
+        s = cases.p3d(gpu, n, fields=fields, precision=prec)
+        # warm-up of one graph group: captures the step graph before timing
+        s.L.optional["run_async"](s.ctx, 12)
+        s.L.optional["synchronize"](s.ctx)
+        s.init_cells(np.zeros(n ** 3, dtype=np.int32), fields[0], np.ones(n ** 3), fields[1])
+        sync()
+        t0 = time.perf_counter()
+        r = s.L.optional["run_async"](s.ctx, steps)
+        assert r == 0, s.L.last_error(s.ctx)
+        s.L.optional["synchronize"](s.ctx)
+        sync()
+        el = time.perf_counter() - t0
+        st = s.download(["density", "internal_energy", "pressure", "velocity"])
+        assert st["global_iteration"] == steps
+        states[name] = st
+        out[name] = {"value": n ** 3 * steps / el, "unit": "zone-cycles/s", "ms_per_step": 1e3 * el / steps}
+        s.close()
+    out["speedup_fp32_vs_fp64"] = out["fp32"]["value"] / out["fp64"]["value"]
+    err = {}
+    for f in ["density", "internal_energy", "pressure", "velocity"]:
+        a = np.asarray(states["fp32"][f], dtype=np.float64)
+        b = np.asarray(states["fp64"][f], dtype=np.float64)
+        err[f] = {"l1": float(np.sum(np.abs(a - b)) / np.sum(np.abs(b))),
+                  "linf": float(np.max(np.abs(a - b)) / np.max(np.abs(b)))}
+    out["fp32_error_vs_fp64"] = err
+    out["roofline_frac_fp64"] = round(out["fp64"]["value"] * roofline.bytes_per_zone_cycle(1, dim=3)
+                                      / (roofline.PEAK_HBM_GBS * 1e9), 4)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +152,9 @@ def main():
     ap.add_argument("--no-events", action="store_true", help="do not record per-kernel events")
     ap.add_argument("--no-precision-form", action="store_true",
                     help="skip the FP32-bulk precision form run")
+    ap.add_argument("--p3d-n", type=int, default=256, help="cells per side of the 3D random-field case")
+    ap.add_argument("--p3d-steps", type=int, default=200)
+    ap.add_argument("--no-p3d", action="store_true", help="skip the 3D random-field case")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device with torch: the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counts to bytes")
@@ -151,6 +197,7 @@ def main():
         r32["sim"].close()
     nc, nn = sim.nc, sim.nn
     value = nc * args.steps / elapsed
+    p3 = None if args.no_p3d else p3d_run(gpu, args, sync)
 
     rl = None
     if kt:
@@ -195,6 +242,7 @@ def main():
         "roofline": rl,
         "cpu_baseline": cpu,
         "precision_form": p32,
+        "precision_form_3d": p3,
     }
     print(json.dumps(out))
 
