@@ -224,57 +224,68 @@ __host__ __device__ inline double v3dot(V3 a, V3 b) { return a.x * b.x + a.y * b
 // Face centres of a hexahedron, 0.25 * (sum of its 4 nodes in face order)
 // (MahycoModule.cc:1080-1087): faces (0,3,2,1), (0,4,7,3), (0,1,5,4),
 // (4,5,6,7), (1,2,6,5), (2,3,7,6).
+constexpr int kHexFace[6][4] = {{0, 3, 2, 1}, {0, 4, 7, 3}, {0, 1, 5, 4}, {4, 5, 6, 7}, {1, 2, 6, 5}, {2, 3, 7, 6}};
 __host__ __device__ inline V3 hex_face_centre(const V3* c, int f) {
-  const int fn[6][4] = {{0, 3, 2, 1}, {0, 4, 7, 3}, {0, 1, 5, 4}, {4, 5, 6, 7}, {1, 2, 6, 5}, {2, 3, 7, 6}};
-  return v3mul(0.25, v3add(v3add(v3add(c[fn[f][0]], c[fn[f][1]]), c[fn[f][2]]), c[fn[f][3]]));
+  return v3mul(0.25, v3add(v3add(v3add(c[kHexFace[f][0]], c[kHexFace[f][1]]), c[kHexFace[f][2]]), c[kHexFace[f][3]]));
 }
 
 // The 24 half cross products of computeCQs (MahycoModule.cc:2195-2258),
 // term t = 0.5 * cross(c[a] - fc[f], c[b] - fc[f]) with (f, a, b) below, and
 // the 12 terms of each corner in the reference's summation order: corner q
 // is (1/12) * (5 * (sum of the first six) + (sum of the last six)).
-struct HexTerm {
-  int f, a, b;
-};
-__host__ __device__ inline HexTerm hex_term(int t) {
-  const int T[24][3] = {{0, 0, 3}, {0, 3, 2}, {0, 2, 1}, {0, 1, 0},   // n1a04 n1a03 n1a02 n1a01
-                        {1, 0, 4}, {1, 4, 7}, {1, 7, 3}, {1, 3, 0},   // n2a05 n2a12 n2a08 n2a04
-                        {2, 0, 1}, {2, 1, 5}, {2, 5, 4}, {2, 4, 0},   // n3a01 n3a06 n3a09 n3a05
-                        {3, 4, 5}, {3, 5, 6}, {3, 6, 7}, {3, 7, 4},   // n4a09 n4a10 n4a11 n4a12
-                        {4, 1, 2}, {4, 2, 6}, {4, 6, 5}, {4, 5, 1},   // n5a02 n5a07 n5a10 n5a06
-                        {5, 2, 3}, {5, 3, 7}, {5, 7, 6}, {5, 6, 2}};  // n6a03 n6a08 n6a11 n6a07
-  return {T[t][0], T[t][1], T[t][2]};
-}
-__host__ __device__ inline int hex_corner_term(int q, int s) {
-  const int C[8][12] = {{3, 0, 7, 4, 11, 8, 2, 1, 6, 5, 9, 10},
-                        {3, 2, 8, 9, 19, 16, 0, 1, 10, 11, 18, 17},
-                        {2, 1, 17, 16, 23, 20, 3, 0, 19, 18, 22, 21},
-                        {1, 0, 6, 7, 21, 20, 3, 2, 4, 5, 23, 22},
-                        {4, 5, 11, 10, 12, 15, 6, 7, 8, 9, 13, 14},
-                        {9, 10, 12, 13, 18, 19, 8, 11, 15, 14, 17, 16},
-                        {14, 13, 18, 17, 23, 22, 15, 12, 19, 16, 20, 21},
-                        {6, 5, 15, 14, 22, 21, 7, 4, 12, 13, 23, 20}};
-  return C[q][s];
-}
+constexpr int kHexTerm[24][3] = {{0, 0, 3}, {0, 3, 2}, {0, 2, 1}, {0, 1, 0},   // n1a04 n1a03 n1a02 n1a01
+                                  {1, 0, 4}, {1, 4, 7}, {1, 7, 3}, {1, 3, 0},   // n2a05 n2a12 n2a08 n2a04
+                                  {2, 0, 1}, {2, 1, 5}, {2, 5, 4}, {2, 4, 0},   // n3a01 n3a06 n3a09 n3a05
+                                  {3, 4, 5}, {3, 5, 6}, {3, 6, 7}, {3, 7, 4},   // n4a09 n4a10 n4a11 n4a12
+                                  {4, 1, 2}, {4, 2, 6}, {4, 6, 5}, {4, 5, 1},   // n5a02 n5a07 n5a10 n5a06
+                                  {5, 2, 3}, {5, 3, 7}, {5, 7, 6}, {5, 6, 2}};  // n6a03 n6a08 n6a11 n6a07
+constexpr int kHexCorner[8][12] = {{3, 0, 7, 4, 11, 8, 2, 1, 6, 5, 9, 10},
+                                   {3, 2, 8, 9, 19, 16, 0, 1, 10, 11, 18, 17},
+                                   {2, 1, 17, 16, 23, 20, 3, 0, 19, 18, 22, 21},
+                                   {1, 0, 6, 7, 21, 20, 3, 2, 4, 5, 23, 22},
+                                   {4, 5, 11, 10, 12, 15, 6, 7, 8, 9, 13, 14},
+                                   {9, 10, 12, 13, 18, 19, 8, 11, 15, 14, 17, 16},
+                                   {14, 13, 18, 17, 23, 22, 15, 12, 19, 16, 20, 21},
+                                   {6, 5, 15, 14, 22, 21, 7, 4, 12, 13, 23, 20}};
 __host__ __device__ inline V3 hex_term_value(const V3* c, const V3* fc, int t) {
-  const HexTerm h = hex_term(t);
-  return v3mul(0.5, v3cross(v3sub(c[h.a], fc[h.f]), v3sub(c[h.b], fc[h.f])));
+  const int f = kHexTerm[t][0], a = kHexTerm[t][1], b = kHexTerm[t][2];
+  return v3mul(0.5, v3cross(v3sub(c[a], fc[f]), v3sub(c[b], fc[f])));
 }
-// Corner q of the cell whose nodes are c[0..7] (fc: its 6 face centres).
-__host__ __device__ inline V3 hex_corner_normal(const V3* c, const V3* fc, int q) {
-  V3 a = hex_term_value(c, fc, hex_corner_term(q, 0));
-  for (int s = 1; s < 6; ++s) a = v3add(a, hex_term_value(c, fc, hex_corner_term(q, s)));
-  V3 b = hex_term_value(c, fc, hex_corner_term(q, 6));
-  for (int s = 7; s < 12; ++s) b = v3add(b, hex_term_value(c, fc, hex_corner_term(q, s)));
+// Corner q from the 24 terms n[] (or from the coordinates when n is null).
+__host__ __device__ inline V3 hex_corner_sum(const V3* n, int q) {
+  V3 a = n[kHexCorner[q][0]];
+#pragma unroll
+  for (int s = 1; s < 6; ++s) a = v3add(a, n[kHexCorner[q][s]]);
+  V3 b = n[kHexCorner[q][6]];
+#pragma unroll
+  for (int s = 7; s < 12; ++s) b = v3add(b, n[kHexCorner[q][s]]);
   const double w = 1. / 12.;
   return v3mul(w, v3add(v3mul(5., a), b));
 }
-// Volume (1/3) sum_i x_i . C_i (MahycoModule.cc:1244-1256) of a hexahedron.
+// Corner q of the cell whose nodes are c[0..7] (fc: its 6 face centres),
+// forming only the 12 terms it needs.
+template <int Q>
+__host__ __device__ inline V3 hex_corner_normal(const V3* c, const V3* fc) {
+  V3 a = hex_term_value(c, fc, kHexCorner[Q][0]);
+#pragma unroll
+  for (int s = 1; s < 6; ++s) a = v3add(a, hex_term_value(c, fc, kHexCorner[Q][s]));
+  V3 b = hex_term_value(c, fc, kHexCorner[Q][6]);
+#pragma unroll
+  for (int s = 7; s < 12; ++s) b = v3add(b, hex_term_value(c, fc, kHexCorner[Q][s]));
+  const double w = 1. / 12.;
+  return v3mul(w, v3add(v3mul(5., a), b));
+}
+// Volume (1/3) sum_i x_i . C_i (MahycoModule.cc:1244-1256) of a hexahedron:
+// the 24 terms once, then the 8 corners (computeCQs, MahycoModule.cc:2195-2258).
 __host__ __device__ inline double hex_volume(const V3* c) {
-  V3 fc[6];
+  V3 fc[6], n[24];
+#pragma unroll
   for (int f = 0; f < 6; ++f) fc[f] = hex_face_centre(c, f);
+#pragma unroll
+  for (int t = 0; t < 24; ++t) n[t] = hex_term_value(c, fc, t);
   double volume = 0.;
-  for (int q = 0; q < 8; ++q) volume += v3dot(c[q], hex_corner_normal(c, fc, q));
+#pragma unroll
+  for (int q = 0; q < 8; ++q) volume += v3dot(c[q], hex_corner_sum(n, q));
   const double inv_dim = 1. / 3.;
   return volume * inv_dim;
 }

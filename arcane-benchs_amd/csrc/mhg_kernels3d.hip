@@ -59,13 +59,22 @@ __global__ __launch_bounds__(256) void k_node_lagrange_3d(Params P, DevScalars* 
       const int di = (t & 1) - 1, dj = ((t >> 1) & 1) - 1, dk = (t >> 2) - 1;
       const int ci = i + di, cj = j + dj, ck = k + dk;
       if (ci < 0 || ci >= P.nx || cj < 0 || cj >= P.ny || ck < 0 || ck >= P.nz) continue;
-      // the node's index in the cell (nodeCells order, hexa8 numbering)
-      const int q = ((dj == 0) ? (di == 0 ? 0 : 1) : (di == 0 ? 3 : 2)) + (dk == 0 ? 0 : 4);
       V3 cn[8], fc[6];
       hex_nodes(P, A.b.xpx, A.b.xpy, A.xpz, ci, cj, ck, cn);
 #pragma unroll
       for (int f = 0; f < 6; ++f) fc[f] = hex_face_centre(cn, f);
-      const V3 C = hex_corner_normal(cn, fc, q);
+      // the node is corner 6 - t of the cell (nodeCells order, hexa8 numbering)
+      V3 C;
+      switch (t) {
+        case 0: C = hex_corner_normal<6>(cn, fc); break;
+        case 1: C = hex_corner_normal<7>(cn, fc); break;
+        case 2: C = hex_corner_normal<5>(cn, fc); break;
+        case 3: C = hex_corner_normal<4>(cn, fc); break;
+        case 4: C = hex_corner_normal<2>(cn, fc); break;
+        case 5: C = hex_corner_normal<3>(cn, fc); break;
+        case 6: C = hex_corner_normal<1>(cn, fc); break;
+        default: C = hex_corner_normal<0>(cn, fc); break;
+      }
       const long c = cid3(P, ci, cj, ck);
       const double pq = (double)A.b.p[c] + (double)A.b.q[c];
       F = v3add(F, v3mul(pq, C));
