@@ -1,0 +1,824 @@
+// mhg_body2d.hpp -- bodies of the 2D (quad) step kernels.
+//
+// Each body runs one thread block's share of a kernel for the block index it
+// is given: the single-mesh kernels pass their (XCD-swizzled) block index,
+// the persistent batch kernels pass the local index of a work item of one
+// problem.  Everything a body reads about its mesh comes from Params, its
+// scalars from DevScalars and its arrays from the argument block.
+#pragma once
+
+#include <type_traits>
+
+#include "mhg_common.hpp"
+#include "mhg_kernels.hpp"
+#include "mhg_kernels_impl.hpp"
+
+namespace mhg {
+
+__device__ inline double node_x0(const Params& P, int i) { return P.org[0] + i * P.h[0]; }
+__device__ inline double node_y0(const Params& P, int j) { return P.org[1] + j * P.h[1]; }
+
+// ---------------------------------------------------------------------------
+// K1: node phase.  updateForceAndVelocity (MahycoModule.cc:589-665) with the
+// corner normals of the previous Lagrangian coordinates (saveValuesAtN copies
+// m_cell_cqs computed after the previous updatePosition, 344/365),
+// applyBoundaryCondition (936-1004), updatePosition (870-931) from x0.
+// Node mass = 1/4 sum of the cell masses (computeNodeMass 235-266): the cell
+// masses do not change in the Lagrange phase, so it is recomputed here.
+// ---------------------------------------------------------------------------
+template <class R, bool VEL, bool BC, bool POS>
+__device__ __forceinline__ void node_lagrange_2d_body(const Params& P, DevScalars* S, const NodeLagArgsT<R>& A,
+                                                      long blk) {
+  if (S->stop || S->err) return;
+  const long nid = blk * blockDim.x + threadIdx.x;
+  if (nid >= P.nn) return;
+  const int nnx = P.nx + 1;
+  const int i = (int)(nid % nnx), j = (int)(nid / nnx);
+  const double dt = S->dt;
+  double vx = A.ux[nid], vy = A.uy[nid];
+  if (VEL) {
+    const double dtv = 0.5 * (S->dt_old + S->dt);
+    // 3x3 patch of previous coordinates around the node.
+    double X[3][3], Y[3][3];
+#pragma unroll
+    for (int dj = -1; dj <= 1; ++dj)
+#pragma unroll
+      for (int di = -1; di <= 1; ++di) {
+        int ii = i + di, jj = j + dj;
+        double xv = 0., yv = 0.;
+        if (ii >= 0 && ii < nnx && jj >= 0 && jj <= P.ny) {
+          long q = ii + (long)nnx * jj;
+          xv = A.xpx[q];
+          yv = A.xpy[q];
+        }
+        X[dj + 1][di + 1] = xv;
+        Y[dj + 1][di + 1] = yv;
+      }
+    double Fx = 0., Fy = 0., msum = 0.;
+    // cells in ascending id: (i-1,j-1) corner 2, (i,j-1) 3, (i-1,j) 1, (i,j) 0
+    const int cdi[4] = {-1, 0, -1, 0}, cdj[4] = {-1, -1, 0, 0}, cq[4] = {2, 3, 1, 0};
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      int ci = i + cdi[t], cj = j + cdj[t];
+      if (ci < 0 || ci >= P.nx || cj < 0 || cj >= P.ny) continue;
+      long c = ci + (long)P.nx * cj;
+      // quad nodes (ci,cj),(ci+1,cj),(ci+1,cj+1),(ci,cj+1) in patch coordinates
+      int bi = cdi[t] + 1, bj = cdj[t] + 1;
+      double qx[4] = {X[bj][bi], X[bj][bi + 1], X[bj + 1][bi + 1], X[bj + 1][bi]};
+      double qy[4] = {Y[bj][bi], Y[bj][bi + 1], Y[bj + 1][bi + 1], Y[bj + 1][bi]};
+      double cx, cy;
+      cqs2d(qx, qy, cq[t], &cx, &cy);
+      double pq = (double)A.p[c] + (double)A.q[c];
+      Fx += pq * cx;
+      Fy += pq * cy;
+      msum += (double)A.m[c];
+    }
+    double mn = 0.25 * msum;
+    double s = dtv / mn;
+    vx = vx + s * Fx;
+    vy = vy + s * Fy;
+  }
+  if (BC) {
+    for (int b = 0; b < P.nbc; ++b) {
+      const BcRange& r = P.bc[b];
+      int idx = (r.axis == 0) ? i : j;
+      if (r.axis < 0 || (idx >= r.lo && idx <= r.hi)) {
+        if (r.comp == 0) vx = r.value;
+        else if (r.comp == 1) vy = r.value;
+      }
+    }
+  }
+  if (VEL || BC) {
+    st_stream(&A.ux[nid], (R)vx);
+    st_stream(&A.uy[nid], (R)vy);
+  }
+  if (POS) {
+    st_stream(&A.xnx[nid], node_x0(P, i) + dt * vx);
+    st_stream(&A.xny[nid], node_y0(P, j) + dt * vy);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Environment bound.  Every cell kernel is compiled for a maximum number of
+// environments NEM (1, 2, 3, 4 or 8) so that per-environment values live in
+// registers; the run-time count ne <= NEM guards loads and stores.
+// ---------------------------------------------------------------------------
+
+// ---------------------------------------------------------------------------
+// K2: cell phase.  computeGeometricValues (1054-1299), updateDensity
+// (1331-1426), computeArtificialViscosity (454-578),
+// updateEnergyAndPressureforGP (1659-1770), then the remap vector
+// computeVariablesForRemap_PBorn0 (PrepareRemap.cc:258-399).  Pure cells use
+// the cell arrays; cells with >= 2 environments loop over their environments
+// in id order.  Only the live entries of U are kept: the momentum and kinetic
+// energy slots (PrepareRemap.cc:324-328) are identically zero and stay zero
+// through the sweeps (phi = 0 gives a zero gradient, face value and flux).
+// ---------------------------------------------------------------------------
+template <class R, bool PC, int NEM>
+__device__ __forceinline__ void cell_lagrange_2d_body(const Params& P, DevScalars* S, const CellLagArgsT<R>& A,
+                                                      long blk) {
+  if (S->stop || S->err) return;
+  const long c = blk * blockDim.x + threadIdx.x;
+  if (c >= P.nc) return;
+  const int nnx = P.nx + 1;
+  const int i = (int)(c % P.nx), j = (int)(c / P.nx);
+  const long n0 = i + (long)nnx * j;
+  const long nd[4] = {n0, n0 + 1, n0 + 1 + nnx, n0 + nnx};
+  double x[4], y[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    x[q] = A.xnx[nd[q]];
+    y[q] = A.xny[nd[q]];
+  }
+  const double Vd = quad_volume(x, y);
+  const double Ld = (P.lc == MHG_LC_FACES_OPPOSEES) ? length_faces_opposees_2d(x, y) : pow(Vd, 0.5);
+  lagrange_cell_bulk<R, PC, NEM>(P, S, A, c, Vd, Ld);
+}
+
+// ---------------------------------------------------------------------------
+// Sweep arithmetic shared by both directions, for one variable of one cell:
+// computeGradPhiFace (RemapADIService.cc:171-177), the limited cell gradient
+// (497-506), the upwind face value (707-714) and the flux update (909-936).
+// ---------------------------------------------------------------------------
+// Lane sweep: one wavefront holds 64 consecutive cells along the sweep
+// direction D (60 outputs and two halo cells on each side); stencil
+// neighbours move between lanes.  D = 0: a wave is a row segment.  D = 1: a
+// wave is a column segment and a block of 16 waves covers 16 adjacent
+// columns, so each cache line of a row is consumed by the whole block.
+template <class R, int D, int NEM, bool FIRST, int LIM>
+__device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScalars* S, const CellSweepArgsT<R>& A,
+                                                        int blk) {
+  if (S->stop || S->err) return;
+  const int lane = threadIdx.x & 63;
+  const int N = (D == 0) ? P.nx : P.ny;      // cells along D
+  const int T = (D == 0) ? P.ny : P.nx;      // cells across D
+  const int nseg = (N + 59) / 60;
+  int seg, line;                             // segment along D, index across D
+  if (D == 0) {
+    const long wave = ((long)blk * blockDim.x + threadIdx.x) >> 6;
+    line = (int)(wave / nseg);
+    seg = (int)(wave % nseg);
+  } else {
+    const int ncolblk = (T + 15) / 16;
+    line = (blk % ncolblk) * 16 + (threadIdx.x >> 6);
+    seg = blk / ncolblk;
+  }
+  if (line >= T) return;  // uniform across the wave
+  const int s = seg * 60 - 2 + lane;
+  const bool ex = (s >= 0 && s < N);
+  const bool out = ex && lane >= 2 && lane <= 61;
+  const int i = (D == 0) ? s : line, j = (D == 0) ? line : s;
+  const int nnx = P.nx + 1;
+  const long NC = P.nc;
+  const int ne = P.ne;
+  const double dt = S->dt;
+  const R order2 = (R)P.order2;
+  const long c = ex ? i + (long)P.nx * j : 0;
+  const R* __restrict__ Uin = A.Uin;
+  const double* __restrict__ Xd = (D == 0) ? A.xnx : A.xny;   // coordinate along D
+  const double* __restrict__ Xt = (D == 0) ? A.xny : A.xnx;   // transverse coordinate
+  const R* __restrict__ Ud = (D == 0) ? A.ux : A.uy;          // velocity along D
+  // cell centre (component D, double) and phi denominator
+  double xc = 0.;
+  R den = R(1);
+  if (ex) {
+    const long m0 = i + (long)nnx * j;
+    const double a = Xd[m0], b = Xd[m0 + 1], cc = Xd[m0 + 1 + nnx], dd = Xd[m0 + nnx];
+    xc = 0.25 * (((a + b) + cc) + dd);
+    if (FIRST) {
+      den = A.Vlag[c];
+    } else {
+      R sv = R(0);
+#pragma unroll
+      for (int k = 0; k < NEM; ++k)
+        if (k < ne) sv += Uin[k * NC + c];
+      den = sv;
+    }
+  }
+  // back face of the cell: the face at node index s along D, nodes
+  // (i,j),(i,j+1) for D = 0 and (i,j),(i+1,j) for D = 1
+  R uf = R(0);
+  double lf = 0., xf = 0.;
+  if (s >= 0 && s <= N) {
+    const long na = i + (long)nnx * j, nb = (D == 0) ? na + nnx : na + 1;
+    uf = R(0.5) * (Ud[na] + Ud[nb]);
+    xf = 0.5 * Xd[na] + 0.5 * Xd[nb];
+    lf = fabs(Xt[nb] - Xt[na]);
+  }
+  const bool inner = ex && s >= 1;
+  const double xcm = __shfl_up(xc, 1, 64);
+  const double dx = inner ? xc - xcm : 1.;
+  const bool upm = inner && ((double)uf * dx > 0.0);
+  const double dotxf = inner ? (upm ? xf - xcm : xf - xc) : 0.;
+  const R uf_f = __shfl_down(uf, 1, 64);
+  const double lf_f = __shfl_down(lf, 1, 64);
+  // face flux factors u_f L_f dt (double products, then the storage type)
+  const R fb = (R)(-1.0 * uf * lf * dt), ff = (R)(1.0 * uf_f * lf_f * dt);
+  const R dxr = (R)dx, dotr = (R)dotxf;
+  const bool o2 = P.order2 > 0;
+  R pre[3][NEM];
+  const R preq = ex ? Uin[(3 * ne) * NC + c] : R(0);
+#pragma unroll
+  for (int g = 0; g < 3; ++g)
+#pragma unroll
+    for (int k = 0; k < NEM; ++k) pre[g][k] = (ex && k < ne) ? Uin[(g * ne + k) * NC + c] : R(0);
+  auto sweep_var = [&](R u0, R& unew, R& dual) {
+    const R phi = ex ? qdiv(u0, den) : R(0);
+    const R phim = __shfl_up(phi, 1, 64);
+    const R gf = face_grad(inner, phi, phim, dxr);
+    const R gfp = __shfl_down(gf, 1, 64);
+    const R g = o2 ? limited_grad<LIM, true, R, kBranchFree<R>>(gf, gfp) : R(0);
+    const R gm = __shfl_up(g, 1, 64);
+    const R phif = inner ? (upm ? phim + order2 * dotr * gm : phi + order2 * dotr * g) : R(0);
+    const R phifp = __shfl_down(phif, 1, 64);
+    const R flux_b = fb * phif;
+    const R flux_f = ff * phifp;
+    if (D == 0) {  // cell faces in order: bottom, right (front), top, left (back)
+      unew = u0 - (flux_f + flux_b);
+      dual = R(0.5) * flux_f * R(1.0) + R(0.5) * flux_b * R(-1.0);
+    } else {       // bottom (back), right, top (front), left
+      unew = u0 - (flux_b + flux_f);
+      dual = R(0.5) * flux_b * R(-1.0) + R(0.5) * flux_f * R(1.0);
+    }
+  };
+  R* __restrict__ Uout = A.Uout;
+#pragma unroll
+  for (int k = 0; k < NEM; ++k) {
+    if (k >= ne) break;
+    R un, du;
+    sweep_var(pre[0][k], un, du);
+    if (out) st_stream(&Uout[k * NC + c], un);
+  }
+#pragma unroll
+  for (int k = 0; k < NEM; ++k) {
+    if (k >= ne) break;
+    R un, du;
+    sweep_var(pre[1][k], un, du);
+    if (un < R(0)) un = R(0);  // RemapADIService.cc:953-955
+    if (out) {
+      st_stream(&Uout[(ne + k) * NC + c], un);
+      st_stream(&A.contrib[k * NC + c], R(0.5) * du);  // fcontrib (DualPhaseRemap.cc:168-185)
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NEM; ++k) {
+    if (k >= ne) break;
+    R un, du;
+    sweep_var(pre[2][k], un, du);
+    if (un < R(0)) un = R(0);  // RemapADIService.cc:956-958
+    if (out) st_stream(&Uout[(2 * ne + k) * NC + c], un);
+  }
+  {
+    R un, du;
+    sweep_var(preq, un, du);
+    if (out) st_stream(&Uout[(3 * ne) * NC + c], un);
+  }
+}
+
+// y sweep on LDS tiles: a block of 16 wavefronts owns 16 columns (one
+// 128-byte line of a row) and 64 rows (60 outputs, two halo rows on each
+// side).  Global loads and stores are row-major and coalesced; the tile is
+// transposed through LDS so that each wavefront holds one column with one row
+// per lane and runs the lane-exchange arithmetic of the x sweep.
+//
+// SEG (meshes with ny + 4 <= 64): a column with its halos is a segment of
+// SL = ny + 4 lanes and a wavefront holds K = 64 / SL whole columns, so the
+// block's tile is 16 K columns by SL rows and covers the column height at
+// once.  A lane's stencil never leaves its segment: outputs sit in lanes 2 ..
+// SL - 3 and read at most two lanes either side.
+template <bool SEG>
+struct YTileShape {
+  int SL, K, TW, TP;  // segment rows, segments per wavefront, tile columns, padded row
+  __device__ YTileShape(int nrows) {
+    SL = SEG ? nrows + 4 : kTileR;
+    K = SEG ? kTileR / SL : 1;
+    TW = kTileW * K;
+    TP = TW + 1;
+  }
+};
+
+template <class R, int NEM, bool FIRST, int LIM, bool SEG>
+__device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevScalars* S,
+                                                         const CellSweepArgsT<R>& A, int L) {
+  // geometry tiles [0..4]; once read, tiles [0..5] are reused as the staging
+  // tiles (in, out, dual flux) of a pair of variables
+  __shared__ double lds[6][kTileR * kTileP];
+  R* const stg = reinterpret_cast<R*>(&lds[0][0]);
+  constexpr int TS = kTileR * kTileP;  // words of one staging tile
+  if (S->stop || S->err) return;  // uniform over the block
+  const int nx = P.nx, N = P.ny, nnx = nx + 1;
+  const YTileShape<SEG> sh(N);
+  const int TW = SEG ? sh.TW : kTileW, TP = SEG ? sh.TP : kTileP;
+  const int ncolblk = (nx + TW - 1) / TW;
+  const int i0 = (L % ncolblk) * TW;
+  const int s0 = SEG ? -2 : (L / ncolblk) * 60 - 2;  // cell row of tile row 0
+  const int t = threadIdx.x;
+  // row-major position (loads, stores)
+  const int tr = SEG ? t / TW : t >> 4, tc = SEG ? t % TW : t & 15;
+  const bool trow = SEG ? tr < sh.SL : true;
+  // transposed position: column col, row r (lane of its segment)
+  const int w = t >> 6, lane = t & 63;
+  const int col = SEG ? w * sh.K + lane / sh.SL : w, r = SEG ? lane % sh.SL : lane;
+  const bool tlane = SEG ? lane < sh.K * sh.SL : true;
+  const int ta = r * TP + col;   // transposed LDS index
+  const int ra = tr * TP + tc;   // row-major LDS index
+  const long NC = P.nc;
+  const int ne = P.ne;
+  const double dt = S->dt;
+  const R order2 = (R)P.order2;
+  const bool o2 = P.order2 > 0;
+  const R* __restrict__ Uin = A.Uin;
+  // row-major cell of this thread
+  const int gi = i0 + tc, gs = s0 + tr;
+  const bool gex = trow && gi < nx && gs >= 0 && gs < N;
+  const bool gout = SEG ? gex : gex && tr >= 2 && tr <= 61;
+  const long gc = gex ? gi + (long)nx * gs : 0;
+  {  // geometry of the tile, row-major (stored in the double tiles exactly)
+    double xc = 0., xf = 0., lf = 0.;
+    R den = R(1), uf = R(0);
+    if (gex) {
+      const long m0 = gi + (long)nnx * gs;
+      const double a = A.xny[m0], b = A.xny[m0 + 1], cc = A.xny[m0 + 1 + nnx], dd = A.xny[m0 + nnx];
+      xc = 0.25 * (((a + b) + cc) + dd);
+      if (FIRST) {
+        den = A.Vlag[gc];
+      } else {
+        R sv = R(0);
+#pragma unroll
+        for (int k = 0; k < NEM; ++k)
+          if (k < ne) sv += Uin[k * NC + gc];
+        den = sv;
+      }
+    }
+    if (trow && gi < nx && gs >= 0 && gs <= N) {  // back face: nodes (i, s), (i+1, s)
+      const long na = gi + (long)nnx * gs, nb = na + 1;
+      uf = R(0.5) * (A.uy[na] + A.uy[nb]);
+      xf = 0.5 * A.xny[na] + 0.5 * A.xny[nb];
+      lf = fabs(A.xnx[nb] - A.xnx[na]);
+    }
+    if (trow) {
+      lds[0][ra] = xc;
+      lds[1][ra] = den;
+      lds[2][ra] = uf;
+      lds[3][ra] = xf;
+      lds[4][ra] = lf;
+    }
+  }
+  __syncthreads();
+  const int s = s0 + r;
+  const bool ex = tlane && (i0 + col < nx) && s >= 0 && s < N;
+  const bool inner = ex && s >= 1;
+  const double xc = tlane ? lds[0][ta] : 0., xf = tlane ? lds[3][ta] : 0., lf = tlane ? lds[4][ta] : 0.;
+  const R den = tlane ? (R)lds[1][ta] : R(1), uf = tlane ? (R)lds[2][ta] : R(0);
+  const double xcm = __shfl_up(xc, 1, 64);
+  const double dx = inner ? xc - xcm : 1.;
+  const bool upm = inner && ((double)uf * dx > 0.0);
+  const double dotxf = inner ? (upm ? xf - xcm : xf - xc) : 0.;
+  const R uf_f = __shfl_down(uf, 1, 64);
+  const double lf_f = __shfl_down(lf, 1, 64);
+  const R fb = (R)(-1.0 * uf * lf * dt), ff = (R)(1.0 * uf_f * lf_f * dt);
+  const R dxr = (R)dx, dotr = (R)dotxf;
+  // one variable on the transposed layout: new U and half the dual flux
+  auto sweep_var = [&](R u0, R& un, R& dual) {
+    const R phi = ex ? qdiv(u0, den) : R(0);
+    const R phim = __shfl_up(phi, 1, 64);
+    const R gf = face_grad(inner, phi, phim, dxr);
+    const R gfp = __shfl_down(gf, 1, 64);
+    const R g = o2 ? limited_grad<LIM, true, R, kBranchFree<R>>(gf, gfp) : R(0);
+    const R gm = __shfl_up(g, 1, 64);
+    const R phif = inner ? (upm ? phim + order2 * dotr * gm : phi + order2 * dotr * g) : R(0);
+    const R phifp = __shfl_down(phif, 1, 64);
+    const R flux_b = fb * phif;
+    const R flux_f = ff * phifp;
+    // cell faces in order: bottom (back), right, top (front), left
+    un = u0 - (flux_b + flux_f);
+    dual = R(0.5) * flux_b * R(-1.0) + R(0.5) * flux_f * R(1.0);
+  };
+  __syncthreads();  // geometry reads done: the tiles become staging tiles
+  const int nvu = P.nvu;
+  // variables two at a time (a, b = a + 1), one barrier pair per round
+  R nxa = gex ? Uin[gc] : R(0);
+  R nxb = (gex && nvu > 1) ? Uin[NC + gc] : R(0);
+  for (int va = 0; va < nvu; va += 2) {
+    const int vb = va + 1;
+    const bool hb = vb < nvu;  // uniform
+    if (trow) {
+      stg[0 * TS + ra] = nxa;
+      stg[1 * TS + ra] = nxb;
+    }
+    __syncthreads();
+    if (va + 2 < nvu) nxa = gex ? Uin[(va + 2) * NC + gc] : R(0);
+    if (va + 3 < nvu) nxb = gex ? Uin[(va + 3) * NC + gc] : R(0);
+    const R ua = tlane ? stg[0 * TS + ta] : R(0), ub = tlane ? stg[1 * TS + ta] : R(0);
+    R una, dua, unb, dub;
+    sweep_var(ua, una, dua);
+    sweep_var(ub, unb, dub);
+    if (va >= ne && va < 3 * ne && una < R(0)) una = R(0);  // RemapADIService.cc:953-958
+    if (vb >= ne && vb < 3 * ne && unb < R(0)) unb = R(0);
+    const bool ma = (va >= ne && va < 2 * ne), mb = hb && (vb >= ne && vb < 2 * ne);
+    if (tlane) {
+      stg[2 * TS + ta] = una;
+      stg[3 * TS + ta] = unb;
+      if (ma) stg[4 * TS + ta] = R(0.5) * dua;  // fcontrib (DualPhaseRemap.cc:168-185)
+      if (mb) stg[5 * TS + ta] = R(0.5) * dub;
+    }
+    __syncthreads();
+    if (gout) {
+      st_stream(&A.Uout[va * NC + gc], stg[2 * TS + ra]);
+      if (hb) st_stream(&A.Uout[vb * NC + gc], stg[3 * TS + ra]);
+      if (ma) st_stream(&A.contrib[(va - ne) * NC + gc], stg[4 * TS + ra]);
+      if (mb) st_stream(&A.contrib[(vb - ne) * NC + gc], stg[5 * TS + ra]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K5: final cell phase.  remapVariables cell status + membership change
+// (RemapADIFinal.cc:98-226), "moy" (446-643), applyEOS per environment
+// (PerfectGasEOSService.cc:50-137), computePressionMoyenne (MahycoModule.cc:
+// 1836-1915) and the CFL candidate of computeHydroDeltaT (2047-2061) reduced
+// to one minimum per block; k_dt_finalize finishes computeDeltaT (2076-2191).
+// ---------------------------------------------------------------------------
+template <class R, int BLOCK, int NEM>
+__device__ __forceinline__ void cell_final_2d_body(const Params& P, DevScalars* S, const CellFinalArgsT<R>& A,
+                                                   long blk, double* block_min) {
+  if (S->stop || S->err) return;
+  const long c = blk * BLOCK + threadIdx.x;
+  double dx_sound = 1.7976931348623157e308;
+  if (c < P.nc) {
+    const int nnx = P.nx + 1;
+    const int i = (int)(c % P.nx), j = (int)(c / P.nx);
+    // Euler volume: the corner-normal formula on the initial coordinates.
+    double x0[4] = {node_x0(P, i), node_x0(P, i + 1), node_x0(P, i + 1), node_x0(P, i)};
+    double y0[4] = {node_y0(P, j), node_y0(P, j), node_y0(P, j + 1), node_y0(P, j + 1)};
+    const R c_g = final_cell<R, NEM>(P, A, c, quad_volume(x0, y0));
+    // CFL candidate: L / (c + max over the cell nodes of |u|)
+    const long n0 = i + (long)nnx * j;
+    const long nd[4] = {n0, n0 + 1, n0 + 1 + nnx, n0 + nnx};
+    R vmax = R(0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const R ux = A.ux[nd[q]], uy = A.uy[nd[q]];
+      const R un = sqrt(ux * ux + uy * uy + R(0) * R(0));
+      vmax = (un < vmax) ? vmax : un;
+    }
+    // std::min(minimum_aux, dx_sound) fold: a NaN candidate is skipped
+    dx_sound = smin(dx_sound, (double)qdiv(A.L[c], c_g + vmax));
+  }
+  block_min_store<BLOCK>(dx_sound, block_min + blk);
+}
+
+// computeDeltaT (MahycoModule.cc:2076-2191) on the block minima of
+// k_cell_final_2d: dt = min(cfl * min_c dx_sound, 1.05 dt_old, deltat-max),
+// deltat-min check, stop test with the time of this iteration.  ADVANCE
+// first applies this iteration's time += dt, iteration += 1 (the work of
+// k_begin_iteration, which no kernel of the step between them reads).
+template <int BLOCK, bool ADVANCE>
+__device__ __forceinline__ void dt_finalize_body(const Params& P, DevScalars* S, const double* block_min, int nb) {
+  if (S->stop || S->err) return;
+  __shared__ double red[BLOCK / 64];
+  double v = 1.7976931348623157e308;
+  for (int b = threadIdx.x; b < nb; b += BLOCK) v = smin(v, block_min[b]);
+  for (int off = 32; off > 0; off >>= 1) v = smin(v, __shfl_down(v, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double minimum_aux = red[0];
+  for (int w = 1; w < BLOCK / 64; ++w) minimum_aux = smin(minimum_aux, red[w]);
+  const double old = S->dt;
+  if (ADVANCE) {
+    S->time += old;
+    S->iteration += 1;
+  }
+  S->dt_old = old;
+  double new_dt = P.cfl * minimum_aux;
+  new_dt = smin(new_dt, 1.05 * old);
+  new_dt = smin(new_dt, P.dtmax);
+  if (new_dt < P.dtmin) {
+    S->err = 1;
+    return;
+  }
+  S->dt = new_dt;
+  const double t = S->time;
+  const bool not_yet_finish = (t < P.tend);
+  const bool finish = (t > P.tend);
+  const bool too_much = ((t + new_dt) > P.tend);
+  if ((not_yet_finish && too_much) || finish) S->stop = 1;
+}
+
+// ---------------------------------------------------------------------------
+// K4: dual (node) sweep in direction D.  computeDualGradPhi_LimC
+// (DualPhaseRemap.cc:700-779), node mass fluxes (ncontrib 215-260), upwind
+// momentum update (352-491) and the phi_dual refresh (494-523).  FIRST builds
+// the dual state from (u, m_node) as computeVariablesForRemap_PBorn0 does
+// (PrepareRemap.cc:411-433); the last sweep ends with the nodal velocity
+// u = U_dual / m_dual of remapVariables (RemapADIFinal.cc:660-672).
+// ---------------------------------------------------------------------------
+template <class R>
+__device__ inline R node_mass_from_cells(const Params& P, const R* m, int i, int j) {
+  R msum = R(0);
+  const int cdi[4] = {-1, 0, -1, 0}, cdj[4] = {-1, -1, 0, 0};
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    int ci = i + cdi[t], cj = j + cdj[t];
+    if (ci < 0 || ci >= P.nx || cj < 0 || cj >= P.ny) continue;
+    msum += m[ci + (long)P.nx * cj];
+  }
+  return R(0.25) * msum;
+}
+
+// Dual sweep along x with lane exchange: a wavefront holds 64 consecutive
+// nodes of a row (60 outputs, two halo nodes on each side).  Every node forms
+// its phi_dual and its limited gradient once and reads its neighbours' values
+// across lanes.  front(i) sums the cells (i, j-1), (i, j) in the same order as
+// back(i+1) does, so back is the front of the previous lane.
+template <class R, bool FIRST, int NEM, int LIM>
+__device__ __forceinline__ void node_sweep_x_2d_body(const Params& P, DevScalars* S, const NodeSweepArgsT<R>& A,
+                                                     int blk) {
+  if (S->stop || S->err) return;
+  const int lane = threadIdx.x & 63;
+  const int N = P.nx;
+  const int nseg = (N + 1 + 59) / 60;
+  const long wave = ((long)blk * blockDim.x + threadIdx.x) >> 6;
+  const int j = (int)(wave / nseg);
+  if (j > P.ny) return;  // uniform across the wave
+  const int seg = (int)(wave % nseg);
+  const int i = seg * 60 - 2 + lane;
+  const bool ex = (i >= 0 && i <= N);
+  const bool out = ex && lane >= 2 && lane <= 61;
+  const long nid = ex ? i + (long)(N + 1) * j : 0;
+  const bool inner_t = (j >= 1 && j <= P.ny - 1);
+  const bool inner_d = (i >= 1 && i <= N - 1);
+  const long NC = P.nc;
+  const int ne = P.ne;
+  const double dt = S->dt;
+  R U0 = R(0), U1 = R(0), U3 = R(0), p0 = R(0), p1 = R(0);
+  double xd = 0.;
+  if (ex) {
+    xd = A.xnx[nid];
+    if (FIRST) {
+      const R mn = node_mass_from_cells(P, A.m, i, j);
+      p0 = A.ux[nid];
+      p1 = A.uy[nid];
+      U0 = mn * p0;
+      U1 = mn * p1;
+      U3 = mn;
+    } else {
+      U0 = A.Ud0[nid];
+      U1 = A.Ud1[nid];
+      U3 = A.Ud3[nid];
+      // nodes inner in the first direction carry phi = U/m (DualPhaseRemap.cc:494-523)
+      if (inner_t) {
+        p0 = qdiv(U0, U3);
+        p1 = qdiv(U1, U3);
+      } else {
+        p0 = A.ux[nid];
+        p1 = A.uy[nid];
+      }
+    }
+  }
+  // front dual mass flux: cells (i, j-1), (i, j), environments in order
+  R front = R(0);
+  if (ex && i < P.nx) {
+#pragma unroll
+    for (int sh = -1; sh <= 0; ++sh) {
+      const int cj = j + sh;
+      if (cj < 0 || cj >= P.ny) continue;
+      const long cc = i + (long)P.nx * cj;
+#pragma unroll
+      for (int k = 0; k < NEM; ++k) {
+        if (k >= ne) break;
+        front += A.contrib[k * NC + cc];
+      }
+    }
+  }
+  const double xdm = __shfl_up(xd, 1, 64), xdp = __shfl_down(xd, 1, 64);
+  const R p0m = __shfl_up(p0, 1, 64), p0p = __shfl_down(p0, 1, 64);
+  const R p1m = __shfl_up(p1, 1, 64), p1p = __shfl_down(p1, 1, 64);
+  const R back = __shfl_up(front, 1, 64);
+  // limited nodal gradient (computeDualGradPhi_LimC, DualPhaseRemap.cc:700-779)
+  R g0 = R(0), g1 = R(0);
+  if (P.order2 > 0 && ex) {
+    if (inner_d) {
+      const R dxf = (R)(xdp - xd), dxb = (R)(xd - xdm);
+      const R gf0 = qdiv(p0p - p0, dxf), gb0 = qdiv(p0 - p0m, dxb);
+      const R gf1 = qdiv(p1p - p1, dxf), gb1 = qdiv(p1 - p1m, dxb);
+      g0 = limited_grad<LIM, false, R, kBranchFree<R>>(gb0, gf0);
+      g1 = limited_grad<LIM, false, R, kBranchFree<R>>(gb1, gf1);
+      if (!inner_t && out) {  // kept for the sweep in the other direction
+        A.dg0[nid] = g0;
+        A.dg1[nid] = g1;
+      }
+    } else {  // boundary node along x: gradient left by an earlier sweep
+      g0 = A.dg0[nid];
+      g1 = A.dg1[nid];
+    }
+  }
+  const R g0m = __shfl_up(g0, 1, 64), g0p = __shfl_down(g0, 1, 64);
+  const R g1m = __shfl_up(g1, 1, 64), g1p = __shfl_down(g1, 1, 64);
+  if (!out) return;
+  if (inner_d) {
+    U3 += back - front;
+    const R order2 = (R)P.order2;
+    const R ufront = R(0.5) * (p0 + p0);  // DualPhaseRemap.cc:391
+    const int signfront = (ufront > R(0)) ? 1 : -1;
+    const double xfr = xdp - xd;
+    const bool sf = (front < R(0));
+    const R tf = (R)(signfront * xfr - dt * ufront);
+    const R FV0 = (sf ? p0p : p0) + order2 * (R(0.5) * (sf ? g0p : g0) * tf);
+    const R FV1 = (sf ? p1p : p1) + order2 * (R(0.5) * (sf ? g1p : g1) * tf);
+    const R uback = R(0.5) * (p0m + p0);
+    const int signback = (uback > R(0)) ? 1 : -1;
+    const double xbk = xd - xdm;
+    const bool sb = (back > R(0));
+    const R tb = (R)(signback * xbk - dt * uback);
+    const R BV0 = (sb ? p0m : p0) + order2 * (R(0.5) * (sb ? g0m : g0) * tb);
+    const R BV1 = (sb ? p1m : p1) + order2 * (R(0.5) * (sb ? g1m : g1) * tb);
+    U0 += back * BV0 - front * FV0;
+    U1 += back * BV1 - front * FV1;
+    const R thresold = (R)1.e-300;  // 0 in float: the filter only acts in double
+    if (fabs(U0) < thresold) U0 = R(0);
+    if (fabs(U1) < thresold) U1 = R(0);
+  }
+  if (FIRST) {
+    st_stream(&A.Ud0[nid], U0);
+    st_stream(&A.Ud1[nid], U1);
+    st_stream(&A.Ud3[nid], U3);
+  } else if (U3 != R(0)) {
+    st_stream(&A.uox[nid], qdiv(U0, U3));
+    st_stream(&A.uoy[nid], qdiv(U1, U3));
+  } else {
+    st_stream(&A.uox[nid], A.ux[nid]);
+    st_stream(&A.uoy[nid], A.uy[nid]);
+  }
+}
+
+// Dual sweep along y on LDS tiles: a block of 16 wavefronts owns 16 node
+// columns and 64 node rows (60 outputs, two halo rows on each side).  The
+// row-major pass loads the nodes (coalesced) and forms phi_dual and the front
+// flux; the tile is transposed through LDS, each wavefront runs the lane
+// arithmetic of the x dual sweep on one column, and the increments of the
+// dual state travel back through LDS to be added and stored row-major.
+// SEG: whole node columns of SL = ny + 5 lanes (ny + 1 nodes and two halos
+// on each side), K = 64 / SL of them per wavefront, as in the cell sweep.
+template <class R, bool FIRST, int NEM, int LIM, bool SEG>
+__device__ __forceinline__ void node_sweep_ytile_2d_body(const Params& P, DevScalars* S,
+                                                         const NodeSweepArgsT<R>& A, int L) {
+  __shared__ double ldx[kTileR * kTileP];   // coordinate along y
+  __shared__ R lds[3][kTileR * kTileP];     // phi_dual x, y and front flux; then the increments
+  if (S->stop || S->err) return;  // uniform over the block
+  const int nx = P.nx, N = P.ny, nnx = nx + 1;
+  const YTileShape<SEG> sh(N + 1);
+  const int TW = SEG ? sh.TW : kTileW, TP = SEG ? sh.TP : kTileP;
+  const int ncolblk = (nnx + TW - 1) / TW;
+  const int i0 = (L % ncolblk) * TW;
+  const int s0 = SEG ? -2 : (L / ncolblk) * 60 - 2;  // node row of tile row 0
+  const int t = threadIdx.x;
+  const int tr = SEG ? t / TW : t >> 4, tc = SEG ? t % TW : t & 15;
+  const bool trow = SEG ? tr < sh.SL : true;
+  const int w = t >> 6, lane = t & 63;
+  const int col = SEG ? w * sh.K + lane / sh.SL : w, r = SEG ? lane % sh.SL : lane;
+  const bool tlane = SEG ? lane < sh.K * sh.SL : true;
+  const int ta = r * TP + col;   // transposed LDS index
+  const int ra = tr * TP + tc;   // row-major LDS index
+  const long NC = P.nc;
+  const int ne = P.ne;
+  const double dt = S->dt;
+  // ---- row-major pass
+  const int gi = i0 + tc, gs = s0 + tr;
+  const bool gex = trow && gi <= nx && gs >= 0 && gs <= N;
+  const bool gout = SEG ? gex : gex && tr >= 2 && tr <= 61;
+  const long gq = gex ? gi + (long)nnx * gs : 0;
+  R U0 = R(0), U1 = R(0), U3 = R(0);
+  {
+    double xd = 0.;
+    R p0 = R(0), p1 = R(0), f = R(0);
+    if (gex) {
+      const bool inner_t = (gi >= 1 && gi <= nx - 1);
+      xd = A.xny[gq];
+      if (FIRST) {
+        const R mn = node_mass_from_cells(P, A.m, gi, gs);
+        p0 = A.ux[gq];
+        p1 = A.uy[gq];
+        U0 = mn * p0;
+        U1 = mn * p1;
+        U3 = mn;
+      } else {
+        U0 = A.Ud0[gq];
+        U1 = A.Ud1[gq];
+        U3 = A.Ud3[gq];
+        if (inner_t) {  // DualPhaseRemap.cc:494-523
+          p0 = qdiv(U0, U3);
+          p1 = qdiv(U1, U3);
+        } else {
+          p0 = A.ux[gq];
+          p1 = A.uy[gq];
+        }
+      }
+      if (gs < N) {  // front flux: cells (i-1, s), (i, s), environments in order
+#pragma unroll
+        for (int sh = -1; sh <= 0; ++sh) {
+          const int ci = gi + sh;
+          if (ci < 0 || ci >= nx) continue;
+          const long cc = ci + (long)nx * gs;
+#pragma unroll
+          for (int k = 0; k < NEM; ++k) {
+            if (k >= ne) break;
+            f += A.contrib[k * NC + cc];
+          }
+        }
+      }
+    }
+    if (trow) {
+      ldx[ra] = xd;
+      lds[0][ra] = p0;
+      lds[1][ra] = p1;
+      lds[2][ra] = f;
+    }
+  }
+  __syncthreads();
+  // ---- transposed pass: column i, node row s
+  const int i = i0 + col, s = s0 + r;
+  const bool ex = tlane && i <= nx && s >= 0 && s <= N;
+  const bool out = SEG ? ex : ex && lane >= 2 && lane <= 61;
+  const bool inner_t = (i >= 1 && i <= nx - 1);
+  const bool inner_d = (s >= 1 && s <= N - 1);
+  const double xd = tlane ? ldx[ta] : 0.;
+  const R p0 = tlane ? lds[0][ta] : R(0), p1 = tlane ? lds[1][ta] : R(0);
+  const R front = tlane ? lds[2][ta] : R(0);
+  const double xdm = __shfl_up(xd, 1, 64), xdp = __shfl_down(xd, 1, 64);
+  const R p0m = __shfl_up(p0, 1, 64), p0p = __shfl_down(p0, 1, 64);
+  const R p1m = __shfl_up(p1, 1, 64), p1p = __shfl_down(p1, 1, 64);
+  const R back = __shfl_up(front, 1, 64);
+  R g0 = R(0), g1 = R(0);
+  if (P.order2 > 0 && ex) {
+    const long q = i + (long)nnx * s;
+    if (inner_d) {
+      const R dxf = (R)(xdp - xd), dxb = (R)(xd - xdm);
+      const R gf0 = qdiv(p0p - p0, dxf), gb0 = qdiv(p0 - p0m, dxb);
+      const R gf1 = qdiv(p1p - p1, dxf), gb1 = qdiv(p1 - p1m, dxb);
+      g0 = limited_grad<LIM, false, R, kBranchFree<R>>(gb0, gf0);
+      g1 = limited_grad<LIM, false, R, kBranchFree<R>>(gb1, gf1);
+      if (!inner_t && out) {  // kept for the sweep in the other direction
+        A.dg0[q] = g0;
+        A.dg1[q] = g1;
+      }
+    } else {  // boundary node along y: gradient left by an earlier sweep
+      g0 = A.dg0[q];
+      g1 = A.dg1[q];
+    }
+  }
+  const R g0m = __shfl_up(g0, 1, 64), g0p = __shfl_down(g0, 1, 64);
+  const R g1m = __shfl_up(g1, 1, 64), g1p = __shfl_down(g1, 1, 64);
+  R d0 = R(0), d1 = R(0), d3 = R(0);
+  if (out && inner_d) {
+    const R order2 = (R)P.order2;
+    d3 = back - front;
+    const R ufront = R(0.5) * (p1 + p1);  // DualPhaseRemap.cc:391
+    const int signfront = (ufront > R(0)) ? 1 : -1;
+    const double xfr = xdp - xd;
+    const bool sf = (front < R(0));
+    const R tf = (R)(signfront * xfr - dt * ufront);
+    const R FV0 = (sf ? p0p : p0) + order2 * (R(0.5) * (sf ? g0p : g0) * tf);
+    const R FV1 = (sf ? p1p : p1) + order2 * (R(0.5) * (sf ? g1p : g1) * tf);
+    const R uback = R(0.5) * (p1m + p1);
+    const int signback = (uback > R(0)) ? 1 : -1;
+    const double xbk = xd - xdm;
+    const bool sb = (back > R(0));
+    const R tb = (R)(signback * xbk - dt * uback);
+    const R BV0 = (sb ? p0m : p0) + order2 * (R(0.5) * (sb ? g0m : g0) * tb);
+    const R BV1 = (sb ? p1m : p1) + order2 * (R(0.5) * (sb ? g1m : g1) * tb);
+    d0 = back * BV0 - front * FV0;
+    d1 = back * BV1 - front * FV1;
+  }
+  __syncthreads();  // transposed reads done before the tiles are reused
+  if (tlane) {
+    lds[0][ta] = d0;
+    lds[1][ta] = d1;
+    lds[2][ta] = d3;
+  }
+  __syncthreads();
+  // ---- row-major: U += increments, store
+  if (!gout) return;
+  if (gs >= 1 && gs <= N - 1) {
+    U3 += lds[2][ra];
+    U0 += lds[0][ra];
+    U1 += lds[1][ra];
+    const R thresold = (R)1.e-300;  // 0 in float: the filter only acts in double
+    if (fabs(U0) < thresold) U0 = R(0);
+    if (fabs(U1) < thresold) U1 = R(0);
+  }
+  if (FIRST) {
+    st_stream(&A.Ud0[gq], U0);
+    st_stream(&A.Ud1[gq], U1);
+    st_stream(&A.Ud3[gq], U3);
+  } else if (U3 != R(0)) {
+    st_stream(&A.uox[gq], qdiv(U0, U3));
+    st_stream(&A.uoy[gq], qdiv(U1, U3));
+  } else {
+    st_stream(&A.uox[gq], A.ux[gq]);
+    st_stream(&A.uoy[gq], A.uy[gq]);
+  }
+}
+
+}  // namespace mhg

@@ -16,134 +16,47 @@
 #include <vector>
 
 #include "mhg_common.hpp"
+#include "mhg_ctx.hpp"
 #include "mhg_kernels.hpp"
 
 using namespace mhg;
 
-namespace {
+namespace mhg {
 
-struct DBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-};
-
-}  // namespace
-
-struct mhg_ctx {
-  mhg_options opt{};
-  Params P{};
-  std::string err;
-  bool initialised = false;
-  hipStream_t stream = nullptr;
-  int device = 0;
-  std::vector<DBuf> owned;
-
-  // Bulk fields hold `bulk`-byte words: double for the drop-in, float for the
-  // precision form (mhg_options.precision).  Coordinates stay double.
-  int bulk = 8;
-  // cells
-  void *rho = nullptr, *e = nullptr, *p = nullptr, *cs = nullptr, *q = nullptr, *m = nullptr;
-  void *L = nullptr, *Vlag = nullptr;
-  int* mask = nullptr;
-  void* U[2] = {nullptr, nullptr};
-  void* contrib = nullptr;
-  // environments [ne][nc]
-  void *env_rho = nullptr, *env_e = nullptr, *env_p = nullptr, *env_cs = nullptr;
-  void *env_q = nullptr, *env_m = nullptr, *env_f = nullptr, *env_mf = nullptr;
-  // nodes
-  void* ux[2] = {nullptr, nullptr};
-  void* uy[2] = {nullptr, nullptr};
-  double* xx[2] = {nullptr, nullptr};
-  double* xy[2] = {nullptr, nullptr};
-  void *Ud0 = nullptr, *Ud1 = nullptr, *Ud3 = nullptr, *dg0 = nullptr, *dg1 = nullptr;
-  // 3D: z components, a second dual state (the middle sweep reads one and
-  // writes the other), the Euler volumes of the hexahedra
-  double* xz[2] = {nullptr, nullptr};
-  void* uz[2] = {nullptr, nullptr};
-  void *Ud2 = nullptr, *dg2 = nullptr;
-  void* UdB[4] = {nullptr, nullptr, nullptr, nullptr};
-  double* Veuler = nullptr;
-  double* block_min = nullptr;
-  DevScalars* S = nullptr;
-  DevScalars* S_host = nullptr;  // pinned mirror
-
-  // host mirrors of the step bookkeeping
-  int xcur = 0;  // buffer with the coordinates of the last Lagrange phase
-  int ucur = 0;  // buffer with the current velocity
-  int sens = 0;  // SensProjection
-  int yvariant = 0;   // launch variant of the y cell sweep (mhg_autotune)
-  int nxvariant = 1;  // launch variant of the x dual sweep
-  int nyvariant = 1;  // launch variant of the y dual sweep
-
-  // per-entry-point path: the _n, tau and div u arrays (allocated on the
-  // first entry-point call) and which coordinates are current between calls
-  double *rho_n = nullptr, *e_n = nullptr, *q_n = nullptr, *tau = nullptr, *divu = nullptr;
-  double *env_rho_n = nullptr, *env_e_n = nullptr, *env_q_n = nullptr, *env_tau = nullptr;
-  bool ep_ready = false;
-  int coord_phase = 0;  // 0: xx[xcur] (last Lagrange phase); 1: x0 (after
-                        // saveValuesAtN); 2: xx[1 - xcur] (after updatePosition)
-  bool ep_remapped = false;  // k_cell_final_2d ran since the last dt
-
-  // captured groups of fused iterations, one per host buffer state
-  struct StepGraph {
-    int xcur, ucur, sens;
-    hipGraphExec_t exec;
-  };
-  std::vector<StepGraph> graphs;
-  int use_graphs = 1;
-
-  // kernel timing
-  bool timing = false;
-  std::vector<std::string> tnames;
-  std::vector<double> tsum;
-  std::vector<int> tcount;
-  std::vector<hipEvent_t> evpool;
-  std::vector<std::pair<int, int>> pending;  // (name index, event pair index)
-  size_t evused = 0;
-};
-
-namespace {
-
-#define HIPCHK(ctx, x)                                                   \
-  do {                                                                   \
-    hipError_t e_ = (x);                                                 \
-    if (e_ != hipSuccess) {                                              \
-      (ctx)->err = std::string(#x) + ": " + hipGetErrorString(e_);       \
-      return MHG_E_HIP;                                                  \
-    }                                                                    \
-  } while (0)
-
-int fail(mhg_ctx* c, int code, const std::string& msg) {
+int ctx_fail(mhg_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
   return code;
 }
 
-template <class T>
-int dalloc(mhg_ctx* c, T** out, size_t count) {
+int ctx_dalloc_bytes(mhg_ctx* c, void** out, size_t bytes) {
   void* p = nullptr;
-  size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
-  if (hipMalloc(&p, bytes) != hipSuccess) return fail(c, MHG_E_ALLOC, "hipMalloc failed");
-  if (hipMemset(p, 0, bytes) != hipSuccess) return fail(c, MHG_E_HIP, "hipMemset failed");
-  c->owned.push_back({p, bytes});
-  *out = (T*)p;
-  return MHG_OK;
-}
-
-// Bulk array of `count` words of the context's storage type.
-int balloc(mhg_ctx* c, void** out, size_t count) {
-  void* p = nullptr;
-  size_t bytes = std::max<size_t>(count, 1) * (size_t)c->bulk;
-  if (hipMalloc(&p, bytes) != hipSuccess) return fail(c, MHG_E_ALLOC, "hipMalloc failed");
-  if (hipMemset(p, 0, bytes) != hipSuccess) return fail(c, MHG_E_HIP, "hipMemset failed");
+  *out = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) return ctx_fail(c, MHG_E_ALLOC, "hipMalloc failed");
+  if (hipMemset(p, 0, bytes) != hipSuccess) {
+    hipFree(p);
+    return ctx_fail(c, MHG_E_HIP, "hipMemset failed");
+  }
   c->owned.push_back({p, bytes});
   *out = p;
   return MHG_OK;
 }
 
-template <class R>
-R* bp(void* p) {
-  return static_cast<R*>(p);
+}  // namespace mhg
+
+namespace {
+
+int fail(mhg_ctx* c, int code, const std::string& msg) { return ctx_fail(c, code, msg); }
+
+template <class T>
+int dalloc(mhg_ctx* c, T** out, size_t count) {
+  return ctx_dalloc(c, out, count);
 }
+
+// Bulk array of `count` words of the context's storage type.
+int balloc(mhg_ctx* c, void** out, size_t count) {
+  return ctx_dalloc_bytes(c, out, std::max<size_t>(count, 1) * (size_t)c->bulk);
+}
+
 
 // Host doubles to a device bulk array (rounded to float in the precision form).
 hipError_t put(mhg_ctx* c, void* d, const double* h, size_t n) {
@@ -263,116 +176,28 @@ void collect_times(mhg_ctx* c) {
   c->evused = 0;
 }
 
-// ---- argument blocks ---------------------------------------------------
-template <class R>
-NodeLagArgsT<R> node_lag_args(mhg_ctx* c) {
-  NodeLagArgsT<R> A;
-  A.p = bp<R>(c->p);
-  A.q = bp<R>(c->q);
-  A.m = bp<R>(c->m);
-  A.xpx = c->xx[c->xcur];
-  A.xpy = c->xy[c->xcur];
-  A.ux = bp<R>(c->ux[c->ucur]);
-  A.uy = bp<R>(c->uy[c->ucur]);
-  A.xnx = c->xx[1 - c->xcur];
-  A.xny = c->xy[1 - c->xcur];
-  return A;
-}
-
-template <class R>
-CellLagArgsT<R> cell_lag_args(mhg_ctx* c) {
-  CellLagArgsT<R> A;
-  A.xnx = c->xx[1 - c->xcur];
-  A.xny = c->xy[1 - c->xcur];
-  A.rho = bp<R>(c->rho);
-  A.e = bp<R>(c->e);
-  A.p = bp<R>(c->p);
-  A.cs = bp<R>(c->cs);
-  A.q = bp<R>(c->q);
-  A.m = bp<R>(c->m);
-  A.mask = c->mask;
-  A.env_f = bp<R>(c->env_f);
-  A.env_rho = bp<R>(c->env_rho);
-  A.env_m = bp<R>(c->env_m);
-  A.env_q = bp<R>(c->env_q);
-  A.env_p = bp<R>(c->env_p);
-  A.env_e = bp<R>(c->env_e);
-  A.U = bp<R>(c->U[0]);
-  A.Vlag = bp<R>(c->Vlag);
-  A.L = bp<R>(c->L);
-  return A;
-}
-
 // Remap (IRemap::appliRemap, RemapADIService.cc:19-60): for i < dim,
 // idir = (i + SensProjection) % dim, cell sweep then dual sweep; then
 // SensProjection = (SensProjection + 1) % dim and remapVariables.
 template <class R>
 void remap_2d(mhg_ctx* c, int dt_mode) {
   const Params& P = c->P;
-  const double* xnx = c->xx[1 - c->xcur];
-  const double* xny = c->xy[1 - c->xcur];
-  const R* ux = bp<R>(c->ux[c->ucur]);
-  const R* uy = bp<R>(c->uy[c->ucur]);
   for (int pass = 0; pass < P.dim; ++pass) {
     const int idir = (pass + c->sens) % P.dim;
     const bool first = (pass == 0);
-    CellSweepArgsT<R> A;
-    A.Uin = bp<R>(c->U[pass & 1]);
-    A.Uout = bp<R>(c->U[(pass + 1) & 1]);
-    A.Vlag = bp<R>(c->Vlag);
-    A.xnx = xnx;
-    A.xny = xny;
-    A.ux = ux;
-    A.uy = uy;
-    A.contrib = bp<R>(c->contrib);
     {
       TimedLaunch t(c, idir == 0 ? "cell_sweep_x" : "cell_sweep_y");
-      launch_cell_sweep_2d(c->stream, P, c->S, A, idir, first, c->yvariant);
+      launch_cell_sweep_2d(c->stream, P, c->S, cell_sweep_args<R>(c, pass), idir, first, c->yvariant);
     }
-    NodeSweepArgsT<R> B;
-    B.m = bp<R>(c->m);
-    B.ux = ux;
-    B.uy = uy;
-    B.xnx = xnx;
-    B.xny = xny;
-    B.contrib = bp<R>(c->contrib);
-    B.Ud0 = bp<R>(c->Ud0);
-    B.Ud1 = bp<R>(c->Ud1);
-    B.Ud3 = bp<R>(c->Ud3);
-    B.dg0 = bp<R>(c->dg0);
-    B.dg1 = bp<R>(c->dg1);
-    B.uox = bp<R>(c->ux[1 - c->ucur]);
-    B.uoy = bp<R>(c->uy[1 - c->ucur]);
     {
       TimedLaunch t(c, idir == 0 ? "node_sweep_x" : "node_sweep_y");
-      launch_node_sweep_2d(c->stream, P, c->S, B, idir, first, idir == 0 ? c->nxvariant : c->nyvariant);
+      launch_node_sweep_2d(c->stream, P, c->S, node_sweep_args<R>(c), idir, first,
+                           idir == 0 ? c->nxvariant : c->nyvariant);
     }
   }
-  CellFinalArgsT<R> F;
-  F.U = bp<R>(c->U[P.dim & 1]);
-  F.Vlag = bp<R>(c->Vlag);
-  F.L = bp<R>(c->L);
-  F.ux = bp<R>(c->ux[1 - c->ucur]);
-  F.uy = bp<R>(c->uy[1 - c->ucur]);
-  F.rho = bp<R>(c->rho);
-  F.e = bp<R>(c->e);
-  F.p = bp<R>(c->p);
-  F.cs = bp<R>(c->cs);
-  F.q = bp<R>(c->q);
-  F.m = bp<R>(c->m);
-  F.mask = c->mask;
-  F.env_rho = bp<R>(c->env_rho);
-  F.env_e = bp<R>(c->env_e);
-  F.env_p = bp<R>(c->env_p);
-  F.env_cs = bp<R>(c->env_cs);
-  F.env_q = bp<R>(c->env_q);
-  F.env_m = bp<R>(c->env_m);
-  F.env_f = bp<R>(c->env_f);
-  F.env_mf = bp<R>(c->env_mf);
-  F.block_min = c->block_min;
   {
     TimedLaunch t(c, "cell_final");
-    launch_cell_final_2d(c->stream, P, c->S, F, dt_mode);
+    launch_cell_final_2d(c->stream, P, c->S, cell_final_args<R>(c), dt_mode);
   }
   c->sens = (c->sens + 1) % P.dim;
   c->xcur = 1 - c->xcur;
@@ -563,9 +388,17 @@ int check_ready(mhg_ctx* c) {
   return MHG_OK;
 }
 
+// A problem of a batch is advanced by mhg_batch_run only.
+int check_not_batch(mhg_ctx* c) {
+  if (c && c->owner) return fail(c, MHG_E_STATE, "problem of a batch: advance it with mhg_batch_run");
+  return MHG_OK;
+}
+
 // The fused step starts from a state between iterations.
 int check_fused(mhg_ctx* c) {
   int r = check_ready(c);
+  if (r) return r;
+  r = check_not_batch(c);
   if (r) return r;
   if (c->coord_phase != 0)
     return fail(c, MHG_E_STATE, "fused step inside an entry-point iteration: finish it with mhg_remap");
@@ -596,6 +429,7 @@ double euler_volume(const Params& P, long cc) {
 
 int init_pure(mhg_ctx* c, const std::vector<int>& env, const std::vector<double>& rho,
               const std::vector<double>& pres, const double* vel) {
+  if (batch_running(c)) return fail(c, MHG_E_STATE, "problem of a running batch: state fixed at the first run");
   const Params& P = c->P;
   const long NC = P.nc, NN = P.nn;
   std::vector<double> e(NC), cs(NC), m(NC), zero(NC, 0.);
@@ -674,6 +508,72 @@ void cell_centre(const Params& P, long cc, double* xc, double* yc) {
 
 }  // namespace
 
+namespace mhg {
+
+// Mesh and options of mhg_create -> run-time parameters (validation included).
+int build_params(const mhg_mesh_desc* mesh, const mhg_options* opt, Params* Pout) {
+  if (!mesh || !opt || !Pout) return MHG_E_ARG;
+  if (mesh->dim != 2 && mesh->dim != 3) return MHG_E_ARG;
+  for (int d = 0; d < mesh->dim; ++d)
+    if (mesh->n[d] < 1 || !(mesh->length[d] > 0.)) return MHG_E_ARG;
+  if (opt->nb_env < 1 || opt->nb_env > MHG_MAX_ENV || opt->nb_bc < 0 || opt->nb_bc > MHG_MAX_BC)
+    return MHG_E_ARG;
+  if (opt->schema_csts || opt->with_newton || opt->sans_lagrange || !opt->with_projection ||
+      !opt->is_euler_scheme || opt->projection_pente_borne || opt->conservation_energie_totale ||
+      opt->projection_limiteur_id >= MHG_LIM_MINMODG || opt->projection_limiteur_id < 0 ||
+      opt->ordre_projection < 1 || opt->ordre_projection > 2)
+    return MHG_E_UNSUPPORTED;
+  if (opt->precision != MHG_PREC_FP64 && opt->precision != MHG_PREC_FP32_BULK) return MHG_E_UNSUPPORTED;
+  Params P{};
+  P.dim = mesh->dim;
+  P.ne = opt->nb_env;
+  P.nvu = 3 * P.ne + 1;
+  P.nx = mesh->n[0];
+  P.ny = mesh->n[1];
+  P.nz = mesh->dim == 3 ? mesh->n[2] : 1;
+  P.nc = (long)P.nx * P.ny * (P.dim == 3 ? P.nz : 1);
+  P.nn = (long)(P.nx + 1) * (P.ny + 1) * (P.dim == 3 ? P.nz + 1 : 1);
+  for (int d = 0; d < 3; ++d) {
+    P.org[d] = mesh->origin[d];
+    P.h[d] = d < P.dim ? mesh->length[d] / mesh->n[d] : 0.;
+  }
+  for (int k = 0; k < P.ne; ++k) {
+    P.eos.kind[k] = opt->env[k].eos;
+    P.eos.gamma[k] = opt->env[k].adiabatic_cst;
+    P.eos.tension[k] = opt->env[k].limit_tension;
+    P.eos.qcoef[k] = opt->visc_quad_coeff >= 0. ? opt->visc_quad_coeff
+                                               : (opt->env[k].adiabatic_cst + 1) / 2.0;
+  }
+  P.threshold = opt->threshold;
+  P.remap_threshold = opt->remap_threshold;
+  P.order2 = opt->ordre_projection - 1;
+  P.lim = opt->projection_limiteur_id;
+  P.pseudo_centree = opt->pseudo_centree;
+  P.lc = opt->longueur_caracteristique;
+  P.cfl = opt->cfl;
+  P.dtmin = opt->deltat_min;
+  P.dtmax = opt->deltat_max;
+  P.tend = opt->final_time;
+  P.nbc = 0;
+  for (int b = 0; b < opt->nb_bc; ++b) {
+    const mhg_bc& bc = opt->bc[b];
+    if (bc.surface < 0 || bc.surface > 5 || bc.type < 0 || bc.type > 2) return MHG_E_ARG;
+    if (bc.type >= P.dim) {  // Vz in 2D: u_z is identically zero
+      if (bc.value != 0.) return MHG_E_UNSUPPORTED;
+      continue;
+    }
+    BcRange r;
+    if (!surface_range(P, bc.surface, &r.axis, &r.lo, &r.hi, P.threshold)) return MHG_E_UNSUPPORTED;
+    r.comp = bc.type;
+    r.value = bc.value;
+    P.bc[P.nbc++] = r;
+  }
+  *Pout = P;
+  return MHG_OK;
+}
+
+}  // namespace mhg
+
 extern "C" {
 
 int mhg_abi_version(void) { return MHG_ABI_VERSION; }
@@ -711,72 +611,13 @@ void mhg_default_options(mhg_options* o) {
 int mhg_create(const mhg_mesh_desc* mesh, const mhg_options* opt, mhg_ctx** out) {
   if (!mesh || !opt || !out) return MHG_E_ARG;
   *out = nullptr;
-  if (mesh->dim != 2 && mesh->dim != 3) return MHG_E_ARG;
-  for (int d = 0; d < mesh->dim; ++d)
-    if (mesh->n[d] < 1 || !(mesh->length[d] > 0.)) return MHG_E_ARG;
-  if (opt->nb_env < 1 || opt->nb_env > MHG_MAX_ENV || opt->nb_bc < 0 || opt->nb_bc > MHG_MAX_BC)
-    return MHG_E_ARG;
-  if (opt->schema_csts || opt->with_newton || opt->sans_lagrange || !opt->with_projection ||
-      !opt->is_euler_scheme || opt->projection_pente_borne || opt->conservation_energie_totale ||
-      opt->projection_limiteur_id >= MHG_LIM_MINMODG || opt->projection_limiteur_id < 0 ||
-      opt->ordre_projection < 1 || opt->ordre_projection > 2)
-    return MHG_E_UNSUPPORTED;
-  if (opt->precision != MHG_PREC_FP64 && opt->precision != MHG_PREC_FP32_BULK) return MHG_E_UNSUPPORTED;
+  Params Pc{};
+  const int pr = build_params(mesh, opt, &Pc);
+  if (pr) return pr;
   mhg_ctx* c = new mhg_ctx();
   c->opt = *opt;
-  Params& P = c->P;
-  P.dim = mesh->dim;
-  P.ne = opt->nb_env;
-  P.nvu = 3 * P.ne + 1;
-  P.nx = mesh->n[0];
-  P.ny = mesh->n[1];
-  P.nz = mesh->dim == 3 ? mesh->n[2] : 1;
-  P.nc = (long)P.nx * P.ny * (P.dim == 3 ? P.nz : 1);
-  P.nn = (long)(P.nx + 1) * (P.ny + 1) * (P.dim == 3 ? P.nz + 1 : 1);
-  for (int d = 0; d < 3; ++d) {
-    P.org[d] = mesh->origin[d];
-    P.h[d] = d < P.dim ? mesh->length[d] / mesh->n[d] : 0.;
-  }
-  for (int k = 0; k < P.ne; ++k) {
-    P.eos.kind[k] = opt->env[k].eos;
-    P.eos.gamma[k] = opt->env[k].adiabatic_cst;
-    P.eos.tension[k] = opt->env[k].limit_tension;
-    P.eos.qcoef[k] = opt->visc_quad_coeff >= 0. ? opt->visc_quad_coeff
-                                               : (opt->env[k].adiabatic_cst + 1) / 2.0;
-  }
-  P.threshold = opt->threshold;
-  P.remap_threshold = opt->remap_threshold;
-  P.order2 = opt->ordre_projection - 1;
-  P.lim = opt->projection_limiteur_id;
-  P.pseudo_centree = opt->pseudo_centree;
-  P.lc = opt->longueur_caracteristique;
-  P.cfl = opt->cfl;
-  P.dtmin = opt->deltat_min;
-  P.dtmax = opt->deltat_max;
-  P.tend = opt->final_time;
-  P.nbc = 0;
-  for (int b = 0; b < opt->nb_bc; ++b) {
-    const mhg_bc& bc = opt->bc[b];
-    if (bc.surface < 0 || bc.surface > 5 || bc.type < 0 || bc.type > 2) {
-      delete c;
-      return MHG_E_ARG;
-    }
-    if (bc.type >= P.dim) {  // Vz in 2D: u_z is identically zero
-      if (bc.value != 0.) {
-        delete c;
-        return MHG_E_UNSUPPORTED;
-      }
-      continue;
-    }
-    BcRange r;
-    if (!surface_range(P, bc.surface, &r.axis, &r.lo, &r.hi, P.threshold)) {
-      delete c;
-      return MHG_E_UNSUPPORTED;
-    }
-    r.comp = bc.type;
-    r.value = bc.value;
-    P.bc[P.nbc++] = r;
-  }
+  c->P = Pc;
+  const Params& P = c->P;
   if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
     delete c;
     return MHG_E_HIP;
@@ -827,6 +668,7 @@ int mhg_create(const mhg_mesh_desc* mesh, const mhg_options* opt, mhg_ctx** out)
 
 int mhg_destroy(mhg_ctx* c) {
   if (!c) return MHG_OK;
+  if (c->owner) return fail(c, MHG_E_ARG, "problem of a batch: destroyed with the batch");
   if (c->stream) hipStreamSynchronize(c->stream);
   drop_graphs(c);
   for (auto& b : c->owned) hipFree(b.p);
@@ -901,6 +743,7 @@ int mhg_init_cells(mhg_ctx* c, const int32_t* env_of_cell, const double* density
 
 int mhg_upload_state(mhg_ctx* c, const mhg_state* st) {
   if (!c || !st) return MHG_E_ARG;
+  if (batch_running(c)) return fail(c, MHG_E_STATE, "problem of a running batch: state fixed at the first run");
   const Params& P = c->P;
   const long NC = P.nc, NN = P.nn;
   const size_t nb = NN * sizeof(double);
@@ -958,6 +801,10 @@ int mhg_upload_state(mhg_ctx* c, const mhg_state* st) {
 int mhg_download_state(mhg_ctx* c, mhg_state* st) {
   int r = check_ready(c);
   if (r) return r;
+  if (c->owner) {
+    r = batch_refresh_view(c);
+    if (r) return r;
+  }
   const Params& P = c->P;
   const long NC = P.nc, NN = P.nn;
   const size_t nb = NN * sizeof(double), en = (size_t)P.ne * NC;
@@ -1061,6 +908,8 @@ namespace {
 
 int ep_prepare(mhg_ctx* c) {
   int r = check_ready(c);
+  if (r) return r;
+  r = check_not_batch(c);
   if (r) return r;
   if (c->bulk != 8 || c->P.dim != 2)
     return fail(c, MHG_E_UNSUPPORTED, "entry points: 2D FP64 contexts only (use mhg_step)");
@@ -1410,6 +1259,8 @@ int autotune_t(mhg_ctx* c, int32_t reps, char* report, int32_t report_len) {
 
 int mhg_autotune(mhg_ctx* c, int32_t reps, char* report, int32_t report_len) {
   int r = check_ready(c);
+  if (r) return r;
+  r = check_not_batch(c);
   if (r) return r;
   if (reps < 1) reps = 1;
   if (c->P.dim == 3) {  // one launch shape per 3D kernel

@@ -728,7 +728,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell_final_3d(Params P, DevScalars* S
     }
     dx_sound = smin(dx_sound, (double)qdiv(A.b.L[c], c_g + vmax));
   }
-  block_min_store<BLOCK>(dx_sound, A.b.block_min);
+  block_min_store<BLOCK>(dx_sound, A.b.block_min + blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------

@@ -283,9 +283,9 @@ __device__ inline R final_cell(const Params& P, const CellFinalArgsT<R>& A, long
   }
 }
 
-// Reduction of the CFL candidates of a block to one minimum.
+// Reduction of the CFL candidates of a block to one minimum, stored at slot.
 template <int BLOCK>
-__device__ inline void block_min_store(double v, double* block_min) {
+__device__ inline void block_min_store(double v, double* slot) {
   __shared__ double red[BLOCK / 64];
   for (int off = 32; off > 0; off >>= 1) v = smin(v, __shfl_down(v, off, 64));
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -294,7 +294,7 @@ __device__ inline void block_min_store(double v, double* block_min) {
   if (threadIdx.x == 0) {
     double b = red[0];
     for (int w = 1; w < BLOCK / 64; ++w) b = smin(b, red[w]);
-    block_min[blockIdx.x] = b;
+    *slot = b;
   }
 }
 

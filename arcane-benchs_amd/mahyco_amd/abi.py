@@ -135,7 +135,14 @@ class Library:
                 ("set_threads", C.c_int, [C.c_int32]),
                 ("autotune", C.c_int, [vp, C.c_int32, C.c_char_p, C.c_int32]),
                 ("variant_count", C.c_int32, [C.c_char_p]),
-                ("select_variant", C.c_int, [vp, C.c_char_p, C.c_int32])]:
+                ("select_variant", C.c_int, [vp, C.c_char_p, C.c_int32]),
+                ("batch_create", C.c_int, [C.c_int32, C.POINTER(MeshDesc), C.POINTER(Options), C.POINTER(vp)]),
+                ("batch_destroy", C.c_int, [vp]),
+                ("batch_last_error", C.c_char_p, [vp]),
+                ("batch_problem", vp, [vp, C.c_int32]),
+                ("batch_run", C.c_int, [vp, C.c_int32, _IP, C.POINTER(C.c_int64)]),
+                ("batch_status", C.c_int, [vp, C.c_int32, _IP, _DP, _DP, _IP]),
+                ("batch_autotune", C.c_int, [vp, C.c_int32, C.c_char_p, C.c_int32])]:
             if hasattr(L, p + name):
                 self.optional[name] = fn(name, res, args)
 
@@ -187,25 +194,28 @@ def mesh_desc(dim, n, origin, length):
 class Simulation:
     """One context of a C-ABI library (GPU product library or, in tests, the oracle)."""
 
-    def __init__(self, library, mesh, options):
+    def __init__(self, library, mesh, options, ctx=None):
+        """ctx: an existing context owned elsewhere (a problem of a Batch)."""
         self.L = library
         self.mesh = mesh
         self.opt = options
         self.dim = mesh.dim
         self.nenv = options.nb_env
-        ctx = C.c_void_p()
-        r = self.L.create(C.byref(mesh), C.byref(options), C.byref(ctx))
-        if r != OK:
-            raise MahycoError(r, "create")
+        self.owned = ctx is None
+        if ctx is None:
+            ctx = C.c_void_p()
+            r = self.L.create(C.byref(mesh), C.byref(options), C.byref(ctx))
+            if r != OK:
+                raise MahycoError(r, "create")
         self.ctx = ctx
         nc, nn = C.c_int64(), C.c_int64()
         self.L.sizes(self.ctx, C.byref(nc), C.byref(nn))
         self.nc, self.nn = nc.value, nn.value
 
     def close(self):
-        if self.ctx:
+        if self.ctx and self.owned:
             self.L.destroy(self.ctx)
-            self.ctx = None
+        self.ctx = None
 
     def __del__(self):
         try:
@@ -326,3 +336,65 @@ class Simulation:
         out = np.zeros(3)
         self._check(self.L.conservation(self.ctx, out.ctypes.data_as(_DP)), "conservation")
         return out
+
+
+class Batch:
+    """A batch of independent 2D problems (mhg_batch_*): problem(i) is a
+    Simulation over the batch's context for problem i."""
+
+    def __init__(self, library, meshes, options):
+        self.L = library
+        n = len(meshes)
+        assert n == len(options) and n > 0
+        self.meshes = (MeshDesc * n)(*meshes)
+        self.opts = (Options * n)(*options)
+        h = C.c_void_p()
+        r = self.L.optional["batch_create"](n, self.meshes, self.opts, C.byref(h))
+        if r != OK:
+            raise MahycoError(r, "batch_create")
+        self.h = h
+        self.n = n
+        self.problems = [Simulation(library, self.meshes[i], self.opts[i],
+                                    ctx=C.c_void_p(self.L.optional["batch_problem"](h, i)))
+                         for i in range(n)]
+
+    def _check(self, r, what):
+        if r < 0:
+            msg = self.L.optional["batch_last_error"](self.h)
+            raise MahycoError(r, f"{what}: {msg.decode() if msg else ''}")
+        return r
+
+    def problem(self, i):
+        return self.problems[i]
+
+    def run(self, max_steps=1 << 30):
+        """Returns (problems still running, zone-cycles done in this call)."""
+        nr = C.c_int32()
+        zc = C.c_int64()
+        self._check(self.L.optional["batch_run"](self.h, max_steps, C.byref(nr), C.byref(zc)), "batch_run")
+        return nr.value, zc.value
+
+    def status(self, i):
+        it, st = C.c_int32(), C.c_int32()
+        t, dt = C.c_double(), C.c_double()
+        self._check(self.L.optional["batch_status"](self.h, i, C.byref(it), C.byref(t), C.byref(dt),
+                                                    C.byref(st)), "batch_status")
+        return {"iterations": it.value, "time": t.value, "dt": dt.value, "state": st.value}
+
+    def autotune(self, reps=2):
+        buf = C.create_string_buffer(8192)
+        self._check(self.L.optional["batch_autotune"](self.h, reps, buf, 8192), "batch_autotune")
+        return buf.value.decode()
+
+    def close(self):
+        if getattr(self, "h", None):
+            for p in self.problems:
+                p.ctx = None
+            self.L.optional["batch_destroy"](self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

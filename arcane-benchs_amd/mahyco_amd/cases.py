@@ -7,6 +7,8 @@
   (1, 1, 1.5) for x<1, (1, 0.1, 1.4) for x>1, y<1.5, (0.125, 0.1, 1.5) for
   x>1, y>1.5, at rest, reflecting walls, cfl 0.3, run for a fixed number of
   steps (final-time and deltat-max out of reach).
+* Riemann batch: independent 2D x-Riemann problems of uneven sizes, each
+  run to its own final time (batching form).
 """
 import numpy as np
 
@@ -88,3 +90,68 @@ def p3d(lib, n=256, fields=None, **optkw):
     s = abi.Simulation(lib, abi.mesh_desc(3, [n, n, n], [0, 0, 0], [1.0, 1.0, 1.0]), opt)
     s.init_cells(np.zeros(rho.size, dtype=np.int32), rho, np.ones(rho.size), vel)
     return s
+
+
+# Batch of 2D Riemann problems of the batching form: nx ~ U{64..1024},
+# ny ~ U{8..64} on [0, 1] x [0, ny/nx] (square cells); left/right density
+# log-uniform in [0.1, 10], pressure log-uniform in [0.01, 10], x velocity
+# uniform in [-1, 1]; interface x0 ~ U[0.3, 0.7]; one perfect gas with gamma
+# ~ U[1.2, 1.67]; reflecting walls; each problem runs to t_end = 0.15.
+BATCH_SEED = 20261204
+BATCH_TEND = 0.15
+
+
+def riemann_batch_specs(n=1024, seed=BATCH_SEED, nx_range=(64, 1024), ny_range=(8, 64)):
+    """Seeded problem parameters, one dict per problem."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for _ in range(n):
+        nx = int(rng.integers(nx_range[0], nx_range[1] + 1))
+        ny = int(rng.integers(ny_range[0], ny_range[1] + 1))
+        rho = np.exp(rng.uniform(np.log(0.1), np.log(10.0), 2))
+        p = np.exp(rng.uniform(np.log(0.01), np.log(10.0), 2))
+        u = rng.uniform(-1.0, 1.0, 2)
+        specs.append(dict(nx=nx, ny=ny, rho=(float(rho[0]), float(rho[1])), p=(float(p[0]), float(p[1])),
+                          u=(float(u[0]), float(u[1])), x0=float(rng.uniform(0.3, 0.7)),
+                          gamma=float(rng.uniform(1.2, 1.67))))
+    return specs
+
+
+def riemann_mesh_options(lib, spec, t_end=BATCH_TEND, **optkw):
+    opt = lib.options(**optkw)
+    abi.set_bcs(opt, abi.reflecting_walls(2))
+    abi.set_envs(opt, [(abi.EOS_PERFECT_GAS, spec["gamma"], 0.)])
+    opt.final_time = t_end
+    m = abi.mesh_desc(2, [spec["nx"], spec["ny"]], [0, 0], [1.0, spec["ny"] / spec["nx"]])
+    return m, opt
+
+
+def riemann_fields(spec):
+    """Cell env/density/pressure (i fastest) and nodal velocity (2, N_n):
+    left state for cell centres and nodes with x < x0."""
+    nx, ny = spec["nx"], spec["ny"]
+    xc = (np.arange(nx) + 0.5) / nx
+    left = np.tile(xc < spec["x0"], ny)
+    rho = np.where(left, spec["rho"][0], spec["rho"][1])
+    p = np.where(left, spec["p"][0], spec["p"][1])
+    xn = np.arange(nx + 1) / nx
+    ux = np.tile(np.where(xn < spec["x0"], spec["u"][0], spec["u"][1]), ny + 1)
+    vel = np.stack([ux, np.zeros_like(ux)])
+    return np.zeros(nx * ny, dtype=np.int32), rho, p, vel
+
+
+def riemann_problem(lib, spec, t_end=BATCH_TEND, **optkw):
+    """One problem of the batch as a context of its own."""
+    m, opt = riemann_mesh_options(lib, spec, t_end, **optkw)
+    s = abi.Simulation(lib, m, opt)
+    s.init_cells(*riemann_fields(spec))
+    return s
+
+
+def riemann_batch(lib, specs, t_end=BATCH_TEND, **optkw):
+    """The problems of `specs` as one batch (mhg_batch_*), initialised."""
+    mo = [riemann_mesh_options(lib, sp, t_end, **optkw) for sp in specs]
+    b = abi.Batch(lib, [m for m, _ in mo], [o for _, o in mo])
+    for i, sp in enumerate(specs):
+        b.problem(i).init_cells(*riemann_fields(sp))
+    return b

@@ -213,6 +213,39 @@ int32_t mhg_variant_count(const char* kernel);
  * (MHG_E_ARG for an unknown kernel or an index out of range). */
 int mhg_select_variant(mhg_ctx* ctx, const char* kernel, int32_t variant);
 
+/* ---- batching form (extension, no counterpart in the reference) ------- */
+/* n independent 2D problems of uneven sizes advanced together.  Each problem
+ * runs the fused iteration of mhg_step on its own mesh with its own options
+ * (final time, cfl, dt bounds, EOS constants, boundary conditions), dt, time,
+ * iteration count and stop test, bit for bit as a context of its own would.
+ * One launch of each persistent kernel covers every running problem; the
+ * iterations are replayed from a captured HIP graph; problems that stop drop
+ * out of the kernels' work lists.  All problems share dim = 2, the precision,
+ * nb_env (at most 3), projection_limiteur_id, ordre_projection and
+ * pseudo_centree. */
+typedef struct mhg_batch mhg_batch;
+int mhg_batch_create(int32_t n, const mhg_mesh_desc* meshes, const mhg_options* opts, mhg_batch** out);
+int mhg_batch_destroy(mhg_batch* b);
+const char* mhg_batch_last_error(const mhg_batch* b);
+/* The context of problem i, owned by the batch.  mhg_init_*, mhg_upload_state
+ * (before the first mhg_batch_run), mhg_download_state, mhg_sizes and
+ * mhg_conservation work on it; the stepping and tuning calls return
+ * MHG_E_STATE. */
+mhg_ctx* mhg_batch_problem(mhg_batch* b, int32_t i);
+/* Advances every running problem until it stops, or for at most max_steps
+ * iterations of the batch.  *n_running receives the problems still running,
+ * *zone_cycles the cells x iterations done by all problems in this call.
+ * Returns MHG_STOP when no problem is left running, MHG_E_DT_MIN if a problem
+ * stopped on deltat-min (mhg_batch_status names it). */
+int mhg_batch_run(mhg_batch* b, int32_t max_steps, int32_t* n_running, int64_t* zone_cycles);
+/* Problem i: iterations done, time, dt, and 0 running / 1 stopped at the end
+ * of its run / 2 stopped on deltat-min. */
+int mhg_batch_status(mhg_batch* b, int32_t i, int32_t* iterations, double* time, double* dt, int32_t* state);
+/* Times the launch shapes of the batch's sweep kernels (persistent grid size;
+ * for the y sweeps also the column-segment or 64-row tile form) on the
+ * batch's meshes and keeps the fastest; the simulation state is unchanged. */
+int mhg_batch_autotune(mhg_batch* b, int32_t reps, char* report, int32_t report_len);
+
 /* ---- diagnostics ---------------------------------------------------- */
 /* Compensated global sums: total mass, total internal energy
  * (sum m*e), total kinetic energy (sum 1/2 m_n |u|^2). */

@@ -44,7 +44,7 @@ def test_oracle_exports_same_interface():
     have = exported(H.ORACLE_LIB)
     gpu_only = ("mho_run_async", "mho_synchronize", "mho_kernel_times", "mho_autotune",
                 "mho_variant_count", "mho_select_variant")
-    missing = [n for n in names if n not in have and n not in gpu_only]
+    missing = [n for n in names if n not in have and n not in gpu_only and not n.startswith("mho_batch_")]
     assert not missing, missing
 
 
@@ -55,3 +55,26 @@ def test_options_struct_matches_header():
     o = abi.Options()
     lib.default_options(C.byref(o))
     assert o.remap_threshold == 1e-10 and o.is_euler_scheme == 1 and o.visc_quad_coeff == -1.0
+
+
+def test_batch_create_rejects_mixed_problems():
+    """mhg_batch_create validates every problem before any device call."""
+    import ctypes as C
+    lib = abi.Library(GPU_LIB, "mhg_")
+    create = lib.optional["batch_create"]
+
+    def make(n, dims=(2, 2), envs=(1, 1), prec=(0, 0)):
+        meshes = (abi.MeshDesc * 2)(*[abi.mesh_desc(d, [8, 4, 4], [0, 0, 0], [1., .5, .5]) for d in dims])
+        opts = (abi.Options * 2)()
+        for i in range(2):
+            opts[i] = lib.options(precision=prec[i])
+            abi.set_bcs(opts[i], abi.reflecting_walls(dims[i]))
+            abi.set_envs(opts[i], [(abi.EOS_PERFECT_GAS, 1.4, 0.)] * envs[i])
+        h = C.c_void_p()
+        return create(n, meshes, opts, C.byref(h))
+
+    assert make(0) == abi.E_ARG
+    assert make(2, dims=(2, 3)) == abi.E_UNSUPPORTED
+    assert make(2, envs=(1, 2)) == abi.E_UNSUPPORTED
+    assert make(2, envs=(4, 4)) == abi.E_UNSUPPORTED
+    assert make(2, prec=(0, 1)) == abi.E_UNSUPPORTED
