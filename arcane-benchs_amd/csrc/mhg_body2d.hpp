@@ -206,12 +206,12 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
     lf = fabs(Xt[nb] - Xt[na]);
   }
   const bool inner = ex && s >= 1;
-  const double xcm = __shfl_up(xc, 1, 64);
+  const double xcm = lane_up1(xc);
   const double dx = inner ? xc - xcm : 1.;
   const bool upm = inner && ((double)uf * dx > 0.0);
   const double dotxf = inner ? (upm ? xf - xcm : xf - xc) : 0.;
-  const R uf_f = __shfl_down(uf, 1, 64);
-  const double lf_f = __shfl_down(lf, 1, 64);
+  const R uf_f = lane_down1(uf);
+  const double lf_f = lane_down1(lf);
   // face flux factors u_f L_f dt (double products, then the storage type)
   const R fb = (R)(-1.0 * uf * lf * dt), ff = (R)(1.0 * uf_f * lf_f * dt);
   const R dxr = (R)dx, dotr = (R)dotxf;
@@ -224,13 +224,13 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
     for (int k = 0; k < NEM; ++k) pre[g][k] = (ex && k < ne) ? Uin[(g * ne + k) * NC + c] : R(0);
   auto sweep_var = [&](R u0, R& unew, R& dual) {
     const R phi = ex ? qdiv(u0, den) : R(0);
-    const R phim = __shfl_up(phi, 1, 64);
+    const R phim = lane_up1(phi);
     const R gf = face_grad(inner, phi, phim, dxr);
-    const R gfp = __shfl_down(gf, 1, 64);
+    const R gfp = lane_down1(gf);
     const R g = o2 ? limited_grad<LIM, true, R, kBranchFree<R>>(gf, gfp) : R(0);
-    const R gm = __shfl_up(g, 1, 64);
+    const R gm = lane_up1(g);
     const R phif = inner ? (upm ? phim + order2 * dotr * gm : phi + order2 * dotr * g) : R(0);
-    const R phifp = __shfl_down(phif, 1, 64);
+    const R phifp = lane_down1(phif);
     const R flux_b = fb * phif;
     const R flux_f = ff * phifp;
     if (D == 0) {  // cell faces in order: bottom, right (front), top, left (back)
@@ -370,24 +370,24 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
   const bool inner = ex && s >= 1;
   const double xc = tlane ? lds[0][ta] : 0., xf = tlane ? lds[3][ta] : 0., lf = tlane ? lds[4][ta] : 0.;
   const R den = tlane ? (R)lds[1][ta] : R(1), uf = tlane ? (R)lds[2][ta] : R(0);
-  const double xcm = __shfl_up(xc, 1, 64);
+  const double xcm = lane_up1(xc);
   const double dx = inner ? xc - xcm : 1.;
   const bool upm = inner && ((double)uf * dx > 0.0);
   const double dotxf = inner ? (upm ? xf - xcm : xf - xc) : 0.;
-  const R uf_f = __shfl_down(uf, 1, 64);
-  const double lf_f = __shfl_down(lf, 1, 64);
+  const R uf_f = lane_down1(uf);
+  const double lf_f = lane_down1(lf);
   const R fb = (R)(-1.0 * uf * lf * dt), ff = (R)(1.0 * uf_f * lf_f * dt);
   const R dxr = (R)dx, dotr = (R)dotxf;
   // one variable on the transposed layout: new U and half the dual flux
   auto sweep_var = [&](R u0, R& un, R& dual) {
     const R phi = ex ? qdiv(u0, den) : R(0);
-    const R phim = __shfl_up(phi, 1, 64);
+    const R phim = lane_up1(phi);
     const R gf = face_grad(inner, phi, phim, dxr);
-    const R gfp = __shfl_down(gf, 1, 64);
+    const R gfp = lane_down1(gf);
     const R g = o2 ? limited_grad<LIM, true, R, kBranchFree<R>>(gf, gfp) : R(0);
-    const R gm = __shfl_up(g, 1, 64);
+    const R gm = lane_up1(g);
     const R phif = inner ? (upm ? phim + order2 * dotr * gm : phi + order2 * dotr * g) : R(0);
-    const R phifp = __shfl_down(phif, 1, 64);
+    const R phifp = lane_down1(phif);
     const R flux_b = fb * phif;
     const R flux_f = ff * phifp;
     // cell faces in order: bottom (back), right, top (front), left
@@ -592,10 +592,10 @@ __device__ __forceinline__ void node_sweep_x_2d_body(const Params& P, DevScalars
       }
     }
   }
-  const double xdm = __shfl_up(xd, 1, 64), xdp = __shfl_down(xd, 1, 64);
-  const R p0m = __shfl_up(p0, 1, 64), p0p = __shfl_down(p0, 1, 64);
-  const R p1m = __shfl_up(p1, 1, 64), p1p = __shfl_down(p1, 1, 64);
-  const R back = __shfl_up(front, 1, 64);
+  const double xdm = lane_up1(xd), xdp = lane_down1(xd);
+  const R p0m = lane_up1(p0), p0p = lane_down1(p0);
+  const R p1m = lane_up1(p1), p1p = lane_down1(p1);
+  const R back = lane_up1(front);
   // limited nodal gradient (computeDualGradPhi_LimC, DualPhaseRemap.cc:700-779)
   R g0 = R(0), g1 = R(0);
   if (P.order2 > 0 && ex) {
@@ -614,8 +614,8 @@ __device__ __forceinline__ void node_sweep_x_2d_body(const Params& P, DevScalars
       g1 = A.dg1[nid];
     }
   }
-  const R g0m = __shfl_up(g0, 1, 64), g0p = __shfl_down(g0, 1, 64);
-  const R g1m = __shfl_up(g1, 1, 64), g1p = __shfl_down(g1, 1, 64);
+  const R g0m = lane_up1(g0), g0p = lane_down1(g0);
+  const R g1m = lane_up1(g1), g1p = lane_down1(g1);
   if (!out) return;
   if (inner_d) {
     U3 += back - front;
@@ -746,10 +746,10 @@ __device__ __forceinline__ void node_sweep_ytile_2d_body(const Params& P, DevSca
   const double xd = tlane ? ldx[ta] : 0.;
   const R p0 = tlane ? lds[0][ta] : R(0), p1 = tlane ? lds[1][ta] : R(0);
   const R front = tlane ? lds[2][ta] : R(0);
-  const double xdm = __shfl_up(xd, 1, 64), xdp = __shfl_down(xd, 1, 64);
-  const R p0m = __shfl_up(p0, 1, 64), p0p = __shfl_down(p0, 1, 64);
-  const R p1m = __shfl_up(p1, 1, 64), p1p = __shfl_down(p1, 1, 64);
-  const R back = __shfl_up(front, 1, 64);
+  const double xdm = lane_up1(xd), xdp = lane_down1(xd);
+  const R p0m = lane_up1(p0), p0p = lane_down1(p0);
+  const R p1m = lane_up1(p1), p1p = lane_down1(p1);
+  const R back = lane_up1(front);
   R g0 = R(0), g1 = R(0);
   if (P.order2 > 0 && ex) {
     const long q = i + (long)nnx * s;
@@ -768,8 +768,8 @@ __device__ __forceinline__ void node_sweep_ytile_2d_body(const Params& P, DevSca
       g1 = A.dg1[q];
     }
   }
-  const R g0m = __shfl_up(g0, 1, 64), g0p = __shfl_down(g0, 1, 64);
-  const R g1m = __shfl_up(g1, 1, 64), g1p = __shfl_down(g1, 1, 64);
+  const R g0m = lane_up1(g0), g0p = lane_down1(g0);
+  const R g1m = lane_up1(g1), g1p = lane_down1(g1);
   R d0 = R(0), d1 = R(0), d3 = R(0);
   if (out && inner_d) {
     const R order2 = (R)P.order2;

@@ -317,6 +317,37 @@ __device__ inline void st_stream(T* p, T v) {
   __builtin_nontemporal_store(v, p);
 }
 
+// Neighbour-lane reads within a 64-lane wavefront through the DPP whole-wave
+// shifts of gfx9-family VALU instructions (wave_shr:1 = 0x138, wave_shl:1 =
+// 0x130) instead of an LDS permute.  lane_up1: lane i receives lane i - 1;
+// lane_down1: lane i receives lane i + 1; a lane whose source lies outside
+// the wavefront keeps its own value (the convention of __shfl_up/__shfl_down).
+// 64-bit values move as two 32-bit halves.
+constexpr int kDppWaveShr1 = 0x138, kDppWaveShl1 = 0x130;
+
+template <int CTRL>
+__device__ inline int dpp_shift(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ inline float dpp_shift(float v) {
+  return __int_as_float(dpp_shift<CTRL>(__float_as_int(v)));
+}
+template <int CTRL>
+__device__ inline double dpp_shift(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_shift<CTRL>((int)(b & 0xffffffffLL)), hi = dpp_shift<CTRL>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <class T>
+__device__ inline T lane_up1(T v) {
+  return dpp_shift<kDppWaveShr1>(v);
+}
+template <class T>
+__device__ inline T lane_down1(T v) {
+  return dpp_shift<kDppWaveShl1>(v);
+}
+
 // XCD-aware block order.  The hardware hands block b to XCD b % 8; this maps
 // it to logical block (b % 8) * (n / 8) + b / 8 so that every XCD walks a
 // contiguous band of logical blocks and neighbouring tiles share its L2.

@@ -235,24 +235,24 @@ __global__ __launch_bounds__(256) void k_cell_sweep_x_3d(Params P, DevScalars* S
     }
   }
   const bool inner = ex && i >= 1;
-  const double xcm = __shfl_up(xc, 1, 64);
+  const double xcm = lane_up1(xc);
   const double dx = inner ? xc - xcm : 1.;
   const bool upm = inner && ((double)uf * dx > 0.0);
   const double dotxf = inner ? (upm ? xf - xcm : xf - xc) : 0.;
-  const R uf_f = __shfl_down(uf, 1, 64);
-  const double lf_f = __shfl_down(lf, 1, 64);
+  const R uf_f = lane_down1(uf);
+  const double lf_f = lane_down1(lf);
   const R fb = (R)(-1.0 * uf * lf * dt), ff = (R)(1.0 * uf_f * lf_f * dt);
   const R dxr = (R)dx, dotr = (R)dotxf;
   const bool o2 = P.order2 > 0;
   auto sweep_var = [&](R u0, R& un, R& dual) {
     const R phi = ex ? qdiv(u0, den) : R(0);
-    const R phim = __shfl_up(phi, 1, 64);
+    const R phim = lane_up1(phi);
     const R gf = face_grad(inner, phi, phim, dxr);
-    const R gfp = __shfl_down(gf, 1, 64);
+    const R gfp = lane_down1(gf);
     const R g = o2 ? limited_grad<LIM, true, R, kBranchFree<R>>(gf, gfp) : R(0);
-    const R gm = __shfl_up(g, 1, 64);
+    const R gm = lane_up1(g);
     const R phif = inner ? (upm ? phim + order2 * dotr * gm : phi + order2 * dotr * g) : R(0);
-    const R phifp = __shfl_down(phif, 1, 64);
+    const R phifp = lane_down1(phif);
     const R flux_b = fb * phif, flux_f = ff * phifp;
     R fsum = R(0);  // faces in hexa8 order: back (x-) before front (x+)
     fsum = fsum + flux_b;
@@ -350,23 +350,23 @@ __global__ __launch_bounds__(1024) void k_cell_sweep_tile_3d(Params P, DevScalar
   const bool inner = ex && s >= 1;
   const double xc = lds[0][lane][w], xf = lds[3][lane][w], lf = lds[4][lane][w];
   const R den = (R)lds[1][lane][w], uf = (R)lds[2][lane][w];
-  const double xcm = __shfl_up(xc, 1, 64);
+  const double xcm = lane_up1(xc);
   const double dx = inner ? xc - xcm : 1.;
   const bool upm = inner && ((double)uf * dx > 0.0);
   const double dotxf = inner ? (upm ? xf - xcm : xf - xc) : 0.;
-  const R uf_f = __shfl_down(uf, 1, 64);
-  const double lf_f = __shfl_down(lf, 1, 64);
+  const R uf_f = lane_down1(uf);
+  const double lf_f = lane_down1(lf);
   const R fb = (R)(-1.0 * uf * lf * dt), ff = (R)(1.0 * uf_f * lf_f * dt);
   const R dxr = (R)dx, dotr = (R)dotxf;
   auto sweep_var = [&](R u0, R& un, R& dual) {
     const R phi = ex ? qdiv(u0, den) : R(0);
-    const R phim = __shfl_up(phi, 1, 64);
+    const R phim = lane_up1(phi);
     const R gf = face_grad(inner, phi, phim, dxr);
-    const R gfp = __shfl_down(gf, 1, 64);
+    const R gfp = lane_down1(gf);
     const R g = o2 ? limited_grad<LIM, true, R, kBranchFree<R>>(gf, gfp) : R(0);
-    const R gm = __shfl_up(g, 1, 64);
+    const R gm = lane_up1(g);
     const R phif = inner ? (upm ? phim + order2 * dotr * gm : phi + order2 * dotr * g) : R(0);
-    const R phifp = __shfl_down(phif, 1, 64);
+    const R phifp = lane_down1(phif);
     const R flux_b = fb * phif, flux_f = ff * phifp;
     R fsum = R(0);  // faces in hexa8 order: back before front
     fsum = fsum + flux_b;
@@ -566,13 +566,13 @@ __global__ __launch_bounds__(256) void k_node_sweep_x_3d(Params P, DevScalars* S
   const DualNode<R> nd = dual_node_load<R, 0, FIRST, NEM>(P, A, i, j, k, ex);
   const long nid = ex ? nid3(P, i, j, k) : 0;
   R pm[3], pp[3], g0[3] = {R(0), R(0), R(0)}, gm[3], gp[3];
-  const double xdm = __shfl_up(nd.xd, 1, 64), xdp = __shfl_down(nd.xd, 1, 64);
+  const double xdm = lane_up1(nd.xd), xdp = lane_down1(nd.xd);
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    pm[q] = __shfl_up(nd.p[q], 1, 64);
-    pp[q] = __shfl_down(nd.p[q], 1, 64);
+    pm[q] = lane_up1(nd.p[q]);
+    pp[q] = lane_down1(nd.p[q]);
   }
-  const R back = __shfl_up(nd.front, 1, 64);
+  const R back = lane_up1(nd.front);
   R* dg[3] = {A.dg0, A.dg1, A.dg2};
   if (P.order2 > 0 && ex) {
     if (inner_d) {
@@ -590,8 +590,8 @@ __global__ __launch_bounds__(256) void k_node_sweep_x_3d(Params P, DevScalars* S
   }
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    gm[q] = __shfl_up(g0[q], 1, 64);
-    gp[q] = __shfl_down(g0[q], 1, 64);
+    gm[q] = lane_up1(g0[q]);
+    gp[q] = lane_down1(g0[q]);
   }
   if (!out) return;
   R U[4] = {nd.U[0], nd.U[1], nd.U[2], nd.U[3]};
@@ -656,14 +656,14 @@ __global__ __launch_bounds__(1024) void k_node_sweep_tile_3d(Params P, DevScalar
   const double xd = ldx[lane][w];
   R p0[3] = {lds[0][lane][w], lds[1][lane][w], lds[2][lane][w]};
   const R front = lds[3][lane][w];
-  const double xdm = __shfl_up(xd, 1, 64), xdp = __shfl_down(xd, 1, 64);
+  const double xdm = lane_up1(xd), xdp = lane_down1(xd);
   R pm[3], pp[3], g0[3] = {R(0), R(0), R(0)}, gm[3], gp[3];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    pm[q] = __shfl_up(p0[q], 1, 64);
-    pp[q] = __shfl_down(p0[q], 1, 64);
+    pm[q] = lane_up1(p0[q]);
+    pp[q] = lane_down1(p0[q]);
   }
-  const R back = __shfl_up(front, 1, 64);
+  const R back = lane_up1(front);
   R* dg[3] = {A.dg0, A.dg1, A.dg2};
   if (P.order2 > 0 && ex) {
     const long q0 = nid3(P, ti, tj, tk);
@@ -682,8 +682,8 @@ __global__ __launch_bounds__(1024) void k_node_sweep_tile_3d(Params P, DevScalar
   }
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    gm[q] = __shfl_up(g0[q], 1, 64);
-    gp[q] = __shfl_down(g0[q], 1, 64);
+    gm[q] = lane_up1(g0[q]);
+    gp[q] = lane_down1(g0[q]);
   }
   R dU[4] = {R(0), R(0), R(0), R(0)};
   if (out && inner_d) dual_increments<R, D>(P, dt, pm, p0, pp, xdm, xd, xdp, gm, g0, gp, back, front, dU);

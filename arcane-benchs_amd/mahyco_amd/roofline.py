@@ -29,6 +29,19 @@ def bytes_per_zone_cycle(n_env, dim=2, word=8):
     return words_per_zone(n_env, dim) * word
 
 
+# Coordinate words of the per-zone count (kept in FP64 by the precision form):
+# 2D: L-node 6 (previous Lagrangian x, x0, new x), L-cell 2, cell sweeps 2 x 2,
+# dual sweeps 2 x 1; 3D: 24.
+COORD_WORDS = {2: 14, 3: 24}
+
+
+def bytes_per_zone_cycle_fp32(n_env, dim=2):
+    """Precision form: coordinate words at 8 bytes, every other word at 4
+    (556 B on the triple point, 412 B for one material in 2D, 624 B in 3D)."""
+    c = COORD_WORDS[dim]
+    return c * 8 + (words_per_zone(n_env, dim) - c) * 4
+
+
 def kernel_bytes(name, n_cells, n_nodes, n_env, word=8):
     """Algorithmic bytes of one launch of a library kernel."""
     w = {

@@ -142,6 +142,51 @@ def p3d_run(gpu, args, sync):
     return out
 
 
+def batch_run(gpu, args, sync):
+    """Batching form: the seeded batch of 2D Riemann problems, every problem to
+    its own t_end, in FP32 bulk and in FP64.  Throughput = cells x iterations
+    summed over the problems / wall time of the whole run (one mhg_batch_run
+    call, host list updates included); the launch shapes are autotuned first
+    (state restored).  The FP32 densities are compared with the FP64 batch's,
+    over all cells of all problems."""
+    specs = cases.riemann_batch_specs(args.batch_n)
+    out = {"config": f"{args.batch_n} 2D x-Riemann problems, nx ~ U{{64..1024}}, ny ~ U{{8..64}}, "
+                     f"rho ~ logU[0.1,10], p ~ logU[0.01,10], u ~ U[-1,1], x0 ~ U[0.3,0.7], "
+                     f"gamma ~ U[1.2,1.67], reflecting walls, each to t_end = {cases.BATCH_TEND}, "
+                     f"seed {cases.BATCH_SEED}",
+           "cells": int(sum(sp["nx"] * sp["ny"] for sp in specs))}
+    dens = {}
+    for name, prec in [("fp32", abi.PREC_FP32_BULK), ("fp64", abi.PREC_FP64)]:
+        t0 = time.perf_counter()
+        b = cases.riemann_batch(gpu, specs, precision=prec)
+        t_init = time.perf_counter() - t0
+        tuning = b.autotune(2)
+        print(f"batch {name} autotune:", tuning, file=sys.stderr)
+        sync()
+        t0 = time.perf_counter()
+        nrun, zc = b.run()
+        sync()
+        el = time.perf_counter() - t0
+        assert nrun == 0
+        its = np.array([b.status(i)["iterations"] for i in range(b.n)])
+        dens[name] = np.concatenate([b.problem(i).download(["density"])["density"] for i in range(b.n)])
+        assert np.all(np.isfinite(dens[name]))
+        v = zc / el
+        bpz = (roofline.bytes_per_zone_cycle_fp32(1) if prec == abi.PREC_FP32_BULK
+               else roofline.bytes_per_zone_cycle(1))
+        out[name] = {"value": v, "unit": "zone-cycles/s", "wall_s": round(el, 4), "zone_cycles": int(zc),
+                     "iterations_max": int(its.max()), "iterations_mean": round(float(its.mean()), 1),
+                     "init_s": round(t_init, 2), "bytes_per_zone_cycle": bpz,
+                     "roofline_frac": round(v * bpz / (roofline.PEAK_HBM_GBS * 1e9), 4),
+                     "tuning": tuning.strip()}
+        b.close()
+    out["speedup_fp32_vs_fp64"] = out["fp32"]["value"] / out["fp64"]["value"]
+    a, r = dens["fp32"], dens["fp64"]
+    out["l1_density_err_fp32_vs_fp64"] = float(np.sum(np.abs(a - r)) / np.sum(np.abs(r)))
+    out["linf_density_err_fp32_vs_fp64"] = float(np.max(np.abs(a - r)) / np.max(np.abs(r)))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +200,8 @@ def main():
     ap.add_argument("--p3d-n", type=int, default=256, help="cells per side of the 3D random-field case")
     ap.add_argument("--p3d-steps", type=int, default=200)
     ap.add_argument("--no-p3d", action="store_true", help="skip the 3D random-field case")
+    ap.add_argument("--batch-n", type=int, default=1024, help="problems of the Riemann batch")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batching form")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device with torch: the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counts to bytes")
@@ -198,6 +245,9 @@ def main():
     nc, nn = sim.nc, sim.nn
     value = nc * args.steps / elapsed
     p3 = None if args.no_p3d else p3d_run(gpu, args, sync)
+    pb = None if args.no_batch else batch_run(gpu, args, sync)
+    if pb is not None:
+        pb["fp32_vs_triple_point_fp64"] = pb["fp32"]["value"] / value
 
     rl = None
     if kt:
@@ -243,6 +293,7 @@ def main():
         "cpu_baseline": cpu,
         "precision_form": p32,
         "precision_form_3d": p3,
+        "batch_form": pb,
     }
     print(json.dumps(out))
 
