@@ -167,7 +167,7 @@ void enqueue_batch_step_t(mhg_batch* b) {
   launch_batch_cell_lagrange_2d<R>(st, b->dprobs, b->S, cell_lag_args<R>(c), b->L);
   for (int pass = 0; pass < 2; ++pass) {
     const int idir = (pass + c->sens) % 2;
-    launch_batch_cell_sweep_2d<R>(st, b->dprobs, b->S, cell_sweep_args<R>(c, pass), b->L, idir, pass == 0);
+    launch_batch_cell_sweep_2d<R>(st, b->dprobs, b->S, cell_sweep_args<R>(c, pass, idir), b->L, idir, pass == 0);
     launch_batch_node_sweep_2d<R>(st, b->dprobs, b->S, node_sweep_args<R>(c), b->L, idir, pass == 0);
   }
   launch_batch_cell_final_2d<R>(st, b->dprobs, b->S, cell_final_args<R>(c), b->L);
@@ -282,6 +282,7 @@ int mhg_batch_create(int32_t n, const mhg_mesh_desc* meshes, const mhg_options* 
     const mhg_options& o0 = opts[0];
     const mhg_options& o = opts[p];
     if (r || o.nb_env > 3 || o.nb_env != o0.nb_env || o.precision != o0.precision ||
+        o.projection_limiteur_id == MHG_LIM_LEARNED ||
         o.projection_limiteur_id != o0.projection_limiteur_id || o.ordre_projection != o0.ordre_projection ||
         o.pseudo_centree != o0.pseudo_centree) {
       delete b;

@@ -145,7 +145,9 @@ __device__ __forceinline__ void cell_lagrange_2d_body(const Params& P, DevScalar
 // neighbours move between lanes.  D = 0: a wave is a row segment.  D = 1: a
 // wave is a column segment and a block of 16 waves covers 16 adjacent
 // columns, so each cache line of a row is consumed by the whole block.
-template <class R, int D, int NEM, bool FIRST, int LIM>
+// LEARN: the cell slopes of the learned limiter (learned_grad) with the
+// cell's coefficient A.alpha[c]; LIM is then minmod, the fallback.
+template <class R, int D, int NEM, bool FIRST, int LIM, bool LEARN = false>
 __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScalars* S, const CellSweepArgsT<R>& A,
                                                         int blk) {
   if (S->stop || S->err) return;
@@ -216,6 +218,15 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
   const R fb = (R)(-1.0 * uf * lf * dt), ff = (R)(1.0 * uf_f * lf_f * dt);
   const R dxr = (R)dx, dotr = (R)dotxf;
   const bool o2 = P.order2 > 0;
+  // learned limiter: coefficient, front-neighbour existence, face offsets
+  R alpha = R(0), dbr = R(0), dfr = R(0);
+  bool front_in = false;
+  if (LEARN) {
+    alpha = ex ? A.alpha[c] : R(0);
+    front_in = lane_down1((int)inner) != 0;
+    dbr = (R)(xf - xc);
+    dfr = (R)(lane_down1(xf) - xc);
+  }
   R pre[3][NEM];
   const R preq = ex ? Uin[(3 * ne) * NC + c] : R(0);
 #pragma unroll
@@ -227,7 +238,11 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
     const R phim = lane_up1(phi);
     const R gf = face_grad(inner, phi, phim, dxr);
     const R gfp = lane_down1(gf);
-    const R g = o2 ? limited_grad<LIM, true, R, kBranchFree<R>>(gf, gfp) : R(0);
+    R g = R(0);
+    if (o2) {
+      if (LEARN) g = learned_grad(alpha, phim, phi, lane_down1(phi), inner, front_in, gf, gfp, dbr, dfr);
+      else g = limited_grad<LIM, true, R, kBranchFree<R>>(gf, gfp);
+    }
     const R gm = lane_up1(g);
     const R phif = inner ? (upm ? phim + order2 * dotr * gm : phi + order2 * dotr * g) : R(0);
     const R phifp = lane_down1(phif);
@@ -297,7 +312,7 @@ struct YTileShape {
   }
 };
 
-template <class R, int NEM, bool FIRST, int LIM, bool SEG>
+template <class R, int NEM, bool FIRST, int LIM, bool SEG, bool LEARN = false>
 __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevScalars* S,
                                                          const CellSweepArgsT<R>& A, int L) {
   // geometry tiles [0..4]; once read, tiles [0..5] are reused as the staging
@@ -378,13 +393,26 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
   const double lf_f = lane_down1(lf);
   const R fb = (R)(-1.0 * uf * lf * dt), ff = (R)(1.0 * uf_f * lf_f * dt);
   const R dxr = (R)dx, dotr = (R)dotxf;
+  // learned limiter: coefficient, front-neighbour existence, face offsets
+  R alpha = R(0), dbr = R(0), dfr = R(0);
+  bool front_in = false;
+  if (LEARN) {
+    alpha = ex ? A.alpha[(i0 + col) + (long)nx * s] : R(0);
+    front_in = lane_down1((int)inner) != 0;
+    dbr = (R)(xf - xc);
+    dfr = (R)(lane_down1(xf) - xc);
+  }
   // one variable on the transposed layout: new U and half the dual flux
   auto sweep_var = [&](R u0, R& un, R& dual) {
     const R phi = ex ? qdiv(u0, den) : R(0);
     const R phim = lane_up1(phi);
     const R gf = face_grad(inner, phi, phim, dxr);
     const R gfp = lane_down1(gf);
-    const R g = o2 ? limited_grad<LIM, true, R, kBranchFree<R>>(gf, gfp) : R(0);
+    R g = R(0);
+    if (o2) {
+      if (LEARN) g = learned_grad(alpha, phim, phi, lane_down1(phi), inner, front_in, gf, gfp, dbr, dfr);
+      else g = limited_grad<LIM, true, R, kBranchFree<R>>(gf, gfp);
+    }
     const R gm = lane_up1(g);
     const R phif = inner ? (upm ? phim + order2 * dotr * gm : phi + order2 * dotr * g) : R(0);
     const R phifp = lane_down1(phif);

@@ -378,6 +378,7 @@ struct Params {
   EosParams eos;
   double threshold, remap_threshold;
   int order2, lim, pseudo_centree, lc;
+  int learned;  // cell sweeps use the learned-limiter slope (lim is then minmod)
   double cfl, dtmin, dtmax, tend;
   int nbc;
   BcRange bc[kMaxBc];

@@ -52,6 +52,8 @@ struct mhg_ctx {
   void *Ud2 = nullptr, *dg2 = nullptr;
   void* UdB[4] = {nullptr, nullptr, nullptr, nullptr};
   double* Veuler = nullptr;
+  // learned limiter: coefficients [2][nc] in the storage type (MHG_LIM_LEARNED)
+  void* alpha = nullptr;
   double* block_min = nullptr;
   mhg::DevScalars* S = nullptr;
   mhg::DevScalars* S_host = nullptr;  // pinned mirror
@@ -173,9 +175,9 @@ CellLagArgsT<R> cell_lag_args(mhg_ctx* c) {
   return A;
 }
 
-// Cell sweep of remap pass `pass` (the passes alternate the U buffers).
+// Cell sweep of remap pass `pass` along idir (the passes alternate the U buffers).
 template <class R>
-CellSweepArgsT<R> cell_sweep_args(mhg_ctx* c, int pass) {
+CellSweepArgsT<R> cell_sweep_args(mhg_ctx* c, int pass, int idir) {
   CellSweepArgsT<R> A;
   A.Uin = bp<R>(c->U[pass & 1]);
   A.Uout = bp<R>(c->U[(pass + 1) & 1]);
@@ -185,6 +187,7 @@ CellSweepArgsT<R> cell_sweep_args(mhg_ctx* c, int pass) {
   A.ux = bp<R>(c->ux[c->ucur]);
   A.uy = bp<R>(c->uy[c->ucur]);
   A.contrib = bp<R>(c->contrib);
+  A.alpha = c->alpha ? bp<R>(c->alpha) + (size_t)idir * c->P.nc : nullptr;
   return A;
 }
 

@@ -187,7 +187,7 @@ void remap_2d(mhg_ctx* c, int dt_mode) {
     const bool first = (pass == 0);
     {
       TimedLaunch t(c, idir == 0 ? "cell_sweep_x" : "cell_sweep_y");
-      launch_cell_sweep_2d(c->stream, P, c->S, cell_sweep_args<R>(c, pass), idir, first, c->yvariant);
+      launch_cell_sweep_2d(c->stream, P, c->S, cell_sweep_args<R>(c, pass, idir), idir, first, c->yvariant);
     }
     {
       TimedLaunch t(c, idir == 0 ? "node_sweep_x" : "node_sweep_y");
@@ -520,9 +520,11 @@ int build_params(const mhg_mesh_desc* mesh, const mhg_options* opt, Params* Pout
     return MHG_E_ARG;
   if (opt->schema_csts || opt->with_newton || opt->sans_lagrange || !opt->with_projection ||
       !opt->is_euler_scheme || opt->projection_pente_borne || opt->conservation_energie_totale ||
-      opt->projection_limiteur_id >= MHG_LIM_MINMODG || opt->projection_limiteur_id < 0 ||
-      opt->ordre_projection < 1 || opt->ordre_projection > 2)
+      (opt->projection_limiteur_id >= MHG_LIM_MINMODG && opt->projection_limiteur_id != MHG_LIM_LEARNED) ||
+      opt->projection_limiteur_id < 0 || opt->ordre_projection < 1 || opt->ordre_projection > 2)
     return MHG_E_UNSUPPORTED;
+  const bool learned = opt->projection_limiteur_id == MHG_LIM_LEARNED;
+  if (learned && mesh->dim != 2) return MHG_E_UNSUPPORTED;
   if (opt->precision != MHG_PREC_FP64 && opt->precision != MHG_PREC_FP32_BULK) return MHG_E_UNSUPPORTED;
   Params P{};
   P.dim = mesh->dim;
@@ -547,7 +549,8 @@ int build_params(const mhg_mesh_desc* mesh, const mhg_options* opt, Params* Pout
   P.threshold = opt->threshold;
   P.remap_threshold = opt->remap_threshold;
   P.order2 = opt->ordre_projection - 1;
-  P.lim = opt->projection_limiteur_id;
+  P.lim = learned ? MHG_LIM_MINMOD : opt->projection_limiteur_id;
+  P.learned = learned ? 1 : 0;
   P.pseudo_centree = opt->pseudo_centree;
   P.lc = opt->longueur_caracteristique;
   P.cfl = opt->cfl;
@@ -652,6 +655,7 @@ int mhg_create(const mhg_mesh_desc* mesh, const mhg_options* opt, mhg_ctx** out)
         r = MHG_E_HIP;
     }
   }
+  if (P.learned) r |= balloc(c, &c->alpha, 2 * (size_t)NC);  // coefficients 0 until set
   r |= dalloc(c, &c->block_min, cell_final_blocks(P));
   r |= dalloc(c, &c->S, 1);
   if (r) {
@@ -1110,6 +1114,13 @@ int mhg_run(mhg_ctx* c, int32_t max_steps, int32_t* steps_done) {
   return c->S_host->stop ? MHG_STOP : MHG_OK;
 }
 
+int mhg_set_learned_alpha(mhg_ctx* c, const double* alpha) {
+  if (!c || !alpha) return MHG_E_ARG;
+  if (!c->P.learned) return fail(c, MHG_E_UNSUPPORTED, "set_learned_alpha: the context's limiter is not MHG_LIM_LEARNED");
+  HIPCHK(c, put(c, c->alpha, alpha, 2 * (size_t)c->P.nc));
+  return MHG_OK;
+}
+
 int mhg_conservation(mhg_ctx* c, double out[3]) {
   int r = check_ready(c);
   if (r) return r;
@@ -1190,6 +1201,7 @@ int autotune_t(mhg_ctx* c, int32_t reps, char* report, int32_t report_len) {
   A.ux = bp<R>(c->ux[c->ucur]);
   A.uy = bp<R>(c->uy[c->ucur]);
   A.contrib = bp<R>(c->contrib);
+  A.alpha = c->alpha ? bp<R>(c->alpha) + P.nc : nullptr;  // the y sweep's coefficients
   NodeSweepArgsT<R> B;
   B.m = bp<R>(c->m);
   B.ux = A.ux;

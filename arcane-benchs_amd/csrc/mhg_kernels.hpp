@@ -33,6 +33,7 @@ struct CellSweepArgsT {
   const double *xnx, *xny;
   const R *ux, *uy;
   R* contrib;
+  const R* alpha;  // learned-limiter coefficients of the sweep direction (learned mode)
 };
 
 template <class R>

@@ -47,15 +47,15 @@ __global__ __launch_bounds__(256) void k_cell_lagrange_2d(Params P, DevScalars* 
   cell_lagrange_2d_body<R, PC, NEM>(P, S, A, blockIdx.x);
 }
 
-template <class R, int D, int NEM, bool FIRST, int LIM>
+template <class R, int D, int NEM, bool FIRST, int LIM, bool LEARN = false>
 __global__ __launch_bounds__(D == 0 ? 256 : 1024) void k_cell_sweep_lane_2d(Params P, DevScalars* S,
                                                                           CellSweepArgsT<R> A) {
-  cell_sweep_lane_2d_body<R, D, NEM, FIRST, LIM>(P, S, A, xcd_block(blockIdx.x, gridDim.x));
+  cell_sweep_lane_2d_body<R, D, NEM, FIRST, LIM, LEARN>(P, S, A, xcd_block(blockIdx.x, gridDim.x));
 }
 
-template <class R, int NEM, bool FIRST, int LIM>
+template <class R, int NEM, bool FIRST, int LIM, bool LEARN = false>
 __global__ __launch_bounds__(1024) void k_cell_sweep_ytile_2d(Params P, DevScalars* S, CellSweepArgsT<R> A) {
-  cell_sweep_ytile_2d_body<R, NEM, FIRST, LIM, false>(P, S, A, xcd_block(blockIdx.x, gridDim.x));
+  cell_sweep_ytile_2d_body<R, NEM, FIRST, LIM, false, LEARN>(P, S, A, xcd_block(blockIdx.x, gridDim.x));
 }
 
 template <class R, int BLOCK, int NEM>
@@ -742,6 +742,11 @@ static void launch_lane_x(hipStream_t st, const Params& P, DevScalars* S, const 
                           bool first) {
   const long waves = (long)((P.nx + 59) / 60) * P.ny;
   dim3 g(nblocks(waves * 64, 256)), b(256);
+  if (P.learned) {
+    if (first) hipLaunchKernelGGL((k_cell_sweep_lane_2d<R, 0, NEM, true, 0, true>), g, b, 0, st, P, S, A);
+    else hipLaunchKernelGGL((k_cell_sweep_lane_2d<R, 0, NEM, false, 0, true>), g, b, 0, st, P, S, A);
+    return;
+  }
 #define XL_(LM)                                                                                    \
   if (first) hipLaunchKernelGGL((k_cell_sweep_lane_2d<R, 0, NEM, true, LM>), g, b, 0, st, P, S, A); \
   else hipLaunchKernelGGL((k_cell_sweep_lane_2d<R, 0, NEM, false, LM>), g, b, 0, st, P, S, A)
@@ -753,6 +758,11 @@ template <class R, int NEM>
 static void launch_tile_y(hipStream_t st, const Params& P, DevScalars* S, const CellSweepArgsT<R>& A,
                           bool first) {
   dim3 g((unsigned)(((P.nx + kTileW - 1) / kTileW) * ((P.ny + 59) / 60))), b(1024);
+  if (P.learned) {
+    if (first) hipLaunchKernelGGL((k_cell_sweep_ytile_2d<R, NEM, true, 0, true>), g, b, 0, st, P, S, A);
+    else hipLaunchKernelGGL((k_cell_sweep_ytile_2d<R, NEM, false, 0, true>), g, b, 0, st, P, S, A);
+    return;
+  }
 #define YT_(LM)                                                                                  \
   if (first) hipLaunchKernelGGL((k_cell_sweep_ytile_2d<R, NEM, true, LM>), g, b, 0, st, P, S, A); \
   else hipLaunchKernelGGL((k_cell_sweep_ytile_2d<R, NEM, false, LM>), g, b, 0, st, P, S, A)
@@ -770,7 +780,7 @@ void launch_cell_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const 
     return;
   }
   if constexpr (std::is_same<R, double>::value) {
-    switch (variant) {
+    switch (P.learned ? -1 : variant) {  // the column walkers have no learned form
       case 0: launch_y<32, 4, 2>(st, P, S, A, first); return;
       case 1: launch_y<8, 4, 2>(st, P, S, A, first); return;
       case 2: launch_y<16, 4, 4>(st, P, S, A, first); return;

@@ -22,6 +22,28 @@ __device__ inline T face_grad(bool inner, T phi, T phim, T dx) {
   return inner ? qdiv(phi - phim, dx) : T(0);
 }
 
+// Learned-limiter slope of a cell (MHG_LIM_LEARNED): alpha (g_b + g_f) / 2,
+// kept if the face values phi + g db (back) and phi + g df (front) stay
+// within [min, max] of phi and the existing neighbours along the sweep,
+// otherwise the minmod slope.
+template <class R>
+__device__ inline R learned_grad(R alpha, R phim, R phi, R phip, bool back_in, bool front_in, R gb, R gf, R db,
+                                 R df) {
+  const R gcl = limited_grad<0, true, R, kBranchFree<R>>(gb, gf);
+  const R gc = alpha * (R(0.5) * (gb + gf));
+  R mn = phi, mx = phi;
+  if (back_in) {
+    mn = smin(mn, phim);
+    mx = smax(mx, phim);
+  }
+  if (front_in) {
+    mn = smin(mn, phip);
+    mx = smax(mx, phip);
+  }
+  const R vb = phi + gc * db, vf = phi + gc * df;
+  return (vb >= mn && vb <= mx && vf >= mn && vf <= mx) ? gc : gcl;
+}
+
 // Density, viscosity, energy and the remap vector of cell c from its
 // Lagrangian volume and characteristic length (both dimensions).
 template <class R, bool PC, int NEM>

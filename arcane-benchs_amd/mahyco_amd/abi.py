@@ -25,6 +25,7 @@ SURF = {"XMIN": 0, "XMAX": 1, "YMIN": 2, "YMAX": 3, "ZMIN": 4, "ZMAX": 5}
 EOS_PERFECT_GAS, EOS_STIFFENED_GAS = 0, 1
 LC_RACINE_CUBIQUE_VOLUME, LC_FACES_OPPOSEES = 0, 1
 PREC_FP64, PREC_FP32_BULK = 0, 1
+LIM_MINMOD, LIM_SUPERBEE, LIM_VANLEER, LIM_LEARNED = 0, 1, 2, 64
 
 
 class MeshDesc(C.Structure):
@@ -142,7 +143,8 @@ class Library:
                 ("batch_problem", vp, [vp, C.c_int32]),
                 ("batch_run", C.c_int, [vp, C.c_int32, _IP, C.POINTER(C.c_int64)]),
                 ("batch_status", C.c_int, [vp, C.c_int32, _IP, _DP, _DP, _IP]),
-                ("batch_autotune", C.c_int, [vp, C.c_int32, C.c_char_p, C.c_int32])]:
+                ("batch_autotune", C.c_int, [vp, C.c_int32, C.c_char_p, C.c_int32]),
+                ("set_learned_alpha", C.c_int, [vp, _DP])]:
             if hasattr(L, p + name):
                 self.optional[name] = fn(name, res, args)
 
@@ -331,6 +333,11 @@ class Simulation:
     def select_variant(self, variant, kernel="cell_sweep_y"):
         self._check(self.L.optional["select_variant"](self.ctx, kernel.encode(), variant),
                     "select_variant")
+
+    def set_learned_alpha(self, alpha):
+        """Fixed learned-limiter coefficients [dim][N_c] (MHG_LIM_LEARNED)."""
+        a = np.ascontiguousarray(alpha, dtype=np.float64).reshape(self.dim, self.nc)
+        self._check(self.L.optional["set_learned_alpha"](self.ctx, a.ctypes.data_as(_DP)), "set_learned_alpha")
 
     def conservation(self):
         out = np.zeros(3)

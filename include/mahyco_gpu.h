@@ -84,8 +84,14 @@ typedef struct {
 /* Characteristic length (Mahyco.axl "longueur-caracteristique"). */
 enum { MHG_LC_RACINE_CUBIQUE_VOLUME = 0, MHG_LC_FACES_OPPOSEES = 1 };
 
-/* Limiters (TypesMahyco.h:65-74). */
-enum { MHG_LIM_MINMOD = 0, MHG_LIM_SUPERBEE = 1, MHG_LIM_VANLEER = 2, MHG_LIM_MINMODG = 3 };
+/* Limiters (TypesMahyco.h:65-74).  MHG_LIM_LEARNED is an extension with no
+ * counterpart in the reference: the cell sweeps take the slope
+ * alpha_dir(c) * (g_back + g_front) / 2, with alpha_dir(c) a per-cell,
+ * per-direction coefficient, and fall back to the minmod-limited slope
+ * wherever that slope puts a face value of the cell outside the min/max of
+ * the cell and its neighbours along the sweep; the dual sweeps use minmod. */
+enum { MHG_LIM_MINMOD = 0, MHG_LIM_SUPERBEE = 1, MHG_LIM_VANLEER = 2, MHG_LIM_MINMODG = 3,
+       MHG_LIM_LEARNED = 64 };
 
 /* Arithmetic of the bulk fields. */
 enum { MHG_PREC_FP64 = 0, MHG_PREC_FP32_BULK = 1 };
@@ -112,7 +118,7 @@ typedef struct {
   double remap_threshold;             /* default 1e-10 */
   int32_t is_euler_scheme;            /* must be 1     */
   int32_t ordre_projection;           /* 1 or 2        */
-  int32_t projection_limiteur_id;     /* < MHG_LIM_MINMODG */
+  int32_t projection_limiteur_id;     /* < MHG_LIM_MINMODG, or MHG_LIM_LEARNED */
   int32_t projection_pente_borne;     /* must be 0     */
   int32_t conservation_energie_totale;/* must be 0     */
   /* Extensions (no counterpart in the reference). */
@@ -245,6 +251,11 @@ int mhg_batch_status(mhg_batch* b, int32_t i, int32_t* iterations, double* time,
  * for the y sweeps also the column-segment or 64-row tile form) on the
  * batch's meshes and keeps the fastest; the simulation state is unchanged. */
 int mhg_batch_autotune(mhg_batch* b, int32_t reps, char* report, int32_t report_len);
+
+/* ---- learned limiter (extension, no counterpart in the reference) ----- */
+/* Fixed coefficients alpha [2][N_c] (x then y) of a context created with
+ * projection_limiteur_id = MHG_LIM_LEARNED; until set they are 0. */
+int mhg_set_learned_alpha(mhg_ctx* ctx, const double* alpha);
 
 /* ---- diagnostics ---------------------------------------------------- */
 /* Compensated global sums: total mass, total internal energy

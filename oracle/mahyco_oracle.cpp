@@ -77,6 +77,39 @@ inline double limiter(int id, double r) {
   }
 }
 
+// Learned slope of one cell and variable (extension, MHG_LIM_LEARNED): the
+// central slope alpha (g_b + g_f) / 2, kept if the face values it gives,
+// phi + g db at the back face and phi + g df at the front face, lie within
+// [min, max] of the cell's phi and its existing neighbours' along the sweep;
+// otherwise the minmod slope (RemapADIService.cc:497-506 with Lim = minmod).
+inline double learned_grad(double alpha, double phim, double phi, double phip, bool back_in, bool front_in,
+                           double gb, double gf, double db, double df) {
+  double gcl = 0.;
+  if (gb != 0.) gcl += 0.5 * (limiter(0, gf / gb) * gb);
+  if (gf != 0.) gcl += 0.5 * (limiter(0, gb / gf) * gf);
+  const double gc = alpha * (0.5 * (gb + gf));
+  double mn = phi, mx = phi;
+  if (back_in) {
+    mn = smin(mn, phim);
+    mx = smax(mx, phim);
+  }
+  if (front_in) {
+    mn = smin(mn, phip);
+    mx = smax(mx, phip);
+  }
+  const double vb = phi + gc * db, vf = phi + gc * df;
+  return (vb >= mn && vb <= mx && vf >= mn && vf <= mx) ? gc : gcl;
+}
+
+// Limiter of the classical branches: the options' limiter, minmod in the
+// learned mode (its fallback and its dual sweeps).
+inline int classical_limiter(const mhg_options& o) {
+  return o.projection_limiteur_id == MHG_LIM_LEARNED ? MHG_LIM_MINMOD : o.projection_limiteur_id;
+}
+inline bool has_limited_gradients(const mhg_options& o) {
+  return o.projection_limiteur_id < MHG_LIM_MINMODG || o.projection_limiteur_id == MHG_LIM_LEARNED;
+}
+
 // compute_eint, MahycoModule.cc:1630-1654.
 inline double compute_eint(bool pseudo_centree, double adiabatic_cst, double pseudo_viscosity_n,
                            double pseudo_viscosity, double density_n, double density,
@@ -216,6 +249,8 @@ struct mho_ctx {
   std::vector<R3> velocity, velocity_n, node_coord, node_coord_0, force;
   std::vector<double> node_mass;
   std::vector<double> u_dual_lagrange, phi_dual_lagrange, dual_grad_phi;  // [nn][nvar]
+  // learned-limiter coefficients [dim][nc] (MHG_LIM_LEARNED)
+  std::vector<double> learned_alpha;
   std::vector<double> back_flux_mass, front_flux_mass, back_flux_mass_env, front_flux_mass_env;
   // face variables
   std::vector<R3> face_coord, face_normal, face_length_lagrange;
@@ -947,7 +982,26 @@ void computeGradPhiCell(mho_ctx& S, int idir) {
   const int nv = S.nvar;
   std::fill(S.grad_phi.begin(), S.grad_phi.end(), 0.);
   if (S.opt.ordre_projection > 1 && S.opt.projection_pente_borne == 0 &&
-      S.opt.projection_limiteur_id < MHG_LIM_MINMODG) {
+      S.opt.projection_limiteur_id == MHG_LIM_LEARNED) {
+#pragma omp parallel for schedule(static)
+    for (long c = 0; c < S.NC; ++c) {
+      long bf, ff, pb, nb, pf, nf;
+      cellDirFaces(S, c, idir, &bf, &ff);
+      const bool back_in = isInnerFaceOfDir(S, bf, idir, &pb, &nb);
+      const bool front_in = isInnerFaceOfDir(S, ff, idir, &pf, &nf);
+      const double db = dot(S.face_coord[bf] - S.cell_coord[c], S.face_normal[bf]);
+      const double df = dot(S.face_coord[ff] - S.cell_coord[c], S.face_normal[ff]);
+      const double alpha = S.learned_alpha.empty() ? 0. : S.learned_alpha[(size_t)idir * S.NC + c];
+      for (int v = 0; v < nv; ++v) {
+        const double phi = S.phi_lagrange[c * nv + v];
+        const double phim = back_in ? S.phi_lagrange[pb * nv + v] : phi;
+        const double phip = front_in ? S.phi_lagrange[nf * nv + v] : phi;
+        S.grad_phi[c * nv + v] = learned_grad(alpha, phim, phi, phip, back_in, front_in,
+                                              S.grad_phi_face[bf * nv + v], S.grad_phi_face[ff * nv + v], db, df);
+      }
+    }
+  } else if (S.opt.ordre_projection > 1 && S.opt.projection_pente_borne == 0 &&
+             S.opt.projection_limiteur_id < MHG_LIM_MINMODG) {
     const int lim = S.opt.projection_limiteur_id;
 #pragma omp parallel for schedule(static)
     for (long c = 0; c < S.NC; ++c) {
@@ -1039,7 +1093,7 @@ void computeUremap(mho_ctx& S, int idir) {
 // computeDualGradPhi_LimC, DualPhaseRemap.cc:700-845.
 void computeDualGradPhi(mho_ctx& S, int idir) {
   const int nv = S.nvar;
-  const int lim = S.opt.projection_limiteur_id;
+  const int lim = classical_limiter(S.opt);
   long delta = (idir == 0) ? 1 : (idir == 1 ? S.nn[0] : (long)S.nn[0] * S.nn[1]);
 #pragma omp parallel for schedule(static)
   for (long nd = 0; nd < S.NN; ++nd) {
@@ -1067,8 +1121,7 @@ void computeDualGradPhi(mho_ctx& S, int idir) {
 void computeDualUremap(mho_ctx& S, int idir) {
   const int nv = S.nvar, ne = S.nenv;
   const double deltat = S.global_deltat;
-  if (S.opt.ordre_projection > 1 && S.opt.projection_limiteur_id < MHG_LIM_MINMODG)
-    computeDualGradPhi(S, idir);
+  if (S.opt.ordre_projection > 1 && has_limited_gradients(S.opt)) computeDualGradPhi(S, idir);
   const double oneovernbcell = (S.dim == 2 ? 0.5 : 0.25);
   // fcontrib (147-188): every cell is the back cell of its front face and the
   // front cell of its back face, so both contributions are set for every cell.
@@ -1553,9 +1606,8 @@ int mho_create(const mhg_mesh_desc* mesh, const mhg_options* opt, mho_ctx** out)
   if (opt->nb_env < 1 || opt->nb_env > MHG_MAX_ENV || opt->nb_bc < 0 || opt->nb_bc > MHG_MAX_BC)
     return MHG_E_ARG;
   if (opt->schema_csts || opt->with_newton || opt->sans_lagrange || !opt->is_euler_scheme ||
-      opt->projection_pente_borne || opt->conservation_energie_totale ||
-      opt->projection_limiteur_id >= MHG_LIM_MINMODG || opt->projection_limiteur_id < 0 ||
-      opt->ordre_projection < 1 || opt->ordre_projection > 2)
+      opt->projection_pente_borne || opt->conservation_energie_totale || !has_limited_gradients(*opt) ||
+      opt->projection_limiteur_id < 0 || opt->ordre_projection < 1 || opt->ordre_projection > 2)
     return MHG_E_UNSUPPORTED;
   for (int b = 0; b < opt->nb_bc; ++b)
     if (opt->bc[b].surface < 0 || opt->bc[b].surface > 5 || opt->bc[b].type < 0 ||
@@ -1892,6 +1944,17 @@ double mho_probe_cqs(int32_t dim, const double* coord, double* out_cqs) {
   return volume * (1. / dim);
 }
 double mho_probe_limiter(int32_t id, double r) { return limiter(id, r); }
+
+double mho_probe_learned_grad(double alpha, double phim, double phi, double phip, int32_t back_in,
+                              int32_t front_in, double gb, double gf, double db, double df) {
+  return learned_grad(alpha, phim, phi, phip, back_in != 0, front_in != 0, gb, gf, db, df);
+}
+
+int mho_set_learned_alpha(mho_ctx* c, const double* alpha) {
+  if (!c || !alpha) return MHG_E_ARG;
+  c->learned_alpha.assign(alpha, alpha + (size_t)c->dim * c->NC);
+  return MHG_OK;
+}
 double mho_probe_eint(int32_t pc, double gamma, double q_n, double q, double rho_n, double rho,
                       double p, double e_n) {
   return compute_eint(pc != 0, gamma, q_n, q, rho_n, rho, p, e_n);

@@ -64,6 +64,11 @@ double mho_probe_cqs(int32_t dim, const double* coord, double* out_cqs);
 /* Limiters of RemapADIService.h:260-286 (id 0 minmod, 1 superbee, 2 vanLeer,
  * other: order 1). */
 double mho_probe_limiter(int32_t id, double r);
+/* Learned-limiter slope of one cell and variable (MHG_LIM_LEARNED). */
+double mho_probe_learned_grad(double alpha, double phim, double phi, double phip, int32_t back_in,
+                              int32_t front_in, double gb, double gf, double db, double df);
+/* Coefficients of the learned limiter, [dim][N_c] (kept until replaced). */
+int mho_set_learned_alpha(mho_ctx* ctx, const double* alpha);
 /* compute_eint (MahycoModule.cc:1630-1654). */
 double mho_probe_eint(int32_t pseudo_centree, double gamma, double q_n, double q,
                       double rho_n, double rho, double p, double e_n);

@@ -1,0 +1,83 @@
+"""Learned limiter (MHG_LIM_LEARNED) in the oracle: the slope rule against
+closed forms, and the fallback path against the minmod trajectory."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import helpers as H
+from mahyco_amd import abi
+
+
+@pytest.fixture(scope="module")
+def lib():
+    L = H.oracle()
+    f = L.lib.mho_probe_learned_grad
+    f.restype = C.c_double
+    f.argtypes = [C.c_double] * 4 + [C.c_int32] * 2 + [C.c_double] * 4
+    return L
+
+
+def lg(lib, alpha, phim, phi, phip, back_in, front_in, gb, gf, db=-0.5, df=0.5):
+    return lib.lib.mho_probe_learned_grad(alpha, phim, phi, phip, back_in, front_in, gb, gf, db, df)
+
+
+def test_learned_grad_closed_forms(lib):
+    # linear profile 1, 2, 3 on unit cells: central slope 1, minmod slope 1
+    assert lg(lib, 0.0, 1, 2, 3, 1, 1, 1, 1) == 0.0
+    assert lg(lib, 1.0, 1, 2, 3, 1, 1, 1, 1) == 1.0
+    assert lg(lib, 2.0, 1, 2, 3, 1, 1, 1, 1) == 2.0     # face values 1 and 3: on the bounds
+    assert lg(lib, 2.5, 1, 2, 3, 1, 1, 1, 1) == 1.0     # 0.75 and 3.25: outside, minmod
+    assert lg(lib, 0.5, 1, 2, 3, 1, 1, 1, 1) == 0.5
+    # local maximum 1, 3, 2: central slope 0.5 overshoots 3; minmod of (2, -1) is 0
+    assert lg(lib, 1.0, 1, 3, 2, 1, 1, 2, -1) == 0.0
+    assert lg(lib, 0.0, 1, 3, 2, 1, 1, 2, -1) == 0.0
+    # unequal slopes 1, 2, 5: minmod 1, central 2; alpha 1 gives faces 1 and 3
+    assert lg(lib, 1.0, 1, 2, 5, 1, 1, 1, 3) == 2.0
+    assert lg(lib, 4.0, 1, 2, 5, 1, 1, 1, 3) == 1.0    # faces -2, 6: outside
+    # domain boundary behind the cell: gb = 0, bounds from the cell and its front
+    assert lg(lib, 1.0, 0, 2, 3, 0, 1, 0, 1) == 0.0
+    # NaN coefficient: the comparison fails, minmod
+    assert lg(lib, float("nan"), 1, 2, 3, 1, 1, 1, 1) == 1.0
+    # non-uniform faces: db = -0.25, df = 0.75, linear data: accepted up to the bounds
+    assert lg(lib, 1.0, 1, 2, 3, 1, 1, 1, 1, -0.25, 0.75) == 1.0
+    assert lg(lib, 1.5, 1, 2, 3, 1, 1, 1, 1, -0.25, 0.75) == 1.0   # 2 + 1.125 > 3
+
+
+def _state(s):
+    return s.download(H.CELL_FIELDS + ["velocity", "node_coord", "env_mask"] + H.ENV_FIELDS)
+
+
+@pytest.mark.parametrize("case", ["sod", "tp"])
+def test_learned_with_huge_alpha_is_minmod(lib, case):
+    """alpha = 1e30 puts every non-zero central slope outside the bounds, so
+    every cell falls back to minmod: the trajectory is minmod's bit for bit."""
+    if case == "sod":
+        mk = lambda **kw: H.sod_setup(lib, 40, 4, **kw)  # noqa: E731
+        steps = 40
+    else:
+        mk = lambda **kw: H.tp_setup(lib, 28, 12, **kw)  # noqa: E731
+        steps = 30
+    ref = mk()
+    ref.run(steps)
+    s = mk(projection_limiteur_id=abi.LIM_LEARNED)
+    s.set_learned_alpha(np.full((2, s.nc), 1e30))
+    s.run(steps)
+    a, b = _state(s), _state(ref)
+    for k in a:
+        if isinstance(a[k], np.ndarray):
+            assert np.array_equal(a[k], b[k]), k
+
+
+def test_learned_changes_the_solution(lib):
+    """alpha = 1 (central slopes where they keep the bounds) differs from
+    minmod and still conserves mass."""
+    ref = H.sod_setup(lib, 100, 2)
+    ref.run(150)
+    s = H.sod_setup(lib, 100, 2, projection_limiteur_id=abi.LIM_LEARNED)
+    m0 = s.conservation()[0]
+    s.set_learned_alpha(np.ones((2, s.nc)))
+    s.run(150)
+    d = np.max(np.abs(s.download(["density"])["density"] - ref.download(["density"])["density"]))
+    assert d > 1e-6
+    assert abs(s.conservation()[0] - m0) < 1e-13 * m0
