@@ -54,6 +54,10 @@ struct mhg_ctx {
   double* Veuler = nullptr;
   // learned limiter: coefficients [2][nc] in the storage type (MHG_LIM_LEARNED)
   void* alpha = nullptr;
+  // attached network (mhg_set_limiter_mlp): FP32 parameters and their BF16 copy
+  float* mlp_prm = nullptr;
+  unsigned short* mlp_wb = nullptr;
+  bool use_mlp = false;
   double* block_min = nullptr;
   mhg::DevScalars* S = nullptr;
   mhg::DevScalars* S_host = nullptr;  // pinned mirror
@@ -126,6 +130,12 @@ int build_params(const mhg_mesh_desc* mesh, const mhg_options* opt, Params* P);
 // steps the batch has run (defined with the batch).
 int batch_refresh_view(mhg_ctx* c);
 bool batch_running(const mhg_ctx* c);
+// Drops the captured step graphs (launch arguments changed).
+void ctx_drop_graphs(mhg_ctx* c);
+// Features of the state and, with write_alpha, the coefficients of the
+// attached network (defined with the network).
+template <class R>
+void enqueue_learned_alpha(mhg_ctx* c, float* Xout, float* Sout, bool write_alpha);
 }  // namespace mhg
 
 namespace mhg {

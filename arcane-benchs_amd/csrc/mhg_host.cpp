@@ -208,6 +208,10 @@ void remap_2d(mhg_ctx* c, int dt_mode) {
 // advance of the iteration start is applied by the final dt kernel.
 template <class R>
 void enqueue_step_t(mhg_ctx* c) {
+  if (c->use_mlp) {
+    TimedLaunch t(c, "learned_alpha");
+    enqueue_learned_alpha<R>(c, nullptr, nullptr, true);
+  }
   {
     TimedLaunch t(c, "node_lagrange");
     launch_node_lagrange_2d(c->stream, c->P, c->S, node_lag_args<R>(c), true, true, true);
@@ -353,6 +357,11 @@ void drop_graphs(mhg_ctx* c) {
   for (auto& g : c->graphs) hipGraphExecDestroy(g.exec);
   c->graphs.clear();
 }
+}  // namespace
+namespace mhg {
+void ctx_drop_graphs(mhg_ctx* c) { drop_graphs(c); }
+}  // namespace mhg
+namespace {
 
 // Enqueues n fused iterations.  Whole groups of kGraphSteps replay a graph
 // captured once per buffer state (one launch for the group's kernels); the
@@ -995,6 +1004,7 @@ int mhg_begin_iteration(mhg_ctx* c) {
 // (the lagged corner normals are formed from them), so the Euler reset
 // x <- x0 only changes which coordinates are current.
 int mhg_save_values_at_n(mhg_ctx* c) {
+  if (c && c->use_mlp && c->initialised && !c->owner) enqueue_learned_alpha<double>(c, nullptr, nullptr, true);
   int r = ep_stage(c, STAGE_SAVE);
   if (r) return r;
   c->coord_phase = 1;

@@ -1,0 +1,345 @@
+// mhg_mlp.hip -- kernels of the learned limiter's network on the matrix
+// cores: feature extraction from a mesh state, forward pass (inference and
+// training), backward pass to per-block partial gradients, their reduction
+// and the Adam update.
+//
+// One block is one wavefront and owns 32 rows (samples or cells), one
+// 32 x 32 MFMA tile high.  The 128-64-64 layers and every weight gradient run
+// as v_mfma_f32_32x32x8_bf16 tiles (BF16 operands staged in LDS, FP32
+// accumulation); the 64-2 output layer and the bias sums run on the VALU in
+// FP32.  Partial gradients are written per block and summed in a fixed order,
+// so training is deterministic.
+#include "mhg_kernels.hpp"
+#include "mhg_mlp.hpp"
+#include "mhg_mlp_kernels.hpp"
+
+namespace mhg {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kXs = kMlpIn + 8;   // padded LDS row of the input tile (bf16 words)
+constexpr int kHs = kMlpHid + 8;  // padded LDS row of the hidden tiles
+
+struct MlpLds {
+  unsigned short x[kMlpRows][kXs];
+  unsigned short h1[kMlpRows][kHs];
+  unsigned short h2[kMlpRows][kHs];
+  unsigned short d2[kMlpRows][kHs];  // dL/dH2, masked by the ReLU
+  unsigned short d1[kMlpRows][kHs];  // dL/dH1, masked by the ReLU
+  float y[kMlpRows][kMlpOut];
+  float dy[kMlpRows][kMlpOut];
+};
+
+// acc += A (32 x K) * B (K x 32) with A(i, k) = a[i * ai + k * ak] and
+// B(k, j) = b[k * bk + j * bj] (bf16, LDS or global).  Operand layout of
+// v_mfma_f32_32x32x8_bf16: lane l holds A(l % 32, k0 + 4 (l / 32) + t) and
+// B(k0 + 4 (l / 32) + t, l % 32), t = 0..3.
+__device__ inline f32x16 mfma_tile(f32x16 acc, const unsigned short* a, int ai, int ak, const unsigned short* b,
+                                   int bk, int bj, int K) {
+  const int l = threadIdx.x & 63;
+  const int r = l & 31, q = l >> 5;
+  for (int k0 = 0; k0 < K; k0 += 8) {
+    bf16x4 av, bv;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int k = k0 + q * 4 + t;
+      av[t] = (short)a[r * ai + k * ak];
+      bv[t] = (short)b[k * bk + r * bj];
+    }
+    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, acc, 0, 0, 0);
+  }
+  return acc;
+}
+
+// Row of accumulator element v in this lane (its column is lane % 32).
+__device__ inline int acc_row(int v) {
+  const int l = threadIdx.x & 63;
+  return 8 * (v >> 2) + 4 * (l >> 5) + (v & 3);
+}
+
+__device__ inline f32x16 zero16() {
+  f32x16 z;
+#pragma unroll
+  for (int v = 0; v < 16; ++v) z[v] = 0.f;
+  return z;
+}
+
+// Forward pass of the 32 rows staged in L.x: L.h1, L.h2 (bf16) and L.y.
+__device__ inline void mlp_forward_tile(MlpLds& L, const MlpWeights& W) {
+  const int l = threadIdx.x & 63;
+  for (int nt = 0; nt < 2; ++nt) {  // H1 = relu(X W1^T + b1), B(k, j) = W1[32 nt + j][k]
+    const f32x16 acc = mfma_tile(zero16(), &L.x[0][0], kXs, 1, W.w1 + nt * 32 * kMlpIn, 1, kMlpIn, kMlpIn);
+    const int col = nt * 32 + (l & 31);
+    const float b = W.b1[col];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) L.h1[acc_row(v)][col] = f2bf(fmaxf(acc[v] + b, 0.f));
+  }
+  __syncthreads();
+  for (int nt = 0; nt < 2; ++nt) {  // H2 = relu(H1 W2^T + b2)
+    const f32x16 acc = mfma_tile(zero16(), &L.h1[0][0], kHs, 1, W.w2 + nt * 32 * kMlpHid, 1, kMlpHid, kMlpHid);
+    const int col = nt * 32 + (l & 31);
+    const float b = W.b2[col];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) L.h2[acc_row(v)][col] = f2bf(fmaxf(acc[v] + b, 0.f));
+  }
+  __syncthreads();
+  {  // Y = H2 W3^T + b3: lane l -> row l % 32, output l / 32
+    const int row = l & 31, o = l >> 5;
+    float s = 0.f;
+    for (int k = 0; k < kMlpHid; ++k) s += bf2f(L.h2[row][k]) * bf2f(W.w3[o * kMlpHid + k]);
+    L.y[row][o] = s + W.b3[o];
+  }
+  __syncthreads();
+}
+
+// Loss of the slope regression and its gradient: for each direction d and
+// channel k < 4, e = y_d sc - st; loss = sum e^2 / m; dL/dy_d = 2 sum_k e sc / m.
+// Returns this lane's share of the tile's loss (rows beyond nvalid count 0).
+__device__ inline float mlp_loss_tile(MlpLds& L, const float* T, const long* rows, int nvalid, float inv_m) {
+  const int l = threadIdx.x & 63;
+  const int row = l & 31, o = l >> 5;
+  float lsum = 0.f, g = 0.f;
+  if (row < nvalid) {
+    const float* t = T + rows[row] * kMlpTgt + o * 8;
+    const float y = L.y[row][o];
+    for (int k = 0; k < 4; ++k) {
+      const float sc = t[2 * k], st = t[2 * k + 1];
+      const float e = y * sc - st;
+      g += e * sc;
+      lsum += e * e;
+    }
+  }
+  L.dy[row][o] = 2.f * g * inv_m;
+  __syncthreads();
+  return lsum * inv_m;
+}
+
+// Backward pass of one tile to its partial gradient part[kMlpParams].
+__device__ inline void mlp_backward_tile(MlpLds& L, const MlpWeights& W, float* part) {
+  const int l = threadIdx.x & 63;
+  // output layer
+  for (int e = l; e < kMlpOut * kMlpHid; e += 64) {
+    const int o = e / kMlpHid, k = e % kMlpHid;
+    float s = 0.f;
+    for (int row = 0; row < kMlpRows; ++row) s += L.dy[row][o] * bf2f(L.h2[row][k]);
+    part[kPW3 + e] = s;
+  }
+  if (l < kMlpOut) {
+    float s = 0.f;
+    for (int row = 0; row < kMlpRows; ++row) s += L.dy[row][l];
+    part[kPB3 + l] = s;
+  }
+  // dL/dH2 = dY W3, masked
+  for (int e = l; e < kMlpRows * kMlpHid; e += 64) {
+    const int row = e / kMlpHid, k = e % kMlpHid;
+    const float s = L.dy[row][0] * bf2f(W.w3[k]) + L.dy[row][1] * bf2f(W.w3[kMlpHid + k]);
+    L.d2[row][k] = f2bf(bf2f(L.h2[row][k]) > 0.f ? s : 0.f);
+  }
+  __syncthreads();
+  {  // db2
+    float s = 0.f;
+    for (int row = 0; row < kMlpRows; ++row) s += bf2f(L.d2[row][l]);
+    part[kPB2 + l] = s;
+  }
+  // dW2[n][k] = sum_rows d2[row][n] h1[row][k]: A(n, row) = d2[row][n], B(row, k) = h1[row][k]
+  for (int nt = 0; nt < 2; ++nt)
+    for (int kt = 0; kt < 2; ++kt) {
+      const f32x16 acc = mfma_tile(zero16(), &L.d2[0][nt * 32], 1, kHs, &L.h1[0][kt * 32], kHs, 1, kMlpRows);
+#pragma unroll
+      for (int v = 0; v < 16; ++v) part[kPW2 + (nt * 32 + acc_row(v)) * kMlpHid + kt * 32 + (l & 31)] = acc[v];
+    }
+  // dL/dH1 = d2 W2, masked: A(row, n) = d2[row][n], B(n, k) = W2[n][k]
+  for (int kt = 0; kt < 2; ++kt) {
+    const f32x16 acc = mfma_tile(zero16(), &L.d2[0][0], kHs, 1, W.w2 + kt * 32, kMlpHid, 1, kMlpHid);
+    const int col = kt * 32 + (l & 31);
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int row = acc_row(v);
+      L.d1[row][col] = f2bf(bf2f(L.h1[row][col]) > 0.f ? acc[v] : 0.f);
+    }
+  }
+  __syncthreads();
+  {  // db1
+    float s = 0.f;
+    for (int row = 0; row < kMlpRows; ++row) s += bf2f(L.d1[row][l]);
+    part[kPB1 + l] = s;
+  }
+  // dW1[n][k] = sum_rows d1[row][n] x[row][k]
+  for (int nt = 0; nt < 2; ++nt)
+    for (int kt = 0; kt < 4; ++kt) {
+      const f32x16 acc = mfma_tile(zero16(), &L.d1[0][nt * 32], 1, kHs, &L.x[0][kt * 32], kXs, 1, kMlpRows);
+#pragma unroll
+      for (int v = 0; v < 16; ++v) part[kPW1 + (nt * 32 + acc_row(v)) * kMlpIn + kt * 32 + (l & 31)] = acc[v];
+    }
+}
+
+// Rows [row0 + 32 blk, ...) of the sample set (through perm if given):
+// forward pass; TRAIN adds the loss and the backward pass into part[blk].
+template <bool TRAIN>
+__global__ __launch_bounds__(64) void k_mlp_rows(const unsigned short* __restrict__ X, const float* __restrict__ T,
+                                                 const int* __restrict__ perm, long row0, long row_end,
+                                                 MlpWeights W, float* Y, float* part, float inv_m) {
+  __shared__ MlpLds L;
+  __shared__ long rows[kMlpRows];
+  const int l = threadIdx.x;
+  const long first = row0 + (long)blockIdx.x * kMlpRows;
+  const int nvalid = (int)min((long)kMlpRows, row_end - first);
+  if (l < kMlpRows) rows[l] = l < nvalid ? (perm ? (long)perm[first + l] : first + l) : 0;
+  __syncthreads();
+  for (int e = l; e < kMlpRows * kMlpIn; e += 64) {
+    const int row = e / kMlpIn, k = e % kMlpIn;
+    L.x[row][k] = row < nvalid ? X[rows[row] * kMlpIn + k] : (unsigned short)0;
+  }
+  __syncthreads();
+  mlp_forward_tile(L, W);
+  if (!TRAIN) {
+    const int row = l & 31, o = l >> 5;
+    if (row < nvalid) Y[rows[row] * kMlpOut + o] = L.y[row][o];
+    return;
+  }
+  float* p = part + (long)blockIdx.x * (kMlpParams + 1);
+  float loss = mlp_loss_tile(L, T, rows, nvalid, inv_m);
+  for (int off = 32; off > 0; off >>= 1) loss += __shfl_down(loss, off, 64);
+  if (l == 0) p[kMlpParams] = loss;
+  mlp_backward_tile(L, W, p);
+}
+
+// grad[p] = sum over blocks of part[b][p] in block order (p <= kMlpParams:
+// the last entry is the loss).
+__global__ void k_mlp_reduce(const float* __restrict__ part, int nblk, float* __restrict__ grad) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > kMlpParams) return;
+  float s = 0.f;
+  for (int b = 0; b < nblk; ++b) s += part[(long)b * (kMlpParams + 1) + p];
+  grad[p] = s;
+}
+
+// Adam (Kingma & Ba) on the FP32 master parameters, then their BF16 copies.
+__global__ void k_mlp_adam(float* __restrict__ prm, float* __restrict__ m1, float* __restrict__ m2,
+                           const float* __restrict__ grad, float lr, float beta1, float beta2, float eps,
+                           float corr1, float corr2, unsigned short* __restrict__ wb) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= kMlpParams) return;
+  const float g = grad[p];
+  const float a = beta1 * m1[p] + (1.f - beta1) * g;
+  const float b = beta2 * m2[p] + (1.f - beta2) * g * g;
+  m1[p] = a;
+  m2[p] = b;
+  const float w = prm[p] - lr * (a / corr1) / (sqrtf(b / corr2) + eps);
+  prm[p] = w;
+  wb[p] = f2bf(w);
+}
+
+// BF16 copy of the master parameters.
+__global__ void k_mlp_to_bf16(const float* __restrict__ prm, unsigned short* __restrict__ wb) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < kMlpParams) wb[p] = f2bf(prm[p]);
+}
+
+// ---- features of a mesh state and inference --------------------------------
+// Channel value of cell (ci, cj): density, specific internal energy, mean of
+// the 4 node velocities (x, y; in node order (i,j), (i+1,j), (i+1,j+1),
+// (i,j+1)), largest environment volume fraction (1 for a pure cell).
+template <class R>
+__device__ inline float cell_channel(const Params& P, const LearnedArgsT<R>& A, int k, int ci, int cj) {
+  const long c = ci + (long)P.nx * cj;
+  if (k == 0) return (float)A.rho[c];
+  if (k == 1) return (float)A.e[c];
+  if (k == 2 || k == 3) {
+    const R* u = k == 2 ? A.ux : A.uy;
+    const long n0 = ci + (long)(P.nx + 1) * cj;
+    return (float)(R(0.25) * (((u[n0] + u[n0 + 1]) + u[n0 + 1 + P.nx + 1]) + u[n0 + P.nx + 1]));
+  }
+  const int mask = A.mask[c];
+  if (__popc(mask) <= 1) return 1.f;
+  float f = 0.f;
+  for (int e = 0; e < P.ne; ++e)
+    if ((mask >> e) & 1) f = fmaxf(f, (float)A.env_f[(long)e * P.nc + c]);
+  return f;
+}
+
+// One block: 32 cells.  Lanes l and l + 32 build cell l's features (the
+// first lane channels 0-2, the second channels 3-4 and the padding), then the
+// forward pass gives alpha = clamp(y, 0, 2).  EXPORT also writes the float
+// features and the per-channel scales.
+template <class R, bool EXPORT>
+__global__ __launch_bounds__(64) void k_learned_alpha_2d(Params P, LearnedArgsT<R> A, MlpWeights W, float* Xout,
+                                                         float* Sout) {
+  __shared__ MlpLds L;
+  const int l = threadIdx.x;
+  const int cl = l & 31, half = l >> 5;
+  const long c = (long)blockIdx.x * kMlpRows + cl;
+  const bool valid = c < P.nc;
+  const int ci = valid ? (int)(c % P.nx) : 0, cj = valid ? (int)(c / P.nx) : 0;
+  const int k0 = half ? 3 : 0, k1 = half ? 5 : 3;
+  for (int k = k0; k < k1; ++k) {
+    float v[kMlpSten];
+#pragma unroll
+    for (int s = 0; s < kMlpSten; ++s) {
+      const int ii = min(max(ci + s % 5 - 2, 0), P.nx - 1), jj = min(max(cj + s / 5 - 2, 0), P.ny - 1);
+      v[s] = valid ? cell_channel(P, A, k, ii, jj) : 0.f;
+    }
+    const float ctr = v[12];
+    float sc = 0.f;
+#pragma unroll
+    for (int s = 0; s < kMlpSten; ++s) sc = fmaxf(sc, fabsf(v[s] - ctr));
+    const float S = sc + (1e-6f * fabsf(ctr) + 1e-30f);
+#pragma unroll
+    for (int s = 0; s < kMlpSten; ++s) {
+      const float x = (v[s] - ctr) / S;
+      L.x[cl][k * kMlpSten + s] = f2bf(x);
+      if (EXPORT && valid) Xout[c * kMlpIn + k * kMlpSten + s] = x;
+    }
+    if (EXPORT && valid) Sout[c * kMlpChan + k] = S;
+  }
+  if (half)
+    for (int k = kMlpFeat; k < kMlpIn; ++k) {
+      L.x[cl][k] = 0;
+      if (EXPORT && valid) Xout[c * kMlpIn + k] = 0.f;
+    }
+  __syncthreads();
+  mlp_forward_tile(L, W);
+  if (valid && A.alpha) {
+    const float y = L.y[cl][half];
+    A.alpha[(long)half * P.nc + c] = (R)fminf(fmaxf(y, 0.f), 2.f);
+  }
+}
+
+// ---- launchers ------------------------------------------------------------------
+void launch_mlp_rows(hipStream_t st, bool train, const unsigned short* X, const float* T, const int* perm, long row0,
+                     long row_end, const MlpWeights& W, float* Y, float* part, float inv_m) {
+  const unsigned nblk = (unsigned)((row_end - row0 + kMlpRows - 1) / kMlpRows);
+  if (nblk == 0) return;
+  if (train) hipLaunchKernelGGL(k_mlp_rows<true>, dim3(nblk), dim3(64), 0, st, X, T, perm, row0, row_end, W, Y, part, inv_m);
+  else hipLaunchKernelGGL(k_mlp_rows<false>, dim3(nblk), dim3(64), 0, st, X, T, perm, row0, row_end, W, Y, part, inv_m);
+}
+
+void launch_mlp_reduce(hipStream_t st, const float* part, int nblk, float* grad) {
+  hipLaunchKernelGGL(k_mlp_reduce, dim3((kMlpParams + 1 + 255) / 256), dim3(256), 0, st, part, nblk, grad);
+}
+
+void launch_mlp_adam(hipStream_t st, float* prm, float* m1, float* m2, const float* grad, float lr, float beta1,
+                     float beta2, float eps, float corr1, float corr2, unsigned short* wb) {
+  hipLaunchKernelGGL(k_mlp_adam, dim3((kMlpParams + 255) / 256), dim3(256), 0, st, prm, m1, m2, grad, lr, beta1, beta2,
+                     eps, corr1, corr2, wb);
+}
+
+void launch_mlp_to_bf16(hipStream_t st, const float* prm, unsigned short* wb) {
+  hipLaunchKernelGGL(k_mlp_to_bf16, dim3((kMlpParams + 255) / 256), dim3(256), 0, st, prm, wb);
+}
+
+template <class R>
+void launch_learned_alpha_2d(hipStream_t st, const Params& P, const LearnedArgsT<R>& A, const MlpWeights& W,
+                             float* Xout, float* Sout) {
+  const dim3 g((unsigned)((P.nc + kMlpRows - 1) / kMlpRows)), b(64);
+  if (Xout) hipLaunchKernelGGL((k_learned_alpha_2d<R, true>), g, b, 0, st, P, A, W, Xout, Sout);
+  else hipLaunchKernelGGL((k_learned_alpha_2d<R, false>), g, b, 0, st, P, A, W, Xout, Sout);
+}
+
+template void launch_learned_alpha_2d<double>(hipStream_t, const Params&, const LearnedArgsT<double>&,
+                                              const MlpWeights&, float*, float*);
+template void launch_learned_alpha_2d<float>(hipStream_t, const Params&, const LearnedArgsT<float>&,
+                                             const MlpWeights&, float*, float*);
+
+}  // namespace mhg

@@ -61,6 +61,7 @@ class Options(C.Structure):
 
 _DP = C.POINTER(C.c_double)
 _IP = C.POINTER(C.c_int32)
+_FP = C.POINTER(C.c_float)
 
 STATE_CELL = ["density", "internal_energy", "pressure", "sound_speed", "pseudo_viscosity",
               "cell_mass", "fracvol", "mass_fraction"]
@@ -144,7 +145,20 @@ class Library:
                 ("batch_run", C.c_int, [vp, C.c_int32, _IP, C.POINTER(C.c_int64)]),
                 ("batch_status", C.c_int, [vp, C.c_int32, _IP, _DP, _DP, _IP]),
                 ("batch_autotune", C.c_int, [vp, C.c_int32, C.c_char_p, C.c_int32]),
-                ("set_learned_alpha", C.c_int, [vp, _DP])]:
+                ("set_learned_alpha", C.c_int, [vp, _DP]),
+                ("learned_alpha", C.c_int, [vp, _DP]),
+                ("learned_features", C.c_int, [vp, _FP, _FP]),
+                ("set_limiter_mlp", C.c_int, [vp, vp]),
+                ("mlp_create", C.c_int, [C.POINTER(vp)]),
+                ("mlp_destroy", C.c_int, [vp]),
+                ("mlp_last_error", C.c_char_p, [vp]),
+                ("mlp_param_count", C.c_int32, []),
+                ("mlp_set_params", C.c_int, [vp, _FP]),
+                ("mlp_get_params", C.c_int, [vp, _FP]),
+                ("mlp_forward", C.c_int, [vp, _FP, C.c_int64, _FP]),
+                ("mlp_set_data", C.c_int, [vp, _FP, _FP, C.c_int64]),
+                ("mlp_gradient", C.c_int, [vp, C.c_int64, C.c_int64, _FP, _FP]),
+                ("mlp_train", C.c_int, [vp, C.c_int32, C.c_int32, C.c_float, C.c_uint64, _FP])]:
             if hasattr(L, p + name):
                 self.optional[name] = fn(name, res, args)
 
@@ -339,6 +353,21 @@ class Simulation:
         a = np.ascontiguousarray(alpha, dtype=np.float64).reshape(self.dim, self.nc)
         self._check(self.L.optional["set_learned_alpha"](self.ctx, a.ctypes.data_as(_DP)), "set_learned_alpha")
 
+    def learned_alpha(self):
+        a = np.zeros((2, self.nc))
+        self._check(self.L.optional["learned_alpha"](self.ctx, a.ctypes.data_as(_DP)), "learned_alpha")
+        return a
+
+    def learned_features(self):
+        X = np.zeros((self.nc, 128), dtype=np.float32)
+        S = np.zeros((self.nc, 5), dtype=np.float32)
+        self._check(self.L.optional["learned_features"](self.ctx, X.ctypes.data_as(_FP), S.ctypes.data_as(_FP)),
+                    "learned_features")
+        return X, S
+
+    def set_limiter_mlp(self, mlp):
+        self._check(self.L.optional["set_limiter_mlp"](self.ctx, mlp.h), "set_limiter_mlp")
+
     def conservation(self):
         out = np.zeros(3)
         self._check(self.L.conservation(self.ctx, out.ctypes.data_as(_DP)), "conservation")
@@ -398,6 +427,77 @@ class Batch:
             for p in self.problems:
                 p.ctx = None
             self.L.optional["batch_destroy"](self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a if shape is None else a.reshape(shape)
+
+
+class Mlp:
+    """The learned limiter's network (mhg_mlp_*)."""
+
+    def __init__(self, library, params=None):
+        self.L = library
+        self.f = library.optional
+        h = C.c_void_p()
+        r = self.f["mlp_create"](C.byref(h))
+        if r != OK:
+            raise MahycoError(r, "mlp_create")
+        self.h = h
+        self.nparam = int(self.f["mlp_param_count"]())
+        if params is not None:
+            self.set_params(params)
+
+    def _check(self, r, what):
+        if r < 0:
+            msg = self.f["mlp_last_error"](self.h)
+            raise MahycoError(r, f"{what}: {msg.decode() if msg else ''}")
+        return r
+
+    def set_params(self, p):
+        p = _f32(p)
+        assert p.size == self.nparam
+        self._check(self.f["mlp_set_params"](self.h, p.ctypes.data_as(_FP)), "mlp_set_params")
+
+    def params(self):
+        p = np.zeros(self.nparam, dtype=np.float32)
+        self._check(self.f["mlp_get_params"](self.h, p.ctypes.data_as(_FP)), "mlp_get_params")
+        return p
+
+    def forward(self, X):
+        X = _f32(X)
+        Y = np.zeros((X.shape[0], 2), dtype=np.float32)
+        self._check(self.f["mlp_forward"](self.h, X.ctypes.data_as(_FP), X.shape[0], Y.ctypes.data_as(_FP)),
+                    "mlp_forward")
+        return Y
+
+    def set_data(self, X, T):
+        self._X, self._T = _f32(X), _f32(T)
+        self._check(self.f["mlp_set_data"](self.h, self._X.ctypes.data_as(_FP), self._T.ctypes.data_as(_FP),
+                                           self._X.shape[0]), "mlp_set_data")
+
+    def gradient(self, row0, m):
+        g = np.zeros(self.nparam, dtype=np.float32)
+        loss = C.c_float()
+        self._check(self.f["mlp_gradient"](self.h, row0, m, C.byref(loss), g.ctypes.data_as(_FP)), "mlp_gradient")
+        return loss.value, g
+
+    def train(self, epochs, batch, lr, seed=0):
+        out = np.zeros(epochs, dtype=np.float32)
+        self._check(self.f["mlp_train"](self.h, epochs, batch, lr, seed, out.ctypes.data_as(_FP)), "mlp_train")
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.f["mlp_destroy"](self.h)
             self.h = None
 
     def __del__(self):

@@ -256,6 +256,44 @@ int mhg_batch_autotune(mhg_batch* b, int32_t reps, char* report, int32_t report_
 /* Fixed coefficients alpha [2][N_c] (x then y) of a context created with
  * projection_limiteur_id = MHG_LIM_LEARNED; until set they are 0. */
 int mhg_set_learned_alpha(mhg_ctx* ctx, const double* alpha);
+/* The coefficients the next step uses: with a network attached, computed
+ * from the current state; otherwise the fixed ones.  alpha [2][N_c]. */
+int mhg_learned_alpha(mhg_ctx* ctx, double* alpha);
+
+/* The network: a 128-64-64-2 perceptron with ReLU, BF16 weights and FP32
+ * accumulation on the matrix cores.  Input: a 5 x 5 stencil of density,
+ * specific internal energy, cell-mean x and y velocity and largest
+ * environment volume fraction, each channel centred on the cell's value and
+ * divided by its largest deviation over the stencil (cells beyond the domain
+ * repeat the boundary cell), channel-major, zero-padded to 128.  Output:
+ * alpha_x, alpha_y, clamped to [0, 2] where they are used.  Parameters, FP32:
+ * W1 [64][128], b1 [64], W2 [64][64], b2 [64], W3 [2][64], b3 [2]. */
+typedef struct mhg_mlp mhg_mlp;
+int mhg_mlp_create(mhg_mlp** out);
+int mhg_mlp_destroy(mhg_mlp* net);
+const char* mhg_mlp_last_error(const mhg_mlp* net);
+int32_t mhg_mlp_param_count(void);
+int mhg_mlp_set_params(mhg_mlp* net, const float* params);  /* also resets the optimiser */
+int mhg_mlp_get_params(mhg_mlp* net, float* params);
+/* Y [m][2] of X [m][128] (rounded to BF16). */
+int mhg_mlp_forward(mhg_mlp* net, const float* X, int64_t m, float* Y);
+/* Training samples kept on the device: X [m][128], targets T [m][16] =
+ * for direction d and channel k < 4 (density, energy, x and y velocity), the
+ * normalised central slope and the normalised reference slope; the loss is
+ * sum over d, k of (alpha_d sc - st)^2, averaged over the samples. */
+int mhg_mlp_set_data(mhg_mlp* net, const float* X, const float* T, int64_t m);
+/* Loss and gradient over samples [row0, row0 + m) of the data. */
+int mhg_mlp_gradient(mhg_mlp* net, int64_t row0, int64_t m, float* loss, float* grad);
+/* Adam (beta 0.9, 0.999, eps 1e-8) over shuffled minibatches; the mean
+ * minibatch loss of each epoch in epoch_loss [epochs]. */
+int mhg_mlp_train(mhg_mlp* net, int32_t epochs, int32_t batch, float lr, uint64_t seed, float* epoch_loss);
+/* Attaches a copy of the network to a learned-limiter context: every later
+ * step first computes alpha from the state at its start. */
+int mhg_set_limiter_mlp(mhg_ctx* ctx, const mhg_mlp* net);
+/* Network inputs of every cell of a 2D context's current state, X [N_c][128],
+ * and the channel scales (largest deviation plus 1e-6 |centre|), [N_c][5]
+ * (may be NULL); the coefficients are not touched. */
+int mhg_learned_features(mhg_ctx* ctx, float* X, float* scale);
 
 /* ---- diagnostics ---------------------------------------------------- */
 /* Compensated global sums: total mass, total internal energy
