@@ -526,8 +526,17 @@ __device__ __forceinline__ void dt_finalize_body(const Params& P, DevScalars* S,
     S->err = 1;
     return;
   }
-  S->dt = new_dt;
   const double t = S->time;
+  if (P.land) {  // extension: the last step ends at tend
+    if (S->landing || t >= P.tend) S->stop = 1;
+    else if (t + new_dt >= P.tend) {
+      new_dt = P.tend - t;
+      S->landing = 1;
+    }
+    S->dt = new_dt;
+    return;
+  }
+  S->dt = new_dt;
   const bool not_yet_finish = (t < P.tend);
   const bool finish = (t > P.tend);
   const bool too_much = ((t + new_dt) > P.tend);

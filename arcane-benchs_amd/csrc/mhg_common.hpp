@@ -366,7 +366,7 @@ struct DevScalars {
   int iteration;
   int stop;         // end of run reached
   int err;          // MHG_E_DT_MIN raised
-  int pad;
+  int landing;      // the step under way ends at the final time (land_on_final_time)
 };
 
 // Run-time parameters of a context.
@@ -379,6 +379,7 @@ struct Params {
   double threshold, remap_threshold;
   int order2, lim, pseudo_centree, lc;
   int learned;  // cell sweeps use the learned-limiter slope (lim is then minmod)
+  int land;     // shorten the last step to end at tend
   double cfl, dtmin, dtmax, tend;
   int nbc;
   BcRange bc[kMaxBc];

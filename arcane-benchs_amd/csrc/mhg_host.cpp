@@ -566,6 +566,7 @@ int build_params(const mhg_mesh_desc* mesh, const mhg_options* opt, Params* Pout
   P.dtmin = opt->deltat_min;
   P.dtmax = opt->deltat_max;
   P.tend = opt->final_time;
+  P.land = opt->land_on_final_time ? 1 : 0;
   P.nbc = 0;
   for (int b = 0; b < opt->nb_bc; ++b) {
     const mhg_bc& bc = opt->bc[b];

@@ -55,7 +55,7 @@ class Options(C.Structure):
         ("ordre_projection", C.c_int32), ("projection_limiteur_id", C.c_int32),
         ("projection_pente_borne", C.c_int32), ("conservation_energie_totale", C.c_int32),
         ("visc_quad_coeff", C.c_double), ("precision", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("land_on_final_time", C.c_int32), ("reserved", C.c_int32 * 6),
     ]
 
 

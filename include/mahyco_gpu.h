@@ -124,7 +124,10 @@ typedef struct {
   /* Extensions (no counterpart in the reference). */
   double visc_quad_coeff;   /* < 0: reference value (gamma+1)/2 */
   int32_t precision;        /* MHG_PREC_FP64 (the drop-in) or MHG_PREC_FP32_BULK */
-  int32_t reserved[7];
+  int32_t land_on_final_time; /* 1: the last step is shortened to end at final_time
+                                 (0, the reference: the run stops before the step
+                                 that would pass it, MahycoModule.cc:2177-2183) */
+  int32_t reserved[6];
 } mhg_options;
 
 /* Full per-step state, structure-of-arrays, caller-owned host memory.

@@ -251,6 +251,7 @@ struct mho_ctx {
   std::vector<double> u_dual_lagrange, phi_dual_lagrange, dual_grad_phi;  // [nn][nvar]
   // learned-limiter coefficients [dim][nc] (MHG_LIM_LEARNED)
   std::vector<double> learned_alpha;
+  bool landing = false;  // land_on_final_time: the step under way ends at final_time
   std::vector<double> back_flux_mass, front_flux_mass, back_flux_mass_env, front_flux_mass_env;
   // face variables
   std::vector<R3> face_coord, face_normal, face_length_lagrange;
@@ -1447,8 +1448,18 @@ int computeDeltaT(mho_ctx& S, double* out_dt, int* stop) {
     *out_dt = new_dt;
     return MHG_E_DT_MIN;
   }
-  S.global_deltat = new_dt;
   const double stop_time = S.opt.final_time;
+  if (S.opt.land_on_final_time) {  // extension: the last step ends at final_time
+    if (S.landing || S.global_time >= stop_time) *stop = 1;
+    else if (S.global_time + new_dt >= stop_time) {
+      new_dt = stop_time - S.global_time;
+      S.landing = true;
+    }
+    S.global_deltat = new_dt;
+    *out_dt = new_dt;
+    return MHG_OK;
+  }
+  S.global_deltat = new_dt;
   bool not_yet_finish = (S.global_time < stop_time);
   bool finish = (S.global_time > stop_time);
   bool too_much = ((S.global_time + new_dt) > stop_time);
@@ -1550,6 +1561,7 @@ void resetPhysics(mho_ctx& S) {
   S.global_deltat = S.opt.deltat_init;
   S.global_old_deltat = S.old_deltat = S.global_time = 0.;
   S.global_iteration = 0;
+  S.landing = false;
   S.sens_projection = 0;
   cellCoordFromNodes(S);
 }
@@ -1781,6 +1793,7 @@ int mho_upload_state(mho_ctx* ctx, const mhg_state* st) {
   S.global_deltat = st->global_deltat;
   S.old_deltat = st->old_deltat;
   S.global_time = st->global_time;
+  S.landing = false;
   S.sens_projection = st->sens_projection;
   S.global_iteration = st->global_iteration;
   S.initialised = true;

@@ -37,8 +37,8 @@ def tp_setup(L, nx=56, ny=24, **optkw):
     opt = L.options(**optkw)
     abi.set_bcs(opt, abi.reflecting_walls(2))
     abi.set_envs(opt, [(abi.EOS_PERFECT_GAS, g, 0.) for g in TP_GAMMA])
-    opt.final_time = 1e30
-    opt.deltat_max = 1e30
+    opt.final_time = optkw.get("final_time", 1e30)
+    opt.deltat_max = optkw.get("deltat_max", 1e30)
     m = abi.mesh_desc(2, [nx, ny], [0, 0], [7.0, 3.0])
     s = abi.Simulation(L, m, opt)
     s.init_triple_point(TP_RHO, TP_P)

@@ -91,3 +91,14 @@ def test_learned_options_rejected_where_unsupported(gpu):
     with pytest.raises(abi.MahycoError) as e:
         s.set_learned_alpha(np.zeros((2, s.nc)))
     assert e.value.code == abi.E_UNSUPPORTED
+
+
+def test_land_on_final_time_matches_oracle(gpu):
+    o = H.tp_setup(H.oracle(), 56, 24, final_time=0.2, land_on_final_time=1)
+    g = H.tp_setup(gpu, 56, 24, final_time=0.2, land_on_final_time=1)
+    no, ng = o.run(100000), g.run(100000)
+    assert no == ng
+    a, b = g.download(), o.download()
+    assert abs(a["global_time"] - 0.2) <= 1e-15 and a["global_time"] == b["global_time"]
+    d = H.compare_states(a, b, 3, 56, 24, metric=H.rel_max)
+    assert max(d.values()) < 1e-10, d

@@ -81,3 +81,15 @@ def test_learned_changes_the_solution(lib):
     d = np.max(np.abs(s.download(["density"])["density"] - ref.download(["density"])["density"]))
     assert d > 1e-6
     assert abs(s.conservation()[0] - m0) < 1e-13 * m0
+
+
+def test_land_on_final_time(lib):
+    """land_on_final_time: the run ends at final_time exactly (the last dt is
+    shortened); without it the run stops before the step that would pass it."""
+    a = H.sod_setup(lib, 100, 2, final_time=0.05)
+    na = a.run(100000)
+    b = H.sod_setup(lib, 100, 2, final_time=0.05, land_on_final_time=1)
+    nb = b.run(100000)
+    ta, tb = a.download(["density"])["global_time"], b.download(["density"])["global_time"]
+    assert ta < 0.05 and nb == na + 1
+    assert abs(tb - 0.05) <= 1e-15
