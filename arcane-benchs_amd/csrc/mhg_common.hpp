@@ -378,7 +378,8 @@ struct Params {
   EosParams eos;
   double threshold, remap_threshold;
   int order2, lim, pseudo_centree, lc;
-  int learned;  // cell sweeps use the learned-limiter slope (lim is then minmod)
+  int learned;  // cell sweeps use the learned-limiter slope (lim is then minmod); 2: the
+                // network's coefficients also at material interfaces
   int land;     // shorten the last step to end at tend
   double cfl, dtmin, dtmax, tend;
   int nbc;

@@ -529,10 +529,12 @@ int build_params(const mhg_mesh_desc* mesh, const mhg_options* opt, Params* Pout
     return MHG_E_ARG;
   if (opt->schema_csts || opt->with_newton || opt->sans_lagrange || !opt->with_projection ||
       !opt->is_euler_scheme || opt->projection_pente_borne || opt->conservation_energie_totale ||
-      (opt->projection_limiteur_id >= MHG_LIM_MINMODG && opt->projection_limiteur_id != MHG_LIM_LEARNED) ||
+      (opt->projection_limiteur_id >= MHG_LIM_MINMODG && opt->projection_limiteur_id != MHG_LIM_LEARNED &&
+       opt->projection_limiteur_id != MHG_LIM_LEARNED_ALL) ||
       opt->projection_limiteur_id < 0 || opt->ordre_projection < 1 || opt->ordre_projection > 2)
     return MHG_E_UNSUPPORTED;
-  const bool learned = opt->projection_limiteur_id == MHG_LIM_LEARNED;
+  const bool learned = opt->projection_limiteur_id == MHG_LIM_LEARNED ||
+                       opt->projection_limiteur_id == MHG_LIM_LEARNED_ALL;
   if (learned && mesh->dim != 2) return MHG_E_UNSUPPORTED;
   if (opt->precision != MHG_PREC_FP64 && opt->precision != MHG_PREC_FP32_BULK) return MHG_E_UNSUPPORTED;
   Params P{};
@@ -559,7 +561,7 @@ int build_params(const mhg_mesh_desc* mesh, const mhg_options* opt, Params* Pout
   P.remap_threshold = opt->remap_threshold;
   P.order2 = opt->ordre_projection - 1;
   P.lim = learned ? MHG_LIM_MINMOD : opt->projection_limiteur_id;
-  P.learned = learned ? 1 : 0;
+  P.learned = !learned ? 0 : (opt->projection_limiteur_id == MHG_LIM_LEARNED_ALL ? 2 : 1);
   P.pseudo_centree = opt->pseudo_centree;
   P.lc = opt->longueur_caracteristique;
   P.cfl = opt->cfl;

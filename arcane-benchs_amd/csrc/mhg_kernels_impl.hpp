@@ -25,11 +25,12 @@ __device__ inline T face_grad(bool inner, T phi, T phim, T dx) {
 // Learned-limiter slope of a cell (MHG_LIM_LEARNED): alpha (g_b + g_f) / 2,
 // kept if the face values phi + g db (back) and phi + g df (front) stay
 // within [min, max] of phi and the existing neighbours along the sweep,
-// otherwise the minmod slope.
+// otherwise the minmod slope; alpha < 0 selects the minmod slope outright.
 template <class R>
 __device__ inline R learned_grad(R alpha, R phim, R phi, R phip, bool back_in, bool front_in, R gb, R gf, R db,
                                  R df) {
   const R gcl = limited_grad<0, true, R, kBranchFree<R>>(gb, gf);
+  if (alpha < R(0)) return gcl;
   const R gc = alpha * (R(0.5) * (gb + gf));
   R mn = phi, mx = phi;
   if (back_in) {

@@ -261,8 +261,13 @@ __device__ inline float cell_channel(const Params& P, const LearnedArgsT<R>& A, 
 
 // One block: 32 cells.  Lanes l and l + 32 build cell l's features (the
 // first lane channels 0-2, the second channels 3-4 and the padding), then the
-// forward pass gives alpha = clamp(y, 0, 2).  EXPORT also writes the float
-// features and the per-channel scales.
+// forward pass gives alpha = clamp(y, 0, 2).  A cell whose 3 x 3
+// neighbourhood does not hold one and the same single environment gets
+// alpha = -1 (the classical limiter) unless P.learned == 2: at material
+// interfaces independent
+// steep slopes of the environments' volumes, masses and energies leave
+// slivers of extreme specific energy.  EXPORT also writes the float features
+// and the per-channel scales.
 template <class R, bool EXPORT>
 __global__ __launch_bounds__(64) void k_learned_alpha_2d(Params P, LearnedArgsT<R> A, MlpWeights W, float* Xout,
                                                          float* Sout) {
@@ -298,11 +303,23 @@ __global__ __launch_bounds__(64) void k_learned_alpha_2d(Params P, LearnedArgsT<
       L.x[cl][k] = 0;
       if (EXPORT && valid) Xout[c * kMlpIn + k] = 0.f;
     }
+  __shared__ int uniform[kMlpRows];
+  if (!half) {  // one pure environment over the 3 x 3 neighbourhood
+    const int m0 = valid ? A.mask[c] : 0;
+    bool u = valid && __popc(m0) == 1;
+    for (int dj = -1; dj <= 1 && u; ++dj)
+      for (int di = -1; di <= 1; ++di) {
+        const int ii = ci + di, jj = cj + dj;
+        if (ii < 0 || ii >= P.nx || jj < 0 || jj >= P.ny) continue;
+        if (A.mask[ii + (long)P.nx * jj] != m0) u = false;
+      }
+    uniform[cl] = u ? 1 : 0;
+  }
   __syncthreads();
   mlp_forward_tile(L, W);
   if (valid && A.alpha) {
     const float y = L.y[cl][half];
-    A.alpha[(long)half * P.nc + c] = (R)fminf(fmaxf(y, 0.f), 2.f);
+    A.alpha[(long)half * P.nc + c] = (uniform[cl] || P.learned == 2) ? (R)fminf(fmaxf(y, 0.f), 2.f) : R(-1);
   }
 }
 

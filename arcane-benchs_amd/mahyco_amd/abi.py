@@ -25,7 +25,7 @@ SURF = {"XMIN": 0, "XMAX": 1, "YMIN": 2, "YMAX": 3, "ZMIN": 4, "ZMAX": 5}
 EOS_PERFECT_GAS, EOS_STIFFENED_GAS = 0, 1
 LC_RACINE_CUBIQUE_VOLUME, LC_FACES_OPPOSEES = 0, 1
 PREC_FP64, PREC_FP32_BULK = 0, 1
-LIM_MINMOD, LIM_SUPERBEE, LIM_VANLEER, LIM_LEARNED = 0, 1, 2, 64
+LIM_MINMOD, LIM_SUPERBEE, LIM_VANLEER, LIM_LEARNED, LIM_LEARNED_ALL = 0, 1, 2, 64, 65
 
 
 class MeshDesc(C.Structure):

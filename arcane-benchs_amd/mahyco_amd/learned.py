@@ -231,8 +231,8 @@ def evaluate_tp(lib, params, t_end=1.0, base=(896, 384), factors=(1.0, 1.5, 2.0)
     """Relative L1 density error on the base grid against a plain run at
     ref_factor x the base resolution (block means): plain runs at base x
     factors and the learned run at the base resolution."""
-    def run(nx, ny, net=None):
-        kw = {"projection_limiteur_id": abi.LIM_LEARNED} if net is not None else {}
+    def run(nx, ny, net=None, mode=abi.LIM_LEARNED):
+        kw = {"projection_limiteur_id": mode} if net is not None else {}
         s = triple_point_to(lib, nx, ny, t_end, **kw)
         if net is not None:
             s.set_limiter_mlp(net)
@@ -264,12 +264,32 @@ def evaluate_tp(lib, params, t_end=1.0, base=(896, 384), factors=(1.0, 1.5, 2.0)
         out["plain"][f"{nx}x{ny}"] = {"l1": err, "steps": n, "time": t, "s": round(el, 2)}
         log(f"  plain {nx}x{ny}: L1 {err:.4e} ({n} steps, t = {t:.5f})")
     net = abi.Mlp(lib, params)
-    rho, t, n, el = run(bx, by, net)
-    net.close()
+    s = triple_point_to(lib, bx, by, t_end, projection_limiteur_id=abi.LIM_LEARNED)
+    s.set_limiter_mlp(net)
+    a0 = s.learned_alpha()
+    s.run(max(1, out["plain"][f"{bx}x{by}"]["steps"] // 2))
+    a1 = s.learned_alpha()
+    s.close()
+    out["alpha_stats"] = {"start": [float(np.mean(a0)), float(np.min(a0)), float(np.max(a0))],
+                          "midway": [float(np.mean(a1)), float(np.percentile(a1, 5)), float(np.percentile(a1, 95))]}
+    log(f"  alpha mean/p5/p95 midway: {out['alpha_stats']['midway']}")
     refb = block_mean(ref, rx, ry, ref_factor)
-    err = float(np.sum(np.abs(rho - refb)) / np.sum(np.abs(refb)))
-    out["learned"] = {f"{bx}x{by}": {"l1": err, "steps": n, "time": t, "s": round(el, 2)}}
-    log(f"  learned {bx}x{by}: L1 {err:.4e} ({n} steps, t = {t:.5f})")
+    out["learned"] = {}
+    for name, mode in [("interfaces_minmod", abi.LIM_LEARNED), ("everywhere", abi.LIM_LEARNED_ALL)]:
+        rho, t, n, el = run(bx, by, net, mode)
+        err = float(np.sum(np.abs(rho - refb)) / np.sum(np.abs(refb)))
+        out["learned"][name] = {"mesh": [bx, by], "l1": err, "steps": n, "time": t, "s": round(el, 2)}
+        log(f"  learned ({name}) {bx}x{by}: L1 {err:.4e} ({n} steps, t = {t:.5f})")
+    net.close()
+    # plain-method resolution factor with the same error (power law through
+    # the base and 1.5x plain errors)
+    e0 = out["plain"][f"{bx}x{by}"]["l1"]
+    f15 = [k for k in out["plain"] if k != f"{bx}x{by}"][0]
+    e1 = out["plain"][f15]["l1"]
+    order = np.log(e0 / e1) / np.log(1.5)
+    out["convergence_order_plain"] = float(order)
+    for v in out["learned"].values():
+        v["equivalent_plain_resolution_factor"] = float((e0 / v["l1"]) ** (1.0 / order))
     return out
 
 
@@ -292,8 +312,14 @@ def main(argv=None):
     ap.add_argument("--eval-t", type=float, default=1.0)
     ap.add_argument("--eval-base", type=int, nargs=2, default=[896, 384])
     ap.add_argument("--no-eval", action="store_true")
+    ap.add_argument("--eval-only", metavar="NPZ", help="evaluate saved parameters, no training")
     a = ap.parse_args(argv)
     lib = load_gpu()
+    if a.eval_only:
+        params = np.load(a.eval_only)["params"]
+        ev = evaluate_tp(lib, params, a.eval_t, base=tuple(a.eval_base), log=lambda s: print(s, flush=True))
+        print(json.dumps(ev))
+        return 0
     t0 = time.perf_counter()
     print(f"data: {a.problems} four-quadrant problems at {a.fine}^2, restricted {REFINE}x", flush=True)
     X, T, info = make_dataset(lib, a.problems, a.fine, a.chunk, a.per_problem,

@@ -187,6 +187,25 @@ def batch_run(gpu, args, sync):
     return out
 
 
+def learned_run(gpu, args):
+    """Learned form: the network trained by mahyco_amd.learned (weights in
+    mahyco_amd/data/limiter_mlp.npz) on the triple point at 896x384 against
+    the plain method at 1x, 1.5x and 2x that resolution; errors are relative
+    L1 densities on the 896x384 grid against a plain run at 4x (block means),
+    every run ending at t exactly (land_on_final_time)."""
+    from mahyco_amd import learned
+    path = os.path.join(ROOT, "arcane-benchs_amd", "mahyco_amd", "data", "limiter_mlp.npz")
+    d = np.load(path)
+    params = d["params"]
+    train = json.loads(str(d["summary"]))
+    ev = learned.evaluate_tp(gpu, params, args.learned_t, log=lambda s: print(s, file=sys.stderr))
+    keep = ["problems", "fine", "samples", "epochs", "batch", "lr", "validation_loss", "validation_loss_central",
+            "validation_loss_minmod", "validation_loss_first_order", "data_s", "train_s", "fine_zone_cycles",
+            "fine_run_s"]
+    return {"evaluation": ev, "training": {k: train[k] for k in keep if k in train},
+            "network": "128-64-64-2 ReLU, BF16 weights, FP32 accumulation (MFMA)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -202,6 +221,8 @@ def main():
     ap.add_argument("--no-p3d", action="store_true", help="skip the 3D random-field case")
     ap.add_argument("--batch-n", type=int, default=1024, help="problems of the Riemann batch")
     ap.add_argument("--no-batch", action="store_true", help="skip the batching form")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned form")
+    ap.add_argument("--learned-t", type=float, default=0.5, help="final time of the learned-form evaluation")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device with torch: the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counts to bytes")
@@ -246,6 +267,7 @@ def main():
     value = nc * args.steps / elapsed
     p3 = None if args.no_p3d else p3d_run(gpu, args, sync)
     pb = None if args.no_batch else batch_run(gpu, args, sync)
+    pl = None if args.no_learned else learned_run(gpu, args)
     if pb is not None:
         pb["fp32_vs_triple_point_fp64"] = pb["fp32"]["value"] / value
 
@@ -294,6 +316,7 @@ def main():
         "precision_form": p32,
         "precision_form_3d": p3,
         "batch_form": pb,
+        "learned_form": pl,
     }
     print(json.dumps(out))
 

@@ -89,9 +89,10 @@ enum { MHG_LC_RACINE_CUBIQUE_VOLUME = 0, MHG_LC_FACES_OPPOSEES = 1 };
  * alpha_dir(c) * (g_back + g_front) / 2, with alpha_dir(c) a per-cell,
  * per-direction coefficient, and fall back to the minmod-limited slope
  * wherever that slope puts a face value of the cell outside the min/max of
- * the cell and its neighbours along the sweep; the dual sweeps use minmod. */
+ * the cell and its neighbours along the sweep, and wherever alpha_dir(c) < 0;
+ * the dual sweeps use minmod. */
 enum { MHG_LIM_MINMOD = 0, MHG_LIM_SUPERBEE = 1, MHG_LIM_VANLEER = 2, MHG_LIM_MINMODG = 3,
-       MHG_LIM_LEARNED = 64 };
+       MHG_LIM_LEARNED = 64, MHG_LIM_LEARNED_ALL = 65 };
 
 /* Arithmetic of the bulk fields. */
 enum { MHG_PREC_FP64 = 0, MHG_PREC_FP32_BULK = 1 };
@@ -269,7 +270,10 @@ int mhg_learned_alpha(mhg_ctx* ctx, double* alpha);
  * environment volume fraction, each channel centred on the cell's value and
  * divided by its largest deviation over the stencil (cells beyond the domain
  * repeat the boundary cell), channel-major, zero-padded to 128.  Output:
- * alpha_x, alpha_y, clamped to [0, 2] where they are used.  Parameters, FP32:
+ * alpha_x, alpha_y, clamped to [0, 2]; with MHG_LIM_LEARNED a cell whose
+ * 3 x 3 neighbourhood is not one and the same single environment gets -1
+ * (minmod at material interfaces), with MHG_LIM_LEARNED_ALL every cell takes
+ * the network's value.  Parameters, FP32:
  * W1 [64][128], b1 [64], W2 [64][64], b2 [64], W3 [2][64], b3 [2]. */
 typedef struct mhg_mlp mhg_mlp;
 int mhg_mlp_create(mhg_mlp** out);

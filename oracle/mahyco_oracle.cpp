@@ -81,12 +81,14 @@ inline double limiter(int id, double r) {
 // central slope alpha (g_b + g_f) / 2, kept if the face values it gives,
 // phi + g db at the back face and phi + g df at the front face, lie within
 // [min, max] of the cell's phi and its existing neighbours' along the sweep;
-// otherwise the minmod slope (RemapADIService.cc:497-506 with Lim = minmod).
+// otherwise, and for alpha < 0, the minmod slope (RemapADIService.cc:497-506
+// with Lim = minmod).
 inline double learned_grad(double alpha, double phim, double phi, double phip, bool back_in, bool front_in,
                            double gb, double gf, double db, double df) {
   double gcl = 0.;
   if (gb != 0.) gcl += 0.5 * (limiter(0, gf / gb) * gb);
   if (gf != 0.) gcl += 0.5 * (limiter(0, gb / gf) * gf);
+  if (alpha < 0.) return gcl;
   const double gc = alpha * (0.5 * (gb + gf));
   double mn = phi, mx = phi;
   if (back_in) {
@@ -103,11 +105,14 @@ inline double learned_grad(double alpha, double phim, double phi, double phip, b
 
 // Limiter of the classical branches: the options' limiter, minmod in the
 // learned mode (its fallback and its dual sweeps).
+inline bool is_learned(const mhg_options& o) {
+  return o.projection_limiteur_id == MHG_LIM_LEARNED || o.projection_limiteur_id == MHG_LIM_LEARNED_ALL;
+}
 inline int classical_limiter(const mhg_options& o) {
-  return o.projection_limiteur_id == MHG_LIM_LEARNED ? MHG_LIM_MINMOD : o.projection_limiteur_id;
+  return is_learned(o) ? MHG_LIM_MINMOD : o.projection_limiteur_id;
 }
 inline bool has_limited_gradients(const mhg_options& o) {
-  return o.projection_limiteur_id < MHG_LIM_MINMODG || o.projection_limiteur_id == MHG_LIM_LEARNED;
+  return o.projection_limiteur_id < MHG_LIM_MINMODG || is_learned(o);
 }
 
 // compute_eint, MahycoModule.cc:1630-1654.
@@ -983,7 +988,7 @@ void computeGradPhiCell(mho_ctx& S, int idir) {
   const int nv = S.nvar;
   std::fill(S.grad_phi.begin(), S.grad_phi.end(), 0.);
   if (S.opt.ordre_projection > 1 && S.opt.projection_pente_borne == 0 &&
-      S.opt.projection_limiteur_id == MHG_LIM_LEARNED) {
+      is_learned(S.opt)) {
 #pragma omp parallel for schedule(static)
     for (long c = 0; c < S.NC; ++c) {
       long bf, ff, pb, nb, pf, nf;

@@ -39,6 +39,9 @@ def test_learned_grad_closed_forms(lib):
     assert lg(lib, 1.0, 0, 2, 3, 0, 1, 0, 1) == 0.0
     # NaN coefficient: the comparison fails, minmod
     assert lg(lib, float("nan"), 1, 2, 3, 1, 1, 1, 1) == 1.0
+    # negative coefficient: minmod outright (unequal slopes 1, 3: minmod 1)
+    assert lg(lib, -1.0, 1, 2, 5, 1, 1, 1, 3) == 1.0
+    assert lg(lib, -1.0, 1, 3, 2, 1, 1, 2, -1) == 0.0
     # non-uniform faces: db = -0.25, df = 0.75, linear data: accepted up to the bounds
     assert lg(lib, 1.0, 1, 2, 3, 1, 1, 1, 1, -0.25, 0.75) == 1.0
     assert lg(lib, 1.5, 1, 2, 3, 1, 1, 1, 1, -0.25, 0.75) == 1.0   # 2 + 1.125 > 3
