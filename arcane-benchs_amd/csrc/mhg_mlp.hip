@@ -65,33 +65,38 @@ __device__ inline f32x16 zero16() {
   return z;
 }
 
-// Forward pass of the 32 rows staged in L.x: L.h1, L.h2 (bf16) and L.y.
-__device__ inline void mlp_forward_tile(MlpLds& L, const MlpWeights& W) {
+// Forward pass of the 32 rows staged in x (one wavefront): h1, h2 (bf16)
+// and y.  h2 may alias x (x is dead once h1 is formed).  Every wavefront of
+// the block calls it together (block barriers).
+__device__ inline void mlp_forward(const unsigned short (*x)[kXs], unsigned short (*h1)[kHs],
+                                   unsigned short (*h2)[kHs], float (*y)[kMlpOut], const MlpWeights& W) {
   const int l = threadIdx.x & 63;
   for (int nt = 0; nt < 2; ++nt) {  // H1 = relu(X W1^T + b1), B(k, j) = W1[32 nt + j][k]
-    const f32x16 acc = mfma_tile(zero16(), &L.x[0][0], kXs, 1, W.w1 + nt * 32 * kMlpIn, 1, kMlpIn, kMlpIn);
+    const f32x16 acc = mfma_tile(zero16(), &x[0][0], kXs, 1, W.w1 + nt * 32 * kMlpIn, 1, kMlpIn, kMlpIn);
     const int col = nt * 32 + (l & 31);
     const float b = W.b1[col];
 #pragma unroll
-    for (int v = 0; v < 16; ++v) L.h1[acc_row(v)][col] = f2bf(fmaxf(acc[v] + b, 0.f));
+    for (int v = 0; v < 16; ++v) h1[acc_row(v)][col] = f2bf(fmaxf(acc[v] + b, 0.f));
   }
   __syncthreads();
   for (int nt = 0; nt < 2; ++nt) {  // H2 = relu(H1 W2^T + b2)
-    const f32x16 acc = mfma_tile(zero16(), &L.h1[0][0], kHs, 1, W.w2 + nt * 32 * kMlpHid, 1, kMlpHid, kMlpHid);
+    const f32x16 acc = mfma_tile(zero16(), &h1[0][0], kHs, 1, W.w2 + nt * 32 * kMlpHid, 1, kMlpHid, kMlpHid);
     const int col = nt * 32 + (l & 31);
     const float b = W.b2[col];
 #pragma unroll
-    for (int v = 0; v < 16; ++v) L.h2[acc_row(v)][col] = f2bf(fmaxf(acc[v] + b, 0.f));
+    for (int v = 0; v < 16; ++v) h2[acc_row(v)][col] = f2bf(fmaxf(acc[v] + b, 0.f));
   }
   __syncthreads();
   {  // Y = H2 W3^T + b3: lane l -> row l % 32, output l / 32
     const int row = l & 31, o = l >> 5;
     float s = 0.f;
-    for (int k = 0; k < kMlpHid; ++k) s += bf2f(L.h2[row][k]) * bf2f(W.w3[o * kMlpHid + k]);
-    L.y[row][o] = s + W.b3[o];
+    for (int k = 0; k < kMlpHid; ++k) s += bf2f(h2[row][k]) * bf2f(W.w3[o * kMlpHid + k]);
+    y[row][o] = s + W.b3[o];
   }
   __syncthreads();
 }
+
+__device__ inline void mlp_forward_tile(MlpLds& L, const MlpWeights& W) { mlp_forward(L.x, L.h1, L.h2, L.y, W); }
 
 // Loss of the slope regression and its gradient: for each direction d and
 // channel k < 4, e = y_d sc - st; loss = sum e^2 / m; dL/dy_d = 2 sum_k e sc / m.
@@ -238,53 +243,70 @@ __global__ void k_mlp_to_bf16(const float* __restrict__ prm, unsigned short* __r
 }
 
 // ---- features of a mesh state and inference --------------------------------
-// Channel value of cell (ci, cj): density, specific internal energy, mean of
-// the 4 node velocities (x, y; in node order (i,j), (i+1,j), (i+1,j+1),
-// (i,j+1)), largest environment volume fraction (1 for a pure cell).
-template <class R>
-__device__ inline float cell_channel(const Params& P, const LearnedArgsT<R>& A, int k, int ci, int cj) {
-  const long c = ci + (long)P.nx * cj;
-  if (k == 0) return (float)A.rho[c];
-  if (k == 1) return (float)A.e[c];
-  if (k == 2 || k == 3) {
-    const R* u = k == 2 ? A.ux : A.uy;
-    const long n0 = ci + (long)(P.nx + 1) * cj;
-    return (float)(R(0.25) * (((u[n0] + u[n0 + 1]) + u[n0 + 1 + P.nx + 1]) + u[n0 + P.nx + 1]));
-  }
-  const int mask = A.mask[c];
-  if (__popc(mask) <= 1) return 1.f;
-  float f = 0.f;
-  for (int e = 0; e < P.ne; ++e)
-    if ((mask >> e) & 1) f = fmaxf(f, (float)A.env_f[(long)e * P.nc + c]);
-  return f;
-}
+// A block of 4 wavefronts owns a tile of 32 x 4 cells, one row per wavefront.
+// The five channels (density, specific internal energy, mean of the 4 node
+// velocities x and y in node order (i,j), (i+1,j), (i+1,j+1), (i,j+1),
+// largest environment volume fraction, 1 for a pure cell) and the membership
+// masks of the tile and a 2-cell halo are staged in LDS once, with indices
+// clamped to the mesh (cells beyond the domain repeat the boundary cell).
+// Lanes l and l + 32 of a wavefront build cell l's features (the first lane
+// channels 0-2, the second channels 3-4 and the padding); the forward pass
+// gives alpha = clamp(y, 0, 2).  A cell whose 3 x 3 neighbourhood does not
+// hold one and the same single environment gets alpha = -1 (the classical
+// limiter) unless P.learned == 2: at material interfaces independent steep
+// slopes of the environments' volumes, masses and energies leave slivers of
+// extreme specific energy.  EXPORT also writes the float features and the
+// per-channel scales.
+constexpr int kInfW = kMlpRows, kInfH = 4, kInfPW = kInfW + 4, kInfPH = kInfH + 4;
 
-// One block: 32 cells.  Lanes l and l + 32 build cell l's features (the
-// first lane channels 0-2, the second channels 3-4 and the padding), then the
-// forward pass gives alpha = clamp(y, 0, 2).  A cell whose 3 x 3
-// neighbourhood does not hold one and the same single environment gets
-// alpha = -1 (the classical limiter) unless P.learned == 2: at material
-// interfaces independent
-// steep slopes of the environments' volumes, masses and energies leave
-// slivers of extreme specific energy.  EXPORT also writes the float features
-// and the per-channel scales.
+struct InferLds {
+  float ch[kMlpChan][kInfPH][kInfPW];
+  int mask[kInfPH][kInfPW];
+  unsigned short x[kInfH][kMlpRows][kXs];   // inputs, then (aliased) the second hidden layer
+  unsigned short h1[kInfH][kMlpRows][kHs];
+  float y[kInfH][kMlpRows][kMlpOut];
+  int uniform[kInfH][kMlpRows];
+};
+
 template <class R, bool EXPORT>
-__global__ __launch_bounds__(64) void k_learned_alpha_2d(Params P, LearnedArgsT<R> A, MlpWeights W, float* Xout,
-                                                         float* Sout) {
-  __shared__ MlpLds L;
-  const int l = threadIdx.x;
+__global__ __launch_bounds__(64 * kInfH) void k_learned_alpha_2d(Params P, LearnedArgsT<R> A, MlpWeights W,
+                                                                 float* Xout, float* Sout) {
+  __shared__ InferLds L;
+  const int t = threadIdx.x;
+  const int ntx = (P.nx + kInfW - 1) / kInfW;
+  const int i0 = (int)(blockIdx.x % ntx) * kInfW, j0 = (int)(blockIdx.x / ntx) * kInfH;
+  // ---- stage the patch
+  for (int q = t; q < kInfPH * kInfPW; q += 64 * kInfH) {
+    const int pj = q / kInfPW, pi = q % kInfPW;
+    const int ci = min(max(i0 + pi - 2, 0), P.nx - 1), cj = min(max(j0 + pj - 2, 0), P.ny - 1);
+    const long c = ci + (long)P.nx * cj;
+    const long n0 = ci + (long)(P.nx + 1) * cj, nup = P.nx + 1;
+    L.ch[0][pj][pi] = (float)A.rho[c];
+    L.ch[1][pj][pi] = (float)A.e[c];
+    L.ch[2][pj][pi] = (float)(R(0.25) * (((A.ux[n0] + A.ux[n0 + 1]) + A.ux[n0 + 1 + nup]) + A.ux[n0 + nup]));
+    L.ch[3][pj][pi] = (float)(R(0.25) * (((A.uy[n0] + A.uy[n0 + 1]) + A.uy[n0 + 1 + nup]) + A.uy[n0 + nup]));
+    const int mask = A.mask[c];
+    float f = 1.f;
+    if (__popc(mask) > 1) {
+      f = 0.f;
+      for (int e = 0; e < P.ne; ++e)
+        if ((mask >> e) & 1) f = fmaxf(f, (float)A.env_f[(long)e * P.nc + c]);
+    }
+    L.ch[4][pj][pi] = f;
+    L.mask[pj][pi] = mask;
+  }
+  __syncthreads();
+  // ---- features of this wavefront's row
+  const int w = t >> 6, l = t & 63;
   const int cl = l & 31, half = l >> 5;
-  const long c = (long)blockIdx.x * kMlpRows + cl;
-  const bool valid = c < P.nc;
-  const int ci = valid ? (int)(c % P.nx) : 0, cj = valid ? (int)(c / P.nx) : 0;
+  const int ci = i0 + cl, cj = j0 + w;
+  const bool valid = ci < P.nx && cj < P.ny;
+  const long c = valid ? ci + (long)P.nx * cj : 0;
   const int k0 = half ? 3 : 0, k1 = half ? 5 : 3;
   for (int k = k0; k < k1; ++k) {
     float v[kMlpSten];
 #pragma unroll
-    for (int s = 0; s < kMlpSten; ++s) {
-      const int ii = min(max(ci + s % 5 - 2, 0), P.nx - 1), jj = min(max(cj + s / 5 - 2, 0), P.ny - 1);
-      v[s] = valid ? cell_channel(P, A, k, ii, jj) : 0.f;
-    }
+    for (int s = 0; s < kMlpSten; ++s) v[s] = L.ch[k][w + s / 5][cl + s % 5];
     const float ctr = v[12];
     float sc = 0.f;
 #pragma unroll
@@ -293,33 +315,28 @@ __global__ __launch_bounds__(64) void k_learned_alpha_2d(Params P, LearnedArgsT<
 #pragma unroll
     for (int s = 0; s < kMlpSten; ++s) {
       const float x = (v[s] - ctr) / S;
-      L.x[cl][k * kMlpSten + s] = f2bf(x);
+      L.x[w][cl][k * kMlpSten + s] = f2bf(x);
       if (EXPORT && valid) Xout[c * kMlpIn + k * kMlpSten + s] = x;
     }
     if (EXPORT && valid) Sout[c * kMlpChan + k] = S;
   }
-  if (half)
+  if (half) {
     for (int k = kMlpFeat; k < kMlpIn; ++k) {
-      L.x[cl][k] = 0;
+      L.x[w][cl][k] = 0;
       if (EXPORT && valid) Xout[c * kMlpIn + k] = 0.f;
     }
-  __shared__ int uniform[kMlpRows];
-  if (!half) {  // one pure environment over the 3 x 3 neighbourhood
-    const int m0 = valid ? A.mask[c] : 0;
-    bool u = valid && __popc(m0) == 1;
-    for (int dj = -1; dj <= 1 && u; ++dj)
-      for (int di = -1; di <= 1; ++di) {
-        const int ii = ci + di, jj = cj + dj;
-        if (ii < 0 || ii >= P.nx || jj < 0 || jj >= P.ny) continue;
-        if (A.mask[ii + (long)P.nx * jj] != m0) u = false;
-      }
-    uniform[cl] = u ? 1 : 0;
+  } else {  // one pure environment over the 3 x 3 neighbourhood
+    const int m0 = L.mask[w + 2][cl + 2];
+    bool u = __popc(m0) == 1;
+    for (int dj = 0; dj < 3; ++dj)
+      for (int di = 0; di < 3; ++di) u = u && L.mask[w + 1 + dj][cl + 1 + di] == m0;
+    L.uniform[w][cl] = u ? 1 : 0;
   }
   __syncthreads();
-  mlp_forward_tile(L, W);
+  mlp_forward(L.x[w], L.h1[w], reinterpret_cast<unsigned short(*)[kHs]>(&L.x[w][0][0]), L.y[w], W);
   if (valid && A.alpha) {
-    const float y = L.y[cl][half];
-    A.alpha[(long)half * P.nc + c] = (uniform[cl] || P.learned == 2) ? (R)fminf(fmaxf(y, 0.f), 2.f) : R(-1);
+    const float y = L.y[w][cl][half];
+    A.alpha[(long)half * P.nc + c] = (L.uniform[w][cl] || P.learned == 2) ? (R)fminf(fmaxf(y, 0.f), 2.f) : R(-1);
   }
 }
 
@@ -349,7 +366,7 @@ void launch_mlp_to_bf16(hipStream_t st, const float* prm, unsigned short* wb) {
 template <class R>
 void launch_learned_alpha_2d(hipStream_t st, const Params& P, const LearnedArgsT<R>& A, const MlpWeights& W,
                              float* Xout, float* Sout) {
-  const dim3 g((unsigned)((P.nc + kMlpRows - 1) / kMlpRows)), b(64);
+  const dim3 g((unsigned)(((P.nx + kInfW - 1) / kInfW) * ((P.ny + kInfH - 1) / kInfH))), b(64 * kInfH);
   if (Xout) hipLaunchKernelGGL((k_learned_alpha_2d<R, true>), g, b, 0, st, P, A, W, Xout, Sout);
   else hipLaunchKernelGGL((k_learned_alpha_2d<R, false>), g, b, 0, st, P, A, W, Xout, Sout);
 }
