@@ -243,40 +243,150 @@ __global__ void k_mlp_to_bf16(const float* __restrict__ prm, unsigned short* __r
 }
 
 // ---- features of a mesh state and inference --------------------------------
-// A block of 4 wavefronts owns a tile of 32 x 4 cells, one row per wavefront.
-// The five channels (density, specific internal energy, mean of the 4 node
-// velocities x and y in node order (i,j), (i+1,j), (i+1,j+1), (i,j+1),
-// largest environment volume fraction, 1 for a pure cell) and the membership
-// masks of the tile and a 2-cell halo are staged in LDS once, with indices
-// clamped to the mesh (cells beyond the domain repeat the boundary cell).
-// Lanes l and l + 32 of a wavefront build cell l's features (the first lane
-// channels 0-2, the second channels 3-4 and the padding); the forward pass
-// gives alpha = clamp(y, 0, 2).  A cell whose 3 x 3 neighbourhood does not
-// hold one and the same single environment gets alpha = -1 (the classical
-// limiter) unless P.learned == 2: at material interfaces independent steep
-// slopes of the environments' volumes, masses and energies leave slivers of
-// extreme specific energy.  EXPORT also writes the float features and the
+// A block of 8 wavefronts owns a tile of 32 x 8 cells, one 32-cell row per
+// wavefront.  The block stages in LDS, once: the network's BF16
+// weights (rows padded by 4 words so that the 32 rows of an MFMA operand
+// fall on distinct banks) and biases, and the five channels (density,
+// specific internal energy, mean of the 4 node velocities x and y in node
+// order (i,j), (i+1,j), (i+1,j+1), (i,j+1), largest environment volume
+// fraction, 1 for a pure cell) and the membership masks of the tile and a
+// 2-cell halo, with indices clamped to the mesh (cells beyond the domain
+// repeat the boundary cell).
+//
+// The layers run transposed, H^T = W X^T, on v_mfma_f32_32x32x16_bf16 (lane
+// l holds A(l % 32, 8 h + t) and B(8 h + t, l % 32), h = l / 32, t = 0..7),
+// so that one layer's accumulator is the next layer's B operand: lane l holds
+// column l % 32 (a cell) and rows 8 (v >> 2) + 4 h + (v & 3) of each 32-row
+// accumulator tile, i.e. for 16-row block m the rows 16 m + 4 h + (0..3) and
+// 16 m + 8 + 4 h + (0..3) (elements v = 8 (m % 2) + t of tile m / 2).  The
+// order of the inputs inside a product is free as long as the weight columns
+// follow it, so W2 and W3 are staged with the middle two 4-column chunks of
+// every 16 swapped (perm16).  The first B operand, the features, is built by
+// each lane for its own cell and its own 64 inputs: lanes 0-31 take channels
+// 0, 1 and stencil entries 0-13 of channel 2, lanes 32-63 channels 3, 4 and
+// entries 14-24 of channel 2 (then the 3 padding inputs), so that for 50 of
+// their 64 inputs the two halves of a wavefront read LDS words a fixed
+// distance apart, and each half computes the centre and scale of 3 channels
+// only (inf_feature); W1 is staged in that order.
+//
+// alpha = clamp(y, 0, 2).  A cell whose 3 x 3 neighbourhood does not hold one
+// and the same single environment gets alpha = -1 (the classical limiter)
+// unless P.learned == 2: at material interfaces independent steep slopes of
+// the environments' volumes, masses and energies leave slivers of extreme
+// specific energy.  EXPORT also writes the float features and the
 // per-channel scales.
-constexpr int kInfW = kMlpRows, kInfH = 4, kInfPW = kInfW + 4, kInfPH = kInfH + 4;
+constexpr int kInfW = kMlpRows, kInfH = 8, kInfWaves = kInfH, kInfPW = kInfW + 4, kInfPH = kInfH + 4;
+constexpr int kW1s = kMlpIn + 8, kW2s = kMlpHid + 8;  // padded LDS weight rows (bf16 words, 16-byte multiples)
 
-struct InferLds {
+struct alignas(16) InferLds {
+  unsigned short w1[kMlpHid][kW1s];
+  unsigned short w2[kMlpHid][kW2s];
+  unsigned short w3[kMlpOut][kW2s];
+  float b1[kMlpHid], b2[kMlpHid], b3[4];
   float ch[kMlpChan][kInfPH][kInfPW];
   int mask[kInfPH][kInfPW];
-  unsigned short x[kInfH][kMlpRows][kXs];   // inputs, then (aliased) the second hidden layer
-  unsigned short h1[kInfH][kMlpRows][kHs];
-  float y[kInfH][kMlpRows][kMlpOut];
-  int uniform[kInfH][kMlpRows];
 };
 
+// LDS word offset of feature f (channel f / 25, stencil entry f % 25) from
+// the cell's (channel 0, top-left stencil) word.
+__host__ __device__ constexpr int feat_off(int f) {
+  return (f / kMlpSten) * kInfPH * kInfPW + ((f % kMlpSten) / 5) * kInfPW + (f % kMlpSten) % 5;
+}
+constexpr int kChanWords = kInfPH * kInfPW;
+
+// Network input (feature index, 0-127) held in slot m (0-63) of the lanes of
+// half h (0: lanes 0-31, 1: lanes 32-63) of a wavefront; slot m sits at
+// k = 16 (m / 8) + 8 h + m % 8 of the first product.
+__host__ __device__ constexpr int inf_feature(int h, int m) {
+  return m < 2 * kMlpSten ? (m / kMlpSten + 3 * h) * kMlpSten + m % kMlpSten
+         : h == 0         ? m
+         : m <= 60        ? m + 14
+                          : kMlpFeat + (m - 61);
+}
+// Source column of the 4-column chunk c of a staged W2 / W3 row: inside each
+// 16 columns the chunks are stored in the order 0, 2, 1, 3, the order of the
+// accumulator rows a lane holds.
+__host__ __device__ constexpr int perm16_chunk(int c) { return 4 * (c >> 2) + 2 * (c & 1) + ((c >> 1) & 1); }
+
+typedef __bf16 hbf16x2 __attribute__((ext_vector_type(2)));
+typedef float hf32x2 __attribute__((ext_vector_type(2)));
+typedef short hs16x2 __attribute__((ext_vector_type(2)));
+typedef short hs16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 hbf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ inline f32x16 mfma16(hs16x8 a, hs16x8 b, f32x16 acc) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(hbf16x8, a), __builtin_bit_cast(hbf16x8, b), acc,
+                                                 0, 0, 0);
+}
+
+// Two floats to two BF16 (round to nearest even, one v_cvt_pk_bf16_f32).
+__device__ inline hs16x2 bf16_pair(float a, float b) {
+  const hf32x2 f = {a, b};
+  return __builtin_bit_cast(hs16x2, __builtin_convertvector(f, hbf16x2));
+}
+
+// B fragment of a hidden tile for 16-row block m: relu(acc) in BF16 of
+// elements 8 (m % 2) + 0..7 of tile m / 2 (the bias is in the accumulator).
+// The ReLU acts on the rounded values as 16-bit integers (v_pk_max_i16 with
+// 0): rounding keeps the sign and the order, so relu(bf16(x)) = bf16(relu(x)).
+__device__ inline hs16x8 hidden_frag(const f32x16& acc, int half) {
+  const hs16x2 z = {0, 0};
+  hs16x8 h;
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    const hs16x2 v = __builtin_elementwise_max(bf16_pair(acc[8 * half + 2 * p], acc[8 * half + 2 * p + 1]), z);
+    h[2 * p] = v[0];
+    h[2 * p + 1] = v[1];
+  }
+  return h;
+}
+
+// Accumulator tile initialised with the bias of its rows (32 nt + acc_row).
+__device__ inline f32x16 bias_tile(const float* bias, int nt) {
+  const int q = (threadIdx.x & 63) >> 5;
+  f32x16 acc;
+#pragma unroll
+  for (int jj = 0; jj < 4; ++jj) {
+    const float4 b = *reinterpret_cast<const float4*>(bias + 32 * nt + 8 * jj + 4 * q);
+    acc[4 * jj + 0] = b.x;
+    acc[4 * jj + 1] = b.y;
+    acc[4 * jj + 2] = b.z;
+    acc[4 * jj + 3] = b.w;
+  }
+  return acc;
+}
+
 template <class R, bool EXPORT>
-__global__ __launch_bounds__(64 * kInfH) void k_learned_alpha_2d(Params P, LearnedArgsT<R> A, MlpWeights W,
-                                                                 float* Xout, float* Sout) {
+__global__ __launch_bounds__(64 * kInfWaves) void k_learned_alpha_2d(Params P, LearnedArgsT<R> A, MlpWeights W,
+                                                                     float* Xout, float* Sout) {
   __shared__ InferLds L;
   const int t = threadIdx.x;
   const int ntx = (P.nx + kInfW - 1) / kInfW;
   const int i0 = (int)(blockIdx.x % ntx) * kInfW, j0 = (int)(blockIdx.x / ntx) * kInfH;
+  // ---- stage the weights and biases
+  {  // W1 with its columns in the lanes' input order: one column per thread
+    static_assert(64 * kInfWaves % kMlpIn == 0, "whole columns per pass");
+    const int k = t % kMlpIn, src = inf_feature((k >> 3) & 1, 8 * (k >> 4) + (k & 7));
+#pragma unroll 4
+    for (int r = t / kMlpIn; r < kMlpHid; r += 64 * kInfWaves / kMlpIn) L.w1[r][k] = W.w1[r * kMlpIn + src];
+  }
+  for (int q = t; q < kMlpHid * (kMlpHid / 4); q += 64 * kInfWaves) {
+    const int r = q / (kMlpHid / 4), c4 = q % (kMlpHid / 4);
+    *reinterpret_cast<uint2*>(&L.w2[r][4 * c4]) =
+        *reinterpret_cast<const uint2*>(W.w2 + r * kMlpHid + 4 * perm16_chunk(c4));
+  }
+  if (t < kMlpOut * (kMlpHid / 4)) {
+    const int r = t / (kMlpHid / 4), c4 = t % (kMlpHid / 4);
+    *reinterpret_cast<uint2*>(&L.w3[r][4 * c4]) =
+        *reinterpret_cast<const uint2*>(W.w3 + r * kMlpHid + 4 * perm16_chunk(c4));
+  }
+  if (t < kMlpHid) {
+    L.b1[t] = W.b1[t];
+    L.b2[t] = W.b2[t];
+  }
+  if (t < kMlpOut) L.b3[t] = W.b3[t];
   // ---- stage the patch
-  for (int q = t; q < kInfPH * kInfPW; q += 64 * kInfH) {
+  for (int q = t; q < kInfPH * kInfPW; q += 64 * kInfWaves) {
     const int pj = q / kInfPW, pi = q % kInfPW;
     const int ci = min(max(i0 + pi - 2, 0), P.nx - 1), cj = min(max(j0 + pj - 2, 0), P.ny - 1);
     const long c = ci + (long)P.nx * cj;
@@ -296,47 +406,100 @@ __global__ __launch_bounds__(64 * kInfH) void k_learned_alpha_2d(Params P, Learn
     L.mask[pj][pi] = mask;
   }
   __syncthreads();
-  // ---- features of this wavefront's row
   const int w = t >> 6, l = t & 63;
-  const int cl = l & 31, half = l >> 5;
-  const int ci = i0 + cl, cj = j0 + w;
-  const bool valid = ci < P.nx && cj < P.ny;
-  const long c = valid ? ci + (long)P.nx * cj : 0;
-  const int k0 = half ? 3 : 0, k1 = half ? 5 : 3;
-  for (int k = k0; k < k1; ++k) {
-    float v[kMlpSten];
+  const int cl = l & 31, q = l >> 5;
+  {
+    const int row = w;
+    const int ci = i0 + cl, cj = j0 + row;
+    const bool valid = ci < P.nx && cj < P.ny;
+    const long c = valid ? ci + (long)P.nx * cj : 0;
+    const float* base = &L.ch[0][row][cl];
+    // ---- centre and scale of this half's channels: 3 q and 3 q + 1 (through
+    // bq), and channel 2
+    const float* bq = base + 3 * q * kChanWords;
+    float ctr[3], S[3], inv[3];
 #pragma unroll
-    for (int s = 0; s < kMlpSten; ++s) v[s] = L.ch[k][w + s / 5][cl + s % 5];
-    const float ctr = v[12];
-    float sc = 0.f;
+    for (int it = 0; it < 3; ++it) {
+      const float* p = it < 2 ? bq + it * kChanWords : base + 2 * kChanWords;
+      const float cv = p[feat_off(12)];
+      float sc = 0.f;
 #pragma unroll
-    for (int s = 0; s < kMlpSten; ++s) sc = fmaxf(sc, fabsf(v[s] - ctr));
-    const float S = sc + (1e-6f * fabsf(ctr) + 1e-30f);
-#pragma unroll
-    for (int s = 0; s < kMlpSten; ++s) {
-      const float x = (v[s] - ctr) / S;
-      L.x[w][cl][k * kMlpSten + s] = f2bf(x);
-      if (EXPORT && valid) Xout[c * kMlpIn + k * kMlpSten + s] = x;
+      for (int s = 0; s < kMlpSten; ++s) sc = fmaxf(sc, fabsf(p[feat_off(s)] - cv));
+      ctr[it] = cv;
+      S[it] = sc + (1e-6f * fabsf(cv) + 1e-30f);
+      inv[it] = 1.f / S[it];
+      __builtin_amdgcn_sched_barrier(0);  // keep one channel's 25 loads in flight, not all 75
     }
-    if (EXPORT && valid) Sout[c * kMlpChan + k] = S;
-  }
-  if (half) {
-    for (int k = kMlpFeat; k < kMlpIn; ++k) {
-      L.x[w][cl][k] = 0;
-      if (EXPORT && valid) Xout[c * kMlpIn + k] = 0.f;
+    // ---- layer 1: H1^T = W1 X^T + b1, the features of this lane's slots
+    f32x16 acc1[2] = {bias_tile(L.b1, 0), bias_tile(L.b1, 1)};
+#pragma unroll
+    for (int j = 0; j < kMlpIn / 16; ++j) {
+      float xf[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int m = 8 * j + u;
+        float x;
+        if (m < 2 * kMlpSten) {
+          x = (bq[(m / kMlpSten) * kChanWords + feat_off(m % kMlpSten)] - ctr[m / kMlpSten]) * inv[m / kMlpSten];
+        } else {  // channel 2: entry m - 50 (lanes 0-31) or m - 36 (lanes 32-63, padding past 24)
+          const int s0 = m - 2 * kMlpSten, s1 = s0 + 14;
+          if (s1 < kMlpSten) {
+            const float v = base[2 * kChanWords + (q ? feat_off(s1) : feat_off(s0))];
+            x = (v - ctr[2]) * inv[2];
+          } else {
+            x = q ? 0.f : (base[2 * kChanWords + feat_off(s0)] - ctr[2]) * inv[2];
+          }
+        }
+        xf[u] = x;
+        if (EXPORT && valid) Xout[c * kMlpIn + (q ? inf_feature(1, m) : inf_feature(0, m))] = x;
+      }
+      hs16x8 xb;
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        const hs16x2 v = bf16_pair(xf[2 * p], xf[2 * p + 1]);
+        xb[2 * p] = v[0];
+        xb[2 * p + 1] = v[1];
+      }
+#pragma unroll
+      for (int nt = 0; nt < 2; ++nt)
+        acc1[nt] = mfma16(*reinterpret_cast<const hs16x8*>(&L.w1[32 * nt + cl][16 * j + 8 * q]), xb, acc1[nt]);
+      __builtin_amdgcn_sched_barrier(0);  // bounds the features held in registers to one k-step
     }
-  } else {  // one pure environment over the 3 x 3 neighbourhood
-    const int m0 = L.mask[w + 2][cl + 2];
-    bool u = __popc(m0) == 1;
-    for (int dj = 0; dj < 3; ++dj)
-      for (int di = 0; di < 3; ++di) u = u && L.mask[w + 1 + dj][cl + 1 + di] == m0;
-    L.uniform[w][cl] = u ? 1 : 0;
-  }
-  __syncthreads();
-  mlp_forward(L.x[w], L.h1[w], reinterpret_cast<unsigned short(*)[kHs]>(&L.x[w][0][0]), L.y[w], W);
-  if (valid && A.alpha) {
-    const float y = L.y[w][cl][half];
-    A.alpha[(long)half * P.nc + c] = (L.uniform[w][cl] || P.learned == 2) ? (R)fminf(fmaxf(y, 0.f), 2.f) : R(-1);
+    if (EXPORT && valid) {
+      Sout[c * kMlpChan + 3 * q] = S[0];
+      if (q == 0) Sout[c * kMlpChan + 1] = S[1], Sout[c * kMlpChan + 2] = S[2];
+      else Sout[c * kMlpChan + 4] = S[1];
+    }
+    // ---- layer 2: H2^T = W2 relu(H1^T) + b2
+    f32x16 acc2[2] = {bias_tile(L.b2, 0), bias_tile(L.b2, 1)};
+#pragma unroll
+    for (int j = 0; j < kMlpHid / 16; ++j) {
+      const hs16x8 hb = hidden_frag(acc1[j >> 1], j & 1);
+#pragma unroll
+      for (int nt = 0; nt < 2; ++nt)
+        acc2[nt] = mfma16(*reinterpret_cast<const hs16x8*>(&L.w2[32 * nt + cl][16 * j + 8 * q]), hb, acc2[nt]);
+    }
+    // ---- output layer: Y^T = W3 relu(H2^T), W3 padded to 32 rows with zeros
+    f32x16 acc3 = zero16();
+#pragma unroll
+    for (int j = 0; j < kMlpHid / 16; ++j) {
+      const hs16x8 hb = hidden_frag(acc2[j >> 1], j & 1);
+      hs16x8 a = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (cl < kMlpOut) a = *reinterpret_cast<const hs16x8*>(&L.w3[cl][16 * j + 8 * q]);
+      acc3 = mfma16(a, hb, acc3);
+    }
+    // lanes 0-31 hold rows 0 and 1 (y_x, y_y) of their cell in elements 0 and 1
+    if (q == 0 && valid) {
+      const int m0 = L.mask[row + 2][cl + 2];
+      bool uni = __popc(m0) == 1;
+      for (int dj = 0; dj < 3; ++dj)
+        for (int di = 0; di < 3; ++di) uni = uni && L.mask[row + 1 + dj][cl + 1 + di] == m0;
+      if (A.alpha) {
+        const bool use = uni || P.learned == 2;
+        A.alpha[c] = use ? (R)fminf(fmaxf(acc3[0] + L.b3[0], 0.f), 2.f) : R(-1);
+        A.alpha[P.nc + c] = use ? (R)fminf(fmaxf(acc3[1] + L.b3[1], 0.f), 2.f) : R(-1);
+      }
+    }
   }
 }
 
@@ -366,7 +529,7 @@ void launch_mlp_to_bf16(hipStream_t st, const float* prm, unsigned short* wb) {
 template <class R>
 void launch_learned_alpha_2d(hipStream_t st, const Params& P, const LearnedArgsT<R>& A, const MlpWeights& W,
                              float* Xout, float* Sout) {
-  const dim3 g((unsigned)(((P.nx + kInfW - 1) / kInfW) * ((P.ny + kInfH - 1) / kInfH))), b(64 * kInfH);
+  const dim3 g((unsigned)(((P.nx + kInfW - 1) / kInfW) * ((P.ny + kInfH - 1) / kInfH))), b(64 * kInfWaves);
   if (Xout) hipLaunchKernelGGL((k_learned_alpha_2d<R, true>), g, b, 0, st, P, A, W, Xout, Sout);
   else hipLaunchKernelGGL((k_learned_alpha_2d<R, false>), g, b, 0, st, P, A, W, Xout, Sout);
 }

@@ -34,12 +34,13 @@ struct MlpWeights {
 };
 
 __host__ __device__ inline unsigned short f2bf(float x) {
-  // round to nearest even (finite inputs; NaN stays NaN)
+  // round to nearest even (finite inputs; infinities stay, NaN stays NaN);
+  // written with selects only, so that device code has no branch per value
   unsigned int u;
   memcpy(&u, &x, 4);
-  if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (unsigned short)(u >> 16);
+  const unsigned int rounded = (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+  const unsigned int special = (u >> 16) | ((u & 0xffffu) ? 0x40u : 0u);
+  return (unsigned short)(((u & 0x7f800000u) == 0x7f800000u) ? special : rounded);
 }
 __host__ __device__ inline float bf2f(unsigned short h) {
   const unsigned int u = (unsigned int)h << 16;

@@ -30,14 +30,44 @@ STEP_KERNELS = {
 }
 
 
+# labels of the kernels outside the 2D single-mesh step (statistics only)
+OTHER_KERNELS = {
+    "node_lagrange_3d": ["k_node_lagrange_3d"],
+    "cell_lagrange_3d": ["k_cell_lagrange_3d"],
+    "cell_sweep_3d": ["k_cell_sweep_x_3d", "k_cell_sweep_tile_3d"],
+    "node_sweep_3d": ["k_node_sweep_x_3d", "k_node_sweep_tile_3d"],
+    "cell_final_3d": ["k_cell_final_3d"],
+    "batch_node_lagrange": ["kb_node_lagrange_2d"],
+    "batch_cell_lagrange": ["kb_cell_lagrange_2d"],
+    "batch_cell_sweep_x": ["kb_cell_sweep_x_2d"],
+    "batch_cell_sweep_y": ["kb_cell_sweep_y_2d"],
+    "batch_node_sweep_x": ["kb_node_sweep_x_2d"],
+    "batch_node_sweep_y": ["kb_node_sweep_y_2d"],
+    "batch_cell_final": ["kb_cell_final_2d"],
+    "batch_dt": ["kb_dt_finalize"],
+    "learned_alpha": ["k_learned_alpha_2d"],
+    "mlp": ["k_mlp_"],
+}
+
+
 def short(name):
     return name.split("(")[0].replace("void ", "").replace("mhg::", "")
 
 
+def untyped(kernel):
+    """Kernel name without the leading storage-type template argument
+    (k_x<double, 0, ...> -> k_x<0, ...>)."""
+    for t in ("double, ", "float, "):
+        kernel = kernel.replace("<" + t, "<")
+    return kernel.replace("<double>", "").replace("<float>", "")
+
+
 def label_of(kernel):
-    for lab, prefixes in STEP_KERNELS.items():
-        if any(kernel.startswith(p) for p in prefixes):
-            return lab
+    k = untyped(kernel)
+    for table in (STEP_KERNELS, OTHER_KERNELS):
+        for lab, prefixes in table.items():
+            if any(k.startswith(p) for p in prefixes):
+                return lab
     return ""
 
 
@@ -97,7 +127,7 @@ def main(argv):
         print("|---|---|")
         for lab, prefixes in STEP_KERNELS.items():
             pre = [tuned[lab]] if lab in tuned else prefixes
-            ks = [k for k in out if any(k.startswith(p) for p in pre)]
+            ks = [k for k in out if any(untyped(k).startswith(p) for p in pre) and "float" not in k]
             if not ks:
                 continue
             b = sum(out[k]["bytes"] for k in ks) / len(ks)

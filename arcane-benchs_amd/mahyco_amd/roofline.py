@@ -64,21 +64,32 @@ def last_node_sweep_extra(n_nodes, word=8):
 TRAFFIC_PROFILE = "triple_point_1792x768_fp64_hbm_traffic.md"
 
 
-def measured_traffic(label, profiles_dir):
-    """HBM bytes per launch of a step kernel on the 1792x768 triple point, as
-    measured with rocprofv3 FETCH_SIZE / WRITE_SIZE counters (copy-calibrated)
-    and recorded in the committed traffic profile; None if absent."""
+def measured_step_bytes(profiles_dir):
+    """HBM bytes of one fused triple-point step (every step kernel launches
+    once per step), summed from the committed traffic profile; None if absent."""
+    tab = _traffic_table(profiles_dir)
+    return sum(tab.values()) if tab else None
+
+
+def _traffic_table(profiles_dir):
     import os
     path = os.path.join(profiles_dir, TRAFFIC_PROFILE)
     if not os.path.exists(path):
         return None
-    in_table = False
+    tab, in_table = {}, False
     for line in open(path):
         if line.startswith("| label | HBM bytes per launch |"):
             in_table = True
             continue
-        if in_table and line.startswith("| "):
+        if in_table and line.startswith("| ") and not line.startswith("|---"):
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
-            if cells[0] == label:
-                return float(cells[1])
-    return None
+            tab[cells[0]] = float(cells[1])
+    return tab
+
+
+def measured_traffic(label, profiles_dir):
+    """HBM bytes per launch of a step kernel on the 1792x768 triple point, as
+    measured with rocprofv3 FETCH_SIZE / WRITE_SIZE counters (copy-calibrated)
+    and recorded in the committed traffic profile; None if absent."""
+    tab = _traffic_table(profiles_dir)
+    return None if not tab else tab.get(label)

@@ -285,6 +285,12 @@ def main():
               "avg_ms": round(avg_ms, 5), "launches": launches,
               "bytes_per_launch": b}
     step_frac = value * roofline.bytes_per_zone_cycle(NENV) / (roofline.PEAK_HBM_GBS * 1e9)
+    step_bytes = roofline.measured_step_bytes(os.path.join(ROOT, "profiles"))
+    step_meas = None if step_bytes is None else {
+        "bytes_per_step": step_bytes,
+        "frac": round(step_bytes / (elapsed / args.steps) / (roofline.PEAK_HBM_GBS * 1e9), 4),
+        "source": "FETCH_SIZE + WRITE_SIZE per step kernel, copy-calibrated, from the committed "
+                  "triple-point traffic profile, over this run's ms_per_step"}
 
     cpu, l1 = None, None
     if not args.no_cpu_baseline:
@@ -308,6 +314,7 @@ def main():
         "l1_density_err_vs_oracle": (None if l1 is None else
                                      {"value": l1, "after_steps": args.cpu_steps}),
         "step_roofline_frac": round(step_frac, 4),
+        "step_measured_hbm": step_meas,
         "kernel_ms": {k: round(v[0], 5) for k, v in kt.items()},
         "kernel_ms_source": "HIP events around each launch on the library stream, second pass "
                             "of the same K steps launched one kernel at a time",
