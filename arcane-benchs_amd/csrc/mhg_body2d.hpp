@@ -172,7 +172,7 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
   const int i = (D == 0) ? s : line, j = (D == 0) ? line : s;
   const int nnx = P.nx + 1;
   const long NC = P.nc;
-  const int ne = P.ne;
+  const int ne = env_count<NEM>(P);
   const double dt = S->dt;
   const R order2 = (R)P.order2;
   const long c = ex ? i + (long)P.nx * j : 0;
@@ -338,7 +338,7 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
   const int ta = r * TP + col;   // transposed LDS index
   const int ra = tr * TP + tc;   // row-major LDS index
   const long NC = P.nc;
-  const int ne = P.ne;
+  const int ne = env_count<NEM>(P);
   const double dt = S->dt;
   const R order2 = (R)P.order2;
   const bool o2 = P.order2 > 0;
@@ -587,7 +587,7 @@ __device__ __forceinline__ void node_sweep_x_2d_body(const Params& P, DevScalars
   const bool inner_t = (j >= 1 && j <= P.ny - 1);
   const bool inner_d = (i >= 1 && i <= N - 1);
   const long NC = P.nc;
-  const int ne = P.ne;
+  const int ne = env_count<NEM>(P);
   const double dt = S->dt;
   R U0 = R(0), U1 = R(0), U3 = R(0), p0 = R(0), p1 = R(0);
   double xd = 0.;
@@ -719,7 +719,7 @@ __device__ __forceinline__ void node_sweep_ytile_2d_body(const Params& P, DevSca
   const int ta = r * TP + col;   // transposed LDS index
   const int ra = tr * TP + tc;   // row-major LDS index
   const long NC = P.nc;
-  const int ne = P.ne;
+  const int ne = env_count<NEM>(P);
   const double dt = S->dt;
   // ---- row-major pass
   const int gi = i0 + tc, gs = s0 + tr;

@@ -386,4 +386,13 @@ struct Params {
   BcRange bc[kMaxBc];
 };
 
+// Environment count inside a kernel compiled for at most NEM environments.
+// The launchers choose NEM from {1, 2, 3, 4, 8} as the smallest bound >= ne,
+// so NEM = ne whenever NEM <= 4: the count is then a compile-time constant
+// and the per-environment guards (k < ne) fold away.
+template <int NEM>
+__host__ __device__ inline int env_count(const Params& P) {
+  return NEM <= 4 ? NEM : P.ne;
+}
+
 }  // namespace mhg

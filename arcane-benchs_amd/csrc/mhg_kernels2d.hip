@@ -97,7 +97,7 @@ __global__ __launch_bounds__(256) void k_cell_stage_2d(Params P, DevScalars* S, 
   if (S->stop || S->err) return;
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= P.nc) return;
-  const int ne = P.ne;
+  const int ne = env_count<NEM>(P);
   const long NC = P.nc;
   const int mask = A.mask[c];
   const int nbe = __popc(mask);
@@ -523,7 +523,7 @@ __global__ __launch_bounds__(64) void k_node_sweep_y_2d(Params P, DevScalars* S,
   const int j1 = min(j0 + STRIP, N + 1);  // output node rows [j0, j1)
   const bool inner_t = (i >= 1 && i <= P.nx - 1);
   const long NC = P.nc;
-  const int ne = P.ne;
+  const int ne = env_count<NEM>(P);
   const double dt = S->dt;
   const bool o2 = P.order2 > 0;
   const double order2 = (double)P.order2;

@@ -208,7 +208,7 @@ __global__ __launch_bounds__(256) void k_cell_sweep_x_3d(Params P, DevScalars* S
   const bool ex = (i >= 0 && i < N);
   const bool out = ex && lane >= 2 && lane <= 61;
   const long NC = P.nc;
-  const int ne = P.ne;
+  const int ne = env_count<NEM>(P);
   const double dt = S->dt;
   const R order2 = (R)P.order2;
   const long c = ex ? cid3(P, i, j, k) : 0;
@@ -307,7 +307,7 @@ __global__ __launch_bounds__(1024) void k_cell_sweep_tile_3d(Params P, DevScalar
   const int tr = t >> 4, tc = t & 15;
   const int w = t >> 6, lane = t & 63;
   const long NC = P.nc;
-  const int ne = P.ne;
+  const int ne = env_count<NEM>(P);
   const double dt = S->dt;
   const R order2 = (R)P.order2;
   const bool o2 = P.order2 > 0;
@@ -481,7 +481,7 @@ __device__ inline DualNode<R> dual_node_load(const Params& P, const NodeSweep3Ar
         const long cell = cid3(P, cc[0], cc[1], cc[2]);
 #pragma unroll
         for (int e = 0; e < NEM; ++e) {
-          if (e >= P.ne) break;
+          if (e >= env_count<NEM>(P)) break;
           f += A.contrib[e * P.nc + cell];
         }
       }

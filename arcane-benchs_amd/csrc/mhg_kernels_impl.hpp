@@ -59,7 +59,7 @@ __device__ inline void lagrange_cell_bulk(const Params& P, const DevScalars* S, 
   const R divu = qdiv(inv_dt * (qdiv(R(1.0), rho) - qdiv(R(1.0), rho_n)), tau);
   const R cs = A.cs[c];
   const int mask = A.mask[c];
-  const int ne = P.ne;
+  const int ne = env_count<NEM>(P);
   const long NC = P.nc;
   R uv[NEM], um[NEM], ue[NEM];
   R uq = R(0);
@@ -118,7 +118,7 @@ template <class R, int NEM>
 __device__ inline R final_cell(const Params& P, const CellFinalArgsT<R>& A, long c, double vold) {
   {
     const long NC = P.nc;
-    const int ne = P.ne;
+    const int ne = env_count<NEM>(P);
     const R thr = (R)P.remap_threshold;
     const R vol = (R)vold;
     R Uvol[NEM], Umas[NEM], Uene[NEM];
