@@ -233,15 +233,23 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
   for (int g = 0; g < 3; ++g)
 #pragma unroll
     for (int k = 0; k < NEM; ++k) pre[g][k] = (ex && k < ne) ? Uin[(g * ne + k) * NC + c] : R(0);
-  auto sweep_var = [&](R u0, R& unew, R& dual) {
+  EnvSlopes<R, NEM> es(P.remap_threshold);  // learned mode: consistent environment slopes
+  // role: 0 volume, 1 mass, 2 energy of environment k, 3 the other variable
+  auto sweep_var = [&](R u0, R& unew, R& dual, int role, int k) {
     const R phi = ex ? qdiv(u0, den) : R(0);
     const R phim = lane_up1(phi);
     const R gf = face_grad(inner, phi, phim, dxr);
     const R gfp = lane_down1(gf);
     R g = R(0);
     if (o2) {
-      if (LEARN) g = learned_grad(alpha, phim, phi, lane_down1(phi), inner, front_in, gf, gfp, dbr, dfr);
-      else g = limited_grad<LIM, true, R, kBranchFree<R>>(gf, gfp);
+      if (LEARN) {
+        bool used;
+        const R gl = learned_grad(es.trace(role, phi) ? R(-1) : alpha, phim, phi, lane_down1(phi), inner, front_in,
+                                  gf, gfp, dbr, dfr, &used);
+        g = es.slope(role, k, phi, gl, used, limited_grad<0, true, R, kBranchFree<R>>(gf, gfp));
+      } else {
+        g = limited_grad<LIM, true, R, kBranchFree<R>>(gf, gfp);
+      }
     }
     const R gm = lane_up1(g);
     const R phif = inner ? (upm ? phim + order2 * dotr * gm : phi + order2 * dotr * g) : R(0);
@@ -261,14 +269,15 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
   for (int k = 0; k < NEM; ++k) {
     if (k >= ne) break;
     R un, du;
-    sweep_var(pre[0][k], un, du);
+    sweep_var(pre[0][k], un, du, 0, k);
     if (out) st_stream(&Uout[k * NC + c], un);
   }
+  if (LEARN) es.close_volumes(ne, alpha);
 #pragma unroll
   for (int k = 0; k < NEM; ++k) {
     if (k >= ne) break;
     R un, du;
-    sweep_var(pre[1][k], un, du);
+    sweep_var(pre[1][k], un, du, 1, k);
     if (un < R(0)) un = R(0);  // RemapADIService.cc:953-955
     if (out) {
       st_stream(&Uout[(ne + k) * NC + c], un);
@@ -279,13 +288,13 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
   for (int k = 0; k < NEM; ++k) {
     if (k >= ne) break;
     R un, du;
-    sweep_var(pre[2][k], un, du);
+    sweep_var(pre[2][k], un, du, 2, k);
     if (un < R(0)) un = R(0);  // RemapADIService.cc:956-958
     if (out) st_stream(&Uout[(2 * ne + k) * NC + c], un);
   }
   {
     R un, du;
-    sweep_var(preq, un, du);
+    sweep_var(preq, un, du, 3, 0);
     if (out) st_stream(&Uout[(3 * ne) * NC + c], un);
   }
 }
@@ -402,16 +411,26 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
     dbr = (R)(xf - xc);
     dfr = (R)(lane_down1(xf) - xc);
   }
-  // one variable on the transposed layout: new U and half the dual flux
-  auto sweep_var = [&](R u0, R& un, R& dual) {
+  EnvSlopes<R, NEM> es(P.remap_threshold);  // learned mode: consistent environment slopes
+  // one variable v on the transposed layout: new U and half the dual flux
+  auto sweep_var = [&](R u0, R& un, R& dual, int v) {
     const R phi = ex ? qdiv(u0, den) : R(0);
     const R phim = lane_up1(phi);
     const R gf = face_grad(inner, phi, phim, dxr);
     const R gfp = lane_down1(gf);
     R g = R(0);
     if (o2) {
-      if (LEARN) g = learned_grad(alpha, phim, phi, lane_down1(phi), inner, front_in, gf, gfp, dbr, dfr);
-      else g = limited_grad<LIM, true, R, kBranchFree<R>>(gf, gfp);
+      if (LEARN) {
+        const int role = v < ne ? 0 : v < 2 * ne ? 1 : v < 3 * ne ? 2 : 3;
+        const int k = role == 0 ? v : role == 1 ? v - ne : role == 2 ? v - 2 * ne : 0;
+        if (role == 1 && v == ne) es.close_volumes(ne, alpha);  // every volume is done
+        bool used;
+        const R gl = learned_grad(es.trace(role, phi) ? R(-1) : alpha, phim, phi, lane_down1(phi), inner, front_in,
+                                  gf, gfp, dbr, dfr, &used);
+        g = es.slope(role, k, phi, gl, used, limited_grad<0, true, R, kBranchFree<R>>(gf, gfp));
+      } else {
+        g = limited_grad<LIM, true, R, kBranchFree<R>>(gf, gfp);
+      }
     }
     const R gm = lane_up1(g);
     const R phif = inner ? (upm ? phim + order2 * dotr * gm : phi + order2 * dotr * g) : R(0);
@@ -427,7 +446,7 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
   // variables two at a time (a, b = a + 1), one barrier pair per round
   R nxa = gex ? Uin[gc] : R(0);
   R nxb = (gex && nvu > 1) ? Uin[NC + gc] : R(0);
-  for (int va = 0; va < nvu; va += 2) {
+  auto round = [&](const int va) {
     const int vb = va + 1;
     const bool hb = vb < nvu;  // uniform
     if (trow) {
@@ -439,8 +458,8 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
     if (va + 3 < nvu) nxb = gex ? Uin[(va + 3) * NC + gc] : R(0);
     const R ua = tlane ? stg[0 * TS + ta] : R(0), ub = tlane ? stg[1 * TS + ta] : R(0);
     R una, dua, unb, dub;
-    sweep_var(ua, una, dua);
-    sweep_var(ub, unb, dub);
+    sweep_var(ua, una, dua, va);
+    sweep_var(ub, unb, dub, vb);
     if (va >= ne && va < 3 * ne && una < R(0)) una = R(0);  // RemapADIService.cc:953-958
     if (vb >= ne && vb < 3 * ne && unb < R(0)) unb = R(0);
     const bool ma = (va >= ne && va < 2 * ne), mb = hb && (vb >= ne && vb < 2 * ne);
@@ -457,6 +476,13 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
       if (ma) st_stream(&A.contrib[(va - ne) * NC + gc], stg[4 * TS + ra]);
       if (mb) st_stream(&A.contrib[(vb - ne) * NC + gc], stg[5 * TS + ra]);
     }
+  };
+  if (LEARN) {  // compile-time variable indices: the environment slopes stay in registers
+#pragma unroll
+    for (int va = 0; va < 3 * NEM + 1; va += 2)
+      if (va < nvu) round(va);
+  } else {
+    for (int va = 0; va < nvu; va += 2) round(va);
   }
 }
 

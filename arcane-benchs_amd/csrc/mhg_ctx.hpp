@@ -58,6 +58,7 @@ struct mhg_ctx {
   float* mlp_prm = nullptr;
   unsigned short* mlp_wb = nullptr;
   bool use_mlp = false;
+  double iface_alpha = -1.0;  // coefficient of material-interface cells (< 0: classical limiter)
   double* block_min = nullptr;
   mhg::DevScalars* S = nullptr;
   mhg::DevScalars* S_host = nullptr;  // pinned mirror

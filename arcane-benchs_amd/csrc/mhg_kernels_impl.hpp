@@ -28,8 +28,9 @@ __device__ inline T face_grad(bool inner, T phi, T phim, T dx) {
 // otherwise the minmod slope; alpha < 0 selects the minmod slope outright.
 template <class R>
 __device__ inline R learned_grad(R alpha, R phim, R phi, R phip, bool back_in, bool front_in, R gb, R gf, R db,
-                                 R df) {
+                                 R df, bool* used = nullptr) {
   const R gcl = limited_grad<0, true, R, kBranchFree<R>>(gb, gf);
+  if (used) *used = false;
   if (alpha < R(0)) return gcl;
   const R gc = alpha * (R(0.5) * (gb + gf));
   R mn = phi, mx = phi;
@@ -42,8 +43,52 @@ __device__ inline R learned_grad(R alpha, R phim, R phi, R phip, bool back_in, b
     mx = smax(mx, phip);
   }
   const R vb = phi + gc * db, vf = phi + gc * df;
-  return (vb >= mn && vb <= mx && vf >= mn && vf <= mx) ? gc : gcl;
+  const bool ok = vb >= mn && vb <= mx && vf >= mn && vf <= mx;
+  if (used) *used = ok && gc != R(0);
+  return ok ? gc : gcl;
 }
+
+// Consistent environment slopes of the learned mode, per lane.  In a mixed
+// cell (two or more environments with phi_V > 0) with alpha >= 0, the mass
+// and energy slopes of environment k follow its volume slope, g_m = (phi_m /
+// phi_V) g_V and g_E = (phi_E / phi_V) g_V, when a non-zero learned volume
+// slope was kept and phi_V exceeds the remap threshold; otherwise they take
+// the classical slope.  An environment's mass and energy face values are
+// then its volume face value times the upwind cell's density and energy
+// density, so no face carries mass or energy without volume.  The volume
+// slope of a trace environment (0 < phi_V <= remap threshold) is classical,
+// so that its three slopes are the classical method's.
+template <class R, int NEM>
+struct EnvSlopes {
+  R gV[NEM], pV[NEM];
+  bool uV[NEM];
+  bool cons = false;
+  R thr;
+  __device__ explicit EnvSlopes(double threshold) : thr((R)threshold) {}
+  // a trace environment's volume (slope forced classical)
+  __device__ __forceinline__ bool trace(int role, R phi) const { return role == 0 && phi > R(0) && phi <= thr; }
+  // slope of variable v (role: 0 volume, 1 mass, 2 energy, 3 other; k its
+  // environment) from its learned slope gl (kept: used) and classical gcl
+  __device__ __forceinline__ R slope(int role, int k, R phi, R gl, bool used, R gcl) {
+    R g = gl;
+    if (role == 0) {
+      gV[k] = gl;
+      pV[k] = phi;
+      uV[k] = used;
+    } else if ((role == 1 || role == 2) && cons) {
+      g = (uV[k] && pV[k] > thr) ? qdiv(phi, pV[k]) * gV[k] : gcl;
+    }
+    return g;
+  }
+  // after the volumes: a mixed cell with an active learned coefficient
+  __device__ __forceinline__ void close_volumes(int ne, R alpha) {
+    int present = 0;
+#pragma unroll
+    for (int k = 0; k < NEM; ++k)
+      if (k < ne && pV[k] > R(0)) ++present;
+    cons = alpha >= R(0) && present >= 2;
+  }
+};
 
 // Density, viscosity, energy and the remap vector of cell c from its
 // Lagrangian volume and characteristic length (both dimensions).

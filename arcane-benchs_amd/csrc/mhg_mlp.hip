@@ -496,8 +496,8 @@ __global__ __launch_bounds__(64 * kInfWaves) void k_learned_alpha_2d(Params P, L
         for (int di = 0; di < 3; ++di) uni = uni && L.mask[row + 1 + dj][cl + 1 + di] == m0;
       if (A.alpha) {
         const bool use = uni || P.learned == 2;
-        A.alpha[c] = use ? (R)fminf(fmaxf(acc3[0] + L.b3[0], 0.f), 2.f) : R(-1);
-        A.alpha[P.nc + c] = use ? (R)fminf(fmaxf(acc3[1] + L.b3[1], 0.f), 2.f) : R(-1);
+        A.alpha[c] = use ? (R)fminf(fmaxf(acc3[0] + L.b3[0], 0.f), 2.f) : (R)A.iface;
+        A.alpha[P.nc + c] = use ? (R)fminf(fmaxf(acc3[1] + L.b3[1], 0.f), 2.f) : (R)A.iface;
       }
     }
   }

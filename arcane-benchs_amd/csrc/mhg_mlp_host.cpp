@@ -94,6 +94,7 @@ void enqueue_learned_alpha(mhg_ctx* c, float* Xout, float* Sout, bool write_alph
   A.mask = c->mask;
   A.env_f = bp<R>(c->env_f);
   A.alpha = write_alpha ? bp<R>(c->alpha) : nullptr;
+  A.iface = c->iface_alpha;
   launch_learned_alpha_2d<R>(c->stream, c->P, A, mlp_weights(c->mlp_wb, c->mlp_prm), Xout, Sout);
 }
 template void enqueue_learned_alpha<double>(mhg_ctx*, float*, float*, bool);
@@ -259,6 +260,16 @@ int mhg_set_limiter_mlp(mhg_ctx* c, const mhg_mlp* n) {
   HIPCHK(c, hipMemcpy(c->mlp_wb, n->wb, kMlpParams * sizeof(unsigned short), hipMemcpyDeviceToDevice));
   c->use_mlp = true;
   ctx_drop_graphs(c);  // captured steps do not run the inference
+  return MHG_OK;
+}
+
+int mhg_set_learned_interface_alpha(mhg_ctx* c, double alpha) {
+  if (!c) return MHG_E_ARG;
+  if (!c->P.learned)
+    return ctx_fail(c, MHG_E_UNSUPPORTED, "set_learned_interface_alpha: the context's limiter is not MHG_LIM_LEARNED");
+  if (!(alpha <= 2.0)) return ctx_fail(c, MHG_E_ARG, "set_learned_interface_alpha: alpha must be <= 2 (< 0: classical)");
+  c->iface_alpha = alpha < 0.0 ? -1.0 : alpha;
+  ctx_drop_graphs(c);
   return MHG_OK;
 }
 

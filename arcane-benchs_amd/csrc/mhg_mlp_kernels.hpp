@@ -14,6 +14,7 @@ struct LearnedArgsT {
   const int* mask;
   const R* env_f;
   R* alpha;
+  double iface;  // coefficient of cells at material interfaces (< 0: classical limiter)
 };
 
 // Forward pass of rows [row0, row_end) of X ([rows][128] bf16, through perm

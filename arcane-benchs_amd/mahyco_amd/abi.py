@@ -149,6 +149,7 @@ class Library:
                 ("learned_alpha", C.c_int, [vp, _DP]),
                 ("learned_features", C.c_int, [vp, _FP, _FP]),
                 ("set_limiter_mlp", C.c_int, [vp, vp]),
+                ("set_learned_interface_alpha", C.c_int, [vp, C.c_double]),
                 ("mlp_create", C.c_int, [C.POINTER(vp)]),
                 ("mlp_destroy", C.c_int, [vp]),
                 ("mlp_last_error", C.c_char_p, [vp]),
@@ -367,6 +368,10 @@ class Simulation:
 
     def set_limiter_mlp(self, mlp):
         self._check(self.L.optional["set_limiter_mlp"](self.ctx, mlp.h), "set_limiter_mlp")
+
+    def set_learned_interface_alpha(self, alpha):
+        self._check(self.L.optional["set_learned_interface_alpha"](self.ctx, float(alpha)),
+                    "set_learned_interface_alpha")
 
     def conservation(self):
         out = np.zeros(3)

@@ -227,15 +227,19 @@ def block_mean(a, nx, ny, r):
 
 
 def evaluate_tp(lib, params, t_end=1.0, base=(896, 384), factors=(1.0, 1.5, 2.0), ref_factor=4,
-                log=print):
+                iface_alphas=(), log=print):
     """Relative L1 density error on the base grid against a plain run at
     ref_factor x the base resolution (block means): plain runs at base x
-    factors and the learned run at the base resolution."""
-    def run(nx, ny, net=None, mode=abi.LIM_LEARNED):
+    factors and the learned run at the base resolution (material interfaces
+    on minmod, network everywhere, and a fixed interface coefficient for each
+    of iface_alphas)."""
+    def run(nx, ny, net=None, mode=abi.LIM_LEARNED, iface=-1.0):
         kw = {"projection_limiteur_id": mode} if net is not None else {}
         s = triple_point_to(lib, nx, ny, t_end, **kw)
         if net is not None:
             s.set_limiter_mlp(net)
+            if iface >= 0:
+                s.set_learned_interface_alpha(iface)
         t0 = time.perf_counter()
         steps = s.run(1 << 30)
         el = time.perf_counter() - t0
@@ -275,8 +279,10 @@ def evaluate_tp(lib, params, t_end=1.0, base=(896, 384), factors=(1.0, 1.5, 2.0)
     log(f"  alpha mean/p5/p95 midway: {out['alpha_stats']['midway']}")
     refb = block_mean(ref, rx, ry, ref_factor)
     out["learned"] = {}
-    for name, mode in [("interfaces_minmod", abi.LIM_LEARNED), ("everywhere", abi.LIM_LEARNED_ALL)]:
-        rho, t, n, el = run(bx, by, net, mode)
+    variants = [("interfaces_minmod", abi.LIM_LEARNED, -1.0), ("everywhere", abi.LIM_LEARNED_ALL, -1.0)]
+    variants += [(f"interfaces_alpha_{a:g}", abi.LIM_LEARNED, a) for a in iface_alphas]
+    for name, mode, iface in variants:
+        rho, t, n, el = run(bx, by, net, mode, iface)
         err = float(np.sum(np.abs(rho - refb)) / np.sum(np.abs(refb)))
         out["learned"][name] = {"mesh": [bx, by], "l1": err, "steps": n, "time": t, "s": round(el, 2)}
         log(f"  learned ({name}) {bx}x{by}: L1 {err:.4e} ({n} steps, t = {t:.5f})")

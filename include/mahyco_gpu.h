@@ -297,6 +297,13 @@ int mhg_mlp_train(mhg_mlp* net, int32_t epochs, int32_t batch, float lr, uint64_
 /* Attaches a copy of the network to a learned-limiter context: every later
  * step first computes alpha from the state at its start. */
 int mhg_set_limiter_mlp(mhg_ctx* ctx, const mhg_mlp* net);
+/* Coefficient given, under MHG_LIM_LEARNED with a network attached, to the
+ * cells at material interfaces (whose 3 x 3 neighbourhood does not hold one
+ * and the same single environment): alpha < 0 (the default, -1) keeps the
+ * classical limiter there, 0 <= alpha <= 2 applies the slope alpha (g_b +
+ * g_f) / 2 with the same bounds check as the network's.  No counterpart in
+ * the reference. */
+int mhg_set_learned_interface_alpha(mhg_ctx* ctx, double alpha);
 /* Network inputs of every cell of a 2D context's current state, X [N_c][128],
  * and the channel scales (largest deviation plus 1e-6 |centre|), [N_c][5]
  * (may be NULL); the coefficients are not touched. */

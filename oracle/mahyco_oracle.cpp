@@ -83,11 +83,13 @@ inline double limiter(int id, double r) {
 // [min, max] of the cell's phi and its existing neighbours' along the sweep;
 // otherwise, and for alpha < 0, the minmod slope (RemapADIService.cc:497-506
 // with Lim = minmod).
+// *used tells whether a non-zero learned slope was kept.
 inline double learned_grad(double alpha, double phim, double phi, double phip, bool back_in, bool front_in,
-                           double gb, double gf, double db, double df) {
+                           double gb, double gf, double db, double df, bool* used = nullptr) {
   double gcl = 0.;
   if (gb != 0.) gcl += 0.5 * (limiter(0, gf / gb) * gb);
   if (gf != 0.) gcl += 0.5 * (limiter(0, gb / gf) * gf);
+  if (used) *used = false;
   if (alpha < 0.) return gcl;
   const double gc = alpha * (0.5 * (gb + gf));
   double mn = phi, mx = phi;
@@ -100,7 +102,40 @@ inline double learned_grad(double alpha, double phim, double phi, double phip, b
     mx = smax(mx, phip);
   }
   const double vb = phi + gc * db, vf = phi + gc * df;
-  return (vb >= mn && vb <= mx && vf >= mn && vf <= mx) ? gc : gcl;
+  const bool ok = vb >= mn && vb <= mx && vf >= mn && vf <= mx;
+  if (used) *used = ok && gc != 0.;
+  return ok ? gc : gcl;
+}
+
+// Consistent environment slopes of the learned mode (extension): in a mixed
+// cell (two or more environments with phi_V > 0) with alpha >= 0, the mass
+// and energy slopes of environment k follow its volume slope, scaled by the
+// environment's mass and energy per unit of its volume in the cell:
+//   g_m = (phi_m / phi_V) g_V,  g_E = (phi_E / phi_V) g_V,
+// when a non-zero learned volume slope was kept and phi_V exceeds the remap
+// threshold (so the ratios stay finite); otherwise they take the classical
+// (minmod) slope, so that alpha = 1e30 (every non-zero learned slope
+// rejected) is still minmod.  The volume slope of a trace environment (0 <
+// phi_V <= remap threshold) is classical too, so that its three slopes are
+// the classical method's (a learned volume slope beside classical mass and
+// energy slopes moves volume without mass, or mass without volume).  A face value of an
+// environment's mass or energy is then its volume face value times the
+// upwind cell's density or density times specific energy, so no face carries
+// mass or energy without volume (the source of sliver cells of extreme
+// specific energy when the three slopes are chosen independently).
+// phi, g: the cell's nv values and slopes; gcl: the classical slopes.
+inline void consistent_env_slopes(int ne, const double* phi, double* g, const double* gcl, const bool* used_v,
+                                  double threshold) {
+  for (int k = 0; k < ne; ++k) {
+    const int vm = ne + k, ve = 2 * ne + k;
+    if (used_v[k] && phi[k] > threshold) {
+      g[vm] = (phi[vm] / phi[k]) * g[k];
+      g[ve] = (phi[ve] / phi[k]) * g[k];
+    } else {
+      g[vm] = gcl[vm];
+      g[ve] = gcl[ve];
+    }
+  }
 }
 
 // Limiter of the classical branches: the options' limiter, minmod in the
@@ -998,13 +1033,26 @@ void computeGradPhiCell(mho_ctx& S, int idir) {
       const double db = dot(S.face_coord[bf] - S.cell_coord[c], S.face_normal[bf]);
       const double df = dot(S.face_coord[ff] - S.cell_coord[c], S.face_normal[ff]);
       const double alpha = S.learned_alpha.empty() ? 0. : S.learned_alpha[(size_t)idir * S.NC + c];
+      const double* ph = &S.phi_lagrange[c * nv];
+      double* g = &S.grad_phi[c * nv];
+      int present = 0;
+      for (int k = 0; k < S.nenv; ++k) present += ph[k] > 0. ? 1 : 0;
+      const bool cons = alpha >= 0. && present >= 2;
+      double gcl[3 * MHG_MAX_ENV + 8];
+      bool used[MHG_MAX_ENV] = {};
       for (int v = 0; v < nv; ++v) {
-        const double phi = S.phi_lagrange[c * nv + v];
+        const double phi = ph[v];
         const double phim = back_in ? S.phi_lagrange[pb * nv + v] : phi;
         const double phip = front_in ? S.phi_lagrange[nf * nv + v] : phi;
-        S.grad_phi[c * nv + v] = learned_grad(alpha, phim, phi, phip, back_in, front_in,
-                                              S.grad_phi_face[bf * nv + v], S.grad_phi_face[ff * nv + v], db, df);
+        const double gb = S.grad_phi_face[bf * nv + v], gf = S.grad_phi_face[ff * nv + v];
+        bool u = false;
+        // a trace environment (0 < phi_V <= remap threshold) keeps the classical slope
+        const bool trace = v < S.nenv && phi > 0. && phi <= S.opt.remap_threshold;
+        g[v] = learned_grad(trace ? -1. : alpha, phim, phi, phip, back_in, front_in, gb, gf, db, df, &u);
+        if (v < S.nenv) used[v] = u;
+        if (cons) gcl[v] = learned_grad(-1., phim, phi, phip, back_in, front_in, gb, gf, db, df);
       }
+      if (cons) consistent_env_slopes(S.nenv, ph, g, gcl, used, S.opt.remap_threshold);
     }
   } else if (S.opt.ordre_projection > 1 && S.opt.projection_pente_borne == 0 &&
              S.opt.projection_limiteur_id < MHG_LIM_MINMODG) {
@@ -1966,6 +2014,12 @@ double mho_probe_limiter(int32_t id, double r) { return limiter(id, r); }
 double mho_probe_learned_grad(double alpha, double phim, double phi, double phip, int32_t back_in,
                               int32_t front_in, double gb, double gf, double db, double df) {
   return learned_grad(alpha, phim, phi, phip, back_in != 0, front_in != 0, gb, gf, db, df);
+}
+
+void mho_probe_consistent_slopes(int32_t ne, const double* phi, double* g, const double* gcl, const int32_t* used) {
+  bool u[MHG_MAX_ENV] = {};
+  for (int k = 0; k < ne && k < MHG_MAX_ENV; ++k) u[k] = used[k] != 0;
+  consistent_env_slopes(ne, phi, g, gcl, u, 1e-10);
 }
 
 int mho_set_learned_alpha(mho_ctx* c, const double* alpha) {

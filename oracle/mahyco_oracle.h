@@ -67,6 +67,11 @@ double mho_probe_limiter(int32_t id, double r);
 /* Learned-limiter slope of one cell and variable (MHG_LIM_LEARNED). */
 double mho_probe_learned_grad(double alpha, double phim, double phi, double phip, int32_t back_in,
                               int32_t front_in, double gb, double gf, double db, double df);
+/* Consistent environment slopes of the learned mode in a mixed cell: phi and
+ * g [3 ne] (volumes, masses, energies), gcl the classical slopes, used[k]
+ * whether the learned volume slope of environment k was kept; g is updated
+ * in place (remap threshold 1e-10). */
+void mho_probe_consistent_slopes(int32_t ne, const double* phi, double* g, const double* gcl, const int32_t* used);
 /* Coefficients of the learned limiter, [dim][N_c] (kept until replaced). */
 int mho_set_learned_alpha(mho_ctx* ctx, const double* alpha);
 /* compute_eint (MahycoModule.cc:1630-1654). */

@@ -45,7 +45,8 @@ def test_oracle_exports_same_interface():
     gpu_only = ("mho_run_async", "mho_synchronize", "mho_kernel_times", "mho_autotune",
                 "mho_variant_count", "mho_select_variant")
     # the batch form, the network and the per-step inference exist on the GPU only
-    gpu_only += ("mho_learned_alpha", "mho_learned_features", "mho_set_limiter_mlp")
+    gpu_only += ("mho_learned_alpha", "mho_learned_features", "mho_set_limiter_mlp",
+                 "mho_set_learned_interface_alpha")
     missing = [n for n in names if n not in have and n not in gpu_only and not n.startswith("mho_batch_")
                and not n.startswith("mho_mlp_")]
     assert not missing, missing

@@ -140,6 +140,14 @@ def test_context_inference_and_step_match_oracle(gpu):
     assert np.any(ao < 0) and np.any(ao >= 0)
     print("alpha range", a.min(), a.max(), "mean |d|", np.mean(np.abs(a - ao)))
     assert np.mean(np.abs(a - ao)) < 1e-2
+    # a fixed coefficient at material interfaces instead of the classical limiter
+    g.set_learned_interface_alpha(0.7)
+    a07 = g.learned_alpha()
+    assert np.all(a07[ao < 0] == 0.7) and np.array_equal(a07[ao >= 0], a[ao >= 0])
+    with pytest.raises(abi.MahycoError):
+        g.set_learned_interface_alpha(2.5)
+    g.set_learned_interface_alpha(-1)
+    assert np.array_equal(g.learned_alpha(), a)
     o.set_learned_alpha(a)
     o.step()
     g.step()

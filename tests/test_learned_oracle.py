@@ -47,6 +47,44 @@ def test_learned_grad_closed_forms(lib):
     assert lg(lib, 1.5, 1, 2, 3, 1, 1, 1, 1, -0.25, 0.75) == 1.0   # 2 + 1.125 > 3
 
 
+def test_consistent_env_slopes_closed_forms(lib):
+    """Mixed cells of the learned mode: an environment's mass and energy
+    slopes follow its kept volume slope (g_m = (phi_m / phi_V) g_V, g_E =
+    (phi_E / phi_V) g_V), so its mass and energy face values are its volume
+    face value times rho and rho e; without a kept volume slope, or with
+    phi_V at or below the remap threshold (1e-10), they take the classical
+    slopes."""
+    f = lib.lib.mho_probe_consistent_slopes
+    f.restype = None
+    dp = np.ctypeslib.ndpointer(np.float64, flags="C")
+    f.argtypes = [C.c_int32, dp, dp, dp, np.ctypeslib.ndpointer(np.int32, flags="C")]
+    # three environments: volumes, masses (rho 2, 5, 3), energies (e 3, 4, 7)
+    V = np.array([0.6, 0.4, 0.0])
+    m = V * np.array([2.0, 5.0, 3.0])
+    E = m * np.array([3.0, 4.0, 7.0])
+    phi = np.concatenate([V, m, E])
+    gV = np.array([0.1, -0.2, 0.05])
+    g = np.concatenate([gV, [9.0] * 6])
+    gcl = np.arange(9, dtype=float) + 100.0
+    f(3, phi, g, gcl, np.array([1, 0, 1], dtype=np.int32))
+    assert g[3] == (m[0] / V[0]) * gV[0] and g[6] == (E[0] / V[0]) * gV[0]  # env 0 follows its volume
+    assert g[4] == gcl[4] and g[7] == gcl[7]                            # env 1: volume slope not kept
+    assert g[5] == gcl[5] and g[8] == gcl[8]                            # env 2: phi_V = 0
+    for d in (-0.5, 0.37):  # face values: mass = rho * volume, energy = rho e * volume
+        assert abs((m[0] + g[3] * d) - 2.0 * (V[0] + gV[0] * d)) < 1e-15
+        assert abs((E[0] + g[6] * d) - 6.0 * (V[0] + gV[0] * d)) < 1e-14
+    # a volume fraction at the remap threshold: classical; just above: follows
+    phi2 = phi.copy()
+    phi2[0] = 1e-10
+    g2 = np.concatenate([gV, [9.0] * 6])
+    f(3, phi2, g2, gcl, np.array([1, 1, 1], dtype=np.int32))
+    assert g2[3] == gcl[3] and g2[6] == gcl[6]
+    phi2[0] = 2e-10
+    g2 = np.concatenate([gV, [9.0] * 6])
+    f(3, phi2, g2, gcl, np.array([1, 1, 1], dtype=np.int32))
+    assert g2[3] == (m[0] / 2e-10) * gV[0] and g2[6] == (E[0] / 2e-10) * gV[0]
+
+
 def _state(s):
     return s.download(H.CELL_FIELDS + ["velocity", "node_coord", "env_mask"] + H.ENV_FIELDS)
 
