@@ -196,7 +196,6 @@ __device__ inline R final_cell(const Params& P, const CellFinalArgsT<R>& A, long
         if (fn > thr && Umas[k] != R(0)) mask |= (1 << k);
       }
     }
-    const int nbe = __popc(mask);
     // "moy"
     R masset = R(0);
     volt = R(0);
@@ -225,10 +224,17 @@ __device__ inline R final_cell(const Params& P, const CellFinalArgsT<R>& A, long
 #pragma unroll
     for (int k = 0; k < NEM; ++k) {
       if (!((mask >> k) & 1)) continue;
-      const R f = fv[k] * unsursomme_frac;
+      R f = fv[k] * unsursomme_frac;
+      // learned mode: an environment whose fraction ends at or below the
+      // threshold leaves the cell (no member with zero fraction and density)
+      if (P.learned && f <= thr) {
+        f = R(0);
+        mask &= ~(1 << k);
+      }
       if (f > R(0)) matcell++;
       fv[k] = f;
     }
+    const int nbe = __popc(mask);
 #pragma unroll
     for (int k = 0; k < NEM; ++k) {
       // a zero total mass leaves the mass fractions as they were (513-538)

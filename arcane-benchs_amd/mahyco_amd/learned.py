@@ -21,6 +21,13 @@ central slope into the sub-cell slope the refined solution has.
 Cells whose stencil carries no structure (relative variation below 1e-4 in
 every channel) are sampled at a tenth of the rate of the others; at most
 `per_problem` samples are kept per problem.
+
+With materials > 1 the quadrants belong to 2 or 3 perfect gases (each with
+its own gamma, uniform in [1.2, 1.67]; at least two different gases per
+problem), so the data holds material interfaces: the coarse volume fractions
+are the means of the fine ones, and the coarse state, with its membership
+masks and fractions, is uploaded to a context of as many environments before
+the feature kernel runs.
 """
 import time
 
@@ -32,7 +39,7 @@ QUAD_SEED = 20261205
 REFINE = 4
 
 
-def quadrant_specs(n, seed=QUAD_SEED):
+def quadrant_specs(n, seed=QUAD_SEED, materials=1):
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(n):
@@ -43,6 +50,14 @@ def quadrant_specs(n, seed=QUAD_SEED):
                           u=float(rng.uniform(-1.0, 1.0)), v=float(rng.uniform(-1.0, 1.0))))
         out.append(dict(quads=q, x0=float(rng.uniform(0.3, 0.7)), y0=float(rng.uniform(0.3, 0.7)),
                         gamma=float(rng.uniform(1.2, 1.67)), t_end=float(rng.uniform(0.05, 0.15))))
+    if materials > 1:  # gases of the quadrants from a second stream: the draws above are unchanged
+        rng2 = np.random.default_rng(seed + 7)
+        for sp in out:
+            env = rng2.integers(0, materials, 4)
+            while len(set(env.tolist())) < 2:
+                env = rng2.integers(0, materials, 4)
+            sp["env"] = [int(k) for k in env]
+            sp["gammas"] = [float(g) for g in rng2.uniform(1.2, 1.67, materials)]
     return out
 
 
@@ -61,13 +76,18 @@ def quadrant_fields(spec, n):
     XN, YN = np.meshgrid(xn, xn)
     qn = _quadrant(spec, XN, YN).ravel()
     vel = np.stack([np.array([q["u"] for q in spec["quads"]])[qn], np.array([q["v"] for q in spec["quads"]])[qn]])
-    return np.zeros(n * n, dtype=np.int32), rho, p, vel
+    env = np.array(spec.get("env", [0, 0, 0, 0]), dtype=np.int32)[qc]
+    return env, rho, p, vel
+
+
+def spec_gammas(spec):
+    return spec.get("gammas", [spec["gamma"]])
 
 
 def quadrant_mesh_options(lib, spec, n, **optkw):
     opt = lib.options(**optkw)
     abi.set_bcs(opt, abi.reflecting_walls(2))
-    abi.set_envs(opt, [(abi.EOS_PERFECT_GAS, spec["gamma"], 0.)])
+    abi.set_envs(opt, [(abi.EOS_PERFECT_GAS, g, 0.) for g in spec_gammas(spec)])
     opt.final_time = spec["t_end"]
     return abi.mesh_desc(2, [n, n], [0, 0], [1.0, 1.0]), opt
 
@@ -82,6 +102,13 @@ def restrict(n, rho, e, vel, r=REFINE):
     ec = (R * E).sum(axis=(1, 3)) / R.sum(axis=(1, 3))
     V = vel.reshape(2, n + 1, n + 1)[:, ::r, ::r]
     return rc.ravel(), ec.ravel(), V.reshape(2, -1)
+
+
+def restrict_fractions(n, frac, r=REFINE):
+    """Fine volume fractions [ne][n*n] -> coarse [ne][(n/r)^2] (block means)."""
+    m = n // r
+    f = frac.reshape(-1, m, r, m, r)
+    return f.mean(axis=(2, 4)).reshape(f.shape[0], -1)
 
 
 def fine_slopes(n, rho, e, vel, r=REFINE):
@@ -102,13 +129,20 @@ def fine_slopes(n, rho, e, vel, r=REFINE):
     return out
 
 
-def samples_from_coarse(lib, m, rc, ec, vc, slopes, gamma, rng, per_problem):
-    """Features of the coarse state (GPU kernel) and targets of the kept cells."""
+def samples_from_coarse(lib, m, rc, ec, vc, slopes, gammas, rng, per_problem, frac=None):
+    """Features of the coarse state (GPU kernel) and targets of the kept cells;
+    frac: coarse volume fractions [ne][m*m] of a multi-material problem."""
     opt = lib.options()
     abi.set_bcs(opt, abi.reflecting_walls(2))
-    abi.set_envs(opt, [(abi.EOS_PERFECT_GAS, gamma, 0.)])
+    abi.set_envs(opt, [(abi.EOS_PERFECT_GAS, g, 0.) for g in gammas])
     s = abi.Simulation(lib, abi.mesh_desc(2, [m, m], [0, 0], [1.0, 1.0]), opt)
-    s.init_cells(np.zeros(m * m, dtype=np.int32), rc, (gamma - 1.0) * rc * ec, vc)
+    s.init_cells(np.zeros(m * m, dtype=np.int32), rc, (gammas[0] - 1.0) * rc * ec, vc)
+    if frac is not None:  # membership and fractions (the features read density, energy, u, mask, fractions)
+        mask = np.zeros(m * m, dtype=np.int32)
+        for k in range(frac.shape[0]):
+            mask |= (frac[k] > 0).astype(np.int32) << k
+        s.upload({"density": np.ascontiguousarray(rc), "internal_energy": np.ascontiguousarray(ec),
+                  "env_mask": mask, "fracvol_env": np.ascontiguousarray(frac.ravel())})
     X, S = s.learned_features()
     s.close()
     # central slopes from the features (x: cells (i+1, j), (i-1, j); y: (i, j+1), (i, j-1))
@@ -133,10 +167,10 @@ def samples_from_coarse(lib, m, rc, ec, vc, slopes, gamma, rng, per_problem):
 
 
 def make_dataset(lib, n_problems, fine=512, chunk=256, per_problem=512, seed=QUAD_SEED,
-                 precision=abi.PREC_FP32_BULK, log=print):
+                 precision=abi.PREC_FP32_BULK, materials=1, log=print):
     """Runs the seeded four-quadrant problems (in batches of `chunk`) and
     returns the training samples (X [m][128], T [m][16])."""
-    specs = quadrant_specs(n_problems, seed)
+    specs = quadrant_specs(n_problems, seed, materials)
     rng = np.random.default_rng(seed + 1)
     Xs, Ts = [], []
     zc_total, t_run = 0, 0.0
@@ -151,13 +185,14 @@ def make_dataset(lib, n_problems, fine=512, chunk=256, per_problem=512, seed=QUA
         t_run += time.perf_counter() - t0
         zc_total += zc
         for i, sp in enumerate(sub):
-            st = b.problem(i).download(["density", "internal_energy", "velocity"])
+            st = b.problem(i).download(["density", "internal_energy", "velocity", "fracvol_env"])
             rho, e, vel = st["density"], st["internal_energy"], st["velocity"]
             if not (np.all(np.isfinite(rho)) and np.all(np.isfinite(e)) and rho.min() > 0):
                 continue
             rc, ec, vc = restrict(fine, rho, e, vel)
+            fc = restrict_fractions(fine, st["fracvol_env"]) if materials > 1 else None
             X, T = samples_from_coarse(lib, fine // REFINE, rc, ec, vc, fine_slopes(fine, rho, e, vel),
-                                       sp["gamma"], rng, per_problem)
+                                       spec_gammas(sp), rng, per_problem, fc)
             Xs.append(X)
             Ts.append(T)
         b.close()
@@ -315,6 +350,7 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--out", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
                                                   "limiter_mlp.npz"))
+    ap.add_argument("--materials", type=int, default=1, help="gases per four-quadrant problem (1-3)")
     ap.add_argument("--eval-t", type=float, default=1.0)
     ap.add_argument("--eval-base", type=int, nargs=2, default=[896, 384])
     ap.add_argument("--no-eval", action="store_true")
@@ -328,7 +364,7 @@ def main(argv=None):
         return 0
     t0 = time.perf_counter()
     print(f"data: {a.problems} four-quadrant problems at {a.fine}^2, restricted {REFINE}x", flush=True)
-    X, T, info = make_dataset(lib, a.problems, a.fine, a.chunk, a.per_problem,
+    X, T, info = make_dataset(lib, a.problems, a.fine, a.chunk, a.per_problem, materials=a.materials,
                               log=lambda s: print(s, flush=True))
     t_data = time.perf_counter() - t0
     nval = max(1, len(X) // 20)
@@ -347,7 +383,8 @@ def main(argv=None):
           f"validation {val_loss:.4f} (central {baseline_loss(Tva, 1.0):.4f}, minmod {minmod_loss(Xva, Tva):.4f})",
           flush=True)
     os.makedirs(os.path.dirname(a.out), exist_ok=True)
-    summary = {"problems": a.problems, "fine": a.fine, "samples": int(len(Xtr)), "epochs": a.epochs,
+    summary = {"problems": a.problems, "materials": a.materials, "fine": a.fine, "samples": int(len(Xtr)),
+               "epochs": a.epochs,
                "batch": a.batch, "lr": a.lr, "epoch_loss": [float(x) for x in losses],
                "validation_loss": float(val_loss), "validation_loss_central": baseline_loss(Tva, 1.0),
                "validation_loss_first_order": baseline_loss(Tva, 0.0),

@@ -1336,15 +1336,28 @@ void remapVariables(mho_ctx& S) {
     }
     int matcell = 0, imatpure = -1;
     double unsursomme_frac = 1. / somme_frac;
+    // learned mode (extension): an environment whose fraction ends at or
+    // below the threshold leaves the cell, so that no member has a zero
+    // fraction and density (the EOS would give c = sqrt(0/0) and the next
+    // Lagrange phase rho_k = 0/0); the reference keeps it (membership tested
+    // on U_V / V_euler, the fraction on U_V / sum U_V)
+    const bool prune = is_learned(S.opt);
+    bool pruned = false;
     for (int q = 0; q < nbe; ++q) {
       int k = ev[q];
       double fvol = S.EV(S.fracvol, k, c) * unsursomme_frac;
+      if (prune && fvol <= threshold) {
+        fvol = 0.;
+        S.env_mask[c] &= ~(1 << k);
+        pruned = true;
+      }
       if (fvol > 0.) {
         matcell++;
         imatpure = k;
       }
       S.EV(S.fracvol, k, c) = fvol;
     }
+    if (pruned) nbe = S.envsOf(c, ev);
     if (matcell > 1) {
       S.est_mixte[c] = 1;
       S.est_pure[c] = -1;
@@ -1410,6 +1423,7 @@ void remapVariables(mho_ctx& S) {
       }
     }
   }
+  if (is_learned(S.opt)) computeEnvId(S);  // after the learned mode's pruning
   // velocity from the dual (660-672)
 #pragma omp parallel for schedule(static)
   for (long nd = 0; nd < S.NN; ++nd) {
