@@ -209,14 +209,16 @@ def init_params(seed):
     return np.concatenate([p.ravel() for p in parts]).astype(np.float32)
 
 
-def train(lib, X, T, epochs=20, batch=1024, lr=1e-3, seed=0, params=None, log=None):
+def train(lib, X, T, epochs=20, batch=1024, lr=1e-3, seed=0, params=None, lr_end=None, log=None):
     """Adam on the GPU, one epoch per call so that progress is reported; the
-    optimiser state carries over between the calls."""
+    optimiser state carries over between the calls.  With lr_end the rate
+    follows a cosine from lr (first epoch) to lr_end (last)."""
     net = abi.Mlp(lib, init_params(seed) if params is None else params)
     net.set_data(X, T)
     losses = []
     for ep in range(epochs):
-        losses.append(float(net.train(1, batch, lr, seed + 1000 * ep)[0]))
+        r = lr if lr_end is None else lr_end + 0.5 * (lr - lr_end) * (1 + np.cos(np.pi * ep / max(1, epochs - 1)))
+        losses.append(float(net.train(1, batch, r, seed + 1000 * ep)[0]))
         if log:
             log(f"  epoch {ep + 1}/{epochs}: loss {losses[-1]:.5f}")
     p = net.params()
@@ -348,6 +350,7 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--batch", type=int, default=1024)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--lr-end", type=float, default=None, help="cosine decay of the rate down to this value")
     ap.add_argument("--out", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
                                                   "limiter_mlp.npz"))
     ap.add_argument("--materials", type=int, default=1, help="gases per four-quadrant problem (1-3)")
@@ -373,7 +376,8 @@ def main(argv=None):
           f"central-slope loss {baseline_loss(Ttr, 1.0):.4f}, first-order loss {baseline_loss(Ttr, 0.0):.4f}",
           flush=True)
     t0 = time.perf_counter()
-    params, losses = train(lib, Xtr, Ttr, a.epochs, a.batch, a.lr, log=lambda s: print(s, flush=True))
+    params, losses = train(lib, Xtr, Ttr, a.epochs, a.batch, a.lr, lr_end=a.lr_end,
+                           log=lambda s: print(s, flush=True) if a.epochs <= 100 else None)
     t_train = time.perf_counter() - t0
     net = abi.Mlp(lib, params)
     net.set_data(Xva, Tva)
@@ -385,7 +389,7 @@ def main(argv=None):
     os.makedirs(os.path.dirname(a.out), exist_ok=True)
     summary = {"problems": a.problems, "materials": a.materials, "fine": a.fine, "samples": int(len(Xtr)),
                "epochs": a.epochs,
-               "batch": a.batch, "lr": a.lr, "epoch_loss": [float(x) for x in losses],
+               "batch": a.batch, "lr": a.lr, "lr_end": a.lr_end, "epoch_loss": [float(x) for x in losses],
                "validation_loss": float(val_loss), "validation_loss_central": baseline_loss(Tva, 1.0),
                "validation_loss_first_order": baseline_loss(Tva, 0.0),
                "validation_loss_minmod": minmod_loss(Xva, Tva), "data_s": round(t_data, 1),
