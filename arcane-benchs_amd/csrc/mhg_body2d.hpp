@@ -244,9 +244,10 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
     if (o2) {
       if (LEARN) {
         bool used;
+        R gcl;
         const R gl = learned_grad(es.trace(role, phi) ? R(-1) : alpha, phim, phi, lane_down1(phi), inner, front_in,
-                                  gf, gfp, dbr, dfr, &used);
-        g = es.slope(role, k, phi, gl, used, limited_grad<0, true, R, kBranchFree<R>>(gf, gfp));
+                                  gf, gfp, dbr, dfr, &used, &gcl);
+        g = es.slope(role, k, phi, gl, used, gcl);
       } else {
         g = limited_grad<LIM, true, R, kBranchFree<R>>(gf, gfp);
       }
@@ -425,9 +426,10 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
         const int k = role == 0 ? v : role == 1 ? v - ne : role == 2 ? v - 2 * ne : 0;
         if (role == 1 && v == ne) es.close_volumes(ne, alpha);  // every volume is done
         bool used;
+        R gcl;
         const R gl = learned_grad(es.trace(role, phi) ? R(-1) : alpha, phim, phi, lane_down1(phi), inner, front_in,
-                                  gf, gfp, dbr, dfr, &used);
-        g = es.slope(role, k, phi, gl, used, limited_grad<0, true, R, kBranchFree<R>>(gf, gfp));
+                                  gf, gfp, dbr, dfr, &used, &gcl);
+        g = es.slope(role, k, phi, gl, used, gcl);
       } else {
         g = limited_grad<LIM, true, R, kBranchFree<R>>(gf, gfp);
       }
@@ -577,6 +579,25 @@ __device__ __forceinline__ void dt_finalize_body(const Params& P, DevScalars* S,
 // (PrepareRemap.cc:411-433); the last sweep ends with the nodal velocity
 // u = U_dual / m_dual of remapVariables (RemapADIFinal.cc:660-672).
 // ---------------------------------------------------------------------------
+// The same sum with the four loads issued unconditionally (clamped cell
+// indices; a missing cell adds +0 to a sum that starts at +0).
+template <class R>
+__device__ inline R node_mass_from_cells_nb(const Params& P, const R* m, int i, int j) {
+  const int cdi[4] = {-1, 0, -1, 0}, cdj[4] = {-1, -1, 0, 0};
+  R v[4];
+  bool ok[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int ci = i + cdi[t], cj = j + cdj[t];
+    ok[t] = ci >= 0 && ci < P.nx && cj >= 0 && cj < P.ny;
+    v[t] = m[ok[t] ? ci + (long)P.nx * cj : 0];
+  }
+  R msum = R(0);
+#pragma unroll
+  for (int t = 0; t < 4; ++t) msum += ok[t] ? v[t] : R(0);
+  return R(0.25) * msum;
+}
+
 template <class R>
 __device__ inline R node_mass_from_cells(const Params& P, const R* m, int i, int j) {
   R msum = R(0);
@@ -615,21 +636,40 @@ __device__ __forceinline__ void node_sweep_x_2d_body(const Params& P, DevScalars
   const long NC = P.nc;
   const int ne = env_count<NEM>(P);
   const double dt = S->dt;
+  // Every load of the node is issued first, from clamped indices and
+  // without branches, so that a wavefront waits for memory once: a lane out
+  // of the mesh reads node 0, and a missing cell adds +0 (front starts at +0
+  // and a sum of doubles never becomes -0, so the result is the guarded
+  // sum's bit for bit).
+  const double xd_l = A.xnx[nid];
+  R cf[2][NEM];
+  bool cv[2];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int cj = j - 1 + q;
+    cv[q] = ex && i < P.nx && cj >= 0 && cj < P.ny;
+    const long cc = cv[q] ? i + (long)P.nx * cj : 0;
+#pragma unroll
+    for (int k = 0; k < NEM; ++k) cf[q][k] = (k < ne) ? A.contrib[k * NC + cc] : R(0);
+  }
   R U0 = R(0), U1 = R(0), U3 = R(0), p0 = R(0), p1 = R(0);
   double xd = 0.;
-  if (ex) {
-    xd = A.xnx[nid];
-    if (FIRST) {
-      const R mn = node_mass_from_cells(P, A.m, i, j);
-      p0 = A.ux[nid];
-      p1 = A.uy[nid];
+  if (FIRST) {
+    const R mn = node_mass_from_cells_nb(P, A.m, i, j);
+    const R ux = A.ux[nid], uy = A.uy[nid];
+    if (ex) {
+      p0 = ux;
+      p1 = uy;
       U0 = mn * p0;
       U1 = mn * p1;
       U3 = mn;
-    } else {
-      U0 = A.Ud0[nid];
-      U1 = A.Ud1[nid];
-      U3 = A.Ud3[nid];
+    }
+  } else {
+    const R u0 = A.Ud0[nid], u1 = A.Ud1[nid], u3 = A.Ud3[nid];
+    if (ex) {
+      U0 = u0;
+      U1 = u1;
+      U3 = u3;
       // nodes inner in the first direction carry phi = U/m (DualPhaseRemap.cc:494-523)
       if (inner_t) {
         p0 = qdiv(U0, U3);
@@ -640,21 +680,14 @@ __device__ __forceinline__ void node_sweep_x_2d_body(const Params& P, DevScalars
       }
     }
   }
+  if (ex) xd = xd_l;
   // front dual mass flux: cells (i, j-1), (i, j), environments in order
   R front = R(0);
-  if (ex && i < P.nx) {
 #pragma unroll
-    for (int sh = -1; sh <= 0; ++sh) {
-      const int cj = j + sh;
-      if (cj < 0 || cj >= P.ny) continue;
-      const long cc = i + (long)P.nx * cj;
+  for (int q = 0; q < 2; ++q)
 #pragma unroll
-      for (int k = 0; k < NEM; ++k) {
-        if (k >= ne) break;
-        front += A.contrib[k * NC + cc];
-      }
-    }
-  }
+    for (int k = 0; k < NEM; ++k)
+      if (k < ne) front += cv[q] ? cf[q][k] : R(0);
   const double xdm = lane_up1(xd), xdp = lane_down1(xd);
   const R p0m = lane_up1(p0), p0p = lane_down1(p0);
   const R p1m = lane_up1(p1), p1p = lane_down1(p1);
@@ -754,22 +787,38 @@ __device__ __forceinline__ void node_sweep_ytile_2d_body(const Params& P, DevSca
   const long gq = gex ? gi + (long)nnx * gs : 0;
   R U0 = R(0), U1 = R(0), U3 = R(0);
   {
+    // every load first, from clamped indices and without branches (one wait
+    // for memory; a missing cell adds +0 to a sum that starts at +0)
+    const double xd_l = A.xny[gq];
+    R cf[2][NEM];
+    bool cv[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int ci = gi - 1 + q;
+      cv[q] = gex && gs < N && ci >= 0 && ci < nx;
+      const long cc = cv[q] ? ci + (long)nx * gs : 0;
+#pragma unroll
+      for (int k = 0; k < NEM; ++k) cf[q][k] = (k < ne) ? A.contrib[k * NC + cc] : R(0);
+    }
     double xd = 0.;
     R p0 = R(0), p1 = R(0), f = R(0);
-    if (gex) {
-      const bool inner_t = (gi >= 1 && gi <= nx - 1);
-      xd = A.xny[gq];
-      if (FIRST) {
-        const R mn = node_mass_from_cells(P, A.m, gi, gs);
-        p0 = A.ux[gq];
-        p1 = A.uy[gq];
+    if (FIRST) {
+      const R mn = node_mass_from_cells_nb(P, A.m, gi, gs);
+      const R ux = A.ux[gq], uy = A.uy[gq];
+      if (gex) {
+        p0 = ux;
+        p1 = uy;
         U0 = mn * p0;
         U1 = mn * p1;
         U3 = mn;
-      } else {
-        U0 = A.Ud0[gq];
-        U1 = A.Ud1[gq];
-        U3 = A.Ud3[gq];
+      }
+    } else {
+      const R u0 = A.Ud0[gq], u1 = A.Ud1[gq], u3 = A.Ud3[gq];
+      if (gex) {
+        const bool inner_t = (gi >= 1 && gi <= nx - 1);
+        U0 = u0;
+        U1 = u1;
+        U3 = u3;
         if (inner_t) {  // DualPhaseRemap.cc:494-523
           p0 = qdiv(U0, U3);
           p1 = qdiv(U1, U3);
@@ -778,20 +827,14 @@ __device__ __forceinline__ void node_sweep_ytile_2d_body(const Params& P, DevSca
           p1 = A.uy[gq];
         }
       }
-      if (gs < N) {  // front flux: cells (i-1, s), (i, s), environments in order
-#pragma unroll
-        for (int sh = -1; sh <= 0; ++sh) {
-          const int ci = gi + sh;
-          if (ci < 0 || ci >= nx) continue;
-          const long cc = ci + (long)nx * gs;
-#pragma unroll
-          for (int k = 0; k < NEM; ++k) {
-            if (k >= ne) break;
-            f += A.contrib[k * NC + cc];
-          }
-        }
-      }
     }
+    if (gex) xd = xd_l;
+    // front flux: cells (i-1, s), (i, s), environments in order
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int k = 0; k < NEM; ++k)
+        if (k < ne) f += cv[q] ? cf[q][k] : R(0);
     if (trow) {
       ldx[ra] = xd;
       lds[0][ra] = p0;

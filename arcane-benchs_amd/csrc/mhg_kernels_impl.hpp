@@ -28,8 +28,9 @@ __device__ inline T face_grad(bool inner, T phi, T phim, T dx) {
 // otherwise the minmod slope; alpha < 0 selects the minmod slope outright.
 template <class R>
 __device__ inline R learned_grad(R alpha, R phim, R phi, R phip, bool back_in, bool front_in, R gb, R gf, R db,
-                                 R df, bool* used = nullptr) {
+                                 R df, bool* used = nullptr, R* classical = nullptr) {
   const R gcl = limited_grad<0, true, R, kBranchFree<R>>(gb, gf);
+  if (classical) *classical = gcl;
   if (used) *used = false;
   if (alpha < R(0)) return gcl;
   const R gc = alpha * (R(0.5) * (gb + gf));
