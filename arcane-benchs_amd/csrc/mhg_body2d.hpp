@@ -38,40 +38,55 @@ __device__ __forceinline__ void node_lagrange_2d_body(const Params& P, DevScalar
   double vx = A.ux[nid], vy = A.uy[nid];
   if (VEL) {
     const double dtv = 0.5 * (S->dt_old + S->dt);
+    // Every load first, from clamped indices and without branches, so that
+    // a wavefront waits for memory once; out-of-mesh patch nodes read node
+    // 0 and are then zeroed, out-of-mesh cells read cell 0 and are skipped.
     // 3x3 patch of previous coordinates around the node.
     double X[3][3], Y[3][3];
+    bool nv[3][3];
 #pragma unroll
     for (int dj = -1; dj <= 1; ++dj)
 #pragma unroll
       for (int di = -1; di <= 1; ++di) {
-        int ii = i + di, jj = j + dj;
-        double xv = 0., yv = 0.;
-        if (ii >= 0 && ii < nnx && jj >= 0 && jj <= P.ny) {
-          long q = ii + (long)nnx * jj;
-          xv = A.xpx[q];
-          yv = A.xpy[q];
-        }
-        X[dj + 1][di + 1] = xv;
-        Y[dj + 1][di + 1] = yv;
+        const int ii = i + di, jj = j + dj;
+        const bool ok = ii >= 0 && ii < nnx && jj >= 0 && jj <= P.ny;
+        const long q = ok ? ii + (long)nnx * jj : 0;
+        nv[dj + 1][di + 1] = ok;
+        X[dj + 1][di + 1] = A.xpx[q];
+        Y[dj + 1][di + 1] = A.xpy[q];
       }
-    double Fx = 0., Fy = 0., msum = 0.;
     // cells in ascending id: (i-1,j-1) corner 2, (i,j-1) 3, (i-1,j) 1, (i,j) 0
     const int cdi[4] = {-1, 0, -1, 0}, cdj[4] = {-1, -1, 0, 0}, cq[4] = {2, 3, 1, 0};
+    R cp[4], cqv[4], cm[4];
+    bool cv[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      int ci = i + cdi[t], cj = j + cdj[t];
-      if (ci < 0 || ci >= P.nx || cj < 0 || cj >= P.ny) continue;
-      long c = ci + (long)P.nx * cj;
+      const int ci = i + cdi[t], cj = j + cdj[t];
+      cv[t] = ci >= 0 && ci < P.nx && cj >= 0 && cj < P.ny;
+      const long c = cv[t] ? ci + (long)P.nx * cj : 0;
+      cp[t] = A.p[c];
+      cqv[t] = A.q[c];
+      cm[t] = A.m[c];
+    }
+#pragma unroll
+    for (int dj = 0; dj < 3; ++dj)
+#pragma unroll
+      for (int di = 0; di < 3; ++di)
+        if (!nv[dj][di]) X[dj][di] = Y[dj][di] = 0.;
+    double Fx = 0., Fy = 0., msum = 0.;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      if (!cv[t]) continue;
       // quad nodes (ci,cj),(ci+1,cj),(ci+1,cj+1),(ci,cj+1) in patch coordinates
       int bi = cdi[t] + 1, bj = cdj[t] + 1;
       double qx[4] = {X[bj][bi], X[bj][bi + 1], X[bj + 1][bi + 1], X[bj + 1][bi]};
       double qy[4] = {Y[bj][bi], Y[bj][bi + 1], Y[bj + 1][bi + 1], Y[bj + 1][bi]};
       double cx, cy;
       cqs2d(qx, qy, cq[t], &cx, &cy);
-      double pq = (double)A.p[c] + (double)A.q[c];
+      double pq = (double)cp[t] + (double)cqv[t];
       Fx += pq * cx;
       Fy += pq * cy;
-      msum += (double)A.m[c];
+      msum += (double)cm[t];
     }
     double mn = 0.25 * msum;
     double s = dtv / mn;
@@ -124,6 +139,7 @@ __device__ __forceinline__ void cell_lagrange_2d_body(const Params& P, DevScalar
   const int i = (int)(c % P.nx), j = (int)(c / P.nx);
   const long n0 = i + (long)nnx * j;
   const long nd[4] = {n0, n0 + 1, n0 + 1 + nnx, n0 + nnx};
+  const CellPre<R> in = cell_pre_load<R, PC>(A, c);
   double x[4], y[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
@@ -132,7 +148,7 @@ __device__ __forceinline__ void cell_lagrange_2d_body(const Params& P, DevScalar
   }
   const double Vd = quad_volume(x, y);
   const double Ld = (P.lc == MHG_LC_FACES_OPPOSEES) ? length_faces_opposees_2d(x, y) : pow(Vd, 0.5);
-  lagrange_cell_bulk<R, PC, NEM>(P, S, A, c, Vd, Ld);
+  lagrange_cell_bulk<R, PC, NEM>(P, S, A, c, Vd, Ld, in);
 }
 
 // ---------------------------------------------------------------------------
@@ -180,32 +196,49 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
   const double* __restrict__ Xd = (D == 0) ? A.xnx : A.xny;   // coordinate along D
   const double* __restrict__ Xt = (D == 0) ? A.xny : A.xnx;   // transverse coordinate
   const R* __restrict__ Ud = (D == 0) ? A.ux : A.uy;          // velocity along D
+  // Every load of the cell is issued first, from clamped indices (a lane
+  // out of the mesh reads cell / node 0), so that a wavefront waits for
+  // memory once; the values are selected afterwards.
+  const long m0 = ex ? i + (long)nnx * j : 0;
+  const double xa = Xd[m0], xb = Xd[m0 + 1], xcc = Xd[m0 + 1 + nnx], xdd = Xd[m0 + nnx];
+  // back face of the cell: the face at node index s along D, nodes
+  // (i,j),(i,j+1) for D = 0 and (i,j),(i+1,j) for D = 1
+  const bool fin = s >= 0 && s <= N;
+  const long na = fin ? i + (long)nnx * j : 0, nb = (D == 0) ? na + nnx : na + 1;
+  const R uda = Ud[na], udb = Ud[nb];
+  const double xfa = Xd[na], xfb = Xd[nb], xta = Xt[na], xtb = Xt[nb];
+  const R vlag = FIRST ? A.Vlag[c] : R(1);
+  R pre[3][NEM];
+#pragma unroll
+  for (int g = 0; g < 3; ++g)
+#pragma unroll
+    for (int k = 0; k < NEM; ++k) {
+      const R u = (k < ne) ? Uin[(g * ne + k) * NC + c] : R(0);
+      pre[g][k] = (ex && k < ne) ? u : R(0);
+    }
+  const R preq_l = Uin[(3 * ne) * NC + c];
+  const R preq = ex ? preq_l : R(0);
   // cell centre (component D, double) and phi denominator
   double xc = 0.;
   R den = R(1);
   if (ex) {
-    const long m0 = i + (long)nnx * j;
-    const double a = Xd[m0], b = Xd[m0 + 1], cc = Xd[m0 + 1 + nnx], dd = Xd[m0 + nnx];
-    xc = 0.25 * (((a + b) + cc) + dd);
+    xc = 0.25 * (((xa + xb) + xcc) + xdd);
     if (FIRST) {
-      den = A.Vlag[c];
+      den = vlag;
     } else {
       R sv = R(0);
 #pragma unroll
       for (int k = 0; k < NEM; ++k)
-        if (k < ne) sv += Uin[k * NC + c];
+        if (k < ne) sv += pre[0][k];
       den = sv;
     }
   }
-  // back face of the cell: the face at node index s along D, nodes
-  // (i,j),(i,j+1) for D = 0 and (i,j),(i+1,j) for D = 1
   R uf = R(0);
   double lf = 0., xf = 0.;
-  if (s >= 0 && s <= N) {
-    const long na = i + (long)nnx * j, nb = (D == 0) ? na + nnx : na + 1;
-    uf = R(0.5) * (Ud[na] + Ud[nb]);
-    xf = 0.5 * Xd[na] + 0.5 * Xd[nb];
-    lf = fabs(Xt[nb] - Xt[na]);
+  if (fin) {
+    uf = R(0.5) * (uda + udb);
+    xf = 0.5 * xfa + 0.5 * xfb;
+    lf = fabs(xtb - xta);
   }
   const bool inner = ex && s >= 1;
   const double xcm = lane_up1(xc);
@@ -222,17 +255,12 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
   R alpha = R(0), dbr = R(0), dfr = R(0);
   bool front_in = false;
   if (LEARN) {
-    alpha = ex ? A.alpha[c] : R(0);
+    const R al = A.alpha[c];
+    alpha = ex ? al : R(0);
     front_in = lane_down1((int)inner) != 0;
     dbr = (R)(xf - xc);
     dfr = (R)(lane_down1(xf) - xc);
   }
-  R pre[3][NEM];
-  const R preq = ex ? Uin[(3 * ne) * NC + c] : R(0);
-#pragma unroll
-  for (int g = 0; g < 3; ++g)
-#pragma unroll
-    for (int k = 0; k < NEM; ++k) pre[g][k] = (ex && k < ne) ? Uin[(g * ne + k) * NC + c] : R(0);
   EnvSlopes<R, NEM> es(P.remap_threshold);  // learned mode: consistent environment slopes
   // role: 0 volume, 1 mass, 2 energy of environment k, 3 the other variable
   auto sweep_var = [&](R u0, R& unew, R& dual, int role, int k) {

@@ -119,11 +119,12 @@ __global__ __launch_bounds__(256) void k_cell_lagrange_3d(Params P, DevScalars* 
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= P.nc) return;
   const int i = (int)(c % P.nx), j = (int)((c / P.nx) % P.ny), k = (int)(c / ((long)P.nx * P.ny));
+  const CellPre<R> in = cell_pre_load<R, PC>(A.b, c);
   V3 cn[8];
   hex_nodes(P, A.b.xnx, A.b.xny, A.xnz, i, j, k, cn);
   const double Vd = hex_volume(cn);
   const double Ld = (P.lc == MHG_LC_FACES_OPPOSEES) ? hex_length_faces_opposees(cn) : pow(Vd, 1. / 3.);
-  lagrange_cell_bulk<R, PC, NEM>(P, S, A.b, c, Vd, Ld);
+  lagrange_cell_bulk<R, PC, NEM>(P, S, A.b, c, Vd, Ld, in);
 }
 
 // ---------------------------------------------------------------------------

@@ -91,20 +91,40 @@ struct EnvSlopes {
   }
 };
 
+// Cell fields the Lagrange cell phase reads, loaded by the caller before the
+// node coordinates so that a wavefront issues all of them together.
+template <class R>
+struct CellPre {
+  R rho_n, m, cs, p, e, q;
+  int mask;
+};
+template <class R, bool PC>
+__device__ __forceinline__ CellPre<R> cell_pre_load(const CellLagArgsT<R>& A, long c) {
+  CellPre<R> r;
+  r.rho_n = A.rho[c];
+  r.m = A.m[c];
+  r.cs = A.cs[c];
+  r.p = A.p[c];
+  r.e = A.e[c];
+  r.q = PC ? A.q[c] : R(0);
+  r.mask = A.mask[c];
+  return r;
+}
+
 // Density, viscosity, energy and the remap vector of cell c from its
 // Lagrangian volume and characteristic length (both dimensions).
 template <class R, bool PC, int NEM>
 __device__ inline void lagrange_cell_bulk(const Params& P, const DevScalars* S, const CellLagArgsT<R>& A,
-                                          long c, double Vd, double Ld) {
+                                          long c, double Vd, double Ld, const CellPre<R>& in) {
   // geometry computed in double, the rest in the storage type
   const R V = (R)Vd, L = (R)Ld;
   const R inv_dt = qdiv(R(1.0), (R)S->dt);
-  const R rho_n = A.rho[c];
-  const R rho = qdiv(A.m[c], V);
+  const R rho_n = in.rho_n;
+  const R rho = qdiv(in.m, V);
   const R tau = R(0.5) * (qdiv(R(1.0), rho_n) + qdiv(R(1.0), rho));
   const R divu = qdiv(inv_dt * (qdiv(R(1.0), rho) - qdiv(R(1.0), rho_n)), tau);
-  const R cs = A.cs[c];
-  const int mask = A.mask[c];
+  const R cs = in.cs;
+  const int mask = in.mask;
   const int ne = env_count<NEM>(P);
   const long NC = P.nc;
   R uv[NEM], um[NEM], ue[NEM];
@@ -115,8 +135,8 @@ __device__ inline void lagrange_cell_bulk(const Params& P, const DevScalars* S, 
     const int kk = __ffs(mask) - 1;
     R qv = R(0);
     if (divu < R(0)) qv = viscosity((R)P.eos.qcoef[kk], tau, L, cs, divu);
-    const R qn = PC ? A.q[c] : R(0);
-    const R e = compute_eint(PC, (R)P.eos.gamma[kk], qn, qv, rho_n, rho, A.p[c], A.e[c]);
+    const R qn = in.q;
+    const R e = compute_eint(PC, (R)P.eos.gamma[kk], qn, qv, rho_n, rho, in.p, in.e);
 #pragma unroll
     for (int k = 0; k < NEM; ++k)
       if (k == kk) {
