@@ -386,28 +386,40 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
   const bool gex = trow && gi < nx && gs >= 0 && gs < N;
   const bool gout = SEG ? gex : gex && tr >= 2 && tr <= 61;
   const long gc = gex ? gi + (long)nx * gs : 0;
+  // Every load of the geometry phase, and the first pair of variables, is
+  // issued first from clamped indices (a thread out of the mesh reads cell /
+  // node 0), so that a wavefront waits for memory once before the barrier.
+  const bool fvalid = trow && gi < nx && gs >= 0 && gs <= N;  // back face: nodes (i, s), (i+1, s)
+  const long m0 = gex ? gi + (long)nnx * gs : 0;
+  const long na = fvalid ? gi + (long)nnx * gs : 0, nb = na + 1;
+  const double ya = A.xny[m0], yb = A.xny[m0 + 1], yc = A.xny[m0 + 1 + nnx], yd = A.xny[m0 + nnx];
+  const R uya = A.uy[na], uyb = A.uy[nb];
+  const double yfa = A.xny[na], yfb = A.xny[nb], xfa = A.xnx[na], xfb = A.xnx[nb];
+  const R vlag = FIRST ? A.Vlag[gc] : R(1);
+  R uvol[NEM];
+#pragma unroll
+  for (int k = 0; k < NEM; ++k) uvol[k] = (k < ne) ? Uin[k * NC + gc] : R(0);
+  const int nvu = P.nvu;
+  const R nxa0 = Uin[gc], nxb0 = nvu > 1 ? Uin[NC + gc] : R(0);
   {  // geometry of the tile, row-major (stored in the double tiles exactly)
     double xc = 0., xf = 0., lf = 0.;
     R den = R(1), uf = R(0);
     if (gex) {
-      const long m0 = gi + (long)nnx * gs;
-      const double a = A.xny[m0], b = A.xny[m0 + 1], cc = A.xny[m0 + 1 + nnx], dd = A.xny[m0 + nnx];
-      xc = 0.25 * (((a + b) + cc) + dd);
+      xc = 0.25 * (((ya + yb) + yc) + yd);
       if (FIRST) {
-        den = A.Vlag[gc];
+        den = vlag;
       } else {
         R sv = R(0);
 #pragma unroll
         for (int k = 0; k < NEM; ++k)
-          if (k < ne) sv += Uin[k * NC + gc];
+          if (k < ne) sv += uvol[k];
         den = sv;
       }
     }
-    if (trow && gi < nx && gs >= 0 && gs <= N) {  // back face: nodes (i, s), (i+1, s)
-      const long na = gi + (long)nnx * gs, nb = na + 1;
-      uf = R(0.5) * (A.uy[na] + A.uy[nb]);
-      xf = 0.5 * A.xny[na] + 0.5 * A.xny[nb];
-      lf = fabs(A.xnx[nb] - A.xnx[na]);
+    if (fvalid) {
+      uf = R(0.5) * (uya + uyb);
+      xf = 0.5 * yfa + 0.5 * yfb;
+      lf = fabs(xfb - xfa);
     }
     if (trow) {
       lds[0][ra] = xc;
@@ -472,10 +484,9 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
     dual = R(0.5) * flux_b * R(-1.0) + R(0.5) * flux_f * R(1.0);
   };
   __syncthreads();  // geometry reads done: the tiles become staging tiles
-  const int nvu = P.nvu;
   // variables two at a time (a, b = a + 1), one barrier pair per round
-  R nxa = gex ? Uin[gc] : R(0);
-  R nxb = (gex && nvu > 1) ? Uin[NC + gc] : R(0);
+  R nxa = gex ? nxa0 : R(0);
+  R nxb = (gex && nvu > 1) ? nxb0 : R(0);
   auto round = [&](const int va) {
     const int vb = va + 1;
     const bool hb = vb < nvu;  // uniform
@@ -532,22 +543,31 @@ __device__ __forceinline__ void cell_final_2d_body(const Params& P, DevScalars* 
   if (c < P.nc) {
     const int nnx = P.nx + 1;
     const int i = (int)(c % P.nx), j = (int)(c / P.nx);
+    // node velocities of the CFL candidate, loaded before the cell phase
+    // writes its fields so that they travel with its loads
+    const long n0 = i + (long)nnx * j;
+    const long nd[4] = {n0, n0 + 1, n0 + 1 + nnx, n0 + nnx};
+    R vx[4], vy[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      vx[q] = A.ux[nd[q]];
+      vy[q] = A.uy[nd[q]];
+    }
+    const R Lc = A.L[c];
     // Euler volume: the corner-normal formula on the initial coordinates.
     double x0[4] = {node_x0(P, i), node_x0(P, i + 1), node_x0(P, i + 1), node_x0(P, i)};
     double y0[4] = {node_y0(P, j), node_y0(P, j), node_y0(P, j + 1), node_y0(P, j + 1)};
     const R c_g = final_cell<R, NEM>(P, A, c, quad_volume(x0, y0));
     // CFL candidate: L / (c + max over the cell nodes of |u|)
-    const long n0 = i + (long)nnx * j;
-    const long nd[4] = {n0, n0 + 1, n0 + 1 + nnx, n0 + nnx};
     R vmax = R(0);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const R ux = A.ux[nd[q]], uy = A.uy[nd[q]];
+      const R ux = vx[q], uy = vy[q];
       const R un = sqrt(ux * ux + uy * uy + R(0) * R(0));
       vmax = (un < vmax) ? vmax : un;
     }
     // std::min(minimum_aux, dx_sound) fold: a NaN candidate is skipped
-    dx_sound = smin(dx_sound, (double)qdiv(A.L[c], c_g + vmax));
+    dx_sound = smin(dx_sound, (double)qdiv(Lc, c_g + vmax));
   }
   block_min_store<BLOCK>(dx_sound, block_min + blk);
 }
