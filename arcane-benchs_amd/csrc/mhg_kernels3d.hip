@@ -158,37 +158,48 @@ __device__ inline void sweep_geometry3(const Params& P, const CellSweep3ArgsT<R>
   const double* X[3] = {A.b.xnx, A.b.xny, A.xnz};
   const R* U3[3] = {A.b.ux, A.b.uy, A.uz};
   const double* Xd = X[D];
+  // every load first, from clamped indices (cell / face (0, 0, 0) when out of
+  // the mesh), so that a wavefront waits for memory once
+  const int ci = cell_ok ? i : 0, cj = cell_ok ? j : 0, ck = cell_ok ? k : 0;
+  double cx[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int di = (q == 1 || q == 2 || q == 5 || q == 6) ? 1 : 0;
+    const int dj = (q == 2 || q == 3 || q == 6 || q == 7) ? 1 : 0;
+    const int dk = q >= 4 ? 1 : 0;
+    cx[q] = Xd[nid3(P, ci + di, cj + dj, ck + dk)];
+  }
+  long n[4];
+  face_nodes3(P, D, face_ok ? i : 0, face_ok ? j : 0, face_ok ? k : 0, n);
+  double fx[4];
+  R fu[4];
+  V3 xq[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    fx[q] = Xd[n[q]];
+    fu[q] = U3[D][n[q]];
+    xq[q] = V3{A.b.xnx[n[q]], A.b.xny[n[q]], A.xnz[n[q]]};
+  }
   *xc = 0.;
   if (cell_ok) {
     double somme = 0.;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int di = (q == 1 || q == 2 || q == 5 || q == 6) ? 1 : 0;
-      const int dj = (q == 2 || q == 3 || q == 6 || q == 7) ? 1 : 0;
-      const int dk = q >= 4 ? 1 : 0;
-      somme += Xd[nid3(P, i + di, j + dj, k + dk)];
-    }
+    for (int q = 0; q < 8; ++q) somme += cx[q];
     *xc = 0.125 * somme;
   }
   *xf = 0.;
   *uf = R(0);
   *lf = 0.;
   if (face_ok) {
-    long n[4];
-    face_nodes3(P, D, i, j, k, n);
     double fc = 0., vm = 0.;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      fc += 0.25 * Xd[n[q]];
-      vm += (double)U3[D][n[q]];
+      fc += 0.25 * fx[q];
+      vm += (double)fu[q];
     }
     *xf = fc;
     *uf = (R)(0.25 * vm);
-    const V3 x0 = {A.b.xnx[n[0]], A.b.xny[n[0]], A.xnz[n[0]]};
-    const V3 x1 = {A.b.xnx[n[1]], A.b.xny[n[1]], A.xnz[n[1]]};
-    const V3 x2 = {A.b.xnx[n[2]], A.b.xny[n[2]], A.xnz[n[2]]};
-    const V3 x3 = {A.b.xnx[n[3]], A.b.xny[n[3]], A.xnz[n[3]]};
-    const V3 t = v3mul(0.5, v3cross(v3sub(x2, x0), v3sub(x3, x1)));
+    const V3 t = v3mul(0.5, v3cross(v3sub(xq[2], xq[0]), v3sub(xq[3], xq[1])));
     *lf = fabs(D == 0 ? t.x : (D == 1 ? t.y : t.z));
   }
 }
@@ -219,14 +230,19 @@ __global__ __launch_bounds__(256) void k_cell_sweep_x_3d(Params P, DevScalars* S
   sweep_geometry3<R, 0>(P, A, i, j, k, ex, i >= 0 && i <= N, &xc, &xf, &uf, &lf);
   R den = R(1);
   R pre[3][NEM];
-  const R preq = ex ? Uin[(3 * ne) * NC + c] : R(0);
+  const R preq_l = Uin[(3 * ne) * NC + c];  // c is 0 for a lane out of the mesh
+  const R preq = ex ? preq_l : R(0);
 #pragma unroll
   for (int g = 0; g < 3; ++g)
 #pragma unroll
-    for (int q = 0; q < NEM; ++q) pre[g][q] = (ex && q < ne) ? Uin[(g * ne + q) * NC + c] : R(0);
+    for (int q = 0; q < NEM; ++q) {
+      const R u = (q < ne) ? Uin[(g * ne + q) * NC + c] : R(0);
+      pre[g][q] = (ex && q < ne) ? u : R(0);
+    }
+  const R vlag = FIRST ? A.b.Vlag[c] : R(1);
   if (ex) {
     if (FIRST) {
-      den = A.b.Vlag[c];
+      den = vlag;
     } else {
       R sv = R(0);
 #pragma unroll
@@ -429,24 +445,60 @@ struct DualNode {
 template <class R, int D, bool FIRST, int NEM>
 __device__ inline DualNode<R> dual_node_load(const Params& P, const NodeSweep3ArgsT<R>& A, int i, int j, int k,
                                              bool ok) {
+  // Every load is issued first, from clamped indices and without branches,
+  // so that a wavefront waits for memory once: a node out of the mesh reads
+  // node 0 and is zeroed below, a missing cell reads cell 0 and adds +0 to a
+  // sum that starts at +0 (so the sums are the guarded sums bit for bit).
+  const long nid = ok ? nid3(P, i, j, k) : 0;
+  const double* X[3] = {A.xnx, A.xny, A.xnz};
+  const double xd_l = X[D][nid];
+  const R u[3] = {A.ux[nid], A.uy[nid], A.uz[nid]};
+  R cm[8];
+  bool cmv[8];
+  R Ul[4];
+  if (FIRST) {
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {  // node mass 0.125 * sum of the 8 cells (computeNodeMass)
+      const int ci = i + (t & 1) - 1, cj = j + ((t >> 1) & 1) - 1, ck = k + (t >> 2) - 1;
+      cmv[t] = ci >= 0 && ci < P.nx && cj >= 0 && cj < P.ny && ck >= 0 && ck < P.nz;
+      cm[t] = A.m[cmv[t] ? cid3(P, ci, cj, ck) : 0];
+    }
+  } else {
+    Ul[0] = A.Udi0[nid];
+    Ul[1] = A.Udi1[nid];
+    Ul[2] = A.Udi2[nid];
+    Ul[3] = A.Udi3[nid];
+  }
+  // front cells: index D at the node, dir1 and dir2 shifted by -1 and 0
+  const int d1 = (D + 1) % 3, d2 = (D + 2) % 3;
+  const int nn3[3] = {P.nx, P.ny, P.nz};
+  const int base[3] = {i, j, k};
+  R cf[4][NEM];
+  bool cfv[4];
+#pragma unroll
+  for (int s1 = -1; s1 <= 0; ++s1)
+#pragma unroll
+    for (int s2 = -1; s2 <= 0; ++s2) {
+      const int q = 2 * (s1 + 1) + (s2 + 1);
+      int cc[3] = {i, j, k};
+      cc[d1] += s1;
+      cc[d2] += s2;
+      cfv[q] = ok && base[D] < nn3[D] && cc[d1] >= 0 && cc[d1] < nn3[d1] && cc[d2] >= 0 && cc[d2] < nn3[d2];
+      const long cell = cfv[q] ? cid3(P, cc[0], cc[1], cc[2]) : 0;
+#pragma unroll
+      for (int e = 0; e < NEM; ++e) cf[q][e] = (e < env_count<NEM>(P)) ? A.contrib[e * P.nc + cell] : R(0);
+    }
   DualNode<R> d;
   for (int q = 0; q < 4; ++q) d.U[q] = R(0);
   for (int q = 0; q < 3; ++q) d.p[q] = R(0);
   d.xd = 0.;
   d.front = R(0);
   if (!ok) return d;
-  const long nid = nid3(P, i, j, k);
-  const double* X[3] = {A.xnx, A.xny, A.xnz};
-  d.xd = X[D][nid];
-  const R u[3] = {A.ux[nid], A.uy[nid], A.uz[nid]};
+  d.xd = xd_l;
   if (FIRST) {
-    R msum = R(0);  // node mass 0.125 * sum of the 8 cells (computeNodeMass)
+    R msum = R(0);
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      const int ci = i + (t & 1) - 1, cj = j + ((t >> 1) & 1) - 1, ck = k + (t >> 2) - 1;
-      if (ci < 0 || ci >= P.nx || cj < 0 || cj >= P.ny || ck < 0 || ck >= P.nz) continue;
-      msum += A.m[cid3(P, ci, cj, ck)];
-    }
+    for (int t = 0; t < 8; ++t) msum += cmv[t] ? cm[t] : R(0);
     const R mn = R(0.125) * msum;
     for (int q = 0; q < 3; ++q) {
       d.p[q] = u[q];
@@ -454,10 +506,7 @@ __device__ inline DualNode<R> dual_node_load(const Params& P, const NodeSweep3Ar
     }
     d.U[3] = mn;
   } else {
-    d.U[0] = A.Udi0[nid];
-    d.U[1] = A.Udi1[nid];
-    d.U[2] = A.Udi2[nid];
-    d.U[3] = A.Udi3[nid];
+    for (int q = 0; q < 4; ++q) d.U[q] = Ul[q];
     const int idx[3] = {i, j, k};
     const int nn[3] = {P.nx, P.ny, P.nz};
     bool inner_prev = false;
@@ -465,27 +514,13 @@ __device__ inline DualNode<R> dual_node_load(const Params& P, const NodeSweep3Ar
       if (((A.prevmask >> e) & 1) && idx[e] >= 1 && idx[e] <= nn[e] - 1) inner_prev = true;
     for (int q = 0; q < 3; ++q) d.p[q] = inner_prev ? qdiv(d.U[q], d.U[3]) : u[q];
   }
-  // front cells: index D at the node, dir1 and dir2 shifted by -1 and 0
-  const int d1 = (D + 1) % 3, d2 = (D + 2) % 3;
-  const int nn3[3] = {P.nx, P.ny, P.nz};
-  const int base[3] = {i, j, k};
   if (base[D] < nn3[D]) {
     R f = R(0);
 #pragma unroll
-    for (int s1 = -1; s1 <= 0; ++s1)
+    for (int q = 0; q < 4; ++q)
 #pragma unroll
-      for (int s2 = -1; s2 <= 0; ++s2) {
-        int cc[3] = {i, j, k};
-        cc[d1] += s1;
-        cc[d2] += s2;
-        if (cc[d1] < 0 || cc[d1] >= nn3[d1] || cc[d2] < 0 || cc[d2] >= nn3[d2]) continue;
-        const long cell = cid3(P, cc[0], cc[1], cc[2]);
-#pragma unroll
-        for (int e = 0; e < NEM; ++e) {
-          if (e >= env_count<NEM>(P)) break;
-          f += A.contrib[e * P.nc + cell];
-        }
-      }
+      for (int e = 0; e < NEM; ++e)
+        if (e < env_count<NEM>(P)) f += cfv[q] ? cf[q][e] : R(0);
     d.front = f;
   }
   return d;
