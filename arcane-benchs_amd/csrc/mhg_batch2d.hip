@@ -211,11 +211,12 @@ __global__ __launch_bounds__(256) void kb_dt_finalize(MHG_BATCH_ARGS, const doub
     else if ((ne) <= 2) F(2); \
     else F(3);                \
   } while (0)
-#define MHG_BLIM(lim, F)      \
-  do {                        \
-    if ((lim) == 0) F(0);     \
+#define MHG_BLIM(lim, F)       \
+  do {                         \
+    if ((lim) == 0) F(0);      \
     else if ((lim) == 1) F(1); \
-    else F(2);                \
+    else if ((lim) == 2) F(2); \
+    else F(kLimG);             \
   } while (0)
 #define MHG_BLARGS(list) probs, S, L.items[list], L.count + (list)
 

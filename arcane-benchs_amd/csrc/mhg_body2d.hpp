@@ -163,6 +163,23 @@ __device__ __forceinline__ void cell_lagrange_2d_body(const Params& P, DevScalar
 // columns, so each cache line of a row is consumed by the whole block.
 // LEARN: the cell slopes of the learned limiter (learned_grad) with the
 // cell's coefficient A.alpha[c]; LIM is then minmod, the fallback.
+// Width of a quadrilateral cell along a sweep direction (computeGradPhiFace,
+// HCELL_BY_CELLS, RemapADIService.cc:223-255): the distances from the cell
+// centre to the centres of its back and front faces, a face on the mesh
+// boundary left out.  a[4], t[4]: coordinates along and across the sweep of
+// the nodes (i,j), (i+1,j), (i+1,j+1), (i,j+1); back face nodes b0, b1, front
+// face nodes f0, f1; first / last: the cell is the first / last along the
+// sweep.
+__device__ __forceinline__ double cell_width_2d(const double* a, const double* t, int b0, int b1, int f0, int f1,
+                                                bool first, bool last) {
+  const double ca = 0.25 * (((a[0] + a[1]) + a[2]) + a[3]);
+  const double ct = 0.25 * (((t[0] + t[1]) + t[2]) + t[3]);
+  const double ba = (0.5 * a[b0] + 0.5 * a[b1]) - ca, bt = (0.5 * t[b0] + 0.5 * t[b1]) - ct;
+  const double fa = (0.5 * a[f0] + 0.5 * a[f1]) - ca, ft = (0.5 * t[f0] + 0.5 * t[f1]) - ct;
+  const double db = sqrt(ba * ba + bt * bt + 0. * 0.), df = sqrt(fa * fa + ft * ft + 0. * 0.);
+  return (first ? 0. : db) + (last ? 0. : df);
+}
+
 template <class R, int D, int NEM, bool FIRST, int LIM, bool LEARN = false>
 __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScalars* S, const CellSweepArgsT<R>& A,
                                                         int blk) {
@@ -208,6 +225,14 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
   const R uda = Ud[na], udb = Ud[nb];
   const double xfa = Xd[na], xfb = Xd[nb], xta = Xt[na], xtb = Xt[nb];
   const R vlag = FIRST ? A.Vlag[c] : R(1);
+  // generalised limiters: coordinates across D of the cell's four nodes
+  double tn[4] = {0., 0., 0., 0.};
+  if (LIM == kLimG) {
+    tn[0] = Xt[m0];
+    tn[1] = Xt[m0 + 1];
+    tn[2] = Xt[m0 + 1 + nnx];
+    tn[3] = Xt[m0 + nnx];
+  }
   R pre[3][NEM];
 #pragma unroll
   for (int g = 0; g < 3; ++g)
@@ -261,6 +286,20 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
     dbr = (R)(xf - xc);
     dfr = (R)(lane_down1(xf) - xc);
   }
+  // generalised limiters: widths of the cell and of its back / front
+  // neighbours along D (the cell itself where the neighbour is missing)
+  R hc = R(0), hcm = R(0), hcp = R(0);
+  if (LIM == kLimG) {
+    front_in = lane_down1((int)inner) != 0;
+    const double an[4] = {xa, xb, xcc, xdd};
+    const double h = ex ? (D == 0 ? cell_width_2d(an, tn, 0, 3, 1, 2, s == 0, s == N - 1)
+                                  : cell_width_2d(an, tn, 0, 1, 3, 2, s == 0, s == N - 1))
+                        : 0.;
+    const double hm = lane_up1(h), hp = lane_down1(h);
+    hc = (R)h;
+    hcm = (R)(inner ? hm : h);
+    hcp = (R)(front_in ? hp : h);
+  }
   EnvSlopes<R, NEM> es(P.remap_threshold);  // learned mode: consistent environment slopes
   // role: 0 volume, 1 mass, 2 energy of environment k, 3 the other variable
   auto sweep_var = [&](R u0, R& unew, R& dual, int role, int k) {
@@ -269,7 +308,10 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
     const R gf = face_grad(inner, phi, phim, dxr);
     const R gfp = lane_down1(gf);
     R g = R(0);
-    if (o2) {
+    if (LIM == kLimG && o2) {  // computeAndLimitGradPhi, UtilesRemap.cc:329-338
+      const R phip = lane_down1(phi);
+      g = limiter_g<R>(P.lim, gfp, gf, front_in ? phip : phi, inner ? phim : phi, hc, hcp, hcm);
+    } else if (o2) {
       if (LEARN) {
         bool used;
         R gcl;
@@ -396,6 +438,14 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
   const R uya = A.uy[na], uyb = A.uy[nb];
   const double yfa = A.xny[na], yfb = A.xny[nb], xfa = A.xnx[na], xfb = A.xnx[nb];
   const R vlag = FIRST ? A.Vlag[gc] : R(1);
+  // generalised limiters: x coordinates of the cell's four nodes
+  double tn[4] = {0., 0., 0., 0.};
+  if (LIM == kLimG) {
+    tn[0] = A.xnx[m0];
+    tn[1] = A.xnx[m0 + 1];
+    tn[2] = A.xnx[m0 + 1 + nnx];
+    tn[3] = A.xnx[m0 + nnx];
+  }
   R uvol[NEM];
 #pragma unroll
   for (int k = 0; k < NEM; ++k) uvol[k] = (k < ne) ? Uin[k * NC + gc] : R(0);
@@ -428,6 +478,10 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
       lds[3][ra] = xf;
       lds[4][ra] = lf;
     }
+    if (LIM == kLimG && trow) {  // width along y (faces at node rows gs and gs + 1)
+      const double an[4] = {ya, yb, yc, yd};
+      lds[5][ra] = gex ? cell_width_2d(an, tn, 0, 1, 3, 2, gs == 0, gs == N - 1) : 0.;
+    }
   }
   __syncthreads();
   const int s = s0 + r;
@@ -452,6 +506,17 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
     dbr = (R)(xf - xc);
     dfr = (R)(lane_down1(xf) - xc);
   }
+  // generalised limiters: widths of the cell and of its back / front
+  // neighbours along y (the cell itself where the neighbour is missing)
+  R hc = R(0), hcm = R(0), hcp = R(0);
+  if (LIM == kLimG) {
+    front_in = lane_down1((int)inner) != 0;
+    const double h = tlane ? lds[5][ta] : 0.;
+    const double hm = lane_up1(h), hp = lane_down1(h);
+    hc = (R)h;
+    hcm = (R)(inner ? hm : h);
+    hcp = (R)(front_in ? hp : h);
+  }
   EnvSlopes<R, NEM> es(P.remap_threshold);  // learned mode: consistent environment slopes
   // one variable v on the transposed layout: new U and half the dual flux
   auto sweep_var = [&](R u0, R& un, R& dual, int v) {
@@ -460,7 +525,10 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
     const R gf = face_grad(inner, phi, phim, dxr);
     const R gfp = lane_down1(gf);
     R g = R(0);
-    if (o2) {
+    if (LIM == kLimG && o2) {  // computeAndLimitGradPhi, UtilesRemap.cc:329-338
+      const R phip = lane_down1(phi);
+      g = limiter_g<R>(P.lim, gfp, gf, front_in ? phip : phi, inner ? phim : phi, hc, hcp, hcm);
+    } else if (o2) {
       if (LEARN) {
         const int role = v < ne ? 0 : v < 2 * ne ? 1 : v < 3 * ne ? 2 : 3;
         const int k = role == 0 ? v : role == 1 ? v - ne : role == 2 ? v - 2 * ne : 0;
@@ -690,6 +758,13 @@ __device__ __forceinline__ void node_sweep_x_2d_body(const Params& P, DevScalars
   // and a sum of doubles never becomes -0, so the result is the guarded
   // sum's bit for bit).
   const double xd_l = A.xnx[nid];
+  // generalised limiters: x of the nodes two places back and front (beyond
+  // the lanes a wavefront exchanges)
+  double xmm = 0., xpp = 0.;
+  if (LIM == kLimG) {
+    xmm = A.xnx[(ex && i >= 2) ? nid - 2 : 0];
+    xpp = A.xnx[(ex && i + 2 <= N) ? nid + 2 : 0];
+  }
   R cf[2][NEM];
   bool cv[2];
 #pragma unroll
@@ -747,8 +822,15 @@ __device__ __forceinline__ void node_sweep_x_2d_body(const Params& P, DevScalars
       const R dxf = (R)(xdp - xd), dxb = (R)(xd - xdm);
       const R gf0 = qdiv(p0p - p0, dxf), gb0 = qdiv(p0 - p0m, dxb);
       const R gf1 = qdiv(p1p - p1, dxf), gb1 = qdiv(p1 - p1m, dxb);
-      g0 = limited_grad<LIM, false, R, kBranchFree<R>>(gb0, gf0);
-      g1 = limited_grad<LIM, false, R, kBranchFree<R>>(gb1, gf1);
+      if (LIM == kLimG) {  // computeDualGradPhi, UtilesRemap.cc:15-68, 350-382
+        const R h0 = (R)(0.5 * (xdp - xdm));
+        const R hm = (R)(i >= 2 ? 0.5 * (xd - xmm) : 0.), hp = (R)(i <= N - 2 ? 0.5 * (xpp - xd) : 0.);
+        g0 = limiter_g<R>(P.lim, gf0, gb0, p0p, p0m, h0, hp, hm);
+        g1 = limiter_g<R>(P.lim, gf1, gb1, p1p, p1m, h0, hp, hm);
+      } else {
+        g0 = limited_grad<LIM, false, R, kBranchFree<R>>(gb0, gf0);
+        g1 = limited_grad<LIM, false, R, kBranchFree<R>>(gb1, gf1);
+      }
       if (!inner_t && out) {  // kept for the sweep in the other direction
         A.dg0[nid] = g0;
         A.dg1[nid] = g1;
@@ -904,6 +986,13 @@ __device__ __forceinline__ void node_sweep_ytile_2d_body(const Params& P, DevSca
   const R p0m = lane_up1(p0), p0p = lane_down1(p0);
   const R p1m = lane_up1(p1), p1p = lane_down1(p1);
   const R back = lane_up1(front);
+  // generalised limiters: y of the nodes two rows back and front (beyond
+  // the tile's halo rows)
+  double xmm = 0., xpp = 0.;
+  if (LIM == kLimG) {
+    xmm = A.xny[(ex && s >= 2) ? i + (long)nnx * (s - 2) : 0];
+    xpp = A.xny[(ex && s + 2 <= N) ? i + (long)nnx * (s + 2) : 0];
+  }
   R g0 = R(0), g1 = R(0);
   if (P.order2 > 0 && ex) {
     const long q = i + (long)nnx * s;
@@ -911,8 +1000,15 @@ __device__ __forceinline__ void node_sweep_ytile_2d_body(const Params& P, DevSca
       const R dxf = (R)(xdp - xd), dxb = (R)(xd - xdm);
       const R gf0 = qdiv(p0p - p0, dxf), gb0 = qdiv(p0 - p0m, dxb);
       const R gf1 = qdiv(p1p - p1, dxf), gb1 = qdiv(p1 - p1m, dxb);
-      g0 = limited_grad<LIM, false, R, kBranchFree<R>>(gb0, gf0);
-      g1 = limited_grad<LIM, false, R, kBranchFree<R>>(gb1, gf1);
+      if (LIM == kLimG) {  // computeDualGradPhi, UtilesRemap.cc:15-68, 350-382
+        const R h0 = (R)(0.5 * (xdp - xdm));
+        const R hm = (R)(s >= 2 ? 0.5 * (xd - xmm) : 0.), hp = (R)(s <= N - 2 ? 0.5 * (xpp - xd) : 0.);
+        g0 = limiter_g<R>(P.lim, gf0, gb0, p0p, p0m, h0, hp, hm);
+        g1 = limiter_g<R>(P.lim, gf1, gb1, p1p, p1m, h0, hp, hm);
+      } else {
+        g0 = limited_grad<LIM, false, R, kBranchFree<R>>(gb0, gf0);
+        g1 = limited_grad<LIM, false, R, kBranchFree<R>>(gb1, gf1);
+      }
       if (!inner_t && out) {  // kept for the sweep in the other direction
         A.dg0[q] = g0;
         A.dg1[q] = g1;

@@ -112,6 +112,44 @@ __host__ __device__ inline T limited_grad(T gb, T gf) {
   return g;
 }
 
+// Template value of the sweeps' LIM for the generalised limiters; the
+// limiter id (MHG_LIM_MINMODG .. MHG_LIM_ULTRABEEG) is then Params::lim.
+constexpr int kLimG = 3;
+
+// Generalised limiters, fluxLimiterG (UtilesRemap.cc:127-173): slope of a
+// cell (or dual cell) of width h0 between a front neighbour of width hp and a
+// back neighbour of width hm, from the front / back gradients gp, gm and the
+// front / back values yp, ym.  Zero at a slope break; otherwise the
+// limiter's candidate, bounded by min(|gp (h0+hp)/h0|, |gm (h0+hm)/h0|), with
+// the sign of yp - ym.
+template <class T>
+__host__ __device__ inline T limiter_g(int id, T gp, T gm, T yp, T ym, T h0, T hp, T hm) {
+  if (gp * gm < T(0)) return T(0);
+  const bool up = (yp - ym) > T(0);
+  const T agp = fabs(gp), agm = fabs(gm);
+  T grady = T(0);
+  if (id == MHG_LIM_MINMODG) {  // formula 9c
+    grady = up ? smin(agp, agm) : -smin(agp, agm);
+  } else if (id == MHG_LIM_SUPERBEEG) {  // formula 9g
+    grady = up ? smax(agp, agm) : -smax(agp, agm);
+  } else if (id == MHG_LIM_VANLEERG || id == MHG_LIM_ARITHMETICG) {  // formulas 9e and the mean
+    const T lp = qdiv(h0 / T(2) + hp, h0 + hp + hm);
+    const T lm = qdiv(h0 / T(2) + hm, h0 + hp + hm);
+    if (id == MHG_LIM_VANLEERG) {
+      const T den = lp * gp + lm * gm;
+      grady = den != T(0) ? qdiv(gp * gm, den) : T(0);
+    } else {
+      grady = lm * gp + lp * gm;
+    }
+  } else if (id == MHG_LIM_ULTRABEEG) {
+    grady = qdiv(yp - ym, h0);
+  }
+  // simple-slope limit (formula 10)
+  const T gM = smin(fabs(qdiv(gp * (h0 + hp), h0)), fabs(qdiv(gm * (h0 + hm), h0)));
+  const T g = smin(fabs(gM), fabs(grady));
+  return up ? g : -g;
+}
+
 // compute_eint, MahycoModule.cc:1630-1654.
 template <class T>
 __host__ __device__ inline T compute_eint(bool pseudo_centree, T adiabatic_cst, T q_n, T q, T rho_n,

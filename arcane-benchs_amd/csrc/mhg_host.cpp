@@ -529,13 +529,17 @@ int build_params(const mhg_mesh_desc* mesh, const mhg_options* opt, Params* Pout
     return MHG_E_ARG;
   if (opt->schema_csts || opt->with_newton || opt->sans_lagrange || !opt->with_projection ||
       !opt->is_euler_scheme || opt->projection_pente_borne || opt->conservation_energie_totale ||
-      (opt->projection_limiteur_id >= MHG_LIM_MINMODG && opt->projection_limiteur_id != MHG_LIM_LEARNED &&
+      (opt->projection_limiteur_id > MHG_LIM_ULTRABEEG && opt->projection_limiteur_id != MHG_LIM_LEARNED &&
        opt->projection_limiteur_id != MHG_LIM_LEARNED_ALL) ||
       opt->projection_limiteur_id < 0 || opt->ordre_projection < 1 || opt->ordre_projection > 2)
     return MHG_E_UNSUPPORTED;
   const bool learned = opt->projection_limiteur_id == MHG_LIM_LEARNED ||
                        opt->projection_limiteur_id == MHG_LIM_LEARNED_ALL;
   if (learned && mesh->dim != 2) return MHG_E_UNSUPPORTED;
+  // the generalised limiters (fluxLimiterG) run on 2D meshes
+  if (opt->projection_limiteur_id >= MHG_LIM_MINMODG && opt->projection_limiteur_id <= MHG_LIM_ULTRABEEG &&
+      mesh->dim != 2)
+    return MHG_E_UNSUPPORTED;
   if (opt->precision != MHG_PREC_FP64 && opt->precision != MHG_PREC_FP32_BULK) return MHG_E_UNSUPPORTED;
   Params P{};
   P.dim = mesh->dim;

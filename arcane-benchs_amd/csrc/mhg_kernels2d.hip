@@ -34,12 +34,15 @@ __global__ void k_begin_iteration(DevScalars* S) {
 }
 
 // ---------------------------------------------------------------------------
-// Single-mesh step kernels: one block of the grid is one block of the body
-// (the sweeps map the block index over the XCDs first).
+// Single-mesh step kernels: one block of the grid is one block of the body.
+// The sweeps, the node phase and the final cell phase map the block index so
+// that each XCD runs a contiguous range of body blocks (the rows a block
+// shares with its neighbours then stay in that XCD's L2); the cell Lagrange
+// phase measured no gain from it and keeps the hardware order.
 // ---------------------------------------------------------------------------
 template <class R, bool VEL, bool BC, bool POS>
 __global__ void k_node_lagrange_2d(Params P, DevScalars* S, NodeLagArgsT<R> A) {
-  node_lagrange_2d_body<R, VEL, BC, POS>(P, S, A, blockIdx.x);
+  node_lagrange_2d_body<R, VEL, BC, POS>(P, S, A, xcd_block(blockIdx.x, gridDim.x));
 }
 
 template <class R, bool PC, int NEM>
@@ -60,7 +63,7 @@ __global__ __launch_bounds__(1024) void k_cell_sweep_ytile_2d(Params P, DevScala
 
 template <class R, int BLOCK, int NEM>
 __global__ __launch_bounds__(BLOCK) void k_cell_final_2d(Params P, DevScalars* S, CellFinalArgsT<R> A) {
-  cell_final_2d_body<R, BLOCK, NEM>(P, S, A, blockIdx.x, A.block_min);
+  cell_final_2d_body<R, BLOCK, NEM>(P, S, A, xcd_block(blockIdx.x, gridDim.x), A.block_min);
 }
 
 template <int BLOCK, bool ADVANCE>
@@ -750,7 +753,11 @@ static void launch_lane_x(hipStream_t st, const Params& P, DevScalars* S, const 
 #define XL_(LM)                                                                                    \
   if (first) hipLaunchKernelGGL((k_cell_sweep_lane_2d<R, 0, NEM, true, LM>), g, b, 0, st, P, S, A); \
   else hipLaunchKernelGGL((k_cell_sweep_lane_2d<R, 0, NEM, false, LM>), g, b, 0, st, P, S, A)
-  MHG_LIM_DISPATCH(P.lim, XL_);
+  if (P.lim >= kLimG) {
+    XL_(kLimG);
+  } else {
+    MHG_LIM_DISPATCH(P.lim, XL_);
+  }
 #undef XL_
 }
 
@@ -766,7 +773,11 @@ static void launch_tile_y(hipStream_t st, const Params& P, DevScalars* S, const 
 #define YT_(LM)                                                                                  \
   if (first) hipLaunchKernelGGL((k_cell_sweep_ytile_2d<R, NEM, true, LM>), g, b, 0, st, P, S, A); \
   else hipLaunchKernelGGL((k_cell_sweep_ytile_2d<R, NEM, false, LM>), g, b, 0, st, P, S, A)
-  MHG_LIM_DISPATCH(P.lim, YT_);
+  if (P.lim >= kLimG) {
+    YT_(kLimG);
+  } else {
+    MHG_LIM_DISPATCH(P.lim, YT_);
+  }
 #undef YT_
 }
 
@@ -780,7 +791,8 @@ void launch_cell_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const 
     return;
   }
   if constexpr (std::is_same<R, double>::value) {
-    switch (P.learned ? -1 : variant) {  // the column walkers have no learned form
+    // the column walkers have no learned or generalised-limiter form
+    switch (P.learned || P.lim >= kLimG ? -1 : variant) {
       case 0: launch_y<32, 4, 2>(st, P, S, A, first); return;
       case 1: launch_y<8, 4, 2>(st, P, S, A, first); return;
       case 2: launch_y<16, 4, 4>(st, P, S, A, first); return;
@@ -853,8 +865,9 @@ template <class R>
 void launch_node_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const NodeSweepArgsT<R>& A,
                           int dir, bool first, int variant) {
   if constexpr (std::is_same<R, double>::value) {
-    // the 5-node stencil (both directions) and the y column walkers
-    if (variant <= 0 || variant >= node_sweep_variants(dir) || (dir == 1 && variant <= 3)) {
+    // the 5-node stencil (both directions) and the y column walkers (no
+    // generalised-limiter form)
+    if (P.lim < kLimG && (variant <= 0 || variant >= node_sweep_variants(dir) || (dir == 1 && variant <= 3))) {
       launch_node_fp64_variant(st, P, S, A, dir, first, variant <= 0 || variant >= node_sweep_variants(dir) ? 0 : variant);
       return;
     }
@@ -873,7 +886,11 @@ void launch_node_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const 
     else if (P.ne <= 4) NX_(4, LM); \
     else NX_(8, LM);                \
   } while (0)
-    MHG_LIM_DISPATCH(P.lim, NXL_);
+    if (P.lim >= kLimG) {
+      NXL_(kLimG);
+    } else {
+      MHG_LIM_DISPATCH(P.lim, NXL_);
+    }
 #undef NXL_
 #undef NX_
     return;
@@ -890,7 +907,11 @@ void launch_node_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const 
     else if (P.ne <= 4) NT_(4, LM); \
     else NT_(8, LM);                \
   } while (0)
-  MHG_LIM_DISPATCH(P.lim, NTL_);
+  if (P.lim >= kLimG) {
+    NTL_(kLimG);
+  } else {
+    MHG_LIM_DISPATCH(P.lim, NTL_);
+  }
 #undef NTL_
 #undef NT_
 }

@@ -26,6 +26,7 @@ EOS_PERFECT_GAS, EOS_STIFFENED_GAS = 0, 1
 LC_RACINE_CUBIQUE_VOLUME, LC_FACES_OPPOSEES = 0, 1
 PREC_FP64, PREC_FP32_BULK = 0, 1
 LIM_MINMOD, LIM_SUPERBEE, LIM_VANLEER, LIM_LEARNED, LIM_LEARNED_ALL = 0, 1, 2, 64, 65
+LIM_MINMODG, LIM_SUPERBEEG, LIM_VANLEERG, LIM_ARITHMETICG, LIM_ULTRABEEG = 3, 4, 5, 6, 7
 
 
 class MeshDesc(C.Structure):

@@ -84,7 +84,12 @@ typedef struct {
 /* Characteristic length (Mahyco.axl "longueur-caracteristique"). */
 enum { MHG_LC_RACINE_CUBIQUE_VOLUME = 0, MHG_LC_FACES_OPPOSEES = 1 };
 
-/* Limiters (TypesMahyco.h:65-74).  MHG_LIM_LEARNED is an extension with no
+/* Limiters (TypesMahyco.h:65-74).  The generalised limiters MHG_LIM_MINMODG
+ * .. MHG_LIM_ULTRABEEG (fluxLimiterG, UtilesRemap.cc:127-173) take the cell
+ * widths along the sweep into account (h_cell, RemapADIService.cc:223-255, and
+ * the dual widths of UtilesRemap.cc:46-57); they run on 2D meshes (single and
+ * batched), where the reference runs them on the host only.
+ * MHG_LIM_LEARNED is an extension with no
  * counterpart in the reference: the cell sweeps take the slope
  * alpha_dir(c) * (g_back + g_front) / 2, with alpha_dir(c) a per-cell,
  * per-direction coefficient, and fall back to the minmod-limited slope
@@ -92,6 +97,7 @@ enum { MHG_LC_RACINE_CUBIQUE_VOLUME = 0, MHG_LC_FACES_OPPOSEES = 1 };
  * the cell and its neighbours along the sweep, and wherever alpha_dir(c) < 0;
  * the dual sweeps use minmod. */
 enum { MHG_LIM_MINMOD = 0, MHG_LIM_SUPERBEE = 1, MHG_LIM_VANLEER = 2, MHG_LIM_MINMODG = 3,
+       MHG_LIM_SUPERBEEG = 4, MHG_LIM_VANLEERG = 5, MHG_LIM_ARITHMETICG = 6, MHG_LIM_ULTRABEEG = 7,
        MHG_LIM_LEARNED = 64, MHG_LIM_LEARNED_ALL = 65 };
 
 /* Arithmetic of the bulk fields. */
@@ -119,7 +125,7 @@ typedef struct {
   double remap_threshold;             /* default 1e-10 */
   int32_t is_euler_scheme;            /* must be 1     */
   int32_t ordre_projection;           /* 1 or 2        */
-  int32_t projection_limiteur_id;     /* < MHG_LIM_MINMODG, or MHG_LIM_LEARNED */
+  int32_t projection_limiteur_id;     /* MHG_LIM_MINMOD .. MHG_LIM_ULTRABEEG, or MHG_LIM_LEARNED[_ALL] */
   int32_t projection_pente_borne;     /* must be 0     */
   int32_t conservation_energie_totale;/* must be 0     */
   /* Extensions (no counterpart in the reference). */

@@ -77,6 +77,39 @@ inline double limiter(int id, double r) {
   }
 }
 
+// Generalised limiters, fluxLimiterG (UtilesRemap.cc:127-173): the slope of
+// a cell (or dual cell) of width h0 between neighbours of widths hp (front)
+// and hm (back), from the front and back gradients gp, gm and the front and
+// back values yp, ym.  Zero at a slope break (gp gm < 0); otherwise the
+// limiter's candidate slope, bounded by the simple-slope limit
+// min(|gp (h0+hp)/h0|, |gm (h0+hm)/h0|), with the sign of yp - ym.
+inline double flux_limiter_g(int id, double gp, double gm, double yp, double ym, double h0, double hp,
+                             double hm) {
+  if (gp * gm < 0.0) return 0.;
+  const bool up = (yp - ym) > 0.;
+  double grady = 0.;
+  if (id == MHG_LIM_MINMODG) {         // formula 9c
+    grady = up ? smin(std::fabs(gp), std::fabs(gm)) : -smin(std::fabs(gp), std::fabs(gm));
+  } else if (id == MHG_LIM_SUPERBEEG) {  // formula 9g
+    grady = up ? smax(std::fabs(gp), std::fabs(gm)) : -smax(std::fabs(gp), std::fabs(gm));
+  } else if (id == MHG_LIM_VANLEERG) {   // formula 9e
+    const double lp = (h0 / 2. + hp) / (h0 + hp + hm);
+    const double lm = (h0 / 2. + hm) / (h0 + hp + hm);
+    grady = (lp * gp + lm * gm) != 0. ? gp * gm / (lp * gp + lm * gm) : 0.;
+  } else if (id == MHG_LIM_ULTRABEEG) {
+    grady = (yp - ym) / h0;
+  } else if (id == MHG_LIM_ARITHMETICG) {
+    const double lp = (h0 / 2. + hp) / (h0 + hp + hm);
+    const double lm = (h0 / 2. + hm) / (h0 + hp + hm);
+    grady = lm * gp + lp * gm;
+  }
+  // simple-slope limit (formula 10)
+  const double gMp = gp * (h0 + hp) / h0;
+  const double gMm = gm * (h0 + hm) / h0;
+  const double gM = smin(std::fabs(gMp), std::fabs(gMm));
+  return up ? smin(std::fabs(gM), std::fabs(grady)) : -smin(std::fabs(gM), std::fabs(grady));
+}
+
 // Learned slope of one cell and variable (extension, MHG_LIM_LEARNED): the
 // central slope alpha (g_b + g_f) / 2, kept if the face values it gives,
 // phi + g db at the back face and phi + g df at the front face, lie within
@@ -146,8 +179,11 @@ inline bool is_learned(const mhg_options& o) {
 inline int classical_limiter(const mhg_options& o) {
   return is_learned(o) ? MHG_LIM_MINMOD : o.projection_limiteur_id;
 }
+inline bool is_generalised(const mhg_options& o) {
+  return o.projection_limiteur_id >= MHG_LIM_MINMODG && o.projection_limiteur_id <= MHG_LIM_ULTRABEEG;
+}
 inline bool has_limited_gradients(const mhg_options& o) {
-  return o.projection_limiteur_id < MHG_LIM_MINMODG || is_learned(o);
+  return o.projection_limiteur_id < MHG_LIM_MINMODG || is_learned(o) || is_generalised(o);
 }
 
 // compute_eint, MahycoModule.cc:1630-1654.
@@ -297,6 +333,7 @@ struct mho_ctx {
   std::vector<R3> face_coord, face_normal, face_length_lagrange;
   std::vector<double> face_normal_velocity, deltax_lagrange;
   std::vector<double> grad_phi_face, phi_face;  // [nf][nvar]
+  std::vector<double> h_cell;                   // [nc] cell width along the sweep
   // boundary conditions: node lists per bc
   std::vector<std::vector<long>> bc_nodes;
   // scalars
@@ -1018,7 +1055,28 @@ void cellDirFaces(const mho_ctx& S, long c, int d, long* back, long* front) {
   *front = S.faceId(d, q[0], q[1], q[2]);
 }
 
+// Cell widths along idir (computeGradPhiFace, HCELL_BY_CELLS,
+// RemapADIService.cc:223-255): the distances from the cell centre to the
+// centres of its back and front faces, the face on the mesh boundary
+// excluded.
+void computeCellWidths(mho_ctx& S, int idir) {
+#pragma omp parallel for schedule(static)
+  for (long c = 0; c < S.NC; ++c) {
+    int id[3];
+    S.cellIdx(c, id);
+    long bf, ff;
+    cellDirFaces(S, c, idir, &bf, &ff);
+    double h = 0.;
+    if (id[idir] > 0) h += normL2(S.face_coord[bf] - S.cell_coord[c]);
+    if (id[idir] < S.n[idir] - 1) h += normL2(S.face_coord[ff] - S.cell_coord[c]);
+    S.h_cell[c] = h;
+  }
+}
+
 // computeGradPhiCell, RemapADIService.cc:283-317 -> _PBorn0_LimC 447-517.
+// Generalised limiters: the generic loop 349-396 with computeAndLimitGradPhi
+// (UtilesRemap.cc:306-339), whose back / front cell is the cell itself on the
+// mesh boundary.
 void computeGradPhiCell(mho_ctx& S, int idir) {
   const int nv = S.nvar;
   std::fill(S.grad_phi.begin(), S.grad_phi.end(), 0.);
@@ -1053,6 +1111,21 @@ void computeGradPhiCell(mho_ctx& S, int idir) {
         if (cons) gcl[v] = learned_grad(-1., phim, phi, phip, back_in, front_in, gb, gf, db, df);
       }
       if (cons) consistent_env_slopes(S.nenv, ph, g, gcl, used, S.opt.remap_threshold);
+    }
+  } else if (S.opt.ordre_projection > 1 && S.opt.projection_pente_borne == 0 && is_generalised(S.opt)) {
+    computeCellWidths(S, idir);
+    const int lim = S.opt.projection_limiteur_id;
+#pragma omp parallel for schedule(static)
+    for (long c = 0; c < S.NC; ++c) {
+      long bf, ff, pb, nb, pf, nf;
+      cellDirFaces(S, c, idir, &bf, &ff);
+      const long bc = isInnerFaceOfDir(S, bf, idir, &pb, &nb) ? pb : c;
+      const long fc = isInnerFaceOfDir(S, ff, idir, &pf, &nf) ? nf : c;
+      for (int v = 0; v < nv; ++v)
+        S.grad_phi[c * nv + v] =
+            flux_limiter_g(lim, S.grad_phi_face[ff * nv + v], S.grad_phi_face[bf * nv + v],
+                           S.phi_lagrange[fc * nv + v], S.phi_lagrange[bc * nv + v], S.h_cell[c],
+                           S.h_cell[fc], S.h_cell[bc]);
     }
   } else if (S.opt.ordre_projection > 1 && S.opt.projection_pente_borne == 0 &&
              S.opt.projection_limiteur_id < MHG_LIM_MINMODG) {
@@ -1161,6 +1234,31 @@ void computeDualGradPhi(mho_ctx& S, int idir) {
               (S.node_coord[fn][idir] - S.node_coord[nd][idir]);
       gb[v] = (S.phi_dual_lagrange[nd * nv + v] - S.phi_dual_lagrange[bn * nv + v]) /
               (S.node_coord[nd][idir] - S.node_coord[bn][idir]);
+    }
+    if (is_generalised(S.opt)) {
+      // computeDualGradPhi / computeAndLimitGradPhiDual, UtilesRemap.cc:15-68,
+      // 350-382: dual widths from the node coordinates along idir, the width
+      // beyond a boundary node taken as 0.  (On 3 nodes along idir the
+      // reference reads a missing node here; its width is taken as 0 too.)
+      auto X = [&](long n) { return S.node_coord[n][idir]; };
+      const bool bbe = id[idir] >= 2, ffe = id[idir] <= S.nn[idir] - 3;
+      const double h0 = 0.5 * (X(fn) - X(bn));
+      double hm, hp;
+      if (!bbe) {
+        hm = 0.;
+        hp = ffe ? 0.5 * (X(fn + delta) - X(nd)) : 0.;
+      } else if (!ffe) {
+        hp = 0.;
+        hm = 0.5 * (X(nd) - X(bn - delta));
+      } else {
+        hm = 0.5 * (X(nd) - X(bn - delta));
+        hp = 0.5 * (X(fn + delta) - X(nd));
+      }
+      for (int v = 0; v < 3; ++v)
+        S.dual_grad_phi[nd * nv + v] =
+            flux_limiter_g(S.opt.projection_limiteur_id, gf[v], gb[v], S.phi_dual_lagrange[fn * nv + v],
+                           S.phi_dual_lagrange[bn * nv + v], h0, hp, hm);
+      continue;
     }
     for (int v = 0; v < 3; ++v) {
       double& g = S.dual_grad_phi[nd * nv + v];
@@ -1589,6 +1687,7 @@ void allocAll(mho_ctx& S) {
   S.deltax_lagrange.assign(NF, 0.);
   S.grad_phi_face.assign((size_t)NF * nv, 0.);
   S.phi_face.assign((size_t)NF * nv, 0.);
+  S.h_cell.assign(NC, 0.);
 }
 
 // Mesh generation and the geometry-only part of hydroStartInit.
@@ -2024,6 +2123,19 @@ double mho_probe_cqs(int32_t dim, const double* coord, double* out_cqs) {
   return volume * (1. / dim);
 }
 double mho_probe_limiter(int32_t id, double r) { return limiter(id, r); }
+
+int mho_probe_cell_widths(mho_ctx* c, int32_t idir, double* out) {
+  if (!c || !out || idir < 0 || idir >= c->dim) return MHG_E_ARG;
+  computeFaceQuantitesForRemap(*c);
+  computeCellWidths(*c, idir);
+  std::copy(c->h_cell.begin(), c->h_cell.end(), out);
+  return MHG_OK;
+}
+
+double mho_probe_limiter_g(int32_t id, double gp, double gm, double yp, double ym, double h0, double hp,
+                           double hm) {
+  return flux_limiter_g(id, gp, gm, yp, ym, h0, hp, hm);
+}
 
 double mho_probe_learned_grad(double alpha, double phim, double phi, double phip, int32_t back_in,
                               int32_t front_in, double gb, double gf, double db, double df) {

@@ -64,6 +64,15 @@ double mho_probe_cqs(int32_t dim, const double* coord, double* out_cqs);
 /* Limiters of RemapADIService.h:260-286 (id 0 minmod, 1 superbee, 2 vanLeer,
  * other: order 1). */
 double mho_probe_limiter(int32_t id, double r);
+/* Generalised limiters (fluxLimiterG, UtilesRemap.cc:127-173; id
+ * MHG_LIM_MINMODG .. MHG_LIM_ULTRABEEG): slope from the front / back
+ * gradients, front / back values and the widths h0 (cell), hp (front
+ * neighbour), hm (back neighbour). */
+double mho_probe_limiter_g(int32_t id, double gp, double gm, double yp, double ym, double h0, double hp,
+                           double hm);
+/* Cell widths along idir (RemapADIService.cc:223-255) of the present
+ * coordinates, out [N_c]. */
+int mho_probe_cell_widths(mho_ctx* ctx, int32_t idir, double* out);
 /* Learned-limiter slope of one cell and variable (MHG_LIM_LEARNED). */
 double mho_probe_learned_grad(double alpha, double phim, double phi, double phip, int32_t back_in,
                               int32_t front_in, double gb, double gf, double db, double df);
