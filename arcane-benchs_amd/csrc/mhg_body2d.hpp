@@ -622,10 +622,21 @@ __device__ __forceinline__ void cell_final_2d_body(const Params& P, DevScalars* 
       vy[q] = A.uy[nd[q]];
     }
     const R Lc = A.L[c];
+    // total-energy fix-up: the deposit, the nodes' terms in cell node order
+    R delta_ec = R(0);
+    if (P.etot) {
+      R dk[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) dk[q] = A.dek[nd[q]];
+      R sum = R(0);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) sum += dk[q];
+      delta_ec = smax(R(0), sum);
+    }
     // Euler volume: the corner-normal formula on the initial coordinates.
     double x0[4] = {node_x0(P, i), node_x0(P, i + 1), node_x0(P, i + 1), node_x0(P, i)};
     double y0[4] = {node_y0(P, j), node_y0(P, j), node_y0(P, j + 1), node_y0(P, j + 1)};
-    const R c_g = final_cell<R, NEM>(P, A, c, quad_volume(x0, y0));
+    const R c_g = final_cell<R, NEM>(P, A, c, quad_volume(x0, y0), delta_ec);
     // CFL candidate: L / (c + max over the cell nodes of |u|)
     R vmax = R(0);
 #pragma unroll
@@ -727,6 +738,27 @@ __device__ inline R node_mass_from_cells(const Params& P, const R* m, int i, int
   return R(0.25) * msum;
 }
 
+// Total-energy fix-up (RemapADIFinal.cc:751-776).  Start of the dual
+// kinetic energy (PrepareRemap.cc:232, 243): U_4 = 1/2 m |u| and phi_4 =
+// 1/2 |u| (the norm, not its square, as the reference has it).  U4 may be
+// null (phi only).
+template <class R>
+__device__ __forceinline__ void dual_ecin_start(R m, R ux, R uy, R* U4, R* p4) {
+  const R nrm = sqrt(ux * ux + uy * uy + R(0) * R(0));
+  *p4 = R(0.5) * nrm;
+  if (U4) *U4 = R(0.5) * m * nrm;
+}
+// A node's term of the cells' deposit after the last dual sweep: a quarter
+// of U_4 / U_3 minus 1/2 |u|^2 of the remapped velocity U / U_3; +0 for a node
+// without mass (skipped by the reference; a sum that starts at +0 is the
+// same either way).
+template <class R>
+__device__ __forceinline__ R dual_ecin_term(R U0, R U1, R U3, R U4) {
+  if (U3 == R(0)) return R(0);
+  const R vx = qdiv(U0, U3), vy = qdiv(U1, U3);
+  return R(0.25) * (qdiv(U4, U3) - R(0.5) * (vx * vx + vy * vy + R(0) * R(0)));
+}
+
 // Dual sweep along x with lane exchange: a wavefront holds 64 consecutive
 // nodes of a row (60 outputs, two halo nodes on each side).  Every node forms
 // its phi_dual and its limited gradient once and reads its neighbours' values
@@ -776,6 +808,7 @@ __device__ __forceinline__ void node_sweep_x_2d_body(const Params& P, DevScalars
     for (int k = 0; k < NEM; ++k) cf[q][k] = (k < ne) ? A.contrib[k * NC + cc] : R(0);
   }
   R U0 = R(0), U1 = R(0), U3 = R(0), p0 = R(0), p1 = R(0);
+  R U4 = R(0), p4 = R(0);  // dual kinetic energy (total-energy fix-up only)
   double xd = 0.;
   if (FIRST) {
     const R mn = node_mass_from_cells_nb(P, A.m, i, j);
@@ -786,20 +819,25 @@ __device__ __forceinline__ void node_sweep_x_2d_body(const Params& P, DevScalars
       U0 = mn * p0;
       U1 = mn * p1;
       U3 = mn;
+      if (P.etot) dual_ecin_start(mn, p0, p1, &U4, &p4);
     }
   } else {
     const R u0 = A.Ud0[nid], u1 = A.Ud1[nid], u3 = A.Ud3[nid];
+    const R u4 = P.etot ? A.Ud4[nid] : R(0);
     if (ex) {
       U0 = u0;
       U1 = u1;
       U3 = u3;
+      U4 = u4;
       // nodes inner in the first direction carry phi = U/m (DualPhaseRemap.cc:494-523)
       if (inner_t) {
         p0 = qdiv(U0, U3);
         p1 = qdiv(U1, U3);
+        p4 = qdiv(U4, U3);
       } else {
         p0 = A.ux[nid];
         p1 = A.uy[nid];
+        if (P.etot) dual_ecin_start<R>(R(0), p0, p1, nullptr, &p4);
       }
     }
   }
@@ -842,9 +880,16 @@ __device__ __forceinline__ void node_sweep_x_2d_body(const Params& P, DevScalars
   }
   const R g0m = lane_up1(g0), g0p = lane_down1(g0);
   const R g1m = lane_up1(g1), g1p = lane_down1(g1);
+  R p4m = R(0), p4p = R(0);
+  if (P.etot) {
+    p4m = lane_up1(p4);
+    p4p = lane_down1(p4);
+  }
   if (!out) return;
   if (inner_d) {
     U3 += back - front;
+    // the kinetic energy is carried with the upwind value (its gradient is 0)
+    if (P.etot) U4 += back * (back > R(0) ? p4m : p4) - front * (front < R(0) ? p4p : p4);
     const R order2 = (R)P.order2;
     const R ufront = R(0.5) * (p0 + p0);  // DualPhaseRemap.cc:391
     const int signfront = (ufront > R(0)) ? 1 : -1;
@@ -870,6 +915,7 @@ __device__ __forceinline__ void node_sweep_x_2d_body(const Params& P, DevScalars
     st_stream(&A.Ud0[nid], U0);
     st_stream(&A.Ud1[nid], U1);
     st_stream(&A.Ud3[nid], U3);
+    if (P.etot) st_stream(&A.Ud4[nid], U4);
   } else if (U3 != R(0)) {
     st_stream(&A.uox[nid], qdiv(U0, U3));
     st_stream(&A.uoy[nid], qdiv(U1, U3));
@@ -877,6 +923,7 @@ __device__ __forceinline__ void node_sweep_x_2d_body(const Params& P, DevScalars
     st_stream(&A.uox[nid], A.ux[nid]);
     st_stream(&A.uoy[nid], A.uy[nid]);
   }
+  if (!FIRST && P.etot) st_stream(&A.dek[nid], dual_ecin_term(U0, U1, U3, U4));
 }
 
 // Dual sweep along y on LDS tiles: a block of 16 wavefronts owns 16 node
@@ -891,7 +938,7 @@ template <class R, bool FIRST, int NEM, int LIM, bool SEG>
 __device__ __forceinline__ void node_sweep_ytile_2d_body(const Params& P, DevScalars* S,
                                                          const NodeSweepArgsT<R>& A, int L) {
   __shared__ double ldx[kTileR * kTileP];   // coordinate along y
-  __shared__ R lds[3][kTileR * kTileP];     // phi_dual x, y and front flux; then the increments
+  __shared__ R lds[4][kTileR * kTileP];     // phi_dual x, y, front flux, kinetic energy; then the increments
   if (S->stop || S->err) return;  // uniform over the block
   const int nx = P.nx, N = P.ny, nnx = nx + 1;
   const YTileShape<SEG> sh(N + 1);
@@ -915,7 +962,7 @@ __device__ __forceinline__ void node_sweep_ytile_2d_body(const Params& P, DevSca
   const bool gex = trow && gi <= nx && gs >= 0 && gs <= N;
   const bool gout = SEG ? gex : gex && tr >= 2 && tr <= 61;
   const long gq = gex ? gi + (long)nnx * gs : 0;
-  R U0 = R(0), U1 = R(0), U3 = R(0);
+  R U0 = R(0), U1 = R(0), U3 = R(0), U4 = R(0);
   {
     // every load first, from clamped indices and without branches (one wait
     // for memory; a missing cell adds +0 to a sum that starts at +0)
@@ -931,7 +978,7 @@ __device__ __forceinline__ void node_sweep_ytile_2d_body(const Params& P, DevSca
       for (int k = 0; k < NEM; ++k) cf[q][k] = (k < ne) ? A.contrib[k * NC + cc] : R(0);
     }
     double xd = 0.;
-    R p0 = R(0), p1 = R(0), f = R(0);
+    R p0 = R(0), p1 = R(0), f = R(0), p4 = R(0);
     if (FIRST) {
       const R mn = node_mass_from_cells_nb(P, A.m, gi, gs);
       const R ux = A.ux[gq], uy = A.uy[gq];
@@ -941,20 +988,25 @@ __device__ __forceinline__ void node_sweep_ytile_2d_body(const Params& P, DevSca
         U0 = mn * p0;
         U1 = mn * p1;
         U3 = mn;
+        if (P.etot) dual_ecin_start(mn, p0, p1, &U4, &p4);
       }
     } else {
       const R u0 = A.Ud0[gq], u1 = A.Ud1[gq], u3 = A.Ud3[gq];
+      const R u4 = P.etot ? A.Ud4[gq] : R(0);
       if (gex) {
         const bool inner_t = (gi >= 1 && gi <= nx - 1);
         U0 = u0;
         U1 = u1;
         U3 = u3;
+        U4 = u4;
         if (inner_t) {  // DualPhaseRemap.cc:494-523
           p0 = qdiv(U0, U3);
           p1 = qdiv(U1, U3);
+          p4 = qdiv(U4, U3);
         } else {
           p0 = A.ux[gq];
           p1 = A.uy[gq];
+          if (P.etot) dual_ecin_start<R>(R(0), p0, p1, nullptr, &p4);
         }
       }
     }
@@ -970,6 +1022,7 @@ __device__ __forceinline__ void node_sweep_ytile_2d_body(const Params& P, DevSca
       lds[0][ra] = p0;
       lds[1][ra] = p1;
       lds[2][ra] = f;
+      lds[3][ra] = p4;
     }
   }
   __syncthreads();
@@ -1020,10 +1073,18 @@ __device__ __forceinline__ void node_sweep_ytile_2d_body(const Params& P, DevSca
   }
   const R g0m = lane_up1(g0), g0p = lane_down1(g0);
   const R g1m = lane_up1(g1), g1p = lane_down1(g1);
-  R d0 = R(0), d1 = R(0), d3 = R(0);
+  R p4 = R(0), p4m = R(0), p4p = R(0);
+  if (P.etot) {
+    p4 = tlane ? lds[3][ta] : R(0);
+    p4m = lane_up1(p4);
+    p4p = lane_down1(p4);
+  }
+  R d0 = R(0), d1 = R(0), d3 = R(0), d4 = R(0);
   if (out && inner_d) {
     const R order2 = (R)P.order2;
     d3 = back - front;
+    // the kinetic energy is carried with the upwind value (its gradient is 0)
+    if (P.etot) d4 = back * (back > R(0) ? p4m : p4) - front * (front < R(0) ? p4p : p4);
     const R ufront = R(0.5) * (p1 + p1);  // DualPhaseRemap.cc:391
     const int signfront = (ufront > R(0)) ? 1 : -1;
     const double xfr = xdp - xd;
@@ -1046,6 +1107,7 @@ __device__ __forceinline__ void node_sweep_ytile_2d_body(const Params& P, DevSca
     lds[0][ta] = d0;
     lds[1][ta] = d1;
     lds[2][ta] = d3;
+    lds[3][ta] = d4;
   }
   __syncthreads();
   // ---- row-major: U += increments, store
@@ -1054,6 +1116,7 @@ __device__ __forceinline__ void node_sweep_ytile_2d_body(const Params& P, DevSca
     U3 += lds[2][ra];
     U0 += lds[0][ra];
     U1 += lds[1][ra];
+    if (P.etot) U4 += lds[3][ra];
     const R thresold = (R)1.e-300;  // 0 in float: the filter only acts in double
     if (fabs(U0) < thresold) U0 = R(0);
     if (fabs(U1) < thresold) U1 = R(0);
@@ -1062,6 +1125,7 @@ __device__ __forceinline__ void node_sweep_ytile_2d_body(const Params& P, DevSca
     st_stream(&A.Ud0[gq], U0);
     st_stream(&A.Ud1[gq], U1);
     st_stream(&A.Ud3[gq], U3);
+    if (P.etot) st_stream(&A.Ud4[gq], U4);
   } else if (U3 != R(0)) {
     st_stream(&A.uox[gq], qdiv(U0, U3));
     st_stream(&A.uoy[gq], qdiv(U1, U3));
@@ -1069,6 +1133,7 @@ __device__ __forceinline__ void node_sweep_ytile_2d_body(const Params& P, DevSca
     st_stream(&A.uox[gq], A.ux[gq]);
     st_stream(&A.uoy[gq], A.uy[gq]);
   }
+  if (!FIRST && P.etot) st_stream(&A.dek[gq], dual_ecin_term(U0, U1, U3, U4));
 }
 
 }  // namespace mhg

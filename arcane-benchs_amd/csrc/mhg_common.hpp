@@ -419,6 +419,7 @@ struct Params {
   int learned;  // cell sweeps use the learned-limiter slope (lim is then minmod); 2: the
                 // network's coefficients also at material interfaces
   int land;     // shorten the last step to end at tend
+  int etot;     // total-energy fix-up of the remap (conservation-energie-totale)
   double cfl, dtmin, dtmax, tend;
   int nbc;
   BcRange bc[kMaxBc];

@@ -45,6 +45,7 @@ struct mhg_ctx {
   double* xx[2] = {nullptr, nullptr};
   double* xy[2] = {nullptr, nullptr};
   void *Ud0 = nullptr, *Ud1 = nullptr, *Ud3 = nullptr, *dg0 = nullptr, *dg1 = nullptr;
+  void *Ud4 = nullptr, *dek = nullptr;  // total-energy fix-up only
   // 3D: z components, a second dual state (the middle sweep reads one and
   // writes the other), the Euler volumes of the hexahedra
   double* xz[2] = {nullptr, nullptr};
@@ -218,6 +219,8 @@ NodeSweepArgsT<R> node_sweep_args(mhg_ctx* c) {
   B.dg1 = bp<R>(c->dg1);
   B.uox = bp<R>(c->ux[1 - c->ucur]);
   B.uoy = bp<R>(c->uy[1 - c->ucur]);
+  B.Ud4 = bp<R>(c->Ud4);
+  B.dek = bp<R>(c->dek);
   return B;
 }
 
@@ -245,6 +248,7 @@ CellFinalArgsT<R> cell_final_args(mhg_ctx* c) {
   F.env_f = bp<R>(c->env_f);
   F.env_mf = bp<R>(c->env_mf);
   F.block_min = c->block_min;
+  F.dek = bp<R>(c->dek);
   return F;
 }
 

@@ -528,7 +528,7 @@ int build_params(const mhg_mesh_desc* mesh, const mhg_options* opt, Params* Pout
   if (opt->nb_env < 1 || opt->nb_env > MHG_MAX_ENV || opt->nb_bc < 0 || opt->nb_bc > MHG_MAX_BC)
     return MHG_E_ARG;
   if (opt->schema_csts || opt->with_newton || opt->sans_lagrange || !opt->with_projection ||
-      !opt->is_euler_scheme || opt->projection_pente_borne || opt->conservation_energie_totale ||
+      !opt->is_euler_scheme || opt->projection_pente_borne ||
       (opt->projection_limiteur_id > MHG_LIM_ULTRABEEG && opt->projection_limiteur_id != MHG_LIM_LEARNED &&
        opt->projection_limiteur_id != MHG_LIM_LEARNED_ALL) ||
       opt->projection_limiteur_id < 0 || opt->ordre_projection < 1 || opt->ordre_projection > 2)
@@ -536,6 +536,8 @@ int build_params(const mhg_mesh_desc* mesh, const mhg_options* opt, Params* Pout
   const bool learned = opt->projection_limiteur_id == MHG_LIM_LEARNED ||
                        opt->projection_limiteur_id == MHG_LIM_LEARNED_ALL;
   if (learned && mesh->dim != 2) return MHG_E_UNSUPPORTED;
+  // the total-energy fix-up runs on 2D meshes
+  if (opt->conservation_energie_totale && mesh->dim != 2) return MHG_E_UNSUPPORTED;
   // the generalised limiters (fluxLimiterG) run on 2D meshes
   if (opt->projection_limiteur_id >= MHG_LIM_MINMODG && opt->projection_limiteur_id <= MHG_LIM_ULTRABEEG &&
       mesh->dim != 2)
@@ -566,6 +568,7 @@ int build_params(const mhg_mesh_desc* mesh, const mhg_options* opt, Params* Pout
   P.order2 = opt->ordre_projection - 1;
   P.lim = learned ? MHG_LIM_MINMOD : opt->projection_limiteur_id;
   P.learned = !learned ? 0 : (opt->projection_limiteur_id == MHG_LIM_LEARNED_ALL ? 2 : 1);
+  P.etot = opt->conservation_energie_totale ? 1 : 0;
   P.pseudo_centree = opt->pseudo_centree;
   P.lc = opt->longueur_caracteristique;
   P.cfl = opt->cfl;
@@ -659,6 +662,7 @@ int mhg_create(const mhg_mesh_desc* mesh, const mhg_options* opt, mhg_ctx** out)
   }
   r |= balloc(c, &c->Ud0, NN) | balloc(c, &c->Ud1, NN) | balloc(c, &c->Ud3, NN);
   r |= balloc(c, &c->dg0, NN) | balloc(c, &c->dg1, NN);
+  if (P.etot) r |= balloc(c, &c->Ud4, NN) | balloc(c, &c->dek, NN);
   if (P.dim == 3) {
     for (int b = 0; b < 2; ++b) r |= dalloc(c, &c->xz[b], NN) | balloc(c, &c->uz[b], NN);
     r |= balloc(c, &c->Ud2, NN) | balloc(c, &c->dg2, NN);
@@ -1233,6 +1237,8 @@ int autotune_t(mhg_ctx* c, int32_t reps, char* report, int32_t report_len) {
   B.dg1 = bp<R>(c->dg1);
   B.uox = bp<R>(c->ux[1 - c->ucur]);
   B.uoy = bp<R>(c->uy[1 - c->ucur]);
+  B.Ud4 = bp<R>(c->Ud4);
+  B.dek = bp<R>(c->dek);
   const size_t gbytes = sizeof(R) * (size_t)P.nn;
   R* saved = nullptr;
   if (hipMalloc(&saved, 2 * gbytes) != hipSuccess) return fail(c, MHG_E_ALLOC, "autotune: hipMalloc failed");

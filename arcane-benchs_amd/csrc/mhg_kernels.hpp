@@ -43,6 +43,9 @@ struct NodeSweepArgsT {
   const double *xnx, *xny;
   const R* contrib;
   R *Ud0, *Ud1, *Ud3, *dg0, *dg1, *uox, *uoy;
+  // total-energy fix-up (Params::etot): dual kinetic energy between the
+  // sweeps, and the last sweep's per-node term of the cells' energy deposit
+  R *Ud4, *dek;
 };
 
 template <class R>
@@ -53,6 +56,7 @@ struct CellFinalArgsT {
   int* mask;
   R *env_rho, *env_e, *env_p, *env_cs, *env_q, *env_m, *env_f, *env_mf;
   double* block_min;  // one CFL minimum per block of k_cell_final_2d
+  const R* dek;       // per-node terms of the total-energy fix-up (Params::etot)
 };
 
 // 3D argument blocks: the 2D block (x and y components) plus the z

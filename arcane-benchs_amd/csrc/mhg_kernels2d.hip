@@ -866,8 +866,9 @@ void launch_node_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const 
                           int dir, bool first, int variant) {
   if constexpr (std::is_same<R, double>::value) {
     // the 5-node stencil (both directions) and the y column walkers (no
-    // generalised-limiter form)
-    if (P.lim < kLimG && (variant <= 0 || variant >= node_sweep_variants(dir) || (dir == 1 && variant <= 3))) {
+    // generalised-limiter or total-energy form)
+    if (P.lim < kLimG && !P.etot &&
+        (variant <= 0 || variant >= node_sweep_variants(dir) || (dir == 1 && variant <= 3))) {
       launch_node_fp64_variant(st, P, S, A, dir, first, variant <= 0 || variant >= node_sweep_variants(dir) ? 0 : variant);
       return;
     }

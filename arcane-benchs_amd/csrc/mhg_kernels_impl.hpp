@@ -181,7 +181,8 @@ __device__ inline void lagrange_cell_bulk(const Params& P, const DevScalars* S, 
 // Final cell phase of cell c with Euler volume vold: membership, "moy", EOS,
 // pressure average; returns the cell's sound speed for the CFL candidate.
 template <class R, int NEM>
-__device__ inline R final_cell(const Params& P, const CellFinalArgsT<R>& A, long c, double vold) {
+__device__ inline R final_cell(const Params& P, const CellFinalArgsT<R>& A, long c, double vold,
+                               R delta_ec = R(0)) {
   {
     const long NC = P.nc;
     const int ne = env_count<NEM>(P);
@@ -325,10 +326,17 @@ __device__ inline R final_cell(const Params& P, const CellFinalArgsT<R>& A, long
 #pragma unroll
       for (int k = 0; k < NEM; ++k)
         if (k == kpure) mass_g = mk[k];
+      R mfp = R(0);
+#pragma unroll
+      for (int k = 0; k < NEM; ++k)
+        if (k == kpure) mfp = mf[k];
       if (rho_g < R(0) || e_g < R(0)) {
         e_g = R(0);
         rho_g = R(0);
+        mfp = R(0);
       }
+      // total-energy fix-up (RemapADIFinal.cc:751-776), before the EOS
+      if (P.etot) e_g += mfp * delta_ec;
 #pragma unroll
       for (int k = 0; k < NEM; ++k)
         if (k == kpure) eos_apply(P.eos, k, rho_g, e_g, &p_g, &c_g);
@@ -342,6 +350,14 @@ __device__ inline R final_cell(const Params& P, const CellFinalArgsT<R>& A, long
           fv[k] = R(0);
           mf[k] = R(0);
         }
+      }
+      // total-energy fix-up (RemapADIFinal.cc:751-776), before the EOS: the
+      // environments by mass fraction, the cell's average in full
+      if (P.etot) {
+#pragma unroll
+        for (int k = 0; k < NEM; ++k)
+          if ((mask >> k) & 1) ek[k] += mf[k] * delta_ec;
+        e_g += delta_ec;
       }
       // EOS per environment, then computePressionMoyenne
 #pragma unroll

@@ -1538,6 +1538,34 @@ void remapVariables(mho_ctx& S) {
   for (int k = 0; k < ne; ++k)
     for (long c = 0; c < S.NC; ++c)
       if (S.isImpure(c, k)) S.cell_volume.env(k, c) = S.fracvol.env(k, c) * S.cell_volume.g[c];
+  // total-energy fix-up (751-776): per node of the cell, a quarter of the
+  // kinetic energy the dual remap carried (U_4 / U_3, from 1/2 m |u| at the
+  // start of the remap) minus 1/2 |u|^2 of the remapped velocity; the sum,
+  // where positive, goes into the internal energies (by mass fraction) and,
+  // for a cell with other than one environment, into the cell's average.
+  if (S.opt.conservation_energie_totale) {
+#pragma omp parallel for schedule(static)
+    for (long c = 0; c < S.NC; ++c) {
+      long nodes[8];
+      const int nb = S.cellNodes(c, nodes);
+      double delta_ec = 0.;
+      for (int i = 0; i < nb; ++i) {
+        const double* Ud = &S.u_dual_lagrange[nodes[i] * nv];
+        if (Ud[3] != 0.) {
+          const R3& u = S.velocity[nodes[i]];
+          const double ec_proj = Ud[4] / Ud[3];
+          const double ec_reconst = 0.5 * (u.x * u.x + u.y * u.y + u.z * u.z);
+          delta_ec += 0.25 * (ec_proj - ec_reconst);
+        }
+      }
+      delta_ec = smax(0., delta_ec);
+      int ev[MHG_MAX_ENV];
+      const int nbe = S.envsOf(c, ev);
+      for (int q = 0; q < nbe; ++q)
+        S.EV(S.internal_energy, ev[q], c) += S.EV(S.mass_fraction, ev[q], c) * delta_ec;
+      if (nbe != 1) S.internal_energy.g[c] += delta_ec;
+    }
+  }
 }
 
 // appliRemap, RemapADIService.cc:19-60.
@@ -1784,7 +1812,7 @@ int mho_create(const mhg_mesh_desc* mesh, const mhg_options* opt, mho_ctx** out)
   if (opt->nb_env < 1 || opt->nb_env > MHG_MAX_ENV || opt->nb_bc < 0 || opt->nb_bc > MHG_MAX_BC)
     return MHG_E_ARG;
   if (opt->schema_csts || opt->with_newton || opt->sans_lagrange || !opt->is_euler_scheme ||
-      opt->projection_pente_borne || opt->conservation_energie_totale || !has_limited_gradients(*opt) ||
+      opt->projection_pente_borne || !has_limited_gradients(*opt) ||
       opt->projection_limiteur_id < 0 || opt->ordre_projection < 1 || opt->ordre_projection > 2)
     return MHG_E_UNSUPPORTED;
   for (int b = 0; b < opt->nb_bc; ++b)

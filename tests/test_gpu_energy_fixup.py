@@ -1,0 +1,145 @@
+"""Total-energy fix-up of the remap (conservation-energie-totale,
+RemapADIFinal.cc:751-776) on the GPU.
+
+FP64 against the oracle: one step within 1e-12 per field from the same state
+(Sod along x and along y, the three-material triple point with mixed cells),
+the per-entry-point path after every entry point, and the closed form of a
+uniform flow (each interior cell gains |u0| (1 - |u0|) / 2 = 0.125).  FP32-bulk
+stays within 1e-5 (L1, relative) of FP64.  3D contexts and batches refuse the
+option."""
+import numpy as np
+import pytest
+
+import helpers as H  # noqa: E402
+from mahyco_amd import abi, cases, load_gpu
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+ET = dict(conservation_energie_totale=1)
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    return load_gpu()
+
+
+def _sod_y(L, nx, ny, **kw):
+    return H.sod_setup(L, nx, ny, cas=4, **kw)
+
+
+CASES = {"sod_x": (H.sod_setup, 40, 4, 1, 7), "sod_y": (_sod_y, 4, 40, 1, 7), "tp": (H.tp_setup, 56, 24, 3, 40)}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+@pytest.mark.parametrize("lim", [abi.LIM_MINMOD, abi.LIM_VANLEERG])
+def test_one_step_matches_oracle(gpu, case, lim):
+    setup, nx, ny, ne, warm = CASES[case]
+    o = setup(H.oracle(), nx, ny, projection_limiteur_id=lim, **ET)
+    o.run(warm)
+    st = o.download()
+    g = setup(gpu, nx, ny, projection_limiteur_id=lim, **ET)
+    g.upload(st)
+    o.step()
+    g.step()
+    d = H.compare_states(g.download(), o.download(), ne, nx, ny)
+    print(case, lim, {k: f"{v:.1e}" for k, v in d.items()})
+    bad = {k: v for k, v in d.items() if v > TOL}
+    assert not bad, bad
+    g.close()
+    o.close()
+
+
+def test_trajectory_and_effect(gpu):
+    o = H.tp_setup(H.oracle(), 56, 24, **ET)
+    g = H.tp_setup(gpu, 56, 24, **ET)
+    ref = H.tp_setup(gpu, 56, 24)
+    o.run(60)
+    g.run(60)
+    ref.run(60)
+    a, b = g.download(), o.download()
+    d = H.compare_states(a, b, 3, 56, 24, metric=H.rel_max)
+    print("tp trajectory", {k: f"{v:.1e}" for k, v in d.items()})
+    assert max(d.values()) < 1e-12, d
+    # the fix-up changes the run (energy deposited where kinetic energy was lost)
+    assert not np.array_equal(a["internal_energy"], ref.download()["internal_energy"])
+    for s in (o, g, ref):
+        s.close()
+
+
+def test_uniform_flow_closed_form(gpu):
+    n = 14
+    r = {}
+    for cons in (0, 1):
+        opt = gpu.options(conservation_energie_totale=cons)
+        abi.set_bcs(opt, abi.reflecting_walls(2))
+        abi.set_envs(opt, [(abi.EOS_PERFECT_GAS, 1.4, 0.), (abi.EOS_PERFECT_GAS, 1.6, 0.)])
+        s = abi.Simulation(gpu, abi.mesh_desc(2, [n, n], [0, 0], [1.0, 1.0]), opt)
+        env = np.zeros(n * n, dtype=np.int32)
+        env.reshape(n, n)[:, n // 2:] = 1
+        s.init_cells(env, np.ones(n * n), np.ones(n * n))
+        st = s.download()
+        st["velocity"][0] = 0.3
+        st["velocity"][1] = 0.4
+        s.upload(st)
+        s.step()
+        r[cons] = np.asarray(s.download()["internal_energy"]).reshape(n, n).copy()
+        s.close()
+    de = (r[1] - r[0])[3:n - 3, 3:n - 3]
+    assert np.allclose(de, 0.125, rtol=1e-12, atol=0), (de.min(), de.max())
+
+
+def test_entry_points_match_oracle(gpu):
+    ne, nx, ny = 3, 56, 24
+    o = H.tp_setup(H.oracle(), nx, ny, **ET)
+    o.run(30)
+    g = H.tp_setup(gpu, nx, ny, **ET)
+    g.upload(o.download())
+    seq = ["begin_iteration", "save_values_at_n", "update_velocity", "apply_boundary_condition",
+           "update_position", "compute_geometric_values", "update_density",
+           "compute_artificial_viscosity", "update_energy_and_pressure", "remap"]
+    for it in range(2):
+        for name in seq:
+            o.entry(name)
+            g.entry(name)
+            d = H.compare_states(g.download(), o.download(), ne, nx, ny)
+            bad = {k: v for k, v in d.items() if v > TOL}
+            assert not bad, (it, name, bad)
+        o.compute_delta_t()
+        g.compute_delta_t()
+    g.close()
+    o.close()
+
+
+def test_autotune_keeps_result(gpu):
+    a = H.tp_setup(gpu, 112, 48, **ET)
+    b = H.tp_setup(gpu, 112, 48, **ET)
+    b.autotune(2)
+    a.run(20)
+    b.run(20)
+    da, db = a.download(), b.download()
+    for f in ["density", "internal_energy", "velocity"]:
+        assert np.array_equal(da[f], db[f]), f
+
+
+def test_fp32_close_to_fp64(gpu):
+    res = {}
+    for prec in (abi.PREC_FP64, abi.PREC_FP32_BULK):
+        g = H.tp_setup(gpu, 112, 48, precision=prec, **ET)
+        g.run(100)
+        res[prec] = np.asarray(g.download(["internal_energy"])["internal_energy"]).copy()
+        g.close()
+    l1 = np.abs(res[1] - res[0]).sum() / np.abs(res[0]).sum()
+    print("fp32 vs fp64 L1", l1)
+    assert l1 < 1e-5, l1
+
+
+def test_3d_and_batch_refused(gpu):
+    opt = gpu.options(**ET)
+    abi.set_bcs(opt, abi.reflecting_walls(3))
+    with pytest.raises(abi.MahycoError):
+        abi.Simulation(gpu, abi.mesh_desc(3, [8, 8, 8], [0, 0, 0], [1.0, 1.0, 1.0]), opt)
+    specs = cases.riemann_batch_specs(2, seed=3)
+    for s in specs:
+        s["nx"], s["ny"] = 16, 4
+    with pytest.raises(abi.MahycoError):
+        cases.riemann_batch(gpu, specs, 0.01, **ET)
