@@ -538,10 +538,6 @@ int build_params(const mhg_mesh_desc* mesh, const mhg_options* opt, Params* Pout
   if (learned && mesh->dim != 2) return MHG_E_UNSUPPORTED;
   // the total-energy fix-up runs on 2D meshes
   if (opt->conservation_energie_totale && mesh->dim != 2) return MHG_E_UNSUPPORTED;
-  // the generalised limiters (fluxLimiterG) run on 2D meshes
-  if (opt->projection_limiteur_id >= MHG_LIM_MINMODG && opt->projection_limiteur_id <= MHG_LIM_ULTRABEEG &&
-      mesh->dim != 2)
-    return MHG_E_UNSUPPORTED;
   if (opt->precision != MHG_PREC_FP64 && opt->precision != MHG_PREC_FP32_BULK) return MHG_E_UNSUPPORTED;
   Params P{};
   P.dim = mesh->dim;

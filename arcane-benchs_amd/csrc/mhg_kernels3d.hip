@@ -204,6 +204,46 @@ __device__ inline void sweep_geometry3(const Params& P, const CellSweep3ArgsT<R>
   }
 }
 
+// Width of a hexahedron along D (computeGradPhiFace, HCELL_BY_CELLS,
+// RemapADIService.cc:223-255): the distances from the cell centre (1/8 of
+// the sum of its nodes) to the centres of its back and front faces (1/4 of
+// each node, in the face's node order), the face on the mesh boundary left
+// out.  s: the cell's index along D, N: cells along D.
+template <class R, int D>
+__device__ inline double cell_width_3d(const Params& P, const CellSweep3ArgsT<R>& A, int i, int j, int k, int s,
+                                       int N) {
+  const double* X[3] = {A.b.xnx, A.b.xny, A.xnz};
+  double c[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double somme = 0.;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int di = (q == 1 || q == 2 || q == 5 || q == 6) ? 1 : 0;
+      const int dj = (q == 2 || q == 3 || q == 6 || q == 7) ? 1 : 0;
+      const int dk = q >= 4 ? 1 : 0;
+      somme += X[d][nid3(P, i + di, j + dj, k + dk)];
+    }
+    c[d] = 0.125 * somme;
+  }
+  auto face_dist = [&](int fi, int fj, int fk) {
+    long n[4];
+    face_nodes3(P, D, fi, fj, fk, n);
+    double v[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      double fc = 0.;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) fc += 0.25 * X[d][n[q]];
+      v[d] = fc - c[d];
+    }
+    return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  };
+  const double db = face_dist(i, j, k);
+  const double df = face_dist(i + (D == 0), j + (D == 1), k + (D == 2));
+  return (s == 0 ? 0. : db) + (s == N - 1 ? 0. : df);
+}
+
 // x sweep: a wavefront holds 64 consecutive cells of a row (j, k).
 template <class R, int NEM, bool FIRST, int LIM>
 __global__ __launch_bounds__(256) void k_cell_sweep_x_3d(Params P, DevScalars* S, CellSweep3ArgsT<R> A) {
@@ -261,12 +301,30 @@ __global__ __launch_bounds__(256) void k_cell_sweep_x_3d(Params P, DevScalars* S
   const R fb = (R)(-1.0 * uf * lf * dt), ff = (R)(1.0 * uf_f * lf_f * dt);
   const R dxr = (R)dx, dotr = (R)dotxf;
   const bool o2 = P.order2 > 0;
+  // generalised limiters: widths of the cell and of its back / front
+  // neighbours along D (the cell itself where the neighbour is missing)
+  bool front_in = false;
+  R hc = R(0), hcm = R(0), hcp = R(0);
+  if (LIM == kLimG) {
+    front_in = lane_down1((int)inner) != 0;
+    const double h = ex ? cell_width_3d<R, 0>(P, A, i, j, k, i, N) : 0.;
+    const double hm = lane_up1(h), hp = lane_down1(h);
+    hc = (R)h;
+    hcm = (R)(inner ? hm : h);
+    hcp = (R)(front_in ? hp : h);
+  }
   auto sweep_var = [&](R u0, R& un, R& dual) {
     const R phi = ex ? qdiv(u0, den) : R(0);
     const R phim = lane_up1(phi);
     const R gf = face_grad(inner, phi, phim, dxr);
     const R gfp = lane_down1(gf);
-    const R g = o2 ? limited_grad<LIM, true, R, kBranchFree<R>>(gf, gfp) : R(0);
+    R g = R(0);
+    if (LIM == kLimG && o2) {  // computeAndLimitGradPhi, UtilesRemap.cc:329-338
+      const R phip = lane_down1(phi);
+      g = limiter_g<R>(P.lim, gfp, gf, front_in ? phip : phi, inner ? phim : phi, hc, hcp, hcm);
+    } else if (o2) {
+      g = limited_grad<LIM, true, R, kBranchFree<R>>(gf, gfp);
+    }
     const R gm = lane_up1(g);
     const R phif = inner ? (upm ? phim + order2 * dotr * gm : phi + order2 * dotr * g) : R(0);
     const R phifp = lane_down1(phif);
@@ -360,6 +418,7 @@ __global__ __launch_bounds__(1024) void k_cell_sweep_tile_3d(Params P, DevScalar
     lds[2][tr][tc] = (double)uf;
     lds[3][tr][tc] = xf;
     lds[4][tr][tc] = lf;
+    if (LIM == kLimG) lds[5][tr][tc] = gex ? cell_width_3d<R, D>(P, A, ci, cj, ck, gs, N) : 0.;
   }
   __syncthreads();
   const int s = s0 + lane;
@@ -375,12 +434,30 @@ __global__ __launch_bounds__(1024) void k_cell_sweep_tile_3d(Params P, DevScalar
   const double lf_f = lane_down1(lf);
   const R fb = (R)(-1.0 * uf * lf * dt), ff = (R)(1.0 * uf_f * lf_f * dt);
   const R dxr = (R)dx, dotr = (R)dotxf;
+  // generalised limiters: widths of the cell and of its back / front
+  // neighbours along D (the cell itself where the neighbour is missing)
+  bool front_in = false;
+  R hc = R(0), hcm = R(0), hcp = R(0);
+  if (LIM == kLimG) {
+    front_in = lane_down1((int)inner) != 0;
+    const double h = lds[5][lane][w];
+    const double hm = lane_up1(h), hp = lane_down1(h);
+    hc = (R)h;
+    hcm = (R)(inner ? hm : h);
+    hcp = (R)(front_in ? hp : h);
+  }
   auto sweep_var = [&](R u0, R& un, R& dual) {
     const R phi = ex ? qdiv(u0, den) : R(0);
     const R phim = lane_up1(phi);
     const R gf = face_grad(inner, phi, phim, dxr);
     const R gfp = lane_down1(gf);
-    const R g = o2 ? limited_grad<LIM, true, R, kBranchFree<R>>(gf, gfp) : R(0);
+    R g = R(0);
+    if (LIM == kLimG && o2) {  // computeAndLimitGradPhi, UtilesRemap.cc:329-338
+      const R phip = lane_down1(phi);
+      g = limiter_g<R>(P.lim, gfp, gf, front_in ? phip : phi, inner ? phim : phi, hc, hcp, hcm);
+    } else if (o2) {
+      g = limited_grad<LIM, true, R, kBranchFree<R>>(gf, gfp);
+    }
     const R gm = lane_up1(g);
     const R phif = inner ? (upm ? phim + order2 * dotr * gm : phi + order2 * dotr * g) : R(0);
     const R phifp = lane_down1(phif);
@@ -610,13 +687,23 @@ __global__ __launch_bounds__(256) void k_node_sweep_x_3d(Params P, DevScalars* S
   }
   const R back = lane_up1(nd.front);
   R* dg[3] = {A.dg0, A.dg1, A.dg2};
+  // generalised limiters: dual widths (UtilesRemap.cc:46-57) from the nodes
+  // one and two places back and front along x
+  R h0 = R(0), hm = R(0), hp = R(0);
+  if (LIM == kLimG) {
+    const double xmm = A.xnx[(ex && i >= 2) ? nid - 2 : 0], xpp = A.xnx[(ex && i + 2 <= N) ? nid + 2 : 0];
+    h0 = (R)(0.5 * (xdp - xdm));
+    hm = (R)(i >= 2 ? 0.5 * (nd.xd - xmm) : 0.);
+    hp = (R)(i <= N - 2 ? 0.5 * (xpp - nd.xd) : 0.);
+  }
   if (P.order2 > 0 && ex) {
     if (inner_d) {
       const R dxf = (R)(xdp - nd.xd), dxb = (R)(nd.xd - xdm);
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
         const R gf = qdiv(pp[q] - nd.p[q], dxf), gb = qdiv(nd.p[q] - pm[q], dxb);
-        g0[q] = limited_grad<LIM, false, R, kBranchFree<R>>(gb, gf);
+        g0[q] = (LIM == kLimG) ? limiter_g<R>(P.lim, gf, gb, pp[q], pm[q], h0, hp, hm)
+                               : limited_grad<LIM, false, R, kBranchFree<R>>(gb, gf);
       }
       if (out && on_other_boundary<0>(P, i, j, k))
         for (int q = 0; q < 3; ++q) dg[q][nid] = g0[q];
@@ -701,6 +788,20 @@ __global__ __launch_bounds__(1024) void k_node_sweep_tile_3d(Params P, DevScalar
   }
   const R back = lane_up1(front);
   R* dg[3] = {A.dg0, A.dg1, A.dg2};
+  // generalised limiters: dual widths (UtilesRemap.cc:46-57) from the nodes
+  // one and two places back and front along D
+  R h0 = R(0), hm = R(0), hp = R(0);
+  if (LIM == kLimG) {
+    const double* XD = (D == 1) ? A.xny : A.xnz;
+    int ai, aj, ak, bi, bj, bk;
+    node_ijk(i, s - 2, &ai, &aj, &ak);
+    node_ijk(i, s + 2, &bi, &bj, &bk);
+    const double xmm = XD[(ex && s >= 2) ? nid3(P, ai, aj, ak) : 0];
+    const double xpp = XD[(ex && s + 2 <= N) ? nid3(P, bi, bj, bk) : 0];
+    h0 = (R)(0.5 * (xdp - xdm));
+    hm = (R)(s >= 2 ? 0.5 * (xd - xmm) : 0.);
+    hp = (R)(s <= N - 2 ? 0.5 * (xpp - xd) : 0.);
+  }
   if (P.order2 > 0 && ex) {
     const long q0 = nid3(P, ti, tj, tk);
     if (inner_d) {
@@ -708,7 +809,8 @@ __global__ __launch_bounds__(1024) void k_node_sweep_tile_3d(Params P, DevScalar
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
         const R gf = qdiv(pp[q] - p0[q], dxf), gb = qdiv(p0[q] - pm[q], dxb);
-        g0[q] = limited_grad<LIM, false, R, kBranchFree<R>>(gb, gf);
+        g0[q] = (LIM == kLimG) ? limiter_g<R>(P.lim, gf, gb, pp[q], pm[q], h0, hp, hm)
+                               : limited_grad<LIM, false, R, kBranchFree<R>>(gb, gf);
       }
       if (out && on_other_boundary<D>(P, ti, tj, tk))
         for (int q = 0; q < 3; ++q) dg[q][q0] = g0[q];
@@ -784,7 +886,8 @@ static inline unsigned nblocks3(long n, int b) { return (unsigned)((n + b - 1) /
   do {                         \
     if ((lim) == 0) F(0);      \
     else if ((lim) == 1) F(1); \
-    else F(2);                 \
+    else if ((lim) == 2) F(2); \
+    else F(kLimG);             \
   } while (0)
 
 template <class R>

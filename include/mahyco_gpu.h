@@ -87,8 +87,8 @@ enum { MHG_LC_RACINE_CUBIQUE_VOLUME = 0, MHG_LC_FACES_OPPOSEES = 1 };
 /* Limiters (TypesMahyco.h:65-74).  The generalised limiters MHG_LIM_MINMODG
  * .. MHG_LIM_ULTRABEEG (fluxLimiterG, UtilesRemap.cc:127-173) take the cell
  * widths along the sweep into account (h_cell, RemapADIService.cc:223-255, and
- * the dual widths of UtilesRemap.cc:46-57); they run on 2D meshes (single and
- * batched), where the reference runs them on the host only.
+ * the dual widths of UtilesRemap.cc:46-57); they run on 2D (single and
+ * batched) and 3D meshes, where the reference runs them on the host only.
  * MHG_LIM_LEARNED is an extension with no
  * counterpart in the reference: the cell sweeps take the slope
  * alpha_dir(c) * (g_back + g_front) / 2, with alpha_dir(c) a per-cell,
