@@ -46,6 +46,7 @@ struct mhg_ctx {
   double* xy[2] = {nullptr, nullptr};
   void *Ud0 = nullptr, *Ud1 = nullptr, *Ud3 = nullptr, *dg0 = nullptr, *dg1 = nullptr;
   void *Ud4 = nullptr, *dek = nullptr;  // total-energy fix-up only
+  void* UdB4 = nullptr;                 // its 3D second dual state
   // 3D: z components, a second dual state (the middle sweep reads one and
   // writes the other), the Euler volumes of the hexahedra
   double* xz[2] = {nullptr, nullptr};

@@ -278,6 +278,9 @@ void remap_3d(mhg_ctx* c, int dt_mode) {
     B.uoy = bp<R>(c->uy[1 - c->ucur]);
     B.uoz = bp<R>(c->uz[1 - c->ucur]);
     B.prevmask = prevmask;
+    B.Udi4 = bp<R>(pass == 2 ? c->UdB4 : c->Ud4);
+    B.Udo4 = bp<R>(pass == 0 ? c->Ud4 : c->UdB4);
+    B.dek = bp<R>(c->dek);
     {
       TimedLaunch t(c, nname[idir]);
       launch_node_sweep_3d(c->stream, P, c->S, B, idir, first, last);
@@ -291,6 +294,7 @@ void remap_3d(mhg_ctx* c, int dt_mode) {
   F.b.ux = bp<R>(c->ux[1 - c->ucur]);
   F.b.uy = bp<R>(c->uy[1 - c->ucur]);
   F.uz = bp<R>(c->uz[1 - c->ucur]);
+  F.b.dek = bp<R>(c->dek);
   F.b.rho = bp<R>(c->rho);
   F.b.e = bp<R>(c->e);
   F.b.p = bp<R>(c->p);
@@ -536,8 +540,6 @@ int build_params(const mhg_mesh_desc* mesh, const mhg_options* opt, Params* Pout
   const bool learned = opt->projection_limiteur_id == MHG_LIM_LEARNED ||
                        opt->projection_limiteur_id == MHG_LIM_LEARNED_ALL;
   if (learned && mesh->dim != 2) return MHG_E_UNSUPPORTED;
-  // the total-energy fix-up runs on 2D meshes
-  if (opt->conservation_energie_totale && mesh->dim != 2) return MHG_E_UNSUPPORTED;
   if (opt->precision != MHG_PREC_FP64 && opt->precision != MHG_PREC_FP32_BULK) return MHG_E_UNSUPPORTED;
   Params P{};
   P.dim = mesh->dim;
@@ -659,6 +661,7 @@ int mhg_create(const mhg_mesh_desc* mesh, const mhg_options* opt, mhg_ctx** out)
   r |= balloc(c, &c->Ud0, NN) | balloc(c, &c->Ud1, NN) | balloc(c, &c->Ud3, NN);
   r |= balloc(c, &c->dg0, NN) | balloc(c, &c->dg1, NN);
   if (P.etot) r |= balloc(c, &c->Ud4, NN) | balloc(c, &c->dek, NN);
+  if (P.etot && P.dim == 3) r |= balloc(c, &c->UdB4, NN);
   if (P.dim == 3) {
     for (int b = 0; b < 2; ++b) r |= dalloc(c, &c->xz[b], NN) | balloc(c, &c->uz[b], NN);
     r |= balloc(c, &c->Ud2, NN) | balloc(c, &c->dg2, NN);

@@ -93,6 +93,10 @@ struct NodeSweep3ArgsT {
   R *dg0, *dg1, *dg2;                  // persistent nodal gradients
   R *uox, *uoy, *uoz;                  // velocity out (last sweep)
   int prevmask;                        // directions swept before this one
+  // total-energy fix-up (Params::etot): dual kinetic energy before / after
+  // the sweep, and the last sweep's per-node term of the cells' deposit
+  const R* Udi4;
+  R *Udo4, *dek;
 };
 
 template <class R>

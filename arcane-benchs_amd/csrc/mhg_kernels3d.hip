@@ -515,6 +515,7 @@ template <class R>
 struct DualNode {
   R U[4];    // dual state before the sweep
   R p[3];    // phi_dual
+  R U4, p4;  // dual kinetic energy and its phi (total-energy fix-up only)
   double xd; // coordinate along D
   R front;   // front mass flux
 };
@@ -546,6 +547,7 @@ __device__ inline DualNode<R> dual_node_load(const Params& P, const NodeSweep3Ar
     Ul[2] = A.Udi2[nid];
     Ul[3] = A.Udi3[nid];
   }
+  const R u4l = (!FIRST && P.etot) ? A.Udi4[nid] : R(0);
   // front cells: index D at the node, dir1 and dir2 shifted by -1 and 0
   const int d1 = (D + 1) % 3, d2 = (D + 2) % 3;
   const int nn3[3] = {P.nx, P.ny, P.nz};
@@ -568,6 +570,7 @@ __device__ inline DualNode<R> dual_node_load(const Params& P, const NodeSweep3Ar
   DualNode<R> d;
   for (int q = 0; q < 4; ++q) d.U[q] = R(0);
   for (int q = 0; q < 3; ++q) d.p[q] = R(0);
+  d.U4 = d.p4 = R(0);
   d.xd = 0.;
   d.front = R(0);
   if (!ok) return d;
@@ -582,6 +585,11 @@ __device__ inline DualNode<R> dual_node_load(const Params& P, const NodeSweep3Ar
       d.U[q] = mn * u[q];
     }
     d.U[3] = mn;
+    if (P.etot) {  // 1/2 m |u| and 1/2 |u| (PrepareRemap.cc:232, 243)
+      const R nrm = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+      d.p4 = R(0.5) * nrm;
+      d.U4 = R(0.5) * mn * nrm;
+    }
   } else {
     for (int q = 0; q < 4; ++q) d.U[q] = Ul[q];
     const int idx[3] = {i, j, k};
@@ -590,6 +598,10 @@ __device__ inline DualNode<R> dual_node_load(const Params& P, const NodeSweep3Ar
     for (int e = 0; e < 3; ++e)
       if (((A.prevmask >> e) & 1) && idx[e] >= 1 && idx[e] <= nn[e] - 1) inner_prev = true;
     for (int q = 0; q < 3; ++q) d.p[q] = inner_prev ? qdiv(d.U[q], d.U[3]) : u[q];
+    if (P.etot) {
+      d.U4 = u4l;
+      d.p4 = inner_prev ? qdiv(d.U4, d.U[3]) : R(0.5) * sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    }
   }
   if (base[D] < nn3[D]) {
     R f = R(0);
@@ -630,7 +642,19 @@ __device__ inline void dual_increments(const Params& P, double dt, const R* pm, 
 }
 
 template <class R, bool FIRST, bool LAST>
-__device__ inline void dual_store(const NodeSweep3ArgsT<R>& A, long nid, const R* U) {
+__device__ inline void dual_store(const Params& P, const NodeSweep3ArgsT<R>& A, long nid, const R* U, R U4) {
+  if (P.etot) {
+    if (!LAST) {
+      st_stream(&A.Udo4[nid], U4);
+    } else {  // the node's term of the total-energy deposit (RemapADIFinal.cc:757-764)
+      R t = R(0);
+      if (U[3] != R(0)) {
+        const R vx = qdiv(U[0], U[3]), vy = qdiv(U[1], U[3]), vz = qdiv(U[2], U[3]);
+        t = R(0.25) * (qdiv(U4, U[3]) - R(0.5) * (vx * vx + vy * vy + vz * vz));
+      }
+      st_stream(&A.dek[nid], t);
+    }
+  }
   if (!LAST) {
     st_stream(&A.Udo0[nid], U[0]);
     st_stream(&A.Udo1[nid], U[1]);
@@ -716,25 +740,33 @@ __global__ __launch_bounds__(256) void k_node_sweep_x_3d(Params P, DevScalars* S
     gm[q] = lane_up1(g0[q]);
     gp[q] = lane_down1(g0[q]);
   }
+  R p4m = R(0), p4p = R(0);
+  if (P.etot) {
+    p4m = lane_up1(nd.p4);
+    p4p = lane_down1(nd.p4);
+  }
   if (!out) return;
   R U[4] = {nd.U[0], nd.U[1], nd.U[2], nd.U[3]};
+  R U4 = nd.U4;
   if (inner_d) {
     R dU[4];
     dual_increments<R, 0>(P, dt, pm, nd.p, pp, xdm, nd.xd, xdp, gm, g0, gp, back, nd.front, dU);
+    // the kinetic energy is carried with the upwind value (its gradient is 0)
+    if (P.etot) U4 += back * (back > R(0) ? p4m : nd.p4) - nd.front * (nd.front < R(0) ? p4p : nd.p4);
     U[3] += dU[3];
     for (int q = 0; q < 3; ++q) U[q] += dU[q];
     const R thresold = (R)1.e-300;  // 0 in float: the filter only acts in double
     for (int q = 0; q < 3; ++q)
       if (fabs(U[q]) < thresold) U[q] = R(0);
   }
-  dual_store<R, FIRST, LAST>(A, nid, U);
+  dual_store<R, FIRST, LAST>(P, A, nid, U, U4);
 }
 
 // y and z dual sweeps on LDS tiles (columns i, rows along D).
 template <class R, int D, bool FIRST, bool LAST, int NEM, int LIM>
 __global__ __launch_bounds__(1024) void k_node_sweep_tile_3d(Params P, DevScalars* S, NodeSweep3ArgsT<R> A) {
   __shared__ double ldx[kTileR][kTileP];
-  __shared__ R lds[4][kTileR][kTileP];
+  __shared__ R lds[5][kTileR][kTileP];  // phi x, y, z, front flux, kinetic energy; then the increments
   if (S->stop || S->err) return;
   const int nx = P.nx, nnx = nx + 1;
   const int N = (D == 1) ? P.ny : P.nz;  // nodes 0..N along D
@@ -768,6 +800,7 @@ __global__ __launch_bounds__(1024) void k_node_sweep_tile_3d(Params P, DevScalar
   lds[1][tr][tc] = nd.p[1];
   lds[2][tr][tc] = nd.p[2];
   lds[3][tr][tc] = nd.front;
+  lds[4][tr][tc] = nd.p4;
   __syncthreads();
   // ---- transposed: column i, row s
   const int i = i0 + w, s = s0 + lane;
@@ -823,23 +856,36 @@ __global__ __launch_bounds__(1024) void k_node_sweep_tile_3d(Params P, DevScalar
     gm[q] = lane_up1(g0[q]);
     gp[q] = lane_down1(g0[q]);
   }
-  R dU[4] = {R(0), R(0), R(0), R(0)};
-  if (out && inner_d) dual_increments<R, D>(P, dt, pm, p0, pp, xdm, xd, xdp, gm, g0, gp, back, front, dU);
+  R p4 = R(0), p4m = R(0), p4p = R(0);
+  if (P.etot) {
+    p4 = lds[4][lane][w];
+    p4m = lane_up1(p4);
+    p4p = lane_down1(p4);
+  }
+  R dU[4] = {R(0), R(0), R(0), R(0)}, dU4 = R(0);
+  if (out && inner_d) {
+    dual_increments<R, D>(P, dt, pm, p0, pp, xdm, xd, xdp, gm, g0, gp, back, front, dU);
+    // the kinetic energy is carried with the upwind value (its gradient is 0)
+    if (P.etot) dU4 = back * (back > R(0) ? p4m : p4) - front * (front < R(0) ? p4p : p4);
+  }
   __syncthreads();  // transposed reads done before the tiles are reused
 #pragma unroll
   for (int q = 0; q < 4; ++q) lds[q][lane][w] = dU[q];
+  lds[4][lane][w] = dU4;
   __syncthreads();
   // ---- row-major: U += increments, store
   if (!gout) return;
   R U[4] = {nd.U[0], nd.U[1], nd.U[2], nd.U[3]};
+  R U4 = nd.U4;
   if (gs >= 1 && gs <= N - 1) {
+    if (P.etot) U4 += lds[4][tr][tc];
     U[3] += lds[3][tr][tc];
     for (int q = 0; q < 3; ++q) U[q] += lds[q][tr][tc];
     const R thresold = (R)1.e-300;  // 0 in float: the filter only acts in double
     for (int q = 0; q < 3; ++q)
       if (fabs(U[q]) < thresold) U[q] = R(0);
   }
-  dual_store<R, FIRST, LAST>(A, gq, U);
+  dual_store<R, FIRST, LAST>(P, A, gq, U, U4);
 }
 
 // ---------------------------------------------------------------------------
@@ -852,7 +898,19 @@ __global__ __launch_bounds__(BLOCK) void k_cell_final_3d(Params P, DevScalars* S
   double dx_sound = 1.7976931348623157e308;
   if (c < P.nc) {
     const int i = (int)(c % P.nx), j = (int)((c / P.nx) % P.ny), k = (int)(c / ((long)P.nx * P.ny));
-    const R c_g = final_cell<R, NEM>(P, A.b, c, A.Veuler[c]);
+    // total-energy fix-up: the deposit, the 8 nodes' terms in cell node order
+    R delta_ec = R(0);
+    if (P.etot) {
+      R sum = R(0);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int di = (q == 1 || q == 2 || q == 5 || q == 6) ? 1 : 0;
+        const int dj = (q == 2 || q == 3 || q == 6 || q == 7) ? 1 : 0;
+        sum += A.b.dek[nid3(P, i + di, j + dj, k + (q >= 4 ? 1 : 0))];
+      }
+      delta_ec = smax(R(0), sum);
+    }
+    const R c_g = final_cell<R, NEM>(P, A.b, c, A.Veuler[c], delta_ec);
     R vmax = R(0);
 #pragma unroll
     for (int q = 0; q < 8; ++q) {

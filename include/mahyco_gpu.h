@@ -128,8 +128,8 @@ typedef struct {
   int32_t projection_limiteur_id;     /* MHG_LIM_MINMOD .. MHG_LIM_ULTRABEEG, or MHG_LIM_LEARNED[_ALL] */
   int32_t projection_pente_borne;     /* must be 0     */
   int32_t conservation_energie_totale;/* total-energy fix-up of the remap
-                                         (RemapADIFinal.cc:751-776); 2D, not
-                                         in batches */
+                                         (RemapADIFinal.cc:751-776); not in
+                                         batches */
   /* Extensions (no counterpart in the reference). */
   double visc_quad_coeff;   /* < 0: reference value (gamma+1)/2 */
   int32_t precision;        /* MHG_PREC_FP64 (the drop-in) or MHG_PREC_FP32_BULK */
