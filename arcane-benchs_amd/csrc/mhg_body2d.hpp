@@ -180,14 +180,70 @@ __device__ __forceinline__ double cell_width_2d(const double* a, const double* t
   return (first ? 0. : db) + (last ? 0. : df);
 }
 
+// Order 3 (RemapADIService.cc:583-640): the six values bbb, bb, b | f, ff,
+// fff around the back face of the cell in this lane (s: its index along the
+// sweep, N: cells along it), from the lanes three behind to two ahead; a
+// cell beyond the mesh is replaced by the last existing one, as the
+// reference's previous() / next() walk does.  Every lane runs the shifts.
+template <class R>
+__device__ __forceinline__ void o3_stencil(R v, int s, int N, R* y) {
+  const R m1 = lane_up1(v), m2 = lane_up1(m1), m3 = lane_up1(m2);
+  const R p1 = lane_down1(v), p2 = lane_down1(p1);
+  y[2] = m1;
+  y[1] = s >= 2 ? m2 : m1;
+  y[0] = s >= 3 ? m3 : y[1];
+  y[3] = v;
+  y[4] = s + 1 <= N - 1 ? p1 : v;
+  y[5] = s + 2 <= N - 1 ? p2 : y[4];
+}
+
+// Order 3 at a face whose neighbourhood is not pure (RemapADIService.cc:
+// 623-638): an environment's mass face value is its volume face value times
+// the upwind cell's mass per volume, and the energy face value the new mass
+// face value times its energy per mass, when the upwind cell (b for vdt >
+// 0, f for vdt < 0) holds the environment's volume and mass.  Roles come in
+// the order volumes, masses, energies; the per-environment registers carry
+// the volume and mass values between them.  pb, pf: the variable's value in
+// cells b and f.
+template <class R, int NEM>
+__device__ __forceinline__ void o3_mixed_faces(bool vpure, int role, int k, R vdt, R pb, R pf, R* phif, R* V,
+                                               R* Vb, R* Vf, R* M, R* Mb, R* Mf, int* up) {
+#pragma unroll
+  for (int q = 0; q < NEM; ++q) {
+    if (q != k) continue;
+    if (role == 0) {
+      V[q] = *phif;
+      Vb[q] = pb;
+      Vf[q] = pf;
+    } else if (role == 1) {
+      int u = 0;
+      if (!vpure) {
+        if (vdt > R(0) && Vb[q] != R(0) && pb != R(0)) u = 1;
+        else if (vdt < R(0) && Vf[q] != R(0) && pf != R(0)) u = 2;
+      }
+      if (u == 1) *phif = qdiv(pb, Vb[q]) * V[q];
+      if (u == 2) *phif = qdiv(pf, Vf[q]) * V[q];
+      up[q] = u;
+      M[q] = *phif;
+      Mb[q] = pb;
+      Mf[q] = pf;
+    } else if (role == 2) {
+      if (up[q] == 1) *phif = qdiv(pb, Mb[q]) * M[q];
+      if (up[q] == 2) *phif = qdiv(pf, Mf[q]) * M[q];
+    }
+  }
+}
+
 template <class R, int D, int NEM, bool FIRST, int LIM, bool LEARN = false>
 __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScalars* S, const CellSweepArgsT<R>& A,
                                                         int blk) {
   if (S->stop || S->err) return;
+  // halo lanes on each side: 3 for the order-3 stencil (six cells per face)
+  constexpr int HL = (LIM == kLimO3) ? 3 : 2, NO = 64 - 2 * HL;
   const int lane = threadIdx.x & 63;
   const int N = (D == 0) ? P.nx : P.ny;      // cells along D
   const int T = (D == 0) ? P.ny : P.nx;      // cells across D
-  const int nseg = (N + 59) / 60;
+  const int nseg = (N + NO - 1) / NO;
   int seg, line;                             // segment along D, index across D
   if (D == 0) {
     const long wave = ((long)blk * blockDim.x + threadIdx.x) >> 6;
@@ -199,9 +255,9 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
     seg = blk / ncolblk;
   }
   if (line >= T) return;  // uniform across the wave
-  const int s = seg * 60 - 2 + lane;
+  const int s = seg * NO - HL + lane;
   const bool ex = (s >= 0 && s < N);
-  const bool out = ex && lane >= 2 && lane <= 61;
+  const bool out = ex && lane >= HL && lane <= 63 - HL;
   const int i = (D == 0) ? s : line, j = (D == 0) ? line : s;
   const int nnx = P.nx + 1;
   const long NC = P.nc;
@@ -225,9 +281,9 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
   const R uda = Ud[na], udb = Ud[nb];
   const double xfa = Xd[na], xfb = Xd[nb], xta = Xt[na], xtb = Xt[nb];
   const R vlag = FIRST ? A.Vlag[c] : R(1);
-  // generalised limiters: coordinates across D of the cell's four nodes
+  // generalised limiters, order 3: coordinates across D of the cell's nodes
   double tn[4] = {0., 0., 0., 0.};
-  if (LIM == kLimG) {
+  if (LIM == kLimG || LIM == kLimO3) {
     tn[0] = Xt[m0];
     tn[1] = Xt[m0 + 1];
     tn[2] = Xt[m0 + 1 + nnx];
@@ -289,7 +345,8 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
   // generalised limiters: widths of the cell and of its back / front
   // neighbours along D (the cell itself where the neighbour is missing)
   R hc = R(0), hcm = R(0), hcp = R(0);
-  if (LIM == kLimG) {
+  R h6[6] = {R(0), R(0), R(0), R(0), R(0), R(0)};  // order 3: widths bbb .. fff of the back face
+  if (LIM == kLimG || LIM == kLimO3) {
     front_in = lane_down1((int)inner) != 0;
     const double an[4] = {xa, xb, xcc, xdd};
     const double h = ex ? (D == 0 ? cell_width_2d(an, tn, 0, 3, 1, 2, s == 0, s == N - 1)
@@ -299,7 +356,34 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
     hc = (R)h;
     hcm = (R)(inner ? hm : h);
     hcp = (R)(front_in ? hp : h);
+    if (LIM == kLimO3) o3_stencil<R>((R)h, s, N, h6);
   }
+  // order 3: the cell's mixed / pure code (-2 mixed, else its environment,
+  // -1 none) from the previous step's final fractions in the first sweep,
+  // else from phi_V > 0 (computeUremap), and the face velocity times dt
+  int est = 0;
+  R vdt = R(0);
+  if (LIM == kLimO3) {
+    if (FIRST) {
+      const int el = A.est[c];
+      est = ex ? el : 0;
+    } else {
+      int cnt = 0, last = -1;
+#pragma unroll
+      for (int k = 0; k < NEM; ++k)
+        if (k < ne && qdiv(pre[0][k], den) > R(0)) {
+          cnt++;
+          last = k;
+        }
+      est = cnt > 1 ? -2 : last;
+    }
+    vdt = (R)((double)uf * dt);
+  }
+  int estb = 0;
+  if (LIM == kLimO3) estb = lane_up1(est);
+  const bool vpure = estb != -2 && est != -2 && estb == est;
+  R o3V[NEM], o3Vb[NEM], o3Vf[NEM], o3M[NEM], o3Mb[NEM], o3Mf[NEM];
+  int o3up[NEM];
   EnvSlopes<R, NEM> es(P.remap_threshold);  // learned mode: consistent environment slopes
   // role: 0 volume, 1 mass, 2 energy of environment k, 3 the other variable
   auto sweep_var = [&](R u0, R& unew, R& dual, int role, int k) {
@@ -307,23 +391,31 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
     const R phim = lane_up1(phi);
     const R gf = face_grad(inner, phi, phim, dxr);
     const R gfp = lane_down1(gf);
-    R g = R(0);
-    if (LIM == kLimG && o2) {  // computeAndLimitGradPhi, UtilesRemap.cc:329-338
-      const R phip = lane_down1(phi);
-      g = limiter_g<R>(P.lim, gfp, gf, front_in ? phip : phi, inner ? phim : phi, hc, hcp, hcm);
-    } else if (o2) {
-      if (LEARN) {
-        bool used;
-        R gcl;
-        const R gl = learned_grad(es.trace(role, phi) ? R(-1) : alpha, phim, phi, lane_down1(phi), inner, front_in,
-                                  gf, gfp, dbr, dfr, &used, &gcl);
-        g = es.slope(role, k, phi, gl, used, gcl);
-      } else {
-        g = limited_grad<LIM, true, R, kBranchFree<R>>(gf, gfp);
+    R phif;
+    if (LIM == kLimO3) {
+      R y6[6];
+      o3_stencil<R>(phi, s, N, y6);
+      phif = inner ? o3_flux<R>(y6, h6, vdt) : R(0);
+      o3_mixed_faces<R, NEM>(vpure, role, k, vdt, phim, phi, &phif, o3V, o3Vb, o3Vf, o3M, o3Mb, o3Mf, o3up);
+    } else {
+      R g = R(0);
+      if (LIM == kLimG && o2) {  // computeAndLimitGradPhi, UtilesRemap.cc:329-338
+        const R phip = lane_down1(phi);
+        g = limiter_g<R>(P.lim, gfp, gf, front_in ? phip : phi, inner ? phim : phi, hc, hcp, hcm);
+      } else if (o2) {
+        if (LEARN) {
+          bool used;
+          R gcl;
+          const R gl = learned_grad(es.trace(role, phi) ? R(-1) : alpha, phim, phi, lane_down1(phi), inner,
+                                    front_in, gf, gfp, dbr, dfr, &used, &gcl);
+          g = es.slope(role, k, phi, gl, used, gcl);
+        } else {
+          g = limited_grad<LIM, true, R, kBranchFree<R>>(gf, gfp);
+        }
       }
+      const R gm = lane_up1(g);
+      phif = inner ? (upm ? phim + order2 * dotr * gm : phi + order2 * dotr * g) : R(0);
     }
-    const R gm = lane_up1(g);
-    const R phif = inner ? (upm ? phim + order2 * dotr * gm : phi + order2 * dotr * g) : R(0);
     const R phifp = lane_down1(phif);
     const R flux_b = fb * phif;
     const R flux_f = ff * phifp;
@@ -397,16 +489,18 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
                                                          const CellSweepArgsT<R>& A, int L) {
   // geometry tiles [0..4]; once read, tiles [0..5] are reused as the staging
   // tiles (in, out, dual flux) of a pair of variables
-  __shared__ double lds[6][kTileR * kTileP];
+  // order 3: a seventh tile for the cells' mixed / pure codes
+  __shared__ double lds[LIM == kLimO3 ? 7 : 6][kTileR * kTileP];
   R* const stg = reinterpret_cast<R*>(&lds[0][0]);
   constexpr int TS = kTileR * kTileP;  // words of one staging tile
+  constexpr int HL = (LIM == kLimO3) ? 3 : 2, NO = 64 - 2 * HL;  // halo rows, output rows
   if (S->stop || S->err) return;  // uniform over the block
   const int nx = P.nx, N = P.ny, nnx = nx + 1;
   const YTileShape<SEG> sh(N);
   const int TW = SEG ? sh.TW : kTileW, TP = SEG ? sh.TP : kTileP;
   const int ncolblk = (nx + TW - 1) / TW;
   const int i0 = (L % ncolblk) * TW;
-  const int s0 = SEG ? -2 : (L / ncolblk) * 60 - 2;  // cell row of tile row 0
+  const int s0 = SEG ? -2 : (L / ncolblk) * NO - HL;  // cell row of tile row 0
   const int t = threadIdx.x;
   // row-major position (loads, stores)
   const int tr = SEG ? t / TW : t >> 4, tc = SEG ? t % TW : t & 15;
@@ -426,7 +520,7 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
   // row-major cell of this thread
   const int gi = i0 + tc, gs = s0 + tr;
   const bool gex = trow && gi < nx && gs >= 0 && gs < N;
-  const bool gout = SEG ? gex : gex && tr >= 2 && tr <= 61;
+  const bool gout = SEG ? gex : gex && tr >= HL && tr <= 63 - HL;
   const long gc = gex ? gi + (long)nx * gs : 0;
   // Every load of the geometry phase, and the first pair of variables, is
   // issued first from clamped indices (a thread out of the mesh reads cell /
@@ -438,9 +532,9 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
   const R uya = A.uy[na], uyb = A.uy[nb];
   const double yfa = A.xny[na], yfb = A.xny[nb], xfa = A.xnx[na], xfb = A.xnx[nb];
   const R vlag = FIRST ? A.Vlag[gc] : R(1);
-  // generalised limiters: x coordinates of the cell's four nodes
+  // generalised limiters, order 3: x coordinates of the cell's four nodes
   double tn[4] = {0., 0., 0., 0.};
-  if (LIM == kLimG) {
+  if (LIM == kLimG || LIM == kLimO3) {
     tn[0] = A.xnx[m0];
     tn[1] = A.xnx[m0 + 1];
     tn[2] = A.xnx[m0 + 1 + nnx];
@@ -478,9 +572,26 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
       lds[3][ra] = xf;
       lds[4][ra] = lf;
     }
-    if (LIM == kLimG && trow) {  // width along y (faces at node rows gs and gs + 1)
+    if ((LIM == kLimG || LIM == kLimO3) && trow) {  // width along y (faces at node rows gs and gs + 1)
       const double an[4] = {ya, yb, yc, yd};
       lds[5][ra] = gex ? cell_width_2d(an, tn, 0, 1, 3, 2, gs == 0, gs == N - 1) : 0.;
+    }
+    if (LIM == kLimO3 && trow) {  // the cell's mixed / pure code (see the x sweep)
+      int e = 0;
+      if (FIRST) {
+        const int el = A.est[gc];
+        e = gex ? el : 0;
+      } else {
+        int cnt = 0, last = -1;
+#pragma unroll
+        for (int k = 0; k < NEM; ++k)
+          if (k < ne && qdiv(uvol[k], den) > R(0)) {
+            cnt++;
+            last = k;
+          }
+        e = cnt > 1 ? -2 : last;
+      }
+      lds[LIM == kLimO3 ? 6 : 5][ra] = (double)e;
     }
   }
   __syncthreads();
@@ -509,14 +620,26 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
   // generalised limiters: widths of the cell and of its back / front
   // neighbours along y (the cell itself where the neighbour is missing)
   R hc = R(0), hcm = R(0), hcp = R(0);
-  if (LIM == kLimG) {
+  R h6[6] = {R(0), R(0), R(0), R(0), R(0), R(0)};  // order 3: widths bbb .. fff of the back face
+  if (LIM == kLimG || LIM == kLimO3) {
     front_in = lane_down1((int)inner) != 0;
     const double h = tlane ? lds[5][ta] : 0.;
     const double hm = lane_up1(h), hp = lane_down1(h);
     hc = (R)h;
     hcm = (R)(inner ? hm : h);
     hcp = (R)(front_in ? hp : h);
+    if (LIM == kLimO3) o3_stencil<R>((R)h, s, N, h6);
   }
+  int est = 0, estb = 0;
+  R vdt = R(0);
+  if (LIM == kLimO3) {
+    est = tlane ? (int)lds[LIM == kLimO3 ? 6 : 5][ta] : 0;
+    estb = lane_up1(est);
+    vdt = (R)((double)uf * dt);
+  }
+  const bool vpure = estb != -2 && est != -2 && estb == est;
+  R o3V[NEM], o3Vb[NEM], o3Vf[NEM], o3M[NEM], o3Mb[NEM], o3Mf[NEM];
+  int o3up[NEM];
   EnvSlopes<R, NEM> es(P.remap_threshold);  // learned mode: consistent environment slopes
   // one variable v on the transposed layout: new U and half the dual flux
   auto sweep_var = [&](R u0, R& un, R& dual, int v) {
@@ -524,26 +647,36 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
     const R phim = lane_up1(phi);
     const R gf = face_grad(inner, phi, phim, dxr);
     const R gfp = lane_down1(gf);
-    R g = R(0);
-    if (LIM == kLimG && o2) {  // computeAndLimitGradPhi, UtilesRemap.cc:329-338
-      const R phip = lane_down1(phi);
-      g = limiter_g<R>(P.lim, gfp, gf, front_in ? phip : phi, inner ? phim : phi, hc, hcp, hcm);
-    } else if (o2) {
-      if (LEARN) {
-        const int role = v < ne ? 0 : v < 2 * ne ? 1 : v < 3 * ne ? 2 : 3;
-        const int k = role == 0 ? v : role == 1 ? v - ne : role == 2 ? v - 2 * ne : 0;
-        if (role == 1 && v == ne) es.close_volumes(ne, alpha);  // every volume is done
-        bool used;
-        R gcl;
-        const R gl = learned_grad(es.trace(role, phi) ? R(-1) : alpha, phim, phi, lane_down1(phi), inner, front_in,
-                                  gf, gfp, dbr, dfr, &used, &gcl);
-        g = es.slope(role, k, phi, gl, used, gcl);
-      } else {
-        g = limited_grad<LIM, true, R, kBranchFree<R>>(gf, gfp);
+    R phif;
+    if (LIM == kLimO3) {
+      const int role = v < ne ? 0 : v < 2 * ne ? 1 : v < 3 * ne ? 2 : 3;
+      const int k = role == 0 ? v : role == 1 ? v - ne : role == 2 ? v - 2 * ne : 0;
+      R y6[6];
+      o3_stencil<R>(phi, s, N, y6);
+      phif = inner ? o3_flux<R>(y6, h6, vdt) : R(0);
+      o3_mixed_faces<R, NEM>(vpure, role, k, vdt, phim, phi, &phif, o3V, o3Vb, o3Vf, o3M, o3Mb, o3Mf, o3up);
+    } else {
+      R g = R(0);
+      if (LIM == kLimG && o2) {  // computeAndLimitGradPhi, UtilesRemap.cc:329-338
+        const R phip = lane_down1(phi);
+        g = limiter_g<R>(P.lim, gfp, gf, front_in ? phip : phi, inner ? phim : phi, hc, hcp, hcm);
+      } else if (o2) {
+        if (LEARN) {
+          const int role = v < ne ? 0 : v < 2 * ne ? 1 : v < 3 * ne ? 2 : 3;
+          const int k = role == 0 ? v : role == 1 ? v - ne : role == 2 ? v - 2 * ne : 0;
+          if (role == 1 && v == ne) es.close_volumes(ne, alpha);  // every volume is done
+          bool used;
+          R gcl;
+          const R gl = learned_grad(es.trace(role, phi) ? R(-1) : alpha, phim, phi, lane_down1(phi), inner,
+                                    front_in, gf, gfp, dbr, dfr, &used, &gcl);
+          g = es.slope(role, k, phi, gl, used, gcl);
+        } else {
+          g = limited_grad<LIM, true, R, kBranchFree<R>>(gf, gfp);
+        }
       }
+      const R gm = lane_up1(g);
+      phif = inner ? (upm ? phim + order2 * dotr * gm : phi + order2 * dotr * g) : R(0);
     }
-    const R gm = lane_up1(g);
-    const R phif = inner ? (upm ? phim + order2 * dotr * gm : phi + order2 * dotr * g) : R(0);
     const R phifp = lane_down1(phif);
     const R flux_b = fb * phif;
     const R flux_f = ff * phifp;
@@ -586,7 +719,7 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
       if (mb) st_stream(&A.contrib[(vb - ne) * NC + gc], stg[5 * TS + ra]);
     }
   };
-  if (LEARN) {  // compile-time variable indices: the environment slopes stay in registers
+  if (LEARN || LIM == kLimO3) {  // compile-time variable indices: per-environment values stay in registers
 #pragma unroll
     for (int va = 0; va < 3 * NEM + 1; va += 2)
       if (va < nvu) round(va);

@@ -150,6 +150,69 @@ __host__ __device__ inline T limiter_g(int id, T gp, T gm, T yp, T ym, T h0, T h
   return up ? g : -g;
 }
 
+// Template value of the cell sweeps' LIM for the order-3 face values
+// (ordre-projection 3); the dual sweeps keep Params::lim.
+constexpr int kLimO3 = 4;
+
+// Order-3 face value, ComputeFluxOrdre3 (Remap-ordre-3.cc:69-131): the mean
+// over the swept region of the parabolic reconstruction of the upwind cell,
+// bounded by the TVD interval.  y / h: values and widths of the cells bbb,
+// bb, b | f, ff, fff around the face; vdt: face velocity times dt.  min and
+// max follow Arcane's math::min / math::max.
+template <class T>
+__host__ __device__ inline T amin3(T a, T b) { return (a < b) ? a : b; }
+template <class T>
+__host__ __device__ inline T amax3(T a, T b) { return (a < b) ? b : a; }
+template <class T>
+__host__ __device__ inline T o3_grad(T hm, T h0, T hp, T ym, T y0, T yp) {  // evaluate_grad (7-14)
+  return qdiv(h0, hm + h0 + hp) *
+         (qdiv(T(2) * hm + h0, h0 + hp) * (yp - y0) + qdiv(h0 + T(2) * hp, hm + h0) * (y0 - ym));
+}
+template <class T>
+__host__ __device__ inline T o3_ystar(T hmm, T hm, T hp, T hpp, T ym, T yp, T gradm, T gradp) {  // (17-30)
+  const T tmp1 = qdiv(T(2) * hp * hm, hm + hp) *
+                 (qdiv(hmm + hm, T(2) * hm + hp) - qdiv(hpp + hp, T(2) * hp + hm)) * (yp - ym);
+  const T tmp2 = qdiv(-hm * (hmm + hm), T(2) * hm + hp) * gradp + qdiv(hp * (hp + hpp), hm + T(2) * hp) * gradm;
+  return ym + qdiv(hm, hm + hp) * (yp - ym) + qdiv(T(1), hmm + hm + hp + hpp) * (tmp1 + tmp2);
+}
+template <class T>
+__host__ __device__ inline T o3_flux(const T* y, const T* h, T vdt) {
+  const T gradmm = o3_grad(h[0], h[1], h[2], y[0], y[1], y[2]);
+  const T gradm = o3_grad(h[1], h[2], h[3], y[1], y[2], y[3]);
+  const T gradp = o3_grad(h[2], h[3], h[4], y[2], y[3], y[4]);
+  const T gradpp = o3_grad(h[3], h[4], h[5], y[3], y[4], y[5]);
+  const T ystarm = o3_ystar(h[0], h[1], h[2], h[3], y[1], y[2], gradmm, gradm);
+  const T ystar = o3_ystar(h[1], h[2], h[3], h[4], y[2], y[3], gradm, gradp);
+  const T ystarp = o3_ystar(h[2], h[3], h[4], h[5], y[3], y[4], gradp, gradpp);
+  const T ym = y[2], yp = y[3], hm = h[2], hp = h[3];
+  const T grad_m = ystar - ystarm, grad_p = ystarp - ystar;
+  const T ym6 = T(6) * (ym - T(0.5) * (ystarm + ystar));
+  const T yp6 = T(6) * (yp - T(0.5) * (ystar + ystarp));
+  T flux;
+  if (vdt > T(0)) {  // evaluate_fm (33-38)
+    flux = ystar - qdiv(T(0.5) * vdt, hm) * (grad_m - (T(1) - qdiv(T(2.0 / 3.0) * vdt, hm)) * ym6);
+  } else if (vdt < T(0)) {  // evaluate_fp (41-46), with the reference's sign of the curvature term
+    const T x = -vdt;
+    flux = ystar + qdiv(T(0.5) * x, hp) * (grad_p - (T(1) - qdiv(T(2.0 / 3.0) * x, hp)) * yp6);
+  } else {
+    return T(0);
+  }
+  const T num = qdiv(vdt, hm), nup = qdiv(vdt, hp);
+  const T ym_ym = ym + qdiv(T(1) - num, num) * (ym - y[1]);
+  const T yp_ym = yp - qdiv(T(1) + nup, nup) * (yp - y[4]);
+  const T i1a = amin3(ym, yp), i1b = amax3(ym, yp);
+  const T i2a = vdt >= T(0) ? amin3(ym, ym_ym) : amin3(yp, yp_ym);
+  const T i2b = vdt >= T(0) ? amax3(ym, ym_ym) : amax3(yp, yp_ym);
+  T la = T(0), lb = T(0);
+  if (!((i1b < i2a) || (i2b < i1a))) {
+    la = amax3(i1a, i2a);
+    lb = amin3(i1b, i2b);
+  }
+  if (flux < la) flux = la;
+  if (flux > lb) flux = lb;
+  return flux;
+}
+
 // compute_eint, MahycoModule.cc:1630-1654.
 template <class T>
 __host__ __device__ inline T compute_eint(bool pseudo_centree, T adiabatic_cst, T q_n, T q, T rho_n,

@@ -47,6 +47,7 @@ struct mhg_ctx {
   void *Ud0 = nullptr, *Ud1 = nullptr, *Ud3 = nullptr, *dg0 = nullptr, *dg1 = nullptr;
   void *Ud4 = nullptr, *dek = nullptr;  // total-energy fix-up only
   void* UdB4 = nullptr;                 // its 3D second dual state
+  int* est = nullptr;                   // order 3: the cells' mixed / pure codes
   // 3D: z components, a second dual state (the middle sweep reads one and
   // writes the other), the Euler volumes of the hexahedra
   double* xz[2] = {nullptr, nullptr};
@@ -201,6 +202,7 @@ CellSweepArgsT<R> cell_sweep_args(mhg_ctx* c, int pass, int idir) {
   A.uy = bp<R>(c->uy[c->ucur]);
   A.contrib = bp<R>(c->contrib);
   A.alpha = c->alpha ? bp<R>(c->alpha) + (size_t)idir * c->P.nc : nullptr;
+  A.est = c->est;
   return A;
 }
 
@@ -250,6 +252,7 @@ CellFinalArgsT<R> cell_final_args(mhg_ctx* c) {
   F.env_mf = bp<R>(c->env_mf);
   F.block_min = c->block_min;
   F.dek = bp<R>(c->dek);
+  F.est = c->est;
   return F;
 }
 

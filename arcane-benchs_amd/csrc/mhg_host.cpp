@@ -237,7 +237,7 @@ void remap_3d(mhg_ctx* c, int dt_mode) {
   for (int pass = 0; pass < 3; ++pass) {
     const int idir = (pass + c->sens) % 3;
     const bool first = pass == 0, last = pass == 2;
-    CellSweep3ArgsT<R> A;
+    CellSweep3ArgsT<R> A{};
     A.b.Uin = bp<R>(c->U[pass & 1]);
     A.b.Uout = bp<R>(c->U[(pass + 1) & 1]);
     A.b.Vlag = bp<R>(c->Vlag);
@@ -287,7 +287,7 @@ void remap_3d(mhg_ctx* c, int dt_mode) {
     }
     prevmask |= 1 << idir;
   }
-  CellFinal3ArgsT<R> F;
+  CellFinal3ArgsT<R> F{};
   F.b.U = bp<R>(c->U[1]);
   F.b.Vlag = bp<R>(c->Vlag);
   F.b.L = bp<R>(c->L);
@@ -535,11 +535,14 @@ int build_params(const mhg_mesh_desc* mesh, const mhg_options* opt, Params* Pout
       !opt->is_euler_scheme || opt->projection_pente_borne ||
       (opt->projection_limiteur_id > MHG_LIM_ULTRABEEG && opt->projection_limiteur_id != MHG_LIM_LEARNED &&
        opt->projection_limiteur_id != MHG_LIM_LEARNED_ALL) ||
-      opt->projection_limiteur_id < 0 || opt->ordre_projection < 1 || opt->ordre_projection > 2)
+      opt->projection_limiteur_id < 0 || opt->ordre_projection < 1 || opt->ordre_projection > 3)
     return MHG_E_UNSUPPORTED;
   const bool learned = opt->projection_limiteur_id == MHG_LIM_LEARNED ||
                        opt->projection_limiteur_id == MHG_LIM_LEARNED_ALL;
   if (learned && mesh->dim != 2) return MHG_E_UNSUPPORTED;
+  // order 3 (Remap-ordre-3.cc) runs on 2D meshes with the classical or
+  // generalised limiters
+  if (opt->ordre_projection == 3 && (mesh->dim != 2 || learned)) return MHG_E_UNSUPPORTED;
   if (opt->precision != MHG_PREC_FP64 && opt->precision != MHG_PREC_FP32_BULK) return MHG_E_UNSUPPORTED;
   Params P{};
   P.dim = mesh->dim;
@@ -661,6 +664,7 @@ int mhg_create(const mhg_mesh_desc* mesh, const mhg_options* opt, mhg_ctx** out)
   r |= balloc(c, &c->Ud0, NN) | balloc(c, &c->Ud1, NN) | balloc(c, &c->Ud3, NN);
   r |= balloc(c, &c->dg0, NN) | balloc(c, &c->dg1, NN);
   if (P.etot) r |= balloc(c, &c->Ud4, NN) | balloc(c, &c->dek, NN);
+  if (P.order2 == 2) r |= dalloc(c, &c->est, NC);  // zero: pure in environment 0, as never written
   if (P.etot && P.dim == 3) r |= balloc(c, &c->UdB4, NN);
   if (P.dim == 3) {
     for (int b = 0; b < 2; ++b) r |= dalloc(c, &c->xz[b], NN) | balloc(c, &c->uz[b], NN);
@@ -785,6 +789,25 @@ int mhg_upload_state(mhg_ctx* c, const mhg_state* st) {
   HIPCHK(c, put(c, c->q, st->pseudo_viscosity, NC));
   HIPCHK(c, put(c, c->m, st->cell_mass, NC));
   HIPCHK(c, h2d(c->mask, st->env_mask, NC * sizeof(int)));
+  if (c->est && st->env_mask && st->fracvol && st->fracvol_env) {
+    // order 3: mixed / pure codes from the uploaded fractions (two or more
+    // environments with f > 0: mixed, else the last one with f > 0)
+    std::vector<int> e(NC);
+    for (long cc = 0; cc < NC; ++cc) {
+      const int mk = st->env_mask[cc], nbe = __builtin_popcount((unsigned)mk);
+      int cnt = 0, last = -1;
+      for (int k = 0; k < P.ne; ++k) {
+        if (!((mk >> k) & 1)) continue;
+        const double f = nbe == 1 ? st->fracvol[cc] : st->fracvol_env[(size_t)k * NC + cc];
+        if (f > 0.) {
+          cnt++;
+          last = k;
+        }
+      }
+      e[cc] = cnt > 1 ? -2 : last;
+    }
+    HIPCHK(c, h2d(c->est, e.data(), NC * sizeof(int)));
+  }
   HIPCHK(c, put(c, c->env_rho, st->density_env, en));
   HIPCHK(c, put(c, c->env_e, st->internal_energy_env, en));
   HIPCHK(c, put(c, c->env_p, st->pressure_env, en));
@@ -1222,6 +1245,7 @@ int autotune_t(mhg_ctx* c, int32_t reps, char* report, int32_t report_len) {
   A.uy = bp<R>(c->uy[c->ucur]);
   A.contrib = bp<R>(c->contrib);
   A.alpha = c->alpha ? bp<R>(c->alpha) + P.nc : nullptr;  // the y sweep's coefficients
+  A.est = c->est;
   NodeSweepArgsT<R> B;
   B.m = bp<R>(c->m);
   B.ux = A.ux;

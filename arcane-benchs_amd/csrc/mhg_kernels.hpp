@@ -34,6 +34,7 @@ struct CellSweepArgsT {
   const R *ux, *uy;
   R* contrib;
   const R* alpha;  // learned-limiter coefficients of the sweep direction (learned mode)
+  const int* est;  // order 3: the cells' mixed / pure codes of the last remap
 };
 
 template <class R>
@@ -57,6 +58,7 @@ struct CellFinalArgsT {
   R *env_rho, *env_e, *env_p, *env_cs, *env_q, *env_m, *env_f, *env_mf;
   double* block_min;  // one CFL minimum per block of k_cell_final_2d
   const R* dek;       // per-node terms of the total-energy fix-up (Params::etot)
+  int* est;           // order 3: mixed / pure code of each cell (-2 mixed, else its environment, -1 none)
 };
 
 // 3D argument blocks: the 2D block (x and y components) plus the z

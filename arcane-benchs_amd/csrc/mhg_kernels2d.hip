@@ -743,7 +743,8 @@ static void launch_y(hipStream_t st, const Params& P, DevScalars* S, const CellS
 template <class R, int NEM>
 static void launch_lane_x(hipStream_t st, const Params& P, DevScalars* S, const CellSweepArgsT<R>& A,
                           bool first) {
-  const long waves = (long)((P.nx + 59) / 60) * P.ny;
+  const int no = P.order2 == 2 ? 58 : 60;  // output lanes per wavefront (order 3: three halo lanes a side)
+  const long waves = (long)((P.nx + no - 1) / no) * P.ny;
   dim3 g(nblocks(waves * 64, 256)), b(256);
   if (P.learned) {
     if (first) hipLaunchKernelGGL((k_cell_sweep_lane_2d<R, 0, NEM, true, 0, true>), g, b, 0, st, P, S, A);
@@ -753,7 +754,9 @@ static void launch_lane_x(hipStream_t st, const Params& P, DevScalars* S, const 
 #define XL_(LM)                                                                                    \
   if (first) hipLaunchKernelGGL((k_cell_sweep_lane_2d<R, 0, NEM, true, LM>), g, b, 0, st, P, S, A); \
   else hipLaunchKernelGGL((k_cell_sweep_lane_2d<R, 0, NEM, false, LM>), g, b, 0, st, P, S, A)
-  if (P.lim >= kLimG) {
+  if (P.order2 == 2) {
+    XL_(kLimO3);
+  } else if (P.lim >= kLimG) {
     XL_(kLimG);
   } else {
     MHG_LIM_DISPATCH(P.lim, XL_);
@@ -764,7 +767,8 @@ static void launch_lane_x(hipStream_t st, const Params& P, DevScalars* S, const 
 template <class R, int NEM>
 static void launch_tile_y(hipStream_t st, const Params& P, DevScalars* S, const CellSweepArgsT<R>& A,
                           bool first) {
-  dim3 g((unsigned)(((P.nx + kTileW - 1) / kTileW) * ((P.ny + 59) / 60))), b(1024);
+  const int no = P.order2 == 2 ? 58 : 60;  // output rows per tile (order 3: three halo rows a side)
+  dim3 g((unsigned)(((P.nx + kTileW - 1) / kTileW) * ((P.ny + no - 1) / no))), b(1024);
   if (P.learned) {
     if (first) hipLaunchKernelGGL((k_cell_sweep_ytile_2d<R, NEM, true, 0, true>), g, b, 0, st, P, S, A);
     else hipLaunchKernelGGL((k_cell_sweep_ytile_2d<R, NEM, false, 0, true>), g, b, 0, st, P, S, A);
@@ -773,7 +777,9 @@ static void launch_tile_y(hipStream_t st, const Params& P, DevScalars* S, const 
 #define YT_(LM)                                                                                  \
   if (first) hipLaunchKernelGGL((k_cell_sweep_ytile_2d<R, NEM, true, LM>), g, b, 0, st, P, S, A); \
   else hipLaunchKernelGGL((k_cell_sweep_ytile_2d<R, NEM, false, LM>), g, b, 0, st, P, S, A)
-  if (P.lim >= kLimG) {
+  if (P.order2 == 2) {
+    YT_(kLimO3);
+  } else if (P.lim >= kLimG) {
     YT_(kLimG);
   } else {
     MHG_LIM_DISPATCH(P.lim, YT_);
@@ -792,7 +798,7 @@ void launch_cell_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const 
   }
   if constexpr (std::is_same<R, double>::value) {
     // the column walkers have no learned or generalised-limiter form
-    switch (P.learned || P.lim >= kLimG ? -1 : variant) {
+    switch (P.learned || P.lim >= kLimG || P.order2 == 2 ? -1 : variant) {
       case 0: launch_y<32, 4, 2>(st, P, S, A, first); return;
       case 1: launch_y<8, 4, 2>(st, P, S, A, first); return;
       case 2: launch_y<16, 4, 4>(st, P, S, A, first); return;

@@ -241,7 +241,7 @@ __device__ inline R final_cell(const Params& P, const CellFinalArgsT<R>& A, long
       fv[k] = f;
       somme_frac += f;
     }
-    int matcell = 0;
+    int matcell = 0, imatpure = -1;
     const R unsursomme_frac = qdiv(R(1), somme_frac);
 #pragma unroll
     for (int k = 0; k < NEM; ++k) {
@@ -253,9 +253,14 @@ __device__ inline R final_cell(const Params& P, const CellFinalArgsT<R>& A, long
         f = R(0);
         mask &= ~(1 << k);
       }
-      if (f > R(0)) matcell++;
+      if (f > R(0)) {
+        matcell++;
+        imatpure = k;
+      }
       fv[k] = f;
     }
+    // order 3: the cell's mixed / pure code for the next remap (RemapADIFinal.cc:286-293)
+    if (P.order2 == 2 && A.est) A.est[c] = matcell > 1 ? -2 : imatpure;
     const int nbe = __popc(mask);
 #pragma unroll
     for (int k = 0; k < NEM; ++k) {

@@ -1146,6 +1146,106 @@ void computeGradPhiCell(mho_ctx& S, int idir) {
   }
 }
 
+// Order-3 face value (Remap-ordre-3.cc): Arcane's math::min / math::max
+// ((a < b) ? a : b and (a < b) ? b : a).
+inline double amin(double a, double b) { return (a < b) ? a : b; }
+inline double amax(double a, double b) { return (a < b) ? b : a; }
+// evaluate_grad (7-14)
+inline double o3_grad(double hm, double h0, double hp, double ym, double y0, double yp) {
+  return h0 / (hm + h0 + hp) * ((2. * hm + h0) / (h0 + hp) * (yp - y0) + (h0 + 2. * hp) / (hm + h0) * (y0 - ym));
+}
+// evaluate_ystar (17-30)
+inline double o3_ystar(double hmm, double hm, double hp, double hpp, double ymm, double ym, double yp, double ypp,
+                       double gradm, double gradp) {
+  (void)ymm;
+  (void)ypp;
+  const double tmp1 = (2. * hp * hm) / (hm + hp) *
+                      ((hmm + hm) / (2. * hm + hp) - (hpp + hp) / (2. * hp + hm)) * (yp - ym);
+  const double tmp2 = -hm * (hmm + hm) / (2. * hm + hp) * gradp + hp * (hp + hpp) / (hm + 2. * hp) * gradm;
+  return ym + hm / (hm + hp) * (yp - ym) + 1. / (hmm + hm + hp + hpp) * (tmp1 + tmp2);
+}
+// ComputeFluxOrdre3 (69-131): the mean over the swept region of the
+// parabolic reconstruction of the upwind cell, bounded by the TVD interval.
+inline double o3_flux(double ymmm, double ymm, double ym, double yp, double ypp, double yppp, double hmmm,
+                      double hmm, double hm, double hp, double hpp, double hppp, double vdt) {
+  const double gradmm = o3_grad(hmmm, hmm, hm, ymmm, ymm, ym);
+  const double gradm = o3_grad(hmm, hm, hp, ymm, ym, yp);
+  const double gradp = o3_grad(hm, hp, hpp, ym, yp, ypp);
+  const double gradpp = o3_grad(hp, hpp, hppp, yp, ypp, yppp);
+  const double ystarm = o3_ystar(hmmm, hmm, hm, hp, ymmm, ymm, ym, yp, gradmm, gradm);
+  const double ystar = o3_ystar(hmm, hm, hp, hpp, ymm, ym, yp, ypp, gradm, gradp);
+  const double ystarp = o3_ystar(hm, hp, hpp, hppp, ym, yp, ypp, yppp, gradp, gradpp);
+  const double ym_m = ystarm, ym_p = ystar, yp_m = ystar, yp_p = ystarp;
+  const double grad_m = ym_p - ym_m, grad_p = yp_p - yp_m;
+  const double ym6 = 6. * (ym - 0.5 * (ym_m + ym_p));
+  const double yp6 = 6. * (yp - 0.5 * (yp_m + yp_p));
+  double flux;
+  if (vdt > 0.) {  // evaluate_fm (33-38)
+    flux = ym_p - 0.5 * vdt / hm * (grad_m - (1. - 2. / 3. * vdt / hm) * ym6);
+  } else if (vdt < 0.) {  // evaluate_fp (41-46)
+    flux = yp_m + 0.5 * (-vdt) / hp * (grad_p - (1. - 2. / 3. * (-vdt) / hp) * yp6);
+  } else {
+    return 0.;
+  }
+  const double num = vdt / hm, nup = vdt / hp;
+  const double ym_ym = ym + (1. - num) / num * (ym - ymm);
+  const double yp_ym = yp - (1. + nup) / nup * (yp - ypp);
+  const double i1a = amin(ym, yp), i1b = amax(ym, yp);
+  const double i2a = vdt >= 0. ? amin(ym, ym_ym) : amin(yp, yp_ym);
+  const double i2b = vdt >= 0. ? amax(ym, ym_ym) : amax(yp, yp_ym);
+  double la = 0., lb = 0.;
+  if (!((i1b < i2a) || (i2b < i1a))) {
+    la = amax(i1a, i2a);
+    lb = amin(i1b, i2b);
+  }
+  if (flux < la) flux = la;
+  if (flux > lb) flux = lb;
+  return flux;
+}
+
+// Order-3 face values (RemapADIService.cc:583-640): six cells along idir
+// around each inner face, the last existing cell standing in for the missing
+// ones; at a face whose neighbourhood is not pure, an environment's mass and
+// energy face values follow its volume face value through the upwind cell's
+// ratios.
+void computeUpwindFaceQuantities3(mho_ctx& S, int idir) {
+  const int nv = S.nvar, ne = S.nenv;
+  std::fill(S.phi_face.begin(), S.phi_face.end(), 0.);
+  computeCellWidths(S, idir);
+  const double deltat = S.global_deltat;
+  const long delta = (idir == 0) ? 1 : (idir == 1 ? S.n[0] : (long)S.n[0] * S.n[1]);
+#pragma omp parallel for schedule(static)
+  for (long q = 0; q < S.nfd[idir]; ++q) {
+    long f = S.foff[idir] + q, b, fr;
+    if (!isInnerFaceOfDir(S, f, idir, &b, &fr)) continue;
+    int ib[3], iff[3];
+    S.cellIdx(b, ib);
+    S.cellIdx(fr, iff);
+    const long bb = ib[idir] >= 1 ? b - delta : b;
+    const long bbb = ib[idir] >= 2 ? b - 2 * delta : bb;
+    const long ff = iff[idir] <= S.n[idir] - 2 ? fr + delta : fr;
+    const long fff = iff[idir] <= S.n[idir] - 3 ? fr + 2 * delta : ff;
+    const double vdt = S.face_normal_velocity[f] * deltat;
+    const double* P = S.phi_lagrange.data();
+    const double* h = S.h_cell.data();
+    double* F = &S.phi_face[f * nv];
+    for (int v = 0; v < nv; ++v)
+      F[v] = o3_flux(P[bbb * nv + v], P[bb * nv + v], P[b * nv + v], P[fr * nv + v], P[ff * nv + v],
+                     P[fff * nv + v], h[bbb], h[bb], h[b], h[fr], h[ff], h[fff], vdt);
+    const bool voisinage_pure = S.est_mixte[b] == 0 && S.est_mixte[fr] == 0 && S.est_pure[b] == S.est_pure[fr];
+    if (!voisinage_pure) {
+      for (int k = 0; k < ne; ++k) {
+        long up = -1;
+        if (vdt > 0. && P[b * nv + k] != 0. && P[b * nv + ne + k] != 0.) up = b;
+        else if (vdt < 0. && P[fr * nv + k] != 0. && P[fr * nv + ne + k] != 0.) up = fr;
+        if (up < 0) continue;
+        F[ne + k] = (P[up * nv + ne + k] / P[up * nv + k]) * F[k];
+        F[2 * ne + k] = (P[up * nv + 2 * ne + k] / P[up * nv + ne + k]) * F[ne + k];
+      }
+    }
+  }
+}
+
 // computeUpwindFaceQuantitiesForProjection_PBorn0_O2, RemapADIService.cc:658-720.
 void computeUpwindFaceQuantities(mho_ctx& S, int idir) {
   const int nv = S.nvar;
@@ -1574,7 +1674,8 @@ void appliRemap(mho_ctx& S) {
     int idir = (i + S.sens_projection) % S.dim;
     computeGradPhiFace(S, idir);
     computeGradPhiCell(S, idir);
-    computeUpwindFaceQuantities(S, idir);
+    if (S.opt.ordre_projection == 3) computeUpwindFaceQuantities3(S, idir);
+    else computeUpwindFaceQuantities(S, idir);
     computeUremap(S, idir);
     computeDualUremap(S, idir);
   }
@@ -1813,7 +1914,7 @@ int mho_create(const mhg_mesh_desc* mesh, const mhg_options* opt, mho_ctx** out)
     return MHG_E_ARG;
   if (opt->schema_csts || opt->with_newton || opt->sans_lagrange || !opt->is_euler_scheme ||
       opt->projection_pente_borne || !has_limited_gradients(*opt) ||
-      opt->projection_limiteur_id < 0 || opt->ordre_projection < 1 || opt->ordre_projection > 2)
+      opt->projection_limiteur_id < 0 || opt->ordre_projection < 1 || opt->ordre_projection > 3)
     return MHG_E_UNSUPPORTED;
   for (int b = 0; b < opt->nb_bc; ++b)
     if (opt->bc[b].surface < 0 || opt->bc[b].surface > 5 || opt->bc[b].type < 0 ||
@@ -1983,6 +2084,22 @@ int mho_upload_state(mho_ctx* ctx, const mhg_state* st) {
     } else {
       computeCQs2D(coord, &S.cell_cqs[c * S.nbnode]);
     }
+  }
+  // mixed / pure flags (read by the order-3 face values) from the uploaded
+  // fractions: mixed if two or more environments have f > 0, else the last
+  // environment with f > 0 (-1: none), as remapVariables sets them
+  // (RemapADIFinal.cc:270-293)
+  for (long c = 0; c < NC; ++c) {
+    int ev[MHG_MAX_ENV];
+    const int nbe = S.envsOf(c, ev);
+    int matcell = 0, imatpure = -1;
+    for (int q = 0; q < nbe; ++q)
+      if (S.EV(S.fracvol, ev[q], c) > 0.) {
+        matcell++;
+        imatpure = ev[q];
+      }
+    S.est_mixte[c] = matcell > 1 ? 1 : 0;
+    S.est_pure[c] = matcell > 1 ? -1 : imatpure;
   }
   S.global_deltat = st->global_deltat;
   S.old_deltat = st->old_deltat;
@@ -2158,6 +2275,10 @@ int mho_probe_cell_widths(mho_ctx* c, int32_t idir, double* out) {
   computeCellWidths(*c, idir);
   std::copy(c->h_cell.begin(), c->h_cell.end(), out);
   return MHG_OK;
+}
+
+double mho_probe_flux3(const double* y6, const double* h6, double vdt) {
+  return o3_flux(y6[0], y6[1], y6[2], y6[3], y6[4], y6[5], h6[0], h6[1], h6[2], h6[3], h6[4], h6[5], vdt);
 }
 
 double mho_probe_limiter_g(int32_t id, double gp, double gm, double yp, double ym, double h0, double hp,

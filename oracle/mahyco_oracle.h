@@ -70,6 +70,10 @@ double mho_probe_limiter(int32_t id, double r);
  * neighbour), hm (back neighbour). */
 double mho_probe_limiter_g(int32_t id, double gp, double gm, double yp, double ym, double h0, double hp,
                            double hm);
+/* Order-3 face value (ComputeFluxOrdre3, Remap-ordre-3.cc:69-131): y6 and h6
+ * the values and widths of the cells bbb, bb, b | f, ff, fff around the face,
+ * vdt the face velocity times dt. */
+double mho_probe_flux3(const double* y6, const double* h6, double vdt);
 /* Cell widths along idir (RemapADIService.cc:223-255) of the present
  * coordinates, out [N_c]. */
 int mho_probe_cell_widths(mho_ctx* ctx, int32_t idir, double* out);
