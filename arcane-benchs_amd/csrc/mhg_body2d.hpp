@@ -180,60 +180,6 @@ __device__ __forceinline__ double cell_width_2d(const double* a, const double* t
   return (first ? 0. : db) + (last ? 0. : df);
 }
 
-// Order 3 (RemapADIService.cc:583-640): the six values bbb, bb, b | f, ff,
-// fff around the back face of the cell in this lane (s: its index along the
-// sweep, N: cells along it), from the lanes three behind to two ahead; a
-// cell beyond the mesh is replaced by the last existing one, as the
-// reference's previous() / next() walk does.  Every lane runs the shifts.
-template <class R>
-__device__ __forceinline__ void o3_stencil(R v, int s, int N, R* y) {
-  const R m1 = lane_up1(v), m2 = lane_up1(m1), m3 = lane_up1(m2);
-  const R p1 = lane_down1(v), p2 = lane_down1(p1);
-  y[2] = m1;
-  y[1] = s >= 2 ? m2 : m1;
-  y[0] = s >= 3 ? m3 : y[1];
-  y[3] = v;
-  y[4] = s + 1 <= N - 1 ? p1 : v;
-  y[5] = s + 2 <= N - 1 ? p2 : y[4];
-}
-
-// Order 3 at a face whose neighbourhood is not pure (RemapADIService.cc:
-// 623-638): an environment's mass face value is its volume face value times
-// the upwind cell's mass per volume, and the energy face value the new mass
-// face value times its energy per mass, when the upwind cell (b for vdt >
-// 0, f for vdt < 0) holds the environment's volume and mass.  Roles come in
-// the order volumes, masses, energies; the per-environment registers carry
-// the volume and mass values between them.  pb, pf: the variable's value in
-// cells b and f.
-template <class R, int NEM>
-__device__ __forceinline__ void o3_mixed_faces(bool vpure, int role, int k, R vdt, R pb, R pf, R* phif, R* V,
-                                               R* Vb, R* Vf, R* M, R* Mb, R* Mf, int* up) {
-#pragma unroll
-  for (int q = 0; q < NEM; ++q) {
-    if (q != k) continue;
-    if (role == 0) {
-      V[q] = *phif;
-      Vb[q] = pb;
-      Vf[q] = pf;
-    } else if (role == 1) {
-      int u = 0;
-      if (!vpure) {
-        if (vdt > R(0) && Vb[q] != R(0) && pb != R(0)) u = 1;
-        else if (vdt < R(0) && Vf[q] != R(0) && pf != R(0)) u = 2;
-      }
-      if (u == 1) *phif = qdiv(pb, Vb[q]) * V[q];
-      if (u == 2) *phif = qdiv(pf, Vf[q]) * V[q];
-      up[q] = u;
-      M[q] = *phif;
-      Mb[q] = pb;
-      Mf[q] = pf;
-    } else if (role == 2) {
-      if (up[q] == 1) *phif = qdiv(pb, Mb[q]) * M[q];
-      if (up[q] == 2) *phif = qdiv(pf, Mf[q]) * M[q];
-    }
-  }
-}
-
 template <class R, int D, int NEM, bool FIRST, int LIM, bool LEARN = false>
 __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScalars* S, const CellSweepArgsT<R>& A,
                                                         int blk) {

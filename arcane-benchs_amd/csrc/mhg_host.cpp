@@ -239,6 +239,7 @@ void remap_3d(mhg_ctx* c, int dt_mode) {
     const bool first = pass == 0, last = pass == 2;
     CellSweep3ArgsT<R> A{};
     A.b.Uin = bp<R>(c->U[pass & 1]);
+    A.b.est = c->est;
     A.b.Uout = bp<R>(c->U[(pass + 1) & 1]);
     A.b.Vlag = bp<R>(c->Vlag);
     A.b.xnx = xn[0];
@@ -289,6 +290,7 @@ void remap_3d(mhg_ctx* c, int dt_mode) {
   }
   CellFinal3ArgsT<R> F{};
   F.b.U = bp<R>(c->U[1]);
+  F.b.est = c->est;
   F.b.Vlag = bp<R>(c->Vlag);
   F.b.L = bp<R>(c->L);
   F.b.ux = bp<R>(c->ux[1 - c->ucur]);
@@ -540,9 +542,9 @@ int build_params(const mhg_mesh_desc* mesh, const mhg_options* opt, Params* Pout
   const bool learned = opt->projection_limiteur_id == MHG_LIM_LEARNED ||
                        opt->projection_limiteur_id == MHG_LIM_LEARNED_ALL;
   if (learned && mesh->dim != 2) return MHG_E_UNSUPPORTED;
-  // order 3 (Remap-ordre-3.cc) runs on 2D meshes with the classical or
-  // generalised limiters
-  if (opt->ordre_projection == 3 && (mesh->dim != 2 || learned)) return MHG_E_UNSUPPORTED;
+  // order 3 (Remap-ordre-3.cc) runs with the classical or generalised
+  // limiters
+  if (opt->ordre_projection == 3 && learned) return MHG_E_UNSUPPORTED;
   if (opt->precision != MHG_PREC_FP64 && opt->precision != MHG_PREC_FP32_BULK) return MHG_E_UNSUPPORTED;
   Params P{};
   P.dim = mesh->dim;

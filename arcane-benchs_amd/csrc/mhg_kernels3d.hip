@@ -250,15 +250,16 @@ __global__ __launch_bounds__(256) void k_cell_sweep_x_3d(Params P, DevScalars* S
   if (S->stop || S->err) return;
   const int lane = threadIdx.x & 63;
   const int N = P.nx;
-  const int nseg = (N + 59) / 60;
+  constexpr int HL = (LIM == kLimO3) ? 3 : 2, NO = 64 - 2 * HL;  // halo lanes (order 3: 3), outputs
+  const int nseg = (N + NO - 1) / NO;
   const long wave = ((long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
   const long line = wave / nseg;
   if (line >= (long)P.ny * P.nz) return;  // uniform across the wave
   const int seg = (int)(wave % nseg);
   const int j = (int)(line % P.ny), k = (int)(line / P.ny);
-  const int i = seg * 60 - 2 + lane;
+  const int i = seg * NO - HL + lane;
   const bool ex = (i >= 0 && i < N);
-  const bool out = ex && lane >= 2 && lane <= 61;
+  const bool out = ex && lane >= HL && lane <= 63 - HL;
   const long NC = P.nc;
   const int ne = env_count<NEM>(P);
   const double dt = S->dt;
@@ -301,32 +302,70 @@ __global__ __launch_bounds__(256) void k_cell_sweep_x_3d(Params P, DevScalars* S
   const R fb = (R)(-1.0 * uf * lf * dt), ff = (R)(1.0 * uf_f * lf_f * dt);
   const R dxr = (R)dx, dotr = (R)dotxf;
   const bool o2 = P.order2 > 0;
+  const int sd = i;  // index along the sweep
+  int est_own = 0;   // order 3: the cell's code (first sweep: the last remap's; then phi_V > 0)
+  if (LIM == kLimO3) {
+    if (FIRST) {
+      const int el = A.b.est[c];
+      est_own = ex ? el : 0;
+    } else {
+      int cnt = 0, last = -1;
+#pragma unroll
+      for (int q = 0; q < NEM; ++q)
+        if (q < ne && qdiv(pre[0][q], den) > R(0)) {
+          cnt++;
+          last = q;
+        }
+      est_own = cnt > 1 ? -2 : last;
+    }
+  }
   // generalised limiters: widths of the cell and of its back / front
   // neighbours along D (the cell itself where the neighbour is missing)
   bool front_in = false;
   R hc = R(0), hcm = R(0), hcp = R(0);
-  if (LIM == kLimG) {
+  R h6[6] = {R(0), R(0), R(0), R(0), R(0), R(0)};  // order 3: widths bbb .. fff of the back face
+  if (LIM == kLimG || LIM == kLimO3) {
     front_in = lane_down1((int)inner) != 0;
     const double h = ex ? cell_width_3d<R, 0>(P, A, i, j, k, i, N) : 0.;
     const double hm = lane_up1(h), hp = lane_down1(h);
     hc = (R)h;
     hcm = (R)(inner ? hm : h);
     hcp = (R)(front_in ? hp : h);
+    if (LIM == kLimO3) o3_stencil<R>((R)h, sd, N, h6);
   }
-  auto sweep_var = [&](R u0, R& un, R& dual) {
+  // order 3: mixed / pure codes of the cell and its back neighbour
+  int est = 0, estb = 0;
+  R vdt = R(0);
+  if (LIM == kLimO3) {
+    est = est_own;
+    estb = lane_up1(est);
+    vdt = (R)((double)uf * dt);
+  }
+  const bool vpure = estb != -2 && est != -2 && estb == est;
+  R o3V[NEM], o3Vb[NEM], o3Vf[NEM], o3M[NEM], o3Mb[NEM], o3Mf[NEM];
+  int o3up[NEM];
+  auto sweep_var = [&](R u0, R& un, R& dual, int role, int kq) {
     const R phi = ex ? qdiv(u0, den) : R(0);
     const R phim = lane_up1(phi);
     const R gf = face_grad(inner, phi, phim, dxr);
     const R gfp = lane_down1(gf);
-    R g = R(0);
-    if (LIM == kLimG && o2) {  // computeAndLimitGradPhi, UtilesRemap.cc:329-338
-      const R phip = lane_down1(phi);
-      g = limiter_g<R>(P.lim, gfp, gf, front_in ? phip : phi, inner ? phim : phi, hc, hcp, hcm);
-    } else if (o2) {
-      g = limited_grad<LIM, true, R, kBranchFree<R>>(gf, gfp);
+    R phif;
+    if (LIM == kLimO3) {  // RemapADIService.cc:583-640
+      R y6[6];
+      o3_stencil<R>(phi, sd, N, y6);
+      phif = inner ? o3_flux<R>(y6, h6, vdt) : R(0);
+      o3_mixed_faces<R, NEM>(vpure, role, kq, vdt, phim, phi, &phif, o3V, o3Vb, o3Vf, o3M, o3Mb, o3Mf, o3up);
+    } else {
+      R g = R(0);
+      if (LIM == kLimG && o2) {  // computeAndLimitGradPhi, UtilesRemap.cc:329-338
+        const R phip = lane_down1(phi);
+        g = limiter_g<R>(P.lim, gfp, gf, front_in ? phip : phi, inner ? phim : phi, hc, hcp, hcm);
+      } else if (o2) {
+        g = limited_grad<LIM, true, R, kBranchFree<R>>(gf, gfp);
+      }
+      const R gm = lane_up1(g);
+      phif = inner ? (upm ? phim + order2 * dotr * gm : phi + order2 * dotr * g) : R(0);
     }
-    const R gm = lane_up1(g);
-    const R phif = inner ? (upm ? phim + order2 * dotr * gm : phi + order2 * dotr * g) : R(0);
     const R phifp = lane_down1(phif);
     const R flux_b = fb * phif, flux_f = ff * phifp;
     R fsum = R(0);  // faces in hexa8 order: back (x-) before front (x+)
@@ -345,7 +384,7 @@ __global__ __launch_bounds__(256) void k_cell_sweep_x_3d(Params P, DevScalars* S
     for (int q = 0; q < NEM; ++q) {
       if (q >= ne) break;
       R un, du;
-      sweep_var(pre[g][q], un, du);
+      sweep_var(pre[g][q], un, du, g, q);
       if (g > 0 && un < R(0)) un = R(0);  // RemapADIService.cc:953-958
       if (out) {
         st_stream(&Uout[(g * ne + q) * NC + c], un);
@@ -354,7 +393,7 @@ __global__ __launch_bounds__(256) void k_cell_sweep_x_3d(Params P, DevScalars* S
     }
   {
     R un, du;
-    sweep_var(preq, un, du);
+    sweep_var(preq, un, du, 3, 0);
     if (out) st_stream(&Uout[(3 * ne) * NC + c], un);
   }
 }
@@ -364,7 +403,7 @@ __global__ __launch_bounds__(256) void k_cell_sweep_x_3d(Params P, DevScalars* S
 // tile is transposed through LDS so that a wavefront holds one column.
 template <class R, int D, int NEM, bool FIRST, int LIM>
 __global__ __launch_bounds__(1024) void k_cell_sweep_tile_3d(Params P, DevScalars* S, CellSweep3ArgsT<R> A) {
-  __shared__ double lds[6][kTileR][kTileP];
+  __shared__ double lds[LIM == kLimO3 ? 7 : 6][kTileR][kTileP];  // order 3: a tile for the cells' codes
   typedef R Tile[kTileR][kTileP];
   Tile* stg = reinterpret_cast<Tile*>(&lds[0][0][0]);
   if (S->stop || S->err) return;  // uniform over the grid
@@ -372,10 +411,11 @@ __global__ __launch_bounds__(1024) void k_cell_sweep_tile_3d(Params P, DevScalar
   const int N = (D == 1) ? P.ny : P.nz;      // cells along D
   const int T = (D == 1) ? P.nz : P.ny;      // the third direction
   const int ncolblk = (nx + kTileW - 1) / kTileW;
-  const int nseg = (N + 59) / 60;
+  constexpr int HL = (LIM == kLimO3) ? 3 : 2, NO = 64 - 2 * HL;  // halo rows (order 3: 3), outputs
+  const int nseg = (N + NO - 1) / NO;
   const int L = xcd_block(blockIdx.x, gridDim.x);
   const int i0 = (L % ncolblk) * kTileW;
-  const int s0 = ((L / ncolblk) % nseg) * 60 - 2;
+  const int s0 = ((L / ncolblk) % nseg) * NO - HL;
   const int o = L / (ncolblk * nseg);        // index along the third direction
   if (o >= T) return;                        // uniform over the block
   const int t = threadIdx.x;
@@ -394,7 +434,7 @@ __global__ __launch_bounds__(1024) void k_cell_sweep_tile_3d(Params P, DevScalar
   };
   const int gi = i0 + tc, gs = s0 + tr;
   const bool gex = gi < nx && gs >= 0 && gs < N;
-  const bool gout = gex && tr >= 2 && tr <= 61;
+  const bool gout = gex && tr >= HL && tr <= 63 - HL;
   int ci, cj, ck;
   cell_ijk(gi, gs, &ci, &cj, &ck);
   const long gc = gex ? cid3(P, ci, cj, ck) : 0;
@@ -418,7 +458,24 @@ __global__ __launch_bounds__(1024) void k_cell_sweep_tile_3d(Params P, DevScalar
     lds[2][tr][tc] = (double)uf;
     lds[3][tr][tc] = xf;
     lds[4][tr][tc] = lf;
-    if (LIM == kLimG) lds[5][tr][tc] = gex ? cell_width_3d<R, D>(P, A, ci, cj, ck, gs, N) : 0.;
+    if (LIM == kLimG || LIM == kLimO3) lds[5][tr][tc] = gex ? cell_width_3d<R, D>(P, A, ci, cj, ck, gs, N) : 0.;
+    if (LIM == kLimO3) {  // the cell's code: the last remap's in the first sweep, then phi_V > 0
+      int e = 0;
+      if (FIRST) {
+        const int el = A.b.est[gc];
+        e = gex ? el : 0;
+      } else {
+        int cnt = 0, last = -1;
+#pragma unroll
+        for (int q = 0; q < NEM; ++q)
+          if (q < ne && qdiv(Uin[q * NC + gc], den) > R(0)) {
+            cnt++;
+            last = q;
+          }
+        e = cnt > 1 ? -2 : last;
+      }
+      lds[LIM == kLimO3 ? 6 : 5][tr][tc] = (double)e;
+    }
   }
   __syncthreads();
   const int s = s0 + lane;
@@ -434,32 +491,55 @@ __global__ __launch_bounds__(1024) void k_cell_sweep_tile_3d(Params P, DevScalar
   const double lf_f = lane_down1(lf);
   const R fb = (R)(-1.0 * uf * lf * dt), ff = (R)(1.0 * uf_f * lf_f * dt);
   const R dxr = (R)dx, dotr = (R)dotxf;
+  const int sd = s;  // index along the sweep
+  const int est_own = (LIM == kLimO3) ? (int)lds[LIM == kLimO3 ? 6 : 5][lane][w] : 0;
   // generalised limiters: widths of the cell and of its back / front
   // neighbours along D (the cell itself where the neighbour is missing)
   bool front_in = false;
   R hc = R(0), hcm = R(0), hcp = R(0);
-  if (LIM == kLimG) {
+  R h6[6] = {R(0), R(0), R(0), R(0), R(0), R(0)};  // order 3: widths bbb .. fff of the back face
+  if (LIM == kLimG || LIM == kLimO3) {
     front_in = lane_down1((int)inner) != 0;
     const double h = lds[5][lane][w];
     const double hm = lane_up1(h), hp = lane_down1(h);
     hc = (R)h;
     hcm = (R)(inner ? hm : h);
     hcp = (R)(front_in ? hp : h);
+    if (LIM == kLimO3) o3_stencil<R>((R)h, sd, N, h6);
   }
-  auto sweep_var = [&](R u0, R& un, R& dual) {
+  // order 3: mixed / pure codes of the cell and its back neighbour
+  int est = 0, estb = 0;
+  R vdt = R(0);
+  if (LIM == kLimO3) {
+    est = est_own;
+    estb = lane_up1(est);
+    vdt = (R)((double)uf * dt);
+  }
+  const bool vpure = estb != -2 && est != -2 && estb == est;
+  R o3V[NEM], o3Vb[NEM], o3Vf[NEM], o3M[NEM], o3Mb[NEM], o3Mf[NEM];
+  int o3up[NEM];
+  auto sweep_var = [&](R u0, R& un, R& dual, int role, int kq) {
     const R phi = ex ? qdiv(u0, den) : R(0);
     const R phim = lane_up1(phi);
     const R gf = face_grad(inner, phi, phim, dxr);
     const R gfp = lane_down1(gf);
-    R g = R(0);
-    if (LIM == kLimG && o2) {  // computeAndLimitGradPhi, UtilesRemap.cc:329-338
-      const R phip = lane_down1(phi);
-      g = limiter_g<R>(P.lim, gfp, gf, front_in ? phip : phi, inner ? phim : phi, hc, hcp, hcm);
-    } else if (o2) {
-      g = limited_grad<LIM, true, R, kBranchFree<R>>(gf, gfp);
+    R phif;
+    if (LIM == kLimO3) {  // RemapADIService.cc:583-640
+      R y6[6];
+      o3_stencil<R>(phi, sd, N, y6);
+      phif = inner ? o3_flux<R>(y6, h6, vdt) : R(0);
+      o3_mixed_faces<R, NEM>(vpure, role, kq, vdt, phim, phi, &phif, o3V, o3Vb, o3Vf, o3M, o3Mb, o3Mf, o3up);
+    } else {
+      R g = R(0);
+      if (LIM == kLimG && o2) {  // computeAndLimitGradPhi, UtilesRemap.cc:329-338
+        const R phip = lane_down1(phi);
+        g = limiter_g<R>(P.lim, gfp, gf, front_in ? phip : phi, inner ? phim : phi, hc, hcp, hcm);
+      } else if (o2) {
+        g = limited_grad<LIM, true, R, kBranchFree<R>>(gf, gfp);
+      }
+      const R gm = lane_up1(g);
+      phif = inner ? (upm ? phim + order2 * dotr * gm : phi + order2 * dotr * g) : R(0);
     }
-    const R gm = lane_up1(g);
-    const R phif = inner ? (upm ? phim + order2 * dotr * gm : phi + order2 * dotr * g) : R(0);
     const R phifp = lane_down1(phif);
     const R flux_b = fb * phif, flux_f = ff * phifp;
     R fsum = R(0);  // faces in hexa8 order: back before front
@@ -475,7 +555,9 @@ __global__ __launch_bounds__(1024) void k_cell_sweep_tile_3d(Params P, DevScalar
   const int nvu = P.nvu;
   R nxa = gex ? Uin[gc] : R(0);
   R nxb = (gex && nvu > 1) ? Uin[NC + gc] : R(0);
-  for (int va = 0; va < nvu; va += 2) {
+  auto role_of = [&](int v) { return v < ne ? 0 : v < 2 * ne ? 1 : v < 3 * ne ? 2 : 3; };
+  auto env_of = [&](int v) { return v < ne ? v : v < 2 * ne ? v - ne : v < 3 * ne ? v - 2 * ne : 0; };
+  auto round = [&](const int va) {
     const int vb = va + 1;
     const bool hb = vb < nvu;  // uniform
     stg[0][tr][tc] = nxa;
@@ -485,8 +567,8 @@ __global__ __launch_bounds__(1024) void k_cell_sweep_tile_3d(Params P, DevScalar
     if (va + 3 < nvu) nxb = gex ? Uin[(va + 3) * NC + gc] : R(0);
     const R ua = stg[0][lane][w], ub = stg[1][lane][w];
     R una, dua, unb, dub;
-    sweep_var(ua, una, dua);
-    sweep_var(ub, unb, dub);
+    sweep_var(ua, una, dua, role_of(va), env_of(va));
+    sweep_var(ub, unb, dub, role_of(vb), env_of(vb));
     if (va >= ne && va < 3 * ne && una < R(0)) una = R(0);  // RemapADIService.cc:953-958
     if (vb >= ne && vb < 3 * ne && unb < R(0)) unb = R(0);
     const bool ma = (va >= ne && va < 2 * ne), mb = hb && (vb >= ne && vb < 2 * ne);
@@ -501,6 +583,13 @@ __global__ __launch_bounds__(1024) void k_cell_sweep_tile_3d(Params P, DevScalar
       if (ma) st_stream(&A.b.contrib[(va - ne) * NC + gc], stg[4][tr][tc]);
       if (mb) st_stream(&A.b.contrib[(vb - ne) * NC + gc], stg[5][tr][tc]);
     }
+  };
+  if (LIM == kLimO3) {  // compile-time variable indices: per-environment values stay in registers
+#pragma unroll
+    for (int va = 0; va < 3 * NEM + 1; va += 2)
+      if (va < nvu) round(va);
+  } else {
+    for (int va = 0; va < nvu; va += 2) round(va);
   }
 }
 
@@ -971,7 +1060,8 @@ void launch_cell_lagrange_3d(hipStream_t st, const Params& P, DevScalars* S, con
 template <class R, int D, int NEM, int LM>
 static void launch_sweep3(hipStream_t st, const Params& P, DevScalars* S, const CellSweep3ArgsT<R>& A, bool first) {
   if (D == 0) {
-    const long waves = (long)((P.nx + 59) / 60) * P.ny * P.nz;
+    const int no = LM == kLimO3 ? 58 : 60;  // output lanes per wavefront
+    const long waves = (long)((P.nx + no - 1) / no) * P.ny * P.nz;
     dim3 g(nblocks3(waves * 64, 256)), b(256);
     if (first) hipLaunchKernelGGL((k_cell_sweep_x_3d<R, NEM, true, LM>), g, b, 0, st, P, S, A);
     else hipLaunchKernelGGL((k_cell_sweep_x_3d<R, NEM, false, LM>), g, b, 0, st, P, S, A);
@@ -979,7 +1069,8 @@ static void launch_sweep3(hipStream_t st, const Params& P, DevScalars* S, const 
   }
   constexpr int DD = D == 0 ? 1 : D;
   const int N = (DD == 1) ? P.ny : P.nz, T = (DD == 1) ? P.nz : P.ny;
-  dim3 g((unsigned)(((P.nx + kTileW - 1) / kTileW) * ((N + 59) / 60) * T)), b(1024);
+  const int no = LM == kLimO3 ? 58 : 60;  // output rows per tile
+  dim3 g((unsigned)(((P.nx + kTileW - 1) / kTileW) * ((N + no - 1) / no) * T)), b(1024);
   if (first) hipLaunchKernelGGL((k_cell_sweep_tile_3d<R, DD, NEM, true, LM>), g, b, 0, st, P, S, A);
   else hipLaunchKernelGGL((k_cell_sweep_tile_3d<R, DD, NEM, false, LM>), g, b, 0, st, P, S, A);
 }
@@ -1001,7 +1092,11 @@ void launch_cell_sweep_3d(hipStream_t st, const Params& P, DevScalars* S, const 
     else if (P.ne <= 4) SW_(4, LM);  \
     else SW_(8, LM);                 \
   } while (0)
-  MHG3_LIM(P.lim, SWL_);
+  if (P.order2 == 2) {
+    SWL_(kLimO3);
+  } else {
+    MHG3_LIM(P.lim, SWL_);
+  }
 #undef SWL_
 #undef SW_
 }

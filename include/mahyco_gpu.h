@@ -124,7 +124,7 @@ typedef struct {
   /* RemapADI.axl */
   double remap_threshold;             /* default 1e-10 */
   int32_t is_euler_scheme;            /* must be 1     */
-  int32_t ordre_projection;           /* 1, 2 or 3 (3: 2D, not learned, not in batches) */
+  int32_t ordre_projection;           /* 1, 2 or 3 (3: not learned, not in batches) */
   int32_t projection_limiteur_id;     /* MHG_LIM_MINMOD .. MHG_LIM_ULTRABEEG, or MHG_LIM_LEARNED[_ALL] */
   int32_t projection_pente_borne;     /* must be 0     */
   int32_t conservation_energie_totale;/* total-energy fix-up of the remap

@@ -7,8 +7,9 @@ the mass and energy rescaling of non-pure neighbourhoods), with the minmod
 and a generalised limiter for the dual sweeps; a 40-step triple-point
 trajectory from the initialisation (the mixed / pure codes then come from
 the zero-initialised flags and the final cell phase, not from an upload)
-within 1e-12; FP32-bulk within 1e-5 (L1) of FP64.  3D contexts, the learned
-mode and batches refuse order 3."""
+within 1e-12; 3D one steps on Sod along each axis and the three-material box
+and a 15-step box trajectory within 1e-12; FP32-bulk within 1e-5 (L1) of
+FP64.  The learned mode and batches refuse order 3."""
 import numpy as np
 import pytest
 
@@ -80,11 +81,47 @@ def test_fp32_close_to_fp64(gpu):
     assert l1 < 1e-5, l1
 
 
+@pytest.mark.parametrize("case", ["sod0", "sod1", "sod2", "box"])
+def test_3d_one_step_matches_oracle(gpu, case):
+    """3D: the x lane and y / z tile sweeps with three halo cells a side,
+    hexahedral widths, and the codes through the three sweeps."""
+    from test_gpu_3d import box3, compare3
+    if case.startswith("sod"):
+        axis = int(case[3])
+        setup = lambda L: H.sod3d_setup(L, 24, 3, axis, **O3)  # noqa: E731
+        nenv, n, warm = 1, [3, 3, 3], 6
+        n[axis] = 24
+    else:
+        setup = lambda L: box3(L, **O3)  # noqa: E731
+        nenv, n, warm = 3, [14, 10, 8], 8
+    o = setup(H.oracle())
+    o.run(warm)
+    st = o.download()
+    g = setup(gpu)
+    g.upload(st)
+    o.upload(st)
+    o.step()
+    g.step()
+    d = compare3(g.download(), o.download(), nenv, n)
+    print(case, {k: f"{v:.1e}" for k, v in d.items()})
+    bad = {k: v for k, v in d.items() if v > TOL}
+    assert not bad, bad
+    g.close()
+    o.close()
+
+
+def test_3d_box_trajectory(gpu):
+    from test_gpu_3d import box3, compare3
+    o = box3(H.oracle(), **O3)
+    g = box3(gpu, **O3)
+    o.run(15)
+    g.run(15)
+    d = compare3(g.download(), o.download(), 3, [14, 10, 8], metric=H.rel_max)
+    print("box trajectory", {k: f"{v:.1e}" for k, v in d.items()})
+    assert max(d.values()) < 1e-12, d
+
+
 def test_refused(gpu):
-    opt = gpu.options(**O3)
-    abi.set_bcs(opt, abi.reflecting_walls(3))
-    with pytest.raises(abi.MahycoError):
-        abi.Simulation(gpu, abi.mesh_desc(3, [8, 8, 8], [0, 0, 0], [1.0, 1.0, 1.0]), opt)
     with pytest.raises(abi.MahycoError):
         H.tp_setup(gpu, 56, 24, projection_limiteur_id=abi.LIM_LEARNED, **O3)
     specs = cases.riemann_batch_specs(2, seed=3)
