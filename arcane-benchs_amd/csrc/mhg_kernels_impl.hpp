@@ -364,13 +364,25 @@ __device__ inline R final_cell(const Params& P, const CellFinalArgsT<R>& A, long
           if ((mask >> k) & 1) ek[k] += mf[k] * delta_ec;
         e_g += delta_ec;
       }
-      // EOS per environment, then computePressionMoyenne
+      // EOS per environment, then computePressionMoyenne; each environment's
+      // values are stored as soon as they are final (shorter live ranges)
 #pragma unroll
       for (int k = 0; k < NEM; ++k) {
         if (!((mask >> k) & 1)) continue;
         eos_apply(P.eos, k, dk[k], ek[k], &pk[k], &ck[k]);
         p_g += fv[k] * pk[k];
         c_g = smax(ck[k], c_g);
+        if (nbe >= 2) {
+          const long e = k * NC + c;
+          A.env_rho[e] = dk[k];
+          A.env_e[e] = ek[k];
+          A.env_p[e] = pk[k];
+          A.env_cs[e] = ck[k];
+          A.env_q[e] = pv;
+          A.env_m[e] = mk[k];
+          A.env_f[e] = fv[k];
+          A.env_mf[e] = mf[k];
+        }
       }
     }
     st_stream(&A.rho[c], rho_g);
@@ -380,21 +392,6 @@ __device__ inline R final_cell(const Params& P, const CellFinalArgsT<R>& A, long
     st_stream(&A.q[c], q_g);
     st_stream(&A.m[c], mass_g);
     st_stream(&A.mask[c], mask);
-    if (nbe >= 2) {
-#pragma unroll
-      for (int k = 0; k < NEM; ++k) {
-        if (!((mask >> k) & 1)) continue;
-        const long e = k * NC + c;
-        A.env_rho[e] = dk[k];
-        A.env_e[e] = ek[k];
-        A.env_p[e] = pk[k];
-        A.env_cs[e] = ck[k];
-        A.env_q[e] = pv;
-        A.env_m[e] = mk[k];
-        A.env_f[e] = fv[k];
-        A.env_mf[e] = mf[k];
-      }
-    }
     return c_g;
   }
 }
