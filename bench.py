@@ -7,7 +7,10 @@ re-initialised and exactly K steps are timed (K=500 is the configuration's
 run), bracketed by device synchronisations.  Per-kernel durations come from
 HIP events recorded on the library's stream around every launch of the timed
 region.  The CPU baseline is the C++ oracle (a port of the reference
-arithmetic) timed on this host on a bounded sample of the same workload.
+arithmetic) timed on this host on a bounded sample of the same workload, on
+all host threads and on one.  The `sod` field is the 400x20 Sod shock tube
+run to t = 0.2 on the GPU, with its L1 density error against the exact
+Riemann solution; the oracle times the same run inside the CPU baseline.
 """
 import argparse
 import json
@@ -40,8 +43,10 @@ def set_timing(sim, on):
     sim.L.optional["kernel_times"](sim.ctx, 1 if on else 0, None, None, None, 0)
 
 
-def cpu_baseline(gpu_density_after, steps):
-    """Oracle on the same triple point for `steps` steps, all host threads."""
+def cpu_baseline(gpu_density_after, steps, sod_gpu=None):
+    """Oracle on the same triple point for `steps` steps, all host threads,
+    then one thread on a fifth of those steps; and, if sod_gpu (the GPU's
+    Sod run) is given, the whole Sod case on all threads and on one."""
     from importlib import util
     spec = util.spec_from_file_location("oracle_lib", os.path.join(ROOT, "tests", "helpers.py"))
     H = util.module_from_spec(spec)
@@ -54,11 +59,67 @@ def cpu_baseline(gpu_density_after, steps):
     o.run(steps)
     t1 = time.perf_counter()
     rho = o.download(["density"])["density"]
+    o.close()
     l1 = float(np.sum(np.abs(gpu_density_after - rho)) / np.sum(np.abs(rho)))
-    return {"value": NX * NY * steps / (t1 - t0), "unit": "zone-cycles/s", "cores": threads,
-            "kind": "port",
-            "sample": f"C++ oracle (OpenMP, {threads} threads) on the same 1792x768 triple point, "
-                      f"steps 1..{steps}, {t1 - t0:.1f} s"}, l1
+    out = {"value": NX * NY * steps / (t1 - t0), "unit": "zone-cycles/s", "cores": threads,
+           "kind": "port",
+           "sample": f"C++ oracle (OpenMP, {threads} threads) on the same 1792x768 triple point, "
+                     f"steps 1..{steps}, {t1 - t0:.1f} s"}
+    L.optional["set_threads"](1)
+    s1 = max(1, steps // 5)
+    o = cases.triple_point(L, NX, NY)
+    t0 = time.perf_counter()
+    o.run(s1)
+    t1 = time.perf_counter()
+    o.close()
+    out["value_1thread"] = NX * NY * s1 / (t1 - t0)
+    out["sample_1thread"] = f"the same, 1 thread, steps 1..{s1}, {t1 - t0:.1f} s"
+    if sod_gpu is not None:
+        sod = {}
+        for th in (threads, 1):
+            L.optional["set_threads"](th)
+            o = cases.sod(L, SOD_NX, SOD_NY, **SOD_OPT)
+            t0 = time.perf_counter()
+            n = o.run(1 << 30)
+            t1 = time.perf_counter()
+            rho_o = o.download(["density"])["density"]
+            o.close()
+            sod[f"zone_cycles_per_s_{th}_threads"] = SOD_NX * SOD_NY * n / (t1 - t0)
+        sod["steps"] = n
+        sod["l1_density_gpu_vs_oracle"] = float(np.sum(np.abs(sod_gpu - rho_o)) / np.sum(np.abs(rho_o)))
+        out["sod"] = sod
+    L.optional["set_threads"](threads)
+    return out, l1
+
+
+# Sod shock tube (SODService.cc:11-39, cas-test 3): 400x20 cells on [0,1]x[0,0.05],
+# deltat-init 1e-5, deltat-min 1e-8, deltat-max 0.01, final time 0.2 (Data.1.arc)
+SOD_NX, SOD_NY = 400, 20
+SOD_OPT = dict(deltat_init=1e-5, deltat_min=1e-8, deltat_max=0.01, final_time=0.2)
+
+
+def sod_run(gpu, sync):
+    """The Sod case to t = 0.2 on the GPU (FP64): zone-cycles/s of the whole
+    run and its L1 density error against the exact Riemann solution."""
+    from mahyco_amd import riemann
+    s = cases.sod(gpu, SOD_NX, SOD_NY, **SOD_OPT)
+    s.run(3)  # first launches and graph capture out of the timed region
+    s.close()
+    s = cases.sod(gpu, SOD_NX, SOD_NY, **SOD_OPT)
+    sync()
+    t0 = time.perf_counter()
+    n = s.run(1 << 30)
+    sync()
+    t1 = time.perf_counter()
+    st = s.download()
+    s.close()
+    rho = np.asarray(st["density"])
+    xc = (np.arange(SOD_NX) + 0.5) / SOD_NX
+    rex, _, _ = riemann.sample(xc, st["global_time"], 0.5, (1, 0, 1), (0.125, 0, 0.1), 1.4)
+    l1 = float(np.mean(np.abs(rho.reshape(SOD_NY, SOD_NX) - rex[None, :])))
+    return {"config": "Sod 400x20 on [0,1]x[0,0.05], gamma 1.4, order 2 minmod, to t = 0.2 (cas-test 3)",
+            "steps": n, "time": st["global_time"], "value": SOD_NX * SOD_NY * n / (t1 - t0),
+            "unit": "zone-cycles/s", "l1_density_vs_exact_riemann": l1}, rho
 
 
 def timed_run(gpu, precision, args, sync):
@@ -222,6 +283,7 @@ def main():
     ap.add_argument("--batch-n", type=int, default=1024, help="problems of the Riemann batch")
     ap.add_argument("--no-batch", action="store_true", help="skip the batching form")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned form")
+    ap.add_argument("--no-sod", action="store_true", help="skip the Sod case")
     ap.add_argument("--learned-t", type=float, default=0.5, help="final time of the learned-form evaluation")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device with torch: the reference kernel "
@@ -292,13 +354,14 @@ def main():
         "source": "FETCH_SIZE + WRITE_SIZE per step kernel, copy-calibrated, from the committed "
                   "triple-point traffic profile, over this run's ms_per_step"}
 
+    ps, sod_rho = (None, None) if args.no_sod else sod_run(gpu, sync)
     cpu, l1 = None, None
     if not args.no_cpu_baseline:
         g2 = cases.triple_point(gpu, NX, NY)
         g2.run(args.cpu_steps)
         rho_g = g2.download(["density"])["density"]
         g2.close()
-        cpu, l1 = cpu_baseline(rho_g, args.cpu_steps)
+        cpu, l1 = cpu_baseline(rho_g, args.cpu_steps, sod_rho)
 
     out = {
         "metric": "zone-cycles/s on 1792x768 triple-point; L1 density err vs FP64 oracle",
@@ -324,6 +387,7 @@ def main():
         "precision_form_3d": p3,
         "batch_form": pb,
         "learned_form": pl,
+        "sod": ps,
     }
     print(json.dumps(out))
 
