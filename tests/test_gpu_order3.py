@@ -129,3 +129,36 @@ def test_refused(gpu):
         s["nx"], s["ny"] = 16, 4
     with pytest.raises(abi.MahycoError):
         cases.riemann_batch(gpu, specs, 0.01, **O3)
+
+
+def test_combined_options_match_oracle(gpu):
+    """Order 3 with the vanLeerG dual sweeps and the total-energy fix-up
+    together: one triple-point step within 1e-12 of the oracle."""
+    kw = dict(projection_limiteur_id=abi.LIM_VANLEERG, conservation_energie_totale=1, **O3)
+    o = H.tp_setup(H.oracle(), 56, 24, **kw)
+    o.run(30)
+    st = o.download()
+    g = H.tp_setup(gpu, 56, 24, **kw)
+    g.upload(st)
+    o.upload(st)
+    o.step()
+    g.step()
+    d = H.compare_states(g.download(), o.download(), 3, 56, 24)
+    bad = {k: v for k, v in d.items() if v > TOL}
+    assert not bad, bad
+
+
+def test_3d_combined_fp32_close_to_fp64(gpu):
+    """3D FP32-bulk with order 3, superBeeG dual sweeps and the fix-up stays
+    within 1e-5 (L1, relative) of FP64 on the three-material box."""
+    from test_gpu_3d import box3
+    kw = dict(projection_limiteur_id=abi.LIM_SUPERBEEG, conservation_energie_totale=1, **O3)
+    res = {}
+    for prec in (abi.PREC_FP64, abi.PREC_FP32_BULK):
+        s = box3(gpu, precision=prec, **kw)
+        s.run(30)
+        res[prec] = np.asarray(s.download(["density"])["density"]).copy()
+        s.close()
+    l1 = np.abs(res[1] - res[0]).sum() / np.abs(res[0]).sum()
+    print("3d fp32 vs fp64 L1", l1)
+    assert np.all(np.isfinite(res[1])) and l1 < 1e-5, l1
