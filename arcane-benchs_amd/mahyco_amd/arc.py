@@ -22,6 +22,8 @@ import sys
 import xml.etree.ElementTree as ET
 from dataclasses import dataclass, field
 
+import numpy as np
+
 from . import abi
 
 SURFACES = ("XMIN", "XMAX", "YMIN", "YMAX", "ZMIN", "ZMAX")
@@ -111,9 +113,25 @@ class Case:
         """A context for this case on `lib`, initialised by the case's
         initialisation service.  n: cells per direction instead of the
         file's; override: mhg_options fields (precision, ...)."""
-        s = abi.Simulation(lib, self.mesh(n), self.mhg_options(lib, **override))
-        s.init_sod(self.cas_test)
+        nn = list(n) if n is not None else list(self.n)
+        s = abi.Simulation(lib, self.mesh(nn), self.mhg_options(lib, **override))
+        if self.dim == 2:
+            s.init_sod(self.cas_test)
+        else:  # the library initialises 3D states cell by cell
+            s.init_cells(*self.sod_cells(nn))
         return s
+
+    def sod_cells(self, n):
+        """Environment, density and pressure of each cell of the SOD case
+        (SODService.cc:11-39): left state (1, 1) where the cell centre along
+        the case's axis is below 0.5, right state (0.125, 0.1) beyond; cases
+        13-15 put the right state in the second environment."""
+        axis = self.cas_test - (3 if self.cas_test < 13 else 13)
+        idx = np.indices(tuple(reversed(n))).reshape(len(n), -1)[::-1]  # i fastest
+        r = self.origin[axis] + (idx[axis] + 0.5) * (self.length[axis] / n[axis])
+        left = r < 0.5
+        env = np.where(left | (self.cas_test < 13), 0, 1).astype(np.int32)
+        return env, np.where(left, 1.0, 0.125), np.where(left, 1.0, 0.1)
 
 
 def _child_text(el, tag):
