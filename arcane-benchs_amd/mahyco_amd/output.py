@@ -63,7 +63,8 @@ def restart(lib, path, **override):
     # volumes, boundary node sets, EOS constants); the upload then replaces
     # every field it set
     mask = state["env_mask"].astype(np.int64)
-    first_env = np.log2(mask & -mask).astype(np.int32)
+    low = mask & -mask  # lowest set bit; a cell without environments takes environment 0 here
+    first_env = np.where(low > 0, np.log2(np.maximum(low, 1)), 0).astype(np.int32)
     sim.init_cells(first_env, np.ascontiguousarray(state["density"]), np.ascontiguousarray(state["pressure"]))
     state.update(meta["scalars"])
     state = {k: (np.ascontiguousarray(v) if isinstance(v, np.ndarray) else v) for k, v in state.items()}
