@@ -611,21 +611,25 @@ void computePressionMoyenne(mho_ctx& S) {
       }
 }
 
-// initEOS (PerfectGasEOSService.cc:20-31, StiffenedGasEOSService.cc:10-22).
-void initEOS(mho_ctx& S, int k) {
+// initEOS of one cell (PerfectGasEOSService.cc:20-31, StiffenedGasEOSService.cc:10-22):
+// internal energy and sound speed from density and pressure.
+void eosInitCell(const mho_ctx& S, int k, double rho, double p, double* e, double* cs) {
   const double g = S.gamma_env[k];
+  if (S.eos_env[k] == MHG_EOS_STIFFENED_GAS) {
+    const double t = S.ptens_env[k];
+    *e = (p + (g * t)) / ((g - 1.) * rho);
+    *cs = std::sqrt((g / rho) * (p + t));
+  } else {
+    *e = p / ((g - 1.) * rho);
+    *cs = std::sqrt(g * p / rho);
+  }
+}
+
+void initEOS(mho_ctx& S, int k) {
   for (long c = 0; c < S.NC; ++c) {
     if (!(S.env_mask[c] & (1 << k))) continue;
-    double p = S.EV(S.pressure, k, c);
-    double rho = S.EV(S.density, k, c);
-    if (S.eos_env[k] == MHG_EOS_STIFFENED_GAS) {
-      const double t = S.ptens_env[k];
-      S.EV(S.internal_energy, k, c) = (p + (g * t)) / ((g - 1.) * rho);
-      S.EV(S.sound_speed, k, c) = std::sqrt((g / rho) * (p + t));
-    } else {
-      S.EV(S.internal_energy, k, c) = p / ((g - 1.) * rho);
-      S.EV(S.sound_speed, k, c) = std::sqrt(g * p / rho);
-    }
+    eosInitCell(S, k, S.EV(S.density, k, c), S.EV(S.pressure, k, c), &S.EV(S.internal_energy, k, c),
+                &S.EV(S.sound_speed, k, c));
   }
 }
 
@@ -1833,6 +1837,10 @@ void buildMesh(mho_ctx& S) {
   computeGeometricValues(S);
   for (long c = 0; c < S.NC; ++c) S.euler_volume[c] = S.cell_volume.g[c];
   initGeometricValues(S);
+  // the boundary face groups of the initial mesh (PrepareFaceGroup runs at
+  // init on it), so that a context given its state by mho_upload_state
+  // without an init call has them too
+  prepareBoundaryConditions(S);
   S.global_deltat = S.opt.deltat_init;
 }
 
@@ -2310,11 +2318,21 @@ double mho_probe_viscosity(double gamma, double tau, double L, double c, double 
   return viscosity((gamma + 1) / 2.0, tau, L, c, divu);
 }
 void mho_probe_eos_pg(double gamma, double rho, double e, double* out3) {
+  mho_probe_eos(MHG_EOS_PERFECT_GAS, gamma, 0., rho, e, out3);
+}
+void mho_probe_eos(int32_t eos, double gamma, double limit_tension, double rho, double e, double* out3) {
   mho_ctx S;
   S.gamma_env.push_back(gamma);
-  S.ptens_env.push_back(0.);
-  S.eos_env.push_back(MHG_EOS_PERFECT_GAS);
+  S.ptens_env.push_back(limit_tension);
+  S.eos_env.push_back(eos);
   eosCell(S, 0, rho, e, &out3[0], &out3[1], &out3[2]);
+}
+void mho_probe_eos_init(int32_t eos, double gamma, double limit_tension, double rho, double p, double* out2) {
+  mho_ctx S;
+  S.gamma_env.push_back(gamma);
+  S.ptens_env.push_back(limit_tension);
+  S.eos_env.push_back(eos);
+  eosInitCell(S, 0, rho, p, &out2[0], &out2[1]);
 }
 
 }  // extern "C"

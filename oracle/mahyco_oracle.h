@@ -94,6 +94,12 @@ double mho_probe_eint(int32_t pseudo_centree, double gamma, double q_n, double q
 double mho_probe_viscosity(double gamma, double tau, double L, double c, double divu);
 /* Perfect-gas EOS (PerfectGasEOSService.cc:38-45): out[0]=p, out[1]=c, out[2]=dpde. */
 void mho_probe_eos_pg(double gamma, double rho, double e, double* out3);
+/* EOS of one cell, perfect gas (eos 0) or stiffened gas (eos 1,
+ * StiffenedGasEOSService.cc:27-44): out[0]=p, out[1]=c, out[2]=dpde. */
+void mho_probe_eos(int32_t eos, double gamma, double limit_tension, double rho, double e, double* out3);
+/* initEOS of one cell (PerfectGasEOSService.cc:20-31, StiffenedGasEOSService.cc:10-22):
+ * out[0]=e, out[1]=c from rho and p. */
+void mho_probe_eos_init(int32_t eos, double gamma, double limit_tension, double rho, double p, double* out2);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,135 @@
+"""GPU path vs the oracle on the multi-material cases that pin the oracle
+(tests/test_multimat_oracle.py): stiffened-gas materials, identical-twin
+materials in every cell, and a material contact in a uniform flow; one step
+within 1e-12 per field from oracle states, trajectories within 1e-10."""
+import numpy as np
+import pytest
+
+import helpers as H
+import test_multimat_oracle as M
+from mahyco_amd import abi, load_gpu
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    return load_gpu()
+
+
+def sg_tube(L, nx, ny, **kw):
+    """Two-material shock tube along x: perfect gas (gamma 1.4) at rho 1,
+    p 1 for x < 0.5; stiffened gas (gamma 3, p_inf 0.5) at rho 2, p 0.1
+    beyond."""
+    opt = L.options(**kw)
+    abi.set_bcs(opt, abi.reflecting_walls(2))
+    abi.set_envs(opt, [(abi.EOS_PERFECT_GAS, 1.4, 0.0), (abi.EOS_STIFFENED_GAS, 3.0, 0.5)])
+    s = abi.Simulation(L, abi.mesh_desc(2, [nx, ny], [0, 0], [1.0, ny / nx]), opt)
+    xc = ((np.arange(nx) + 0.5) / nx)[None, :].repeat(ny, 0).ravel()
+    env = (xc > 0.5).astype(np.int32)
+    s.init_cells(env, np.where(env == 0, 1.0, 2.0), np.where(env == 0, 1.0, 0.1))
+    return s
+
+
+def tp_sg(L, nx, ny, **kw):
+    """Triple point whose third material is a stiffened gas (gamma 1.5,
+    p_inf 0.05)."""
+    opt = L.options(**kw)
+    abi.set_bcs(opt, abi.reflecting_walls(2))
+    abi.set_envs(opt, [(abi.EOS_PERFECT_GAS, 1.5, 0.0), (abi.EOS_PERFECT_GAS, 1.4, 0.0),
+                       (abi.EOS_STIFFENED_GAS, 1.5, 0.05)])
+    opt.final_time = 1e30
+    opt.deltat_max = 1e30
+    s = abi.Simulation(L, abi.mesh_desc(2, [nx, ny], [0, 0], [7.0, 3.0]), opt)
+    s.init_triple_point(H.TP_RHO, H.TP_P)
+    return s
+
+
+def one_step(gpu, setup, warm, nx, ny, **kw):
+    o = setup(H.oracle(), nx, ny, **kw)
+    if warm:
+        o.run(warm)
+    g = setup(gpu, nx, ny, **kw)
+    g.upload(o.download())
+    o.step()
+    g.step()
+    d = H.compare_states(g.download(), o.download(), o.nenv, nx, ny)
+    print(setup.__name__, warm, {k: f"{v:.1e}" for k, v in d.items()})
+    bad = {k: v for k, v in d.items() if v > TOL}
+    assert not bad, bad
+
+
+def trajectory(gpu, setup, steps, nx, ny, **kw):
+    o = setup(H.oracle(), nx, ny, **kw)
+    g = setup(gpu, nx, ny, **kw)
+    o.run(steps)
+    g.run(steps)
+    d = H.compare_states(g.download(), o.download(), o.nenv, nx, ny, metric=H.rel_max)
+    print(setup.__name__, "trajectory", steps, {k: f"{v:.1e}" for k, v in d.items()})
+    assert max(d.values()) < 1e-10
+    return g
+
+
+@pytest.mark.parametrize("warm", [0, 20, 90])
+def test_stiffened_tube_one_step(gpu, warm):
+    one_step(gpu, sg_tube, warm, 60, 4)
+
+
+def test_stiffened_tube_trajectory(gpu):
+    g = trajectory(gpu, sg_tube, 150, 60, 4)
+    st = g.download()
+    assert (np.array([bin(int(v)).count("1") for v in st["env_mask"]]) >= 2).any()
+
+
+@pytest.mark.parametrize("warm", [0, 40])
+def test_stiffened_triple_point_one_step(gpu, warm):
+    one_step(gpu, tp_sg, warm, 56, 24)
+
+
+def test_stiffened_triple_point_trajectory(gpu):
+    trajectory(gpu, tp_sg, 120, 56, 24)
+
+
+def test_stiffened_fp32_form(gpu):
+    """The precision form on the stiffened triple point: relative L1 density
+    within 1e-5 of the oracle after 100 steps."""
+    o = tp_sg(H.oracle(), 56, 24)
+    g = tp_sg(gpu, 56, 24, precision=abi.PREC_FP32_BULK)
+    o.run(100)
+    g.run(100)
+    a, b = g.download()["density"], o.download()["density"]
+    l1 = np.sum(np.abs(a - b)) / np.sum(np.abs(b))
+    print("stiffened TP FP32 L1", l1)
+    assert l1 < 1e-5
+
+
+@pytest.mark.parametrize("fracs", M.TWIN_FRACS)
+def test_twins_gpu(gpu, fracs):
+    """Identical twins on the GPU: equal to the oracle's twin run, and to the
+    one-material run with the pseudo-viscosity rule emulated."""
+    single, twin_o = M.run_twins(H.oracle(), fracs, steps=100)
+    s1 = H.sod_setup(H.oracle(), 100, 4)
+    g = M.twin_sim(gpu, 100, 4, len(fracs))
+    g.upload(M.twin_state(s1.download(), fracs))
+    g.run(100)
+    twin_g = g.download()
+    d = H.compare_states(twin_g, twin_o, len(fracs), 100, 4, metric=H.rel_max)
+    print("twins gpu vs oracle", {k: f"{v:.1e}" for k, v in d.items()})
+    assert max(d.values()) < 1e-10
+    M.check_twins(single, twin_g, fracs, 1e-11)
+
+
+@pytest.mark.parametrize("eos", ["perfect", "stiffened"])
+def test_contact_gpu(gpu, eos):
+    P = M.CONTACT
+    kw = {} if eos == "perfect" else dict(eos=(M.PG, M.SG), pinf=(0.0, 2.0))
+    rho = P["rho"] if eos == "perfect" else (1.0, 2.0)
+    gam = P["gamma"] if eos == "perfect" else (1.4, 3.0)
+
+    def contact(L, nx, ny):
+        return M.contact_sim(L, nx, ny, P["u0"], rho, gam, P["x0"], **kw)
+    contact.__name__ = f"contact_{eos}"
+    for warm in (0, 30):
+        one_step(gpu, contact, warm, P["nx"], P["ny"])
+    trajectory(gpu, contact, P["steps"], P["nx"], P["ny"])
