@@ -90,6 +90,12 @@ struct mhg_ctx {
   };
   std::vector<StepGraph> graphs;
   int use_graphs = 1;
+  // mhg_run_async: fused iterations enqueued since the last host
+  // synchronisation and the device iteration count before them; iterations
+  // past the end of the run leave the device state alone, and the next call
+  // that reads the host bookkeeping rolls it back to the iterations done
+  int async_launched = 0;
+  int async_it0 = 0;
 
   // kernel timing
   bool timing = false;

@@ -397,10 +397,35 @@ int enqueue_steps(mhg_ctx* c, int n) {
   return MHG_OK;
 }
 
+// Undoes the host bookkeeping (coordinate and velocity buffers, sweep order)
+// of `skipped` enqueued iterations that the device did not run because the
+// run had ended.
+void roll_back(mhg_ctx* c, int skipped) {
+  for (int s = 0; s < skipped; ++s) {
+    c->sens = (c->sens + c->P.dim - 1) % c->P.dim;
+    c->xcur = 1 - c->xcur;
+    c->ucur = 1 - c->ucur;
+  }
+}
+
+// Waits for the iterations of mhg_run_async and brings the host bookkeeping
+// to the iterations the device really did.
+int reconcile_async(mhg_ctx* c) {
+  if (c->async_launched == 0) return MHG_OK;
+  const int launched = c->async_launched;
+  c->async_launched = 0;
+  int r = read_scalars(c);
+  if (r) return r;
+  collect_times(c);
+  const int done = c->S_host->iteration - c->async_it0;
+  roll_back(c, launched - done);
+  return MHG_OK;
+}
+
 int check_ready(mhg_ctx* c) {
   if (!c) return MHG_E_ARG;
   if (!c->initialised) return fail(c, MHG_E_STATE, "context not initialised");
-  return MHG_OK;
+  return reconcile_async(c);
 }
 
 // A problem of a batch is advanced by mhg_batch_run only.
@@ -484,6 +509,8 @@ int init_pure(mhg_ctx* c, const std::vector<int>& env, const std::vector<double>
   c->coord_phase = 0;
   c->ep_remapped = false;
   c->sens = 0;
+  c->async_launched = 0;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   auto up = [&](void* d, const std::vector<double>& h) { return put(c, d, h.data(), h.size()); };
   HIPCHK(c, up(c->rho, rho));
   HIPCHK(c, up(c->e, e));
@@ -780,6 +807,8 @@ int mhg_upload_state(mhg_ctx* c, const mhg_state* st) {
   c->ucur = 0;
   c->coord_phase = 0;
   c->ep_remapped = false;
+  c->async_launched = 0;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   auto h2d = [&](void* d, const void* h, size_t b) {
     return h ? hipMemcpy(d, h, b, hipMemcpyHostToDevice) : hipSuccess;
   };
@@ -1105,10 +1134,14 @@ int mhg_remap_is_euler(const mhg_ctx* c) { return c ? c->opt.is_euler_scheme : M
 int mhg_step(mhg_ctx* c, int32_t* stop) {
   int r = check_fused(c);
   if (r) return r;
+  r = read_scalars(c);
+  if (r) return r;
+  const int it0 = c->S_host->iteration;
   enqueue_step(c);
   HIPCHK(c, hipGetLastError());
   r = read_scalars(c);
   if (r) return r;
+  roll_back(c, 1 - (c->S_host->iteration - it0));  // a step after the end is skipped
   collect_times(c);
   if (stop) *stop = c->S_host->stop;
   if (c->S_host->err) return fail(c, MHG_E_DT_MIN, "dt below deltat-min");
@@ -1118,14 +1151,22 @@ int mhg_step(mhg_ctx* c, int32_t* stop) {
 int mhg_run_async(mhg_ctx* c, int32_t n) {
   int r = check_fused(c);
   if (r) return r;
-  return enqueue_steps(c, n);
+  if (n < 0) return fail(c, MHG_E_ARG, "run_async: negative step count");
+  // check_fused has reconciled earlier iterations: the device count is the
+  // host's starting point for these
+  r = read_scalars(c);
+  if (r) return r;
+  c->async_it0 = c->S_host->iteration;
+  r = enqueue_steps(c, n);
+  c->async_launched = n;
+  return r;
 }
 
 int mhg_synchronize(mhg_ctx* c) {
   if (!c) return MHG_E_ARG;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   collect_times(c);
-  return MHG_OK;
+  return reconcile_async(c);
 }
 
 int mhg_run(mhg_ctx* c, int32_t max_steps, int32_t* steps_done) {
@@ -1148,12 +1189,7 @@ int mhg_run(mhg_ctx* c, int32_t max_steps, int32_t* steps_done) {
   // kernels of steps past the end of the run return at once; the host
   // bookkeeping (buffers, sweep order) must follow the steps really done
   const int done = c->S_host->iteration - it0;
-  const int skipped = launched - done;
-  for (int s = 0; s < skipped; ++s) {
-    c->sens = (c->sens + c->P.dim - 1) % c->P.dim;
-    c->xcur = 1 - c->xcur;
-    c->ucur = 1 - c->ucur;
-  }
+  roll_back(c, launched - done);
   if (steps_done) *steps_done = done;
   if (c->S_host->err) return fail(c, MHG_E_DT_MIN, "dt below deltat-min");
   return c->S_host->stop ? MHG_STOP : MHG_OK;

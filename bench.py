@@ -6,11 +6,14 @@ with all data resident in HBM.  W warmup steps run first, then the state is
 re-initialised and exactly K steps are timed (K=500 is the configuration's
 run), bracketed by device synchronisations.  Per-kernel durations come from
 HIP events recorded on the library's stream around every launch of the timed
-region.  The CPU baseline is the C++ oracle (a port of the reference
-arithmetic) timed on this host on a bounded sample of the same workload, on
-all host threads and on one.  The `sod` field is the 400x20 Sod shock tube
-run to t = 0.2 on the GPU, with its L1 density error against the exact
-Riemann solution; the oracle times the same run inside the CPU baseline.
+region.  Untimed runs of the configuration's 500 steps, in FP64 and in the
+FP32-bulk precision form, give the accuracy fields: their densities against
+the C++ oracle's (a port of the reference arithmetic) after the same 500
+steps.  That oracle run, on all host threads, is the CPU baseline; one
+thread is timed on a 20-step sample.  The `sod` field is the 400x20 Sod
+shock tube run to t = 0.2 on the GPU, with its L1 density error against the
+exact Riemann solution; the oracle times the same run inside the CPU
+baseline.
 """
 import argparse
 import json
@@ -43,10 +46,11 @@ def set_timing(sim, on):
     sim.L.optional["kernel_times"](sim.ctx, 1 if on else 0, None, None, None, 0)
 
 
-def cpu_baseline(gpu_density_after, steps, sod_gpu=None):
-    """Oracle on the same triple point for `steps` steps, all host threads,
-    then one thread on a fifth of those steps; and, if sod_gpu (the GPU's
-    Sod run) is given, the whole Sod case on all threads and on one."""
+def cpu_baseline(gpu_density, steps, steps_1thread, sod_gpu=None, fp32_density=None):
+    """Oracle on the same triple point for `steps` steps on all host threads
+    (the accuracy reference of the GPU runs after the same steps), then one
+    thread for `steps_1thread` steps; and, if sod_gpu (the GPU's Sod run) is
+    given, the whole Sod case on all threads and on one."""
     from importlib import util
     spec = util.spec_from_file_location("oracle_lib", os.path.join(ROOT, "tests", "helpers.py"))
     H = util.module_from_spec(spec)
@@ -56,24 +60,34 @@ def cpu_baseline(gpu_density_after, steps, sod_gpu=None):
     threads = L.optional["set_threads"](0)
     o = cases.triple_point(L, NX, NY)
     t0 = time.perf_counter()
-    o.run(steps)
+    done = 0
+    while done < steps:  # progress line every 100 steps
+        done += o.run(min(100, steps - done))
+        print(f"oracle triple point: {done}/{steps} steps, {time.perf_counter() - t0:.1f} s",
+              file=sys.stderr, flush=True)
     t1 = time.perf_counter()
     rho = o.download(["density"])["density"]
     o.close()
-    l1 = float(np.sum(np.abs(gpu_density_after - rho)) / np.sum(np.abs(rho)))
+
+    def l1(a):
+        return float(np.sum(np.abs(a - rho)) / np.sum(np.abs(rho)))
+
+    acc = {"after_steps": steps, "fp64": l1(gpu_density)}
+    if fp32_density is not None:
+        acc["fp32"] = l1(fp32_density)
+        acc["fp32_linf"] = float(np.max(np.abs(fp32_density - rho)) / np.max(np.abs(rho)))
     out = {"value": NX * NY * steps / (t1 - t0), "unit": "zone-cycles/s", "cores": threads,
            "kind": "port",
            "sample": f"C++ oracle (OpenMP, {threads} threads) on the same 1792x768 triple point, "
                      f"steps 1..{steps}, {t1 - t0:.1f} s"}
     L.optional["set_threads"](1)
-    s1 = max(1, steps // 5)
     o = cases.triple_point(L, NX, NY)
     t0 = time.perf_counter()
-    o.run(s1)
+    o.run(steps_1thread)
     t1 = time.perf_counter()
     o.close()
-    out["value_1thread"] = NX * NY * s1 / (t1 - t0)
-    out["sample_1thread"] = f"the same, 1 thread, steps 1..{s1}, {t1 - t0:.1f} s"
+    out["value_1thread"] = NX * NY * steps_1thread / (t1 - t0)
+    out["sample_1thread"] = f"the same, 1 thread, steps 1..{steps_1thread}, {t1 - t0:.1f} s"
     if sod_gpu is not None:
         sod = {}
         for th in (threads, 1):
@@ -89,19 +103,29 @@ def cpu_baseline(gpu_density_after, steps, sod_gpu=None):
         sod["l1_density_gpu_vs_oracle"] = float(np.sum(np.abs(sod_gpu - rho_o)) / np.sum(np.abs(rho_o)))
         out["sod"] = sod
     L.optional["set_threads"](threads)
-    return out, l1
+    return out, acc
 
 
 # Sod shock tube (SODService.cc:11-39, cas-test 3): 400x20 cells on [0,1]x[0,0.05],
-# deltat-init 1e-5, deltat-min 1e-8, deltat-max 0.01, final time 0.2 (Data.1.arc)
+# deltat-init 1e-5, deltat-min 1e-8, deltat-max 0.01, final time 0.2 (Data.1.arc),
+# artificial viscosity coefficients 0.5 (linear) and 1.0 (quadratic; the
+# reference hard-codes (gamma + 1) / 2 = 1.2, MahycoModule.cc:517-528)
 SOD_NX, SOD_NY = 400, 20
-SOD_OPT = dict(deltat_init=1e-5, deltat_min=1e-8, deltat_max=0.01, final_time=0.2)
+SOD_OPT = dict(deltat_init=1e-5, deltat_min=1e-8, deltat_max=0.01, final_time=0.2, visc_quad_coeff=1.0)
+
+
+def sod_l1(st):
+    from mahyco_amd import riemann
+    rho = np.asarray(st["density"])
+    xc = (np.arange(SOD_NX) + 0.5) / SOD_NX
+    rex, _, _ = riemann.sample(xc, st["global_time"], 0.5, (1, 0, 1), (0.125, 0, 0.1), 1.4)
+    return float(np.mean(np.abs(rho.reshape(SOD_NY, SOD_NX) - rex[None, :])))
 
 
 def sod_run(gpu, sync):
     """The Sod case to t = 0.2 on the GPU (FP64): zone-cycles/s of the whole
-    run and its L1 density error against the exact Riemann solution."""
-    from mahyco_amd import riemann
+    run and its L1 density error against the exact Riemann solution; also
+    the L1 error with the reference's own quadratic coefficient."""
     s = cases.sod(gpu, SOD_NX, SOD_NY, **SOD_OPT)
     s.run(3)  # first launches and graph capture out of the timed region
     s.close()
@@ -114,12 +138,16 @@ def sod_run(gpu, sync):
     st = s.download()
     s.close()
     rho = np.asarray(st["density"])
-    xc = (np.arange(SOD_NX) + 0.5) / SOD_NX
-    rex, _, _ = riemann.sample(xc, st["global_time"], 0.5, (1, 0, 1), (0.125, 0, 0.1), 1.4)
-    l1 = float(np.mean(np.abs(rho.reshape(SOD_NY, SOD_NX) - rex[None, :])))
-    return {"config": "Sod 400x20 on [0,1]x[0,0.05], gamma 1.4, order 2 minmod, to t = 0.2 (cas-test 3)",
+    ref = dict(SOD_OPT, visc_quad_coeff=-1.0)
+    s = cases.sod(gpu, SOD_NX, SOD_NY, **ref)
+    s.run(1 << 30)
+    l1_ref = sod_l1(s.download())
+    s.close()
+    return {"config": "Sod 400x20 on [0,1]x[0,0.05], gamma 1.4, order 2 minmod, viscosity 0.5 / 1.0, "
+                      "to t = 0.2 (cas-test 3)",
             "steps": n, "time": st["global_time"], "value": SOD_NX * SOD_NY * n / (t1 - t0),
-            "unit": "zone-cycles/s", "l1_density_vs_exact_riemann": l1}, rho
+            "unit": "zone-cycles/s", "l1_density_vs_exact_riemann": sod_l1(st),
+            "l1_density_vs_exact_riemann_quadratic_coeff_1.2": l1_ref}, rho
 
 
 def timed_run(gpu, precision, args, sync):
@@ -272,7 +300,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--accuracy-steps", type=int, default=500,
+                    help="steps of the untimed triple-point runs whose densities are compared "
+                         "(FP64 and FP32 bulk against the oracle, FP32 against FP64)")
+    ap.add_argument("--cpu-steps-1thread", type=int, default=20,
+                    help="steps of the one-thread oracle sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-events", action="store_true", help="do not record per-kernel events")
     ap.add_argument("--no-precision-form", action="store_true",
@@ -309,24 +341,35 @@ def main():
     gpu = load_gpu()
     res = timed_run(gpu, abi.PREC_FP64, args, sync)
     sim, elapsed, tuning, kt = res["sim"], res["elapsed"], res["tuning"], res["kt"]
-    rho_timed = res["density"]
     # precision form: FP32 bulk fields, FP64 coordinates, geometry and dt
     p32 = None
     if not args.no_precision_form:
         r32 = timed_run(gpu, abi.PREC_FP32_BULK, args, sync)
         v32 = sim.nc * args.steps / r32["elapsed"]
         v64 = sim.nc * args.steps / elapsed
-        d32 = r32["density"]
         p32 = {"value": v32, "unit": "zone-cycles/s", "ms_per_step": 1e3 * r32["elapsed"] / args.steps,
                "speedup_vs_fp64": v32 / v64,
-               "l1_density_err_vs_fp64": float(np.sum(np.abs(d32 - rho_timed)) / np.sum(np.abs(rho_timed))),
-               "linf_density_err_vs_fp64": float(np.max(np.abs(d32 - rho_timed)) / np.max(np.abs(rho_timed))),
-               "reference": "the FP64 path's density after the same steps (it matches the oracle to 1e-10)",
+               "reference": "densities after the same untimed steps of the FP64 path and of the oracle",
                "dtype": "f32 bulk; f64 coordinates, cell geometry, dt",
                "kernel_ms": {k: round(v[0], 5) for k, v in r32["kt"].items()}}
         r32["sim"].close()
     nc, nn = sim.nc, sim.nn
     value = nc * args.steps / elapsed
+    # accuracy: the configuration's full run (500 steps), FP64 and FP32 bulk
+    rho_acc = {}
+    for name, prec in [("fp64", abi.PREC_FP64)] + ([] if args.no_precision_form else
+                                                    [("fp32", abi.PREC_FP32_BULK)]):
+        g2 = cases.triple_point(gpu, NX, NY, precision=prec)
+        g2.run(args.accuracy_steps)
+        st2 = g2.download(["density"])
+        assert st2["global_iteration"] == args.accuracy_steps
+        rho_acc[name] = st2["density"]
+        g2.close()
+    if p32 is not None:
+        a, r = rho_acc["fp32"], rho_acc["fp64"]
+        p32["l1_density_err_vs_fp64"] = float(np.sum(np.abs(a - r)) / np.sum(np.abs(r)))
+        p32["linf_density_err_vs_fp64"] = float(np.max(np.abs(a - r)) / np.max(np.abs(r)))
+        p32["after_steps"] = args.accuracy_steps
     p3 = None if args.no_p3d else p3d_run(gpu, args, sync)
     pb = None if args.no_batch else batch_run(gpu, args, sync)
     pl = None if args.no_learned else learned_run(gpu, args)
@@ -355,13 +398,13 @@ def main():
                   "triple-point traffic profile, over this run's ms_per_step"}
 
     ps, sod_rho = (None, None) if args.no_sod else sod_run(gpu, sync)
-    cpu, l1 = None, None
+    cpu, acc = None, None
     if not args.no_cpu_baseline:
-        g2 = cases.triple_point(gpu, NX, NY)
-        g2.run(args.cpu_steps)
-        rho_g = g2.download(["density"])["density"]
-        g2.close()
-        cpu, l1 = cpu_baseline(rho_g, args.cpu_steps, sod_rho)
+        cpu, acc = cpu_baseline(rho_acc["fp64"], args.accuracy_steps, args.cpu_steps_1thread, sod_rho,
+                                rho_acc.get("fp32"))
+        if p32 is not None:
+            p32["l1_density_err_vs_oracle"] = acc["fp32"]
+            p32["linf_density_err_vs_oracle"] = acc["fp32_linf"]
 
     out = {
         "metric": "zone-cycles/s on 1792x768 triple-point; L1 density err vs FP64 oracle",
@@ -369,13 +412,14 @@ def main():
         "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "triple point 1792x768, 3 perfect-gas materials, plain FP64 "
-                               "Lagrange + Euler-ADI remap (order 2, minmod), 500 steps",
+        "config": {"workload": f"triple point 1792x768, 3 perfect-gas materials, plain FP64 "
+                               f"Lagrange + Euler-ADI remap (order 2, minmod): {args.steps} steps timed, "
+                               f"densities compared after {args.accuracy_steps} steps",
                    "cells": nc, "nodes": nn, "steps": args.steps, "parallelism": "none (1 GPU)",
                    "tuning": {part.split(":")[0].strip(): part.split("->")[-1].strip()
                               for part in tuning.split(";") if "->" in part} if tuning else None},
-        "l1_density_err_vs_oracle": (None if l1 is None else
-                                     {"value": l1, "after_steps": args.cpu_steps}),
+        "l1_density_err_vs_oracle": (None if acc is None else
+                                     {"value": acc["fp64"], "after_steps": acc["after_steps"]}),
         "step_roofline_frac": round(step_frac, 4),
         "step_measured_hbm": step_meas,
         "kernel_ms": {k: round(v[0], 5) for k, v in kt.items()},

@@ -60,12 +60,20 @@ def one_step(gpu, setup, warm, nx, ny, **kw):
     assert not bad, bad
 
 
-def trajectory(gpu, setup, steps, nx, ny, **kw):
+def trajectory(gpu, setup, steps, nx, ny, q_env_floor=False, **kw):
+    """q_env_floor: where the mixed cells see no compression, so that their
+    materials' pseudo-viscosity is round-off noise, its differences are
+    measured against the largest cell pseudo-viscosity instead of its own
+    (round-off) maximum."""
     o = setup(H.oracle(), nx, ny, **kw)
     g = setup(gpu, nx, ny, **kw)
     o.run(steps)
     g.run(steps)
-    d = H.compare_states(g.download(), o.download(), o.nenv, nx, ny, metric=H.rel_max)
+    got, ref = g.download(), o.download()
+    d = H.compare_states(got, ref, o.nenv, nx, ny, metric=H.rel_max)
+    if q_env_floor:
+        f = "pseudo_viscosity_env"
+        d[f] = float(np.abs(got[f] - ref[f]).max()) / np.abs(ref["pseudo_viscosity"]).max()
     print(setup.__name__, "trajectory", steps, {k: f"{v:.1e}" for k, v in d.items()})
     assert max(d.values()) < 1e-10
     return g
@@ -132,4 +140,4 @@ def test_contact_gpu(gpu, eos):
     contact.__name__ = f"contact_{eos}"
     for warm in (0, 30):
         one_step(gpu, contact, warm, P["nx"], P["ny"])
-    trajectory(gpu, contact, P["steps"], P["nx"], P["ny"])
+    trajectory(gpu, contact, P["steps"], P["nx"], P["ny"], q_env_floor=True)

@@ -275,3 +275,27 @@ def test_fp32_precision_form_sod_exact(gpu):
     e_g = np.mean(np.abs(row_g - exact))
     print("sod L1 vs exact: fp64 oracle", e_o, "fp32 gpu", e_g)
     assert e_g < 1.01 * e_o
+
+
+def test_run_async_past_the_end(gpu):
+    """Steps enqueued past final-time are skipped by the device, and the next
+    call sees the state of the steps really done: equal to the oracle's run
+    to the end, and a later step stops at once with the state unchanged."""
+    o = H.sod_setup(H.oracle(), 40, 4, final_time=0.02)
+    g = H.sod_setup(gpu, 40, 4, final_time=0.02)
+    n = o.run(100000)
+    assert 20 < n < 200
+    r = g.L.optional["run_async"](g.ctx, n + 37)
+    assert r == 0
+    assert g.L.optional["synchronize"](g.ctx) == 0
+    got = g.download()
+    assert got["global_iteration"] == n
+    d = H.compare_states(got, o.download(), 1, 40, 4, metric=H.rel_max)
+    print("run_async past the end", n, {k: f"{v:.1e}" for k, v in d.items()})
+    assert max(d.values()) < 1e-10
+    # a second async call from the stopped state: nothing runs, nothing moves
+    assert g.L.optional["run_async"](g.ctx, 13) == 0
+    again = g.download()
+    for f in ["density", "velocity", "node_coord"]:
+        assert np.array_equal(again[f], got[f]), f
+    assert g.step()
