@@ -22,7 +22,8 @@ __device__ inline double node_y0(const Params& P, int j) { return P.org[1] + j *
 // K1: node phase.  updateForceAndVelocity (MahycoModule.cc:589-665) with the
 // corner normals of the previous Lagrangian coordinates (saveValuesAtN copies
 // m_cell_cqs computed after the previous updatePosition, 344/365),
-// applyBoundaryCondition (936-1004), updatePosition (870-931) from x0.
+// applyBoundaryCondition (936-1004), updatePosition (870-931) from x0 (from
+// the previous coordinates with with-projection = false).
 // Node mass = 1/4 sum of the cell masses (computeNodeMass 235-266): the cell
 // masses do not change in the Lagrange phase, so it is recomputed here.
 // ---------------------------------------------------------------------------
@@ -108,8 +109,11 @@ __device__ __forceinline__ void node_lagrange_2d_body(const Params& P, DevScalar
     st_stream(&A.uy[nid], (R)vy);
   }
   if (POS) {
-    st_stream(&A.xnx[nid], node_x0(P, i) + dt * vx);
-    st_stream(&A.xny[nid], node_y0(P, j) + dt * vy);
+    // Euler scheme: saveValuesAtN reset x to x0; Lagrange-only: x moves on
+    const double bx = P.lagonly ? A.xpx[nid] : node_x0(P, i);
+    const double by = P.lagonly ? A.xpy[nid] : node_y0(P, j);
+    st_stream(&A.xnx[nid], bx + dt * vx);
+    st_stream(&A.xny[nid], by + dt * vy);
   }
 }
 
@@ -681,10 +685,29 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
 // 1836-1915) and the CFL candidate of computeHydroDeltaT (2047-2061) reduced
 // to one minimum per block; k_dt_finalize finishes computeDeltaT (2076-2191).
 // ---------------------------------------------------------------------------
-template <class R, int BLOCK, int NEM>
+template <class R, int BLOCK, int NEM, bool EOS = true>
 __device__ __forceinline__ void cell_final_2d_body(const Params& P, DevScalars* S, const CellFinalArgsT<R>& A,
                                                    long blk, double* block_min) {
   if (S->stop || S->err) return;
+  if (!EOS) {  // remapVariables only: no EOS, no CFL candidate
+    const long c = blk * BLOCK + threadIdx.x;
+    if (c >= P.nc) return;
+    const int i = (int)(c % P.nx), j = (int)(c / P.nx);
+    R delta_ec = R(0);
+    if (P.etot) {
+      const int nnx = P.nx + 1;
+      const long n0 = i + (long)nnx * j;
+      const long nd[4] = {n0, n0 + 1, n0 + 1 + nnx, n0 + nnx};
+      R sum = R(0);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) sum += A.dek[nd[q]];
+      delta_ec = smax(R(0), sum);
+    }
+    double x0[4] = {node_x0(P, i), node_x0(P, i + 1), node_x0(P, i + 1), node_x0(P, i)};
+    double y0[4] = {node_y0(P, j), node_y0(P, j), node_y0(P, j + 1), node_y0(P, j + 1)};
+    final_cell<R, NEM, false>(P, A, c, quad_volume(x0, y0), delta_ec);
+    return;
+  }
   const long c = blk * BLOCK + threadIdx.x;
   double dx_sound = 1.7976931348623157e308;
   if (c < P.nc) {

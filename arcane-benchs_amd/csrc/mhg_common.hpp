@@ -483,6 +483,9 @@ struct Params {
                 // network's coefficients also at material interfaces
   int land;     // shorten the last step to end at tend
   int etot;     // total-energy fix-up of the remap (conservation-energie-totale)
+  int lagonly;  // with-projection = false: no remap, the mesh keeps moving
+                // (updatePosition from the previous coordinates), the CFL
+                // has no velocity term
   double cfl, dtmin, dtmax, tend;
   int nbc;
   BcRange bc[kMaxBc];

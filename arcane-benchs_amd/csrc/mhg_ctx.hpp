@@ -81,7 +81,8 @@ struct mhg_ctx {
   bool ep_ready = false;
   int coord_phase = 0;  // 0: xx[xcur] (last Lagrange phase); 1: x0 (after
                         // saveValuesAtN); 2: xx[1 - xcur] (after updatePosition)
-  bool ep_remapped = false;  // k_cell_final_2d ran since the last dt
+  bool ep_remapped = false;  // the remap of this iteration ran (cells remapped, dt not yet computed)
+  int remap_stage = 0;       // 1: mhg_remap_prepare built the remap vectors of this iteration
 
   // captured groups of fused iterations, one per host buffer state
   struct StepGraph {

@@ -180,7 +180,7 @@ void collect_times(mhg_ctx* c) {
 // idir = (i + SensProjection) % dim, cell sweep then dual sweep; then
 // SensProjection = (SensProjection + 1) % dim and remapVariables.
 template <class R>
-void remap_2d(mhg_ctx* c, int dt_mode) {
+void remap_2d(mhg_ctx* c, int dt_mode, bool eos = true) {
   const Params& P = c->P;
   for (int pass = 0; pass < P.dim; ++pass) {
     const int idir = (pass + c->sens) % P.dim;
@@ -197,7 +197,7 @@ void remap_2d(mhg_ctx* c, int dt_mode) {
   }
   {
     TimedLaunch t(c, "cell_final");
-    launch_cell_final_2d(c->stream, P, c->S, cell_final_args<R>(c), dt_mode);
+    launch_cell_final_2d(c->stream, P, c->S, cell_final_args<R>(c), dt_mode, eos);
   }
   c->sens = (c->sens + 1) % P.dim;
   c->xcur = 1 - c->xcur;
@@ -225,8 +225,41 @@ void enqueue_step_t(mhg_ctx* c) {
 
 // 3D remap: three cell sweeps and three dual sweeps; the dual state goes
 // velocity -> Ud -> UdB -> velocity.
+// Final cell phase arguments of a 3D context (remapped velocity in the
+// buffer the last dual sweep writes).
 template <class R>
-void remap_3d(mhg_ctx* c, int dt_mode) {
+CellFinal3ArgsT<R> cell_final3_args(mhg_ctx* c, int unew) {
+  CellFinal3ArgsT<R> F{};
+  F.b.U = bp<R>(c->U[1]);
+  F.b.est = c->est;
+  F.b.Vlag = bp<R>(c->Vlag);
+  F.b.L = bp<R>(c->L);
+  F.b.ux = bp<R>(c->ux[unew]);
+  F.b.uy = bp<R>(c->uy[unew]);
+  F.uz = bp<R>(c->uz[unew]);
+  F.b.dek = bp<R>(c->dek);
+  F.b.rho = bp<R>(c->rho);
+  F.b.e = bp<R>(c->e);
+  F.b.p = bp<R>(c->p);
+  F.b.cs = bp<R>(c->cs);
+  F.b.q = bp<R>(c->q);
+  F.b.m = bp<R>(c->m);
+  F.b.mask = c->mask;
+  F.b.env_rho = bp<R>(c->env_rho);
+  F.b.env_e = bp<R>(c->env_e);
+  F.b.env_p = bp<R>(c->env_p);
+  F.b.env_cs = bp<R>(c->env_cs);
+  F.b.env_q = bp<R>(c->env_q);
+  F.b.env_m = bp<R>(c->env_m);
+  F.b.env_f = bp<R>(c->env_f);
+  F.b.env_mf = bp<R>(c->env_mf);
+  F.b.block_min = c->block_min;
+  F.Veuler = c->Veuler;
+  return F;
+}
+
+template <class R>
+void remap_3d(mhg_ctx* c, int dt_mode, bool eos = true) {
   const Params& P = c->P;
   const double* xn[3] = {c->xx[1 - c->xcur], c->xy[1 - c->xcur], c->xz[1 - c->xcur]};
   const R* u[3] = {bp<R>(c->ux[c->ucur]), bp<R>(c->uy[c->ucur]), bp<R>(c->uz[c->ucur])};
@@ -288,35 +321,9 @@ void remap_3d(mhg_ctx* c, int dt_mode) {
     }
     prevmask |= 1 << idir;
   }
-  CellFinal3ArgsT<R> F{};
-  F.b.U = bp<R>(c->U[1]);
-  F.b.est = c->est;
-  F.b.Vlag = bp<R>(c->Vlag);
-  F.b.L = bp<R>(c->L);
-  F.b.ux = bp<R>(c->ux[1 - c->ucur]);
-  F.b.uy = bp<R>(c->uy[1 - c->ucur]);
-  F.uz = bp<R>(c->uz[1 - c->ucur]);
-  F.b.dek = bp<R>(c->dek);
-  F.b.rho = bp<R>(c->rho);
-  F.b.e = bp<R>(c->e);
-  F.b.p = bp<R>(c->p);
-  F.b.cs = bp<R>(c->cs);
-  F.b.q = bp<R>(c->q);
-  F.b.m = bp<R>(c->m);
-  F.b.mask = c->mask;
-  F.b.env_rho = bp<R>(c->env_rho);
-  F.b.env_e = bp<R>(c->env_e);
-  F.b.env_p = bp<R>(c->env_p);
-  F.b.env_cs = bp<R>(c->env_cs);
-  F.b.env_q = bp<R>(c->env_q);
-  F.b.env_m = bp<R>(c->env_m);
-  F.b.env_f = bp<R>(c->env_f);
-  F.b.env_mf = bp<R>(c->env_mf);
-  F.b.block_min = c->block_min;
-  F.Veuler = c->Veuler;
   {
     TimedLaunch t(c, "cell_final");
-    launch_cell_final_3d(c->stream, P, c->S, F, dt_mode);
+    launch_cell_final_3d(c->stream, P, c->S, cell_final3_args<R>(c, 1 - c->ucur), dt_mode, eos);
   }
   c->sens = (c->sens + 1) % 3;
   c->xcur = 1 - c->xcur;
@@ -560,7 +567,7 @@ int build_params(const mhg_mesh_desc* mesh, const mhg_options* opt, Params* Pout
     if (mesh->n[d] < 1 || !(mesh->length[d] > 0.)) return MHG_E_ARG;
   if (opt->nb_env < 1 || opt->nb_env > MHG_MAX_ENV || opt->nb_bc < 0 || opt->nb_bc > MHG_MAX_BC)
     return MHG_E_ARG;
-  if (opt->schema_csts || opt->with_newton || opt->sans_lagrange || !opt->with_projection ||
+  if (opt->schema_csts || opt->with_newton || opt->sans_lagrange ||
       !opt->is_euler_scheme || opt->projection_pente_borne ||
       (opt->projection_limiteur_id > MHG_LIM_ULTRABEEG && opt->projection_limiteur_id != MHG_LIM_LEARNED &&
        opt->projection_limiteur_id != MHG_LIM_LEARNED_ALL) ||
@@ -573,6 +580,9 @@ int build_params(const mhg_mesh_desc* mesh, const mhg_options* opt, Params* Pout
   // limiters
   if (opt->ordre_projection == 3 && learned) return MHG_E_UNSUPPORTED;
   if (opt->precision != MHG_PREC_FP64 && opt->precision != MHG_PREC_FP32_BULK) return MHG_E_UNSUPPORTED;
+  // the Lagrange-only mode (with-projection = false) runs the FP64
+  // entry-point kernels
+  if (!opt->with_projection && (opt->precision != MHG_PREC_FP64 || learned)) return MHG_E_UNSUPPORTED;
   Params P{};
   P.dim = mesh->dim;
   P.ne = opt->nb_env;
@@ -599,6 +609,7 @@ int build_params(const mhg_mesh_desc* mesh, const mhg_options* opt, Params* Pout
   P.lim = learned ? MHG_LIM_MINMOD : opt->projection_limiteur_id;
   P.learned = !learned ? 0 : (opt->projection_limiteur_id == MHG_LIM_LEARNED_ALL ? 2 : 1);
   P.etot = opt->conservation_energie_totale ? 1 : 0;
+  P.lagonly = opt->with_projection ? 0 : 1;
   P.pseudo_centree = opt->pseudo_centree;
   P.lc = opt->longueur_caracteristique;
   P.cfl = opt->cfl;
@@ -930,10 +941,12 @@ int mhg_download_state(mhg_ctx* c, mhg_state* st) {
     if (P.dim == 3) HIPCHK(c, get(c, st->velocity + 2 * NN, c->uz[c->ucur], NN));
   }
   if (st->node_coord) {
-    if (c->coord_phase == 1) {  // Euler reset of saveValuesAtN: x = x0
+    if (c->coord_phase == 1 && !P.lagonly) {  // Euler reset of saveValuesAtN: x = x0
       for (long n = 0; n < NN; ++n) {
-        st->node_coord[n] = x0_of(P, 0, (int)(n % (P.nx + 1)));
-        st->node_coord[NN + n] = x0_of(P, 1, (int)(n / (P.nx + 1)));
+        const int i = (int)(n % (P.nx + 1)), j = (int)((n / (P.nx + 1)) % (P.ny + 1));
+        st->node_coord[n] = x0_of(P, 0, i);
+        st->node_coord[NN + n] = x0_of(P, 1, j);
+        if (P.dim == 3) st->node_coord[2 * NN + n] = x0_of(P, 2, (int)(n / ((long)(P.nx + 1) * (P.ny + 1))));
       }
     } else {
       const int b = c->coord_phase == 2 ? 1 - c->xcur : c->xcur;
@@ -978,9 +991,10 @@ int mhg_download_state(mhg_ctx* c, mhg_state* st) {
 }
 
 // ---- per-entry-point path (Mahyco.config:30-41) ---------------------------
-// Each call enqueues the kernels of one reference entry point on the context
-// stream and waits for them, so the downloaded state between calls is the
-// state the reference holds between its entry points.
+// Each call enqueues the kernels of one reference entry point (or service
+// method) on the context stream and waits for them, so the downloaded state
+// between calls is the state the reference holds between its calls.  2D and
+// 3D, FP64 contexts.
 namespace {
 
 int ep_prepare(mhg_ctx* c) {
@@ -988,8 +1002,7 @@ int ep_prepare(mhg_ctx* c) {
   if (r) return r;
   r = check_not_batch(c);
   if (r) return r;
-  if (c->bulk != 8 || c->P.dim != 2)
-    return fail(c, MHG_E_UNSUPPORTED, "entry points: 2D FP64 contexts only (use mhg_step)");
+  if (c->bulk != 8) return fail(c, MHG_E_UNSUPPORTED, "entry points: FP64 contexts only (use mhg_step)");
   if (c->ep_ready) return MHG_OK;
   const size_t NC = c->P.nc, envn = (size_t)c->P.ne * NC;
   r |= dalloc(c, &c->rho_n, NC) | dalloc(c, &c->e_n, NC) | dalloc(c, &c->q_n, NC);
@@ -1034,24 +1047,96 @@ StageArgs stage_args(mhg_ctx* c) {
   return A;
 }
 
+EosArgs eos_args(mhg_ctx* c) {
+  EosArgs A;
+  A.rho = bp<double>(c->rho);
+  A.e = bp<double>(c->e);
+  A.p = bp<double>(c->p);
+  A.cs = bp<double>(c->cs);
+  A.mask = c->mask;
+  A.env_rho = bp<double>(c->env_rho);
+  A.env_e = bp<double>(c->env_e);
+  A.env_p = bp<double>(c->env_p);
+  A.env_cs = bp<double>(c->env_cs);
+  return A;
+}
+
+PmArgs pm_args(mhg_ctx* c) {
+  PmArgs A;
+  A.p = bp<double>(c->p);
+  A.cs = bp<double>(c->cs);
+  A.mask = c->mask;
+  A.env_f = bp<double>(c->env_f);
+  A.env_p = bp<double>(c->env_p);
+  A.env_cs = bp<double>(c->env_cs);
+  return A;
+}
+
 int ep_finish(mhg_ctx* c) {
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return MHG_OK;
 }
 
+void enqueue_stage(mhg_ctx* c, int stage) {
+  if (stage == STAGE_GEOMETRY && c->P.dim == 3)
+    launch_cell_geometry_3d(c->stream, c->P, c->S, c->xx[1 - c->xcur], c->xy[1 - c->xcur], c->xz[1 - c->xcur],
+                            bp<double>(c->Vlag), bp<double>(c->L));
+  else
+    launch_cell_stage_2d(c->stream, c->P, c->S, stage_args(c), stage);  // cell-indexed beyond the geometry
+}
+
 int ep_stage(mhg_ctx* c, int stage) {
   int r = ep_prepare(c);
   if (r) return r;
-  launch_cell_stage_2d(c->stream, c->P, c->S, stage_args(c), stage);
+  enqueue_stage(c, stage);
   return ep_finish(c);
 }
 
 int ep_node(mhg_ctx* c, bool vel, bool bc, bool pos) {
   int r = ep_prepare(c);
   if (r) return r;
-  launch_node_lagrange_2d(c->stream, c->P, c->S, node_lag_args<double>(c), vel, bc, pos);
+  if (c->P.dim == 3) {
+    NodeLag3ArgsT<double> N;
+    N.b = node_lag_args<double>(c);
+    N.xpz = c->xz[c->xcur];
+    N.uz = bp<double>(c->uz[c->ucur]);
+    N.xnz = c->xz[1 - c->xcur];
+    launch_node_lagrange_3d(c->stream, c->P, c->S, N, vel, bc, pos);
+  } else {
+    launch_node_lagrange_2d(c->stream, c->P, c->S, node_lag_args<double>(c), vel, bc, pos);
+  }
   return ep_finish(c);
+}
+
+// applyEOS of every environment, then computePressionMoyenne (the end of
+// remap(), PrepareRemap.cc:552-559, and of updateEnergyAndPressure without
+// projection, MahycoModule.cc:1468-1476).
+void enqueue_eos_and_average(mhg_ctx* c) {
+  for (int k = 0; k < c->P.ne; ++k) launch_eos_env(c->stream, c->P, c->S, eos_args(c), k, EOS_APPLY, 0, c->P.nc);
+  launch_pression_moyenne(c->stream, c->P, c->S, pm_args(c));
+}
+
+// The arguments of the remap services: the dimension of the mesh, the dual
+// projection (always on: sans-lagrange is refused), nb_vars_to_project =
+// 3 nb_env + 5 (volumes, masses, energies, momentum, kinetic energy, pseudo;
+// MahycoModule's resizeRemapVariables call) and nb_env.
+int check_remap_args(mhg_ctx* c, int32_t dim, int32_t with_dual, int32_t nb_vars, int32_t nb_env) {
+  if (dim != c->P.dim || with_dual != 1 || nb_env != c->P.ne || nb_vars != 3 * c->P.ne + 5)
+    return fail(c, MHG_E_ARG, "remap: dimension, dual projection, variable or environment count "
+                              "differs from the context's");
+  if (c->P.lagonly) return fail(c, MHG_E_STATE, "remap: the context runs without projection");
+  return MHG_OK;
+}
+
+int check_env(mhg_ctx* c, int32_t env) {
+  if (env < 0 || env >= c->P.ne) return fail(c, MHG_E_ARG, "environment index out of range");
+  return MHG_OK;
+}
+
+void enqueue_remap_fp64(mhg_ctx* c, bool eos) {
+  if (c->P.dim == 3) remap_3d<double>(c, 0, eos);
+  else remap_2d<double>(c, 0, eos);
 }
 
 }  // namespace
@@ -1092,28 +1177,172 @@ int mhg_compute_geometric_values(mhg_ctx* c) {
 }
 int mhg_update_density(mhg_ctx* c) { return ep_stage(c, STAGE_DENSITY); }
 int mhg_compute_artificial_viscosity(mhg_ctx* c) { return ep_stage(c, STAGE_VISCOSITY); }
-int mhg_update_energy_and_pressure(mhg_ctx* c) { return ep_stage(c, STAGE_ENERGY); }
+
+// updateEnergyAndPressure (MahycoModule.cc:1439-1478): the energy update;
+// without projection also the EOS of every environment and the pressure
+// average.
+int mhg_update_energy_and_pressure(mhg_ctx* c) {
+  int r = ep_prepare(c);
+  if (r) return r;
+  enqueue_stage(c, STAGE_ENERGY);
+  if (c->P.lagonly) enqueue_eos_and_average(c);
+  return ep_finish(c);
+}
 
 // remap (PrepareRemap.cc:448-563): U from the state, the sweeps, the final
-// cell phase with the EOS and the pressure average; dt is left to
-// mhg_compute_delta_t.
+// cell phase with the EOS and the pressure average, fused; dt is left to
+// mhg_compute_delta_t.  Without projection it does nothing but end the
+// iteration's Lagrange phase (the mesh stays where it moved).
 int mhg_remap(mhg_ctx* c) {
   if (c && c->coord_phase != 2) return fail(c, MHG_E_STATE, "remap: the Lagrange phase of this iteration has not run");
   int r = ep_prepare(c);
   if (r) return r;
-  launch_cell_stage_2d(c->stream, c->P, c->S, stage_args(c), STAGE_BUILD_U);
-  remap_2d<double>(c, 0);
-  r = ep_finish(c);
-  if (r) return r;
+  if (c->P.lagonly) {
+    c->xcur = 1 - c->xcur;
+  } else {
+    launch_cell_stage_2d(c->stream, c->P, c->S, stage_args(c), STAGE_BUILD_U);
+    enqueue_remap_fp64(c, true);
+    r = ep_finish(c);
+    if (r) return r;
+  }
   c->coord_phase = 0;
+  c->remap_stage = 0;
   c->ep_remapped = true;
   return MHG_OK;
 }
 
+// ---- the remap split as the reference's services call it ---------------
+// computeVariablesForRemap + computeFaceQuantitesForRemap
+// (PrepareRemap.cc:15-437): the remap vector of every cell; the face
+// quantities are formed inside the sweeps from the node data.
+int mhg_remap_prepare(mhg_ctx* c) {
+  if (c && c->coord_phase != 2)
+    return fail(c, MHG_E_STATE, "remap_prepare: the Lagrange phase of this iteration has not run");
+  int r = ep_prepare(c);
+  if (r) return r;
+  if (c->P.lagonly) return fail(c, MHG_E_STATE, "remap_prepare: the context runs without projection");
+  launch_cell_stage_2d(c->stream, c->P, c->S, stage_args(c), STAGE_BUILD_U);
+  r = ep_finish(c);
+  if (r) return r;
+  c->remap_stage = 1;
+  return MHG_OK;
+}
+
+// IRemap::appliRemap (RemapADIService.cc:19-60): the cell and dual sweeps in
+// the SensProjection order, then remapVariables.  Pressures and sound speeds
+// are left to the EOS and pressure-average calls.
+int mhg_remap_appli(mhg_ctx* c, int32_t dimension, int32_t with_dual_projection, int32_t nb_vars_to_project,
+                    int32_t nb_env) {
+  int r = ep_prepare(c);
+  if (r) return r;
+  r = check_remap_args(c, dimension, with_dual_projection, nb_vars_to_project, nb_env);
+  if (r) return r;
+  if (c->remap_stage != 1) return fail(c, MHG_E_STATE, "remap_appli: call remap_prepare first");
+  enqueue_remap_fp64(c, false);
+  r = ep_finish(c);
+  if (r) return r;
+  c->coord_phase = 0;
+  c->remap_stage = 0;
+  c->ep_remapped = true;
+  return MHG_OK;
+}
+
+// IRemap::remapVariables (RemapADIFinal.cc:19-778) from the remap vectors of
+// the last appliRemap: membership, "moy", negative-value resets and the
+// total-energy fix-up.  appliRemap already ends with it; a second call
+// derives the same state from the same vectors.
+int mhg_remap_variables(mhg_ctx* c, int32_t dimension, int32_t with_dual_projection, int32_t nb_vars_to_project,
+                        int32_t nb_env) {
+  int r = ep_prepare(c);
+  if (r) return r;
+  r = check_remap_args(c, dimension, with_dual_projection, nb_vars_to_project, nb_env);
+  if (r) return r;
+  if (!c->ep_remapped) return fail(c, MHG_E_STATE, "remap_variables: no remap vectors of this iteration");
+  if (c->P.dim == 3) launch_cell_final_3d(c->stream, c->P, c->S, cell_final3_args<double>(c, c->ucur), 0, false);
+  else launch_cell_final_2d(c->stream, c->P, c->S, cell_final_args<double>(c), 0, false);
+  return ep_finish(c);
+}
+
+// IRemap::resizeRemapVariables (RemapADIService.cc:63-90): the context sized
+// its remap arrays at creation; the call checks the counts.
+int mhg_remap_resize_variables(mhg_ctx* c, int32_t nb_vars_to_project, int32_t nb_env) {
+  if (!c) return MHG_E_ARG;
+  if (nb_env != c->P.ne || nb_vars_to_project != 3 * c->P.ne + 5)
+    return fail(c, MHG_E_ARG, "resize_remap_variables: counts differ from the context's");
+  return MHG_OK;
+}
+
+// IRemap::synchronizeUremap (RemapADIService.cc:1050-1057): ghost-cell
+// synchronisation; a context holds the whole mesh on one device, so there
+// are no ghosts.
+int mhg_remap_synchronize_uremap(mhg_ctx* c) {
+  if (!c) return MHG_E_ARG;
+  return check_ready(c);
+}
+
+// ---- IEquationOfState (mahyco/src/eos/IEquationOfState.h:45-63) --------
+int mhg_eos_init(mhg_ctx* c, int32_t env) {
+  int r = ep_prepare(c);
+  if (r) return r;
+  r = check_env(c, env);
+  if (r) return r;
+  launch_eos_env(c->stream, c->P, c->S, eos_args(c), env, EOS_INIT, 0, c->P.nc);
+  return ep_finish(c);
+}
+
+int mhg_eos_apply(mhg_ctx* c, int32_t env) {
+  int r = ep_prepare(c);
+  if (r) return r;
+  r = check_env(c, env);
+  if (r) return r;
+  launch_eos_env(c->stream, c->P, c->S, eos_args(c), env, EOS_APPLY, 0, c->P.nc);
+  return ep_finish(c);
+}
+
+int mhg_eos_apply_one_cell(mhg_ctx* c, int32_t env, int64_t cell) {
+  int r = ep_prepare(c);
+  if (r) return r;
+  r = check_env(c, env);
+  if (r) return r;
+  if (cell < 0 || cell >= c->P.nc) return fail(c, MHG_E_ARG, "eos_apply_one_cell: cell out of range");
+  launch_eos_env(c->stream, c->P, c->S, eos_args(c), env, EOS_APPLY, cell, cell + 1);
+  return ep_finish(c);
+}
+
+int mhg_eos_get_adiabatic_cst(const mhg_ctx* c, int32_t env, double* value) {
+  if (!c || !value || env < 0 || env >= c->P.ne) return MHG_E_ARG;
+  *value = c->opt.env[env].adiabatic_cst;
+  return MHG_OK;
+}
+
+int mhg_eos_get_tension_limit_cst(const mhg_ctx* c, int32_t env, double* value) {
+  if (!c || !value || env < 0 || env >= c->P.ne) return MHG_E_ARG;
+  *value = c->opt.env[env].limit_tension;
+  return MHG_OK;
+}
+
+// computePressionMoyenne (MahycoModule.cc:1836-1915).
+int mhg_compute_pression_moyenne(mhg_ctx* c) {
+  int r = ep_prepare(c);
+  if (r) return r;
+  launch_pression_moyenne(c->stream, c->P, c->S, pm_args(c));
+  return ep_finish(c);
+}
+
+// computeDeltaT (MahycoModule.cc:2027-2191): the CFL candidates of the
+// current state, then dt, the deltat-min check and the stop test.
 int mhg_compute_delta_t(mhg_ctx* c, double* dt, int32_t* stop) {
   int r = ep_prepare(c);
   if (r) return r;
   if (!c->ep_remapped) return fail(c, MHG_E_STATE, "compute_delta_t: call remap first");
+  CflArgs A;
+  A.L = bp<double>(c->L);
+  A.cs = bp<double>(c->cs);
+  A.ux = bp<double>(c->ux[c->ucur]);
+  A.uy = bp<double>(c->uy[c->ucur]);
+  A.uz = c->P.dim == 3 ? bp<double>(c->uz[c->ucur]) : nullptr;
+  A.block_min = c->block_min;
+  launch_cfl(c->stream, c->P, c->S, A);
   launch_dt_finalize(c->stream, c->P, c->S, c->block_min, false);
   r = ep_finish(c);
   if (r) return r;
@@ -1131,9 +1360,34 @@ int mhg_remap_has_projection_pente_borne(const mhg_ctx* c) { return c ? c->opt.p
 int mhg_remap_has_conservation_energie_totale(const mhg_ctx* c) { return c ? c->opt.conservation_energie_totale : MHG_E_ARG; }
 int mhg_remap_is_euler(const mhg_ctx* c) { return c ? c->opt.is_euler_scheme : MHG_E_ARG; }
 
+namespace {
+// One iteration of a Lagrange-only context (with-projection = false)
+// through its entry-point kernels, in the order of Mahyco.config:30-41.
+int step_entry_points(mhg_ctx* c, int32_t* stop) {
+  int r = read_scalars(c);
+  if (r) return r;
+  if (c->S_host->stop || c->S_host->err) {  // the run has ended: nothing runs
+    if (stop) *stop = c->S_host->stop;
+    return c->S_host->err ? fail(c, MHG_E_DT_MIN, "dt below deltat-min") : MHG_STOP;
+  }
+  int (*const calls[])(mhg_ctx*) = {mhg_begin_iteration, mhg_save_values_at_n, mhg_update_velocity,
+                                    mhg_apply_boundary_condition, mhg_update_position,
+                                    mhg_compute_geometric_values, mhg_update_density,
+                                    mhg_compute_artificial_viscosity, mhg_update_energy_and_pressure, mhg_remap};
+  for (auto f : calls)
+    if ((r = f(c))) return r;
+  int32_t st = 0;
+  r = mhg_compute_delta_t(c, nullptr, &st);
+  if (stop) *stop = st;
+  if (r) return r;
+  return st ? MHG_STOP : MHG_OK;
+}
+}  // namespace
+
 int mhg_step(mhg_ctx* c, int32_t* stop) {
   int r = check_fused(c);
   if (r) return r;
+  if (c->P.lagonly) return step_entry_points(c, stop);
   r = read_scalars(c);
   if (r) return r;
   const int it0 = c->S_host->iteration;
@@ -1152,6 +1406,7 @@ int mhg_run_async(mhg_ctx* c, int32_t n) {
   int r = check_fused(c);
   if (r) return r;
   if (n < 0) return fail(c, MHG_E_ARG, "run_async: negative step count");
+  if (c->P.lagonly) return mhg_run(c, n, nullptr) < 0 ? MHG_E_HIP : MHG_OK;  // host-synchronous
   // check_fused has reconciled earlier iterations: the device count is the
   // host's starting point for these
   r = read_scalars(c);
@@ -1172,6 +1427,17 @@ int mhg_synchronize(mhg_ctx* c) {
 int mhg_run(mhg_ctx* c, int32_t max_steps, int32_t* steps_done) {
   int r = check_fused(c);
   if (r) return r;
+  if (c->P.lagonly) {
+    r = read_scalars(c);
+    if (r) return r;
+    const int it0 = c->S_host->iteration;
+    int rs = MHG_OK;
+    for (int s = 0; s < max_steps && rs == MHG_OK; ++s) rs = step_entry_points(c, nullptr);
+    r = read_scalars(c);
+    if (r) return r;
+    if (steps_done) *steps_done = c->S_host->iteration - it0;
+    return rs;
+  }
   r = read_scalars(c);
   if (r) return r;
   const int it0 = c->S_host->iteration;

@@ -136,6 +136,38 @@ enum CellStage {
   STAGE_BUILD_U = 5     // computeVariablesForRemap (cells)
 };
 void launch_cell_stage_2d(hipStream_t st, const Params& P, DevScalars* S, const StageArgs& A, int stage);
+// computeGeometricValues of hexahedra (MahycoModule.cc:1054-1299): V and L
+// from the coordinates of this iteration.
+void launch_cell_geometry_3d(hipStream_t st, const Params& P, DevScalars* S, const double* xnx,
+                             const double* xny, const double* xnz, double* Vlag, double* L);
+
+// IEquationOfState entry points (per environment) and computePressionMoyenne.
+struct EosArgs {
+  double *rho, *e, *p, *cs;
+  const int* mask;
+  double *env_rho, *env_e, *env_p, *env_cs;
+};
+enum EosMode { EOS_APPLY = 0, EOS_INIT = 1 };
+// applyEOS (p, c from rho, e) or initEOS (e, c from rho, p) of environment k
+// on the cells [c0, c1) that hold it: the cell arrays for a cell holding k
+// alone, the environment arrays for a cell holding several.
+void launch_eos_env(hipStream_t st, const Params& P, DevScalars* S, const EosArgs& A, int k, int mode, long c0,
+                    long c1);
+struct PmArgs {
+  double *p, *cs;
+  const int* mask;
+  const double *env_f, *env_p, *env_cs;
+};
+void launch_pression_moyenne(hipStream_t st, const Params& P, DevScalars* S, const PmArgs& A);
+// CFL candidates of computeHydroDeltaT (MahycoModule.cc:2027-2071), one
+// minimum per block of 256 cells into block_min (cell_final_blocks slots):
+// L / (c + max over the cell's nodes of |u|), without the velocity term in
+// the Lagrange-only mode.
+struct CflArgs {
+  const double *L, *cs, *ux, *uy, *uz;
+  double* block_min;
+};
+void launch_cfl(hipStream_t st, const Params& P, DevScalars* S, const CflArgs& A);
 void launch_dt_finalize(hipStream_t st, const Params& P, DevScalars* S, const double* block_min, bool advance);
 
 int cell_final_blocks(const Params& P);
@@ -160,9 +192,11 @@ const char* node_sweep_variant_name(int dir, int v);
 // dt_mode 0: no dt kernel (the caller computes dt later); 1: the final dt
 // kernel follows; 2: it also applies time += dt, iteration += 1 of the
 // iteration (fused step without a separate begin-iteration launch)
+// eos = false: remapVariables only (no EOS, pressure average or CFL
+// candidates; dt_mode must be 0).
 template <class R>
 void launch_cell_final_2d(hipStream_t st, const Params& P, DevScalars* S, const CellFinalArgsT<R>& A,
-                          int dt_mode);
+                          int dt_mode, bool eos = true);
 
 // 3D fused step.
 template <class R>
@@ -177,6 +211,7 @@ template <class R>
 void launch_node_sweep_3d(hipStream_t st, const Params& P, DevScalars* S, const NodeSweep3ArgsT<R>& A, int dir,
                           bool first, bool last);
 template <class R>
-void launch_cell_final_3d(hipStream_t st, const Params& P, DevScalars* S, const CellFinal3ArgsT<R>& A, int dt_mode);
+void launch_cell_final_3d(hipStream_t st, const Params& P, DevScalars* S, const CellFinal3ArgsT<R>& A, int dt_mode,
+                          bool eos = true);
 
 }  // namespace mhg

@@ -61,9 +61,9 @@ __global__ __launch_bounds__(1024) void k_cell_sweep_ytile_2d(Params P, DevScala
   cell_sweep_ytile_2d_body<R, NEM, FIRST, LIM, false, LEARN>(P, S, A, xcd_block(blockIdx.x, gridDim.x));
 }
 
-template <class R, int BLOCK, int NEM>
+template <class R, int BLOCK, int NEM, bool EOS = true>
 __global__ __launch_bounds__(BLOCK) void k_cell_final_2d(Params P, DevScalars* S, CellFinalArgsT<R> A) {
-  cell_final_2d_body<R, BLOCK, NEM>(P, S, A, xcd_block(blockIdx.x, gridDim.x), A.block_min);
+  cell_final_2d_body<R, BLOCK, NEM, EOS>(P, S, A, xcd_block(blockIdx.x, gridDim.x), A.block_min);
 }
 
 template <int BLOCK, bool ADVANCE>
@@ -950,6 +950,114 @@ void launch_cell_stage_2d(hipStream_t st, const Params& P, DevScalars* S, const 
 #undef CS_
 }
 
+// ---------------------------------------------------------------------------
+// Entry-point kernels of the EOS services and of computePressionMoyenne.
+// ---------------------------------------------------------------------------
+// applyEOS / initEOS of environment k (PerfectGasEOSService.cc:20-137,
+// StiffenedGasEOSService.cc:10-59) on the cells [c0, c1) holding it.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_eos_env(Params P, DevScalars* S, EosArgs A, int k, long c0, long c1) {
+  if (S->stop || S->err) return;
+  const long c = c0 + (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= c1) return;
+  const int mask = A.mask[c];
+  if (!((mask >> k) & 1)) return;
+  const bool pure = __popc(mask) == 1;
+  const long ek = (long)k * P.nc + c;
+  double* rho = pure ? A.rho + c : A.env_rho + ek;
+  double* e = pure ? A.e + c : A.env_e + ek;
+  double* p = pure ? A.p + c : A.env_p + ek;
+  double* cs = pure ? A.cs + c : A.env_cs + ek;
+  if (MODE == EOS_APPLY) {
+    double pv, cv;
+    eos_apply(P.eos, k, *rho, *e, &pv, &cv);
+    *p = pv;
+    *cs = cv;
+  } else {
+    const double g = P.eos.gamma[k], r = *rho, pr = *p;
+    if (P.eos.kind[k] == MHG_EOS_STIFFENED_GAS) {
+      const double t = P.eos.tension[k];
+      *e = (pr + (g * t)) / ((g - 1.) * r);
+      *cs = sqrt((g / r) * (pr + t));
+    } else {
+      *e = pr / ((g - 1.) * r);
+      *cs = sqrt(g * pr / r);
+    }
+  }
+}
+
+// computePressionMoyenne (MahycoModule.cc:1836-1915): a cell holding
+// several environments takes p = sum_k f_k p_k and c = max(1e-20, max_k c_k)
+// in ascending k; a cell holding none p = 0, c = 1e-20; pure cells keep
+// their values.
+template <int NEM>
+__global__ __launch_bounds__(256) void k_pression_moyenne(Params P, DevScalars* S, PmArgs A) {
+  if (S->stop || S->err) return;
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= P.nc) return;
+  const int mask = A.mask[c];
+  if (__popc(mask) == 1) return;
+  double p = 0., cs = 1.e-20;
+#pragma unroll
+  for (int k = 0; k < NEM; ++k) {
+    if (!((mask >> k) & 1)) continue;
+    const long ek = (long)k * P.nc + c;
+    p += A.env_f[ek] * A.env_p[ek];
+    cs = smax(A.env_cs[ek], cs);
+  }
+  A.p[c] = p;
+  A.cs[c] = cs;
+}
+
+// CFL candidates of quads: the same arithmetic as k_cell_final_2d's.
+__global__ __launch_bounds__(256) void k_cfl_2d(Params P, DevScalars* S, CflArgs A) {
+  if (S->stop || S->err) return;
+  const long c = (long)blockIdx.x * 256 + threadIdx.x;
+  double dx_sound = 1.7976931348623157e308;
+  if (c < P.nc) {
+    double vmax = 0.;
+    if (!P.lagonly) {
+      const int nnx = P.nx + 1;
+      const int i = (int)(c % P.nx), j = (int)(c / P.nx);
+      const long n0 = i + (long)nnx * j;
+      const long nd[4] = {n0, n0 + 1, n0 + 1 + nnx, n0 + nnx};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const double ux = A.ux[nd[q]], uy = A.uy[nd[q]];
+        const double un = sqrt(ux * ux + uy * uy + 0. * 0.);
+        vmax = (un < vmax) ? vmax : un;
+      }
+    }
+    dx_sound = smin(dx_sound, A.L[c] / (A.cs[c] + vmax));
+  }
+  block_min_store<256>(dx_sound, A.block_min + blockIdx.x);
+}
+
+void launch_eos_env(hipStream_t st, const Params& P, DevScalars* S, const EosArgs& A, int k, int mode, long c0,
+                    long c1) {
+  if (c1 <= c0) return;
+  dim3 g(nblocks(c1 - c0, 256)), b(256);
+  if (mode == EOS_INIT) hipLaunchKernelGGL((k_eos_env<EOS_INIT>), g, b, 0, st, P, S, A, k, c0, c1);
+  else hipLaunchKernelGGL((k_eos_env<EOS_APPLY>), g, b, 0, st, P, S, A, k, c0, c1);
+}
+
+void launch_pression_moyenne(hipStream_t st, const Params& P, DevScalars* S, const PmArgs& A) {
+  dim3 g(nblocks(P.nc, 256)), b(256);
+#define PM_(N) hipLaunchKernelGGL((k_pression_moyenne<N>), g, b, 0, st, P, S, A)
+  MHG_NEM_DISPATCH(P.ne, PM_);
+#undef PM_
+}
+
+void launch_cfl_3d(hipStream_t st, const Params& P, DevScalars* S, const CflArgs& A);
+
+void launch_cfl(hipStream_t st, const Params& P, DevScalars* S, const CflArgs& A) {
+  if (P.dim == 3) {
+    launch_cfl_3d(st, P, S, A);
+    return;
+  }
+  hipLaunchKernelGGL(k_cfl_2d, dim3(cell_final_blocks(P)), dim3(256), 0, st, P, S, A);
+}
+
 void launch_dt_finalize(hipStream_t st, const Params& P, DevScalars* S, const double* block_min, bool advance) {
   const int nb = cell_final_blocks(P);
   if (advance) hipLaunchKernelGGL((k_dt_finalize<1024, true>), dim3(1), dim3(1024), 0, st, P, S, block_min, nb);
@@ -958,11 +1066,14 @@ void launch_dt_finalize(hipStream_t st, const Params& P, DevScalars* S, const do
 
 template <class R>
 void launch_cell_final_2d(hipStream_t st, const Params& P, DevScalars* S, const CellFinalArgsT<R>& A,
-                          int dt_mode) {
+                          int dt_mode, bool eos) {
   constexpr int B = 256;
   dim3 g(nblocks(P.nc, B)), b(B);
 #define F_(N) hipLaunchKernelGGL((k_cell_final_2d<R, B, N>), g, b, 0, st, P, S, A)
-  MHG_NEM_DISPATCH(P.ne, F_);
+#define FN_(N) hipLaunchKernelGGL((k_cell_final_2d<R, B, N, false>), g, b, 0, st, P, S, A)
+  if (eos) MHG_NEM_DISPATCH(P.ne, F_);
+  else MHG_NEM_DISPATCH(P.ne, FN_);
+#undef FN_
 #undef F_
   if (dt_mode > 0) launch_dt_finalize(st, P, S, A.block_min, dt_mode == 2);
 }
@@ -978,7 +1089,7 @@ void launch_cell_final_2d(hipStream_t st, const Params& P, DevScalars* S, const 
   template void launch_node_sweep_2d<R>(hipStream_t, const Params&, DevScalars*,                     \
                                         const NodeSweepArgsT<R>&, int, bool, int);                   \
   template void launch_cell_final_2d<R>(hipStream_t, const Params&, DevScalars*,                     \
-                                        const CellFinalArgsT<R>&, int);
+                                        const CellFinalArgsT<R>&, int, bool);
 MHG_INSTANTIATE(double)
 MHG_INSTANTIATE(float)
 #undef MHG_INSTANTIATE

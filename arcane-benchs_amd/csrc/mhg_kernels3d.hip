@@ -103,9 +103,14 @@ __global__ __launch_bounds__(256) void k_node_lagrange_3d(Params P, DevScalars* 
     st_stream(&A.uz[nid], (R)vz);
   }
   if (POS) {
-    st_stream(&A.b.xnx[nid], x0_3d(P, 0, i) + dt * vx);
-    st_stream(&A.b.xny[nid], x0_3d(P, 1, j) + dt * vy);
-    st_stream(&A.xnz[nid], x0_3d(P, 2, k) + dt * vz);
+    // Euler scheme: from x0; Lagrange-only (with-projection = false): from
+    // the previous coordinates
+    const double bx = P.lagonly ? A.b.xpx[nid] : x0_3d(P, 0, i);
+    const double by = P.lagonly ? A.b.xpy[nid] : x0_3d(P, 1, j);
+    const double bz = P.lagonly ? A.xpz[nid] : x0_3d(P, 2, k);
+    st_stream(&A.b.xnx[nid], bx + dt * vx);
+    st_stream(&A.b.xny[nid], by + dt * vy);
+    st_stream(&A.xnz[nid], bz + dt * vz);
   }
 }
 
@@ -980,11 +985,12 @@ __global__ __launch_bounds__(1024) void k_node_sweep_tile_3d(Params P, DevScalar
 // ---------------------------------------------------------------------------
 // Final cell phase with the stored Euler volume; CFL over the 8 nodes.
 // ---------------------------------------------------------------------------
-template <class R, int BLOCK, int NEM>
+template <class R, int BLOCK, int NEM, bool EOS = true>
 __global__ __launch_bounds__(BLOCK) void k_cell_final_3d(Params P, DevScalars* S, CellFinal3ArgsT<R> A) {
   if (S->stop || S->err) return;
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   double dx_sound = 1.7976931348623157e308;
+  if (!EOS && c >= P.nc) return;  // remapVariables only: no CFL candidate
   if (c < P.nc) {
     const int i = (int)(c % P.nx), j = (int)((c / P.nx) % P.ny), k = (int)(c / ((long)P.nx * P.ny));
     // total-energy fix-up: the deposit, the 8 nodes' terms in cell node order
@@ -999,7 +1005,8 @@ __global__ __launch_bounds__(BLOCK) void k_cell_final_3d(Params P, DevScalars* S
       }
       delta_ec = smax(R(0), sum);
     }
-    const R c_g = final_cell<R, NEM>(P, A.b, c, A.Veuler[c], delta_ec);
+    const R c_g = final_cell<R, NEM, EOS>(P, A.b, c, A.Veuler[c], delta_ec);
+    if (!EOS) return;
     R vmax = R(0);
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
@@ -1014,6 +1021,49 @@ __global__ __launch_bounds__(BLOCK) void k_cell_final_3d(Params P, DevScalars* S
     dx_sound = smin(dx_sound, (double)qdiv(A.b.L[c], c_g + vmax));
   }
   block_min_store<BLOCK>(dx_sound, A.b.block_min + blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Entry-point kernels of hexahedra: computeGeometricValues (V, L) and the
+// CFL candidates, the same arithmetic as k_cell_lagrange_3d's geometry and
+// k_cell_final_3d's candidates.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_cell_geometry_3d(Params P, DevScalars* S, const double* xnx,
+                                                          const double* xny, const double* xnz, double* Vlag,
+                                                          double* L) {
+  if (S->stop || S->err) return;
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= P.nc) return;
+  const int i = (int)(c % P.nx), j = (int)((c / P.nx) % P.ny), k = (int)(c / ((long)P.nx * P.ny));
+  V3 cn[8];
+  hex_nodes(P, xnx, xny, xnz, i, j, k, cn);
+  const double Vd = hex_volume(cn);
+  Vlag[c] = Vd;
+  L[c] = (P.lc == MHG_LC_FACES_OPPOSEES) ? hex_length_faces_opposees(cn) : pow(Vd, 1. / 3.);
+}
+
+__global__ __launch_bounds__(256) void k_cfl_3d(Params P, DevScalars* S, CflArgs A) {
+  if (S->stop || S->err) return;
+  const long c = (long)blockIdx.x * 256 + threadIdx.x;
+  double dx_sound = 1.7976931348623157e308;
+  if (c < P.nc) {
+    double vmax = 0.;
+    if (!P.lagonly) {
+      const int i = (int)(c % P.nx), j = (int)((c / P.nx) % P.ny), k = (int)(c / ((long)P.nx * P.ny));
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int di = (q == 1 || q == 2 || q == 5 || q == 6) ? 1 : 0;
+        const int dj = (q == 2 || q == 3 || q == 6 || q == 7) ? 1 : 0;
+        const int dk = q >= 4 ? 1 : 0;
+        const long n = nid3(P, i + di, j + dj, k + dk);
+        const double ux = A.ux[n], uy = A.uy[n], uz = A.uz[n];
+        const double un = sqrt(ux * ux + uy * uy + uz * uz);
+        vmax = (un < vmax) ? vmax : un;
+      }
+    }
+    dx_sound = smin(dx_sound, A.L[c] / (A.cs[c] + vmax));
+  }
+  block_min_store<256>(dx_sound, A.block_min + blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1145,13 +1195,26 @@ void launch_node_sweep_3d(hipStream_t st, const Params& P, DevScalars* S, const 
 }
 
 template <class R>
-void launch_cell_final_3d(hipStream_t st, const Params& P, DevScalars* S, const CellFinal3ArgsT<R>& A, int dt_mode) {
+void launch_cell_final_3d(hipStream_t st, const Params& P, DevScalars* S, const CellFinal3ArgsT<R>& A, int dt_mode,
+                          bool eos) {
   constexpr int B = 256;
   dim3 g(nblocks3(P.nc, B)), b(B);
 #define F_(N) hipLaunchKernelGGL((k_cell_final_3d<R, B, N>), g, b, 0, st, P, S, A)
-  MHG3_NEM(P.ne, F_);
+#define FN_(N) hipLaunchKernelGGL((k_cell_final_3d<R, B, N, false>), g, b, 0, st, P, S, A)
+  if (eos) MHG3_NEM(P.ne, F_);
+  else MHG3_NEM(P.ne, FN_);
+#undef FN_
 #undef F_
   if (dt_mode > 0) launch_dt_finalize(st, P, S, A.b.block_min, dt_mode == 2);
+}
+
+void launch_cell_geometry_3d(hipStream_t st, const Params& P, DevScalars* S, const double* xnx,
+                             const double* xny, const double* xnz, double* Vlag, double* L) {
+  hipLaunchKernelGGL(k_cell_geometry_3d, dim3(nblocks3(P.nc, 256)), dim3(256), 0, st, P, S, xnx, xny, xnz, Vlag, L);
+}
+
+void launch_cfl_3d(hipStream_t st, const Params& P, DevScalars* S, const CflArgs& A) {
+  hipLaunchKernelGGL(k_cfl_3d, dim3(cell_final_blocks(P)), dim3(256), 0, st, P, S, A);
 }
 
 #define MHG3_INSTANTIATE(R)                                                                                   \
@@ -1163,7 +1226,7 @@ void launch_cell_final_3d(hipStream_t st, const Params& P, DevScalars* S, const 
   template void launch_node_sweep_3d<R>(hipStream_t, const Params&, DevScalars*, const NodeSweep3ArgsT<R>&,   \
                                         int, bool, bool);                                                    \
   template void launch_cell_final_3d<R>(hipStream_t, const Params&, DevScalars*, const CellFinal3ArgsT<R>&,   \
-                                        int);
+                                        int, bool);
 MHG3_INSTANTIATE(double)
 MHG3_INSTANTIATE(float)
 #undef MHG3_INSTANTIATE

@@ -180,7 +180,10 @@ __device__ inline void lagrange_cell_bulk(const Params& P, const DevScalars* S, 
 
 // Final cell phase of cell c with Euler volume vold: membership, "moy", EOS,
 // pressure average; returns the cell's sound speed for the CFL candidate.
-template <class R, int NEM>
+// EOS = false stops after remapVariables (RemapADIFinal.cc:19-778): the
+// pressures and sound speeds are left to the EOS and pressure-average
+// entry points (PrepareRemap.cc:552-559), and the return value is 0.
+template <class R, int NEM, bool EOS = true>
 __device__ inline R final_cell(const Params& P, const CellFinalArgsT<R>& A, long c, double vold,
                                R delta_ec = R(0)) {
   {
@@ -342,9 +345,11 @@ __device__ inline R final_cell(const Params& P, const CellFinalArgsT<R>& A, long
       }
       // total-energy fix-up (RemapADIFinal.cc:751-776), before the EOS
       if (P.etot) e_g += mfp * delta_ec;
+      if (EOS) {
 #pragma unroll
-      for (int k = 0; k < NEM; ++k)
-        if (k == kpure) eos_apply(P.eos, k, rho_g, e_g, &p_g, &c_g);
+        for (int k = 0; k < NEM; ++k)
+          if (k == kpure) eos_apply(P.eos, k, rho_g, e_g, &p_g, &c_g);
+      }
     } else {
 #pragma unroll
       for (int k = 0; k < NEM; ++k) {
@@ -369,15 +374,19 @@ __device__ inline R final_cell(const Params& P, const CellFinalArgsT<R>& A, long
 #pragma unroll
       for (int k = 0; k < NEM; ++k) {
         if (!((mask >> k) & 1)) continue;
-        eos_apply(P.eos, k, dk[k], ek[k], &pk[k], &ck[k]);
-        p_g += fv[k] * pk[k];
-        c_g = smax(ck[k], c_g);
+        if (EOS) {
+          eos_apply(P.eos, k, dk[k], ek[k], &pk[k], &ck[k]);
+          p_g += fv[k] * pk[k];
+          c_g = smax(ck[k], c_g);
+        }
         if (nbe >= 2) {
           const long e = k * NC + c;
           A.env_rho[e] = dk[k];
           A.env_e[e] = ek[k];
-          A.env_p[e] = pk[k];
-          A.env_cs[e] = ck[k];
+          if (EOS) {
+            A.env_p[e] = pk[k];
+            A.env_cs[e] = ck[k];
+          }
           A.env_q[e] = pv;
           A.env_m[e] = mk[k];
           A.env_f[e] = fv[k];
@@ -387,12 +396,14 @@ __device__ inline R final_cell(const Params& P, const CellFinalArgsT<R>& A, long
     }
     st_stream(&A.rho[c], rho_g);
     st_stream(&A.e[c], e_g);
-    st_stream(&A.p[c], p_g);
-    st_stream(&A.cs[c], c_g);
+    if (EOS) {
+      st_stream(&A.p[c], p_g);
+      st_stream(&A.cs[c], c_g);
+    }
     st_stream(&A.q[c], q_g);
     st_stream(&A.m[c], mass_g);
     st_stream(&A.mask[c], mask);
-    return c_g;
+    return EOS ? c_g : R(0);
   }
 }
 

@@ -160,7 +160,18 @@ class Library:
                 ("mlp_forward", C.c_int, [vp, _FP, C.c_int64, _FP]),
                 ("mlp_set_data", C.c_int, [vp, _FP, _FP, C.c_int64]),
                 ("mlp_gradient", C.c_int, [vp, C.c_int64, C.c_int64, _FP, _FP]),
-                ("mlp_train", C.c_int, [vp, C.c_int32, C.c_int32, C.c_float, C.c_uint64, _FP])]:
+                ("mlp_train", C.c_int, [vp, C.c_int32, C.c_int32, C.c_float, C.c_uint64, _FP]),
+                ("remap_prepare", C.c_int, [vp]),
+                ("remap_appli", C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+                ("remap_variables", C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+                ("remap_resize_variables", C.c_int, [vp, C.c_int32, C.c_int32]),
+                ("remap_synchronize_uremap", C.c_int, [vp]),
+                ("eos_init", C.c_int, [vp, C.c_int32]),
+                ("eos_apply", C.c_int, [vp, C.c_int32]),
+                ("eos_apply_one_cell", C.c_int, [vp, C.c_int32, C.c_int64]),
+                ("eos_get_adiabatic_cst", C.c_int, [vp, C.c_int32, _DP]),
+                ("eos_get_tension_limit_cst", C.c_int, [vp, C.c_int32, _DP]),
+                ("compute_pression_moyenne", C.c_int, [vp])]:
             if hasattr(L, p + name):
                 self.optional[name] = fn(name, res, args)
 
@@ -323,6 +334,51 @@ class Simulation:
         stop = C.c_int32()
         self._check(self.L.compute_delta_t(self.ctx, C.byref(dt), C.byref(stop)), "compute_delta_t")
         return dt.value, bool(stop.value)
+
+    # -- the remap and EOS services, called as MahycoModule calls them --
+    def remap_args(self):
+        """(dimension, with_dual_projection, nb_vars_to_project, nb_env) of
+        IRemap::appliRemap for this context."""
+        return (self.dim, 1, 3 * self.nenv + 5, self.nenv)
+
+    def remap_prepare(self):
+        return self._check(self.L.optional["remap_prepare"](self.ctx), "remap_prepare")
+
+    def remap_appli(self, args=None):
+        return self._check(self.L.optional["remap_appli"](self.ctx, *(args or self.remap_args())), "remap_appli")
+
+    def remap_variables(self, args=None):
+        return self._check(self.L.optional["remap_variables"](self.ctx, *(args or self.remap_args())),
+                           "remap_variables")
+
+    def eos_init(self, env):
+        return self._check(self.L.optional["eos_init"](self.ctx, env), "eos_init")
+
+    def eos_apply(self, env):
+        return self._check(self.L.optional["eos_apply"](self.ctx, env), "eos_apply")
+
+    def eos_apply_one_cell(self, env, cell):
+        return self._check(self.L.optional["eos_apply_one_cell"](self.ctx, env, cell), "eos_apply_one_cell")
+
+    def eos_constants(self, env):
+        g, t = C.c_double(), C.c_double()
+        self._check(self.L.optional["eos_get_adiabatic_cst"](self.ctx, env, C.byref(g)), "eos_get_adiabatic_cst")
+        self._check(self.L.optional["eos_get_tension_limit_cst"](self.ctx, env, C.byref(t)),
+                    "eos_get_tension_limit_cst")
+        return g.value, t.value
+
+    def compute_pression_moyenne(self):
+        return self._check(self.L.optional["compute_pression_moyenne"](self.ctx), "compute_pression_moyenne")
+
+    def remap_split(self):
+        """MahycoModule::remap (PrepareRemap.cc:448-563) through the service
+        calls: prepare, appliRemap, applyEOS per environment, pressure
+        average."""
+        self.remap_prepare()
+        self.remap_appli()
+        for k in range(self.nenv):
+            self.eos_apply(k)
+        self.compute_pression_moyenne()
 
     def step(self):
         stop = C.c_int32()

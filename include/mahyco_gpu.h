@@ -111,7 +111,9 @@ typedef struct {
   double deltat_min;
   double deltat_max;
   double final_time;
-  int32_t with_projection;  /* default 1 */
+  int32_t with_projection;  /* default 1; 0: Lagrange only (no remap, the mesh
+                               moves; FP64 single contexts, run through the
+                               entry-point kernels) */
   int32_t pseudo_centree;   /* default 0 */
   int32_t schema_csts;      /* must be 0 (host-only path in the reference) */
   int32_t with_newton;      /* must be 0 (host-only path in the reference) */
@@ -188,6 +190,7 @@ int mhg_upload_state(mhg_ctx* ctx, const mhg_state* st);
 int mhg_download_state(mhg_ctx* ctx, mhg_state* st);
 
 /* ---- compute loop, one call per Mahyco.config entry point ------------ */
+/* 2D and 3D contexts in FP64 (MHG_E_UNSUPPORTED in the precision form). */
 /* Arcane's time-loop manager increments time by dt and the iteration count
  * before the compute-loop entry points of an iteration; a standalone caller
  * does it with this call (an Arcane shim would not call it). */
@@ -199,15 +202,63 @@ int mhg_update_position(mhg_ctx* ctx);
 int mhg_compute_geometric_values(mhg_ctx* ctx);
 int mhg_update_density(mhg_ctx* ctx);
 int mhg_compute_artificial_viscosity(mhg_ctx* ctx);
+/* updateEnergyAndPressure (MahycoModule.cc:1439-1478); with
+ * with_projection = 0 it also applies every EOS and the pressure average. */
 int mhg_update_energy_and_pressure(mhg_ctx* ctx);
+/* remap (PrepareRemap.cc:448-563) fused: mhg_remap_prepare, mhg_remap_appli,
+ * mhg_eos_apply of every environment and mhg_compute_pression_moyenne in one
+ * call.  With with_projection = 0 it does nothing (the reference returns at
+ * once) but close the iteration's Lagrange phase. */
 int mhg_remap(mhg_ctx* ctx);
+/* computePressionMoyenne (MahycoModule.cc:1836-1915): cells holding several
+ * environments take p = sum f_k p_k and c = max(1e-20, max c_k). */
+int mhg_compute_pression_moyenne(mhg_ctx* ctx);
+/* computeDeltaT (MahycoModule.cc:2027-2191) from the current state. */
 int mhg_compute_delta_t(mhg_ctx* ctx, double* new_dt, int32_t* stop);
 
-/* ---- IRemap (mahyco/src/Remap/IRemap.h:46-64) ------------------------- */
+/* ---- IRemap (mahyco/src/Remap/IRemap.h:32-65), RemapADI ---------------- */
+/* computeVariablesForRemap + computeFaceQuantitesForRemap
+ * (PrepareRemap.cc:15-437), the MahycoModule side of the remap: the remap
+ * vector of every cell from the Lagrangian state; the face quantities are
+ * formed inside the sweeps. */
+int mhg_remap_prepare(mhg_ctx* ctx);
+/* appliRemap (IRemap.h:46, RemapADIService.cc:19-60): the cell and dual
+ * sweeps in SensProjection order, then remapVariables.  Pressures and sound
+ * speeds are the EOS's: call mhg_eos_apply per environment and
+ * mhg_compute_pression_moyenne next.  dimension, with_dual_projection,
+ * nb_vars_to_project (3 nb_env + 5) and nb_env must match the context
+ * (MHG_E_ARG otherwise). */
+int mhg_remap_appli(mhg_ctx* ctx, int32_t dimension, int32_t with_dual_projection,
+                    int32_t nb_vars_to_project, int32_t nb_env);
+/* remapVariables (IRemap.h:64, RemapADIFinal.cc:19-778) from the remap
+ * vectors of this iteration's mhg_remap_appli, which already ends with it. */
+int mhg_remap_variables(mhg_ctx* ctx, int32_t dimension, int32_t with_dual_projection,
+                        int32_t nb_vars_to_project, int32_t nb_env);
+/* resizeRemapVariables (IRemap.h:50): the context sizes its remap arrays at
+ * creation; the call checks the counts (MHG_E_ARG if they differ). */
+int mhg_remap_resize_variables(mhg_ctx* ctx, int32_t nb_vars_to_project, int32_t nb_env);
+/* synchronizeUremap (IRemap.h:54): ghost-cell exchange; a context holds the
+ * whole mesh on one device, so it has none to exchange. */
+int mhg_remap_synchronize_uremap(mhg_ctx* ctx);
 int mhg_remap_get_ordre_projection(const mhg_ctx* ctx);
 int mhg_remap_has_projection_pente_borne(const mhg_ctx* ctx);
 int mhg_remap_has_conservation_energie_totale(const mhg_ctx* ctx);
 int mhg_remap_is_euler(const mhg_ctx* ctx);
+
+/* ---- IEquationOfState (mahyco/src/eos/IEquationOfState.h:32-65) ------- */
+/* Per environment env (its <eos-model>: PerfectGas,
+ * PerfectGasEOSService.cc:20-137, or StiffenedGas,
+ * StiffenedGasEOSService.cc:10-59), on the cells holding it (the cell values
+ * of a cell holding env alone, the environment values of a mixed cell).
+ * FP64 contexts. */
+/* initEOS: internal energy and sound speed from density and pressure. */
+int mhg_eos_init(mhg_ctx* ctx, int32_t env);
+/* applyEOS: pressure and sound speed from density and internal energy. */
+int mhg_eos_apply(mhg_ctx* ctx, int32_t env);
+/* applyOneCellEOS: the same for one cell. */
+int mhg_eos_apply_one_cell(mhg_ctx* ctx, int32_t env, int64_t cell);
+int mhg_eos_get_adiabatic_cst(const mhg_ctx* ctx, int32_t env, double* value);
+int mhg_eos_get_tension_limit_cst(const mhg_ctx* ctx, int32_t env, double* value);
 
 /* ---- fused fast path ---------------------------------------------------- */
 /* One full iteration (all entry points of Mahyco.config:30-41), fused. */

@@ -2186,6 +2186,84 @@ int mho_compute_delta_t(mho_ctx* ctx, double* new_dt, int32_t* stop) {
   if (r == MHG_E_DT_MIN) ctx->err = "dt below deltat-min";
   return r;
 }
+// The remap split as the reference's services call it (PrepareRemap.cc:448-563).
+static int checkRemapArgs(mho_ctx* ctx, int32_t dim, int32_t wd, int32_t nvar, int32_t ne) {
+  if (dim != ctx->dim || wd != 1 || ne != ctx->nenv || nvar != ctx->nvar)
+    return fail(ctx, MHG_E_ARG, "remap: counts differ from the context's");
+  if (!ctx->opt.with_projection) return fail(ctx, MHG_E_STATE, "remap: no projection");
+  return MHG_OK;
+}
+int mho_remap_prepare(mho_ctx* ctx) {
+  int r = checkReady(ctx);
+  if (r) return r;
+  if (!ctx->opt.with_projection) return fail(ctx, MHG_E_STATE, "remap_prepare: no projection");
+  computeVariablesForRemap(*ctx);
+  computeFaceQuantitesForRemap(*ctx);
+  return MHG_OK;
+}
+int mho_remap_appli(mho_ctx* ctx, int32_t dim, int32_t wd, int32_t nvar, int32_t ne) {
+  int r = checkReady(ctx);
+  if (r) return r;
+  if ((r = checkRemapArgs(ctx, dim, wd, nvar, ne))) return r;
+  appliRemap(*ctx);
+  return MHG_OK;
+}
+int mho_remap_variables(mho_ctx* ctx, int32_t dim, int32_t wd, int32_t nvar, int32_t ne) {
+  int r = checkReady(ctx);
+  if (r) return r;
+  if ((r = checkRemapArgs(ctx, dim, wd, nvar, ne))) return r;
+  remapVariables(*ctx);
+  return MHG_OK;
+}
+int mho_remap_resize_variables(mho_ctx* ctx, int32_t nvar, int32_t ne) {
+  if (!ctx) return MHG_E_ARG;
+  return (ne == ctx->nenv && nvar == ctx->nvar) ? MHG_OK : fail(ctx, MHG_E_ARG, "counts differ");
+}
+int mho_remap_synchronize_uremap(mho_ctx* ctx) { return checkReady(ctx); }
+int mho_eos_init(mho_ctx* ctx, int32_t env) {
+  int r = checkReady(ctx);
+  if (r) return r;
+  if (env < 0 || env >= ctx->nenv) return fail(ctx, MHG_E_ARG, "environment index");
+  initEOS(*ctx, env);
+  return MHG_OK;
+}
+int mho_eos_apply(mho_ctx* ctx, int32_t env) {
+  int r = checkReady(ctx);
+  if (r) return r;
+  if (env < 0 || env >= ctx->nenv) return fail(ctx, MHG_E_ARG, "environment index");
+  applyEOS(*ctx, env);
+  return MHG_OK;
+}
+// applyOneCellEOS (PerfectGasEOSService.cc / StiffenedGasEOSService.cc:49-60)
+int mho_eos_apply_one_cell(mho_ctx* ctx, int32_t env, int64_t c) {
+  int r = checkReady(ctx);
+  if (r) return r;
+  mho_ctx& S = *ctx;
+  if (env < 0 || env >= S.nenv || c < 0 || c >= S.NC) return fail(ctx, MHG_E_ARG, "environment or cell index");
+  if (S.isPure(c, env))
+    eosCell(S, env, S.density.g[c], S.internal_energy.g[c], &S.pressure.g[c], &S.sound_speed.g[c], &S.dpde.g[c]);
+  else if (S.isImpure(c, env))
+    eosCell(S, env, S.density.env(env, c), S.internal_energy.env(env, c), &S.pressure.env(env, c),
+            &S.sound_speed.env(env, c), &S.dpde.env(env, c));
+  return MHG_OK;
+}
+int mho_eos_get_adiabatic_cst(const mho_ctx* ctx, int32_t env, double* v) {
+  if (!ctx || !v || env < 0 || env >= ctx->nenv) return MHG_E_ARG;
+  *v = ctx->gamma_env[env];
+  return MHG_OK;
+}
+int mho_eos_get_tension_limit_cst(const mho_ctx* ctx, int32_t env, double* v) {
+  if (!ctx || !v || env < 0 || env >= ctx->nenv) return MHG_E_ARG;
+  *v = ctx->ptens_env[env];
+  return MHG_OK;
+}
+int mho_compute_pression_moyenne(mho_ctx* ctx) {
+  int r = checkReady(ctx);
+  if (r) return r;
+  computePressionMoyenne(*ctx);
+  return MHG_OK;
+}
+
 int mho_remap_get_ordre_projection(const mho_ctx* ctx) { return ctx ? ctx->opt.ordre_projection : MHG_E_ARG; }
 int mho_remap_has_projection_pente_borne(const mho_ctx* ctx) { return ctx ? ctx->opt.projection_pente_borne : MHG_E_ARG; }
 int mho_remap_has_conservation_energie_totale(const mho_ctx* ctx) { return ctx ? ctx->opt.conservation_energie_totale : MHG_E_ARG; }

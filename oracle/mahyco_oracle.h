@@ -46,6 +46,19 @@ int mho_compute_artificial_viscosity(mho_ctx* ctx);
 int mho_update_energy_and_pressure(mho_ctx* ctx);
 int mho_remap(mho_ctx* ctx);
 int mho_compute_delta_t(mho_ctx* ctx, double* new_dt, int32_t* stop);
+int mho_remap_prepare(mho_ctx* ctx);
+int mho_remap_appli(mho_ctx* ctx, int32_t dimension, int32_t with_dual_projection, int32_t nb_vars_to_project,
+                    int32_t nb_env);
+int mho_remap_variables(mho_ctx* ctx, int32_t dimension, int32_t with_dual_projection, int32_t nb_vars_to_project,
+                        int32_t nb_env);
+int mho_remap_resize_variables(mho_ctx* ctx, int32_t nb_vars_to_project, int32_t nb_env);
+int mho_remap_synchronize_uremap(mho_ctx* ctx);
+int mho_eos_init(mho_ctx* ctx, int32_t env);
+int mho_eos_apply(mho_ctx* ctx, int32_t env);
+int mho_eos_apply_one_cell(mho_ctx* ctx, int32_t env, int64_t cell);
+int mho_eos_get_adiabatic_cst(const mho_ctx* ctx, int32_t env, double* value);
+int mho_eos_get_tension_limit_cst(const mho_ctx* ctx, int32_t env, double* value);
+int mho_compute_pression_moyenne(mho_ctx* ctx);
 int mho_remap_get_ordre_projection(const mho_ctx* ctx);
 int mho_remap_has_projection_pente_borne(const mho_ctx* ctx);
 int mho_remap_has_conservation_energie_totale(const mho_ctx* ctx);
