@@ -9,6 +9,8 @@
 // accumulation); the 64-2 output layer and the bias sums run on the VALU in
 // FP32.  Partial gradients are written per block and summed in a fixed order,
 // so training is deterministic.
+#include <mutex>
+
 #include "mhg_kernels.hpp"
 #include "mhg_mlp.hpp"
 #include "mhg_mlp_kernels.hpp"
@@ -504,6 +506,30 @@ __global__ __launch_bounds__(64 * kInfWaves) void k_learned_alpha_2d(Params P, L
 }
 
 // ---- launchers ------------------------------------------------------------------
+// Partial gradients of the allocation-free gradient: a fixed array, one
+// copy per device, shared by every call on that device.
+static __device__ __align__(256) float scratch[(kMlpNoallocRows / kMlpRows) * (kMlpParams + 1)];
+
+int mlp_noalloc_scratch(int device, float** out) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static float* ptr[kMaxDevices] = {};
+  if (device < 0 || device >= kMaxDevices) return MHG_E_UNSUPPORTED;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptr[device]) {
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) return MHG_E_HIP;
+    if (cur != device && hipSetDevice(device) != hipSuccess) return MHG_E_HIP;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) hipSetDevice(cur);
+    if (e != hipSuccess || !p) return MHG_E_HIP;
+    ptr[device] = static_cast<float*>(p);
+  }
+  *out = ptr[device];
+  return MHG_OK;
+}
+
 void launch_mlp_rows(hipStream_t st, bool train, const unsigned short* X, const float* T, const int* perm, long row0,
                      long row_end, const MlpWeights& W, float* Y, float* part, float inv_m) {
   const unsigned nblk = (unsigned)((row_end - row0 + kMlpRows - 1) / kMlpRows);

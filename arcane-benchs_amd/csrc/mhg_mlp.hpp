@@ -26,6 +26,10 @@ constexpr int kPW1 = 0, kPB1 = kPW1 + kMlpHid * kMlpIn, kPW2 = kPB1 + kMlpHid, k
 constexpr int kMlpTgt = 2 * 4 * 2;
 // rows per block of the network kernels (one wavefront, one 32-row MFMA tile)
 constexpr int kMlpRows = 32;
+// largest minibatch of the allocation-free gradient (mhg_mlp_gradient_noalloc):
+// its per-block partial gradients live in a fixed device array of
+// kMlpNoallocRows / kMlpRows * (kMlpParams + 1) floats (6.4 MB per device)
+constexpr long kMlpNoallocRows = 4096;
 
 // Device copies of the parameters used by the kernels.
 struct MlpWeights {

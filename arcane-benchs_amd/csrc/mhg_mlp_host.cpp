@@ -17,6 +17,7 @@ using namespace mhg;
 
 struct mhg_mlp {
   hipStream_t stream = nullptr;
+  int device = 0;  // the device holding the network's arrays
   std::vector<void*> owned;
   float *prm = nullptr, *m1 = nullptr, *m2 = nullptr, *grad = nullptr;  // FP32 master, Adam moments, gradient + loss
   unsigned short* wb = nullptr;                                          // BF16 copy of the parameters
@@ -108,7 +109,7 @@ int mhg_mlp_create(mhg_mlp** out) {
   if (!out) return MHG_E_ARG;
   *out = nullptr;
   mhg_mlp* n = new mhg_mlp();
-  if (hipStreamCreate(&n->stream) != hipSuccess) {
+  if (hipGetDevice(&n->device) != hipSuccess || hipStreamCreate(&n->stream) != hipSuccess) {
     delete n;
     return MHG_E_HIP;
   }
@@ -204,6 +205,25 @@ int mhg_mlp_gradient(mhg_mlp* n, int64_t row0, int64_t m, float* loss, float* gr
   NHIP(n, hipMemcpy(g.data(), n->grad, g.size() * sizeof(float), hipMemcpyDeviceToHost));
   if (grad) std::memcpy(grad, g.data(), kMlpParams * sizeof(float));
   if (loss) *loss = g[kMlpParams];
+  return MHG_OK;
+}
+
+// mhg_mlp_gradient with its partial gradients in the fixed per-device array
+// of mlp_noalloc_scratch and its result written to the caller's device
+// memory: no allocation, no host copy.
+int mhg_mlp_gradient_noalloc(mhg_mlp* n, int64_t row0, int64_t m, float* d_out) {
+  if (!n || !d_out || m < 1 || row0 < 0 || row0 + m > n->rows) return MHG_E_ARG;
+  if (m > kMlpNoallocRows)
+    return nfail(n, MHG_E_ARG, "mlp_gradient_noalloc: at most 4096 rows per call");
+  float* part = nullptr;
+  const int r = mlp_noalloc_scratch(n->device, &part);
+  if (r) return nfail(n, r, "mlp_gradient_noalloc: scratch address lookup failed");
+  const long nblk = (m + kMlpRows - 1) / kMlpRows;
+  launch_mlp_rows(n->stream, true, n->X, n->T, nullptr, row0, row0 + m, mlp_weights(n->wb, n->prm), nullptr, part,
+                  1.f / (float)m);
+  launch_mlp_reduce(n->stream, part, (int)nblk, d_out);
+  NHIP(n, hipGetLastError());
+  NHIP(n, hipStreamSynchronize(n->stream));
   return MHG_OK;
 }
 

@@ -33,6 +33,11 @@ template <class R>
 void launch_learned_alpha_2d(hipStream_t st, const Params& P, const LearnedArgsT<R>& A, const MlpWeights& W,
                              float* Xout, float* Sout);
 
+// Device address of the allocation-free gradient's partial-gradient array
+// on `device` (looked up once per device; the array itself is a static
+// device variable, one copy per GPU, kept until the process ends).
+int mlp_noalloc_scratch(int device, float** out);
+
 inline MlpWeights mlp_weights(const unsigned short* wb, const float* prm) {
   MlpWeights W;
   W.w1 = wb + kPW1;

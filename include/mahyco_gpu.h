@@ -350,6 +350,15 @@ int mhg_mlp_forward(mhg_mlp* net, const float* X, int64_t m, float* Y);
 int mhg_mlp_set_data(mhg_mlp* net, const float* X, const float* T, int64_t m);
 /* Loss and gradient over samples [row0, row0 + m) of the data. */
 int mhg_mlp_gradient(mhg_mlp* net, int64_t row0, int64_t m, float* loss, float* grad);
+/* The same loss and gradient without any allocation, for callers that
+ * cannot allocate (code under HIP graph capture, serving loops, harnesses
+ * that forbid it); no counterpart in the reference.  d_out: the caller's
+ * device memory of mhg_mlp_param_count() + 1 floats (gradient, then loss).
+ * At most 4096 rows (MHG_E_ARG beyond).  The per-block partial gradients
+ * live in one fixed 6.4 MB device array per GPU, shared by every call on
+ * that GPU and overwritten by each: calls that overlap on one GPU race.
+ * Host-synchronous. */
+int mhg_mlp_gradient_noalloc(mhg_mlp* net, int64_t row0, int64_t m, float* d_out);
 /* Adam (beta 0.9, 0.999, eps 1e-8) over shuffled minibatches; the mean
  * minibatch loss of each epoch in epoch_loss [epochs]. */
 int mhg_mlp_train(mhg_mlp* net, int32_t epochs, int32_t batch, float lr, uint64_t seed, float* epoch_loss);
