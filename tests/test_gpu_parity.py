@@ -299,3 +299,15 @@ def test_run_async_past_the_end(gpu):
     for f in ["density", "velocity", "node_coord"]:
         assert np.array_equal(again[f], got[f]), f
     assert g.step()
+
+
+def test_sod_quadratic_viscosity_one(gpu):
+    """The Sod configuration as stated (artificial viscosity 0.5 / 1.0; the
+    reference hard-codes (gamma + 1) / 2 = 1.2): one step from oracle states
+    within 1e-12, the whole run to t = 0.2 within 1e-10."""
+    _one_step_parity(gpu, H.sod_setup, 30, 40, 4, visc_quad_coeff=1.0)
+    o = H.sod_setup(H.oracle(), 100, 4, visc_quad_coeff=1.0)
+    g = H.sod_setup(gpu, 100, 4, visc_quad_coeff=1.0)
+    assert o.run(100000) == g.run(100000)
+    d = H.compare_states(g.download(), o.download(), 1, 100, 4, metric=H.rel_max)
+    assert max(d.values()) < 1e-10
