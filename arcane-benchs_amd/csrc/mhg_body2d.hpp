@@ -184,9 +184,18 @@ __device__ __forceinline__ double cell_width_2d(const double* a, const double* t
   return (first ? 0. : db) + (last ? 0. : df);
 }
 
-template <class R, int D, int NEM, bool FIRST, int LIM, bool LEARN = false>
+// FINAL: the last sweep of the remap fused with the final cell phase
+// (remapVariables, EOS, pressure average, k_cell_final_2d's arithmetic on
+// the values it would have read back): the new remap vector stays in
+// registers and the cell state goes to F.
+// LAG (with FIRST): the first sweep of the remap fused with the Lagrange cell
+// phase (k_cell_lagrange_2d's arithmetic on every lane, halo included): the
+// remap vector comes from registers, and the output lanes store V and L
+// through LA.
+template <class R, int D, int NEM, bool FIRST, int LIM, bool LEARN = false, bool FINAL = false, bool LAG = false>
 __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScalars* S, const CellSweepArgsT<R>& A,
-                                                        int blk) {
+                                                        int blk, const CellFinalArgsT<R>* F = nullptr,
+                                                        const CellLagArgsT<R>* LA = nullptr) {
   if (S->stop || S->err) return;
   // halo lanes on each side: 3 for the order-3 stencil (six cells per face)
   constexpr int HL = (LIM == kLimO3) ? 3 : 2, NO = 64 - 2 * HL;
@@ -230,25 +239,54 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
   const long na = fin ? i + (long)nnx * j : 0, nb = (D == 0) ? na + nnx : na + 1;
   const R uda = Ud[na], udb = Ud[nb];
   const double xfa = Xd[na], xfb = Xd[nb], xta = Xt[na], xtb = Xt[nb];
-  const R vlag = FIRST ? A.Vlag[c] : R(1);
-  // generalised limiters, order 3: coordinates across D of the cell's nodes
+  R vlag = (FIRST && !LAG) ? A.Vlag[c] : R(1);
+  const int fmask = FINAL ? F->mask[c] : 0;  // FINAL: membership mask, loaded with the rest
+  // generalised limiters, order 3, LAG: coordinates across D of the cell's nodes
   double tn[4] = {0., 0., 0., 0.};
-  if (LIM == kLimG || LIM == kLimO3) {
+  if (LIM == kLimG || LIM == kLimO3 || LAG) {
     tn[0] = Xt[m0];
     tn[1] = Xt[m0 + 1];
     tn[2] = Xt[m0 + 1 + nnx];
     tn[3] = Xt[m0 + nnx];
   }
   R pre[3][NEM];
+  R preq = R(0);
+  if (LAG) {  // the Lagrange cell phase of the lane's cell
+    const CellPre<R> lin = cell_pre_load<R, false>(*LA, c);
+    const double an[4] = {xa, xb, xcc, xdd};
+    double xq[4], yq[4];
 #pragma unroll
-  for (int g = 0; g < 3; ++g)
+    for (int q = 0; q < 4; ++q) {
+      xq[q] = D == 0 ? an[q] : tn[q];
+      yq[q] = D == 0 ? tn[q] : an[q];
+    }
+    const double Vd = quad_volume(xq, yq);
+    const double Ld = (P.lc == MHG_LC_FACES_OPPOSEES) ? length_faces_opposees_2d(xq, yq) : pow(Vd, 0.5);
+    R uv[NEM], um[NEM], ue[NEM], uq;
+    lagrange_cell_vals<R, false, NEM>(P, S, *LA, c, (R)Vd, (R)Ld, lin, uv, um, ue, uq);
 #pragma unroll
     for (int k = 0; k < NEM; ++k) {
-      const R u = (k < ne) ? Uin[(g * ne + k) * NC + c] : R(0);
-      pre[g][k] = (ex && k < ne) ? u : R(0);
+      pre[0][k] = (ex && k < ne) ? uv[k] : R(0);
+      pre[1][k] = (ex && k < ne) ? um[k] : R(0);
+      pre[2][k] = (ex && k < ne) ? ue[k] : R(0);
     }
-  const R preq_l = Uin[(3 * ne) * NC + c];
-  const R preq = ex ? preq_l : R(0);
+    preq = ex ? uq : R(0);
+    vlag = (R)Vd;
+    if (out) {
+      st_stream(&LA->Vlag[c], (R)Vd);
+      st_stream(&LA->L[c], (R)Ld);
+    }
+  } else {
+#pragma unroll
+    for (int g = 0; g < 3; ++g)
+#pragma unroll
+      for (int k = 0; k < NEM; ++k) {
+        const R u = (k < ne) ? Uin[(g * ne + k) * NC + c] : R(0);
+        pre[g][k] = (ex && k < ne) ? u : R(0);
+      }
+    const R preq_l = Uin[(3 * ne) * NC + c];
+    preq = ex ? preq_l : R(0);
+  }
   // cell centre (component D, double) and phi denominator
   double xc = 0.;
   R den = R(1);
@@ -378,12 +416,16 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
     }
   };
   R* __restrict__ Uout = A.Uout;
+  R fV[NEM], fM[NEM], fE[NEM];  // FINAL: the new remap vector of the cell
+#pragma unroll
+  for (int k = 0; k < NEM; ++k) fV[k] = fM[k] = fE[k] = R(0);
 #pragma unroll
   for (int k = 0; k < NEM; ++k) {
     if (k >= ne) break;
     R un, du;
     sweep_var(pre[0][k], un, du, 0, k);
-    if (out) st_stream(&Uout[k * NC + c], un);
+    if (FINAL) fV[k] = un;
+    else if (out) st_stream(&Uout[k * NC + c], un);
   }
   if (LEARN) es.close_volumes(ne, alpha);
 #pragma unroll
@@ -392,8 +434,9 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
     R un, du;
     sweep_var(pre[1][k], un, du, 1, k);
     if (un < R(0)) un = R(0);  // RemapADIService.cc:953-955
+    if (FINAL) fM[k] = un;
     if (out) {
-      st_stream(&Uout[(ne + k) * NC + c], un);
+      if (!FINAL) st_stream(&Uout[(ne + k) * NC + c], un);
       st_stream(&A.contrib[k * NC + c], R(0.5) * du);  // fcontrib (DualPhaseRemap.cc:168-185)
     }
   }
@@ -403,12 +446,21 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
     R un, du;
     sweep_var(pre[2][k], un, du, 2, k);
     if (un < R(0)) un = R(0);  // RemapADIService.cc:956-958
-    if (out) st_stream(&Uout[(2 * ne + k) * NC + c], un);
+    if (FINAL) fE[k] = un;
+    else if (out) st_stream(&Uout[(2 * ne + k) * NC + c], un);
   }
+  R fQ = R(0);
   {
     R un, du;
     sweep_var(preq, un, du, 3, 0);
-    if (out) st_stream(&Uout[(3 * ne) * NC + c], un);
+    if (FINAL) fQ = un;
+    else if (out) st_stream(&Uout[(3 * ne) * NC + c], un);
+  }
+  if (FINAL && out) {
+    // Euler volume: the corner-normal formula on the initial coordinates
+    double x0[4] = {node_x0(P, i), node_x0(P, i + 1), node_x0(P, i + 1), node_x0(P, i)};
+    double y0[4] = {node_y0(P, j), node_y0(P, j), node_y0(P, j + 1), node_y0(P, j + 1)};
+    final_cell_vals<R, NEM>(P, *F, c, quad_volume(x0, y0), fV, fM, fE, fQ, R(0), fmask);
   }
 }
 
@@ -434,9 +486,18 @@ struct YTileShape {
   }
 };
 
-template <class R, int NEM, bool FIRST, int LIM, bool SEG, bool LEARN = false>
+// FINAL: as in the lane form, the last sweep fused with the final cell
+// phase; the thread that would store a cell's remap vector row-major keeps it
+// and runs the final phase of that cell.
+// LAG (with FIRST): as in the lane form, the Lagrange cell phase runs on
+// each row-major cell of the tile (halo rows included) and its remap vector
+// feeds the staging tiles from registers.
+template <class R, int NEM, bool FIRST, int LIM, bool SEG, bool LEARN = false, bool FINAL = false,
+          bool LAG = false>
 __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevScalars* S,
-                                                         const CellSweepArgsT<R>& A, int L) {
+                                                         const CellSweepArgsT<R>& A, int L,
+                                                         const CellFinalArgsT<R>* F = nullptr,
+                                                         const CellLagArgsT<R>* LA = nullptr) {
   // geometry tiles [0..4]; once read, tiles [0..5] are reused as the staging
   // tiles (in, out, dual flux) of a pair of variables
   // order 3: a seventh tile for the cells' mixed / pure codes
@@ -481,20 +542,53 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
   const double ya = A.xny[m0], yb = A.xny[m0 + 1], yc = A.xny[m0 + 1 + nnx], yd = A.xny[m0 + nnx];
   const R uya = A.uy[na], uyb = A.uy[nb];
   const double yfa = A.xny[na], yfb = A.xny[nb], xfa = A.xnx[na], xfb = A.xnx[nb];
-  const R vlag = FIRST ? A.Vlag[gc] : R(1);
-  // generalised limiters, order 3: x coordinates of the cell's four nodes
+  R vlag = (FIRST && !LAG) ? A.Vlag[gc] : R(1);
+  // FINAL: the cell's membership mask, loaded with the geometry
+  const int fmask = FINAL ? F->mask[gc] : 0;
+  // generalised limiters, order 3, LAG: x coordinates of the cell's four nodes
   double tn[4] = {0., 0., 0., 0.};
-  if (LIM == kLimG || LIM == kLimO3) {
+  if (LIM == kLimG || LIM == kLimO3 || LAG) {
     tn[0] = A.xnx[m0];
     tn[1] = A.xnx[m0 + 1];
     tn[2] = A.xnx[m0 + 1 + nnx];
     tn[3] = A.xnx[m0 + nnx];
   }
-  R uvol[NEM];
-#pragma unroll
-  for (int k = 0; k < NEM; ++k) uvol[k] = (k < ne) ? Uin[k * NC + gc] : R(0);
   const int nvu = P.nvu;
-  const R nxa0 = Uin[gc], nxb0 = nvu > 1 ? Uin[NC + gc] : R(0);
+  R uvol[NEM];
+  R lU[3 * NEM + 1];  // LAG: the remap vector of the row-major cell
+#pragma unroll
+  for (int v = 0; v < 3 * NEM + 1; ++v) lU[v] = R(0);
+  R nxa0, nxb0;
+  if (LAG) {  // the Lagrange cell phase of the row-major cell
+    const CellPre<R> lin = cell_pre_load<R, false>(*LA, gc);
+    const double yq[4] = {ya, yb, yc, yd};
+    const double Vd = quad_volume(tn, yq);
+    const double Ld = (P.lc == MHG_LC_FACES_OPPOSEES) ? length_faces_opposees_2d(tn, yq) : pow(Vd, 0.5);
+    R uv[NEM], um[NEM], ue[NEM], uq;
+    lagrange_cell_vals<R, false, NEM>(P, S, *LA, gc, (R)Vd, (R)Ld, lin, uv, um, ue, uq);
+#pragma unroll
+    for (int k = 0; k < NEM; ++k) {
+      uvol[k] = (k < ne) ? uv[k] : R(0);
+      if (k < ne) {
+        lU[k] = uv[k];
+        lU[ne + k] = um[k];
+        lU[2 * ne + k] = ue[k];
+      }
+    }
+    lU[3 * ne] = uq;
+    vlag = (R)Vd;
+    nxa0 = lU[0];
+    nxb0 = lU[1];
+    if (gout) {
+      st_stream(&LA->Vlag[gc], (R)Vd);
+      st_stream(&LA->L[gc], (R)Ld);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < NEM; ++k) uvol[k] = (k < ne) ? Uin[k * NC + gc] : R(0);
+    nxa0 = Uin[gc];
+    nxb0 = nvu > 1 ? Uin[NC + gc] : R(0);
+  }
   {  // geometry of the tile, row-major (stored in the double tiles exactly)
     double xc = 0., xf = 0., lf = 0.;
     R den = R(1), uf = R(0);
@@ -638,6 +732,9 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
   // variables two at a time (a, b = a + 1), one barrier pair per round
   R nxa = gex ? nxa0 : R(0);
   R nxb = (gex && nvu > 1) ? nxb0 : R(0);
+  R fU[3 * NEM + 1];  // FINAL: the new remap vector of the row-major cell
+#pragma unroll
+  for (int v = 0; v < 3 * NEM + 1; ++v) fU[v] = R(0);
   auto round = [&](const int va) {
     const int vb = va + 1;
     const bool hb = vb < nvu;  // uniform
@@ -646,8 +743,13 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
       stg[1 * TS + ra] = nxb;
     }
     __syncthreads();
-    if (va + 2 < nvu) nxa = gex ? Uin[(va + 2) * NC + gc] : R(0);
-    if (va + 3 < nvu) nxb = gex ? Uin[(va + 3) * NC + gc] : R(0);
+    if (LAG) {
+      if (va + 2 < nvu) nxa = gex ? lU[va + 2] : R(0);
+      if (va + 3 < nvu) nxb = gex ? lU[va + 3] : R(0);
+    } else {
+      if (va + 2 < nvu) nxa = gex ? Uin[(va + 2) * NC + gc] : R(0);
+      if (va + 3 < nvu) nxb = gex ? Uin[(va + 3) * NC + gc] : R(0);
+    }
     const R ua = tlane ? stg[0 * TS + ta] : R(0), ub = tlane ? stg[1 * TS + ta] : R(0);
     R una, dua, unb, dub;
     sweep_var(ua, una, dua, va);
@@ -662,19 +764,37 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
       if (mb) stg[5 * TS + ta] = R(0.5) * dub;
     }
     __syncthreads();
+    if (FINAL) {
+      fU[va] = trow ? stg[2 * TS + ra] : R(0);
+      if (hb) fU[vb] = trow ? stg[3 * TS + ra] : R(0);
+    }
     if (gout) {
-      st_stream(&A.Uout[va * NC + gc], stg[2 * TS + ra]);
-      if (hb) st_stream(&A.Uout[vb * NC + gc], stg[3 * TS + ra]);
+      if (!FINAL) {
+        st_stream(&A.Uout[va * NC + gc], stg[2 * TS + ra]);
+        if (hb) st_stream(&A.Uout[vb * NC + gc], stg[3 * TS + ra]);
+      }
       if (ma) st_stream(&A.contrib[(va - ne) * NC + gc], stg[4 * TS + ra]);
       if (mb) st_stream(&A.contrib[(vb - ne) * NC + gc], stg[5 * TS + ra]);
     }
   };
-  if (LEARN || LIM == kLimO3) {  // compile-time variable indices: per-environment values stay in registers
+  if (LEARN || LIM == kLimO3 || FINAL || LAG) {  // compile-time variable indices: per-environment values stay in registers
 #pragma unroll
     for (int va = 0; va < 3 * NEM + 1; va += 2)
       if (va < nvu) round(va);
   } else {
     for (int va = 0; va < nvu; va += 2) round(va);
+  }
+  if (FINAL && gout) {
+    R fV[NEM], fM[NEM], fE[NEM];
+#pragma unroll
+    for (int k = 0; k < NEM; ++k) {
+      fV[k] = (k < ne) ? fU[k] : R(0);
+      fM[k] = (k < ne) ? fU[ne + k] : R(0);
+      fE[k] = (k < ne) ? fU[2 * ne + k] : R(0);
+    }
+    double x0[4] = {node_x0(P, gi), node_x0(P, gi + 1), node_x0(P, gi + 1), node_x0(P, gi)};
+    double y0[4] = {node_y0(P, gs), node_y0(P, gs), node_y0(P, gs + 1), node_y0(P, gs + 1)};
+    final_cell_vals<R, NEM>(P, *F, gc, quad_volume(x0, y0), fV, fM, fE, fU[3 * ne], R(0), fmask);
   }
 }
 
@@ -762,8 +882,21 @@ template <int BLOCK, bool ADVANCE>
 __device__ __forceinline__ void dt_finalize_body(const Params& P, DevScalars* S, const double* block_min, int nb) {
   if (S->stop || S->err) return;
   __shared__ double red[BLOCK / 64];
-  double v = 1.7976931348623157e308;
-  for (int b = threadIdx.x; b < nb; b += BLOCK) v = smin(v, block_min[b]);
+  // eight loads in flight per thread (the minimum does not depend on the order)
+  double w[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) w[u] = 1.7976931348623157e308;
+  for (int b = threadIdx.x; b < nb; b += 8 * BLOCK) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int idx = b + u * BLOCK;
+      const double x = block_min[idx < nb ? idx : 0];
+      if (idx < nb) w[u] = smin(w[u], x);
+    }
+  }
+  double v = w[0];
+#pragma unroll
+  for (int u = 1; u < 8; ++u) v = smin(v, w[u]);
   for (int off = 32; off > 0; off >>= 1) v = smin(v, __shfl_down(v, off, 64));
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();

@@ -70,9 +70,13 @@ struct mhg_ctx {
   int xcur = 0;  // buffer with the coordinates of the last Lagrange phase
   int ucur = 0;  // buffer with the current velocity
   int sens = 0;  // SensProjection
-  int yvariant = 0;   // launch variant of the y cell sweep (mhg_autotune)
-  int nxvariant = 1;  // launch variant of the x dual sweep
-  int nyvariant = 1;  // launch variant of the y dual sweep
+  // launch variants (mhg_autotune); the defaults are the ones it picks on
+  // the triple point: the LDS tiles along y, the lane form along x
+  int yvariant = 4;   // y cell sweep
+  int nxvariant = 1;  // x dual sweep
+  int nyvariant = 4;  // y dual sweep
+  int fuse_final = 1; // fused step: the last cell sweep carries the final cell phase
+  int fuse_lag = 1;   // fused step: the first cell sweep carries the Lagrange cell phase
 
   // per-entry-point path: the _n, tau and div u arrays (allocated on the
   // first entry-point call) and which coordinates are current between calls
@@ -215,7 +219,7 @@ CellSweepArgsT<R> cell_sweep_args(mhg_ctx* c, int pass, int idir) {
 
 template <class R>
 NodeSweepArgsT<R> node_sweep_args(mhg_ctx* c) {
-  NodeSweepArgsT<R> B;
+  NodeSweepArgsT<R> B{};
   B.m = bp<R>(c->m);
   B.ux = bp<R>(c->ux[c->ucur]);
   B.uy = bp<R>(c->uy[c->ucur]);

@@ -179,13 +179,25 @@ void collect_times(mhg_ctx* c) {
 // Remap (IRemap::appliRemap, RemapADIService.cc:19-60): for i < dim,
 // idir = (i + SensProjection) % dim, cell sweep then dual sweep; then
 // SensProjection = (SensProjection + 1) % dim and remapVariables.
+// In the fused step (dt_mode 2) the last cell sweep carries the final cell
+// phase (fused_final_supported), and the CFL candidates follow the last dual
+// sweep, whose velocities they read.
+// lag: the first cell sweep carries the Lagrange cell phase (the caller
+// launched no k_cell_lagrange_2d).
 template <class R>
-void remap_2d(mhg_ctx* c, int dt_mode, bool eos = true) {
+void remap_2d(mhg_ctx* c, int dt_mode, bool eos = true, bool lag = false) {
   const Params& P = c->P;
+  const bool fuse = dt_mode == 2 && eos && c->fuse_final && fused_final_supported(P);
   for (int pass = 0; pass < P.dim; ++pass) {
     const int idir = (pass + c->sens) % P.dim;
     const bool first = (pass == 0);
-    {
+    if (lag && first) {
+      TimedLaunch t(c, idir == 0 ? "cell_sweep_x_lag" : "cell_sweep_y_lag");
+      launch_cell_sweep_lag_2d(c->stream, P, c->S, cell_sweep_args<R>(c, pass, idir), cell_lag_args<R>(c), idir);
+    } else if (fuse && pass == P.dim - 1) {
+      TimedLaunch t(c, idir == 0 ? "cell_sweep_x_final" : "cell_sweep_y_final");
+      launch_cell_sweep_final_2d(c->stream, P, c->S, cell_sweep_args<R>(c, pass, idir), cell_final_args<R>(c), idir);
+    } else {
       TimedLaunch t(c, idir == 0 ? "cell_sweep_x" : "cell_sweep_y");
       launch_cell_sweep_2d(c->stream, P, c->S, cell_sweep_args<R>(c, pass, idir), idir, first, c->yvariant);
     }
@@ -195,7 +207,18 @@ void remap_2d(mhg_ctx* c, int dt_mode, bool eos = true) {
                            idir == 0 ? c->nxvariant : c->nyvariant);
     }
   }
-  {
+  if (fuse) {
+    TimedLaunch t(c, "cfl");
+    CflArgsT<R> A;
+    A.L = bp<R>(c->L);
+    A.cs = bp<R>(c->cs);
+    A.ux = bp<R>(c->ux[1 - c->ucur]);
+    A.uy = bp<R>(c->uy[1 - c->ucur]);
+    A.uz = nullptr;
+    A.block_min = c->block_min;
+    launch_cfl_2d(c->stream, P, c->S, A);
+    launch_dt_finalize(c->stream, P, c->S, c->block_min, true);
+  } else if (!fuse) {
     TimedLaunch t(c, "cell_final");
     launch_cell_final_2d(c->stream, P, c->S, cell_final_args<R>(c), dt_mode, eos);
   }
@@ -216,11 +239,12 @@ void enqueue_step_t(mhg_ctx* c) {
     TimedLaunch t(c, "node_lagrange");
     launch_node_lagrange_2d(c->stream, c->P, c->S, node_lag_args<R>(c), true, true, true);
   }
-  {
+  const bool lag = c->fuse_lag && fused_lag_supported(c->P);
+  if (!lag) {
     TimedLaunch t(c, "cell_lagrange");
     launch_cell_lagrange_2d(c->stream, c->P, c->S, cell_lag_args<R>(c));
   }
-  remap_2d<R>(c, 2);
+  remap_2d<R>(c, 2, true, lag);
 }
 
 // 3D remap: three cell sweeps and three dual sweeps; the dual state goes
@@ -1510,10 +1534,11 @@ int mhg_conservation(mhg_ctx* c, double out[3]) {
 // between steps, and the boundary dual gradients, which are saved before and
 // restored after; the simulation state is unchanged.
 namespace {
-const char* kTunable[] = {"cell_sweep_y", "node_sweep_x", "node_sweep_y", "step_graph"};
+const char* kTunable[] = {"cell_sweep_y", "node_sweep_x", "node_sweep_y", "step_graph", "final_fusion",
+                          "lagrange_fusion"};
 int tunable_index(const char* kernel) {
   if (!kernel) return -1;
-  for (int k = 0; k < 4; ++k)
+  for (int k = 0; k < 6; ++k)
     if (std::strcmp(kernel, kTunable[k]) == 0) return k;
   return -1;
 }
@@ -1521,15 +1546,18 @@ int variant_count_of(int k) {
   if (k == 0) return cell_sweep_y_variants();
   if (k == 1) return node_sweep_variants(0);
   if (k == 2) return node_sweep_variants(1);
-  if (k == 3) return 2;
+  if (k >= 3 && k <= 5) return 2;
   return 0;
 }
 const char* variant_name_of(int k, int v) {
   if (k == 3) return v ? "graph12" : "direct";
+  if (k == 4 || k == 5) return v ? "fused" : "separate";
   return k == 0 ? cell_sweep_y_variant_name(v) : node_sweep_variant_name(k - 1, v);
 }
 int* variant_slot(mhg_ctx* c, int k) {
   if (k == 3) return &c->use_graphs;
+  if (k == 4) return &c->fuse_final;
+  if (k == 5) return &c->fuse_lag;
   return k == 0 ? &c->yvariant : (k == 1 ? &c->nxvariant : &c->nyvariant);
 }
 }  // namespace

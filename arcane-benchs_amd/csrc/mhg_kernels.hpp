@@ -163,11 +163,33 @@ void launch_pression_moyenne(hipStream_t st, const Params& P, DevScalars* S, con
 // minimum per block of 256 cells into block_min (cell_final_blocks slots):
 // L / (c + max over the cell's nodes of |u|), without the velocity term in
 // the Lagrange-only mode.
-struct CflArgs {
-  const double *L, *cs, *ux, *uy, *uz;
+template <class R>
+struct CflArgsT {
+  const R *L, *cs, *ux, *uy, *uz;
   double* block_min;
 };
+using CflArgs = CflArgsT<double>;
 void launch_cfl(hipStream_t st, const Params& P, DevScalars* S, const CflArgs& A);
+// The fused step's CFL candidates of a 2D mesh in the storage type R.
+template <class R>
+void launch_cfl_2d(hipStream_t st, const Params& P, DevScalars* S, const CflArgsT<R>& A);
+
+// The last cell sweep of the remap (dir 0: x, 1: y) fused with the final
+// cell phase (remapVariables, EOS, pressure average; no CFL candidates),
+// for the configurations fused_final_supported accepts: classical limiters
+// at order 1 or 2, no learned coefficients, no total-energy fix-up (its
+// deposit comes from the dual sweep that follows), at most 4 environments.
+bool fused_final_supported(const Params& P);
+// The first cell sweep of the remap fused with the Lagrange cell phase
+// (k_cell_lagrange_2d: V, L stored, the remap vector kept in registers), for
+// the configurations of fused_final_supported without pseudo-centring.
+bool fused_lag_supported(const Params& P);
+template <class R>
+void launch_cell_sweep_lag_2d(hipStream_t st, const Params& P, DevScalars* S, const CellSweepArgsT<R>& A,
+                              const CellLagArgsT<R>& LA, int dir);
+template <class R>
+void launch_cell_sweep_final_2d(hipStream_t st, const Params& P, DevScalars* S, const CellSweepArgsT<R>& A,
+                                const CellFinalArgsT<R>& F, int dir);
 void launch_dt_finalize(hipStream_t st, const Params& P, DevScalars* S, const double* block_min, bool advance);
 
 int cell_final_blocks(const Params& P);

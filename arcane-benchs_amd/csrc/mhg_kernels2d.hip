@@ -61,6 +61,37 @@ __global__ __launch_bounds__(1024) void k_cell_sweep_ytile_2d(Params P, DevScala
   cell_sweep_ytile_2d_body<R, NEM, FIRST, LIM, false, LEARN>(P, S, A, xcd_block(blockIdx.x, gridDim.x));
 }
 
+// The last cell sweep of the remap fused with the final cell phase.
+template <class R, int NEM, int LIM>
+__global__ __launch_bounds__(256) void k_cell_sweep_x_final_2d(Params P, DevScalars* S, CellSweepArgsT<R> A,
+                                                               CellFinalArgsT<R> F) {
+  cell_sweep_lane_2d_body<R, 0, NEM, false, LIM, false, true>(P, S, A, xcd_block(blockIdx.x, gridDim.x), &F);
+}
+
+// The first cell sweep of the remap fused with the Lagrange cell phase.
+template <class R, int NEM, int LIM>
+__global__ __launch_bounds__(256) void k_cell_sweep_x_lag_2d(Params P, DevScalars* S, CellSweepArgsT<R> A,
+                                                             CellLagArgsT<R> LA) {
+  cell_sweep_lane_2d_body<R, 0, NEM, true, LIM, false, false, true>(P, S, A, xcd_block(blockIdx.x, gridDim.x),
+                                                                     nullptr, &LA);
+}
+
+template <class R, int NEM, int LIM>
+__global__ __launch_bounds__(1024) void k_cell_sweep_y_lag_2d(Params P, DevScalars* S, CellSweepArgsT<R> A,
+                                                                 CellLagArgsT<R> LA) {
+  cell_sweep_ytile_2d_body<R, NEM, true, LIM, false, false, false, true>(P, S, A, xcd_block(blockIdx.x, gridDim.x),
+                                                                        nullptr, &LA);
+}
+
+// At least 8 waves per SIMD (two 1024-thread blocks per CU): the bound keeps
+// the scalar registers of both argument blocks within the 8-wave budget; the
+// unbounded kernel took 102 and ran one block per CU.
+template <class R, int NEM, int LIM>
+__global__ __launch_bounds__(1024, 8) void k_cell_sweep_y_final_2d(Params P, DevScalars* S, CellSweepArgsT<R> A,
+                                                                   CellFinalArgsT<R> F) {
+  cell_sweep_ytile_2d_body<R, NEM, false, LIM, false, false, true>(P, S, A, xcd_block(blockIdx.x, gridDim.x), &F);
+}
+
 template <class R, int BLOCK, int NEM, bool EOS = true>
 __global__ __launch_bounds__(BLOCK) void k_cell_final_2d(Params P, DevScalars* S, CellFinalArgsT<R> A) {
   cell_final_2d_body<R, BLOCK, NEM, EOS>(P, S, A, xcd_block(blockIdx.x, gridDim.x), A.block_min);
@@ -787,6 +818,78 @@ static void launch_tile_y(hipStream_t st, const Params& P, DevScalars* S, const 
 #undef YT_
 }
 
+bool fused_final_supported(const Params& P) {
+  return !P.learned && !P.etot && !P.lagonly && P.ne <= 4 && P.lim < kLimG && P.order2 != 2;
+}
+
+template <class R>
+void launch_cell_sweep_final_2d(hipStream_t st, const Params& P, DevScalars* S, const CellSweepArgsT<R>& A,
+                                const CellFinalArgsT<R>& F, int dir) {
+  if (dir == 0) {
+    const long waves = (long)((P.nx + 59) / 60) * P.ny;
+    dim3 g(nblocks(waves * 64, 256)), b(256);
+#define XF_(N, LM) hipLaunchKernelGGL((k_cell_sweep_x_final_2d<R, N, LM>), g, b, 0, st, P, S, A, F)
+#define XFN_(LM)                          \
+  do {                                    \
+    if (P.ne <= 1) XF_(1, LM);            \
+    else if (P.ne <= 2) XF_(2, LM);       \
+    else if (P.ne <= 3) XF_(3, LM);       \
+    else XF_(4, LM);                      \
+  } while (0)
+    MHG_LIM_DISPATCH(P.lim, XFN_);
+#undef XFN_
+#undef XF_
+    return;
+  }
+  dim3 g((unsigned)(((P.nx + kTileW - 1) / kTileW) * ((P.ny + 59) / 60))), b(1024);
+#define YF_(N, LM) hipLaunchKernelGGL((k_cell_sweep_y_final_2d<R, N, LM>), g, b, 0, st, P, S, A, F)
+#define YFN_(LM)                          \
+  do {                                    \
+    if (P.ne <= 1) YF_(1, LM);            \
+    else if (P.ne <= 2) YF_(2, LM);       \
+    else if (P.ne <= 3) YF_(3, LM);       \
+    else YF_(4, LM);                      \
+  } while (0)
+  MHG_LIM_DISPATCH(P.lim, YFN_);
+#undef YFN_
+#undef YF_
+}
+
+bool fused_lag_supported(const Params& P) { return fused_final_supported(P) && !P.pseudo_centree; }
+
+template <class R>
+void launch_cell_sweep_lag_2d(hipStream_t st, const Params& P, DevScalars* S, const CellSweepArgsT<R>& A,
+                              const CellLagArgsT<R>& LA, int dir) {
+  if (dir == 0) {
+    const long waves = (long)((P.nx + 59) / 60) * P.ny;
+    dim3 g(nblocks(waves * 64, 256)), b(256);
+#define XG_(N, LM) hipLaunchKernelGGL((k_cell_sweep_x_lag_2d<R, N, LM>), g, b, 0, st, P, S, A, LA)
+#define XGN_(LM)                          \
+  do {                                    \
+    if (P.ne <= 1) XG_(1, LM);            \
+    else if (P.ne <= 2) XG_(2, LM);       \
+    else if (P.ne <= 3) XG_(3, LM);       \
+    else XG_(4, LM);                      \
+  } while (0)
+    MHG_LIM_DISPATCH(P.lim, XGN_);
+#undef XGN_
+#undef XG_
+    return;
+  }
+  dim3 g((unsigned)(((P.nx + kTileW - 1) / kTileW) * ((P.ny + 59) / 60))), b(1024);
+#define YG_(N, LM) hipLaunchKernelGGL((k_cell_sweep_y_lag_2d<R, N, LM>), g, b, 0, st, P, S, A, LA)
+#define YGN_(LM)                          \
+  do {                                    \
+    if (P.ne <= 1) YG_(1, LM);            \
+    else if (P.ne <= 2) YG_(2, LM);       \
+    else if (P.ne <= 3) YG_(3, LM);       \
+    else YG_(4, LM);                      \
+  } while (0)
+  MHG_LIM_DISPATCH(P.lim, YGN_);
+#undef YGN_
+#undef YG_
+}
+
 template <class R>
 void launch_cell_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const CellSweepArgsT<R>& A,
                           int dir, bool first, int variant) {
@@ -1009,28 +1112,37 @@ __global__ __launch_bounds__(256) void k_pression_moyenne(Params P, DevScalars* 
   A.cs[c] = cs;
 }
 
-// CFL candidates of quads: the same arithmetic as k_cell_final_2d's.
-__global__ __launch_bounds__(256) void k_cfl_2d(Params P, DevScalars* S, CflArgs A) {
+// CFL candidates of quads: the same arithmetic as k_cell_final_2d's, in the
+// storage type R of the bulk fields.
+template <class R>
+__global__ __launch_bounds__(256) void k_cfl_2d(Params P, DevScalars* S, CflArgsT<R> A) {
   if (S->stop || S->err) return;
-  const long c = (long)blockIdx.x * 256 + threadIdx.x;
+  const long c = (long)xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   double dx_sound = 1.7976931348623157e308;
   if (c < P.nc) {
-    double vmax = 0.;
+    R vmax = R(0);
+    const R Lc = A.L[c], cc = A.cs[c];
     if (!P.lagonly) {
       const int nnx = P.nx + 1;
       const int i = (int)(c % P.nx), j = (int)(c / P.nx);
       const long n0 = i + (long)nnx * j;
       const long nd[4] = {n0, n0 + 1, n0 + 1 + nnx, n0 + nnx};
+      R vx[4], vy[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const double ux = A.ux[nd[q]], uy = A.uy[nd[q]];
-        const double un = sqrt(ux * ux + uy * uy + 0. * 0.);
+        vx[q] = A.ux[nd[q]];
+        vy[q] = A.uy[nd[q]];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const R ux = vx[q], uy = vy[q];
+        const R un = sqrt(ux * ux + uy * uy + R(0) * R(0));
         vmax = (un < vmax) ? vmax : un;
       }
     }
-    dx_sound = smin(dx_sound, A.L[c] / (A.cs[c] + vmax));
+    dx_sound = smin(dx_sound, (double)qdiv(Lc, cc + vmax));
   }
-  block_min_store<256>(dx_sound, A.block_min + blockIdx.x);
+  block_min_store<256>(dx_sound, A.block_min + xcd_block(blockIdx.x, gridDim.x));
 }
 
 void launch_eos_env(hipStream_t st, const Params& P, DevScalars* S, const EosArgs& A, int k, int mode, long c0,
@@ -1055,7 +1167,12 @@ void launch_cfl(hipStream_t st, const Params& P, DevScalars* S, const CflArgs& A
     launch_cfl_3d(st, P, S, A);
     return;
   }
-  hipLaunchKernelGGL(k_cfl_2d, dim3(cell_final_blocks(P)), dim3(256), 0, st, P, S, A);
+  hipLaunchKernelGGL(k_cfl_2d<double>, dim3(cell_final_blocks(P)), dim3(256), 0, st, P, S, A);
+}
+
+template <class R>
+void launch_cfl_2d(hipStream_t st, const Params& P, DevScalars* S, const CflArgsT<R>& A) {
+  hipLaunchKernelGGL(k_cfl_2d<R>, dim3(cell_final_blocks(P)), dim3(256), 0, st, P, S, A);
 }
 
 void launch_dt_finalize(hipStream_t st, const Params& P, DevScalars* S, const double* block_min, bool advance) {
@@ -1089,7 +1206,12 @@ void launch_cell_final_2d(hipStream_t st, const Params& P, DevScalars* S, const 
   template void launch_node_sweep_2d<R>(hipStream_t, const Params&, DevScalars*,                     \
                                         const NodeSweepArgsT<R>&, int, bool, int);                   \
   template void launch_cell_final_2d<R>(hipStream_t, const Params&, DevScalars*,                     \
-                                        const CellFinalArgsT<R>&, int, bool);
+                                        const CellFinalArgsT<R>&, int, bool);                        \
+  template void launch_cell_sweep_final_2d<R>(hipStream_t, const Params&, DevScalars*,               \
+                                              const CellSweepArgsT<R>&, const CellFinalArgsT<R>&, int); \
+  template void launch_cfl_2d<R>(hipStream_t, const Params&, DevScalars*, const CflArgsT<R>&);              \
+  template void launch_cell_sweep_lag_2d<R>(hipStream_t, const Params&, DevScalars*, const CellSweepArgsT<R>&, \
+                                            const CellLagArgsT<R>&, int);
 MHG_INSTANTIATE(double)
 MHG_INSTANTIATE(float)
 #undef MHG_INSTANTIATE

@@ -112,12 +112,12 @@ __device__ __forceinline__ CellPre<R> cell_pre_load(const CellLagArgsT<R>& A, lo
 }
 
 // Density, viscosity, energy and the remap vector of cell c from its
-// Lagrangian volume and characteristic length (both dimensions).
+// Lagrangian volume V and characteristic length L (both dimensions), into
+// uv, um, ue (per environment) and uq.
 template <class R, bool PC, int NEM>
-__device__ inline void lagrange_cell_bulk(const Params& P, const DevScalars* S, const CellLagArgsT<R>& A,
-                                          long c, double Vd, double Ld, const CellPre<R>& in) {
-  // geometry computed in double, the rest in the storage type
-  const R V = (R)Vd, L = (R)Ld;
+__device__ inline void lagrange_cell_vals(const Params& P, const DevScalars* S, const CellLagArgsT<R>& A, long c,
+                                          R V, R L, const CellPre<R>& in, R (&uv)[NEM], R (&um)[NEM],
+                                          R (&ue)[NEM], R& uq) {
   const R inv_dt = qdiv(R(1.0), (R)S->dt);
   const R rho_n = in.rho_n;
   const R rho = qdiv(in.m, V);
@@ -125,10 +125,8 @@ __device__ inline void lagrange_cell_bulk(const Params& P, const DevScalars* S, 
   const R divu = qdiv(inv_dt * (qdiv(R(1.0), rho) - qdiv(R(1.0), rho_n)), tau);
   const R cs = in.cs;
   const int mask = in.mask;
-  const int ne = env_count<NEM>(P);
   const long NC = P.nc;
-  R uv[NEM], um[NEM], ue[NEM];
-  R uq = R(0);
+  uq = R(0);
 #pragma unroll
   for (int k = 0; k < NEM; ++k) uv[k] = um[k] = ue[k] = R(0);
   if (__popc(mask) == 1) {
@@ -165,6 +163,18 @@ __device__ inline void lagrange_cell_bulk(const Params& P, const DevScalars* S, 
       uq = Vk * qk;  // PrepareRemap.cc:372 assigns: the last environment wins
     }
   }
+}
+
+// The same, stored: the remap vector, the Lagrangian volume and L.
+template <class R, bool PC, int NEM>
+__device__ inline void lagrange_cell_bulk(const Params& P, const DevScalars* S, const CellLagArgsT<R>& A,
+                                          long c, double Vd, double Ld, const CellPre<R>& in) {
+  // geometry computed in double, the rest in the storage type
+  const R V = (R)Vd, L = (R)Ld;
+  const int ne = env_count<NEM>(P);
+  const long NC = P.nc;
+  R uv[NEM], um[NEM], ue[NEM], uq;
+  lagrange_cell_vals<R, PC, NEM>(P, S, A, c, V, L, in, uv, um, ue, uq);
   R* U = A.U;
 #pragma unroll
   for (int k = 0; k < NEM; ++k) {
@@ -180,29 +190,23 @@ __device__ inline void lagrange_cell_bulk(const Params& P, const DevScalars* S, 
 
 // Final cell phase of cell c with Euler volume vold: membership, "moy", EOS,
 // pressure average; returns the cell's sound speed for the CFL candidate.
-// EOS = false stops after remapVariables (RemapADIFinal.cc:19-778): the
-// pressures and sound speeds are left to the EOS and pressure-average
-// entry points (PrepareRemap.cc:552-559), and the return value is 0.
+// The remap vector comes from the caller (volumes, masses and energies of
+// the environments, zero beyond ne, and q V), so that the last cell sweep
+// can run this phase on the values it has just formed.  EOS = false stops
+// after remapVariables (RemapADIFinal.cc:19-778): the pressures and sound
+// speeds are left to the EOS and pressure-average entry points
+// (PrepareRemap.cc:552-559), and the return value is 0.  old_mask_in: the
+// cell's membership mask when the caller loaded it ahead (-1: load it here).
 template <class R, int NEM, bool EOS = true>
-__device__ inline R final_cell(const Params& P, const CellFinalArgsT<R>& A, long c, double vold,
-                               R delta_ec = R(0)) {
+__device__ inline R final_cell_vals(const Params& P, const CellFinalArgsT<R>& A, long c, double vold,
+                                    const R (&Uvol)[NEM], const R (&Umas)[NEM], const R (&Uene)[NEM], R Uq,
+                                    R delta_ec = R(0), int old_mask_in = -1) {
   {
     const long NC = P.nc;
     const int ne = env_count<NEM>(P);
     const R thr = (R)P.remap_threshold;
     const R vol = (R)vold;
-    R Uvol[NEM], Umas[NEM], Uene[NEM];
-#pragma unroll
-    for (int k = 0; k < NEM; ++k) {
-      Uvol[k] = Umas[k] = Uene[k] = R(0);
-      if (k < ne) {
-        Uvol[k] = A.U[k * NC + c];
-        Umas[k] = A.U[(ne + k) * NC + c];
-        Uene[k] = A.U[(2 * ne + k) * NC + c];
-      }
-    }
-    const R Uq = A.U[(3 * ne) * NC + c];
-    const int old_mask = A.mask[c];
+    const int old_mask = old_mask_in >= 0 ? old_mask_in : A.mask[c];
     // cell status per environment (110-143)
     R volt = R(0);
 #pragma unroll
@@ -405,6 +409,26 @@ __device__ inline R final_cell(const Params& P, const CellFinalArgsT<R>& A, long
     st_stream(&A.mask[c], mask);
     return EOS ? c_g : R(0);
   }
+}
+
+// The final cell phase on the remap vector in memory.
+template <class R, int NEM, bool EOS = true>
+__device__ inline R final_cell(const Params& P, const CellFinalArgsT<R>& A, long c, double vold,
+                               R delta_ec = R(0)) {
+  const long NC = P.nc;
+  const int ne = env_count<NEM>(P);
+  R Uvol[NEM], Umas[NEM], Uene[NEM];
+#pragma unroll
+  for (int k = 0; k < NEM; ++k) {
+    Uvol[k] = Umas[k] = Uene[k] = R(0);
+    if (k < ne) {
+      Uvol[k] = A.U[k * NC + c];
+      Umas[k] = A.U[(ne + k) * NC + c];
+      Uene[k] = A.U[(2 * ne + k) * NC + c];
+    }
+  }
+  const R Uq = A.U[(3 * ne) * NC + c];
+  return final_cell_vals<R, NEM, EOS>(P, A, c, vold, Uvol, Umas, Uene, Uq, delta_ec);
 }
 
 // Reduction of the CFL candidates of a block to one minimum, stored at slot.

@@ -13,7 +13,12 @@ Per-zone word counts of the reference step over pure cells with nodes ~ cells
 giving 1000 B per zone-cycle on the 3-material triple point.  The kernels of
 the library map onto these rows; the final-node words are carried by the last
 node sweep (which writes the velocity) and the dt word by the final cell
-kernel.
+kernel.  The fused step carries the Lagrange cell phase in its first cell
+sweep and the final cell phase in its last (the remap vector then never
+goes to memory before the first sweep or after the last) and the CFL
+candidates in a kernel of their own.  A fused kernel's algorithmic bytes are
+the words of the rows it runs; its measured traffic is lower by the remap
+vector it keeps in registers.  The step's model stays 71 + 18 n_m.
 """
 
 PEAK_HBM_GBS = 8000.0  # MI355X HBM3E peak, GB/s
@@ -52,6 +57,16 @@ def kernel_bytes(name, n_cells, n_nodes, n_env, word=8):
         "node_sweep_x": 10 * n_nodes,
         "node_sweep_y": 10 * n_nodes,
         "cell_final": (3 * n_env + 7 + 1) * n_cells,
+        # a kernel that runs several rows of the model counts their words:
+        # the last cell sweep with the final cell phase (the remap vector
+        # between them stays in registers, so the kernel moves less than
+        # this), the first cell sweep with the Lagrange cell phase (likewise)
+        "cell_sweep_x_final": (9 * n_env + 14) * n_cells,
+        "cell_sweep_y_final": (9 * n_env + 14) * n_cells,
+        "cell_sweep_x_lag": (9 * n_env + 16) * n_cells,
+        "cell_sweep_y_lag": (9 * n_env + 16) * n_cells,
+        # the dt row: the CFL candidates after the last dual sweep
+        "cfl": 1 * n_cells,
     }.get(name)
     return None if w is None else w * word
 

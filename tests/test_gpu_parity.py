@@ -311,3 +311,31 @@ def test_sod_quadratic_viscosity_one(gpu):
     assert o.run(100000) == g.run(100000)
     d = H.compare_states(g.download(), o.download(), 1, 100, 4, metric=H.rel_max)
     assert max(d.values()) < 1e-10
+
+
+@pytest.mark.parametrize("fusion", [(1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("prec", [H.abi.PREC_FP64, H.abi.PREC_FP32_BULK])
+@pytest.mark.parametrize("case", ["sod", "tp", "tp_sb", "tp_o1"])
+def test_fusion_bit_identical(gpu, case, prec, fusion):
+    """The fused step's final cell phase inside the last cell sweep (its CFL
+    candidates in a kernel after the last dual sweep) and its Lagrange cell
+    phase inside the first cell sweep give the state of the separate kernels
+    bit for bit, over both sweep orders."""
+    kw = {"tp_sb": dict(projection_limiteur_id=1), "tp_o1": dict(ordre_projection=1)}.get(case, {})
+    setup = H.sod_setup if case == "sod" else H.tp_setup
+    nx, ny = (40, 4) if case == "sod" else (70, 134)
+    a = setup(gpu, nx, ny, precision=prec, **kw)
+    b = setup(gpu, nx, ny, precision=prec, **kw)
+    a.select_variant(0, "final_fusion")
+    a.select_variant(0, "lagrange_fusion")
+    b.select_variant(fusion[0], "final_fusion")
+    b.select_variant(fusion[1], "lagrange_fusion")
+    for n in (1, 2, 25):
+        a.run(n)
+        b.run(n)
+        sa, sb = a.download(), b.download()
+        for f, v in sa.items():
+            if isinstance(v, np.ndarray):
+                assert np.array_equal(v, sb[f]), (n, f)
+            else:
+                assert v == sb[f], (n, f)
