@@ -1,17 +1,22 @@
 """Summaries of rocprofv3 output for the committed profiles.
 
 `stats`: per-kernel durations from a `--kernel-trace --stats` run.
+`counters`: per-kernel means of the counters of one `--pmc` run (millions per
+dispatch, summed over the device).
 `traffic`: HBM bytes per dispatch from two `--pmc` runs (FETCH_SIZE, then
 WRITE_SIZE, KiB per dispatch).  Each counter is scaled by what it reads for a
 device-to-device copy of known size recorded in the same run (the largest
 `__amd_rocclr_copyBuffer` dispatch, `bench.py --copy-calibration MIB`).
 
     python -m mahyco_amd.profiling stats <dir>/tp_kernel_stats.csv
-    python -m mahyco_amd.profiling traffic <fetch csv> <write csv> MIB [label=prefix ...]
+    python -m mahyco_amd.profiling traffic <fetch csv> <write csv> MIB [label=prefix ...] [step=label,...]
+    python -m mahyco_amd.profiling counters <counter csv>
 
 The optional label=prefix arguments name the launch variant that the step
 used for a kernel label (the autotuner's choice); the per-label table averages
-the dispatches of that variant.
+the dispatches of that variant.  step=... lists the labels of the kernels the
+step runs (default: every step label found), leaving out the autotuner's
+candidate launches of kernels the step does not use.
 """
 import collections
 import csv
@@ -21,11 +26,17 @@ import sys
 STEP_KERNELS = {
     "node_lagrange": ["k_node_lagrange_2d"],
     "cell_lagrange": ["k_cell_lagrange_2d"],
+    # first cell sweep with the Lagrange cell phase, last with the final one
+    "cell_sweep_x_lag": ["k_cell_sweep_x_lag_2d"],
+    "cell_sweep_y_lag": ["k_cell_sweep_y_lag_2d"],
+    "cell_sweep_x_final": ["k_cell_sweep_x_final_2d"],
+    "cell_sweep_y_final": ["k_cell_sweep_y_final_2d"],
     "cell_sweep_x": ["k_cell_sweep_lane_2d<0"],
     "cell_sweep_y": ["k_cell_sweep_ytile_2d", "k_cell_sweep_y_2d"],
     "node_sweep_x": ["k_node_sweep_x_2d", "k_node_sweep_2d<0"],
     "node_sweep_y": ["k_node_sweep_ytile_2d", "k_node_sweep_y_2d", "k_node_sweep_2d<1"],
     "cell_final": ["k_cell_final_2d"],
+    "cfl": ["k_cfl_2d"],
     "dt": ["k_dt_finalize"],
 }
 
@@ -37,6 +48,11 @@ OTHER_KERNELS = {
     "cell_sweep_3d": ["k_cell_sweep_x_3d", "k_cell_sweep_tile_3d"],
     "node_sweep_3d": ["k_node_sweep_x_3d", "k_node_sweep_tile_3d"],
     "cell_final_3d": ["k_cell_final_3d"],
+    "cell_geometry_3d": ["k_cell_geometry_3d"],
+    "cfl_3d": ["k_cfl_3d"],
+    "cell_stage": ["k_cell_stage_2d"],
+    "eos_env": ["k_eos_env"],
+    "pression_moyenne": ["k_pression_moyenne"],
     "batch_node_lagrange": ["kb_node_lagrange_2d"],
     "batch_cell_lagrange": ["kb_cell_lagrange_2d"],
     "batch_cell_sweep_x": ["kb_cell_sweep_x_2d"],
@@ -122,16 +138,33 @@ def main(argv):
             print(f"| `{k}` | {label_of(k)} | {v['n']} | {v['fetch_kib']:.0f} | "
                   f"{v['write_kib']:.0f} | {v['bytes']:.0f} |")
         tuned = dict(a.split("=", 1) for a in argv[4:])
+        step = tuned.pop("step").split(",") if "step" in tuned else list(STEP_KERNELS)
         print("\nStep kernels (launch variants used by the step; mean over their dispatches):\n")
         print("| label | HBM bytes per launch |")
         print("|---|---|")
         for lab, prefixes in STEP_KERNELS.items():
+            if lab not in step:
+                continue
             pre = [tuned[lab]] if lab in tuned else prefixes
             ks = [k for k in out if any(untyped(k).startswith(p) for p in pre) and "float" not in k]
             if not ks:
                 continue
             b = sum(out[k]["bytes"] for k in ks) / len(ks)
             print(f"| {lab} | {b:.0f} |")
+        return 0
+    if argv[0] == "counters":
+        per = collections.defaultdict(lambda: collections.defaultdict(list))
+        for r in csv.DictReader(open(argv[1])):
+            per[short(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+        names = sorted({c for d in per.values() for c in d})
+        print("| kernel | label | dispatches | " + " | ".join(names) + " |")
+        print("|---|---|---|" + "---|" * len(names))
+        for k, d in sorted(per.items()):
+            if k.startswith("__amd") or k.startswith("at::"):
+                continue
+            n = len(d[names[0]])
+            vals = " | ".join(f"{sum(d[c]) / len(d[c]) / 1e6:.3f}" for c in names)
+            print(f"| `{k}` | {label_of(k)} | {n} | {vals} |")
         return 0
     print(__doc__)
     return 2

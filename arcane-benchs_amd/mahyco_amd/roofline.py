@@ -79,11 +79,26 @@ def last_node_sweep_extra(n_nodes, word=8):
 TRAFFIC_PROFILE = "triple_point_1792x768_fp64_hbm_traffic.md"
 
 
+# The fused step alternates the sweep order (x then y, y then x): its first
+# cell sweep is cell_sweep_x_lag on one step and cell_sweep_y_lag on the next,
+# likewise for the last; each of these pairs runs once per step on average.
+FUSED_PAIRS = (("cell_sweep_x_lag", "cell_sweep_y_lag"), ("cell_sweep_x_final", "cell_sweep_y_final"))
+
+
 def measured_step_bytes(profiles_dir):
-    """HBM bytes of one fused triple-point step (every step kernel launches
-    once per step), summed from the committed traffic profile; None if absent."""
+    """HBM bytes of one triple-point step from the committed traffic profile
+    (the kernels that launch once per step, plus the mean of each fused pair
+    of the alternating sweep order); None if absent."""
     tab = _traffic_table(profiles_dir)
-    return sum(tab.values()) if tab else None
+    if not tab:
+        return None
+    paired = {k for pair in FUSED_PAIRS for k in pair}
+    tot = sum(v for k, v in tab.items() if k not in paired)
+    for pair in FUSED_PAIRS:
+        got = [tab[k] for k in pair if k in tab]
+        if got:
+            tot += sum(got) / len(got)
+    return tot
 
 
 def _traffic_table(profiles_dir):
