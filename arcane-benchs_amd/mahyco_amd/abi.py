@@ -56,7 +56,9 @@ class Options(C.Structure):
         ("ordre_projection", C.c_int32), ("projection_limiteur_id", C.c_int32),
         ("projection_pente_borne", C.c_int32), ("conservation_energie_totale", C.c_int32),
         ("visc_quad_coeff", C.c_double), ("precision", C.c_int32),
-        ("land_on_final_time", C.c_int32), ("reserved", C.c_int32 * 6),
+        ("land_on_final_time", C.c_int32),
+        ("projection_pente_borne_mixte", C.c_int32), ("projection_limiteur_pure_id", C.c_int32),
+        ("projection_pente_borne_debar_fix", C.c_int32), ("reserved", C.c_int32 * 3),
     ]
 
 

@@ -55,10 +55,12 @@ REMAP_OPTIONS = {
     "conservation-energie-totale": ("conservation_energie_totale", "bool"),
     "projection-limiteur-id": ("projection_limiteur_id", "int"),
     "projection-pente-borne": ("projection_pente_borne", "bool"),
+    "projection-pente-borne-mixte": ("projection_pente_borne_mixte", "bool"),
+    "projection-limiteur-pure-id": ("projection_limiteur_pure_id", "int"),
+    "projection-pente-borne-debar-fix": ("projection_pente_borne_debar_fix", "int"),
 }
 # remap options the library runs only at their default value
-REMAP_DEFAULT_ONLY = {"projection-pente-borne-mixte": ("bool", 0), "projection-limiteur-pure-id": ("int", 0),
-                      "projection-pente-borne-debar-fix": ("int", 0), "calcul_flux_masse": ("int", 0)}
+REMAP_DEFAULT_ONLY = {"calcul_flux_masse": ("int", 0)}
 SKIPPED = {"material", "cartesian-sort-faces", "geometry"}
 
 

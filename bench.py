@@ -46,17 +46,22 @@ def set_timing(sim, on):
     sim.L.optional["kernel_times"](sim.ctx, 1 if on else 0, None, None, None, 0)
 
 
-def cpu_baseline(gpu_density, steps, steps_1thread, sod_gpu=None, fp32_density=None):
-    """Oracle on the same triple point for `steps` steps on all host threads
-    (the accuracy reference of the GPU runs after the same steps), then one
-    thread for `steps_1thread` steps; and, if sod_gpu (the GPU's Sod run) is
-    given, the whole Sod case on all threads and on one."""
+def oracle_lib():
+    """The C++ oracle, loaded as the tests load it (the checker of the GPU runs)."""
     from importlib import util
     spec = util.spec_from_file_location("oracle_lib", os.path.join(ROOT, "tests", "helpers.py"))
     H = util.module_from_spec(spec)
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     spec.loader.exec_module(H)
-    L = H.oracle()
+    return H.oracle()
+
+
+def cpu_baseline(gpu_density, steps, steps_1thread, sod_gpu=None, fp32_density=None):
+    """Oracle on the same triple point for `steps` steps on all host threads
+    (the accuracy reference of the GPU runs after the same steps), then one
+    thread for `steps_1thread` steps; and, if sod_gpu (the GPU's Sod run) is
+    given, the whole Sod case on all threads and on one."""
+    L = oracle_lib()
     threads = L.optional["set_threads"](0)
     o = cases.triple_point(L, NX, NY)
     t0 = time.perf_counter()
@@ -228,6 +233,45 @@ def p3d_run(gpu, args, sync):
     out["fp32_error_vs_fp64"] = err
     out["roofline_frac_fp64"] = round(out["fp64"]["value"] * roofline.bytes_per_zone_cycle(1, dim=3)
                                       / (roofline.PEAK_HBM_GBS * 1e9), 4)
+    # short untimed runs from the same fields, compared with the oracle by p3d_oracle
+    short = {}
+    if args.p3d_oracle_steps and not args.no_cpu_baseline:
+        for name, prec in [("fp64", abi.PREC_FP64), ("fp32", abi.PREC_FP32_BULK)]:
+            s = cases.p3d(gpu, n, fields=fields, precision=prec)
+            s.run(args.p3d_oracle_steps)
+            short[name] = s.download(P3D_FIELDS)
+            s.close()
+    return out, fields, short
+
+
+P3D_FIELDS = ["density", "internal_energy", "pressure", "velocity"]
+
+
+def p3d_oracle(n, fields, steps, short):
+    """The oracle (all host threads) on the 3D random-field case for `steps`
+    steps from the same fields; per-field relative L1 / Linf of the GPU's
+    FP64 and FP32-bulk runs of the same steps against it."""
+    L = oracle_lib()
+    threads = L.optional["set_threads"](0)
+    o = cases.p3d(L, n, fields=fields)
+    t0 = time.perf_counter()
+    for k in range(steps):
+        o.run(1)
+        print(f"oracle {n}^3: {k + 1}/{steps} steps, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    el = time.perf_counter() - t0
+    ref = o.download(P3D_FIELDS)
+    o.close()
+    out = {"after_steps": steps, "oracle_s": round(el, 1), "oracle_threads": threads,
+           "oracle_zone_cycles_per_s": n ** 3 * steps / el}
+    for name, st in short.items():
+        assert st["global_iteration"] == steps
+        err = {}
+        for f in P3D_FIELDS:
+            a = np.asarray(st[f], dtype=np.float64)
+            b = np.asarray(ref[f], dtype=np.float64)
+            err[f] = {"l1": float(np.sum(np.abs(a - b)) / np.sum(np.abs(b))),
+                      "linf": float(np.max(np.abs(a - b)) / np.max(np.abs(b)))}
+        out[name] = err
     return out
 
 
@@ -312,6 +356,8 @@ def main():
     ap.add_argument("--p3d-n", type=int, default=256, help="cells per side of the 3D random-field case")
     ap.add_argument("--p3d-steps", type=int, default=200)
     ap.add_argument("--no-p3d", action="store_true", help="skip the 3D random-field case")
+    ap.add_argument("--p3d-oracle-steps", type=int, default=3,
+                    help="steps of the 3D runs compared with the oracle (0: none; needs the CPU baseline)")
     ap.add_argument("--batch-n", type=int, default=1024, help="problems of the Riemann batch")
     ap.add_argument("--no-batch", action="store_true", help="skip the batching form")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned form")
@@ -370,7 +416,7 @@ def main():
         p32["l1_density_err_vs_fp64"] = float(np.sum(np.abs(a - r)) / np.sum(np.abs(r)))
         p32["linf_density_err_vs_fp64"] = float(np.max(np.abs(a - r)) / np.max(np.abs(r)))
         p32["after_steps"] = args.accuracy_steps
-    p3 = None if args.no_p3d else p3d_run(gpu, args, sync)
+    p3, p3_fields, p3_short = (None, None, None) if args.no_p3d else p3d_run(gpu, args, sync)
     pb = None if args.no_batch else batch_run(gpu, args, sync)
     pl = None if args.no_learned else learned_run(gpu, args)
     if pb is not None:
@@ -405,6 +451,8 @@ def main():
         if p32 is not None:
             p32["l1_density_err_vs_oracle"] = acc["fp32"]
             p32["linf_density_err_vs_oracle"] = acc["fp32_linf"]
+        if p3_short:
+            p3["vs_oracle"] = p3d_oracle(args.p3d_n, p3_fields, args.p3d_oracle_steps, p3_short)
 
     out = {
         "metric": "zone-cycles/s on 1792x768 triple-point; L1 density err vs FP64 oracle",

@@ -128,7 +128,9 @@ typedef struct {
   int32_t is_euler_scheme;            /* must be 1     */
   int32_t ordre_projection;           /* 1, 2 or 3 (3: not learned, not in batches) */
   int32_t projection_limiteur_id;     /* MHG_LIM_MINMOD .. MHG_LIM_ULTRABEEG, or MHG_LIM_LEARNED[_ALL] */
-  int32_t projection_pente_borne;     /* must be 0     */
+  int32_t projection_pente_borne;     /* default 0; 1: pente-borne fluxes
+                                         (order 2, classical or generalised
+                                         limiter; FP64 single contexts) */
   int32_t conservation_energie_totale;/* total-energy fix-up of the remap
                                          (RemapADIFinal.cc:751-776); not in
                                          batches */
@@ -138,7 +140,14 @@ typedef struct {
   int32_t land_on_final_time; /* 1: the last step is shortened to end at final_time
                                  (0, the reference: the run stops before the step
                                  that would pass it, MahycoModule.cc:2177-2183) */
-  int32_t reserved[6];
+  /* RemapADI.axl, pente-borne options (RemapADIService.cc:319-431) */
+  int32_t projection_pente_borne_mixte;     /* default 0; 1: cells with a pure
+                                               neighbourhood along the sweep use
+                                               projection_limiteur_pure_id's
+                                               pente-borne reconstruction */
+  int32_t projection_limiteur_pure_id;      /* default 0 */
+  int32_t projection_pente_borne_debar_fix; /* default 0; 0, 1 or 2 */
+  int32_t reserved[3];
 } mhg_options;
 
 /* Full per-step state, structure-of-arrays, caller-owned host memory.

@@ -182,6 +182,12 @@ inline int classical_limiter(const mhg_options& o) {
 inline bool is_generalised(const mhg_options& o) {
   return o.projection_limiteur_id >= MHG_LIM_MINMODG && o.projection_limiteur_id <= MHG_LIM_ULTRABEEG;
 }
+// projection-pente-borne / -mixte: the generic gradient loop of
+// computeGradPhiCell (RemapADIService.cc:319-431).
+inline bool is_pb(const mhg_options& o) { return o.projection_pente_borne != 0; }
+inline bool generic_grad(const mhg_options& o) {
+  return o.ordre_projection > 1 && (o.projection_pente_borne || o.projection_pente_borne_mixte);
+}
 inline bool has_limited_gradients(const mhg_options& o) {
   return o.projection_limiteur_id < MHG_LIM_MINMODG || is_learned(o) || is_generalised(o);
 }
@@ -334,6 +340,8 @@ struct mho_ctx {
   std::vector<double> face_normal_velocity, deltax_lagrange;
   std::vector<double> grad_phi_face, phi_face;  // [nf][nvar]
   std::vector<double> h_cell;                   // [nc] cell width along the sweep
+  // pente-borne: integrated back / front face fluxes of each cell [nc][nvar]
+  std::vector<double> delta_phi_face_ar, delta_phi_face_av;
   // boundary conditions: node lists per bc
   std::vector<std::vector<long>> bc_nodes;
   // scalars
@@ -977,10 +985,31 @@ void computeVariablesForRemap(mho_ctx& S) {
         U[3 * ne + 3] = 0.;
         U[3 * ne + 4] = Vk * S.pseudo_viscosity.env(k, c);
       }
+  if (is_pb(S.opt)) {
+    // computeVariablesForRemap, PrepareRemap.cc:159-215: each environment of
+    // the cell, in ascending order, writes its fraction, density, energy and
+    // viscosity (the last one's viscosity stays); an environment absent from
+    // the cell leaves its slots as the previous remap left them
 #pragma omp parallel for schedule(static)
-  for (long c = 0; c < S.NC; ++c)
-    for (int v = 0; v < nv; ++v)
-      S.phi_lagrange[c * nv + v] = S.u_lagrange[c * nv + v] / S.cell_volume.g[c];
+    for (long c = 0; c < S.NC; ++c)
+      for (int k = 0; k < ne; ++k) {
+        if (!(S.env_mask[c] & (1 << k))) continue;
+        double* phi = &S.phi_lagrange[c * nv];
+        phi[k] = S.EV(S.fracvol, k, c);
+        phi[ne + k] = S.EV(S.density, k, c);
+        phi[2 * ne + k] = S.EV(S.internal_energy, k, c);
+        phi[3 * ne + 0] = 0.;
+        phi[3 * ne + 1] = 0.;
+        phi[3 * ne + 2] = 0.;
+        phi[3 * ne + 3] = 0.;
+        phi[3 * ne + 4] = S.EV(S.pseudo_viscosity, k, c);
+      }
+  } else {
+#pragma omp parallel for schedule(static)
+    for (long c = 0; c < S.NC; ++c)
+      for (int v = 0; v < nv; ++v)
+        S.phi_lagrange[c * nv + v] = S.u_lagrange[c * nv + v] / S.cell_volume.g[c];
+  }
 #pragma omp parallel for schedule(static)
   for (long nd = 0; nd < S.NN; ++nd) {
     const R3& u = S.velocity[nd];
@@ -1081,9 +1110,14 @@ void computeCellWidths(mho_ctx& S, int idir) {
 // Generalised limiters: the generic loop 349-396 with computeAndLimitGradPhi
 // (UtilesRemap.cc:306-339), whose back / front cell is the cell itself on the
 // mesh boundary.
+void computeGradPhiCellGeneric(mho_ctx& S, int idir);
 void computeGradPhiCell(mho_ctx& S, int idir) {
   const int nv = S.nvar;
   std::fill(S.grad_phi.begin(), S.grad_phi.end(), 0.);
+  if (generic_grad(S.opt)) {
+    computeGradPhiCellGeneric(S, idir);
+    return;
+  }
   if (S.opt.ordre_projection > 1 && S.opt.projection_pente_borne == 0 &&
       is_learned(S.opt)) {
 #pragma omp parallel for schedule(static)
@@ -1205,6 +1239,286 @@ inline double o3_flux(double ymmm, double ymm, double ym, double yp, double ypp,
   if (flux < la) flux = la;
   if (flux > lb) flux = lb;
   return flux;
+}
+
+// ---- pente-borne (UtilesRemap.cc:73-814) -------------------------------
+// Arcane's math::min / math::max (amin / amax) and std::max (smax), as the
+// reference calls them; abs of a Real is the floating-point absolute value.
+
+// INTY (UtilesRemap.cc:83-94): integral from -infinity to X of the linear
+// function through (x0, y0) and (x1, y1) restricted to [x0, x1]; zero for a
+// degenerate interval (|x1 - x0| <= threshold).
+inline double pb_inty(double X, double x0, double y0, double x1, double y1, double thr) {
+  double flux = 0.;
+  if (std::fabs(x1 - x0) > thr) {
+    const double eta = amin(amax(0., (X - x0) / (x1 - x0)), 1.);
+    flux = (y0 + 0.5 * eta * (y1 - y0)) * (x1 - x0) * eta;
+  }
+  return flux;
+}
+
+// computeY0 (UtilesRemap.cc:184-240): the monotonicity thresholds y0plus /
+// y0moins of the single-slope reconstruction for limiter lim (zero for an
+// identifier without a rule).
+inline void pb_y0(int lim, double yp, double ym, double h0, double hp, double hm, double* y0plus,
+                  double* y0moins) {
+  double a = 0., b = 0.;
+  if (lim == MHG_LIM_MINMODG || lim == MHG_LIM_MINMOD) {
+    a = yp;
+    b = ym;
+  } else if (lim == MHG_LIM_SUPERBEEG || lim == MHG_LIM_SUPERBEE) {
+    a = ((h0 + hm) * yp + h0 * ym) / (2 * h0 + hm);
+    b = ((h0 + hp) * ym + h0 * yp) / (2 * h0 + hp);
+  } else if (lim == MHG_LIM_VANLEERG || lim == MHG_LIM_VANLEER) {
+    const double lo = amin(yp, ym), hi = amax(yp, ym);
+    double xp = (h0 * h0 + 3 * h0 * hm + 2 * hm * hm) * yp;
+    double xm = (h0 * h0 + 3 * h0 * hp + 2 * hp * hp) * ym;
+    xp += (h0 * h0 - h0 * hp - 2 * hp * hp + 2 * h0 * hm) * ym;
+    xm += (h0 * h0 - h0 * hm - 2 * hm * hm + 2 * h0 * hp) * yp;
+    xp /= (2 * h0 * h0 + 5 * h0 * hm + 2 * hm * hm - h0 * hp - 2 * hp * hp);
+    xm /= (2 * h0 * h0 + 5 * h0 * hp + 2 * hp * hp - h0 * hm - 2 * hm * hm);
+    a = amin(amax(xp, lo), hi);
+    b = amin(amax(xm, lo), hi);
+  } else if (lim == MHG_LIM_ULTRABEEG) {
+    a = (yp + ym) / 2.;
+    b = (yp + ym) / 2.;
+  } else if (lim == MHG_LIM_ARITHMETICG) {
+    a = ((h0 + hm + hp) * yp + h0 * ym) / (2 * h0 + hm + hp);
+    b = ((h0 + hm + hp) * ym + h0 * yp) / (2 * h0 + hm + hp);
+  }
+  *y0plus = a;
+  *y0moins = b;
+}
+
+// computexgxd and computeygyd (UtilesRemap.cc:252-305): the abscissae xg, xd
+// (cell centre at 0, width h0) and ordinates yg, yd of the inner support
+// points of the three-piece reconstruction through (-h0/2, ym) and (h0/2, yp).
+inline void pb_support(double y0, double yp, double ym, double h0, double y0plus, double y0moins, double grady,
+                       double thr, double* xg, double* yg, double* xd, double* yd) {
+  double xplus = 1.;
+  if (std::fabs(y0plus - yp) > thr) xplus = (y0 - yp) / (y0plus - yp) - 1. / 2.;
+  double xmoins = 1.;
+  if (std::fabs(y0moins - ym) > thr) xmoins = (y0 - ym) / (y0moins - ym) - 1. / 2.;
+  *xd = +h0 * amin(amax(xplus, -1. / 2.), 1. / 2.);
+  *xg = -h0 * amin(amax(xmoins, -1. / 2.), 1. / 2.);
+  const double xtd = y0 + h0 / 2 * grady;
+  const double xtg = y0 - h0 / 2 * grady;
+  const double ad = amin(yp, 2. * y0moins - ym), bd = amax(yp, 2. * y0moins - ym);
+  const double ag = amin(ym, 2. * y0plus - yp), bg = amax(ym, 2. * y0plus - yp);
+  *yd = amin(amax(xtd, ad), bd);
+  *yg = amin(amax(xtg, ag), bg);
+}
+
+// One variable's pente-borne flux of computeFluxPP / computeFluxPPPure
+// (UtilesRemap.cc:420-530): the integral of the reconstruction over the
+// length ppv (the outgoing volume per unit face length) next to the back
+// face (type 0) or the front face (type 1), floored at zero; the first-order
+// value y0 ppv at a local extremum (formula 16).
+inline double pb_var_flux(int type, double ppv, double h0, double y0, double yp, double ym, double xg, double yg,
+                          double xd, double yd, double thr) {
+  double f1, f1m, f2, f2m, f3, f3m;
+  if (type == 0) {
+    f1m = pb_inty(-h0 / 2. + ppv, -h0 / 2., ym, xg, yg, thr);
+    f1 = pb_inty(-h0 / 2., -h0 / 2., ym, xg, yg, thr);
+    f2m = pb_inty(-h0 / 2. + ppv, xg, yg, xd, yd, thr);
+    f2 = pb_inty(-h0 / 2., xg, yg, xd, yd, thr);
+    f3m = pb_inty(-h0 / 2. + ppv, xd, yd, h0 / 2., yp, thr);
+    f3 = pb_inty(-h0 / 2., xd, yd, h0 / 2., yp, thr);
+  } else {
+    f1 = pb_inty(h0 / 2. - ppv, -h0 / 2., ym, xg, yg, thr);
+    f1m = pb_inty(h0 / 2., -h0 / 2., ym, xg, yg, thr);
+    f2 = pb_inty(h0 / 2. - ppv, xg, yg, xd, yd, thr);
+    f2m = pb_inty(h0 / 2., xg, yg, xd, yd, thr);
+    f3 = pb_inty(h0 / 2. - ppv, xd, yd, h0 / 2., yp, thr);
+    f3m = pb_inty(h0 / 2., xd, yd, h0 / 2., yp, thr);
+  }
+  double F = smax(((f1m - f1) + (f2m - f2)) + (f3m - f3), 0.);
+  if (((yp - y0) * (ym - y0)) >= 0.) F = y0 * ppv;
+  return F;
+}
+
+// computeFluxPP (pure = false, UtilesRemap.cc:406-603) and computeFluxPPPure
+// (pure = true, 628-814) for one face of cell c: F[nv] from the cell's and
+// its back / front neighbours' phi, widths and the cell's limited gradient,
+// then the debar fix (1: environment mass and energy fluxes scaled by the
+// volume flux, 2: taken from the cell's density and energy; the pure form
+// fixes energies only) and zero for a face without outgoing volume.  The
+// dual flux the reference also forms here is overwritten by computeUremap
+// before any use, so it is not computed.
+void pb_face_flux(mho_ctx& S, long c, long fc, long bc, double v_out, int type, bool pure, double* F) {
+  const int nv = S.nvar, ne = S.nenv;
+  const double thr = S.opt.remap_threshold, dt = S.global_deltat;
+  const int lim = pure ? S.opt.projection_limiteur_pure_id : S.opt.projection_limiteur_id;
+  const double ppv = 0.5 * (v_out + std::fabs(v_out)) * dt;
+  const double* P = S.phi_lagrange.data();
+  const double h0 = S.h_cell[c], hp = S.h_cell[fc], hm = S.h_cell[bc];
+  for (int v = 0; v < nv; ++v) {
+    const double y0 = P[c * nv + v], yp = P[fc * nv + v], ym = P[bc * nv + v];
+    double y0plus, y0moins, xg, yg, xd, yd;
+    pb_y0(lim, yp, ym, h0, hp, hm, &y0plus, &y0moins);
+    pb_support(y0, yp, ym, h0, y0plus, y0moins, S.grad_phi[c * nv + v], thr, &xg, &yg, &xd, &yd);
+    F[v] = pb_var_flux(type, ppv, h0, y0, yp, ym, xg, yg, xd, yd, thr);
+  }
+  const int fix = S.opt.projection_pente_borne_debar_fix;
+  if (fix == 1) {
+    double sfm = 0., sfv = 0.;
+    for (int k = 0; k < ne; ++k) sfv += F[k];
+    if (std::fabs(sfv) > thr) {
+      for (int k = 0; k < ne; ++k) {
+        if (!pure) F[ne + k] = (F[ne + k] / sfv) * F[k];
+        F[2 * ne + k] = (F[2 * ne + k] / sfv) * F[ne + k];
+        sfm += F[ne + k];
+      }
+      F[3 * ne] = (F[3 * ne] / sfv) * sfm;
+      F[3 * ne + 1] = (F[3 * ne + 1] / sfv) * sfm;
+      F[3 * ne + 2] = (F[3 * ne + 2] / sfv) * sfm;
+      F[3 * ne + 3] = P[c * nv + 3 * ne + 3] * sfv;
+    } else {
+      for (int v = 0; v < nv; ++v) F[v] = 0.;
+    }
+  } else if (fix == 2) {
+    double sfm = 0., sfv = 0.;
+    for (int k = 0; k < ne; ++k) {
+      if (!pure) F[ne + k] = P[c * nv + ne + k] * F[k];
+      F[2 * ne + k] = P[c * nv + 2 * ne + k] * F[ne + k];
+      sfm += F[ne + k];
+      sfv += F[k];
+    }
+    F[3 * ne] = P[c * nv + 3 * ne] * sfm;
+    F[3 * ne + 1] = P[c * nv + 3 * ne + 1] * sfm;
+    F[3 * ne + 2] = P[c * nv + 3 * ne + 2] * sfm;
+    F[3 * ne + 3] = P[c * nv + 3 * ne + 3] * sfv;
+  }
+  if (ppv == 0.)
+    for (int v = 0; v < nv; ++v) F[v] = 0.;
+}
+
+// computeGradPhiCell's generic loop (RemapADIService.cc:319-431) for
+// projection-pente-borne or -mixte: per cell, its back / front neighbours
+// along idir (the cell itself on the mesh boundary), the limiter (the pure
+// one where projection-limiteur-pure-id is 1 and the neighbourhood is pure
+// under -mixte), the limited gradient (computeAndLimitGradPhi,
+// UtilesRemap.cc:306-339), and with pente-borne the integrated fluxes of the
+// back face (outgoing velocity -u_back) and of the front face (u_front).
+void computeGradPhiCellGeneric(mho_ctx& S, int idir) {
+  const int nv = S.nvar, nfc = 2 * S.dim;
+  computeCellWidths(S, idir);
+  std::fill(S.delta_phi_face_ar.begin(), S.delta_phi_face_ar.end(), 0.);
+  std::fill(S.delta_phi_face_av.begin(), S.delta_phi_face_av.end(), 0.);
+  const R3 dirproj(0.5 * (1 - idir) * (2 - idir), 1.0 * idir * (2 - idir), -0.5 * idir * (1 - idir));
+  const bool mixte = S.opt.projection_pente_borne_mixte != 0;
+  const int pure_id = S.opt.projection_limiteur_pure_id;
+#pragma omp parallel for schedule(static)
+  for (long c = 0; c < S.NC; ++c) {
+    long bf, ff, pb, nb, pf, nf, cf[6];
+    cellDirFaces(S, c, idir, &bf, &ff);
+    const long bc = isInnerFaceOfDir(S, bf, idir, &pb, &nb) ? pb : c;
+    const long fc = isInnerFaceOfDir(S, ff, idir, &pf, &nf) ? nf : c;
+    S.cellFaces(c, cf);
+    int ib = 0, iff = 0;
+    for (int q = 0; q < nfc; ++q) {
+      if (cf[q] == bf) ib = q;
+      if (cf[q] == ff) iff = q;
+    }
+    const bool vp = mixte && S.est_mixte[c] == 0 && S.est_mixte[fc] == 0 && S.est_mixte[bc] == 0 &&
+                    S.est_pure[c] == S.est_pure[fc] && S.est_pure[c] == S.est_pure[bc];
+    int lim = S.opt.projection_limiteur_id;
+    if (pure_id == 1 && vp) lim = pure_id;
+    for (int v = 0; v < nv; ++v) {
+      const double gb = S.grad_phi_face[bf * nv + v], gf = S.grad_phi_face[ff * nv + v];
+      double g = 0.;
+      if (lim < MHG_LIM_MINMODG) {
+        if (gb != 0.) g += 0.5 * (limiter(lim, gf / gb) * gb);
+        if (gf != 0.) g += 0.5 * (limiter(lim, gb / gf) * gf);
+      } else {
+        g = flux_limiter_g(lim, gf, gb, S.phi_lagrange[fc * nv + v], S.phi_lagrange[bc * nv + v], S.h_cell[c],
+                           S.h_cell[fc], S.h_cell[bc]);
+      }
+      S.grad_phi[c * nv + v] = g;
+    }
+    if (is_pb(S.opt)) {
+      const double v_ar = dot(S.outer_face_normal[c * nfc + ib], dirproj) * S.face_normal_velocity[bf];
+      const double v_av = dot(S.outer_face_normal[c * nfc + iff], dirproj) * S.face_normal_velocity[ff];
+      pb_face_flux(S, c, fc, bc, v_ar, 0, vp, &S.delta_phi_face_ar[c * nv]);
+      pb_face_flux(S, c, fc, bc, v_av, 1, vp, &S.delta_phi_face_av[c * nv]);
+    }
+  }
+}
+
+// computeUpwindFaceQuantitiesForProjection with pente-borne
+// (RemapADIService.cc:549-581): the net integrated flux through each inner
+// face, the back cell's front-face flux minus the front cell's back-face flux.
+void computeUpwindFaceQuantitiesPB(mho_ctx& S, int idir) {
+  const int nv = S.nvar;
+  std::fill(S.phi_face.begin(), S.phi_face.end(), 0.);
+#pragma omp parallel for schedule(static)
+  for (long q = 0; q < S.nfd[idir]; ++q) {
+    long f = S.foff[idir] + q, b, nf;
+    if (!isInnerFaceOfDir(S, f, idir, &b, &nf)) continue;
+    for (int v = 0; v < nv; ++v)
+      S.phi_face[f * nv + v] = S.delta_phi_face_av[b * nv + v] - S.delta_phi_face_ar[nf * nv + v];
+  }
+}
+
+// computeUremap with pente-borne (RemapADIService.cc:733-872): face flux
+// n.e_dir L phi_face (phi_face is already the integral), dual flux as half
+// the signed sum, non-negative masses and energies, then phi in the
+// pente-borne form (fraction, density, velocity from the cell momentum,
+// specific energy; the kinetic-energy slot with the total-energy fix-up;
+// the viscosity slot kept) and the mixed / pure codes.
+void computeUremapPB(mho_ctx& S, int idir) {
+  const int nv = S.nvar, ne = S.nenv, nfc = 2 * S.dim;
+  const R3 dirproj(0.5 * (1 - idir) * (2 - idir), 1.0 * idir * (2 - idir), -0.5 * idir * (1 - idir));
+  std::fill(S.dual_phi_flux.begin(), S.dual_phi_flux.end(), 0.);
+#pragma omp parallel for schedule(static)
+  for (long c = 0; c < S.NC; ++c) {
+    long cf[6];
+    S.cellFaces(c, cf);
+    double flux_face[3 * MHG_MAX_ENV + 8];
+    for (int v = 0; v < nv; ++v) flux_face[v] = 0.;
+    for (int index = 0; index < nfc; ++index) {
+      const long f = cf[index];
+      if (std::fabs(dot(S.face_normal[f], dirproj)) >= 1.0E-10) {
+        const double face_length = S.face_length_lagrange[f][idir];
+        const R3& ofn = S.outer_face_normal[c * nfc + index];
+        const double ofn_dir = dot(ofn, dirproj);
+        for (int v = 0; v < nv; ++v) {
+          const double flux = ofn_dir * face_length * S.phi_face[f * nv + v];
+          flux_face[v] += flux;
+          S.dual_phi_flux[c * nv + v] += 0.5 * flux * ofn[idir];
+        }
+      }
+    }
+    double* U = &S.u_lagrange[c * nv];
+    for (int v = 0; v < nv; ++v) U[v] -= flux_face[v];
+    for (int k = 0; k < ne; ++k) {
+      if (U[ne + k] < 0.) U[ne + k] = 0.;
+      if (U[2 * ne + k] < 0.) U[2 * ne + k] = 0.;
+    }
+    double somme_volume = 0.;
+    for (int k = 0; k < ne; ++k) somme_volume += U[k];
+    double* phi = &S.phi_lagrange[c * nv];
+    double somme_masse = 0.;
+    for (int k = 0; k < ne; ++k) {
+      phi[k] = U[k] / somme_volume;
+      phi[ne + k] = (U[k] != 0.) ? U[ne + k] / U[k] : 0.;
+      somme_masse += U[ne + k];
+    }
+    if (somme_masse != 0) {
+      phi[3 * ne] = U[3 * ne] / somme_masse;
+      phi[3 * ne + 1] = U[3 * ne + 1] / somme_masse;
+    }
+    for (int k = 0; k < ne; ++k) phi[2 * ne + k] = (U[ne + k] != 0.) ? U[2 * ne + k] / U[ne + k] : 0.;
+    if (S.opt.conservation_energie_totale) phi[3 * ne + 2] = U[3 * ne + 2] / somme_masse;
+    int imatcell = 0, imatpure = -1;
+    for (int k = 0; k < ne; ++k)
+      if (phi[k] > 0.) {
+        imatcell++;
+        imatpure = k;
+      }
+    S.est_mixte[c] = imatcell > 1 ? 1 : 0;
+    S.est_pure[c] = imatcell > 1 ? -1 : imatpure;
+  }
 }
 
 // Order-3 face values (RemapADIService.cc:583-640): six cells along idir
@@ -1679,8 +1993,10 @@ void appliRemap(mho_ctx& S) {
     computeGradPhiFace(S, idir);
     computeGradPhiCell(S, idir);
     if (S.opt.ordre_projection == 3) computeUpwindFaceQuantities3(S, idir);
+    else if (is_pb(S.opt)) computeUpwindFaceQuantitiesPB(S, idir);
     else computeUpwindFaceQuantities(S, idir);
-    computeUremap(S, idir);
+    if (is_pb(S.opt)) computeUremapPB(S, idir);
+    else computeUremap(S, idir);
     computeDualUremap(S, idir);
   }
   S.sens_projection = (S.sens_projection + 1) % S.dim;
@@ -1821,6 +2137,8 @@ void allocAll(mho_ctx& S) {
   S.grad_phi_face.assign((size_t)NF * nv, 0.);
   S.phi_face.assign((size_t)NF * nv, 0.);
   S.h_cell.assign(NC, 0.);
+  S.delta_phi_face_ar.assign((size_t)NC * nv, 0.);
+  S.delta_phi_face_av.assign((size_t)NC * nv, 0.);
 }
 
 // Mesh generation and the geometry-only part of hydroStartInit.
@@ -1861,6 +2179,11 @@ void resetPhysics(mho_ctx& S) {
   std::fill(S.dual_grad_phi.begin(), S.dual_grad_phi.end(), 0.);
   std::fill(S.grad_phi_face.begin(), S.grad_phi_face.end(), 0.);
   std::fill(S.env_mask.begin(), S.env_mask.end(), 0);
+  // the remap's cell variables as Arcane creates them (zero): pente-borne
+  // reads phi of environments absent from a cell, -mixte the mixed / pure codes
+  std::fill(S.phi_lagrange.begin(), S.phi_lagrange.end(), 0.);
+  std::fill(S.est_mixte.begin(), S.est_mixte.end(), 0);
+  std::fill(S.est_pure.begin(), S.est_pure.end(), 0);
   S.global_deltat = S.opt.deltat_init;
   S.global_old_deltat = S.old_deltat = S.global_time = 0.;
   S.global_iteration = 0;
@@ -1921,8 +2244,19 @@ int mho_create(const mhg_mesh_desc* mesh, const mhg_options* opt, mho_ctx** out)
   if (opt->nb_env < 1 || opt->nb_env > MHG_MAX_ENV || opt->nb_bc < 0 || opt->nb_bc > MHG_MAX_BC)
     return MHG_E_ARG;
   if (opt->schema_csts || opt->with_newton || opt->sans_lagrange || !opt->is_euler_scheme ||
-      opt->projection_pente_borne || !has_limited_gradients(*opt) ||
-      opt->projection_limiteur_id < 0 || opt->ordre_projection < 1 || opt->ordre_projection > 3)
+      !has_limited_gradients(*opt) || opt->projection_limiteur_id < 0 || opt->ordre_projection < 1 ||
+      opt->ordre_projection > 3)
+    return MHG_E_UNSUPPORTED;
+  // pente-borne: order 2 (at order 1 the reference's generic face loop moves
+  // nothing, at order 3 it mixes a face value with the integrated-flux
+  // formula), classical or generalised limiters, debar fix 0..2
+  if ((opt->projection_pente_borne || opt->projection_pente_borne_mixte) &&
+      (is_learned(*opt) || opt->projection_limiteur_pure_id < 0 ||
+       opt->projection_limiteur_pure_id > MHG_LIM_ULTRABEEG))
+    return MHG_E_UNSUPPORTED;
+  if (opt->projection_pente_borne &&
+      (opt->ordre_projection != 2 || opt->projection_pente_borne_debar_fix < 0 ||
+       opt->projection_pente_borne_debar_fix > 2))
     return MHG_E_UNSUPPORTED;
   for (int b = 0; b < opt->nb_bc; ++b)
     if (opt->bc[b].surface < 0 || opt->bc[b].surface > 5 || opt->bc[b].type < 0 ||
@@ -2056,6 +2390,9 @@ int mho_upload_state(mho_ctx* ctx, const mhg_state* st) {
   if (st->env_mask)
     for (long c = 0; c < NC; ++c) S.env_mask[c] = st->env_mask[c];
   computeEnvId(S);
+  // pente-borne's phi of absent environments is not part of the state: an
+  // upload starts it from zero
+  std::fill(S.phi_lagrange.begin(), S.phi_lagrange.end(), 0.);
   struct P { double* src; double* srce; MatVar* mv; };
   P ps[] = {{st->density, st->density_env, &S.density},
             {st->internal_energy, st->internal_energy_env, &S.internal_energy},
@@ -2354,6 +2691,16 @@ double mho_probe_cqs(int32_t dim, const double* coord, double* out_cqs) {
   return volume * (1. / dim);
 }
 double mho_probe_limiter(int32_t id, double r) { return limiter(id, r); }
+double mho_probe_pb_inty(double X, double x0, double y0, double x1, double y1, double thr) {
+  return pb_inty(X, x0, y0, x1, y1, thr);
+}
+double mho_probe_pb_flux(int32_t type, int32_t lim, double ppv, double h0, double hp, double hm, double y0,
+                         double yp, double ym, double grady, double thr) {
+  double y0plus, y0moins, xg, yg, xd, yd;
+  pb_y0(lim, yp, ym, h0, hp, hm, &y0plus, &y0moins);
+  pb_support(y0, yp, ym, h0, y0plus, y0moins, grady, thr, &xg, &yg, &xd, &yd);
+  return pb_var_flux(type, ppv, h0, y0, yp, ym, xg, yg, xd, yd, thr);
+}
 
 int mho_probe_cell_widths(mho_ctx* c, int32_t idir, double* out) {
   if (!c || !out || idir < 0 || idir >= c->dim) return MHG_E_ARG;

@@ -87,6 +87,13 @@ double mho_probe_limiter_g(int32_t id, double gp, double gm, double yp, double y
  * the values and widths of the cells bbb, bb, b | f, ff, fff around the face,
  * vdt the face velocity times dt. */
 double mho_probe_flux3(const double* y6, const double* h6, double vdt);
+/* Pente-borne (UtilesRemap.cc:73-603): INTY, and one variable's integrated
+ * face flux (computeY0 with limiter lim, computexgxd / computeygyd with the
+ * cell's limited gradient grady, the type-0 (back face) or type-1 (front
+ * face) integral over ppv, formula 16); thr the remap threshold. */
+double mho_probe_pb_inty(double X, double x0, double y0, double x1, double y1, double thr);
+double mho_probe_pb_flux(int32_t type, int32_t lim, double ppv, double h0, double hp, double hm, double y0,
+                         double yp, double ym, double grady, double thr);
 /* Cell widths along idir (RemapADIService.cc:223-255) of the present
  * coordinates, out [N_c]. */
 int mho_probe_cell_widths(mho_ctx* ctx, int32_t idir, double* out);

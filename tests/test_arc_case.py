@@ -90,7 +90,7 @@ def test_older_layout_and_2d():
 
 
 @pytest.mark.parametrize("change,what", [
-    ("<ordre-projection>1</ordre-projection>", "<projection-pente-borne-mixte>true</projection-pente-borne-mixte>"),
+    ("<ordre-projection>1</ordre-projection>", "<calcul_flux_masse>1</calcul_flux_masse>"),
     ("prx='1.0'", "prx='1.1'"),
     ("StiffenedGas", "MieGruneisen"),
     ("<cas-test>3</cas-test>", "<cas-test>7</cas-test>"),

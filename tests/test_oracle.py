@@ -221,8 +221,12 @@ def test_dt_min_abort_and_stop(L):
 
 def test_unsupported_options_rejected(L):
     m = abi.mesh_desc(2, [4, 4], [0, 0], [1, 1])
-    for kw in [dict(schema_csts=1), dict(projection_pente_borne=1), dict(projection_limiteur_id=8),
-               dict(ordre_projection=4)]:
+    for kw in [dict(schema_csts=1), dict(projection_limiteur_id=8), dict(ordre_projection=4),
+               # pente-borne: order 2 only, not with the learned limiter, debar fix 0..2
+               dict(projection_pente_borne=1, ordre_projection=1), dict(projection_pente_borne=1, ordre_projection=3),
+               dict(projection_pente_borne=1, projection_limiteur_id=abi.LIM_LEARNED),
+               dict(projection_pente_borne=1, projection_pente_borne_debar_fix=3),
+               dict(projection_pente_borne=1, projection_limiteur_pure_id=8)]:
         with pytest.raises(abi.MahycoError) as ei:
             abi.Simulation(L, m, L.options(**kw))
         assert ei.value.code == abi.E_UNSUPPORTED
