@@ -283,7 +283,7 @@ int mhg_batch_create(int32_t n, const mhg_mesh_desc* meshes, const mhg_options* 
     const mhg_options& o = opts[p];
     if (r || o.nb_env > 3 || o.nb_env != o0.nb_env || o.precision != o0.precision ||
         o.projection_limiteur_id == MHG_LIM_LEARNED || o.projection_limiteur_id == MHG_LIM_LEARNED_ALL ||
-        o.conservation_energie_totale || o.ordre_projection == 3 || !o.with_projection ||
+        o.conservation_energie_totale || o.ordre_projection == 3 || !o.with_projection || b->P[p].generic ||
         o.projection_limiteur_id != o0.projection_limiteur_id || o.ordre_projection != o0.ordre_projection ||
         o.pseudo_centree != o0.pseudo_centree) {
       delete b;

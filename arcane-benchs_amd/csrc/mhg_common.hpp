@@ -486,6 +486,10 @@ struct Params {
   int lagonly;  // with-projection = false: no remap, the mesh keeps moving
                 // (updatePosition from the previous coordinates), the CFL
                 // has no velocity term
+  // generic cell sweeps (order 2 with projection-pente-borne or -mixte; FP64,
+  // entry-point kernels): pb the pente-borne fluxes with debar fix pb_fix,
+  // mixte the pure-neighbourhood switch to limiter pure_id
+  int generic, pb, pb_fix, mixte, pure_id;
   double cfl, dtmin, dtmax, tend;
   int nbc;
   BcRange bc[kMaxBc];

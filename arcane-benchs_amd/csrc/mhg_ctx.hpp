@@ -47,7 +47,10 @@ struct mhg_ctx {
   void *Ud0 = nullptr, *Ud1 = nullptr, *Ud3 = nullptr, *dg0 = nullptr, *dg1 = nullptr;
   void *Ud4 = nullptr, *dek = nullptr;  // total-energy fix-up only
   void* UdB4 = nullptr;                 // its 3D second dual state
-  int* est = nullptr;                   // order 3: the cells' mixed / pure codes
+  int* est = nullptr;                   // order 3, generic sweeps: the cells' mixed / pure codes
+  // generic sweeps (pente-borne / -mixte): phi kept between iterations
+  // (pente-borne), the back / front face fluxes or the gradients [3 ne + 1][nc]
+  double *phi = nullptr, *dar = nullptr, *dav = nullptr;
   // 3D: z components, a second dual state (the middle sweep reads one and
   // writes the other), the Euler volumes of the hexahedra
   double* xz[2] = {nullptr, nullptr};

@@ -184,6 +184,27 @@ void collect_times(mhg_ctx* c) {
 // sweep, whose velocities they read.
 // lag: the first cell sweep carries the Lagrange cell phase (the caller
 // launched no k_cell_lagrange_2d).
+// Arguments of the generic sweeps (FP64) of remap pass `pass`.
+GenericSweepArgs generic_args(mhg_ctx* c, int pass) {
+  GenericSweepArgs A;
+  const bool d3 = c->P.dim == 3;
+  A.x = c->xx[1 - c->xcur];
+  A.y = c->xy[1 - c->xcur];
+  A.z = d3 ? c->xz[1 - c->xcur] : nullptr;
+  A.ux = bp<double>(c->ux[c->ucur]);
+  A.uy = bp<double>(c->uy[c->ucur]);
+  A.uz = d3 ? bp<double>(c->uz[c->ucur]) : nullptr;
+  A.Uin = bp<double>(c->U[pass & 1]);
+  A.Uout = bp<double>(c->U[(pass + 1) & 1]);
+  A.Vlag = bp<double>(c->Vlag);
+  A.phi = c->phi;
+  A.dar = c->dar;
+  A.dav = c->dav;
+  A.est = c->est;
+  A.contrib = bp<double>(c->contrib);
+  return A;
+}
+
 template <class R>
 void remap_2d(mhg_ctx* c, int dt_mode, bool eos = true, bool lag = false) {
   const Params& P = c->P;
@@ -191,7 +212,10 @@ void remap_2d(mhg_ctx* c, int dt_mode, bool eos = true, bool lag = false) {
   for (int pass = 0; pass < P.dim; ++pass) {
     const int idir = (pass + c->sens) % P.dim;
     const bool first = (pass == 0);
-    if (lag && first) {
+    if (P.generic) {  // pente-borne / -mixte (FP64 contexts)
+      TimedLaunch t(c, idir == 0 ? "cell_sweep_x" : "cell_sweep_y");
+      launch_generic_sweep(c->stream, P, c->S, generic_args(c, pass), idir, first);
+    } else if (lag && first) {
       TimedLaunch t(c, idir == 0 ? "cell_sweep_x_lag" : "cell_sweep_y_lag");
       launch_cell_sweep_lag_2d(c->stream, P, c->S, cell_sweep_args<R>(c, pass, idir), cell_lag_args<R>(c), idir);
     } else if (fuse && pass == P.dim - 1) {
@@ -308,7 +332,8 @@ void remap_3d(mhg_ctx* c, int dt_mode, bool eos = true) {
     A.b.contrib = bp<R>(c->contrib);
     {
       TimedLaunch t(c, cname[idir]);
-      launch_cell_sweep_3d(c->stream, P, c->S, A, idir, first);
+      if (P.generic) launch_generic_sweep(c->stream, P, c->S, generic_args(c, pass), idir, first);
+      else launch_cell_sweep_3d(c->stream, P, c->S, A, idir, first);
     }
     NodeSweep3ArgsT<R> B;
     B.m = bp<R>(c->m);
@@ -564,6 +589,15 @@ int init_pure(mhg_ctx* c, const std::vector<int>& env, const std::vector<double>
   const size_t envb = (size_t)P.ne * NC * (size_t)c->bulk;
   void* envs[] = {c->env_rho, c->env_e, c->env_p, c->env_cs, c->env_q, c->env_m, c->env_f, c->env_mf};
   for (void* d : envs) HIPCHK(c, hipMemset(d, 0, envb));
+  // the remap's cell variables as Arcane creates them: mixed / pure codes
+  // zero, pente-borne's phi zero, a pure cell's fraction 1 (in its slot)
+  if (c->est) HIPCHK(c, hipMemset(c->est, 0, NC * sizeof(int)));
+  if (c->phi) HIPCHK(c, hipMemset(c->phi, 0, (size_t)P.nvu * NC * sizeof(double)));
+  if (P.pb) {
+    std::vector<double> f((size_t)P.ne * NC, 0.);
+    for (long cc = 0; cc < NC; ++cc) f[(size_t)env[cc] * NC + cc] = 1.0;
+    HIPCHK(c, up(c->env_f, f));
+  }
   std::memset(c->S_host, 0, sizeof(DevScalars));
   c->S_host->dt = c->opt.deltat_init;
   int r = write_scalars(c);
@@ -592,7 +626,7 @@ int build_params(const mhg_mesh_desc* mesh, const mhg_options* opt, Params* Pout
   if (opt->nb_env < 1 || opt->nb_env > MHG_MAX_ENV || opt->nb_bc < 0 || opt->nb_bc > MHG_MAX_BC)
     return MHG_E_ARG;
   if (opt->schema_csts || opt->with_newton || opt->sans_lagrange ||
-      !opt->is_euler_scheme || opt->projection_pente_borne ||
+      !opt->is_euler_scheme ||
       (opt->projection_limiteur_id > MHG_LIM_ULTRABEEG && opt->projection_limiteur_id != MHG_LIM_LEARNED &&
        opt->projection_limiteur_id != MHG_LIM_LEARNED_ALL) ||
       opt->projection_limiteur_id < 0 || opt->ordre_projection < 1 || opt->ordre_projection > 3)
@@ -607,6 +641,20 @@ int build_params(const mhg_mesh_desc* mesh, const mhg_options* opt, Params* Pout
   // the Lagrange-only mode (with-projection = false) runs the FP64
   // entry-point kernels
   if (!opt->with_projection && (opt->precision != MHG_PREC_FP64 || learned)) return MHG_E_UNSUPPORTED;
+  // pente-borne (order 2: at order 1 the reference's generic face loop moves
+  // nothing, at order 3 it mixes a face value with the integrated-flux
+  // formula) and -mixte: classical or generalised limiters, pure limiter
+  // 0..7, debar fix 0..2; the generic sweeps run in FP64 through the
+  // entry-point kernels
+  const bool pb = opt->projection_pente_borne != 0, mixte = opt->projection_pente_borne_mixte != 0;
+  if ((pb || mixte) && (learned || opt->projection_limiteur_pure_id < 0 ||
+                        opt->projection_limiteur_pure_id > MHG_LIM_ULTRABEEG))
+    return MHG_E_UNSUPPORTED;
+  if (pb && (opt->ordre_projection != 2 || opt->projection_pente_borne_debar_fix < 0 ||
+             opt->projection_pente_borne_debar_fix > 2))
+    return MHG_E_UNSUPPORTED;
+  const bool generic = opt->ordre_projection == 2 && (pb || mixte);
+  if (generic && opt->precision != MHG_PREC_FP64) return MHG_E_UNSUPPORTED;
   Params P{};
   P.dim = mesh->dim;
   P.ne = opt->nb_env;
@@ -634,6 +682,11 @@ int build_params(const mhg_mesh_desc* mesh, const mhg_options* opt, Params* Pout
   P.learned = !learned ? 0 : (opt->projection_limiteur_id == MHG_LIM_LEARNED_ALL ? 2 : 1);
   P.etot = opt->conservation_energie_totale ? 1 : 0;
   P.lagonly = opt->with_projection ? 0 : 1;
+  P.generic = generic ? 1 : 0;
+  P.pb = pb ? 1 : 0;
+  P.pb_fix = opt->projection_pente_borne_debar_fix;
+  P.mixte = mixte ? 1 : 0;
+  P.pure_id = opt->projection_limiteur_pure_id;
   P.pseudo_centree = opt->pseudo_centree;
   P.lc = opt->longueur_caracteristique;
   P.cfl = opt->cfl;
@@ -728,7 +781,12 @@ int mhg_create(const mhg_mesh_desc* mesh, const mhg_options* opt, mhg_ctx** out)
   r |= balloc(c, &c->Ud0, NN) | balloc(c, &c->Ud1, NN) | balloc(c, &c->Ud3, NN);
   r |= balloc(c, &c->dg0, NN) | balloc(c, &c->dg1, NN);
   if (P.etot) r |= balloc(c, &c->Ud4, NN) | balloc(c, &c->dek, NN);
-  if (P.order2 == 2) r |= dalloc(c, &c->est, NC);  // zero: pure in environment 0, as never written
+  if (P.order2 == 2 || P.generic) r |= dalloc(c, &c->est, NC);  // zero: pure in environment 0, as never written
+  if (P.generic) {
+    const size_t nvn = (size_t)P.nvu * NC;
+    r |= dalloc(c, &c->dar, nvn);
+    if (P.pb) r |= dalloc(c, &c->phi, nvn) | dalloc(c, &c->dav, nvn);
+  }
   if (P.etot && P.dim == 3) r |= balloc(c, &c->UdB4, NN);
   if (P.dim == 3) {
     for (int b = 0; b < 2; ++b) r |= dalloc(c, &c->xz[b], NN) | balloc(c, &c->uz[b], NN);
@@ -882,6 +940,17 @@ int mhg_upload_state(mhg_ctx* c, const mhg_state* st) {
   HIPCHK(c, put(c, c->env_m, st->cell_mass_env, en));
   HIPCHK(c, put(c, c->env_f, st->fracvol_env, en));
   HIPCHK(c, put(c, c->env_mf, st->mass_fraction_env, en));
+  if (P.pb && st->env_mask && st->fracvol) {
+    // pente-borne: a pure cell's fraction in its environment slot; phi of
+    // absent environments starts from zero
+    std::vector<double> f((size_t)P.ne * NC, 0.);
+    if (st->fracvol_env) std::memcpy(f.data(), st->fracvol_env, en * sizeof(double));
+    for (long cc = 0; cc < NC; ++cc)
+      if (__builtin_popcount((unsigned)st->env_mask[cc]) == 1)
+        f[(size_t)(__builtin_ffs(st->env_mask[cc]) - 1) * NC + cc] = st->fracvol[cc];
+    HIPCHK(c, put(c, c->env_f, f.data(), en));
+  }
+  if (c->phi) HIPCHK(c, hipMemset(c->phi, 0, (size_t)P.nvu * NC * sizeof(double)));
   if (st->velocity) {
     HIPCHK(c, put(c, c->ux[0], st->velocity, NN));
     HIPCHK(c, put(c, c->uy[0], st->velocity + NN, NN));
@@ -943,6 +1012,15 @@ int mhg_download_state(mhg_ctx* c, mhg_state* st) {
                {st->cell_mass_env, c->env_m, st->cell_mass},
                {st->fracvol_env, c->env_f, nullptr},
                {st->mass_fraction_env, c->env_mf, nullptr}};
+  // pente-borne keeps a pure cell's fraction in its environment slot
+  std::vector<double> fpure;
+  if (P.pb) {
+    fpure.resize(en);
+    HIPCHK(c, get(c, fpure.data(), c->env_f, en));
+  }
+  auto pure_frac = [&](long cc) {
+    return P.pb ? fpure[(size_t)(__builtin_ffs(mask[cc]) - 1) * NC + cc] : 1.0;
+  };
   for (int t = 0; t < 8; ++t) {
     if (!eps[t].h) continue;
     HIPCHK(c, get(c, eps[t].h, eps[t].d, en));
@@ -951,13 +1029,14 @@ int mhg_download_state(mhg_ctx* c, mhg_state* st) {
     if (t < 6) HIPCHK(c, get(c, g.data(), gsrc[t], NC));
     for (long cc = 0; cc < NC; ++cc)
       if (__builtin_popcount((unsigned)mask[cc]) == 1)
-        eps[t].h[(size_t)(__builtin_ffs(mask[cc]) - 1) * NC + cc] = g[cc];
+        eps[t].h[(size_t)(__builtin_ffs(mask[cc]) - 1) * NC + cc] = t == 6 ? pure_frac(cc) : g[cc];
   }
   // the cell fracvol / mass fraction of single-environment cells (1 up to
-  // rounding); the GPU path does not keep them, nothing downstream reads them
+  // rounding); only pente-borne keeps and reads the fraction
   for (long cc = 0; cc < NC; ++cc) {
-    if (st->fracvol) st->fracvol[cc] = __builtin_popcount((unsigned)mask[cc]) == 1 ? 1.0 : 0.0;
-    if (st->mass_fraction) st->mass_fraction[cc] = __builtin_popcount((unsigned)mask[cc]) == 1 ? 1.0 : 0.0;
+    const bool pure = __builtin_popcount((unsigned)mask[cc]) == 1;
+    if (st->fracvol) st->fracvol[cc] = pure ? pure_frac(cc) : 0.0;
+    if (st->mass_fraction) st->mass_fraction[cc] = pure ? 1.0 : 0.0;
   }
   if (st->velocity) {
     HIPCHK(c, get(c, st->velocity, c->ux[c->ucur], NN));
@@ -1068,6 +1147,7 @@ StageArgs stage_args(mhg_ctx* c) {
   A.env_q_n = c->env_q_n;
   A.env_tau = c->env_tau;
   A.U = bp<double>(c->U[0]);
+  A.phi = c->phi;
   return A;
 }
 
@@ -1385,8 +1465,10 @@ int mhg_remap_has_conservation_energie_totale(const mhg_ctx* c) { return c ? c->
 int mhg_remap_is_euler(const mhg_ctx* c) { return c ? c->opt.is_euler_scheme : MHG_E_ARG; }
 
 namespace {
-// One iteration of a Lagrange-only context (with-projection = false)
-// through its entry-point kernels, in the order of Mahyco.config:30-41.
+// One iteration through the entry-point kernels, in the order of
+// Mahyco.config:30-41: the Lagrange-only mode (with-projection = false) and
+// the generic sweeps (pente-borne, -mixte).
+bool entry_point_steps(const Params& P) { return P.lagonly || P.generic; }
 int step_entry_points(mhg_ctx* c, int32_t* stop) {
   int r = read_scalars(c);
   if (r) return r;
@@ -1411,7 +1493,7 @@ int step_entry_points(mhg_ctx* c, int32_t* stop) {
 int mhg_step(mhg_ctx* c, int32_t* stop) {
   int r = check_fused(c);
   if (r) return r;
-  if (c->P.lagonly) return step_entry_points(c, stop);
+  if (entry_point_steps(c->P)) return step_entry_points(c, stop);
   r = read_scalars(c);
   if (r) return r;
   const int it0 = c->S_host->iteration;
@@ -1430,7 +1512,7 @@ int mhg_run_async(mhg_ctx* c, int32_t n) {
   int r = check_fused(c);
   if (r) return r;
   if (n < 0) return fail(c, MHG_E_ARG, "run_async: negative step count");
-  if (c->P.lagonly) return mhg_run(c, n, nullptr) < 0 ? MHG_E_HIP : MHG_OK;  // host-synchronous
+  if (entry_point_steps(c->P)) return mhg_run(c, n, nullptr) < 0 ? MHG_E_HIP : MHG_OK;  // host-synchronous
   // check_fused has reconciled earlier iterations: the device count is the
   // host's starting point for these
   r = read_scalars(c);
@@ -1451,7 +1533,7 @@ int mhg_synchronize(mhg_ctx* c) {
 int mhg_run(mhg_ctx* c, int32_t max_steps, int32_t* steps_done) {
   int r = check_fused(c);
   if (r) return r;
-  if (c->P.lagonly) {
+  if (entry_point_steps(c->P)) {
     r = read_scalars(c);
     if (r) return r;
     const int it0 = c->S_host->iteration;

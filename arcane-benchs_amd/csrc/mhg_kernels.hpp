@@ -126,6 +126,7 @@ struct StageArgs {
   double *rho_n, *e_n, *q_n, *tau, *divu;
   double *env_rho_n, *env_e_n, *env_q_n, *env_tau;
   double* U;
+  double* phi;  // pente-borne: [3 ne + 1][nc] phi of the environments present (BUILD_U)
 };
 enum CellStage {
   STAGE_SAVE = 0,       // saveValuesAtN (cell part)
@@ -136,6 +137,21 @@ enum CellStage {
   STAGE_BUILD_U = 5     // computeVariablesForRemap (cells)
 };
 void launch_cell_stage_2d(hipStream_t st, const Params& P, DevScalars* S, const StageArgs& A, int stage);
+// Generic cell sweep of the pente-borne / -mixte options (FP64, 2D and 3D):
+// k_generic_grad then k_generic_update along dir (mhg_generic.hip).
+struct GenericSweepArgs {
+  const double *x, *y, *z;     // Lagrangian coordinates of this iteration (z: 3D)
+  const double *ux, *uy, *uz;  // node velocities at the start of the remap
+  const double* Uin;
+  double* Uout;
+  const double* Vlag;
+  double* phi;        // pente-borne: [3 ne + 1][nc], kept between iterations
+  double *dar, *dav;  // pente-borne back / front face fluxes; -mixte alone: dar = gradients
+  int* est;           // mixed (-2) / pure (environment, -1 none) code of each cell
+  double* contrib;    // dual mass contributions of the node sweep
+};
+void launch_generic_sweep(hipStream_t st, const Params& P, DevScalars* S, const GenericSweepArgs& A, int dir,
+                          bool first);
 // computeGeometricValues of hexahedra (MahycoModule.cc:1054-1299): V and L
 // from the coordinates of this iteration.
 void launch_cell_geometry_3d(hipStream_t st, const Params& P, DevScalars* S, const double* xnx,

@@ -252,6 +252,28 @@ __global__ __launch_bounds__(256) void k_cell_stage_2d(Params P, DevScalars* S, 
       A.U[(2 * ne + k) * NC + c] = ue[k];
     }
     A.U[(3 * ne) * NC + c] = uq;
+    if (P.pb) {
+      // computeVariablesForRemap with pente-borne (PrepareRemap.cc:159-215):
+      // fraction, density, energy of each environment present, the last
+      // one's viscosity; the slots of absent environments are left as they are
+      // (a pure cell's fraction is kept in its environment slot)
+      if (nbe == 1) {
+        A.phi[kk * NC + c] = A.env_f[kk * NC + c];
+        A.phi[(ne + kk) * NC + c] = A.rho[c];
+        A.phi[(2 * ne + kk) * NC + c] = A.e[c];
+        A.phi[(3 * ne) * NC + c] = A.q[c];
+      } else if (nbe >= 2) {
+#pragma unroll
+        for (int k = 0; k < NEM; ++k) {
+          if (!((mask >> k) & 1)) continue;
+          const long ek = k * NC + c;
+          A.phi[k * NC + c] = A.env_f[ek];
+          A.phi[(ne + k) * NC + c] = A.env_rho[ek];
+          A.phi[(2 * ne + k) * NC + c] = A.env_e[ek];
+          A.phi[(3 * ne) * NC + c] = A.env_q[ek];
+        }
+      }
+    }
   }
 }
 
@@ -819,7 +841,7 @@ static void launch_tile_y(hipStream_t st, const Params& P, DevScalars* S, const 
 }
 
 bool fused_final_supported(const Params& P) {
-  return !P.learned && !P.etot && !P.lagonly && P.ne <= 4 && P.lim < kLimG && P.order2 != 2;
+  return !P.learned && !P.etot && !P.lagonly && !P.generic && P.ne <= 4 && P.lim < kLimG && P.order2 != 2;
 }
 
 template <class R>

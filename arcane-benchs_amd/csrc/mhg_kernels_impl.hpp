@@ -266,8 +266,9 @@ __device__ inline R final_cell_vals(const Params& P, const CellFinalArgsT<R>& A,
       }
       fv[k] = f;
     }
-    // order 3: the cell's mixed / pure code for the next remap (RemapADIFinal.cc:286-293)
-    if (P.order2 == 2 && A.est) A.est[c] = matcell > 1 ? -2 : imatpure;
+    // order 3, generic sweeps: the cell's mixed / pure code for the next
+    // remap (RemapADIFinal.cc:286-293)
+    if ((P.order2 == 2 || P.generic) && A.est) A.est[c] = matcell > 1 ? -2 : imatpure;
     const int nbe = __popc(mask);
 #pragma unroll
     for (int k = 0; k < NEM; ++k) {
@@ -342,10 +343,19 @@ __device__ inline R final_cell_vals(const Params& P, const CellFinalArgsT<R>& A,
 #pragma unroll
       for (int k = 0; k < NEM; ++k)
         if (k == kpure) mfp = mf[k];
+      bool reset = false;
       if (rho_g < R(0) || e_g < R(0)) {
         e_g = R(0);
         rho_g = R(0);
         mfp = R(0);
+        reset = true;
+      }
+      // pente-borne reads a pure cell's fraction (1 up to rounding, 0 after
+      // a reset): kept in its environment slot
+      if (P.pb) {
+#pragma unroll
+        for (int k = 0; k < NEM; ++k)
+          if (k == kpure) A.env_f[k * NC + c] = reset ? R(0) : fv[k];
       }
       // total-energy fix-up (RemapADIFinal.cc:751-776), before the EOS
       if (P.etot) e_g += mfp * delta_ec;
