@@ -7,6 +7,7 @@
 // Every Cartesian face normal is +e_dir (InitGeometricValues,
 // MahycoModule.cc:1016-1025, with face nodes ordered from the smallest id), so
 // dot products with face normals reduce to one component.
+#include <algorithm>
 #include <type_traits>
 
 #include "mhg_common.hpp"
@@ -111,6 +112,119 @@ __global__ __launch_bounds__(256) void k_node_lagrange_3d(Params P, DevScalars* 
     st_stream(&A.b.xnx[nid], bx + dt * vx);
     st_stream(&A.b.xny[nid], by + dt * vy);
     st_stream(&A.xnz[nid], bz + dt * vz);
+  }
+}
+
+// The same three phases for every node, on tiles: a block owns 15 x 15 nodes
+// of each node plane of a range of planes along z and marches up the range.
+// For each cell plane it forms once per cell (16 x 16 cells, the tile's
+// nodes and their low-side neighbours) the 24 half cross products of
+// computeCQs and the 8 corner forces (p + q) C_q into LDS; each node then adds
+// its 8 cells' corner forces in ascending cell id, the 4 cells below from
+// the previous plane (kept in registers) and the 4 above from this one, the
+// order and arithmetic of k_node_lagrange_3d, so the two agree bit for bit.
+// A block starting above the first plane forms the cell plane below its
+// range first.
+constexpr int kNl3Tile = 15;                  // nodes per tile side
+constexpr int kNl3Cells = kNl3Tile + 1;       // cells per tile side (low halo)
+template <class R>
+__global__ __launch_bounds__(256) void k_node_lagrange_3d_tiled(Params P, DevScalars* S, NodeLag3ArgsT<R> A, int ntx,
+                                                              int nty, int kchunk) {
+  if (S->stop || S->err) return;
+  constexpr int TC = kNl3Cells;
+  __shared__ double sF[3][8][TC * TC];  // corner forces: component, corner, cell slot
+  __shared__ double sM[TC * TC];        // cell masses
+  const int tid = threadIdx.x;
+  const int ntile = ntx * nty;
+  const int tile = blockIdx.x % ntile, chunk = blockIdx.x / ntile;
+  const int i0 = (tile % ntx) * kNl3Tile, j0 = (tile / ntx) * kNl3Tile;
+  const int nnx = P.nx + 1, nny = P.ny + 1, nnz = P.nz + 1;
+  const int k0 = chunk * kchunk, k1 = min(k0 + kchunk, nnz);
+  const int ti = tid % TC, tj = tid / TC;
+  const int ni = i0 + ti, nj = j0 + tj;  // node of the thread in the node phase
+  const bool node_ok = ti < kNl3Tile && tj < kNl3Tile && ni < nnx && nj < nny;
+  const int ci = i0 - 1 + ti, cj = j0 - 1 + tj;  // cell of the thread in the cell phase
+  const bool cell_ok = ci >= 0 && ci < P.nx && cj >= 0 && cj < P.ny;
+  const double dt = S->dt, dtv = 0.5 * (S->dt_old + S->dt);
+  auto cell_phase = [&](int kc) {
+    if (!cell_ok) return;
+    V3 cn[8], fc[6], n[24];
+    hex_nodes(P, A.b.xpx, A.b.xpy, A.xpz, ci, cj, kc, cn);
+#pragma unroll
+    for (int f = 0; f < 6; ++f) fc[f] = hex_face_centre(cn, f);
+#pragma unroll
+    for (int t = 0; t < 24; ++t) n[t] = hex_term_value(cn, fc, t);
+    const long c = cid3(P, ci, cj, kc);
+    const double pq = (double)A.b.p[c] + (double)A.b.q[c];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const V3 f = v3mul(pq, hex_corner_sum(n, q));
+      sF[0][q][tid] = f.x;
+      sF[1][q][tid] = f.y;
+      sF[2][q][tid] = f.z;
+    }
+    sM[tid] = (double)A.b.m[c];
+  };
+  // the node's cells in a cell plane: t = 0..3 (di, dj) = (-1,-1), (0,-1),
+  // (-1,0), (0,0); corner of the node in the cell: `below` for the plane
+  // under the node, else the plane above
+  auto add_plane = [&](V3& F, double& msum, bool below) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int di = (t & 1) - 1, dj = (t >> 1) - 1;
+      const int gi = ni + di, gj = nj + dj;
+      if (gi < 0 || gi >= P.nx || gj < 0 || gj >= P.ny) continue;
+      const int slot = (ti + di + 1) + TC * (tj + dj + 1);
+      const int q = below ? (t == 0 ? 6 : (t == 1 ? 7 : (t == 2 ? 5 : 4))) : (t == 0 ? 2 : (t == 1 ? 3 : (t == 2 ? 1 : 0)));
+      F = v3add(F, V3{sF[0][q][slot], sF[1][q][slot], sF[2][q][slot]});
+      msum += sM[slot];
+    }
+  };
+  V3 Fn = {0., 0., 0.};
+  double mn = 0.;
+  if (k0 > 0) {
+    cell_phase(k0 - 1);
+    __syncthreads();
+    if (node_ok) add_plane(Fn, mn, true);
+    __syncthreads();
+  }
+  for (int kc = k0; kc < k1; ++kc) {
+    if (kc < P.nz) cell_phase(kc);
+    __syncthreads();
+    if (node_ok) {
+      V3 F = Fn;
+      double msum = mn;
+      if (kc < P.nz) add_plane(F, msum, false);
+      const long nid = nid3(P, ni, nj, kc);
+      double vx = A.b.ux[nid], vy = A.b.uy[nid], vz = A.uz[nid];
+      const double mnode = 0.125 * msum;
+      const double s = dtv / mnode;
+      vx = vx + s * F.x;
+      vy = vy + s * F.y;
+      vz = vz + s * F.z;
+      for (int b = 0; b < P.nbc; ++b) {
+        const BcRange& r = P.bc[b];
+        const int idx = (r.axis == 0) ? ni : (r.axis == 1 ? nj : kc);
+        if (r.axis < 0 || (idx >= r.lo && idx <= r.hi)) {
+          if (r.comp == 0) vx = r.value;
+          else if (r.comp == 1) vy = r.value;
+          else vz = r.value;
+        }
+      }
+      st_stream(&A.b.ux[nid], (R)vx);
+      st_stream(&A.b.uy[nid], (R)vy);
+      st_stream(&A.uz[nid], (R)vz);
+      const double bx = P.lagonly ? A.b.xpx[nid] : x0_3d(P, 0, ni);
+      const double by = P.lagonly ? A.b.xpy[nid] : x0_3d(P, 1, nj);
+      const double bz = P.lagonly ? A.xpz[nid] : x0_3d(P, 2, kc);
+      st_stream(&A.b.xnx[nid], bx + dt * vx);
+      st_stream(&A.b.xny[nid], by + dt * vy);
+      st_stream(&A.xnz[nid], bz + dt * vz);
+      Fn = V3{0., 0., 0.};
+      mn = 0.;
+      if (kc < P.nz) add_plane(Fn, mn, true);
+    }
+    __syncthreads();
   }
 }
 
@@ -1091,7 +1205,15 @@ template <class R>
 void launch_node_lagrange_3d(hipStream_t st, const Params& P, DevScalars* S, const NodeLag3ArgsT<R>& A, bool vel,
                              bool bc, bool pos) {
   dim3 g(nblocks3(P.nn, 256)), b(256);
-  if (vel && bc && pos) hipLaunchKernelGGL((k_node_lagrange_3d<R, true, true, true>), g, b, 0, st, P, S, A);
+  if (vel && bc && pos) {
+    // tiles of 15 x 15 nodes, node planes split into ranges for >= ~2048 blocks
+    const int ntx = (P.nx + 1 + kNl3Tile - 1) / kNl3Tile, nty = (P.ny + 1 + kNl3Tile - 1) / kNl3Tile;
+    const int nnz = P.nz + 1;
+    const int nchunk = std::max(1, std::min(nnz, (2048 + ntx * nty - 1) / (ntx * nty)));
+    const int kchunk = (nnz + nchunk - 1) / nchunk;
+    const int nblk = ntx * nty * ((nnz + kchunk - 1) / kchunk);
+    hipLaunchKernelGGL((k_node_lagrange_3d_tiled<R>), dim3(nblk), dim3(256), 0, st, P, S, A, ntx, nty, kchunk);
+  }
   else if (vel && !bc && !pos) hipLaunchKernelGGL((k_node_lagrange_3d<R, true, false, false>), g, b, 0, st, P, S, A);
   else if (!vel && bc && !pos) hipLaunchKernelGGL((k_node_lagrange_3d<R, false, true, false>), g, b, 0, st, P, S, A);
   else if (!vel && !bc && pos) hipLaunchKernelGGL((k_node_lagrange_3d<R, false, false, true>), g, b, 0, st, P, S, A);
