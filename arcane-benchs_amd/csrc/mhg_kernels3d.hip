@@ -1018,15 +1018,8 @@ __global__ __launch_bounds__(1024) void k_node_sweep_tile_3d(Params P, DevScalar
   int ti, tj, tk;
   node_ijk(i, s, &ti, &tj, &tk);
   const double xd = ldx[lane][w];
-  R p0[3] = {lds[0][lane][w], lds[1][lane][w], lds[2][lane][w]};
   const R front = lds[3][lane][w];
   const double xdm = lane_up1(xd), xdp = lane_down1(xd);
-  R pm[3], pp[3], g0[3] = {R(0), R(0), R(0)}, gm[3], gp[3];
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    pm[q] = lane_up1(p0[q]);
-    pp[q] = lane_down1(p0[q]);
-  }
   const R back = lane_up1(front);
   R* dg[3] = {A.dg0, A.dg1, A.dg2};
   // generalised limiters: dual widths (UtilesRemap.cc:46-57) from the nodes
@@ -1043,38 +1036,53 @@ __global__ __launch_bounds__(1024) void k_node_sweep_tile_3d(Params P, DevScalar
     hm = (R)(s >= 2 ? 0.5 * (xd - xmm) : 0.);
     hp = (R)(s <= N - 2 ? 0.5 * (xpp - xd) : 0.);
   }
-  if (P.order2 > 0 && ex) {
-    const long q0 = nid3(P, ti, tj, tk);
-    if (inner_d) {
-      const R dxf = (R)(xdp - xd), dxb = (R)(xd - xdm);
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const R gf = qdiv(pp[q] - p0[q], dxf), gb = qdiv(p0[q] - pm[q], dxb);
-        g0[q] = (LIM == kLimG) ? limiter_g<R>(P.lim, gf, gb, pp[q], pm[q], h0, hp, hm)
-                               : limited_grad<LIM, false, R, kBranchFree<R>>(gb, gf);
-      }
-      if (out && on_other_boundary<D>(P, ti, tj, tk))
-        for (int q = 0; q < 3; ++q) dg[q][q0] = g0[q];
-    } else {
-      for (int q = 0; q < 3; ++q) g0[q] = dg[q][q0];
-    }
-  }
+  // the increments of the inner nodes (352-491), one velocity component at a
+  // time (dual_increments' arithmetic, fewer values live at once): upwind
+  // terms from the velocity along D, then per component the phi of the node
+  // and its neighbours, the limited gradients and the face values
+  const R order2 = (R)P.order2;
+  const R pD = lds[D][lane][w], pDm = lane_up1(pD);
+  const R ufront = R(0.5) * (pD + pD);  // DualPhaseRemap.cc:391
+  const int signfront = (ufront > R(0)) ? 1 : -1;
+  const bool sf = (front < R(0));
+  const R tf = (R)(signfront * (xdp - xd) - dt * ufront);
+  const R uback = R(0.5) * (pDm + pD);
+  const int signback = (uback > R(0)) ? 1 : -1;
+  const bool sb = (back > R(0));
+  const R tb = (R)(signback * (xd - xdm) - dt * uback);
+  const bool o2ex = P.order2 > 0 && ex;
+  const long q0 = o2ex ? nid3(P, ti, tj, tk) : 0;
+  const bool keep_dg = out && on_other_boundary<D>(P, ti, tj, tk);
+  R dU[4] = {R(0), R(0), R(0), R(0)}, dU4 = R(0);
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    gm[q] = lane_up1(g0[q]);
-    gp[q] = lane_down1(g0[q]);
+    const R p0 = lds[q][lane][w];
+    const R pm = lane_up1(p0), pp = lane_down1(p0);
+    R g0 = R(0);
+    if (o2ex) {
+      if (inner_d) {
+        const R dxf = (R)(xdp - xd), dxb = (R)(xd - xdm);
+        const R gf = qdiv(pp - p0, dxf), gb = qdiv(p0 - pm, dxb);
+        g0 = (LIM == kLimG) ? limiter_g<R>(P.lim, gf, gb, pp, pm, h0, hp, hm)
+                            : limited_grad<LIM, false, R, kBranchFree<R>>(gb, gf);
+        if (keep_dg) dg[q][q0] = g0;
+      } else {
+        g0 = dg[q][q0];
+      }
+    }
+    const R gm = lane_up1(g0), gp = lane_down1(g0);
+    if (out && inner_d) {
+      const R FV = (sf ? pp : p0) + order2 * (R(0.5) * (sf ? gp : g0) * tf);
+      const R BV = (sb ? pm : p0) + order2 * (R(0.5) * (sb ? gm : g0) * tb);
+      dU[q] = back * BV - front * FV;
+    }
   }
-  R p4 = R(0), p4m = R(0), p4p = R(0);
+  if (out && inner_d) dU[3] = back - front;
   if (P.etot) {
-    p4 = lds[4][lane][w];
-    p4m = lane_up1(p4);
-    p4p = lane_down1(p4);
-  }
-  R dU[4] = {R(0), R(0), R(0), R(0)}, dU4 = R(0);
-  if (out && inner_d) {
-    dual_increments<R, D>(P, dt, pm, p0, pp, xdm, xd, xdp, gm, g0, gp, back, front, dU);
     // the kinetic energy is carried with the upwind value (its gradient is 0)
-    if (P.etot) dU4 = back * (back > R(0) ? p4m : p4) - front * (front < R(0) ? p4p : p4);
+    const R p4 = lds[4][lane][w];
+    const R p4m = lane_up1(p4), p4p = lane_down1(p4);
+    if (out && inner_d) dU4 = back * (back > R(0) ? p4m : p4) - front * (front < R(0) ? p4p : p4);
   }
   __syncthreads();  // transposed reads done before the tiles are reused
 #pragma unroll
