@@ -105,15 +105,15 @@ __device__ __forceinline__ void node_lagrange_2d_body(const Params& P, DevScalar
     }
   }
   if (VEL || BC) {
-    st_stream(&A.ux[nid], (R)vx);
-    st_stream(&A.uy[nid], (R)vy);
+    st_stream(P, &A.ux[nid], (R)vx);
+    st_stream(P, &A.uy[nid], (R)vy);
   }
   if (POS) {
     // Euler scheme: saveValuesAtN reset x to x0; Lagrange-only: x moves on
     const double bx = P.lagonly ? A.xpx[nid] : node_x0(P, i);
     const double by = P.lagonly ? A.xpy[nid] : node_y0(P, j);
-    st_stream(&A.xnx[nid], bx + dt * vx);
-    st_stream(&A.xny[nid], by + dt * vy);
+    st_stream(P, &A.xnx[nid], bx + dt * vx);
+    st_stream(P, &A.xny[nid], by + dt * vy);
   }
 }
 
@@ -151,7 +151,7 @@ __device__ __forceinline__ void cell_lagrange_2d_body(const Params& P, DevScalar
     y[q] = A.xny[nd[q]];
   }
   const double Vd = quad_volume(x, y);
-  const double Ld = (P.lc == MHG_LC_FACES_OPPOSEES) ? length_faces_opposees_2d(x, y) : pow(Vd, 0.5);
+  const double Ld = (P.lc == MHG_LC_FACES_OPPOSEES) ? length_faces_opposees_2d(x, y) : length_sqrt_2d(Vd);
   lagrange_cell_bulk<R, PC, NEM>(P, S, A, c, Vd, Ld, in);
 }
 
@@ -261,7 +261,7 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
       yq[q] = D == 0 ? tn[q] : an[q];
     }
     const double Vd = quad_volume(xq, yq);
-    const double Ld = (P.lc == MHG_LC_FACES_OPPOSEES) ? length_faces_opposees_2d(xq, yq) : pow(Vd, 0.5);
+    const double Ld = (P.lc == MHG_LC_FACES_OPPOSEES) ? length_faces_opposees_2d(xq, yq) : length_sqrt_2d(Vd);
     R uv[NEM], um[NEM], ue[NEM], uq;
     lagrange_cell_vals<R, false, NEM>(P, S, *LA, c, (R)Vd, (R)Ld, lin, uv, um, ue, uq);
 #pragma unroll
@@ -273,8 +273,8 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
     preq = ex ? uq : R(0);
     vlag = (R)Vd;
     if (out) {
-      st_stream(&LA->Vlag[c], (R)Vd);
-      st_stream(&LA->L[c], (R)Ld);
+      st_stream(P, &LA->Vlag[c], (R)Vd);
+      st_stream(P, &LA->L[c], (R)Ld);
     }
   } else {
 #pragma unroll
@@ -425,7 +425,7 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
     R un, du;
     sweep_var(pre[0][k], un, du, 0, k);
     if (FINAL) fV[k] = un;
-    else if (out) st_stream(&Uout[k * NC + c], un);
+    else if (out) st_stream(P, &Uout[k * NC + c], un);
   }
   if (LEARN) es.close_volumes(ne, alpha);
 #pragma unroll
@@ -436,8 +436,8 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
     if (un < R(0)) un = R(0);  // RemapADIService.cc:953-955
     if (FINAL) fM[k] = un;
     if (out) {
-      if (!FINAL) st_stream(&Uout[(ne + k) * NC + c], un);
-      st_stream(&A.contrib[k * NC + c], R(0.5) * du);  // fcontrib (DualPhaseRemap.cc:168-185)
+      if (!FINAL) st_stream(P, &Uout[(ne + k) * NC + c], un);
+      st_stream(P, &A.contrib[k * NC + c], R(0.5) * du);  // fcontrib (DualPhaseRemap.cc:168-185)
     }
   }
 #pragma unroll
@@ -447,14 +447,14 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
     sweep_var(pre[2][k], un, du, 2, k);
     if (un < R(0)) un = R(0);  // RemapADIService.cc:956-958
     if (FINAL) fE[k] = un;
-    else if (out) st_stream(&Uout[(2 * ne + k) * NC + c], un);
+    else if (out) st_stream(P, &Uout[(2 * ne + k) * NC + c], un);
   }
   R fQ = R(0);
   {
     R un, du;
     sweep_var(preq, un, du, 3, 0);
     if (FINAL) fQ = un;
-    else if (out) st_stream(&Uout[(3 * ne) * NC + c], un);
+    else if (out) st_stream(P, &Uout[(3 * ne) * NC + c], un);
   }
   if (FINAL && out) {
     // Euler volume: the corner-normal formula on the initial coordinates
@@ -563,7 +563,7 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
     const CellPre<R> lin = cell_pre_load<R, false>(*LA, gc);
     const double yq[4] = {ya, yb, yc, yd};
     const double Vd = quad_volume(tn, yq);
-    const double Ld = (P.lc == MHG_LC_FACES_OPPOSEES) ? length_faces_opposees_2d(tn, yq) : pow(Vd, 0.5);
+    const double Ld = (P.lc == MHG_LC_FACES_OPPOSEES) ? length_faces_opposees_2d(tn, yq) : length_sqrt_2d(Vd);
     R uv[NEM], um[NEM], ue[NEM], uq;
     lagrange_cell_vals<R, false, NEM>(P, S, *LA, gc, (R)Vd, (R)Ld, lin, uv, um, ue, uq);
 #pragma unroll
@@ -580,8 +580,8 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
     nxa0 = lU[0];
     nxb0 = lU[1];
     if (gout) {
-      st_stream(&LA->Vlag[gc], (R)Vd);
-      st_stream(&LA->L[gc], (R)Ld);
+      st_stream(P, &LA->Vlag[gc], (R)Vd);
+      st_stream(P, &LA->L[gc], (R)Ld);
     }
   } else {
 #pragma unroll
@@ -770,11 +770,11 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
     }
     if (gout) {
       if (!FINAL) {
-        st_stream(&A.Uout[va * NC + gc], stg[2 * TS + ra]);
-        if (hb) st_stream(&A.Uout[vb * NC + gc], stg[3 * TS + ra]);
+        st_stream(P, &A.Uout[va * NC + gc], stg[2 * TS + ra]);
+        if (hb) st_stream(P, &A.Uout[vb * NC + gc], stg[3 * TS + ra]);
       }
-      if (ma) st_stream(&A.contrib[(va - ne) * NC + gc], stg[4 * TS + ra]);
-      if (mb) st_stream(&A.contrib[(vb - ne) * NC + gc], stg[5 * TS + ra]);
+      if (ma) st_stream(P, &A.contrib[(va - ne) * NC + gc], stg[4 * TS + ra]);
+      if (mb) st_stream(P, &A.contrib[(vb - ne) * NC + gc], stg[5 * TS + ra]);
     }
   };
   if (LEARN || LIM == kLimO3 || FINAL || LAG) {  // compile-time variable indices: per-environment values stay in registers
@@ -1147,18 +1147,18 @@ __device__ __forceinline__ void node_sweep_x_2d_body(const Params& P, DevScalars
     if (fabs(U1) < thresold) U1 = R(0);
   }
   if (FIRST) {
-    st_stream(&A.Ud0[nid], U0);
-    st_stream(&A.Ud1[nid], U1);
-    st_stream(&A.Ud3[nid], U3);
-    if (P.etot) st_stream(&A.Ud4[nid], U4);
+    st_stream(P, &A.Ud0[nid], U0);
+    st_stream(P, &A.Ud1[nid], U1);
+    st_stream(P, &A.Ud3[nid], U3);
+    if (P.etot) st_stream(P, &A.Ud4[nid], U4);
   } else if (U3 != R(0)) {
-    st_stream(&A.uox[nid], qdiv(U0, U3));
-    st_stream(&A.uoy[nid], qdiv(U1, U3));
+    st_stream(P, &A.uox[nid], qdiv(U0, U3));
+    st_stream(P, &A.uoy[nid], qdiv(U1, U3));
   } else {
-    st_stream(&A.uox[nid], A.ux[nid]);
-    st_stream(&A.uoy[nid], A.uy[nid]);
+    st_stream(P, &A.uox[nid], A.ux[nid]);
+    st_stream(P, &A.uoy[nid], A.uy[nid]);
   }
-  if (!FIRST && P.etot) st_stream(&A.dek[nid], dual_ecin_term(U0, U1, U3, U4));
+  if (!FIRST && P.etot) st_stream(P, &A.dek[nid], dual_ecin_term(U0, U1, U3, U4));
 }
 
 // Dual sweep along y on LDS tiles: a block of 16 wavefronts owns 16 node
@@ -1357,18 +1357,18 @@ __device__ __forceinline__ void node_sweep_ytile_2d_body(const Params& P, DevSca
     if (fabs(U1) < thresold) U1 = R(0);
   }
   if (FIRST) {
-    st_stream(&A.Ud0[gq], U0);
-    st_stream(&A.Ud1[gq], U1);
-    st_stream(&A.Ud3[gq], U3);
-    if (P.etot) st_stream(&A.Ud4[gq], U4);
+    st_stream(P, &A.Ud0[gq], U0);
+    st_stream(P, &A.Ud1[gq], U1);
+    st_stream(P, &A.Ud3[gq], U3);
+    if (P.etot) st_stream(P, &A.Ud4[gq], U4);
   } else if (U3 != R(0)) {
-    st_stream(&A.uox[gq], qdiv(U0, U3));
-    st_stream(&A.uoy[gq], qdiv(U1, U3));
+    st_stream(P, &A.uox[gq], qdiv(U0, U3));
+    st_stream(P, &A.uoy[gq], qdiv(U1, U3));
   } else {
-    st_stream(&A.uox[gq], A.ux[gq]);
-    st_stream(&A.uoy[gq], A.uy[gq]);
+    st_stream(P, &A.uox[gq], A.ux[gq]);
+    st_stream(P, &A.uoy[gq], A.uy[gq]);
   }
-  if (!FIRST && P.etot) st_stream(&A.dek[gq], dual_ecin_term(U0, U1, U3, U4));
+  if (!FIRST && P.etot) st_stream(P, &A.dek[gq], dual_ecin_term(U0, U1, U3, U4));
 }
 
 }  // namespace mhg

@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "../../include/mahyco_gpu.h"
 
 namespace mhg {
@@ -27,6 +29,14 @@ __host__ __device__ inline T smin(T a, T b) { return (b < a) ? b : a; }
 // bound, not bit equality.
 __host__ __device__ inline double qdiv(double a, double b) { return a / b; }
 __device__ inline float qdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
+// Characteristic length of a quadrilateral, math::pow(volume, 1/2)
+// (MahycoModule.cc:1259, racine-cubique-volume in 2D): the exact value is the
+// square root, so the correctly rounded sqrt (about 20 instructions, where
+// the device pow takes about 200).  glibc's pow, which the CPU restatement
+// calls, differs from the correctly rounded root by one ulp in about 0.09% of
+// inputs.
+__host__ __device__ inline double length_sqrt_2d(double V) { return sqrt(V); }
 
 // Limiters, RemapADIService.h:260-286.
 template <int LIM, class T = double>
@@ -61,36 +71,47 @@ __host__ __device__ inline double limited_node_grad(int lim, double gb, double g
   return g;
 }
 
-// One term of the limited gradients, lim(a / b) * b, for the minmod limiter,
-// evaluated without a division where the quotient's only role is to be
-// clamped: a and b of opposite signs (or a = 0) give lim = 0 and a zero term,
-// |a| >= |b| with the same sign gives lim = 1 exactly.  Only 0 < a/b < 1 (and
-// any NaN) divides.  Returns false when the term is a signed zero, which
-// leaves the accumulated gradient (started at +0) unchanged.
-// CELL selects the association of the cell form 0.5 * (lim * b) or the nodal
-// form (0.5 * lim) * b.
+// The minmod limited gradient 0.5 lim(gf/gb) gb + 0.5 lim(gb/gf) gf (each
+// term only where its denominator is non-zero) with one quotient.  With gb
+// and gf of the same sign the term of the larger one has lim = 1 exactly and
+// the other term's quotient is small / large, so the quotient is taken once,
+// of the operands selected by |gf| >= |gb|; with opposite signs or a zero
+// operand both terms vanish and the sum stays +0.  Bit for bit the guarded
+// two-term sum for finite operands; a NaN operand gives NaN.
+// In float (the precision form, an error bound rather than bit equality)
+// the sum is its exact value sign(gb) min(|gb|, |gf|), without a quotient.
 template <bool CELL, class T>
-__host__ __device__ inline bool minmod_term(T a, T b, T* term) {
-  const bool opp = (a > T(0) && b < T(0)) || (a < T(0) && b > T(0)) || (a == T(0));
-  if (opp) return false;
-  const bool same = (a > T(0) && b > T(0)) || (a < T(0) && b < T(0));
-  if (same && fabs(a) >= fabs(b)) {
-    *term = CELL ? T(0.5) * (T(1) * b) : T(0.5) * T(1) * b;
-    return true;
+__host__ __device__ inline T minmod_grad(T gb, T gf) {
+  const bool same = (gf > T(0) && gb > T(0)) || (gf < T(0) && gb < T(0));
+  const bool unord = !(gb == gb) || !(gf == gf);
+  if (std::is_same<T, float>::value) {
+    const T m = smin(fabs(gb), fabs(gf));
+    const T g = same ? (gb > T(0) ? m : -m) : T(0);
+    return unord ? gb + gf : g;
   }
-  const T l = smax(T(0), smin(T(1), qdiv(a, b)));
-  *term = CELL ? T(0.5) * (l * b) : T(0.5) * l * b;
-  return true;
+  // the quotient only where some lane has a non-zero slope: a wavefront over
+  // uniform data (zero gradients, absent environments) skips it
+  T g = T(0);
+  if (same) {
+    const bool big_f = fabs(gf) >= fabs(gb);
+    const T l = smax(T(0), smin(T(1), qdiv(big_f ? gb : gf, big_f ? gf : gb)));
+    const T lb = big_f ? T(1) : l, lf = big_f ? l : T(1);
+    const T t1 = CELL ? T(0.5) * (lb * gb) : T(0.5) * lb * gb;
+    const T t2 = CELL ? T(0.5) * (lf * gf) : T(0.5) * lf * gf;
+    g = (T(0) + t1) + t2;
+  }
+  return unord ? gb + gf : g;
 }
 
 // Limited gradients with the limiter fixed at compile time.  BF (branch-free)
 // evaluates the reference formula for both terms and selects: each term is
 // taken only where its denominator is non-zero, exactly as the guarded sums
 // of the reference (a skipped term adds +0 to a sum that starts at +0).
-// Otherwise minmod uses minmod_term, which divides only where the quotient
-// is not clamped; the other limiters evaluate the quotient as the reference.
+// Minmod always takes minmod_grad; the other limiters evaluate the quotient
+// as the reference.
 template <int LIM, bool CELL, class T, bool BF = false>
 __host__ __device__ inline T limited_grad(T gb, T gf) {
+  if (LIM == 0) return minmod_grad<CELL>(gb, gf);
   if (BF) {
     const T l1 = limiter<LIM>(qdiv(gf, gb)), l2 = limiter<LIM>(qdiv(gb, gf));
     const T t1 = CELL ? T(0.5) * (l1 * gb) : T(0.5) * l1 * gb;
@@ -101,12 +122,6 @@ __host__ __device__ inline T limited_grad(T gb, T gf) {
     return g;
   }
   T g = T(0);
-  if (LIM == 0) {
-    T t;
-    if (gb != T(0) && minmod_term<CELL>(gf, gb, &t)) g += t;
-    if (gf != T(0) && minmod_term<CELL>(gb, gf, &t)) g += t;
-    return g;
-  }
   if (gb != T(0)) g += CELL ? T(0.5) * (limiter<LIM>(qdiv(gf, gb)) * gb) : T(0.5) * limiter<LIM>(qdiv(gf, gb)) * gb;
   if (gf != T(0)) g += CELL ? T(0.5) * (limiter<LIM>(qdiv(gb, gf)) * gf) : T(0.5) * limiter<LIM>(qdiv(gb, gf)) * gf;
   return g;
@@ -410,13 +425,6 @@ struct BcRange {
   double value;
 };
 
-// Store of a value that this kernel writes once and a later kernel reads:
-// non-temporal, so that it streams out instead of staying dirty in the L2 of
-// the writing XCD (each kernel ends with a write-back of that L2).
-template <class T>
-__device__ inline void st_stream(T* p, T v) {
-  __builtin_nontemporal_store(v, p);
-}
 
 // Neighbour-lane reads within a 64-lane wavefront through the DPP whole-wave
 // shifts of gfx9-family VALU instructions (wave_shr:1 = 0x138, wave_shl:1 =
@@ -490,10 +498,30 @@ struct Params {
   // entry-point kernels): pb the pente-borne fluxes with debar fix pb_fix,
   // mixte the pure-neighbourhood switch to limiter pure_id
   int generic, pb, pb_fix, mixte, pure_id;
+  // store policy of the step kernels' outputs: 1 non-temporal (the lines
+  // stream to memory), 0 ordinary stores (they stay in the L2 / memory-side
+  // cache for the next kernel of the step); chosen per context by the
+  // working set against the 256 MB memory-side cache, settable by the caller
+  int nt_store;
   double cfl, dtmin, dtmax, tend;
   int nbc;
   BcRange bc[kMaxBc];
 };
+
+// Store of a value that this kernel writes once and a later kernel reads.
+// With Params::nt_store (a working set beyond the memory-side cache) it is
+// non-temporal, so that it streams out instead of staying dirty in the L2 of
+// the writing XCD (each kernel ends with a write-back of that L2); otherwise
+// an ordinary store, which the next kernels of the step find in the caches.
+// The flag is uniform over the launch (a scalar branch).
+// (The ordinary store is written as a relaxed wavefront-scope atomic store,
+// which is the same plain global store: as `*p = v` the compiler merges the
+// two branches into one store and drops the non-temporal hint.)
+template <class T>
+__device__ inline void st_stream(const Params& P, T* p, T v) {
+  if (P.nt_store) __builtin_nontemporal_store(v, p);
+  else __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
 
 // Environment count inside a kernel compiled for at most NEM environments.
 // The launchers choose NEM from {1, 2, 3, 4, 8} as the smallest bound >= ne,

@@ -694,6 +694,18 @@ int build_params(const mhg_mesh_desc* mesh, const mhg_options* opt, Params* Pout
   P.dtmax = opt->deltat_max;
   P.tend = opt->final_time;
   P.land = opt->land_on_final_time ? 1 : 0;
+  {
+    // store policy (measured, ms per step, ordinary / streaming stores):
+    // 2D keeps ordinary stores while the arrays one step touches fit the 256
+    // MB memory-side cache with room to spare, so that a later kernel of the
+    // step reads what an earlier one wrote from that cache (1792x768 triple
+    // point, FP32 bulk, ~250 MB: 0.124 / 0.159), and streams beyond it (FP64,
+    // ~460 MB: 0.264 / 0.256); 3D keeps ordinary stores (256^3: FP64 5.64 /
+    // 5.79, FP32 4.19 / 4.49)
+    const double wb = opt->precision == MHG_PREC_FP32_BULK ? 4. : 8.;
+    const double ws = (double)P.nc * wb * (6. * opt->nb_env + 20.) + (double)P.nn * 16. * P.dim;
+    P.nt_store = (P.dim == 2 && ws > 320e6) ? 1 : 0;
+  }
   P.nbc = 0;
   for (int b = 0; b < opt->nb_bc; ++b) {
     const mhg_bc& bc = opt->bc[b];
@@ -1617,10 +1629,10 @@ int mhg_conservation(mhg_ctx* c, double out[3]) {
 // restored after; the simulation state is unchanged.
 namespace {
 const char* kTunable[] = {"cell_sweep_y", "node_sweep_x", "node_sweep_y", "step_graph", "final_fusion",
-                          "lagrange_fusion"};
+                          "lagrange_fusion", "store_policy"};
 int tunable_index(const char* kernel) {
   if (!kernel) return -1;
-  for (int k = 0; k < 6; ++k)
+  for (int k = 0; k < 7; ++k)
     if (std::strcmp(kernel, kTunable[k]) == 0) return k;
   return -1;
 }
@@ -1628,18 +1640,20 @@ int variant_count_of(int k) {
   if (k == 0) return cell_sweep_y_variants();
   if (k == 1) return node_sweep_variants(0);
   if (k == 2) return node_sweep_variants(1);
-  if (k >= 3 && k <= 5) return 2;
+  if (k >= 3 && k <= 6) return 2;
   return 0;
 }
 const char* variant_name_of(int k, int v) {
   if (k == 3) return v ? "graph12" : "direct";
   if (k == 4 || k == 5) return v ? "fused" : "separate";
+  if (k == 6) return v ? "streaming" : "cached";
   return k == 0 ? cell_sweep_y_variant_name(v) : node_sweep_variant_name(k - 1, v);
 }
 int* variant_slot(mhg_ctx* c, int k) {
   if (k == 3) return &c->use_graphs;
   if (k == 4) return &c->fuse_final;
   if (k == 5) return &c->fuse_lag;
+  if (k == 6) return &c->P.nt_store;
   return k == 0 ? &c->yvariant : (k == 1 ? &c->nxvariant : &c->nyvariant);
 }
 }  // namespace

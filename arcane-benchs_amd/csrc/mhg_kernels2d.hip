@@ -163,7 +163,7 @@ __global__ __launch_bounds__(256) void k_cell_stage_2d(Params P, DevScalars* S, 
     }
     const double V = quad_volume(x, y);
     A.Vlag[c] = V;
-    A.L[c] = (P.lc == MHG_LC_FACES_OPPOSEES) ? length_faces_opposees_2d(x, y) : pow(V, 0.5);
+    A.L[c] = (P.lc == MHG_LC_FACES_OPPOSEES) ? length_faces_opposees_2d(x, y) : length_sqrt_2d(V);
   } else if (STAGE == STAGE_DENSITY) {
     const double inv_dt = 1.0 / S->dt;
     const double V = A.Vlag[c];
@@ -390,10 +390,10 @@ __global__ __launch_bounds__(64) void k_cell_sweep_y_2d(Params P, DevScalars* S,
           // cell faces in order: bottom (back), right, top (front), left
           double un = U2[s] - (flux_b + flux_f);
           if (v >= ne && v < 3 * ne && un < 0.) un = 0.;
-          st_stream(&Uout[v * NC + co], un);
+          st_stream(P, &Uout[v * NC + co], un);
           if (v >= ne && v < 2 * ne) {
             const double dual = 0.5 * flux_b * -1.0 + 0.5 * flux_f * 1.0;
-            st_stream(&A.contrib[(v - ne) * NC + co], 0.5 * dual);
+            st_stream(P, &A.contrib[(v - ne) * NC + co], 0.5 * dual);
           }
         }
         ph2[s] = ph1[s];
@@ -545,16 +545,16 @@ __global__ void k_node_sweep_2d(Params P, DevScalars* S, NodeSweepArgs A) {
     if (fabs(U1) < thresold) U1 = 0.;
   }
   if (FIRST) {
-    st_stream(&A.Ud0[nid], U0);
-    st_stream(&A.Ud1[nid], U1);
-    st_stream(&A.Ud3[nid], U3);
+    st_stream(P, &A.Ud0[nid], U0);
+    st_stream(P, &A.Ud1[nid], U1);
+    st_stream(P, &A.Ud3[nid], U3);
   } else {
     if (U3 != 0.) {
-      st_stream(&A.uox[nid], U0 / U3);
-      st_stream(&A.uoy[nid], U1 / U3);
+      st_stream(P, &A.uox[nid], U0 / U3);
+      st_stream(P, &A.uoy[nid], U1 / U3);
     } else {
-      st_stream(&A.uox[nid], A.ux[nid]);
-      st_stream(&A.uoy[nid], A.uy[nid]);
+      st_stream(P, &A.uox[nid], A.ux[nid]);
+      st_stream(P, &A.uoy[nid], A.uy[nid]);
     }
   }
 }
@@ -706,15 +706,15 @@ __global__ __launch_bounds__(64) void k_node_sweep_y_2d(Params P, DevScalars* S,
       if (fabs(V1) < thresold) V1 = 0.;
     }
     if (FIRST) {
-      st_stream(&A.Ud0[nid], V0);
-      st_stream(&A.Ud1[nid], V1);
-      st_stream(&A.Ud3[nid], V3);
+      st_stream(P, &A.Ud0[nid], V0);
+      st_stream(P, &A.Ud1[nid], V1);
+      st_stream(P, &A.Ud3[nid], V3);
     } else if (V3 != 0.) {
-      st_stream(&A.uox[nid], V0 / V3);
-      st_stream(&A.uoy[nid], V1 / V3);
+      st_stream(P, &A.uox[nid], V0 / V3);
+      st_stream(P, &A.uoy[nid], V1 / V3);
     } else {
-      st_stream(&A.uox[nid], A.ux[nid]);
-      st_stream(&A.uoy[nid], A.uy[nid]);
+      st_stream(P, &A.uox[nid], A.ux[nid]);
+      st_stream(P, &A.uoy[nid], A.uy[nid]);
     }
   }
 }

@@ -12,8 +12,8 @@ namespace mhg {
 constexpr int kTileW = 16, kTileR = 64, kTileP = kTileW + 1;
 
 // Limited gradients: branch-free in float (two hardware-reciprocal quotients
-// cost less than the divergent branches); in double the exact quotients are
-// dear, so minmod divides only where the quotient is not clamped.
+// cost less than the divergent branches) for the limiters other than minmod,
+// which takes minmod_grad in both types.
 template <class R>
 constexpr bool kBranchFree = std::is_same<R, float>::value;
 
@@ -179,13 +179,13 @@ __device__ inline void lagrange_cell_bulk(const Params& P, const DevScalars* S, 
 #pragma unroll
   for (int k = 0; k < NEM; ++k) {
     if (k >= ne) break;
-    st_stream(&U[k * NC + c], uv[k]);
-    st_stream(&U[(ne + k) * NC + c], um[k]);
-    st_stream(&U[(2 * ne + k) * NC + c], ue[k]);
+    st_stream(P, &U[k * NC + c], uv[k]);
+    st_stream(P, &U[(ne + k) * NC + c], um[k]);
+    st_stream(P, &U[(2 * ne + k) * NC + c], ue[k]);
   }
-  st_stream(&U[(3 * ne) * NC + c], uq);
-  st_stream(&A.Vlag[c], V);
-  st_stream(&A.L[c], L);
+  st_stream(P, &U[(3 * ne) * NC + c], uq);
+  st_stream(P, &A.Vlag[c], V);
+  st_stream(P, &A.L[c], L);
 }
 
 // Final cell phase of cell c with Euler volume vold: membership, "moy", EOS,
@@ -254,9 +254,14 @@ __device__ inline R final_cell_vals(const Params& P, const CellFinalArgsT<R>& A,
     for (int k = 0; k < NEM; ++k) {
       if (!((mask >> k) & 1)) continue;
       R f = fv[k] * unsursomme_frac;
-      // learned mode: an environment whose fraction ends at or below the
-      // threshold leaves the cell (no member with zero fraction and density)
-      if (P.learned && f <= thr) {
+      // learned mode and the float precision form: an environment whose
+      // fraction ends at or below the threshold leaves the cell (no member
+      // with zero fraction and density, whose EOS gives a 0/0 sound speed).
+      // The presence test above divides by the Euler volume and the
+      // fraction by the sum of the environment volumes; in float the two
+      // differ by ~1e-7 relative, so a volume near the threshold can pass
+      // the first and be zeroed by the second.
+      if ((P.learned || std::is_same<R, float>::value) && f <= thr) {
         f = R(0);
         mask &= ~(1 << k);
       }
@@ -408,15 +413,15 @@ __device__ inline R final_cell_vals(const Params& P, const CellFinalArgsT<R>& A,
         }
       }
     }
-    st_stream(&A.rho[c], rho_g);
-    st_stream(&A.e[c], e_g);
+    st_stream(P, &A.rho[c], rho_g);
+    st_stream(P, &A.e[c], e_g);
     if (EOS) {
-      st_stream(&A.p[c], p_g);
-      st_stream(&A.cs[c], c_g);
+      st_stream(P, &A.p[c], p_g);
+      st_stream(P, &A.cs[c], c_g);
     }
-    st_stream(&A.q[c], q_g);
-    st_stream(&A.m[c], mass_g);
-    st_stream(&A.mask[c], mask);
+    st_stream(P, &A.q[c], q_g);
+    st_stream(P, &A.m[c], mass_g);
+    st_stream(P, &A.mask[c], mask);
     return EOS ? c_g : R(0);
   }
 }

@@ -128,3 +128,20 @@ def test_3d_fp32_precision_form(gpu):
     linf = float(np.max(np.abs(got["density"] - ref["density"])) / np.max(np.abs(ref["density"])))
     print("3d fp32 density L1", l1, "Linf", linf)
     assert l1 < 1e-5
+
+
+@pytest.mark.parametrize("prec", [abi.PREC_FP64, abi.PREC_FP32_BULK])
+def test_3d_store_policy_bit_identical(gpu, prec):
+    """Ordinary and streaming stores of the 3D step kernels give the same
+    state bit for bit."""
+    states = []
+    for v in (0, 1):
+        s = box3(gpu, precision=prec)
+        s.select_variant(v, "store_policy")
+        s.run(9)
+        states.append(s.download())
+    for f, v in states[0].items():
+        if isinstance(v, np.ndarray):
+            assert np.array_equal(v, states[1][f]), f
+        else:
+            assert v == states[1][f], f
