@@ -18,6 +18,19 @@ namespace mhg {
 __device__ inline double node_x0(const Params& P, int i) { return P.org[0] + i * P.h[0]; }
 __device__ inline double node_y0(const Params& P, int j) { return P.org[1] + j * P.h[1]; }
 
+// Lagrangian coordinate along axis A of a node with index idx along A, from
+// its Lagrangian velocity component u.  In the Euler scheme every node starts
+// the Lagrange phase from its initial position (saveValuesAtN resets x to x0,
+// MahycoModule.cc:421, 433-441) and moves by dt u (updatePosition, 870-931):
+// the node phase stores x0 + dt u with the stored velocity, so these are the
+// same operations and the same double, bit for bit.  The remap sweeps form
+// their coordinates this way from the velocities they read anyway instead of
+// reading the stored coordinates.
+template <int A, class R>
+__device__ inline double lag_coord(const Params& P, int idx, double dt, R u) {
+  return (P.org[A] + idx * P.h[A]) + dt * (double)u;
+}
+
 // ---------------------------------------------------------------------------
 // K1: node phase.  updateForceAndVelocity (MahycoModule.cc:589-665) with the
 // corner normals of the previous Lagrangian coordinates (saveValuesAtN copies
@@ -112,8 +125,10 @@ __device__ __forceinline__ void node_lagrange_2d_body(const Params& P, DevScalar
     // Euler scheme: saveValuesAtN reset x to x0; Lagrange-only: x moves on
     const double bx = P.lagonly ? A.xpx[nid] : node_x0(P, i);
     const double by = P.lagonly ? A.xpy[nid] : node_y0(P, j);
-    st_stream(P, &A.xnx[nid], bx + dt * vx);
-    st_stream(P, &A.xny[nid], by + dt * vy);
+    // from the velocity as stored (the identity in double), so that the
+    // sweeps' x0 + dt u from the stored velocity is the same value
+    st_stream(P, &A.xnx[nid], bx + dt * (double)(R)vx);
+    st_stream(P, &A.xny[nid], by + dt * (double)(R)vy);
   }
 }
 
@@ -225,29 +240,44 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
   const R order2 = (R)P.order2;
   const long c = ex ? i + (long)P.nx * j : 0;
   const R* __restrict__ Uin = A.Uin;
-  const double* __restrict__ Xd = (D == 0) ? A.xnx : A.xny;   // coordinate along D
-  const double* __restrict__ Xt = (D == 0) ? A.xny : A.xnx;   // transverse coordinate
   const R* __restrict__ Ud = (D == 0) ? A.ux : A.uy;          // velocity along D
+  const R* __restrict__ Ut = (D == 0) ? A.uy : A.ux;          // transverse velocity
   // Every load of the cell is issued first, from clamped indices (a lane
   // out of the mesh reads cell / node 0), so that a wavefront waits for
   // memory once; the values are selected afterwards.
   const long m0 = ex ? i + (long)nnx * j : 0;
-  const double xa = Xd[m0], xb = Xd[m0 + 1], xcc = Xd[m0 + 1 + nnx], xdd = Xd[m0 + nnx];
   // back face of the cell: the face at node index s along D, nodes
   // (i,j),(i,j+1) for D = 0 and (i,j),(i+1,j) for D = 1
   const bool fin = s >= 0 && s <= N;
   const long na = fin ? i + (long)nnx * j : 0, nb = (D == 0) ? na + nnx : na + 1;
-  const R uda = Ud[na], udb = Ud[nb];
-  const double xfa = Xd[na], xfb = Xd[nb], xta = Xt[na], xtb = Xt[nb];
+  const R uda = Ud[na], udb = Ud[nb], uta = Ut[na], utb = Ut[nb];
+  // the cell's two nodes off its back face: (i+1,j), (i+1,j+1) for D = 0,
+  // (i+1,j+1), (i,j+1) for D = 1
+  const long on1 = (D == 0) ? m0 + 1 : m0 + 1 + nnx, on2 = (D == 0) ? m0 + 1 + nnx : m0 + nnx;
+  const R udon1 = Ud[on1], udon2 = Ud[on2];
+  R uton1 = R(0), uton2 = R(0);
+  if (LIM == kLimG || LIM == kLimO3 || LAG) {
+    uton1 = Ut[on1];
+    uton2 = Ut[on2];
+  }
   R vlag = (FIRST && !LAG) ? A.Vlag[c] : R(1);
   const int fmask = FINAL ? F->mask[c] : 0;  // FINAL: membership mask, loaded with the rest
+  // Lagrangian coordinates x0 + dt u (lag_coord); the back-face nodes (the
+  // cell's first node and, along the face, its second) are the face's
+  const double xfa = lag_coord<D>(P, s, dt, uda), xfb = lag_coord<D>(P, s, dt, udb);
+  const double xta = lag_coord<1 - D>(P, line, dt, uta), xtb = lag_coord<1 - D>(P, line + 1, dt, utb);
+  const double xon1 = lag_coord<D>(P, s + 1, dt, udon1), xon2 = lag_coord<D>(P, s + 1, dt, udon2);
+  // the cell's nodes (i,j), (i+1,j), (i+1,j+1), (i,j+1): coordinate along D
+  const double xa = xfa, xb = (D == 0) ? xon1 : xfb, xcc = (D == 0) ? xon2 : xon1, xdd = (D == 0) ? xfb : xon2;
   // generalised limiters, order 3, LAG: coordinates across D of the cell's nodes
   double tn[4] = {0., 0., 0., 0.};
   if (LIM == kLimG || LIM == kLimO3 || LAG) {
-    tn[0] = Xt[m0];
-    tn[1] = Xt[m0 + 1];
-    tn[2] = Xt[m0 + 1 + nnx];
-    tn[3] = Xt[m0 + nnx];
+    const double ton1 = lag_coord<1 - D>(P, (D == 0) ? line : line + 1, dt, uton1);
+    const double ton2 = lag_coord<1 - D>(P, (D == 0) ? line + 1 : line, dt, uton2);
+    tn[0] = xta;
+    tn[1] = (D == 0) ? ton1 : xtb;
+    tn[2] = (D == 0) ? ton2 : ton1;
+    tn[3] = (D == 0) ? xtb : ton2;
   }
   R pre[3][NEM];
   R preq = R(0);
@@ -539,19 +569,29 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
   const bool fvalid = trow && gi < nx && gs >= 0 && gs <= N;  // back face: nodes (i, s), (i+1, s)
   const long m0 = gex ? gi + (long)nnx * gs : 0;
   const long na = fvalid ? gi + (long)nnx * gs : 0, nb = na + 1;
-  const double ya = A.xny[m0], yb = A.xny[m0 + 1], yc = A.xny[m0 + 1 + nnx], yd = A.xny[m0 + nnx];
-  const R uya = A.uy[na], uyb = A.uy[nb];
-  const double yfa = A.xny[na], yfb = A.xny[nb], xfa = A.xnx[na], xfb = A.xnx[nb];
+  const R uya = A.uy[na], uyb = A.uy[nb], uxa = A.ux[na], uxb = A.ux[nb];
+  // the nodes (i+1, s+1), (i, s+1) of the cell off its back face
+  const R uyc = A.uy[m0 + 1 + nnx], uyd = A.uy[m0 + nnx];
+  R uxc = R(0), uxd = R(0);
+  if (LIM == kLimG || LIM == kLimO3 || LAG) {
+    uxc = A.ux[m0 + 1 + nnx];
+    uxd = A.ux[m0 + nnx];
+  }
   R vlag = (FIRST && !LAG) ? A.Vlag[gc] : R(1);
   // FINAL: the cell's membership mask, loaded with the geometry
   const int fmask = FINAL ? F->mask[gc] : 0;
+  // Lagrangian coordinates x0 + dt u (lag_coord); the cell's first two
+  // nodes are its back face's
+  const double yfa = lag_coord<1>(P, gs, dt, uya), yfb = lag_coord<1>(P, gs, dt, uyb);
+  const double xfa = lag_coord<0>(P, gi, dt, uxa), xfb = lag_coord<0>(P, gi + 1, dt, uxb);
+  const double ya = yfa, yb = yfb, yc = lag_coord<1>(P, gs + 1, dt, uyc), yd = lag_coord<1>(P, gs + 1, dt, uyd);
   // generalised limiters, order 3, LAG: x coordinates of the cell's four nodes
   double tn[4] = {0., 0., 0., 0.};
   if (LIM == kLimG || LIM == kLimO3 || LAG) {
-    tn[0] = A.xnx[m0];
-    tn[1] = A.xnx[m0 + 1];
-    tn[2] = A.xnx[m0 + 1 + nnx];
-    tn[3] = A.xnx[m0 + nnx];
+    tn[0] = xfa;
+    tn[1] = xfb;
+    tn[2] = lag_coord<0>(P, gi + 1, dt, uxc);
+    tn[3] = lag_coord<0>(P, gi, dt, uxd);
   }
   const int nvu = P.nvu;
   R uvol[NEM];
@@ -1024,13 +1064,14 @@ __device__ __forceinline__ void node_sweep_x_2d_body(const Params& P, DevScalars
   // of the mesh reads node 0, and a missing cell adds +0 (front starts at +0
   // and a sum of doubles never becomes -0, so the result is the guarded
   // sum's bit for bit).
-  const double xd_l = A.xnx[nid];
+  // the node's Lagrangian x, x0 + dt u (lag_coord)
+  const double xd_l = lag_coord<0>(P, i, dt, A.ux[nid]);
   // generalised limiters: x of the nodes two places back and front (beyond
   // the lanes a wavefront exchanges)
   double xmm = 0., xpp = 0.;
   if (LIM == kLimG) {
-    xmm = A.xnx[(ex && i >= 2) ? nid - 2 : 0];
-    xpp = A.xnx[(ex && i + 2 <= N) ? nid + 2 : 0];
+    xmm = lag_coord<0>(P, i - 2, dt, A.ux[(ex && i >= 2) ? nid - 2 : 0]);
+    xpp = lag_coord<0>(P, i + 2, dt, A.ux[(ex && i + 2 <= N) ? nid + 2 : 0]);
   }
   R cf[2][NEM];
   bool cv[2];
@@ -1201,7 +1242,7 @@ __device__ __forceinline__ void node_sweep_ytile_2d_body(const Params& P, DevSca
   {
     // every load first, from clamped indices and without branches (one wait
     // for memory; a missing cell adds +0 to a sum that starts at +0)
-    const double xd_l = A.xny[gq];
+    const double xd_l = lag_coord<1>(P, gs, dt, A.uy[gq]);  // the node's Lagrangian y, x0 + dt u
     R cf[2][NEM];
     bool cv[2];
 #pragma unroll
@@ -1278,8 +1319,8 @@ __device__ __forceinline__ void node_sweep_ytile_2d_body(const Params& P, DevSca
   // the tile's halo rows)
   double xmm = 0., xpp = 0.;
   if (LIM == kLimG) {
-    xmm = A.xny[(ex && s >= 2) ? i + (long)nnx * (s - 2) : 0];
-    xpp = A.xny[(ex && s + 2 <= N) ? i + (long)nnx * (s + 2) : 0];
+    xmm = lag_coord<1>(P, s - 2, dt, A.uy[(ex && s >= 2) ? i + (long)nnx * (s - 2) : 0]);
+    xpp = lag_coord<1>(P, s + 2, dt, A.uy[(ex && s + 2 <= N) ? i + (long)nnx * (s + 2) : 0]);
   }
   R g0 = R(0), g1 = R(0);
   if (P.order2 > 0 && ex) {

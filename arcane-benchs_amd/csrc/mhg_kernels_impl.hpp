@@ -91,6 +91,18 @@ struct EnvSlopes {
   }
 };
 
+// Mass-conservative arithmetic of the float precision form.  The reference
+// forms a cell's remapped mass as V sum_k f_k rho_k with rho_k = U_m,k /
+// (f_k V) (RemapADIFinal.cc:446-560) and the mass entering the remap as
+// V (m / V) (PrepareRemap.cc:300-372): both are the masses themselves, and the
+// drop-in path keeps those operations bit for bit.  In float the quotients
+// (hardware reciprocal, within 2 ulp) are biased, and the mass of the 1792x768
+// triple point drifted by 3e-7 over 500 steps (FP64: 5e-12).  The float form
+// therefore takes the stored mass as the remapped one, U_m = m, and the new
+// cell mass as sum_k U_m,k, so that the remap only moves mass between cells.
+template <class R>
+constexpr bool kMassCons = std::is_same<R, float>::value;
+
 // Cell fields the Lagrange cell phase reads, loaded by the caller before the
 // node coordinates so that a wavefront issues all of them together.
 template <class R>
@@ -139,8 +151,10 @@ __device__ inline void lagrange_cell_vals(const Params& P, const DevScalars* S, 
     for (int k = 0; k < NEM; ++k)
       if (k == kk) {
         uv[k] = V;
-        um[k] = V * rho;
-        ue[k] = V * rho * e;
+        // float precision form: the stored mass itself, the quantity the
+        // remap conserves (kMassCons), where the reference forms V (m / V)
+        um[k] = kMassCons<R> ? in.m : V * rho;
+        ue[k] = kMassCons<R> ? um[k] * e : V * rho * e;
       }
     uq = V * qv;
   } else if (mask != 0) {
@@ -158,8 +172,8 @@ __device__ inline void lagrange_cell_vals(const Params& P, const DevScalars* S, 
       const R qnk = PC ? A.env_q[ek] : R(0);
       const R ekv = compute_eint(PC, (R)P.eos.gamma[k], qnk, qk, rho_nk, rk, A.env_p[ek], A.env_e[ek]);
       uv[k] = Vk;
-      um[k] = Vk * rk;
-      ue[k] = Vk * rk * ekv;
+      um[k] = kMassCons<R> ? A.env_m[ek] : Vk * rk;
+      ue[k] = kMassCons<R> ? um[k] * ekv : Vk * rk * ekv;
       uq = Vk * qk;  // PrepareRemap.cc:372 assigns: the last environment wins
     }
   }
@@ -324,6 +338,12 @@ __device__ inline R final_cell_vals(const Params& P, const CellFinalArgsT<R>& A,
       rho_g = qdiv(A.m[c], A.Vlag[c]);  // left as updateDensity set it (m/V)
     }
     R mass_g = vol * density_nplus1;
+    if (kMassCons<R>) {  // the sum of the remapped masses it equals (see kMassCons)
+      mass_g = R(0);
+#pragma unroll
+      for (int k = 0; k < NEM; ++k)
+        if (((mask >> k) & 1) && fv[k] > thr) mass_g += Umas[k];
+    }
     R energie = R(0), pseudo = R(0);
     const R pv = Uq * unsurvol;
     R mk[NEM];

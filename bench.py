@@ -402,20 +402,28 @@ def main():
     nc, nn = sim.nc, sim.nn
     value = nc * args.steps / elapsed
     # accuracy: the configuration's full run (500 steps), FP64 and FP32 bulk
-    rho_acc = {}
+    rho_acc, drift = {}, {}
     for name, prec in [("fp64", abi.PREC_FP64)] + ([] if args.no_precision_form else
                                                     [("fp32", abi.PREC_FP32_BULK)]):
         g2 = cases.triple_point(gpu, NX, NY, precision=prec)
+        c0 = g2.conservation()
         g2.run(args.accuracy_steps)
+        c1 = g2.conservation()
         st2 = g2.download(["density"])
         assert st2["global_iteration"] == args.accuracy_steps
         rho_acc[name] = st2["density"]
+        drift[name] = (c1[0] - c0[0]) / c0[0]
         g2.close()
     if p32 is not None:
         a, r = rho_acc["fp32"], rho_acc["fp64"]
         p32["l1_density_err_vs_fp64"] = float(np.sum(np.abs(a - r)) / np.sum(np.abs(r)))
         p32["linf_density_err_vs_fp64"] = float(np.max(np.abs(a - r)) / np.max(np.abs(r)))
         p32["after_steps"] = args.accuracy_steps
+        # total mass (compensated sum over the cells) after the same steps,
+        # relative to the initial mass; the FP64 path's drift is the
+        # reference scheme's (clamps and 1e-10 thresholds)
+        p32["mass_drift"] = drift["fp32"]
+        p32["mass_drift_fp64"] = drift["fp64"]
     p3, p3_fields, p3_short = (None, None, None) if args.no_p3d else p3d_run(gpu, args, sync)
     pb = None if args.no_batch else batch_run(gpu, args, sync)
     pl = None if args.no_learned else learned_run(gpu, args)
