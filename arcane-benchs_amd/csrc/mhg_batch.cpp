@@ -308,7 +308,7 @@ int mhg_batch_create(int32_t n, const mhg_mesh_desc* meshes, const mhg_options* 
     // the whole batch's working set (as build_params does for one mesh)
     const double wb = opts[0].precision == MHG_PREC_FP32_BULK ? 4. : 8.;
     const double ws = (double)b->NC * wb * (6. * opts[0].nb_env + 20.) + (double)b->NN * 32.;
-    for (int p = 0; p < n; ++p) b->P[p].nt_store = ws > 320e6 ? 1 : 0;
+    for (int p = 0; p < n; ++p) b->P[p].nt_store = ws > 320e6 ? (1 << kStCell) : 0;
   }
   mhg_ctx* m = new mhg_ctx();
   b->master = m;

@@ -118,8 +118,8 @@ __device__ __forceinline__ void node_lagrange_2d_body(const Params& P, DevScalar
     }
   }
   if (VEL || BC) {
-    st_stream(P, &A.ux[nid], (R)vx);
-    st_stream(P, &A.uy[nid], (R)vy);
+    st_stream<3>(P, &A.ux[nid], (R)vx);
+    st_stream<3>(P, &A.uy[nid], (R)vy);
   }
   if (POS) {
     // Euler scheme: saveValuesAtN reset x to x0; Lagrange-only: x moves on
@@ -127,8 +127,8 @@ __device__ __forceinline__ void node_lagrange_2d_body(const Params& P, DevScalar
     const double by = P.lagonly ? A.xpy[nid] : node_y0(P, j);
     // from the velocity as stored (the identity in double), so that the
     // sweeps' x0 + dt u from the stored velocity is the same value
-    st_stream(P, &A.xnx[nid], bx + dt * (double)(R)vx);
-    st_stream(P, &A.xny[nid], by + dt * (double)(R)vy);
+    st_stream<3>(P, &A.xnx[nid], bx + dt * (double)(R)vx);
+    st_stream<3>(P, &A.xny[nid], by + dt * (double)(R)vy);
   }
 }
 
@@ -303,8 +303,8 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
     preq = ex ? uq : R(0);
     vlag = (R)Vd;
     if (out) {
-      st_stream(P, &LA->Vlag[c], (R)Vd);
-      st_stream(P, &LA->L[c], (R)Ld);
+      st_stream<0>(P, &LA->Vlag[c], (R)Vd);
+      st_stream<0>(P, &LA->L[c], (R)Ld);
     }
   } else {
 #pragma unroll
@@ -455,7 +455,7 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
     R un, du;
     sweep_var(pre[0][k], un, du, 0, k);
     if (FINAL) fV[k] = un;
-    else if (out) st_stream(P, &Uout[k * NC + c], un);
+    else if (out) st_stream<1>(P, &Uout[k * NC + c], un);
   }
   if (LEARN) es.close_volumes(ne, alpha);
 #pragma unroll
@@ -466,8 +466,8 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
     if (un < R(0)) un = R(0);  // RemapADIService.cc:953-955
     if (FINAL) fM[k] = un;
     if (out) {
-      if (!FINAL) st_stream(P, &Uout[(ne + k) * NC + c], un);
-      st_stream(P, &A.contrib[k * NC + c], R(0.5) * du);  // fcontrib (DualPhaseRemap.cc:168-185)
+      if (!FINAL) st_stream<1>(P, &Uout[(ne + k) * NC + c], un);
+      st_stream<2>(P, &A.contrib[k * NC + c], R(0.5) * du);  // fcontrib (DualPhaseRemap.cc:168-185)
     }
   }
 #pragma unroll
@@ -477,14 +477,14 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
     sweep_var(pre[2][k], un, du, 2, k);
     if (un < R(0)) un = R(0);  // RemapADIService.cc:956-958
     if (FINAL) fE[k] = un;
-    else if (out) st_stream(P, &Uout[(2 * ne + k) * NC + c], un);
+    else if (out) st_stream<1>(P, &Uout[(2 * ne + k) * NC + c], un);
   }
   R fQ = R(0);
   {
     R un, du;
     sweep_var(preq, un, du, 3, 0);
     if (FINAL) fQ = un;
-    else if (out) st_stream(P, &Uout[(3 * ne) * NC + c], un);
+    else if (out) st_stream<1>(P, &Uout[(3 * ne) * NC + c], un);
   }
   if (FINAL && out) {
     // Euler volume: the corner-normal formula on the initial coordinates
@@ -620,8 +620,8 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
     nxa0 = lU[0];
     nxb0 = lU[1];
     if (gout) {
-      st_stream(P, &LA->Vlag[gc], (R)Vd);
-      st_stream(P, &LA->L[gc], (R)Ld);
+      st_stream<0>(P, &LA->Vlag[gc], (R)Vd);
+      st_stream<0>(P, &LA->L[gc], (R)Ld);
     }
   } else {
 #pragma unroll
@@ -810,11 +810,11 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
     }
     if (gout) {
       if (!FINAL) {
-        st_stream(P, &A.Uout[va * NC + gc], stg[2 * TS + ra]);
-        if (hb) st_stream(P, &A.Uout[vb * NC + gc], stg[3 * TS + ra]);
+        st_stream<1>(P, &A.Uout[va * NC + gc], stg[2 * TS + ra]);
+        if (hb) st_stream<1>(P, &A.Uout[vb * NC + gc], stg[3 * TS + ra]);
       }
-      if (ma) st_stream(P, &A.contrib[(va - ne) * NC + gc], stg[4 * TS + ra]);
-      if (mb) st_stream(P, &A.contrib[(vb - ne) * NC + gc], stg[5 * TS + ra]);
+      if (ma) st_stream<2>(P, &A.contrib[(va - ne) * NC + gc], stg[4 * TS + ra]);
+      if (mb) st_stream<2>(P, &A.contrib[(vb - ne) * NC + gc], stg[5 * TS + ra]);
     }
   };
   if (LEARN || LIM == kLimO3 || FINAL || LAG) {  // compile-time variable indices: per-environment values stay in registers
@@ -1188,18 +1188,18 @@ __device__ __forceinline__ void node_sweep_x_2d_body(const Params& P, DevScalars
     if (fabs(U1) < thresold) U1 = R(0);
   }
   if (FIRST) {
-    st_stream(P, &A.Ud0[nid], U0);
-    st_stream(P, &A.Ud1[nid], U1);
-    st_stream(P, &A.Ud3[nid], U3);
-    if (P.etot) st_stream(P, &A.Ud4[nid], U4);
+    st_stream<2>(P, &A.Ud0[nid], U0);
+    st_stream<2>(P, &A.Ud1[nid], U1);
+    st_stream<2>(P, &A.Ud3[nid], U3);
+    if (P.etot) st_stream<2>(P, &A.Ud4[nid], U4);
   } else if (U3 != R(0)) {
-    st_stream(P, &A.uox[nid], qdiv(U0, U3));
-    st_stream(P, &A.uoy[nid], qdiv(U1, U3));
+    st_stream<3>(P, &A.uox[nid], qdiv(U0, U3));
+    st_stream<3>(P, &A.uoy[nid], qdiv(U1, U3));
   } else {
-    st_stream(P, &A.uox[nid], A.ux[nid]);
-    st_stream(P, &A.uoy[nid], A.uy[nid]);
+    st_stream<3>(P, &A.uox[nid], A.ux[nid]);
+    st_stream<3>(P, &A.uoy[nid], A.uy[nid]);
   }
-  if (!FIRST && P.etot) st_stream(P, &A.dek[nid], dual_ecin_term(U0, U1, U3, U4));
+  if (!FIRST && P.etot) st_stream<2>(P, &A.dek[nid], dual_ecin_term(U0, U1, U3, U4));
 }
 
 // Dual sweep along y on LDS tiles: a block of 16 wavefronts owns 16 node
@@ -1398,18 +1398,18 @@ __device__ __forceinline__ void node_sweep_ytile_2d_body(const Params& P, DevSca
     if (fabs(U1) < thresold) U1 = R(0);
   }
   if (FIRST) {
-    st_stream(P, &A.Ud0[gq], U0);
-    st_stream(P, &A.Ud1[gq], U1);
-    st_stream(P, &A.Ud3[gq], U3);
-    if (P.etot) st_stream(P, &A.Ud4[gq], U4);
+    st_stream<2>(P, &A.Ud0[gq], U0);
+    st_stream<2>(P, &A.Ud1[gq], U1);
+    st_stream<2>(P, &A.Ud3[gq], U3);
+    if (P.etot) st_stream<2>(P, &A.Ud4[gq], U4);
   } else if (U3 != R(0)) {
-    st_stream(P, &A.uox[gq], qdiv(U0, U3));
-    st_stream(P, &A.uoy[gq], qdiv(U1, U3));
+    st_stream<3>(P, &A.uox[gq], qdiv(U0, U3));
+    st_stream<3>(P, &A.uoy[gq], qdiv(U1, U3));
   } else {
-    st_stream(P, &A.uox[gq], A.ux[gq]);
-    st_stream(P, &A.uoy[gq], A.uy[gq]);
+    st_stream<3>(P, &A.uox[gq], A.ux[gq]);
+    st_stream<3>(P, &A.uoy[gq], A.uy[gq]);
   }
-  if (!FIRST && P.etot) st_stream(P, &A.dek[gq], dual_ecin_term(U0, U1, U3, U4));
+  if (!FIRST && P.etot) st_stream<2>(P, &A.dek[gq], dual_ecin_term(U0, U1, U3, U4));
 }
 
 }  // namespace mhg

@@ -498,10 +498,11 @@ struct Params {
   // entry-point kernels): pb the pente-borne fluxes with debar fix pb_fix,
   // mixte the pure-neighbourhood switch to limiter pure_id
   int generic, pb, pb_fix, mixte, pure_id;
-  // store policy of the step kernels' outputs: 1 non-temporal (the lines
-  // stream to memory), 0 ordinary stores (they stay in the L2 / memory-side
-  // cache for the next kernel of the step); chosen per context by the
-  // working set against the 256 MB memory-side cache, settable by the caller
+  // store policy of the step kernels' outputs, one bit per kind of array
+  // (st_stream): set, non-temporal stores (the lines stream to memory);
+  // clear, ordinary stores (they stay in the L2 / memory-side cache for the
+  // next kernel of the step); chosen per context from the working set,
+  // settable by the caller
   int nt_store;
   double cfl, dtmin, dtmax, tend;
   int nbc;
@@ -517,9 +518,15 @@ struct Params {
 // (The ordinary store is written as a relaxed wavefront-scope atomic store,
 // which is the same plain global store: as `*p = v` the compiler merges the
 // two branches into one store and drops the non-temporal hint.)
-template <class T>
+// CLS names the kind of array (bit CLS of Params::nt_store selects streaming
+// for it): kStCell the cell state and geometry the next step reads, kStU the
+// remap vector between the cell sweeps, kStFlux the dual mass fluxes and dual
+// state between a cell sweep and the dual sweeps, kStNode node velocities
+// and coordinates.
+constexpr int kStCell = 0, kStU = 1, kStFlux = 2, kStNode = 3;
+template <int CLS, class T>
 __device__ inline void st_stream(const Params& P, T* p, T v) {
-  if (P.nt_store) __builtin_nontemporal_store(v, p);
+  if ((P.nt_store >> CLS) & 1) __builtin_nontemporal_store(v, p);
   else __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
 }
 

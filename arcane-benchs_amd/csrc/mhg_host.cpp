@@ -695,16 +695,18 @@ int build_params(const mhg_mesh_desc* mesh, const mhg_options* opt, Params* Pout
   P.tend = opt->final_time;
   P.land = opt->land_on_final_time ? 1 : 0;
   {
-    // store policy (measured, ms per step, ordinary / streaming stores):
-    // 2D keeps ordinary stores while the arrays one step touches fit the 256
-    // MB memory-side cache with room to spare, so that a later kernel of the
-    // step reads what an earlier one wrote from that cache (1792x768 triple
-    // point, FP32 bulk, ~250 MB: 0.124 / 0.159), and streams beyond it (FP64,
-    // ~460 MB: 0.264 / 0.256); 3D keeps ordinary stores (256^3: FP64 5.64 /
-    // 5.79, FP32 4.19 / 4.49)
+    // store policy (st_stream): the remap vector, the dual fluxes and the
+    // node velocities are read again by the next kernels of the same step and
+    // are written with ordinary stores, which keep them in the L2 and the
+    // 256 MB memory-side cache; the cell state is read again only by the
+    // next step, and streams to memory once the arrays one step touches
+    // exceed that cache.  Measured, ms per step (all ordinary / cell state
+    // streaming / all streaming): 1792x768 triple point FP64 (~460 MB)
+    // 0.261 / 0.246 / 0.255, FP32 bulk (~250 MB) 0.120 / 0.133 / 0.149;
+    // 256^3 FP64 5.65 / 5.59 / 5.83, FP32 4.19 / 4.18 / 4.51.
     const double wb = opt->precision == MHG_PREC_FP32_BULK ? 4. : 8.;
     const double ws = (double)P.nc * wb * (6. * opt->nb_env + 20.) + (double)P.nn * 16. * P.dim;
-    P.nt_store = (P.dim == 2 && ws > 320e6) ? 1 : 0;
+    P.nt_store = ws > 320e6 ? (1 << kStCell) : 0;
   }
   P.nbc = 0;
   for (int b = 0; b < opt->nb_bc; ++b) {
@@ -1640,13 +1642,18 @@ int variant_count_of(int k) {
   if (k == 0) return cell_sweep_y_variants();
   if (k == 1) return node_sweep_variants(0);
   if (k == 2) return node_sweep_variants(1);
-  if (k >= 3 && k <= 6) return 2;
+  if (k >= 3 && k <= 5) return 2;
+  if (k == 6) return 16;  // one bit per kind of array (st_stream)
   return 0;
 }
 const char* variant_name_of(int k, int v) {
   if (k == 3) return v ? "graph12" : "direct";
   if (k == 4 || k == 5) return v ? "fused" : "separate";
-  if (k == 6) return v ? "streaming" : "cached";
+  if (k == 6) {
+    static const char* names[16] = {"cached", "s1", "s2", "s3", "s4", "s5", "s6", "s7",
+                                    "s8", "s9", "s10", "s11", "s12", "s13", "s14", "streaming"};
+    return v >= 0 && v < 16 ? names[v] : "?";
+  }
   return k == 0 ? cell_sweep_y_variant_name(v) : node_sweep_variant_name(k - 1, v);
 }
 int* variant_slot(mhg_ctx* c, int k) {

@@ -390,10 +390,10 @@ __global__ __launch_bounds__(64) void k_cell_sweep_y_2d(Params P, DevScalars* S,
           // cell faces in order: bottom (back), right, top (front), left
           double un = U2[s] - (flux_b + flux_f);
           if (v >= ne && v < 3 * ne && un < 0.) un = 0.;
-          st_stream(P, &Uout[v * NC + co], un);
+          st_stream<1>(P, &Uout[v * NC + co], un);
           if (v >= ne && v < 2 * ne) {
             const double dual = 0.5 * flux_b * -1.0 + 0.5 * flux_f * 1.0;
-            st_stream(P, &A.contrib[(v - ne) * NC + co], 0.5 * dual);
+            st_stream<2>(P, &A.contrib[(v - ne) * NC + co], 0.5 * dual);
           }
         }
         ph2[s] = ph1[s];
@@ -545,16 +545,16 @@ __global__ void k_node_sweep_2d(Params P, DevScalars* S, NodeSweepArgs A) {
     if (fabs(U1) < thresold) U1 = 0.;
   }
   if (FIRST) {
-    st_stream(P, &A.Ud0[nid], U0);
-    st_stream(P, &A.Ud1[nid], U1);
-    st_stream(P, &A.Ud3[nid], U3);
+    st_stream<2>(P, &A.Ud0[nid], U0);
+    st_stream<2>(P, &A.Ud1[nid], U1);
+    st_stream<2>(P, &A.Ud3[nid], U3);
   } else {
     if (U3 != 0.) {
-      st_stream(P, &A.uox[nid], U0 / U3);
-      st_stream(P, &A.uoy[nid], U1 / U3);
+      st_stream<3>(P, &A.uox[nid], U0 / U3);
+      st_stream<3>(P, &A.uoy[nid], U1 / U3);
     } else {
-      st_stream(P, &A.uox[nid], A.ux[nid]);
-      st_stream(P, &A.uoy[nid], A.uy[nid]);
+      st_stream<3>(P, &A.uox[nid], A.ux[nid]);
+      st_stream<3>(P, &A.uoy[nid], A.uy[nid]);
     }
   }
 }
@@ -706,15 +706,15 @@ __global__ __launch_bounds__(64) void k_node_sweep_y_2d(Params P, DevScalars* S,
       if (fabs(V1) < thresold) V1 = 0.;
     }
     if (FIRST) {
-      st_stream(P, &A.Ud0[nid], V0);
-      st_stream(P, &A.Ud1[nid], V1);
-      st_stream(P, &A.Ud3[nid], V3);
+      st_stream<2>(P, &A.Ud0[nid], V0);
+      st_stream<2>(P, &A.Ud1[nid], V1);
+      st_stream<2>(P, &A.Ud3[nid], V3);
     } else if (V3 != 0.) {
-      st_stream(P, &A.uox[nid], V0 / V3);
-      st_stream(P, &A.uoy[nid], V1 / V3);
+      st_stream<3>(P, &A.uox[nid], V0 / V3);
+      st_stream<3>(P, &A.uoy[nid], V1 / V3);
     } else {
-      st_stream(P, &A.uox[nid], A.ux[nid]);
-      st_stream(P, &A.uoy[nid], A.uy[nid]);
+      st_stream<3>(P, &A.uox[nid], A.ux[nid]);
+      st_stream<3>(P, &A.uoy[nid], A.uy[nid]);
     }
   }
 }

@@ -99,9 +99,9 @@ __global__ __launch_bounds__(256) void k_node_lagrange_3d(Params P, DevScalars* 
     }
   }
   if (VEL || BC) {
-    st_stream(P, &A.b.ux[nid], (R)vx);
-    st_stream(P, &A.b.uy[nid], (R)vy);
-    st_stream(P, &A.uz[nid], (R)vz);
+    st_stream<3>(P, &A.b.ux[nid], (R)vx);
+    st_stream<3>(P, &A.b.uy[nid], (R)vy);
+    st_stream<3>(P, &A.uz[nid], (R)vz);
   }
   if (POS) {
     // Euler scheme: from x0; Lagrange-only (with-projection = false): from
@@ -109,9 +109,9 @@ __global__ __launch_bounds__(256) void k_node_lagrange_3d(Params P, DevScalars* 
     const double bx = P.lagonly ? A.b.xpx[nid] : x0_3d(P, 0, i);
     const double by = P.lagonly ? A.b.xpy[nid] : x0_3d(P, 1, j);
     const double bz = P.lagonly ? A.xpz[nid] : x0_3d(P, 2, k);
-    st_stream(P, &A.b.xnx[nid], bx + dt * vx);
-    st_stream(P, &A.b.xny[nid], by + dt * vy);
-    st_stream(P, &A.xnz[nid], bz + dt * vz);
+    st_stream<3>(P, &A.b.xnx[nid], bx + dt * vx);
+    st_stream<3>(P, &A.b.xny[nid], by + dt * vy);
+    st_stream<3>(P, &A.xnz[nid], bz + dt * vz);
   }
 }
 
@@ -211,15 +211,15 @@ __global__ __launch_bounds__(256) void k_node_lagrange_3d_tiled(Params P, DevSca
           else vz = r.value;
         }
       }
-      st_stream(P, &A.b.ux[nid], (R)vx);
-      st_stream(P, &A.b.uy[nid], (R)vy);
-      st_stream(P, &A.uz[nid], (R)vz);
+      st_stream<3>(P, &A.b.ux[nid], (R)vx);
+      st_stream<3>(P, &A.b.uy[nid], (R)vy);
+      st_stream<3>(P, &A.uz[nid], (R)vz);
       const double bx = P.lagonly ? A.b.xpx[nid] : x0_3d(P, 0, ni);
       const double by = P.lagonly ? A.b.xpy[nid] : x0_3d(P, 1, nj);
       const double bz = P.lagonly ? A.xpz[nid] : x0_3d(P, 2, kc);
-      st_stream(P, &A.b.xnx[nid], bx + dt * vx);
-      st_stream(P, &A.b.xny[nid], by + dt * vy);
-      st_stream(P, &A.xnz[nid], bz + dt * vz);
+      st_stream<3>(P, &A.b.xnx[nid], bx + dt * vx);
+      st_stream<3>(P, &A.b.xny[nid], by + dt * vy);
+      st_stream<3>(P, &A.xnz[nid], bz + dt * vz);
       Fn = V3{0., 0., 0.};
       mn = 0.;
       if (kc < P.nz) add_plane(Fn, mn, true);
@@ -506,14 +506,14 @@ __global__ __launch_bounds__(256) void k_cell_sweep_x_3d(Params P, DevScalars* S
       sweep_var(pre[g][q], un, du, g, q);
       if (g > 0 && un < R(0)) un = R(0);  // RemapADIService.cc:953-958
       if (out) {
-        st_stream(P, &Uout[(g * ne + q) * NC + c], un);
-        if (g == 1) st_stream(P, &A.b.contrib[q * NC + c], R(0.25) * du);
+        st_stream<1>(P, &Uout[(g * ne + q) * NC + c], un);
+        if (g == 1) st_stream<2>(P, &A.b.contrib[q * NC + c], R(0.25) * du);
       }
     }
   {
     R un, du;
     sweep_var(preq, un, du, 3, 0);
-    if (out) st_stream(P, &Uout[(3 * ne) * NC + c], un);
+    if (out) st_stream<1>(P, &Uout[(3 * ne) * NC + c], un);
   }
 }
 
@@ -697,10 +697,10 @@ __global__ __launch_bounds__(1024) void k_cell_sweep_tile_3d(Params P, DevScalar
     if (mb) stg[5][lane][w] = R(0.25) * dub;
     __syncthreads();
     if (gout) {
-      st_stream(P, &A.b.Uout[va * NC + gc], stg[2][tr][tc]);
-      if (hb) st_stream(P, &A.b.Uout[vb * NC + gc], stg[3][tr][tc]);
-      if (ma) st_stream(P, &A.b.contrib[(va - ne) * NC + gc], stg[4][tr][tc]);
-      if (mb) st_stream(P, &A.b.contrib[(vb - ne) * NC + gc], stg[5][tr][tc]);
+      st_stream<1>(P, &A.b.Uout[va * NC + gc], stg[2][tr][tc]);
+      if (hb) st_stream<1>(P, &A.b.Uout[vb * NC + gc], stg[3][tr][tc]);
+      if (ma) st_stream<2>(P, &A.b.contrib[(va - ne) * NC + gc], stg[4][tr][tc]);
+      if (mb) st_stream<2>(P, &A.b.contrib[(vb - ne) * NC + gc], stg[5][tr][tc]);
     }
   };
   if (LIM == kLimO3) {  // compile-time variable indices: per-environment values stay in registers
@@ -853,29 +853,29 @@ template <class R, bool FIRST, bool LAST>
 __device__ inline void dual_store(const Params& P, const NodeSweep3ArgsT<R>& A, long nid, const R* U, R U4) {
   if (P.etot) {
     if (!LAST) {
-      st_stream(P, &A.Udo4[nid], U4);
+      st_stream<2>(P, &A.Udo4[nid], U4);
     } else {  // the node's term of the total-energy deposit (RemapADIFinal.cc:757-764)
       R t = R(0);
       if (U[3] != R(0)) {
         const R vx = qdiv(U[0], U[3]), vy = qdiv(U[1], U[3]), vz = qdiv(U[2], U[3]);
         t = R(0.25) * (qdiv(U4, U[3]) - R(0.5) * (vx * vx + vy * vy + vz * vz));
       }
-      st_stream(P, &A.dek[nid], t);
+      st_stream<2>(P, &A.dek[nid], t);
     }
   }
   if (!LAST) {
-    st_stream(P, &A.Udo0[nid], U[0]);
-    st_stream(P, &A.Udo1[nid], U[1]);
-    st_stream(P, &A.Udo2[nid], U[2]);
-    st_stream(P, &A.Udo3[nid], U[3]);
+    st_stream<2>(P, &A.Udo0[nid], U[0]);
+    st_stream<2>(P, &A.Udo1[nid], U[1]);
+    st_stream<2>(P, &A.Udo2[nid], U[2]);
+    st_stream<2>(P, &A.Udo3[nid], U[3]);
   } else if (U[3] != R(0)) {  // RemapADIFinal.cc:660-672
-    st_stream(P, &A.uox[nid], qdiv(U[0], U[3]));
-    st_stream(P, &A.uoy[nid], qdiv(U[1], U[3]));
-    st_stream(P, &A.uoz[nid], qdiv(U[2], U[3]));
+    st_stream<3>(P, &A.uox[nid], qdiv(U[0], U[3]));
+    st_stream<3>(P, &A.uoy[nid], qdiv(U[1], U[3]));
+    st_stream<3>(P, &A.uoz[nid], qdiv(U[2], U[3]));
   } else {
-    st_stream(P, &A.uox[nid], A.ux[nid]);
-    st_stream(P, &A.uoy[nid], A.uy[nid]);
-    st_stream(P, &A.uoz[nid], A.uz[nid]);
+    st_stream<3>(P, &A.uox[nid], A.ux[nid]);
+    st_stream<3>(P, &A.uoy[nid], A.uy[nid]);
+    st_stream<3>(P, &A.uoz[nid], A.uz[nid]);
   }
 }
 

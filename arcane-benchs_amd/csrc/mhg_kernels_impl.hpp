@@ -193,13 +193,13 @@ __device__ inline void lagrange_cell_bulk(const Params& P, const DevScalars* S, 
 #pragma unroll
   for (int k = 0; k < NEM; ++k) {
     if (k >= ne) break;
-    st_stream(P, &U[k * NC + c], uv[k]);
-    st_stream(P, &U[(ne + k) * NC + c], um[k]);
-    st_stream(P, &U[(2 * ne + k) * NC + c], ue[k]);
+    st_stream<1>(P, &U[k * NC + c], uv[k]);
+    st_stream<1>(P, &U[(ne + k) * NC + c], um[k]);
+    st_stream<1>(P, &U[(2 * ne + k) * NC + c], ue[k]);
   }
-  st_stream(P, &U[(3 * ne) * NC + c], uq);
-  st_stream(P, &A.Vlag[c], V);
-  st_stream(P, &A.L[c], L);
+  st_stream<1>(P, &U[(3 * ne) * NC + c], uq);
+  st_stream<0>(P, &A.Vlag[c], V);
+  st_stream<0>(P, &A.L[c], L);
 }
 
 // Final cell phase of cell c with Euler volume vold: membership, "moy", EOS,
@@ -433,15 +433,15 @@ __device__ inline R final_cell_vals(const Params& P, const CellFinalArgsT<R>& A,
         }
       }
     }
-    st_stream(P, &A.rho[c], rho_g);
-    st_stream(P, &A.e[c], e_g);
+    st_stream<0>(P, &A.rho[c], rho_g);
+    st_stream<0>(P, &A.e[c], e_g);
     if (EOS) {
-      st_stream(P, &A.p[c], p_g);
-      st_stream(P, &A.cs[c], c_g);
+      st_stream<0>(P, &A.p[c], p_g);
+      st_stream<0>(P, &A.cs[c], c_g);
     }
-    st_stream(P, &A.q[c], q_g);
-    st_stream(P, &A.m[c], mass_g);
-    st_stream(P, &A.mask[c], mask);
+    st_stream<0>(P, &A.q[c], q_g);
+    st_stream<0>(P, &A.m[c], mass_g);
+    st_stream<0>(P, &A.mask[c], mask);
     return EOS ? c_g : R(0);
   }
 }

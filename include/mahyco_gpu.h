@@ -289,9 +289,12 @@ int mhg_autotune(mhg_ctx* ctx, int32_t reps, char* report, int32_t report_len);
  * replay; "final_fusion": the fused step's final cell phase as a kernel of
  * its own or inside the last cell sweep; "lagrange_fusion": its Lagrange
  * cell phase as a kernel of its own or inside the first cell sweep;
- * "store_policy": the step kernels' outputs written with ordinary stores
- * (0, they stay in the caches for the next kernel) or streaming stores (1);
- * the default follows the working set), 0 for an unknown name. */
+ * "store_policy": which of the step kernels' outputs are written with
+ * streaming stores, one bit per kind of array (1 the cell state, 2 the remap
+ * vector between the cell sweeps, 4 the dual fluxes and dual state, 8 node
+ * velocities and coordinates; 0 all ordinary stores, which stay in the
+ * caches for the next kernel, 15 all streaming); the default follows the
+ * working set), 0 for an unknown name. */
 int32_t mhg_variant_count(const char* kernel);
 /* Selects launch variant `variant` of a tunable kernel for later steps
  * (MHG_E_ARG for an unknown kernel or an index out of range). */

@@ -135,13 +135,14 @@ def test_3d_store_policy_bit_identical(gpu, prec):
     """Ordinary and streaming stores of the 3D step kernels give the same
     state bit for bit."""
     states = []
-    for v in (0, 1):
+    for v in (0, 1, 6, 15):
         s = box3(gpu, precision=prec)
         s.select_variant(v, "store_policy")
         s.run(9)
         states.append(s.download())
-    for f, v in states[0].items():
-        if isinstance(v, np.ndarray):
-            assert np.array_equal(v, states[1][f]), f
-        else:
-            assert v == states[1][f], f
+    for other in states[1:]:
+        for f, v in states[0].items():
+            if isinstance(v, np.ndarray):
+                assert np.array_equal(v, other[f]), f
+            else:
+                assert v == other[f], f

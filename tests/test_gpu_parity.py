@@ -349,13 +349,14 @@ def test_store_policy_bit_identical(gpu, prec):
     where the written lines go, never a value: both give the same state bit
     for bit, FP64 and FP32 bulk."""
     states = []
-    for v in (0, 1):
+    for v in (0, 1, 6, 15):
         s = H.tp_setup(gpu, 70, 134, precision=prec)
         s.select_variant(v, "store_policy")
         s.run(13)
         states.append(s.download())
-    for f, v in states[0].items():
-        if isinstance(v, np.ndarray):
-            assert np.array_equal(v, states[1][f]), f
-        else:
-            assert v == states[1][f], f
+    for other in states[1:]:
+        for f, v in states[0].items():
+            if isinstance(v, np.ndarray):
+                assert np.array_equal(v, other[f]), f
+            else:
+                assert v == other[f], f
