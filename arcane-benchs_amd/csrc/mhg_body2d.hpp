@@ -58,6 +58,7 @@ __device__ __forceinline__ void node_lagrange_2d_body(const Params& P, DevScalar
     // 3x3 patch of previous coordinates around the node.
     double X[3][3], Y[3][3];
     bool nv[3][3];
+    const bool prev_u = A.upx != nullptr;  // uniform over the launch
 #pragma unroll
     for (int dj = -1; dj <= 1; ++dj)
 #pragma unroll
@@ -66,8 +67,13 @@ __device__ __forceinline__ void node_lagrange_2d_body(const Params& P, DevScalar
         const bool ok = ii >= 0 && ii < nnx && jj >= 0 && jj <= P.ny;
         const long q = ok ? ii + (long)nnx * jj : 0;
         nv[dj + 1][di + 1] = ok;
-        X[dj + 1][di + 1] = A.xpx[q];
-        Y[dj + 1][di + 1] = A.xpy[q];
+        if (prev_u) {  // x0 + dt_old u of the previous Lagrange phase (lag_coord)
+          X[dj + 1][di + 1] = lag_coord<0>(P, ii, S->dt_old, A.upx[q]);
+          Y[dj + 1][di + 1] = lag_coord<1>(P, jj, S->dt_old, A.upy[q]);
+        } else {
+          X[dj + 1][di + 1] = A.xpx[q];
+          Y[dj + 1][di + 1] = A.xpy[q];
+        }
       }
     // cells in ascending id: (i-1,j-1) corner 2, (i,j-1) 3, (i-1,j) 1, (i,j) 0
     const int cdi[4] = {-1, 0, -1, 0}, cdj[4] = {-1, -1, 0, 0}, cq[4] = {2, 3, 1, 0};
@@ -832,8 +838,14 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
       fM[k] = (k < ne) ? fU[ne + k] : R(0);
       fE[k] = (k < ne) ? fU[2 * ne + k] : R(0);
     }
-    double x0[4] = {node_x0(P, gi), node_x0(P, gi + 1), node_x0(P, gi + 1), node_x0(P, gi)};
-    double y0[4] = {node_y0(P, gs), node_y0(P, gs), node_y0(P, gs + 1), node_y0(P, gs + 1)};
+    // the Euler volume from the initial node positions, formed here: the
+    // opaque copies of the indices keep the compiler from carrying the
+    // geometry phase's equal x0 / y0 values through the rounds (with 64
+    // VGPRs that spilled 36 bytes per lane)
+    int fi = gi, fs = gs;
+    asm volatile("" : "+v"(fi), "+v"(fs));
+    double x0[4] = {node_x0(P, fi), node_x0(P, fi + 1), node_x0(P, fi + 1), node_x0(P, fi)};
+    double y0[4] = {node_y0(P, fs), node_y0(P, fs), node_y0(P, fs + 1), node_y0(P, fs + 1)};
     final_cell_vals<R, NEM>(P, *F, gc, quad_volume(x0, y0), fV, fM, fE, fU[3 * ne], R(0), fmask);
   }
 }

@@ -71,6 +71,12 @@ struct mhg_ctx {
 
   // host mirrors of the step bookkeeping
   int xcur = 0;  // buffer with the coordinates of the last Lagrange phase
+  // 2D fused steps whose kernels all form node coordinates as x0 + dt u do
+  // not store them: xx[xcur], xy[xcur] are then stale, and the last Lagrange
+  // phase's coordinates are x0 + dt_old u[1 - ucur] (materialize_coords
+  // writes them where a caller needs the arrays)
+  bool x_lazy = false;
+  bool x_lazy_before = false;  // at the start of the last stepping call
   int ucur = 0;  // buffer with the current velocity
   int sens = 0;  // SensProjection
   // launch variants (mhg_autotune); the defaults are the ones it picks on
@@ -94,6 +100,7 @@ struct mhg_ctx {
   // captured groups of fused iterations, one per host buffer state
   struct StepGraph {
     int xcur, ucur, sens;
+    bool lazy, lazy_end;  // x_lazy at the start and at the end of the group
     hipGraphExec_t exec;
   };
   std::vector<StepGraph> graphs;

@@ -259,16 +259,44 @@ void enqueue_step_t(mhg_ctx* c) {
     TimedLaunch t(c, "learned_alpha");
     enqueue_learned_alpha<R>(c, nullptr, nullptr, true);
   }
+  const bool lag = c->fuse_lag && fused_lag_supported(c->P);
+  // the node coordinates are stored only if a kernel of this step reads them:
+  // the fused cell sweeps and the lane / tile dual sweeps form them from the
+  // velocities, the separate cell phase and the FP64 column-walker and
+  // stencil dual sweeps read them
+  const bool formed = lag && c->fuse_final && fused_final_supported(c->P) &&
+                      (std::is_same<R, float>::value || (c->nxvariant == 1 && c->nyvariant == 4));
   {
     TimedLaunch t(c, "node_lagrange");
-    launch_node_lagrange_2d(c->stream, c->P, c->S, node_lag_args<R>(c), true, true, true);
+    NodeLagArgsT<R> A = node_lag_args<R>(c);
+    if (c->x_lazy) {  // the previous coordinates from the previous Lagrangian velocity
+      A.upx = bp<R>(c->ux[1 - c->ucur]);
+      A.upy = bp<R>(c->uy[1 - c->ucur]);
+    }
+    launch_node_lagrange_2d(c->stream, c->P, c->S, A, true, true, !formed);
   }
-  const bool lag = c->fuse_lag && fused_lag_supported(c->P);
   if (!lag) {
     TimedLaunch t(c, "cell_lagrange");
     launch_cell_lagrange_2d(c->stream, c->P, c->S, cell_lag_args<R>(c));
   }
   remap_2d<R>(c, 2, true, lag);
+  c->x_lazy = formed;
+}
+
+// Writes the last Lagrange phase's node coordinates, x0 + dt_old u[1 - ucur],
+// into xx[xcur], xy[xcur] after fused steps that did not store them (the
+// values the node phase would have stored, bit for bit).
+template <class R>
+int materialize_coords_t(mhg_ctx* c) {
+  launch_lag_coords_2d(c->stream, c->P, c->S, bp<R>(c->ux[1 - c->ucur]), bp<R>(c->uy[1 - c->ucur]),
+                       c->xx[c->xcur], c->xy[c->xcur]);
+  HIPCHK(c, hipGetLastError());
+  c->x_lazy = false;
+  return MHG_OK;
+}
+int materialize_coords(mhg_ctx* c) {
+  if (!c->x_lazy) return MHG_OK;
+  return c->bulk == 4 ? materialize_coords_t<float>(c) : materialize_coords_t<double>(c);
 }
 
 // 3D remap: three cell sweeps and three dual sweeps; the dual state goes
@@ -431,10 +459,14 @@ namespace {
 int enqueue_steps(mhg_ctx* c, int n) {
   while (c->use_graphs && !c->timing && n >= kGraphSteps) {
     hipGraphExec_t exec = nullptr;
+    bool lazy_end = false;
     for (auto& g : c->graphs)
-      if (g.xcur == c->xcur && g.ucur == c->ucur && g.sens == c->sens) exec = g.exec;
+      if (g.xcur == c->xcur && g.ucur == c->ucur && g.sens == c->sens && g.lazy == c->x_lazy) {
+        exec = g.exec;
+        lazy_end = g.lazy_end;
+      }
     if (!exec) {
-      mhg_ctx::StepGraph g{c->xcur, c->ucur, c->sens, nullptr};
+      mhg_ctx::StepGraph g{c->xcur, c->ucur, c->sens, c->x_lazy, false, nullptr};
       HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       for (int s = 0; s < kGraphSteps; ++s) enqueue_step(c);
       hipGraph_t graph = nullptr;
@@ -442,10 +474,13 @@ int enqueue_steps(mhg_ctx* c, int n) {
       const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
       hipGraphDestroy(graph);
       HIPCHK(c, e);
+      g.lazy_end = c->x_lazy;
       c->graphs.push_back(g);
       exec = g.exec;
+      lazy_end = g.lazy_end;
     }
     HIPCHK(c, hipGraphLaunch(exec, c->stream));
+    c->x_lazy = lazy_end;  // the host state the group's last step leaves
     n -= kGraphSteps;
   }
   for (int s = 0; s < n; ++s) enqueue_step(c);
@@ -456,7 +491,9 @@ int enqueue_steps(mhg_ctx* c, int n) {
 // Undoes the host bookkeeping (coordinate and velocity buffers, sweep order)
 // of `skipped` enqueued iterations that the device did not run because the
 // run had ended.
-void roll_back(mhg_ctx* c, int skipped) {
+void roll_back(mhg_ctx* c, int skipped, int done) {
+  // no step ran: the coordinate arrays are as the call found them
+  if (done == 0) c->x_lazy = c->x_lazy_before;
   for (int s = 0; s < skipped; ++s) {
     c->sens = (c->sens + c->P.dim - 1) % c->P.dim;
     c->xcur = 1 - c->xcur;
@@ -474,14 +511,19 @@ int reconcile_async(mhg_ctx* c) {
   if (r) return r;
   collect_times(c);
   const int done = c->S_host->iteration - c->async_it0;
-  roll_back(c, launched - done);
+  roll_back(c, launched - done, done);
   return MHG_OK;
 }
 
-int check_ready(mhg_ctx* c) {
+// The context is initialised and the host bookkeeping follows the device;
+// with coords, the node coordinate arrays hold the last Lagrange phase's
+// coordinates (every call but the fused steps, which form them).
+int check_ready(mhg_ctx* c, bool coords = true) {
   if (!c) return MHG_E_ARG;
   if (!c->initialised) return fail(c, MHG_E_STATE, "context not initialised");
-  return reconcile_async(c);
+  const int r = reconcile_async(c);
+  if (r || !coords) return r;
+  return materialize_coords(c);
 }
 
 // A problem of a batch is advanced by mhg_batch_run only.
@@ -492,7 +534,7 @@ int check_not_batch(mhg_ctx* c) {
 
 // The fused step starts from a state between iterations.
 int check_fused(mhg_ctx* c) {
-  int r = check_ready(c);
+  int r = check_ready(c, false);
   if (r) return r;
   r = check_not_batch(c);
   if (r) return r;
@@ -562,6 +604,7 @@ int init_pure(mhg_ctx* c, const std::vector<int>& env, const std::vector<double>
   }
   c->xcur = 0;
   c->ucur = 0;
+  c->x_lazy = false;
   c->coord_phase = 0;
   c->ep_remapped = false;
   c->sens = 0;
@@ -912,6 +955,7 @@ int mhg_upload_state(mhg_ctx* c, const mhg_state* st) {
   const size_t nb = NN * sizeof(double);
   c->xcur = 0;
   c->ucur = 0;
+  c->x_lazy = false;
   c->coord_phase = 0;
   c->ep_remapped = false;
   c->async_launched = 0;
@@ -1511,11 +1555,12 @@ int mhg_step(mhg_ctx* c, int32_t* stop) {
   r = read_scalars(c);
   if (r) return r;
   const int it0 = c->S_host->iteration;
+  c->x_lazy_before = c->x_lazy;
   enqueue_step(c);
   HIPCHK(c, hipGetLastError());
   r = read_scalars(c);
   if (r) return r;
-  roll_back(c, 1 - (c->S_host->iteration - it0));  // a step after the end is skipped
+  roll_back(c, 1 - (c->S_host->iteration - it0), c->S_host->iteration - it0);  // a step after the end is skipped
   collect_times(c);
   if (stop) *stop = c->S_host->stop;
   if (c->S_host->err) return fail(c, MHG_E_DT_MIN, "dt below deltat-min");
@@ -1532,6 +1577,7 @@ int mhg_run_async(mhg_ctx* c, int32_t n) {
   r = read_scalars(c);
   if (r) return r;
   c->async_it0 = c->S_host->iteration;
+  c->x_lazy_before = c->x_lazy;
   r = enqueue_steps(c, n);
   c->async_launched = n;
   return r;
@@ -1561,6 +1607,7 @@ int mhg_run(mhg_ctx* c, int32_t max_steps, int32_t* steps_done) {
   r = read_scalars(c);
   if (r) return r;
   const int it0 = c->S_host->iteration;
+  c->x_lazy_before = c->x_lazy;
   int launched = 0;
   while (launched < max_steps) {
     const int chunk = std::min(5 * kGraphSteps, max_steps - launched);
@@ -1575,7 +1622,7 @@ int mhg_run(mhg_ctx* c, int32_t max_steps, int32_t* steps_done) {
   // kernels of steps past the end of the run return at once; the host
   // bookkeeping (buffers, sweep order) must follow the steps really done
   const int done = c->S_host->iteration - it0;
-  roll_back(c, launched - done);
+  roll_back(c, launched - done, done);
   if (steps_done) *steps_done = done;
   if (c->S_host->err) return fail(c, MHG_E_DT_MIN, "dt below deltat-min");
   return c->S_host->stop ? MHG_STOP : MHG_OK;

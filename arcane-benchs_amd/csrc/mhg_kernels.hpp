@@ -14,6 +14,10 @@ struct NodeLagArgsT {
   const double *xpx, *xpy;    // node coordinates of the previous Lagrange phase
   R *ux, *uy;                 // node velocity (updated in place)
   double *xnx, *xny;          // new Lagrangian coordinates
+  // set: the previous Lagrange phase's coordinates are formed as x0 + dt_old
+  // u from its Lagrangian velocity (2D fused steps of the Euler scheme, whose
+  // coordinates are not stored); null: read from xpx, xpy
+  const R *upx = nullptr, *upy = nullptr;
 };
 
 template <class R>
@@ -210,6 +214,10 @@ void launch_dt_finalize(hipStream_t st, const Params& P, DevScalars* S, const do
 
 int cell_final_blocks(const Params& P);
 void launch_begin_iteration(hipStream_t st, DevScalars* S);
+// Node coordinates x0 + dt_old u of the last Lagrange phase into xx, xy.
+template <class R>
+void launch_lag_coords_2d(hipStream_t st, const Params& P, DevScalars* S, const R* ux, const R* uy, double* xx,
+                          double* xy);
 template <class R>
 void launch_node_lagrange_2d(hipStream_t st, const Params& P, DevScalars* S, const NodeLagArgsT<R>& A,
                              bool vel, bool bc, bool pos);

@@ -746,6 +746,28 @@ void launch_node_lagrange_2d(hipStream_t st, const Params& P, DevScalars* S, con
   else if (vel && !bc && !pos) hipLaunchKernelGGL((k_node_lagrange_2d<R, true, false, false>), g, b, 0, st, P, S, A);
   else if (!vel && bc && !pos) hipLaunchKernelGGL((k_node_lagrange_2d<R, false, true, false>), g, b, 0, st, P, S, A);
   else if (!vel && !bc && pos) hipLaunchKernelGGL((k_node_lagrange_2d<R, false, false, true>), g, b, 0, st, P, S, A);
+  else if (vel && bc && !pos) hipLaunchKernelGGL((k_node_lagrange_2d<R, true, true, false>), g, b, 0, st, P, S, A);
+}
+
+// The last Lagrange phase's node coordinates x0 + dt_old u (lag_coord) into
+// the coordinate arrays, after fused steps that formed them without storing
+// them.  It runs whatever the run's state (an ended run keeps its arrays).
+template <class R>
+__global__ void k_lag_coords_2d(Params P, DevScalars* S, const R* ux, const R* uy, double* xx, double* xy) {
+  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (n >= P.nn) return;
+  const int nnx = P.nx + 1;
+  const int i = (int)(n % nnx), j = (int)(n / nnx);
+  const double dt = S->dt_old;
+  xx[n] = lag_coord<0>(P, i, dt, ux[n]);
+  xy[n] = lag_coord<1>(P, j, dt, uy[n]);
+}
+
+template <class R>
+void launch_lag_coords_2d(hipStream_t st, const Params& P, DevScalars* S, const R* ux, const R* uy, double* xx,
+                          double* xy) {
+  const int B = 256;
+  hipLaunchKernelGGL(k_lag_coords_2d<R>, dim3(nblocks(P.nn, B)), dim3(B), 0, st, P, S, ux, uy, xx, xy);
 }
 
 template <class R>
@@ -1223,6 +1245,8 @@ void launch_cell_final_2d(hipStream_t st, const Params& P, DevScalars* S, const 
                                            const NodeLagArgsT<R>&, bool, bool, bool);                \
   template void launch_cell_lagrange_2d<R>(hipStream_t, const Params&, DevScalars*,                  \
                                            const CellLagArgsT<R>&);                                  \
+  template void launch_lag_coords_2d<R>(hipStream_t, const Params&, DevScalars*, const R*, const R*,  \
+                                        double*, double*);                                           \
   template void launch_cell_sweep_2d<R>(hipStream_t, const Params&, DevScalars*,                     \
                                         const CellSweepArgsT<R>&, int, bool, int);                   \
   template void launch_node_sweep_2d<R>(hipStream_t, const Params&, DevScalars*,                     \

@@ -360,3 +360,36 @@ def test_store_policy_bit_identical(gpu, prec):
                 assert np.array_equal(v, other[f]), f
             else:
                 assert v == other[f], f
+
+
+def test_formed_coordinates_match_stored(gpu):
+    """Fused steps whose kernels all form the node coordinates as x0 + dt u
+    do not store them; a context whose Lagrange cell phase is a kernel of
+    its own (it reads stored coordinates) stores them every step.  Both give the same
+    state bit for bit, coordinates included (written on demand), through
+    graph groups, a download between steps, an entry-point iteration and
+    further fused steps."""
+    a = H.tp_setup(gpu, 70, 134)
+    b = H.tp_setup(gpu, 70, 134)
+    b.select_variant(0, "lagrange_fusion")
+
+    def same(n):
+        sa, sb = a.download(), b.download()
+        for f, v in sa.items():
+            if isinstance(v, np.ndarray):
+                assert np.array_equal(v, sb[f]), (n, f)
+            else:
+                assert v == sb[f], (n, f)
+    for n in (1, 25, 2):
+        a.run(n)
+        b.run(n)
+        same(n)
+    for name in ENTRY_SEQUENCE:
+        a.entry(name)
+        b.entry(name)
+    a.compute_delta_t()
+    b.compute_delta_t()
+    same("entry points")
+    a.run(14)
+    b.run(14)
+    same("after")
