@@ -1753,6 +1753,11 @@ int autotune_t(mhg_ctx* c, int32_t reps, char* report, int32_t report_len) {
   HIPCHK(c, hipEventCreate(&e0));
   HIPCHK(c, hipEventCreate(&e1));
   std::string rep;
+  // the fused step forms node coordinates from velocities only with the
+  // lane (x) and tile (y) dual sweeps; the column walkers and 5-node
+  // stencils read stored coordinates, whose store costs the step more than
+  // a dual sweep can gain, so they are not candidates there
+  const bool formed = c->fuse_lag && fused_lag_supported(P) && c->fuse_final && fused_final_supported(P);
   for (int k = 0; k < 3; ++k) {
     auto launch = [&](int v) {
       if (k == 0) launch_cell_sweep_2d(c->stream, P, c->S, A, 1, true, v);
@@ -1762,6 +1767,8 @@ int autotune_t(mhg_ctx* c, int32_t reps, char* report, int32_t report_len) {
     int best = 0;
     float best_ms = 1e30f;
     for (int v = 0; v < variant_count_of(k); ++v) {
+      if (formed && k >= 1 && v != (k == 1 ? 1 : 4)) continue;
+      if (best_ms == 1e30f) best = v;
       for (int w = 0; w < 2; ++w) launch(v);
       HIPCHK(c, hipEventRecord(e0, c->stream));
       for (int n = 0; n < reps; ++n) launch(v);
