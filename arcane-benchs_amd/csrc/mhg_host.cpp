@@ -241,7 +241,7 @@ void remap_2d(mhg_ctx* c, int dt_mode, bool eos = true, bool lag = false) {
     A.uz = nullptr;
     A.block_min = c->block_min;
     launch_cfl_2d(c->stream, P, c->S, A);
-    launch_dt_finalize(c->stream, P, c->S, c->block_min, true);
+    launch_dt_finalize(c->stream, P, c->S, c->block_min, true, cfl_blocks(P));
   } else if (!fuse) {
     TimedLaunch t(c, "cell_final");
     launch_cell_final_2d(c->stream, P, c->S, cell_final_args<R>(c), dt_mode, eos);
@@ -1505,7 +1505,7 @@ int mhg_compute_delta_t(mhg_ctx* c, double* dt, int32_t* stop) {
   A.uz = c->P.dim == 3 ? bp<double>(c->uz[c->ucur]) : nullptr;
   A.block_min = c->block_min;
   launch_cfl(c->stream, c->P, c->S, A);
-  launch_dt_finalize(c->stream, c->P, c->S, c->block_min, false);
+  launch_dt_finalize(c->stream, c->P, c->S, c->block_min, false, c->P.dim == 2 ? cfl_blocks(c->P) : 0);
   r = ep_finish(c);
   if (r) return r;
   c->ep_remapped = false;

@@ -210,7 +210,11 @@ void launch_cell_sweep_lag_2d(hipStream_t st, const Params& P, DevScalars* S, co
 template <class R>
 void launch_cell_sweep_final_2d(hipStream_t st, const Params& P, DevScalars* S, const CellSweepArgsT<R>& A,
                                 const CellFinalArgsT<R>& F, int dir);
-void launch_dt_finalize(hipStream_t st, const Params& P, DevScalars* S, const double* block_min, bool advance);
+// nb: the minima in block_min (<= 0: cell_final_blocks, k_cell_final_2d's;
+// the fused step's k_cfl_2d leaves cfl_blocks of them)
+void launch_dt_finalize(hipStream_t st, const Params& P, DevScalars* S, const double* block_min, bool advance,
+                        int nb = 0);
+int cfl_blocks(const Params& P);
 
 int cell_final_blocks(const Params& P);
 void launch_begin_iteration(hipStream_t st, DevScalars* S);

@@ -1157,17 +1157,26 @@ __global__ __launch_bounds__(256) void k_pression_moyenne(Params P, DevScalars* 
 }
 
 // CFL candidates of quads: the same arithmetic as k_cell_final_2d's, in the
-// storage type R of the bulk fields.
+// storage type R of the bulk fields.  A block owns a contiguous run of
+// cells, 256 at a time (XCD-contiguous runs), and leaves one minimum: with at
+// most kCflBlocks blocks the dt kernel that follows reduces fewer minima.
+constexpr int kCflBlocks = 2048;
+int cfl_blocks(const Params& P) { return cell_final_blocks(P) < kCflBlocks ? cell_final_blocks(P) : kCflBlocks; }
+
 template <class R>
 __global__ __launch_bounds__(256) void k_cfl_2d(Params P, DevScalars* S, CflArgsT<R> A) {
   if (S->stop || S->err) return;
-  const long c = (long)xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  const int nb = (int)gridDim.x;
+  const int lb = xcd_block(blockIdx.x, nb);
+  const long per = ((P.nc + nb - 1) / nb + 255) / 256 * 256;
+  const long c0 = (long)lb * per, c1 = (c0 + per < P.nc) ? c0 + per : P.nc;
   double dx_sound = 1.7976931348623157e308;
-  if (c < P.nc) {
+  const int nnx = P.nx + 1;
+#pragma unroll 2
+  for (long c = c0 + threadIdx.x; c < c1; c += 256) {
     R vmax = R(0);
     const R Lc = A.L[c], cc = A.cs[c];
     if (!P.lagonly) {
-      const int nnx = P.nx + 1;
       const int i = (int)(c % P.nx), j = (int)(c / P.nx);
       const long n0 = i + (long)nnx * j;
       const long nd[4] = {n0, n0 + 1, n0 + 1 + nnx, n0 + nnx};
@@ -1186,7 +1195,7 @@ __global__ __launch_bounds__(256) void k_cfl_2d(Params P, DevScalars* S, CflArgs
     }
     dx_sound = smin(dx_sound, (double)qdiv(Lc, cc + vmax));
   }
-  block_min_store<256>(dx_sound, A.block_min + xcd_block(blockIdx.x, gridDim.x));
+  block_min_store<256>(dx_sound, A.block_min + lb);
 }
 
 void launch_eos_env(hipStream_t st, const Params& P, DevScalars* S, const EosArgs& A, int k, int mode, long c0,
@@ -1211,16 +1220,17 @@ void launch_cfl(hipStream_t st, const Params& P, DevScalars* S, const CflArgs& A
     launch_cfl_3d(st, P, S, A);
     return;
   }
-  hipLaunchKernelGGL(k_cfl_2d<double>, dim3(cell_final_blocks(P)), dim3(256), 0, st, P, S, A);
+  hipLaunchKernelGGL(k_cfl_2d<double>, dim3(cfl_blocks(P)), dim3(256), 0, st, P, S, A);
 }
 
 template <class R>
 void launch_cfl_2d(hipStream_t st, const Params& P, DevScalars* S, const CflArgsT<R>& A) {
-  hipLaunchKernelGGL(k_cfl_2d<R>, dim3(cell_final_blocks(P)), dim3(256), 0, st, P, S, A);
+  hipLaunchKernelGGL(k_cfl_2d<R>, dim3(cfl_blocks(P)), dim3(256), 0, st, P, S, A);
 }
 
-void launch_dt_finalize(hipStream_t st, const Params& P, DevScalars* S, const double* block_min, bool advance) {
-  const int nb = cell_final_blocks(P);
+void launch_dt_finalize(hipStream_t st, const Params& P, DevScalars* S, const double* block_min, bool advance,
+                        int nb) {
+  if (nb <= 0) nb = cell_final_blocks(P);
   if (advance) hipLaunchKernelGGL((k_dt_finalize<1024, true>), dim3(1), dim3(1024), 0, st, P, S, block_min, nb);
   else hipLaunchKernelGGL((k_dt_finalize<1024, false>), dim3(1), dim3(1024), 0, st, P, S, block_min, nb);
 }
