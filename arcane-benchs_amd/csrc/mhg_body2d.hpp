@@ -529,7 +529,7 @@ struct YTileShape {
 // each row-major cell of the tile (halo rows included) and its remap vector
 // feeds the staging tiles from registers.
 template <class R, int NEM, bool FIRST, int LIM, bool SEG, bool LEARN = false, bool FINAL = false,
-          bool LAG = false>
+          bool LAG = false, int TWC = kTileW>
 __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevScalars* S,
                                                          const CellSweepArgsT<R>& A, int L,
                                                          const CellFinalArgsT<R>* F = nullptr,
@@ -537,20 +537,21 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
   // geometry tiles [0..4]; once read, tiles [0..5] are reused as the staging
   // tiles (in, out, dual flux) of a pair of variables
   // order 3: a seventh tile for the cells' mixed / pure codes
-  __shared__ double lds[LIM == kLimO3 ? 7 : 6][kTileR * kTileP];
+  // TWC columns (one wavefront each) of kTileR rows, rows padded by one word
+  __shared__ double lds[LIM == kLimO3 ? 7 : 6][kTileR * (TWC + 1)];
   R* const stg = reinterpret_cast<R*>(&lds[0][0]);
-  constexpr int TS = kTileR * kTileP;  // words of one staging tile
+  constexpr int TS = kTileR * (TWC + 1);  // words of one staging tile
   constexpr int HL = (LIM == kLimO3) ? 3 : 2, NO = 64 - 2 * HL;  // halo rows, output rows
   if (S->stop || S->err) return;  // uniform over the block
   const int nx = P.nx, N = P.ny, nnx = nx + 1;
   const YTileShape<SEG> sh(N);
-  const int TW = SEG ? sh.TW : kTileW, TP = SEG ? sh.TP : kTileP;
+  const int TW = SEG ? sh.TW : TWC, TP = SEG ? sh.TP : TWC + 1;
   const int ncolblk = (nx + TW - 1) / TW;
   const int i0 = (L % ncolblk) * TW;
   const int s0 = SEG ? -2 : (L / ncolblk) * NO - HL;  // cell row of tile row 0
   const int t = threadIdx.x;
   // row-major position (loads, stores)
-  const int tr = SEG ? t / TW : t >> 4, tc = SEG ? t % TW : t & 15;
+  const int tr = SEG ? t / TW : t / TWC, tc = SEG ? t % TW : t % TWC;
   const bool trow = SEG ? tr < sh.SL : true;
   // transposed position: column col, row r (lane of its segment)
   const int w = t >> 6, lane = t & 63;
@@ -1230,20 +1231,20 @@ __device__ __forceinline__ void node_sweep_x_2d_body(const Params& P, DevScalars
 // dual state travel back through LDS to be added and stored row-major.
 // SEG: whole node columns of SL = ny + 5 lanes (ny + 1 nodes and two halos
 // on each side), K = 64 / SL of them per wavefront, as in the cell sweep.
-template <class R, bool FIRST, int NEM, int LIM, bool SEG>
+template <class R, bool FIRST, int NEM, int LIM, bool SEG, int TWC = kTileW>
 __device__ __forceinline__ void node_sweep_ytile_2d_body(const Params& P, DevScalars* S,
                                                          const NodeSweepArgsT<R>& A, int L) {
-  __shared__ double ldx[kTileR * kTileP];   // coordinate along y
-  __shared__ R lds[4][kTileR * kTileP];     // phi_dual x, y, front flux, kinetic energy; then the increments
+  __shared__ double ldx[kTileR * (TWC + 1)];   // coordinate along y
+  __shared__ R lds[4][kTileR * (TWC + 1)];     // phi_dual x, y, front flux, kinetic energy; then the increments
   if (S->stop || S->err) return;  // uniform over the block
   const int nx = P.nx, N = P.ny, nnx = nx + 1;
   const YTileShape<SEG> sh(N + 1);
-  const int TW = SEG ? sh.TW : kTileW, TP = SEG ? sh.TP : kTileP;
+  const int TW = SEG ? sh.TW : TWC, TP = SEG ? sh.TP : TWC + 1;
   const int ncolblk = (nnx + TW - 1) / TW;
   const int i0 = (L % ncolblk) * TW;
   const int s0 = SEG ? -2 : (L / ncolblk) * 60 - 2;  // node row of tile row 0
   const int t = threadIdx.x;
-  const int tr = SEG ? t / TW : t >> 4, tc = SEG ? t % TW : t & 15;
+  const int tr = SEG ? t / TW : t / TWC, tc = SEG ? t % TW : t % TWC;
   const bool trow = SEG ? tr < sh.SL : true;
   const int w = t >> 6, lane = t & 63;
   const int col = SEG ? w * sh.K + lane / sh.SL : w, r = SEG ? lane % sh.SL : lane;

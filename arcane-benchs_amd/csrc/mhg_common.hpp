@@ -504,6 +504,7 @@ struct Params {
   // next kernel of the step); chosen per context from the working set,
   // settable by the caller
   int nt_store;
+  int ytw8;  // the fused y sweeps on LDS tiles of 8 columns (512-thread blocks) instead of 16
   double cfl, dtmin, dtmax, tend;
   int nbc;
   BcRange bc[kMaxBc];

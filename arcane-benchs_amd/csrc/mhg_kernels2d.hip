@@ -76,20 +76,21 @@ __global__ __launch_bounds__(256) void k_cell_sweep_x_lag_2d(Params P, DevScalar
                                                                      nullptr, &LA);
 }
 
-template <class R, int NEM, int LIM>
-__global__ __launch_bounds__(1024) void k_cell_sweep_y_lag_2d(Params P, DevScalars* S, CellSweepArgsT<R> A,
-                                                                 CellLagArgsT<R> LA) {
-  cell_sweep_ytile_2d_body<R, NEM, true, LIM, false, false, false, true>(P, S, A, xcd_block(blockIdx.x, gridDim.x),
-                                                                        nullptr, &LA);
+template <class R, int NEM, int LIM, int TWC = kTileW>
+__global__ __launch_bounds__(64 * TWC) void k_cell_sweep_y_lag_2d(Params P, DevScalars* S, CellSweepArgsT<R> A,
+                                                                     CellLagArgsT<R> LA) {
+  cell_sweep_ytile_2d_body<R, NEM, true, LIM, false, false, false, true, TWC>(
+      P, S, A, xcd_block(blockIdx.x, gridDim.x), nullptr, &LA);
 }
 
 // At least 8 waves per SIMD (two 1024-thread blocks per CU): the bound keeps
 // the scalar registers of both argument blocks within the 8-wave budget; the
 // unbounded kernel took 102 and ran one block per CU.
-template <class R, int NEM, int LIM>
-__global__ __launch_bounds__(1024, 8) void k_cell_sweep_y_final_2d(Params P, DevScalars* S, CellSweepArgsT<R> A,
-                                                                   CellFinalArgsT<R> F) {
-  cell_sweep_ytile_2d_body<R, NEM, false, LIM, false, false, true>(P, S, A, xcd_block(blockIdx.x, gridDim.x), &F);
+template <class R, int NEM, int LIM, int TWC = kTileW>
+__global__ __launch_bounds__(64 * TWC, 8) void k_cell_sweep_y_final_2d(Params P, DevScalars* S,
+                                                                       CellSweepArgsT<R> A, CellFinalArgsT<R> F) {
+  cell_sweep_ytile_2d_body<R, NEM, false, LIM, false, false, true, false, TWC>(
+      P, S, A, xcd_block(blockIdx.x, gridDim.x), &F);
 }
 
 template <class R, int BLOCK, int NEM, bool EOS = true>
@@ -107,9 +108,9 @@ __global__ __launch_bounds__(256) void k_node_sweep_x_2d(Params P, DevScalars* S
   node_sweep_x_2d_body<R, FIRST, NEM, LIM>(P, S, A, xcd_block(blockIdx.x, gridDim.x));
 }
 
-template <class R, bool FIRST, int NEM, int LIM>
-__global__ __launch_bounds__(1024) void k_node_sweep_ytile_2d(Params P, DevScalars* S, NodeSweepArgsT<R> A) {
-  node_sweep_ytile_2d_body<R, FIRST, NEM, LIM, false>(P, S, A, xcd_block(blockIdx.x, gridDim.x));
+template <class R, bool FIRST, int NEM, int LIM, int TWC = kTileW>
+__global__ __launch_bounds__(64 * TWC) void k_node_sweep_ytile_2d(Params P, DevScalars* S, NodeSweepArgsT<R> A) {
+  node_sweep_ytile_2d_body<R, FIRST, NEM, LIM, false, TWC>(P, S, A, xcd_block(blockIdx.x, gridDim.x));
 }
 
 // ---------------------------------------------------------------------------
@@ -885,8 +886,11 @@ void launch_cell_sweep_final_2d(hipStream_t st, const Params& P, DevScalars* S, 
 #undef XF_
     return;
   }
-  dim3 g((unsigned)(((P.nx + kTileW - 1) / kTileW) * ((P.ny + 59) / 60))), b(1024);
-#define YF_(N, LM) hipLaunchKernelGGL((k_cell_sweep_y_final_2d<R, N, LM>), g, b, 0, st, P, S, A, F)
+  const int tw = P.ytw8 ? 8 : kTileW;
+  dim3 g((unsigned)(((P.nx + tw - 1) / tw) * ((P.ny + 59) / 60))), b(64 * tw);
+#define YF_(N, LM)                                                                                     \
+  if (tw == 8) hipLaunchKernelGGL((k_cell_sweep_y_final_2d<R, N, LM, 8>), g, b, 0, st, P, S, A, F); \
+  else hipLaunchKernelGGL((k_cell_sweep_y_final_2d<R, N, LM>), g, b, 0, st, P, S, A, F)
 #define YFN_(LM)                          \
   do {                                    \
     if (P.ne <= 1) YF_(1, LM);            \
@@ -920,8 +924,11 @@ void launch_cell_sweep_lag_2d(hipStream_t st, const Params& P, DevScalars* S, co
 #undef XG_
     return;
   }
-  dim3 g((unsigned)(((P.nx + kTileW - 1) / kTileW) * ((P.ny + 59) / 60))), b(1024);
-#define YG_(N, LM) hipLaunchKernelGGL((k_cell_sweep_y_lag_2d<R, N, LM>), g, b, 0, st, P, S, A, LA)
+  const int tw = P.ytw8 ? 8 : kTileW;
+  dim3 g((unsigned)(((P.nx + tw - 1) / tw) * ((P.ny + 59) / 60))), b(64 * tw);
+#define YG_(N, LM)                                                                                   \
+  if (tw == 8) hipLaunchKernelGGL((k_cell_sweep_y_lag_2d<R, N, LM, 8>), g, b, 0, st, P, S, A, LA); \
+  else hipLaunchKernelGGL((k_cell_sweep_y_lag_2d<R, N, LM>), g, b, 0, st, P, S, A, LA)
 #define YGN_(LM)                          \
   do {                                    \
     if (P.ne <= 1) YG_(1, LM);            \
@@ -1049,9 +1056,12 @@ void launch_node_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const 
 #undef NX_
     return;
   }
-  dim3 g((unsigned)(((P.nx + 1 + kTileW - 1) / kTileW) * ((P.ny + 1 + 59) / 60))), b(1024);
-#define NT_(N, LM)                                                                                 \
-  if (first) hipLaunchKernelGGL((k_node_sweep_ytile_2d<R, true, N, LM>), g, b, 0, st, P, S, A);    \
+  const int tw = P.ytw8 ? 8 : kTileW;
+  dim3 g((unsigned)(((P.nx + 1 + tw - 1) / tw) * ((P.ny + 1 + 59) / 60))), b(64 * tw);
+#define NT_(N, LM)                                                                                        \
+  if (tw == 8 && first) hipLaunchKernelGGL((k_node_sweep_ytile_2d<R, true, N, LM, 8>), g, b, 0, st, P, S, A); \
+  else if (tw == 8) hipLaunchKernelGGL((k_node_sweep_ytile_2d<R, false, N, LM, 8>), g, b, 0, st, P, S, A);   \
+  else if (first) hipLaunchKernelGGL((k_node_sweep_ytile_2d<R, true, N, LM>), g, b, 0, st, P, S, A);         \
   else hipLaunchKernelGGL((k_node_sweep_ytile_2d<R, false, N, LM>), g, b, 0, st, P, S, A)
 #define NTL_(LM)                    \
   do {                              \

@@ -393,3 +393,21 @@ def test_formed_coordinates_match_stored(gpu):
     a.run(14)
     b.run(14)
     same("after")
+
+
+@pytest.mark.parametrize("prec", [H.abi.PREC_FP64, H.abi.PREC_FP32_BULK])
+def test_ytile_width_bit_identical(gpu, prec):
+    """The fused y sweeps on tiles of 8 columns (512-thread blocks) give the
+    state of the 16-column tiles bit for bit."""
+    states = []
+    for v in (0, 1):
+        s = H.tp_setup(gpu, 70, 134, precision=prec)
+        s.select_variant(v, "ytile_width")
+        s.run(13)
+        states.append(s.download())
+    for other in states[1:]:
+        for f, v in states[0].items():
+            if isinstance(v, np.ndarray):
+                assert np.array_equal(v, other[f]), f
+            else:
+                assert v == other[f], f
