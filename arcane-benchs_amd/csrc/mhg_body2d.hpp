@@ -43,7 +43,7 @@ __device__ inline double lag_coord(const Params& P, int idx, double dt, R u) {
 template <class R, bool VEL, bool BC, bool POS>
 __device__ __forceinline__ void node_lagrange_2d_body(const Params& P, DevScalars* S, const NodeLagArgsT<R>& A,
                                                       long blk) {
-  if (S->stop || S->err) return;
+  const bool halt = halted(S);
   const long nid = blk * blockDim.x + threadIdx.x;
   if (nid >= P.nn) return;
   const int nnx = P.nx + 1;
@@ -123,6 +123,7 @@ __device__ __forceinline__ void node_lagrange_2d_body(const Params& P, DevScalar
       }
     }
   }
+  if (halt) return;
   if (VEL || BC) {
     st_stream<3>(P, &A.ux[nid], (R)vx);
     st_stream<3>(P, &A.uy[nid], (R)vy);
@@ -217,7 +218,7 @@ template <class R, int D, int NEM, bool FIRST, int LIM, bool LEARN = false, bool
 __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScalars* S, const CellSweepArgsT<R>& A,
                                                         int blk, const CellFinalArgsT<R>* F = nullptr,
                                                         const CellLagArgsT<R>* LA = nullptr) {
-  if (S->stop || S->err) return;
+  const bool halt = halted(S);
   // halo lanes on each side: 3 for the order-3 stencil (six cells per face)
   constexpr int HL = (LIM == kLimO3) ? 3 : 2, NO = 64 - 2 * HL;
   const int lane = threadIdx.x & 63;
@@ -308,6 +309,7 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
     }
     preq = ex ? uq : R(0);
     vlag = (R)Vd;
+    if (halt) return;
     if (out) {
       st_stream<0>(P, &LA->Vlag[c], (R)Vd);
       st_stream<0>(P, &LA->L[c], (R)Ld);
@@ -322,6 +324,7 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
       }
     const R preq_l = Uin[(3 * ne) * NC + c];
     preq = ex ? preq_l : R(0);
+    if (halt) return;
   }
   // cell centre (component D, double) and phi denominator
   double xc = 0.;
@@ -542,7 +545,7 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
   R* const stg = reinterpret_cast<R*>(&lds[0][0]);
   constexpr int TS = kTileR * (TWC + 1);  // words of one staging tile
   constexpr int HL = (LIM == kLimO3) ? 3 : 2, NO = 64 - 2 * HL;  // halo rows, output rows
-  if (S->stop || S->err) return;  // uniform over the block
+  const bool halt = halted(S);
   const int nx = P.nx, N = P.ny, nnx = nx + 1;
   const YTileShape<SEG> sh(N);
   const int TW = SEG ? sh.TW : TWC, TP = SEG ? sh.TP : TWC + 1;
@@ -630,6 +633,7 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
     vlag = (R)Vd;
 #pragma unroll
     for (int q = 0; q < VPR; ++q) nx0[q] = lU[q];
+    if (halt) return;
     if (gout) {
       st_stream<0>(P, &LA->Vlag[gc], (R)Vd);
       st_stream<0>(P, &LA->L[gc], (R)Ld);
@@ -639,6 +643,7 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
     for (int k = 0; k < NEM; ++k) uvol[k] = (k < ne) ? Uin[k * NC + gc] : R(0);
 #pragma unroll
     for (int q = 0; q < VPR; ++q) nx0[q] = q < nvu ? Uin[q * NC + gc] : R(0);
+    if (halt) return;
   }
   {  // geometry of the tile, row-major (stored in the double tiles exactly)
     double xc = 0., xf = 0., lf = 0.;
@@ -941,7 +946,8 @@ __device__ __forceinline__ void cell_final_2d_body(const Params& P, DevScalars* 
 // k_begin_iteration, which no kernel of the step between them reads).
 template <int BLOCK, bool ADVANCE>
 __device__ __forceinline__ void dt_finalize_body(const Params& P, DevScalars* S, const double* block_min, int nb) {
-  if (S->stop || S->err) return;
+  const bool halt = halted(S);
+  const double old = S->dt, time0 = S->time;
   __shared__ double red[BLOCK / 64];
   // eight loads in flight per thread (the minimum does not depend on the order)
   double w[8];
@@ -955,6 +961,7 @@ __device__ __forceinline__ void dt_finalize_body(const Params& P, DevScalars* S,
       if (idx < nb) w[u] = smin(w[u], x);
     }
   }
+  if (halt) return;
   double v = w[0];
 #pragma unroll
   for (int u = 1; u < 8; ++u) v = smin(v, w[u]);
@@ -964,9 +971,9 @@ __device__ __forceinline__ void dt_finalize_body(const Params& P, DevScalars* S,
   if (threadIdx.x != 0) return;
   double minimum_aux = red[0];
   for (int w = 1; w < BLOCK / 64; ++w) minimum_aux = smin(minimum_aux, red[w]);
-  const double old = S->dt;
+  const double t = ADVANCE ? time0 + old : time0;
   if (ADVANCE) {
-    S->time += old;
+    S->time = t;
     S->iteration += 1;
   }
   S->dt_old = old;
@@ -977,7 +984,6 @@ __device__ __forceinline__ void dt_finalize_body(const Params& P, DevScalars* S,
     S->err = 1;
     return;
   }
-  const double t = S->time;
   if (P.land) {  // extension: the last step ends at tend
     if (S->landing || t >= P.tend) S->stop = 1;
     else if (t + new_dt >= P.tend) {
@@ -1063,7 +1069,7 @@ __device__ __forceinline__ R dual_ecin_term(R U0, R U1, R U3, R U4) {
 template <class R, bool FIRST, int NEM, int LIM>
 __device__ __forceinline__ void node_sweep_x_2d_body(const Params& P, DevScalars* S, const NodeSweepArgsT<R>& A,
                                                      int blk) {
-  if (S->stop || S->err) return;
+  const bool halt = halted(S);
   const int lane = threadIdx.x & 63;
   const int N = P.nx;
   const int nseg = (N + 1 + 59) / 60;
@@ -1138,6 +1144,7 @@ __device__ __forceinline__ void node_sweep_x_2d_body(const Params& P, DevScalars
       }
     }
   }
+  if (halt) return;
   if (ex) xd = xd_l;
   // front dual mass flux: cells (i, j-1), (i, j), environments in order
   R front = R(0);
@@ -1236,7 +1243,7 @@ __device__ __forceinline__ void node_sweep_ytile_2d_body(const Params& P, DevSca
                                                          const NodeSweepArgsT<R>& A, int L) {
   __shared__ double ldx[kTileR * (TWC + 1)];   // coordinate along y
   __shared__ R lds[4][kTileR * (TWC + 1)];     // phi_dual x, y, front flux, kinetic energy; then the increments
-  if (S->stop || S->err) return;  // uniform over the block
+  const bool halt = halted(S);
   const int nx = P.nx, N = P.ny, nnx = nx + 1;
   const YTileShape<SEG> sh(N + 1);
   const int TW = SEG ? sh.TW : TWC, TP = SEG ? sh.TP : TWC + 1;
@@ -1307,6 +1314,7 @@ __device__ __forceinline__ void node_sweep_ytile_2d_body(const Params& P, DevSca
         }
       }
     }
+    if (halt) return;
     if (gex) xd = xd_l;
     // front flux: cells (i-1, s), (i, s), environments in order
 #pragma unroll

@@ -1175,7 +1175,7 @@ int cfl_blocks(const Params& P) { return cell_final_blocks(P) < kCflBlocks ? cel
 
 template <class R>
 __global__ __launch_bounds__(256) void k_cfl_2d(Params P, DevScalars* S, CflArgsT<R> A) {
-  if (S->stop || S->err) return;
+  const bool halt = halted(S);
   const int nb = (int)gridDim.x;
   const int lb = xcd_block(blockIdx.x, nb);
   const long per = ((P.nc + nb - 1) / nb + 255) / 256 * 256;
@@ -1205,6 +1205,7 @@ __global__ __launch_bounds__(256) void k_cfl_2d(Params P, DevScalars* S, CflArgs
     }
     dx_sound = smin(dx_sound, (double)qdiv(Lc, cc + vmax));
   }
+  if (halt) return;
   block_min_store<256>(dx_sound, A.block_min + lb);
 }
 
