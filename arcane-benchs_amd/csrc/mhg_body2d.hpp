@@ -47,7 +47,8 @@ __device__ __forceinline__ void node_lagrange_2d_body(const Params& P, DevScalar
   const long nid = blk * blockDim.x + threadIdx.x;
   if (nid >= P.nn) return;
   const int nnx = P.nx + 1;
-  const int i = (int)(nid % nnx), j = (int)(nid / nnx);
+  int i, j;
+  divmod32(nid, nnx, j, i);
   const double dt = S->dt;
   double vx = A.ux[nid], vy = A.uy[nid];
   if (VEL) {
@@ -58,23 +59,43 @@ __device__ __forceinline__ void node_lagrange_2d_body(const Params& P, DevScalar
     // 3x3 patch of previous coordinates around the node.
     double X[3][3], Y[3][3];
     bool nv[3][3];
-    const bool prev_u = A.upx != nullptr;  // uniform over the launch
+    long pq[3][3];
 #pragma unroll
     for (int dj = -1; dj <= 1; ++dj)
 #pragma unroll
       for (int di = -1; di <= 1; ++di) {
         const int ii = i + di, jj = j + dj;
         const bool ok = ii >= 0 && ii < nnx && jj >= 0 && jj <= P.ny;
-        const long q = ok ? ii + (long)nnx * jj : 0;
+        pq[dj + 1][di + 1] = ok ? ii + (long)nnx * jj : 0;
         nv[dj + 1][di + 1] = ok;
-        if (prev_u) {  // x0 + dt_old u of the previous Lagrange phase (lag_coord)
-          X[dj + 1][di + 1] = lag_coord<0>(P, ii, S->dt_old, A.upx[q]);
-          Y[dj + 1][di + 1] = lag_coord<1>(P, jj, S->dt_old, A.upy[q]);
-        } else {
-          X[dj + 1][di + 1] = A.xpx[q];
-          Y[dj + 1][di + 1] = A.xpy[q];
-        }
       }
+    // one branch, uniform over the launch, around straight-line loads
+    if (A.upx != nullptr) {  // x0 + dt_old u of the previous Lagrange phase (lag_coord)
+      const double dto = S->dt_old;
+      R ux9[3][3], uy9[3][3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          ux9[a][b] = A.upx[pq[a][b]];
+          uy9[a][b] = A.upy[pq[a][b]];
+        }
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          X[a][b] = lag_coord<0>(P, i + b - 1, dto, ux9[a][b]);
+          Y[a][b] = lag_coord<1>(P, j + a - 1, dto, uy9[a][b]);
+        }
+    } else {
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          X[a][b] = A.xpx[pq[a][b]];
+          Y[a][b] = A.xpy[pq[a][b]];
+        }
+    }
     // cells in ascending id: (i-1,j-1) corner 2, (i,j-1) 3, (i-1,j) 1, (i,j) 0
     const int cdi[4] = {-1, 0, -1, 0}, cdj[4] = {-1, -1, 0, 0}, cq[4] = {2, 3, 1, 0};
     R cp[4], cqv[4], cm[4];
@@ -162,7 +183,8 @@ __device__ __forceinline__ void cell_lagrange_2d_body(const Params& P, DevScalar
   const long c = blk * blockDim.x + threadIdx.x;
   if (c >= P.nc) return;
   const int nnx = P.nx + 1;
-  const int i = (int)(c % P.nx), j = (int)(c / P.nx);
+  int i, j;
+  divmod32(c, P.nx, j, i);
   const long n0 = i + (long)nnx * j;
   const long nd[4] = {n0, n0 + 1, n0 + 1 + nnx, n0 + nnx};
   const CellPre<R> in = cell_pre_load<R, PC>(A, c);
@@ -228,8 +250,7 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
   int seg, line;                             // segment along D, index across D
   if (D == 0) {
     const long wave = ((long)blk * blockDim.x + threadIdx.x) >> 6;
-    line = (int)(wave / nseg);
-    seg = (int)(wave % nseg);
+    divmod32(wave, nseg, line, seg);
   } else {
     const int ncolblk = (T + 15) / 16;
     line = (blk % ncolblk) * 16 + (threadIdx.x >> 6);
@@ -878,7 +899,8 @@ __device__ __forceinline__ void cell_final_2d_body(const Params& P, DevScalars* 
   if (!EOS) {  // remapVariables only: no EOS, no CFL candidate
     const long c = blk * BLOCK + threadIdx.x;
     if (c >= P.nc) return;
-    const int i = (int)(c % P.nx), j = (int)(c / P.nx);
+    int i, j;
+    divmod32(c, P.nx, j, i);
     R delta_ec = R(0);
     if (P.etot) {
       const int nnx = P.nx + 1;
@@ -898,7 +920,8 @@ __device__ __forceinline__ void cell_final_2d_body(const Params& P, DevScalars* 
   double dx_sound = 1.7976931348623157e308;
   if (c < P.nc) {
     const int nnx = P.nx + 1;
-    const int i = (int)(c % P.nx), j = (int)(c / P.nx);
+    int i, j;
+    divmod32(c, P.nx, j, i);
     // node velocities of the CFL candidate, loaded before the cell phase
     // writes its fields so that they travel with its loads
     const long n0 = i + (long)nnx * j;
@@ -1074,9 +1097,9 @@ __device__ __forceinline__ void node_sweep_x_2d_body(const Params& P, DevScalars
   const int N = P.nx;
   const int nseg = (N + 1 + 59) / 60;
   const long wave = ((long)blk * blockDim.x + threadIdx.x) >> 6;
-  const int j = (int)(wave / nseg);
+  int j, seg;
+  divmod32(wave, nseg, j, seg);
   if (j > P.ny) return;  // uniform across the wave
-  const int seg = (int)(wave % nseg);
   const int i = seg * 60 - 2 + lane;
   const bool ex = (i >= 0 && i <= N);
   const bool out = ex && lane >= 2 && lane <= 61;

@@ -486,6 +486,15 @@ struct DevScalars {
 // launch, so the early return is uniform over every block.
 __device__ __forceinline__ bool halted(const DevScalars* S) { return (S->stop | S->err) != 0; }
 
+// (n / d, n % d) in 32-bit unsigned arithmetic: mesh indices are below 2^31
+// (mhg_create refuses larger meshes), and a 64-bit division expands to a
+// branch and twice the instructions.
+__device__ __forceinline__ void divmod32(long n, int d, int& q, int& r) {
+  const unsigned un = (unsigned)n, ud = (unsigned)d, uq = un / ud;
+  q = (int)uq;
+  r = (int)(un - uq * ud);
+}
+
 // Run-time parameters of a context.
 struct Params {
   int dim, ne, nvu;  // nvu = 3*ne+1 live cell variables

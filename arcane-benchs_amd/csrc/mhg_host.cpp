@@ -666,6 +666,11 @@ int build_params(const mhg_mesh_desc* mesh, const mhg_options* opt, Params* Pout
   if (mesh->dim != 2 && mesh->dim != 3) return MHG_E_ARG;
   for (int d = 0; d < mesh->dim; ++d)
     if (mesh->n[d] < 1 || !(mesh->length[d] > 0.)) return MHG_E_ARG;
+  // cell and node indices fit in 31 bits: the kernels split them into mesh
+  // coordinates in 32-bit arithmetic
+  double nodes = 1.;
+  for (int d = 0; d < mesh->dim; ++d) nodes *= mesh->n[d] + 1.;
+  if (nodes >= 2147483648.) return MHG_E_UNSUPPORTED;
   if (opt->nb_env < 1 || opt->nb_env > MHG_MAX_ENV || opt->nb_bc < 0 || opt->nb_bc > MHG_MAX_BC)
     return MHG_E_ARG;
   if (opt->schema_csts || opt->with_newton || opt->sans_lagrange ||

@@ -153,7 +153,8 @@ __global__ __launch_bounds__(256) void k_cell_stage_2d(Params P, DevScalars* S, 
     }
   } else if (STAGE == STAGE_GEOMETRY) {
     const int nnx = P.nx + 1;
-    const int i = (int)(c % P.nx), j = (int)(c / P.nx);
+    int i, j;
+    divmod32(c, P.nx, j, i);
     const long n0 = i + (long)nnx * j;
     const long nd[4] = {n0, n0 + 1, n0 + 1 + nnx, n0 + nnx};
     double x[4], y[4];
@@ -426,7 +427,8 @@ __global__ void k_node_sweep_2d(Params P, DevScalars* S, NodeSweepArgs A) {
   const long nid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (nid >= P.nn) return;
   const int nnx = P.nx + 1;
-  const int i = (int)(nid % nnx), j = (int)(nid / nnx);
+  int i, j;
+  divmod32(nid, nnx, j, i);
   const int s = (D == 0) ? i : j;
   const int N = (D == 0) ? P.nx : P.ny;           // nodes 0..N along D
   const int T = (D == 0) ? P.ny : P.nx;           // nodes 0..T along the other direction
@@ -758,7 +760,8 @@ __global__ void k_lag_coords_2d(Params P, DevScalars* S, const R* ux, const R* u
   const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (n >= P.nn) return;
   const int nnx = P.nx + 1;
-  const int i = (int)(n % nnx), j = (int)(n / nnx);
+  int i, j;
+  divmod32(n, nnx, j, i);
   const double dt = S->dt_old;
   xx[n] = lag_coord<0>(P, i, dt, ux[n]);
   xy[n] = lag_coord<1>(P, j, dt, uy[n]);
@@ -1182,12 +1185,17 @@ __global__ __launch_bounds__(256) void k_cfl_2d(Params P, DevScalars* S, CflArgs
   const long c0 = (long)lb * per, c1 = (c0 + per < P.nc) ? c0 + per : P.nc;
   double dx_sound = 1.7976931348623157e308;
   const int nnx = P.nx + 1;
+  // the thread's first cell (i, j); each later one is 256 cells further on,
+  // so (i, j) advance without a division and the node loads do not wait
+  // for one
+  long c = c0 + threadIdx.x;
+  int i, j;
+  divmod32(c, P.nx, j, i);
 #pragma unroll 2
-  for (long c = c0 + threadIdx.x; c < c1; c += 256) {
+  for (; c < c1; c += 256) {
     R vmax = R(0);
     const R Lc = A.L[c], cc = A.cs[c];
     if (!P.lagonly) {
-      const int i = (int)(c % P.nx), j = (int)(c / P.nx);
       const long n0 = i + (long)nnx * j;
       const long nd[4] = {n0, n0 + 1, n0 + 1 + nnx, n0 + nnx};
       R vx[4], vy[4];
@@ -1204,6 +1212,11 @@ __global__ __launch_bounds__(256) void k_cfl_2d(Params P, DevScalars* S, CflArgs
       }
     }
     dx_sound = smin(dx_sound, (double)qdiv(Lc, cc + vmax));
+    i += 256;
+    while (i >= P.nx) {
+      i -= P.nx;
+      ++j;
+    }
   }
   if (halt) return;
   block_min_store<256>(dx_sound, A.block_min + lb);
