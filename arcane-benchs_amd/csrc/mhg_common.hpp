@@ -430,13 +430,16 @@ struct BcRange {
 // shifts of gfx9-family VALU instructions (wave_shr:1 = 0x138, wave_shl:1 =
 // 0x130) instead of an LDS permute.  lane_up1: lane i receives lane i - 1;
 // lane_down1: lane i receives lane i + 1; a lane whose source lies outside
-// the wavefront keeps its own value (the convention of __shfl_up/__shfl_down).
+// the wavefront (lane 0 up, lane 63 down) receives 0 (bound_ctrl).  Those
+// lanes are halo lanes of every sweep, whose results no output lane reads,
+// and a shift that keeps no old value is one instruction instead of a copy
+// and a shift (or none, where the compiler folds it into its consumer).
 // 64-bit values move as two 32-bit halves.
 constexpr int kDppWaveShr1 = 0x138, kDppWaveShl1 = 0x130;
 
 template <int CTRL>
 __device__ inline int dpp_shift(int v) {
-  return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xF, 0xF, false);
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, true);
 }
 template <int CTRL>
 __device__ inline float dpp_shift(float v) {
