@@ -949,13 +949,16 @@ __device__ __forceinline__ void cell_final_2d_body(const Params& P, DevScalars* 
     double y0[4] = {node_y0(P, j), node_y0(P, j), node_y0(P, j + 1), node_y0(P, j + 1)};
     const R c_g = final_cell<R, NEM>(P, A, c, quad_volume(x0, y0), delta_ec);
     // CFL candidate: L / (c + max over the cell nodes of |u|)
-    R vmax = R(0);
+    // (the root of the largest squared norm: sqrt is correctly rounded, hence
+    // monotone, so it equals the largest root)
+    R n2max = R(0);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const R ux = vx[q], uy = vy[q];
-      const R un = sqrt(ux * ux + uy * uy + R(0) * R(0));
-      vmax = (un < vmax) ? vmax : un;
+      const R n2 = ux * ux + uy * uy + R(0) * R(0);
+      n2max = (n2 < n2max) ? n2max : n2;
     }
+    const R vmax = sqrt(n2max);
     // std::min(minimum_aux, dx_sound) fold: a NaN candidate is skipped
     dx_sound = smin(dx_sound, (double)qdiv(Lc, c_g + vmax));
   }

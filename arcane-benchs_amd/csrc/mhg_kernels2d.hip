@@ -1204,12 +1204,16 @@ __global__ __launch_bounds__(256) void k_cfl_2d(Params P, DevScalars* S, CflArgs
         vx[q] = A.ux[nd[q]];
         vy[q] = A.uy[nd[q]];
       }
+      // the root of the largest squared norm: sqrt is correctly rounded, hence
+      // monotone, so it equals the largest root (one root per cell, not per node)
+      R n2max = R(0);
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const R ux = vx[q], uy = vy[q];
-        const R un = sqrt(ux * ux + uy * uy + R(0) * R(0));
-        vmax = (un < vmax) ? vmax : un;
+        const R n2 = ux * ux + uy * uy + R(0) * R(0);
+        n2max = (n2 < n2max) ? n2max : n2;
       }
+      vmax = sqrt(n2max);
     }
     dx_sound = smin(dx_sound, (double)qdiv(Lc, cc + vmax));
     i += 256;

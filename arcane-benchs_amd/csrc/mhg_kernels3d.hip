@@ -1129,7 +1129,9 @@ __global__ __launch_bounds__(BLOCK) void k_cell_final_3d(Params P, DevScalars* S
     }
     const R c_g = final_cell<R, NEM, EOS>(P, A.b, c, A.Veuler[c], delta_ec);
     if (!EOS) return;
-    R vmax = R(0);
+    // the root of the largest squared norm: sqrt is correctly rounded, hence
+    // monotone, so it equals the largest root (one root per cell, not per node)
+    R n2max = R(0);
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const int di = (q == 1 || q == 2 || q == 5 || q == 6) ? 1 : 0;
@@ -1137,9 +1139,10 @@ __global__ __launch_bounds__(BLOCK) void k_cell_final_3d(Params P, DevScalars* S
       const int dk = q >= 4 ? 1 : 0;
       const long n = nid3(P, i + di, j + dj, k + dk);
       const R ux = A.b.ux[n], uy = A.b.uy[n], uz = A.uz[n];
-      const R un = sqrt(ux * ux + uy * uy + uz * uz);
-      vmax = (un < vmax) ? vmax : un;
+      const R n2 = ux * ux + uy * uy + uz * uz;
+      n2max = (n2 < n2max) ? n2max : n2;
     }
+    const R vmax = sqrt(n2max);
     dx_sound = smin(dx_sound, (double)qdiv(A.b.L[c], c_g + vmax));
   }
   block_min_store<BLOCK>(dx_sound, A.b.block_min + blockIdx.x);
@@ -1172,6 +1175,8 @@ __global__ __launch_bounds__(256) void k_cfl_3d(Params P, DevScalars* S, CflArgs
     double vmax = 0.;
     if (!P.lagonly) {
       const int i = (int)(c % P.nx), j = (int)((c / P.nx) % P.ny), k = (int)(c / ((long)P.nx * P.ny));
+      // the root of the largest squared norm (sqrt is monotone)
+      double n2max = 0.;
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         const int di = (q == 1 || q == 2 || q == 5 || q == 6) ? 1 : 0;
@@ -1179,9 +1184,10 @@ __global__ __launch_bounds__(256) void k_cfl_3d(Params P, DevScalars* S, CflArgs
         const int dk = q >= 4 ? 1 : 0;
         const long n = nid3(P, i + di, j + dj, k + dk);
         const double ux = A.ux[n], uy = A.uy[n], uz = A.uz[n];
-        const double un = sqrt(ux * ux + uy * uy + uz * uz);
-        vmax = (un < vmax) ? vmax : un;
+        const double n2 = ux * ux + uy * uy + uz * uz;
+        n2max = (n2 < n2max) ? n2max : n2;
       }
+      vmax = sqrt(n2max);
     }
     dx_sound = smin(dx_sound, A.L[c] / (A.cs[c] + vmax));
   }
