@@ -50,7 +50,9 @@ __device__ __forceinline__ void node_lagrange_2d_body(const Params& P, DevScalar
   int i, j;
   divmod32(nid, nnx, j, i);
   const double dt = S->dt;
-  double vx = A.ux[nid], vy = A.uy[nid];
+  // the velocity update in the bulk type (FP64: the same operations); the
+  // corner normals stay in double
+  R vx = A.ux[nid], vy = A.uy[nid];
   if (VEL) {
     const double dtv = 0.5 * (S->dt_old + S->dt);
     // Every load first, from clamped indices and without branches, so that
@@ -114,7 +116,7 @@ __device__ __forceinline__ void node_lagrange_2d_body(const Params& P, DevScalar
 #pragma unroll
       for (int di = 0; di < 3; ++di)
         if (!nv[dj][di]) X[dj][di] = Y[dj][di] = 0.;
-    double Fx = 0., Fy = 0., msum = 0.;
+    R Fx = R(0), Fy = R(0), msum = R(0);
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       if (!cv[t]) continue;
@@ -124,13 +126,13 @@ __device__ __forceinline__ void node_lagrange_2d_body(const Params& P, DevScalar
       double qy[4] = {Y[bj][bi], Y[bj][bi + 1], Y[bj + 1][bi + 1], Y[bj + 1][bi]};
       double cx, cy;
       cqs2d(qx, qy, cq[t], &cx, &cy);
-      double pq = (double)cp[t] + (double)cqv[t];
-      Fx += pq * cx;
-      Fy += pq * cy;
-      msum += (double)cm[t];
+      const R pq = cp[t] + cqv[t];
+      Fx += pq * (R)cx;
+      Fy += pq * (R)cy;
+      msum += cm[t];
     }
-    double mn = 0.25 * msum;
-    double s = dtv / mn;
+    const R mn = R(0.25) * msum;
+    const R s = qdiv((R)dtv, mn);
     vx = vx + s * Fx;
     vy = vy + s * Fy;
   }
