@@ -31,6 +31,71 @@ __device__ inline double lag_coord(const Params& P, int idx, double dt, R u) {
   return (P.org[A] + idx * P.h[A]) + dt * (double)u;
 }
 
+// computeDeltaT (MahycoModule.cc:2076-2191) on the minimum of the CFL
+// candidates: dt = min(cfl * min_c dx_sound, 1.05 dt_old, deltat-max),
+// deltat-min check, stop test with the time of this iteration.  ADVANCE
+// first applies this iteration's time += dt, iteration += 1 (the work of
+// k_begin_iteration, which no kernel of the step between them reads).  The
+// scalars after the update; a run that has ended leaves them as they are.
+template <bool ADVANCE>
+__device__ __forceinline__ DevScalars dt_update(const Params& P, DevScalars s, double minimum_aux) {
+  if (s.stop || s.err) return s;
+  const double old = s.dt;
+  const double t = ADVANCE ? s.time + old : s.time;
+  if (ADVANCE) {
+    s.time = t;
+    s.iteration += 1;
+  }
+  s.dt_old = old;
+  double new_dt = P.cfl * minimum_aux;
+  new_dt = smin(new_dt, 1.05 * old);
+  new_dt = smin(new_dt, P.dtmax);
+  if (new_dt < P.dtmin) {
+    s.err = 1;
+    return s;
+  }
+  if (P.land) {  // extension: the last step ends at tend
+    if (s.landing || t >= P.tend) s.stop = 1;
+    else if (t + new_dt >= P.tend) {
+      new_dt = P.tend - t;
+      s.landing = 1;
+    }
+    s.dt = new_dt;
+    return s;
+  }
+  s.dt = new_dt;
+  const bool not_yet_finish = (t < P.tend);
+  const bool finish = (t > P.tend);
+  const bool too_much = ((t + new_dt) > P.tend);
+  if ((not_yet_finish && too_much) || finish) s.stop = 1;
+  return s;
+}
+
+// Arguments of the node phase with the previous step's dt kernel folded in:
+// the previous CFL kernel's nb block minima and the scalars to write.
+struct DtFold {
+  const double* block_min;
+  int nb;
+  DevScalars* Sout;
+};
+
+// The folded dt kernel: the block's minimum of the CFL minima (w: this
+// thread's share), computeDeltaT's update of s (the same in every block),
+// written to F->Sout by block 0.
+template <int BLOCK>
+__device__ __forceinline__ DevScalars fold_dt(const Params& P, const DevScalars& s, double w, const DtFold* F) {
+  __shared__ double red[BLOCK / 64];
+  for (int off = 32; off > 0; off >>= 1) w = smin(w, __shfl_down(w, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = w;
+  __syncthreads();
+  double m = red[0];
+#pragma unroll
+  for (int q = 1; q < BLOCK / 64; ++q) m = smin(m, red[q]);
+  const DevScalars n = dt_update<true>(P, s, m);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *F->Sout = n;
+  return n;
+}
+
 // ---------------------------------------------------------------------------
 // K1: node phase.  updateForceAndVelocity (MahycoModule.cc:589-665) with the
 // corner normals of the previous Lagrangian coordinates (saveValuesAtN copies
@@ -40,21 +105,37 @@ __device__ inline double lag_coord(const Params& P, int idx, double dt, R u) {
 // Node mass = 1/4 sum of the cell masses (computeNodeMass 235-266): the cell
 // masses do not change in the Lagrange phase, so it is recomputed here.
 // ---------------------------------------------------------------------------
-template <class R, bool VEL, bool BC, bool POS>
+// FOLD (with VEL): the previous step's dt kernel folded in.  S holds the
+// scalars before computeDeltaT's update; the block loads the previous CFL
+// kernel's minima with its other loads, forms the coordinates with dt_old =
+// S->dt (the value the update gives it), and reduces the minima before the
+// velocity update, which needs the new dt (fold_dt).  Every thread of the
+// block reaches that reduction.
+template <class R, bool VEL, bool BC, bool POS, bool FOLD = false>
 __device__ __forceinline__ void node_lagrange_2d_body(const Params& P, DevScalars* S, const NodeLagArgsT<R>& A,
-                                                      long blk) {
-  const bool halt = halted(S);
-  const long nid = blk * blockDim.x + threadIdx.x;
-  if (nid >= P.nn) return;
+                                                      long blk, const DtFold* F = nullptr) {
+  bool halt = halted(S);
+  const long nid0 = blk * blockDim.x + threadIdx.x;
+  const bool live = nid0 < P.nn;
+  if (!FOLD && !live) return;
+  const long nid = live ? nid0 : 0;
   const int nnx = P.nx + 1;
   int i, j;
   divmod32(nid, nnx, j, i);
-  const double dt = S->dt;
+  double dt = S->dt;
+  double wmin = 1.7976931348623157e308;  // FOLD: this thread's share of the minima
+  if (FOLD) {
+    const int b0 = threadIdx.x, b1 = threadIdx.x + 256;
+    const double w0 = F->block_min[b0 < F->nb ? b0 : 0], w1 = F->block_min[b1 < F->nb ? b1 : 0];
+    if (b0 < F->nb) wmin = smin(wmin, w0);
+    if (b1 < F->nb) wmin = smin(wmin, w1);
+    for (int b = threadIdx.x + 512; b < F->nb; b += 256) wmin = smin(wmin, F->block_min[b]);
+  }
   // the velocity update in the bulk type (FP64: the same operations); the
   // corner normals stay in double
   R vx = A.ux[nid], vy = A.uy[nid];
   if (VEL) {
-    const double dtv = 0.5 * (S->dt_old + S->dt);
+    double dtv = 0.5 * (S->dt_old + S->dt);
     // Every load first, from clamped indices and without branches, so that
     // a wavefront waits for memory once; out-of-mesh patch nodes read node
     // 0 and are then zeroed, out-of-mesh cells read cell 0 and are skipped.
@@ -73,7 +154,7 @@ __device__ __forceinline__ void node_lagrange_2d_body(const Params& P, DevScalar
       }
     // one branch, uniform over the launch, around straight-line loads
     if (A.upx != nullptr) {  // x0 + dt_old u of the previous Lagrange phase (lag_coord)
-      const double dto = S->dt_old;
+      const double dto = FOLD ? S->dt : S->dt_old;
       R ux9[3][3], uy9[3][3];
 #pragma unroll
       for (int a = 0; a < 3; ++a)
@@ -131,6 +212,12 @@ __device__ __forceinline__ void node_lagrange_2d_body(const Params& P, DevScalar
       Fy += pq * (R)cy;
       msum += cm[t];
     }
+    if (FOLD) {
+      const DevScalars sn = fold_dt<256>(P, *S, wmin, F);
+      halt = sn.stop || sn.err;
+      dt = sn.dt;
+      dtv = 0.5 * (sn.dt_old + sn.dt);
+    }
     const R mn = R(0.25) * msum;
     const R s = qdiv((R)dtv, mn);
     vx = vx + s * Fx;
@@ -146,7 +233,7 @@ __device__ __forceinline__ void node_lagrange_2d_body(const Params& P, DevScalar
       }
     }
   }
-  if (halt) return;
+  if (halt || !live) return;
   if (VEL || BC) {
     st_stream<3>(P, &A.ux[nid], (R)vx);
     st_stream<3>(P, &A.uy[nid], (R)vy);
@@ -967,17 +1054,12 @@ __device__ __forceinline__ void cell_final_2d_body(const Params& P, DevScalars* 
   block_min_store<BLOCK>(dx_sound, block_min + blk);
 }
 
-// computeDeltaT (MahycoModule.cc:2076-2191) on the block minima of
-// k_cell_final_2d: dt = min(cfl * min_c dx_sound, 1.05 dt_old, deltat-max),
-// deltat-min check, stop test with the time of this iteration.  ADVANCE
-// first applies this iteration's time += dt, iteration += 1 (the work of
-// k_begin_iteration, which no kernel of the step between them reads).
-template <int BLOCK, bool ADVANCE>
-__device__ __forceinline__ void dt_finalize_body(const Params& P, DevScalars* S, const double* block_min, int nb) {
-  const bool halt = halted(S);
-  const double old = S->dt, time0 = S->time;
+// Minimum of nb block minima over a block of BLOCK threads, every thread
+// receiving it (the minimum does not depend on the order).
+template <int BLOCK>
+__device__ __forceinline__ double block_minima_min(const double* block_min, int nb) {
   __shared__ double red[BLOCK / 64];
-  // eight loads in flight per thread (the minimum does not depend on the order)
+  // eight loads in flight per thread
   double w[8];
 #pragma unroll
   for (int u = 0; u < 8; ++u) w[u] = 1.7976931348623157e308;
@@ -989,43 +1071,26 @@ __device__ __forceinline__ void dt_finalize_body(const Params& P, DevScalars* S,
       if (idx < nb) w[u] = smin(w[u], x);
     }
   }
-  if (halt) return;
   double v = w[0];
 #pragma unroll
   for (int u = 1; u < 8; ++u) v = smin(v, w[u]);
   for (int off = 32; off > 0; off >>= 1) v = smin(v, __shfl_down(v, off, 64));
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  double minimum_aux = red[0];
-  for (int w = 1; w < BLOCK / 64; ++w) minimum_aux = smin(minimum_aux, red[w]);
-  const double t = ADVANCE ? time0 + old : time0;
-  if (ADVANCE) {
-    S->time = t;
-    S->iteration += 1;
-  }
-  S->dt_old = old;
-  double new_dt = P.cfl * minimum_aux;
-  new_dt = smin(new_dt, 1.05 * old);
-  new_dt = smin(new_dt, P.dtmax);
-  if (new_dt < P.dtmin) {
-    S->err = 1;
-    return;
-  }
-  if (P.land) {  // extension: the last step ends at tend
-    if (S->landing || t >= P.tend) S->stop = 1;
-    else if (t + new_dt >= P.tend) {
-      new_dt = P.tend - t;
-      S->landing = 1;
-    }
-    S->dt = new_dt;
-    return;
-  }
-  S->dt = new_dt;
-  const bool not_yet_finish = (t < P.tend);
-  const bool finish = (t > P.tend);
-  const bool too_much = ((t + new_dt) > P.tend);
-  if ((not_yet_finish && too_much) || finish) S->stop = 1;
+  double m = red[0];
+#pragma unroll
+  for (int q = 1; q < BLOCK / 64; ++q) m = smin(m, red[q]);
+  return m;
+}
+
+// The dt kernel's body: one block reduces the block minima of the CFL
+// kernel and updates the scalars in place.
+template <int BLOCK, bool ADVANCE>
+__device__ __forceinline__ void dt_finalize_body(const Params& P, DevScalars* S, const double* block_min, int nb) {
+  const DevScalars s = *S;
+  const double m = block_minima_min<BLOCK>(block_min, nb);
+  if (s.stop || s.err) return;
+  if (threadIdx.x == 0) *S = dt_update<ADVANCE>(P, s, m);
 }
 
 // ---------------------------------------------------------------------------

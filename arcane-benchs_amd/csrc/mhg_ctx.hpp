@@ -68,6 +68,15 @@ struct mhg_ctx {
   double* block_min = nullptr;
   mhg::DevScalars* S = nullptr;
   mhg::DevScalars* S_host = nullptr;  // pinned mirror
+  // Second device copy of the scalars: a fused 2D step that folds the
+  // previous step's dt kernel into its node phase reads S and leaves the
+  // updated scalars here, and the two pointers swap.
+  mhg::DevScalars* S_alt = nullptr;
+  // The last enqueued fused step left its CFL minima (dt_pending_nb of them
+  // in block_min) for the next step's node phase; false whenever control
+  // returns to the caller.
+  bool dt_pending = false;
+  int dt_pending_nb = 0;
 
   // host mirrors of the step bookkeeping
   int xcur = 0;  // buffer with the coordinates of the last Lagrange phase
@@ -101,6 +110,7 @@ struct mhg_ctx {
   struct StepGraph {
     int xcur, ucur, sens;
     bool lazy, lazy_end;  // x_lazy at the start and at the end of the group
+    mhg::DevScalars *s0, *s_end;  // current scalars at the start and at the end
     hipGraphExec_t exec;
   };
   std::vector<StepGraph> graphs;

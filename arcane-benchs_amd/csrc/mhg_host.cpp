@@ -205,10 +205,13 @@ GenericSweepArgs generic_args(mhg_ctx* c, int pass) {
   return A;
 }
 
+// dt_mode 3: as 2, with the dt kernel left to the next step's node phase
+// (c->dt_pending) when the step is fused.
 template <class R>
 void remap_2d(mhg_ctx* c, int dt_mode, bool eos = true, bool lag = false) {
   const Params& P = c->P;
-  const bool fuse = dt_mode == 2 && eos && c->fuse_final && fused_final_supported(P);
+  const bool fuse = (dt_mode == 2 || dt_mode == 3) && eos && c->fuse_final && fused_final_supported(P);
+  if (dt_mode == 3 && !fuse) dt_mode = 2;
   for (int pass = 0; pass < P.dim; ++pass) {
     const int idir = (pass + c->sens) % P.dim;
     const bool first = (pass == 0);
@@ -241,7 +244,12 @@ void remap_2d(mhg_ctx* c, int dt_mode, bool eos = true, bool lag = false) {
     A.uz = nullptr;
     A.block_min = c->block_min;
     launch_cfl_2d(c->stream, P, c->S, A);
-    launch_dt_finalize(c->stream, P, c->S, c->block_min, true, cfl_blocks(P));
+    if (dt_mode == 3) {
+      c->dt_pending = true;
+      c->dt_pending_nb = cfl_blocks(P);
+    } else {
+      launch_dt_finalize(c->stream, P, c->S, c->block_min, true, cfl_blocks(P));
+    }
   } else if (!fuse) {
     TimedLaunch t(c, "cell_final");
     launch_cell_final_2d(c->stream, P, c->S, cell_final_args<R>(c), dt_mode, eos);
@@ -253,8 +261,10 @@ void remap_2d(mhg_ctx* c, int dt_mode, bool eos = true, bool lag = false) {
 
 // One fused iteration, enqueued on the context stream.  The time and iteration
 // advance of the iteration start is applied by the final dt kernel.
+// last: the step ends a sequence the caller sees, so its dt kernel runs;
+// otherwise a fused step leaves it to the next step's node phase.
 template <class R>
-void enqueue_step_t(mhg_ctx* c) {
+void enqueue_step_t(mhg_ctx* c, bool last) {
   if (c->use_mlp) {
     TimedLaunch t(c, "learned_alpha");
     enqueue_learned_alpha<R>(c, nullptr, nullptr, true);
@@ -273,13 +283,21 @@ void enqueue_step_t(mhg_ctx* c) {
       A.upx = bp<R>(c->ux[1 - c->ucur]);
       A.upy = bp<R>(c->uy[1 - c->ucur]);
     }
-    launch_node_lagrange_2d(c->stream, c->P, c->S, A, true, true, !formed);
+    if (c->dt_pending) {
+      launch_node_lagrange_fold_2d(c->stream, c->P, c->S, c->S_alt, A, !formed, c->block_min, c->dt_pending_nb);
+      std::swap(c->S, c->S_alt);
+      c->dt_pending = false;
+    } else {
+      launch_node_lagrange_2d(c->stream, c->P, c->S, A, true, true, !formed);
+    }
   }
   if (!lag) {
     TimedLaunch t(c, "cell_lagrange");
     launch_cell_lagrange_2d(c->stream, c->P, c->S, cell_lag_args<R>(c));
   }
-  remap_2d<R>(c, 2, true, lag);
+  // the learned coefficients' kernel, which starts the next step, reads the
+  // scalars before any node phase: no deferred dt then
+  remap_2d<R>(c, (last || c->use_mlp) ? 2 : 3, true, lag);
   c->x_lazy = formed;
 }
 
@@ -428,14 +446,14 @@ void enqueue_step_3d(mhg_ctx* c) {
   remap_3d<R>(c, 2);
 }
 
-void enqueue_step(mhg_ctx* c) {
+void enqueue_step(mhg_ctx* c, bool last) {
   if (c->P.dim == 3) {
     if (c->bulk == 4) enqueue_step_3d<float>(c);
     else enqueue_step_3d<double>(c);
     return;
   }
-  if (c->bulk == 4) enqueue_step_t<float>(c);
-  else enqueue_step_t<double>(c);
+  if (c->bulk == 4) enqueue_step_t<float>(c, last);
+  else enqueue_step_t<double>(c, last);
 }
 
 // Steps per captured graph: a multiple of the period of the host buffer state
@@ -460,30 +478,38 @@ int enqueue_steps(mhg_ctx* c, int n) {
   while (c->use_graphs && !c->timing && n >= kGraphSteps) {
     hipGraphExec_t exec = nullptr;
     bool lazy_end = false;
+    mhg::DevScalars* s_end = nullptr;
     for (auto& g : c->graphs)
-      if (g.xcur == c->xcur && g.ucur == c->ucur && g.sens == c->sens && g.lazy == c->x_lazy) {
+      if (g.xcur == c->xcur && g.ucur == c->ucur && g.sens == c->sens && g.lazy == c->x_lazy && g.s0 == c->S) {
         exec = g.exec;
         lazy_end = g.lazy_end;
+        s_end = g.s_end;
       }
     if (!exec) {
-      mhg_ctx::StepGraph g{c->xcur, c->ucur, c->sens, c->x_lazy, false, nullptr};
+      mhg_ctx::StepGraph g{c->xcur, c->ucur, c->sens, c->x_lazy, false, c->S, nullptr, nullptr};
       HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      for (int s = 0; s < kGraphSteps; ++s) enqueue_step(c);
+      for (int s = 0; s < kGraphSteps; ++s) enqueue_step(c, s == kGraphSteps - 1);
       hipGraph_t graph = nullptr;
       HIPCHK(c, hipStreamEndCapture(c->stream, &graph));
       const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
       hipGraphDestroy(graph);
       HIPCHK(c, e);
       g.lazy_end = c->x_lazy;
+      g.s_end = c->S;
       c->graphs.push_back(g);
       exec = g.exec;
       lazy_end = g.lazy_end;
+      // the capture swapped the scalar buffers as a launch would: back to
+      // the start, for the launch below
+      if (c->S != g.s0) std::swap(c->S, c->S_alt);
+      s_end = g.s_end;
     }
     HIPCHK(c, hipGraphLaunch(exec, c->stream));
     c->x_lazy = lazy_end;  // the host state the group's last step leaves
+    if (c->S != s_end) std::swap(c->S, c->S_alt);
     n -= kGraphSteps;
   }
-  for (int s = 0; s < n; ++s) enqueue_step(c);
+  for (int s = 0; s < n; ++s) enqueue_step(c, s == n - 1);
   HIPCHK(c, hipGetLastError());
   return MHG_OK;
 }
@@ -870,6 +896,7 @@ int mhg_create(const mhg_mesh_desc* mesh, const mhg_options* opt, mhg_ctx** out)
   if (P.learned) r |= balloc(c, &c->alpha, 2 * (size_t)NC);  // coefficients 0 until set
   r |= dalloc(c, &c->block_min, cell_final_blocks(P));
   r |= dalloc(c, &c->S, 1);
+  r |= dalloc(c, &c->S_alt, 1);
   if (r) {
     mhg_destroy(c);
     return MHG_E_ALLOC;
@@ -1566,7 +1593,7 @@ int mhg_step(mhg_ctx* c, int32_t* stop) {
   if (r) return r;
   const int it0 = c->S_host->iteration;
   c->x_lazy_before = c->x_lazy;
-  enqueue_step(c);
+  enqueue_step(c, true);
   HIPCHK(c, hipGetLastError());
   r = read_scalars(c);
   if (r) return r;

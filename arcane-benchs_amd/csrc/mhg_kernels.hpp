@@ -222,6 +222,11 @@ void launch_begin_iteration(hipStream_t st, DevScalars* S);
 template <class R>
 void launch_lag_coords_2d(hipStream_t st, const Params& P, DevScalars* S, const R* ux, const R* uy, double* xx,
                           double* xy);
+// The node phase with the previous step's dt kernel folded in: reads the
+// scalars Sin and the nb block minima, writes the updated scalars to Sout.
+template <class R>
+void launch_node_lagrange_fold_2d(hipStream_t st, const Params& P, const DevScalars* Sin, DevScalars* Sout,
+                                  const NodeLagArgsT<R>& A, bool pos, const double* block_min, int nb);
 template <class R>
 void launch_node_lagrange_2d(hipStream_t st, const Params& P, DevScalars* S, const NodeLagArgsT<R>& A,
                              bool vel, bool bc, bool pos);

@@ -45,6 +45,17 @@ __global__ void k_node_lagrange_2d(Params P, DevScalars* S, NodeLagArgsT<R> A) {
   node_lagrange_2d_body<R, VEL, BC, POS>(P, S, A, xcd_block(blockIdx.x, gridDim.x));
 }
 
+// The node phase of a fused step with the previous step's dt kernel folded
+// in (node_lagrange_2d_body's FOLD): Sin is only read, so a block that starts
+// late sees what the first saw; block 0 writes the updated scalars to
+// F.Sout, which the rest of the step reads.  A run that has ended copies Sin
+// to F.Sout unchanged.
+template <class R, bool VEL, bool BC, bool POS>
+__global__ __launch_bounds__(256) void k_node_lagrange_fold_2d(Params P, DevScalars* Sin, DtFold F,
+                                                               NodeLagArgsT<R> A) {
+  node_lagrange_2d_body<R, VEL, BC, POS, true>(P, Sin, A, xcd_block(blockIdx.x, gridDim.x), &F);
+}
+
 template <class R, bool PC, int NEM>
 __global__ __launch_bounds__(256) void k_cell_lagrange_2d(Params P, DevScalars* S, CellLagArgsT<R> A) {
   cell_lagrange_2d_body<R, PC, NEM>(P, S, A, blockIdx.x);
@@ -752,6 +763,17 @@ void launch_node_lagrange_2d(hipStream_t st, const Params& P, DevScalars* S, con
   else if (vel && bc && !pos) hipLaunchKernelGGL((k_node_lagrange_2d<R, true, true, false>), g, b, 0, st, P, S, A);
 }
 
+template <class R>
+void launch_node_lagrange_fold_2d(hipStream_t st, const Params& P, const DevScalars* Sin, DevScalars* Sout,
+                                  const NodeLagArgsT<R>& A, bool pos, const double* block_min, int nb) {
+  const int B = 256;
+  dim3 g(nblocks(P.nn, B)), b(B);
+  const DtFold F{block_min, nb, Sout};
+  DevScalars* in = const_cast<DevScalars*>(Sin);
+  if (pos) hipLaunchKernelGGL((k_node_lagrange_fold_2d<R, true, true, true>), g, b, 0, st, P, in, F, A);
+  else hipLaunchKernelGGL((k_node_lagrange_fold_2d<R, true, true, false>), g, b, 0, st, P, in, F, A);
+}
+
 // The last Lagrange phase's node coordinates x0 + dt_old u (lag_coord) into
 // the coordinate arrays, after fused steps that formed them without storing
 // them.  It runs whatever the run's state (an ended run keeps its arrays).
@@ -1170,29 +1192,33 @@ __global__ __launch_bounds__(256) void k_pression_moyenne(Params P, DevScalars* 
 }
 
 // CFL candidates of quads: the same arithmetic as k_cell_final_2d's, in the
-// storage type R of the bulk fields.  A block owns a contiguous run of
-// cells, 256 at a time (XCD-contiguous runs), and leaves one minimum: with at
-// most kCflBlocks blocks the dt kernel that follows reduces fewer minima.
-constexpr int kCflBlocks = 2048;
-int cfl_blocks(const Params& P) { return cell_final_blocks(P) < kCflBlocks ? cell_final_blocks(P) : kCflBlocks; }
+// storage type R of the bulk fields.  A block of kCflThreads owns a
+// contiguous run of cells (XCD-contiguous runs) and leaves one minimum: with
+// at most kCflBlocks blocks the kernel that reduces them (the dt kernel, or
+// every block of the next step's node phase) reads few minima.
+constexpr int kCflBlocks = 512, kCflThreads = 1024;
+int cfl_blocks(const Params& P) {
+  const int nb = (int)nblocks(P.nc, kCflThreads);
+  return nb < kCflBlocks ? nb : kCflBlocks;
+}
 
 template <class R>
-__global__ __launch_bounds__(256) void k_cfl_2d(Params P, DevScalars* S, CflArgsT<R> A) {
+__global__ __launch_bounds__(kCflThreads) void k_cfl_2d(Params P, DevScalars* S, CflArgsT<R> A) {
   const bool halt = halted(S);
   const int nb = (int)gridDim.x;
   const int lb = xcd_block(blockIdx.x, nb);
-  const long per = ((P.nc + nb - 1) / nb + 255) / 256 * 256;
+  const long per = ((P.nc + nb - 1) / nb + kCflThreads - 1) / kCflThreads * kCflThreads;
   const long c0 = (long)lb * per, c1 = (c0 + per < P.nc) ? c0 + per : P.nc;
   double dx_sound = 1.7976931348623157e308;
   const int nnx = P.nx + 1;
-  // the thread's first cell (i, j); each later one is 256 cells further on,
+  // the thread's first cell (i, j); each later one is kCflThreads further on,
   // so (i, j) advance without a division and the node loads do not wait
   // for one
   long c = c0 + threadIdx.x;
   int i, j;
   divmod32(c, P.nx, j, i);
 #pragma unroll 2
-  for (; c < c1; c += 256) {
+  for (; c < c1; c += kCflThreads) {
     R vmax = R(0);
     const R Lc = A.L[c], cc = A.cs[c];
     if (!P.lagonly) {
@@ -1216,14 +1242,14 @@ __global__ __launch_bounds__(256) void k_cfl_2d(Params P, DevScalars* S, CflArgs
       vmax = sqrt(n2max);
     }
     dx_sound = smin(dx_sound, (double)qdiv(Lc, cc + vmax));
-    i += 256;
+    i += kCflThreads;
     while (i >= P.nx) {
       i -= P.nx;
       ++j;
     }
   }
   if (halt) return;
-  block_min_store<256>(dx_sound, A.block_min + lb);
+  block_min_store<kCflThreads>(dx_sound, A.block_min + lb);
 }
 
 void launch_eos_env(hipStream_t st, const Params& P, DevScalars* S, const EosArgs& A, int k, int mode, long c0,
@@ -1248,12 +1274,12 @@ void launch_cfl(hipStream_t st, const Params& P, DevScalars* S, const CflArgs& A
     launch_cfl_3d(st, P, S, A);
     return;
   }
-  hipLaunchKernelGGL(k_cfl_2d<double>, dim3(cfl_blocks(P)), dim3(256), 0, st, P, S, A);
+  hipLaunchKernelGGL(k_cfl_2d<double>, dim3(cfl_blocks(P)), dim3(kCflThreads), 0, st, P, S, A);
 }
 
 template <class R>
 void launch_cfl_2d(hipStream_t st, const Params& P, DevScalars* S, const CflArgsT<R>& A) {
-  hipLaunchKernelGGL(k_cfl_2d<R>, dim3(cfl_blocks(P)), dim3(256), 0, st, P, S, A);
+  hipLaunchKernelGGL(k_cfl_2d<R>, dim3(cfl_blocks(P)), dim3(kCflThreads), 0, st, P, S, A);
 }
 
 void launch_dt_finalize(hipStream_t st, const Params& P, DevScalars* S, const double* block_min, bool advance,
@@ -1281,6 +1307,9 @@ void launch_cell_final_2d(hipStream_t st, const Params& P, DevScalars* S, const 
 #define MHG_INSTANTIATE(R)                                                                           \
   template void launch_node_lagrange_2d<R>(hipStream_t, const Params&, DevScalars*,                  \
                                            const NodeLagArgsT<R>&, bool, bool, bool);                \
+  template void launch_node_lagrange_fold_2d<R>(hipStream_t, const Params&, const DevScalars*,       \
+                                                DevScalars*, const NodeLagArgsT<R>&, bool,           \
+                                                const double*, int);                                 \
   template void launch_cell_lagrange_2d<R>(hipStream_t, const Params&, DevScalars*,                  \
                                            const CellLagArgsT<R>&);                                  \
   template void launch_lag_coords_2d<R>(hipStream_t, const Params&, DevScalars*, const R*, const R*,  \
