@@ -498,6 +498,11 @@ __device__ __forceinline__ void divmod32(long n, int d, int& q, int& r) {
   r = (int)(un - uq * ud);
 }
 
+// Params::ytw8 bits: the y sweeps that run on 8-column tiles (the others on
+// 16 columns): the first cell sweep with the Lagrange cell phase, the last
+// with the final cell phase, the dual sweeps.
+constexpr int kYtwLag = 1, kYtwFinal = 2, kYtwNode = 4;
+
 // Run-time parameters of a context.
 struct Params {
   int dim, ne, nvu;  // nvu = 3*ne+1 live cell variables
@@ -524,7 +529,7 @@ struct Params {
   // next kernel of the step); chosen per context from the working set,
   // settable by the caller
   int nt_store;
-  int ytw8;  // the fused y sweeps on LDS tiles of 8 columns (512-thread blocks) instead of 16
+  int ytw8;  // y sweeps on LDS tiles of 8 columns (512-thread blocks) instead of 16: kYtw* bits
   double cfl, dtmin, dtmax, tend;
   int nbc;
   BcRange bc[kMaxBc];

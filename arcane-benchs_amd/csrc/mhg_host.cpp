@@ -785,8 +785,9 @@ int build_params(const mhg_mesh_desc* mesh, const mhg_options* opt, Params* Pout
   // fused y sweeps on 8-column tiles, 512-thread blocks: more blocks per CU
   // and half the wavefronts at each barrier; 1792x768 triple point 0.2355 ->
   // 0.2313 ms per FP64 step, 0.1222 -> 0.1195 in FP32 (4 columns: 0.2498,
-  // 0.1328)
-  P.ytw8 = 1;
+  // 0.1328).  The FP64 sweep with the Lagrange cell phase keeps 16 columns
+  // (70.0 us against 75.7 on 8).
+  P.ytw8 = opt->precision == MHG_PREC_FP32_BULK ? (kYtwLag | kYtwFinal | kYtwNode) : (kYtwFinal | kYtwNode);
   P.nbc = 0;
   for (int b = 0; b < opt->nb_bc; ++b) {
     const mhg_bc& bc = opt->bc[b];
@@ -1728,7 +1729,7 @@ int variant_count_of(int k) {
   if (k == 2) return node_sweep_variants(1);
   if (k >= 3 && k <= 5) return 2;
   if (k == 6) return 16;  // one bit per kind of array (st_stream)
-  if (k == 7) return 2;
+  if (k == 7) return 8;
   return 0;
 }
 const char* variant_name_of(int k, int v) {
@@ -1739,7 +1740,11 @@ const char* variant_name_of(int k, int v) {
                                     "s8", "s9", "s10", "s11", "s12", "s13", "s14", "streaming"};
     return v >= 0 && v < 16 ? names[v] : "?";
   }
-  if (k == 7) return v ? "tile8" : "tile16";
+  if (k == 7) {  // 8 or 16 columns for the lag, final and dual y sweeps
+    static const char* names[8] = {"l16f16n16", "l8f16n16", "l16f8n16", "l8f8n16",
+                                   "l16f16n8",  "l8f16n8",  "l16f8n8",  "l8f8n8"};
+    return names[v & 7];
+  }
   return k == 0 ? cell_sweep_y_variant_name(v) : node_sweep_variant_name(k - 1, v);
 }
 int* variant_slot(mhg_ctx* c, int k) {

@@ -911,7 +911,7 @@ void launch_cell_sweep_final_2d(hipStream_t st, const Params& P, DevScalars* S, 
 #undef XF_
     return;
   }
-  const int tw = P.ytw8 ? 8 : kTileW;
+  const int tw = (P.ytw8 & kYtwFinal) ? 8 : kTileW;
   dim3 g((unsigned)(((P.nx + tw - 1) / tw) * ((P.ny + 59) / 60))), b(64 * tw);
 #define YF_(N, LM)                                                                                     \
   if (tw == 8) hipLaunchKernelGGL((k_cell_sweep_y_final_2d<R, N, LM, 8>), g, b, 0, st, P, S, A, F); \
@@ -949,7 +949,7 @@ void launch_cell_sweep_lag_2d(hipStream_t st, const Params& P, DevScalars* S, co
 #undef XG_
     return;
   }
-  const int tw = P.ytw8 ? 8 : kTileW;
+  const int tw = (P.ytw8 & kYtwLag) ? 8 : kTileW;
   dim3 g((unsigned)(((P.nx + tw - 1) / tw) * ((P.ny + 59) / 60))), b(64 * tw);
 #define YG_(N, LM)                                                                                   \
   if (tw == 8) hipLaunchKernelGGL((k_cell_sweep_y_lag_2d<R, N, LM, 8>), g, b, 0, st, P, S, A, LA); \
@@ -1081,7 +1081,7 @@ void launch_node_sweep_2d(hipStream_t st, const Params& P, DevScalars* S, const 
 #undef NX_
     return;
   }
-  const int tw = P.ytw8 ? 8 : kTileW;
+  const int tw = (P.ytw8 & kYtwNode) ? 8 : kTileW;
   dim3 g((unsigned)(((P.nx + 1 + tw - 1) / tw) * ((P.ny + 1 + 59) / 60))), b(64 * tw);
 #define NT_(N, LM)                                                                                        \
   if (tw == 8 && first) hipLaunchKernelGGL((k_node_sweep_ytile_2d<R, true, N, LM, 8>), g, b, 0, st, P, S, A); \

@@ -24,7 +24,8 @@ import sys
 
 # step kernel label (as timed by the library) -> kernel name prefixes
 STEP_KERNELS = {
-    "node_lagrange": ["k_node_lagrange_2d"],
+    # with the previous step's dt kernel folded in (fused steps after the first)
+    "node_lagrange": ["k_node_lagrange_2d", "k_node_lagrange_fold_2d"],
     "cell_lagrange": ["k_cell_lagrange_2d"],
     # first cell sweep with the Lagrange cell phase, last with the final one
     "cell_sweep_x_lag": ["k_cell_sweep_x_lag_2d"],

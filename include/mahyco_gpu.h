@@ -294,8 +294,10 @@ int mhg_autotune(mhg_ctx* ctx, int32_t reps, char* report, int32_t report_len);
  * vector between the cell sweeps, 4 the dual fluxes and dual state, 8 node
  * velocities and coordinates; 0 all ordinary stores, which stay in the
  * caches for the next kernel, 15 all streaming; the default follows the
- * working set); "ytile_width": the fused step's y sweeps on LDS tiles of 16
- * columns (0) or 8 (1, the default)), 0 for an unknown name. */
+ * working set); "ytile_width": which y sweeps run on LDS tiles of 8 columns
+ * instead of 16, one bit per kind (1 the first cell sweep with the Lagrange
+ * cell phase, 2 the last with the final cell phase, 4 the dual sweeps; the
+ * default 6 in FP64, 7 in FP32-bulk)), 0 for an unknown name. */
 int32_t mhg_variant_count(const char* kernel);
 /* Selects launch variant `variant` of a tunable kernel for later steps
  * (MHG_E_ARG for an unknown kernel or an index out of range). */

@@ -398,9 +398,10 @@ def test_formed_coordinates_match_stored(gpu):
 @pytest.mark.parametrize("prec", [H.abi.PREC_FP64, H.abi.PREC_FP32_BULK])
 def test_ytile_width_bit_identical(gpu, prec):
     """The fused y sweeps on tiles of 8 columns (512-thread blocks) give the
-    state of the 16-column tiles bit for bit."""
+    state of the 16-column tiles bit for bit, each sweep kind on its own
+    (bits 1, 2, 4: lag, final, dual sweeps) and all together."""
     states = []
-    for v in (0, 1):
+    for v in (0, 1, 2, 4, 7):
         s = H.tp_setup(gpu, 70, 134, precision=prec)
         s.select_variant(v, "ytile_width")
         s.run(13)
