@@ -48,7 +48,9 @@ __global__ __launch_bounds__(256) void k_node_lagrange_3d(Params P, DevScalars* 
   const long nid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (nid >= P.nn) return;
   const int nnx = P.nx + 1, nny = P.ny + 1;
-  const int i = (int)(nid % nnx), j = (int)((nid / nnx) % nny), k = (int)(nid / ((long)nnx * nny));
+  int i, j, k, t_;
+  divmod32(nid, nnx, t_, i);
+  divmod32(t_, nny, k, j);
   const double dt = S->dt;
   double vx = A.b.ux[nid], vy = A.b.uy[nid], vz = A.uz[nid];
   if (VEL) {
@@ -237,7 +239,9 @@ __global__ __launch_bounds__(256) void k_cell_lagrange_3d(Params P, DevScalars* 
   if (S->stop || S->err) return;
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= P.nc) return;
-  const int i = (int)(c % P.nx), j = (int)((c / P.nx) % P.ny), k = (int)(c / ((long)P.nx * P.ny));
+  int i, j, k, t_;
+  divmod32(c, P.nx, t_, i);
+  divmod32(t_, P.ny, k, j);
   const CellPre<R> in = cell_pre_load<R, PC>(A.b, c);
   V3 cn[8];
   hex_nodes(P, A.b.xnx, A.b.xny, A.xnz, i, j, k, cn);
@@ -372,10 +376,12 @@ __global__ __launch_bounds__(256) void k_cell_sweep_x_3d(Params P, DevScalars* S
   constexpr int HL = (LIM == kLimO3) ? 3 : 2, NO = 64 - 2 * HL;  // halo lanes (order 3: 3), outputs
   const int nseg = (N + NO - 1) / NO;
   const long wave = ((long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
-  const long line = wave / nseg;
+  int line_, seg;
+  divmod32(wave, nseg, line_, seg);
+  const long line = line_;
   if (line >= (long)P.ny * P.nz) return;  // uniform across the wave
-  const int seg = (int)(wave % nseg);
-  const int j = (int)(line % P.ny), k = (int)(line / P.ny);
+  int j, k;
+  divmod32(line, P.ny, k, j);
   const int i = seg * NO - HL + lane;
   const bool ex = (i >= 0 && i < N);
   const bool out = ex && lane >= HL && lane <= 63 - HL;
@@ -899,10 +905,12 @@ __global__ __launch_bounds__(256) void k_node_sweep_x_3d(Params P, DevScalars* S
   const int N = P.nx;
   const int nseg = (N + 1 + 59) / 60;
   const long wave = ((long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
-  const long line = wave / nseg;
+  int line_, seg;
+  divmod32(wave, nseg, line_, seg);
+  const long line = line_;
   if (line >= (long)(P.ny + 1) * (P.nz + 1)) return;  // uniform across the wave
-  const int seg = (int)(wave % nseg);
-  const int j = (int)(line % (P.ny + 1)), k = (int)(line / (P.ny + 1));
+  int j, k;
+  divmod32(line, P.ny + 1, k, j);
   const int i = seg * 60 - 2 + lane;
   const bool ex = (i >= 0 && i <= N);
   const bool out = ex && lane >= 2 && lane <= 61;
@@ -1114,7 +1122,9 @@ __global__ __launch_bounds__(BLOCK) void k_cell_final_3d(Params P, DevScalars* S
   double dx_sound = 1.7976931348623157e308;
   if (!EOS && c >= P.nc) return;  // remapVariables only: no CFL candidate
   if (c < P.nc) {
-    const int i = (int)(c % P.nx), j = (int)((c / P.nx) % P.ny), k = (int)(c / ((long)P.nx * P.ny));
+    int i, j, k, t_;
+  divmod32(c, P.nx, t_, i);
+  divmod32(t_, P.ny, k, j);
     // total-energy fix-up: the deposit, the 8 nodes' terms in cell node order
     R delta_ec = R(0);
     if (P.etot) {
@@ -1159,7 +1169,9 @@ __global__ __launch_bounds__(256) void k_cell_geometry_3d(Params P, DevScalars* 
   if (S->stop || S->err) return;
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= P.nc) return;
-  const int i = (int)(c % P.nx), j = (int)((c / P.nx) % P.ny), k = (int)(c / ((long)P.nx * P.ny));
+  int i, j, k, t_;
+  divmod32(c, P.nx, t_, i);
+  divmod32(t_, P.ny, k, j);
   V3 cn[8];
   hex_nodes(P, xnx, xny, xnz, i, j, k, cn);
   const double Vd = hex_volume(cn);
@@ -1174,7 +1186,9 @@ __global__ __launch_bounds__(256) void k_cfl_3d(Params P, DevScalars* S, CflArgs
   if (c < P.nc) {
     double vmax = 0.;
     if (!P.lagonly) {
-      const int i = (int)(c % P.nx), j = (int)((c / P.nx) % P.ny), k = (int)(c / ((long)P.nx * P.ny));
+      int i, j, k, t_;
+  divmod32(c, P.nx, t_, i);
+  divmod32(t_, P.ny, k, j);
       // the root of the largest squared norm (sqrt is monotone)
       double n2max = 0.;
 #pragma unroll
