@@ -153,6 +153,35 @@ def test_graph_replay_matches_direct_launches(gpu):
         assert np.array_equal(np.asarray(da[k]), np.asarray(db[k])), k
 
 
+@pytest.mark.parametrize("prec", [H.abi.PREC_FP64, H.abi.PREC_FP32_BULK])
+def test_folded_dt_matches_single_steps(gpu, prec):
+    """Inside one run the dt update of a step runs in the next step's node
+    phase (two scalar buffers alternate) and the run's last step keeps its dt
+    kernel; single steps each run the dt kernel.  29 steps in one call (two
+    12-step graphs and a tail of 5) and 29 calls of one step give the same
+    state and scalars bit for bit, and so does a run that stops at its final
+    time inside a graph."""
+    a = H.tp_setup(gpu, 70, 134, precision=prec)
+    b = H.tp_setup(gpu, 70, 134, precision=prec)
+    assert a.run(29) == 29
+    for _ in range(29):
+        assert b.step() == 0
+    da, db = a.download(), b.download()
+    for k in da:
+        assert np.array_equal(np.asarray(da[k]), np.asarray(db[k])), k
+    t_end = da["global_time"] + 0.5 * da["global_deltat"] * 15  # stops some steps into the next graph
+    c = H.tp_setup(gpu, 70, 134, precision=prec, final_time=t_end)
+    d = H.tp_setup(gpu, 70, 134, precision=prec, final_time=t_end)
+    n = c.run(1000)
+    m = 0
+    while d.step() == 0:
+        m += 1
+    dc, dd = c.download(), d.download()
+    assert dc["global_iteration"] == dd["global_iteration"] and n > 29
+    for k in dc:
+        assert np.array_equal(np.asarray(dc[k]), np.asarray(dd[k])), k
+
+
 ENTRY_SEQUENCE = ["begin_iteration", "save_values_at_n", "update_velocity",
                   "apply_boundary_condition", "update_position", "compute_geometric_values",
                   "update_density", "compute_artificial_viscosity", "update_energy_and_pressure",
