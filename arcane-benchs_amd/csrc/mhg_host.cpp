@@ -474,6 +474,16 @@ namespace {
 // Enqueues n fused iterations.  Whole groups of kGraphSteps replay a graph
 // captured once per buffer state (one launch for the group's kernels); the
 // rest, and every step while kernel timing is on, launch directly.
+// A sequence that ended early (a HIP error between its steps) may have left
+// the last step's dt update to a node phase that never ran: run it now.
+int flush_dt(mhg_ctx* c) {
+  if (!c->dt_pending) return MHG_OK;
+  launch_dt_finalize(c->stream, c->P, c->S, c->block_min, true, c->dt_pending_nb);
+  c->dt_pending = false;
+  HIPCHK(c, hipGetLastError());
+  return MHG_OK;
+}
+
 int enqueue_steps(mhg_ctx* c, int n) {
   while (c->use_graphs && !c->timing && n >= kGraphSteps) {
     hipGraphExec_t exec = nullptr;
@@ -566,7 +576,7 @@ int check_fused(mhg_ctx* c) {
   if (r) return r;
   if (c->coord_phase != 0)
     return fail(c, MHG_E_STATE, "fused step inside an entry-point iteration: finish it with mhg_remap");
-  return MHG_OK;
+  return flush_dt(c);
 }
 
 // Pure-cell initial state (hydroStartInit, MahycoModule.cc:48-188, then
