@@ -82,3 +82,22 @@ def test_batch_create_rejects_mixed_problems():
     assert make(2, envs=(1, 2)) == abi.E_UNSUPPORTED
     assert make(2, envs=(4, 4)) == abi.E_UNSUPPORTED
     assert make(2, prec=(0, 1)) == abi.E_UNSUPPORTED
+
+
+def test_create_rejects_meshes_beyond_31_bit_indices():
+    """mhg_create refuses a mesh with 2^31 nodes or more before any device
+    call (the kernels split cell and node indices in 32-bit arithmetic); a
+    mesh just below the limit passes that check (it is not created here)."""
+    import ctypes as C
+    lib = abi.Library(GPU_LIB, "mhg_")
+    o = lib.options()
+    abi.set_bcs(o, abi.reflecting_walls(2))
+    abi.set_envs(o, [(abi.EOS_PERFECT_GAS, 1.4, 0.)])
+    h = C.c_void_p()
+    big = abi.mesh_desc(2, [46341, 46341], [0, 0], [1., 1.])  # 46342^2 > 2^31 nodes
+    assert lib.create(C.byref(big), C.byref(o), C.byref(h)) == abi.E_UNSUPPORTED
+    big3 = abi.mesh_desc(3, [1290, 1290, 1290], [0, 0, 0], [1., 1., 1.])  # 1291^3 > 2^31
+    o3 = lib.options()
+    abi.set_bcs(o3, abi.reflecting_walls(3))
+    abi.set_envs(o3, [(abi.EOS_PERFECT_GAS, 1.4, 0.)])
+    assert lib.create(C.byref(big3), C.byref(o3), C.byref(h)) == abi.E_UNSUPPORTED
