@@ -609,7 +609,8 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
     // Euler volume: the corner-normal formula on the initial coordinates
     double x0[4] = {node_x0(P, i), node_x0(P, i + 1), node_x0(P, i + 1), node_x0(P, i)};
     double y0[4] = {node_y0(P, j), node_y0(P, j), node_y0(P, j + 1), node_y0(P, j + 1)};
-    final_cell_vals<R, NEM>(P, *F, c, quad_volume(x0, y0), fV, fM, fE, fQ, R(0), fmask);
+    const double ve = kCartesianEuler<R> ? P.h[0] * P.h[1] : quad_volume(x0, y0);
+    final_cell_vals<R, NEM>(P, *F, c, ve, fV, fM, fE, fQ, R(0), fmask);
   }
 }
 
@@ -970,7 +971,8 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
     asm volatile("" : "+v"(fi), "+v"(fs));
     double x0[4] = {node_x0(P, fi), node_x0(P, fi + 1), node_x0(P, fi + 1), node_x0(P, fi)};
     double y0[4] = {node_y0(P, fs), node_y0(P, fs), node_y0(P, fs + 1), node_y0(P, fs + 1)};
-    final_cell_vals<R, NEM>(P, *F, gc, quad_volume(x0, y0), fV, fM, fE, fU[3 * ne], R(0), fmask);
+    const double ve = kCartesianEuler<R> ? P.h[0] * P.h[1] : quad_volume(x0, y0);
+    final_cell_vals<R, NEM>(P, *F, gc, ve, fV, fM, fE, fU[3 * ne], R(0), fmask);
   }
 }
 
@@ -1002,7 +1004,7 @@ __device__ __forceinline__ void cell_final_2d_body(const Params& P, DevScalars* 
     }
     double x0[4] = {node_x0(P, i), node_x0(P, i + 1), node_x0(P, i + 1), node_x0(P, i)};
     double y0[4] = {node_y0(P, j), node_y0(P, j), node_y0(P, j + 1), node_y0(P, j + 1)};
-    final_cell<R, NEM, false>(P, A, c, quad_volume(x0, y0), delta_ec);
+    final_cell<R, NEM, false>(P, A, c, kCartesianEuler<R> ? P.h[0] * P.h[1] : quad_volume(x0, y0), delta_ec);
     return;
   }
   const long c = blk * BLOCK + threadIdx.x;
@@ -1036,7 +1038,7 @@ __device__ __forceinline__ void cell_final_2d_body(const Params& P, DevScalars* 
     // Euler volume: the corner-normal formula on the initial coordinates.
     double x0[4] = {node_x0(P, i), node_x0(P, i + 1), node_x0(P, i + 1), node_x0(P, i)};
     double y0[4] = {node_y0(P, j), node_y0(P, j), node_y0(P, j + 1), node_y0(P, j + 1)};
-    const R c_g = final_cell<R, NEM>(P, A, c, quad_volume(x0, y0), delta_ec);
+    const R c_g = final_cell<R, NEM>(P, A, c, kCartesianEuler<R> ? P.h[0] * P.h[1] : quad_volume(x0, y0), delta_ec);
     // CFL candidate: L / (c + max over the cell nodes of |u|)
     // (the root of the largest squared norm: sqrt is correctly rounded, hence
     // monotone, so it equals the largest root)

@@ -1122,7 +1122,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell_final_3d(Params P, DevScalars* S
   double dx_sound = 1.7976931348623157e308;
   if (!EOS && c >= P.nc) return;  // remapVariables only: no CFL candidate
   if (c < P.nc) {
-    int i, j, k, t_;
+  int i, j, k, t_;
   divmod32(c, P.nx, t_, i);
   divmod32(t_, P.ny, k, j);
     // total-energy fix-up: the deposit, the 8 nodes' terms in cell node order
@@ -1137,7 +1137,8 @@ __global__ __launch_bounds__(BLOCK) void k_cell_final_3d(Params P, DevScalars* S
       }
       delta_ec = smax(R(0), sum);
     }
-    const R c_g = final_cell<R, NEM, EOS>(P, A.b, c, A.Veuler[c], delta_ec);
+    const double ve = kCartesianEuler<R> ? P.h[0] * P.h[1] * P.h[2] : A.Veuler[c];
+    const R c_g = final_cell<R, NEM, EOS>(P, A.b, c, ve, delta_ec);
     if (!EOS) return;
     // the root of the largest squared norm: sqrt is correctly rounded, hence
     // monotone, so it equals the largest root (one root per cell, not per node)
@@ -1186,7 +1187,7 @@ __global__ __launch_bounds__(256) void k_cfl_3d(Params P, DevScalars* S, CflArgs
   if (c < P.nc) {
     double vmax = 0.;
     if (!P.lagonly) {
-      int i, j, k, t_;
+    int i, j, k, t_;
   divmod32(c, P.nx, t_, i);
   divmod32(t_, P.ny, k, j);
       // the root of the largest squared norm (sqrt is monotone)
