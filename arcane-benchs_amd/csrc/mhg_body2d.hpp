@@ -284,7 +284,7 @@ __device__ __forceinline__ void cell_lagrange_2d_body(const Params& P, DevScalar
     y[q] = A.xny[nd[q]];
   }
   const double Vd = quad_volume(x, y);
-  const double Ld = (P.lc == MHG_LC_FACES_OPPOSEES) ? length_faces_opposees_2d(x, y) : length_sqrt_2d(Vd);
+  const double Ld = (P.lc == MHG_LC_FACES_OPPOSEES) ? length_faces_opposees_2d(x, y) : length_sqrt_2d_r<R>(Vd);
   lagrange_cell_bulk<R, PC, NEM>(P, S, A, c, Vd, Ld, in);
 }
 
@@ -408,7 +408,7 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
       yq[q] = D == 0 ? tn[q] : an[q];
     }
     const double Vd = quad_volume(xq, yq);
-    const double Ld = (P.lc == MHG_LC_FACES_OPPOSEES) ? length_faces_opposees_2d(xq, yq) : length_sqrt_2d(Vd);
+    const double Ld = (P.lc == MHG_LC_FACES_OPPOSEES) ? length_faces_opposees_2d(xq, yq) : length_sqrt_2d_r<R>(Vd);
     R uv[NEM], um[NEM], ue[NEM], uq;
     lagrange_cell_vals<R, false, NEM>(P, S, *LA, c, (R)Vd, (R)Ld, lin, uv, um, ue, uq);
 #pragma unroll
@@ -728,7 +728,7 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
     const CellPre<R> lin = cell_pre_load<R, false>(*LA, gc);
     const double yq[4] = {ya, yb, yc, yd};
     const double Vd = quad_volume(tn, yq);
-    const double Ld = (P.lc == MHG_LC_FACES_OPPOSEES) ? length_faces_opposees_2d(tn, yq) : length_sqrt_2d(Vd);
+    const double Ld = (P.lc == MHG_LC_FACES_OPPOSEES) ? length_faces_opposees_2d(tn, yq) : length_sqrt_2d_r<R>(Vd);
     R uv[NEM], um[NEM], ue[NEM], uq;
     lagrange_cell_vals<R, false, NEM>(P, S, *LA, gc, (R)Vd, (R)Ld, lin, uv, um, ue, uq);
 #pragma unroll

@@ -37,6 +37,15 @@ __device__ inline float qdiv(float a, float b) { return a * __builtin_amdgcn_rcp
 // calls, differs from the correctly rounded root by one ulp in about 0.09% of
 // inputs.
 __host__ __device__ inline double length_sqrt_2d(double V) { return sqrt(V); }
+// The same in the storage type R of the bulk fields: the float form stores L
+// in float and takes the hardware square root of the rounded volume (one
+// instruction, within 1 ulp of the rounded root); L is a length scale of
+// the viscosity and the CFL bound, where it cancels nothing.
+template <class R>
+__device__ inline double length_sqrt_2d_r(double V) {
+  if constexpr (std::is_same<R, float>::value) return (double)__builtin_amdgcn_sqrtf((float)V);
+  else return sqrt(V);
+}
 
 // Limiters, RemapADIService.h:260-286.
 template <int LIM, class T = double>
