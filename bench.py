@@ -368,7 +368,7 @@ def main():
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counts to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {args.gpus})",
+        print(f"bench.py runs on one GPU: the library has no multi-GPU path (asked for {args.gpus})",
               file=sys.stderr)
         sys.exit(2)
     try:
