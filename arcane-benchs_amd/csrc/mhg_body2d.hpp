@@ -465,8 +465,8 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
   const double dotxf = inner ? (upm ? xf - xcm : xf - xc) : 0.;
   const R uf_f = lane_down1(uf);
   const double lf_f = lane_down1(lf);
-  // face flux factors u_f L_f dt (double products, then the storage type)
-  const R fb = (R)(-1.0 * uf * lf * dt), ff = (R)(1.0 * uf_f * lf_f * dt);
+  // face flux factors u_f L_f dt (flux_factor)
+  const R fb = flux_factor<R>(-uf, lf, dt), ff = flux_factor<R>(uf_f, lf_f, dt);
   const R dxr = (R)dx, dotr = (R)dotxf;
   const bool o2 = P.order2 > 0;
   // learned limiter: coefficient, front-neighbour existence, face offsets
@@ -514,7 +514,7 @@ __device__ __forceinline__ void cell_sweep_lane_2d_body(const Params& P, DevScal
         }
       est = cnt > 1 ? -2 : last;
     }
-    vdt = (R)((double)uf * dt);
+    vdt = vel_dt<R>(uf, dt);
   }
   int estb = 0;
   if (LIM == kLimO3) estb = lane_up1(est);
@@ -817,7 +817,7 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
   const double dotxf = inner ? (upm ? xf - xcm : xf - xc) : 0.;
   const R uf_f = lane_down1(uf);
   const double lf_f = lane_down1(lf);
-  const R fb = (R)(-1.0 * uf * lf * dt), ff = (R)(1.0 * uf_f * lf_f * dt);
+  const R fb = flux_factor<R>(-uf, lf, dt), ff = flux_factor<R>(uf_f, lf_f, dt);
   const R dxr = (R)dx, dotr = (R)dotxf;
   // learned limiter: coefficient, front-neighbour existence, face offsets
   R alpha = R(0), dbr = R(0), dfr = R(0);
@@ -846,7 +846,7 @@ __device__ __forceinline__ void cell_sweep_ytile_2d_body(const Params& P, DevSca
   if (LIM == kLimO3) {
     est = tlane ? (int)lds[LIM == kLimO3 ? 6 : 5][ta] : 0;
     estb = lane_up1(est);
-    vdt = (R)((double)uf * dt);
+    vdt = vel_dt<R>(uf, dt);
   }
   const bool vpure = estb != -2 && est != -2 && estb == est;
   R o3V[NEM], o3Vb[NEM], o3Vf[NEM], o3M[NEM], o3Mb[NEM], o3Mf[NEM];
@@ -1294,14 +1294,14 @@ __device__ __forceinline__ void node_sweep_x_2d_body(const Params& P, DevScalars
     const int signfront = (ufront > R(0)) ? 1 : -1;
     const double xfr = xdp - xd;
     const bool sf = (front < R(0));
-    const R tf = (R)(signfront * xfr - dt * ufront);
+    const R tf = upwind_offset<R>(signfront, xfr, dt, ufront);
     const R FV0 = (sf ? p0p : p0) + order2 * (R(0.5) * (sf ? g0p : g0) * tf);
     const R FV1 = (sf ? p1p : p1) + order2 * (R(0.5) * (sf ? g1p : g1) * tf);
     const R uback = R(0.5) * (p0m + p0);
     const int signback = (uback > R(0)) ? 1 : -1;
     const double xbk = xd - xdm;
     const bool sb = (back > R(0));
-    const R tb = (R)(signback * xbk - dt * uback);
+    const R tb = upwind_offset<R>(signback, xbk, dt, uback);
     const R BV0 = (sb ? p0m : p0) + order2 * (R(0.5) * (sb ? g0m : g0) * tb);
     const R BV1 = (sb ? p1m : p1) + order2 * (R(0.5) * (sb ? g1m : g1) * tb);
     U0 += back * BV0 - front * FV0;
@@ -1489,14 +1489,14 @@ __device__ __forceinline__ void node_sweep_ytile_2d_body(const Params& P, DevSca
     const int signfront = (ufront > R(0)) ? 1 : -1;
     const double xfr = xdp - xd;
     const bool sf = (front < R(0));
-    const R tf = (R)(signfront * xfr - dt * ufront);
+    const R tf = upwind_offset<R>(signfront, xfr, dt, ufront);
     const R FV0 = (sf ? p0p : p0) + order2 * (R(0.5) * (sf ? g0p : g0) * tf);
     const R FV1 = (sf ? p1p : p1) + order2 * (R(0.5) * (sf ? g1p : g1) * tf);
     const R uback = R(0.5) * (p1m + p1);
     const int signback = (uback > R(0)) ? 1 : -1;
     const double xbk = xd - xdm;
     const bool sb = (back > R(0));
-    const R tb = (R)(signback * xbk - dt * uback);
+    const R tb = upwind_offset<R>(signback, xbk, dt, uback);
     const R BV0 = (sb ? p0m : p0) + order2 * (R(0.5) * (sb ? g0m : g0) * tb);
     const R BV1 = (sb ? p1m : p1) + order2 * (R(0.5) * (sb ? g1m : g1) * tb);
     d0 = back * BV0 - front * FV0;

@@ -47,6 +47,26 @@ __device__ inline double length_sqrt_2d_r(double V) {
   else return sqrt(V);
 }
 
+// Face flux factor u_f L_f dt of a cell sweep, and the upwind offset
+// s x - dt u of a dual sweep.  FP64 forms them in double and rounds once, as
+// the reference's double arithmetic does; the float form multiplies in float
+// (float-sized factors, no cancellation: |dt u| stays below the cell size).
+template <class R>
+__device__ inline R flux_factor(R u, double l, double dt) {
+  if constexpr (std::is_same<R, float>::value) return u * (float)l * (float)dt;
+  else return (R)(1.0 * u * l * dt);
+}
+template <class R>
+__device__ inline R upwind_offset(int sign, double dx, double dt, R u) {
+  if constexpr (std::is_same<R, float>::value) return (float)(sign * dx) - (float)dt * u;
+  else return (R)(sign * dx - dt * u);
+}
+template <class R>
+__device__ inline R vel_dt(R u, double dt) {
+  if constexpr (std::is_same<R, float>::value) return u * (float)dt;
+  else return (R)((double)u * dt);
+}
+
 // Limiters, RemapADIService.h:260-286.
 template <int LIM, class T = double>
 __host__ __device__ inline T limiter(T r) {

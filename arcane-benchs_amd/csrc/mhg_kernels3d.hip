@@ -424,7 +424,7 @@ __global__ __launch_bounds__(256) void k_cell_sweep_x_3d(Params P, DevScalars* S
   const double dotxf = inner ? (upm ? xf - xcm : xf - xc) : 0.;
   const R uf_f = lane_down1(uf);
   const double lf_f = lane_down1(lf);
-  const R fb = (R)(-1.0 * uf * lf * dt), ff = (R)(1.0 * uf_f * lf_f * dt);
+  const R fb = flux_factor<R>(-uf, lf, dt), ff = flux_factor<R>(uf_f, lf_f, dt);
   const R dxr = (R)dx, dotr = (R)dotxf;
   const bool o2 = P.order2 > 0;
   const int sd = i;  // index along the sweep
@@ -464,7 +464,7 @@ __global__ __launch_bounds__(256) void k_cell_sweep_x_3d(Params P, DevScalars* S
   if (LIM == kLimO3) {
     est = est_own;
     estb = lane_up1(est);
-    vdt = (R)((double)uf * dt);
+    vdt = vel_dt<R>(uf, dt);
   }
   const bool vpure = estb != -2 && est != -2 && estb == est;
   R o3V[NEM], o3Vb[NEM], o3Vf[NEM], o3M[NEM], o3Mb[NEM], o3Mf[NEM];
@@ -614,7 +614,7 @@ __global__ __launch_bounds__(1024) void k_cell_sweep_tile_3d(Params P, DevScalar
   const double dotxf = inner ? (upm ? xf - xcm : xf - xc) : 0.;
   const R uf_f = lane_down1(uf);
   const double lf_f = lane_down1(lf);
-  const R fb = (R)(-1.0 * uf * lf * dt), ff = (R)(1.0 * uf_f * lf_f * dt);
+  const R fb = flux_factor<R>(-uf, lf, dt), ff = flux_factor<R>(uf_f, lf_f, dt);
   const R dxr = (R)dx, dotr = (R)dotxf;
   const int sd = s;  // index along the sweep
   const int est_own = (LIM == kLimO3) ? (int)lds[LIM == kLimO3 ? 6 : 5][lane][w] : 0;
@@ -638,7 +638,7 @@ __global__ __launch_bounds__(1024) void k_cell_sweep_tile_3d(Params P, DevScalar
   if (LIM == kLimO3) {
     est = est_own;
     estb = lane_up1(est);
-    vdt = (R)((double)uf * dt);
+    vdt = vel_dt<R>(uf, dt);
   }
   const bool vpure = estb != -2 && est != -2 && estb == est;
   R o3V[NEM], o3Vb[NEM], o3Vf[NEM], o3M[NEM], o3Mb[NEM], o3Mf[NEM];
@@ -840,12 +840,12 @@ __device__ inline void dual_increments(const Params& P, double dt, const R* pm, 
   const int signfront = (ufront > R(0)) ? 1 : -1;
   const double xfr = xdp - xd;
   const bool sf = (front < R(0));
-  const R tf = (R)(signfront * xfr - dt * ufront);
+  const R tf = upwind_offset<R>(signfront, xfr, dt, ufront);
   const R uback = R(0.5) * (pm[D] + p0[D]);
   const int signback = (uback > R(0)) ? 1 : -1;
   const double xbk = xd - xdm;
   const bool sb = (back > R(0));
-  const R tb = (R)(signback * xbk - dt * uback);
+  const R tb = upwind_offset<R>(signback, xbk, dt, uback);
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     const R FV = (sf ? pp[q] : p0[q]) + order2 * (R(0.5) * (sf ? gp[q] : g0[q]) * tf);
