@@ -787,10 +787,16 @@ int build_params(const mhg_mesh_desc* mesh, const mhg_options* opt, Params* Pout
     // exceed that cache.  Measured, ms per step (all ordinary / cell state
     // streaming / all streaming): 1792x768 triple point FP64 (~460 MB)
     // 0.261 / 0.246 / 0.255, FP32 bulk (~250 MB) 0.120 / 0.133 / 0.149;
-    // 256^3 FP64 5.65 / 5.59 / 5.83, FP32 4.19 / 4.18 / 4.51.
-    const double wb = opt->precision == MHG_PREC_FP32_BULK ? 4. : 8.;
+    // 256^3 FP64 5.65 / 5.59 / 5.83, FP32 4.19 / 4.18 / 4.51.  In FP64 the
+    // remap vector between the first and the last cell sweep (10 words a
+    // cell on the triple point) streams as well once the fused step keeps
+    // everything else in registers: cell state / cell state and remap
+    // vector, triple point 0.2180 / 0.2159 ms, 256^3 5.469 / 5.429 ms; in
+    // FP32 it stays cached (256^3 4.115 / 4.180 ms).
+    const bool fp32 = opt->precision == MHG_PREC_FP32_BULK;
+    const double wb = fp32 ? 4. : 8.;
     const double ws = (double)P.nc * wb * (6. * opt->nb_env + 20.) + (double)P.nn * 16. * P.dim;
-    P.nt_store = ws > 320e6 ? (1 << kStCell) : 0;
+    P.nt_store = ws > 320e6 ? (fp32 ? (1 << kStCell) : ((1 << kStCell) | (1 << kStU))) : 0;
   }
   // fused y sweeps on 8-column tiles, 512-thread blocks: more blocks per CU
   // and half the wavefronts at each barrier; 1792x768 triple point 0.2355 ->
