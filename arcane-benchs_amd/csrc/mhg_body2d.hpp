@@ -1095,6 +1095,30 @@ __device__ __forceinline__ void dt_finalize_body(const Params& P, DevScalars* S,
   if (threadIdx.x == 0) *S = dt_update<ADVANCE>(P, s, m);
 }
 
+// Node spacings along a dual sweep, front and back.  FP64 (and the
+// generalised limiters, which also need the nodes two places away) forms
+// them from the Lagrangian coordinates x0 + dt u as the reference does; the
+// float form as h + dt (u' - u), in float, from the velocities along the
+// sweep (a difference of velocities: nothing cancels).
+template <class R, int LIM>
+constexpr bool kFloatSpacing = std::is_same<R, float>::value && LIM != kLimG;
+template <class R, int LIM>
+__device__ __forceinline__ void dual_spacing(double h, double dt, double xd, R ud, double* sf, double* sb,
+                                             double* xdm, double* xdp) {
+  if constexpr (kFloatSpacing<R, LIM>) {
+    const R um = lane_up1(ud), up = lane_down1(ud);
+    const R hr = (R)h, dtr = (R)dt;
+    *sf = (double)(hr + dtr * (up - ud));
+    *sb = (double)(hr + dtr * (ud - um));
+    *xdm = *xdp = 0.;
+  } else {
+    *xdm = lane_up1(xd);
+    *xdp = lane_down1(xd);
+    *sf = *xdp - xd;
+    *sb = xd - *xdm;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // K4: dual (node) sweep in direction D.  computeDualGradPhi_LimC
 // (DualPhaseRemap.cc:700-779), node mass fluxes (ncontrib 215-260), upwind
@@ -1187,7 +1211,8 @@ __device__ __forceinline__ void node_sweep_x_2d_body(const Params& P, DevScalars
   // and a sum of doubles never becomes -0, so the result is the guarded
   // sum's bit for bit).
   // the node's Lagrangian x, x0 + dt u (lag_coord)
-  const double xd_l = lag_coord<0>(P, i, dt, A.ux[nid]);
+  const R ud_l = A.ux[nid];
+  const double xd_l = lag_coord<0>(P, i, dt, ud_l);
   // generalised limiters: x of the nodes two places back and front (beyond
   // the lanes a wavefront exchanges)
   double xmm = 0., xpp = 0.;
@@ -1248,7 +1273,8 @@ __device__ __forceinline__ void node_sweep_x_2d_body(const Params& P, DevScalars
 #pragma unroll
     for (int k = 0; k < NEM; ++k)
       if (k < ne) front += cv[q] ? cf[q][k] : R(0);
-  const double xdm = lane_up1(xd), xdp = lane_down1(xd);
+  double xdm, xdp, spf, spb;  // node spacings front and back (dual_spacing)
+  dual_spacing<R, LIM>(P.h[0], dt, xd, ex ? ud_l : R(0), &spf, &spb, &xdm, &xdp);
   const R p0m = lane_up1(p0), p0p = lane_down1(p0);
   const R p1m = lane_up1(p1), p1p = lane_down1(p1);
   const R back = lane_up1(front);
@@ -1256,7 +1282,7 @@ __device__ __forceinline__ void node_sweep_x_2d_body(const Params& P, DevScalars
   R g0 = R(0), g1 = R(0);
   if (P.order2 > 0 && ex) {
     if (inner_d) {
-      const R dxf = (R)(xdp - xd), dxb = (R)(xd - xdm);
+      const R dxf = (R)spf, dxb = (R)spb;
       const R gf0 = qdiv(p0p - p0, dxf), gb0 = qdiv(p0 - p0m, dxb);
       const R gf1 = qdiv(p1p - p1, dxf), gb1 = qdiv(p1 - p1m, dxb);
       if (LIM == kLimG) {  // computeDualGradPhi, UtilesRemap.cc:15-68, 350-382
@@ -1292,14 +1318,14 @@ __device__ __forceinline__ void node_sweep_x_2d_body(const Params& P, DevScalars
     const R order2 = (R)P.order2;
     const R ufront = R(0.5) * (p0 + p0);  // DualPhaseRemap.cc:391
     const int signfront = (ufront > R(0)) ? 1 : -1;
-    const double xfr = xdp - xd;
+    const double xfr = spf;
     const bool sf = (front < R(0));
     const R tf = upwind_offset<R>(signfront, xfr, dt, ufront);
     const R FV0 = (sf ? p0p : p0) + order2 * (R(0.5) * (sf ? g0p : g0) * tf);
     const R FV1 = (sf ? p1p : p1) + order2 * (R(0.5) * (sf ? g1p : g1) * tf);
     const R uback = R(0.5) * (p0m + p0);
     const int signback = (uback > R(0)) ? 1 : -1;
-    const double xbk = xd - xdm;
+    const double xbk = spb;
     const bool sb = (back > R(0));
     const R tb = upwind_offset<R>(signback, xbk, dt, uback);
     const R BV0 = (sb ? p0m : p0) + order2 * (R(0.5) * (sb ? g0m : g0) * tb);
@@ -1365,7 +1391,8 @@ __device__ __forceinline__ void node_sweep_ytile_2d_body(const Params& P, DevSca
   {
     // every load first, from clamped indices and without branches (one wait
     // for memory; a missing cell adds +0 to a sum that starts at +0)
-    const double xd_l = lag_coord<1>(P, gs, dt, A.uy[gq]);  // the node's Lagrangian y, x0 + dt u
+    const R ud_l = A.uy[gq];
+    const double xd_l = lag_coord<1>(P, gs, dt, ud_l);  // the node's Lagrangian y, x0 + dt u
     R cf[2][NEM];
     bool cv[2];
 #pragma unroll
@@ -1418,7 +1445,7 @@ __device__ __forceinline__ void node_sweep_ytile_2d_body(const Params& P, DevSca
       for (int k = 0; k < NEM; ++k)
         if (k < ne) f += cv[q] ? cf[q][k] : R(0);
     if (trow) {
-      ldx[ra] = xd;
+      ldx[ra] = kFloatSpacing<R, LIM> ? (double)(gex ? ud_l : R(0)) : xd;  // float spacing: the velocity
       lds[0][ra] = p0;
       lds[1][ra] = p1;
       lds[2][ra] = f;
@@ -1435,7 +1462,8 @@ __device__ __forceinline__ void node_sweep_ytile_2d_body(const Params& P, DevSca
   const double xd = tlane ? ldx[ta] : 0.;
   const R p0 = tlane ? lds[0][ta] : R(0), p1 = tlane ? lds[1][ta] : R(0);
   const R front = tlane ? lds[2][ta] : R(0);
-  const double xdm = lane_up1(xd), xdp = lane_down1(xd);
+  double xdm, xdp, spf, spb;  // node spacings front and back (dual_spacing)
+  dual_spacing<R, LIM>(P.h[1], dt, xd, (R)xd, &spf, &spb, &xdm, &xdp);
   const R p0m = lane_up1(p0), p0p = lane_down1(p0);
   const R p1m = lane_up1(p1), p1p = lane_down1(p1);
   const R back = lane_up1(front);
@@ -1450,7 +1478,7 @@ __device__ __forceinline__ void node_sweep_ytile_2d_body(const Params& P, DevSca
   if (P.order2 > 0 && ex) {
     const long q = i + (long)nnx * s;
     if (inner_d) {
-      const R dxf = (R)(xdp - xd), dxb = (R)(xd - xdm);
+      const R dxf = (R)spf, dxb = (R)spb;
       const R gf0 = qdiv(p0p - p0, dxf), gb0 = qdiv(p0 - p0m, dxb);
       const R gf1 = qdiv(p1p - p1, dxf), gb1 = qdiv(p1 - p1m, dxb);
       if (LIM == kLimG) {  // computeDualGradPhi, UtilesRemap.cc:15-68, 350-382
@@ -1487,14 +1515,14 @@ __device__ __forceinline__ void node_sweep_ytile_2d_body(const Params& P, DevSca
     if (P.etot) d4 = back * (back > R(0) ? p4m : p4) - front * (front < R(0) ? p4p : p4);
     const R ufront = R(0.5) * (p1 + p1);  // DualPhaseRemap.cc:391
     const int signfront = (ufront > R(0)) ? 1 : -1;
-    const double xfr = xdp - xd;
+    const double xfr = spf;
     const bool sf = (front < R(0));
     const R tf = upwind_offset<R>(signfront, xfr, dt, ufront);
     const R FV0 = (sf ? p0p : p0) + order2 * (R(0.5) * (sf ? g0p : g0) * tf);
     const R FV1 = (sf ? p1p : p1) + order2 * (R(0.5) * (sf ? g1p : g1) * tf);
     const R uback = R(0.5) * (p1m + p1);
     const int signback = (uback > R(0)) ? 1 : -1;
-    const double xbk = xd - xdm;
+    const double xbk = spb;
     const bool sb = (back > R(0));
     const R tb = upwind_offset<R>(signback, xbk, dt, uback);
     const R BV0 = (sb ? p0m : p0) + order2 * (R(0.5) * (sb ? g0m : g0) * tb);
