@@ -294,7 +294,8 @@ int mhg_autotune(mhg_ctx* ctx, int32_t reps, char* report, int32_t report_len);
  * vector between the cell sweeps, 4 the dual fluxes and dual state, 8 node
  * velocities and coordinates; 0 all ordinary stores, which stay in the
  * caches for the next kernel, 15 all streaming; the default follows the
- * working set); "ytile_width": which y sweeps run on LDS tiles of 8 columns
+ * working set: above 320 MB, 1 in FP32-bulk and 3 in FP64, else 0);
+ * "ytile_width": which y sweeps run on LDS tiles of 8 columns
  * instead of 16, one bit per kind (1 the first cell sweep with the Lagrange
  * cell phase, 2 the last with the final cell phase, 4 the dual sweeps; the
  * default 6 in FP64, 7 in FP32-bulk)), 0 for an unknown name. */
