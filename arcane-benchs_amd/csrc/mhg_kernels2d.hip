@@ -94,11 +94,13 @@ __global__ __launch_bounds__(64 * TWC) void k_cell_sweep_y_lag_2d(Params P, DevS
       P, S, A, xcd_block(blockIdx.x, gridDim.x), nullptr, &LA);
 }
 
-// At least 8 waves per SIMD (two 1024-thread blocks per CU): the bound keeps
-// the scalar registers of both argument blocks within the 8-wave budget; the
-// unbounded kernel took 102 and ran one block per CU.
+// A bound on waves per SIMD keeps the scalar registers of both argument
+// blocks within budget (the unbounded kernel took 102 and ran one block per
+// CU): 8 in float; 7 in FP64, whose 8-wave budget of 64 VGPRs spilled 12
+// bytes per lane (0.2137 -> 0.2128-0.2134 ms per triple-point step without;
+// float at 7 ran 0.1103 -> 0.1114).
 template <class R, int NEM, int LIM, int TWC = kTileW>
-__global__ __launch_bounds__(64 * TWC, 8) void k_cell_sweep_y_final_2d(Params P, DevScalars* S,
+__global__ __launch_bounds__(64 * TWC, sizeof(R) == 8 ? 7 : 8) void k_cell_sweep_y_final_2d(Params P, DevScalars* S,
                                                                        CellSweepArgsT<R> A, CellFinalArgsT<R> F) {
   cell_sweep_ytile_2d_body<R, NEM, false, LIM, false, false, true, false, TWC>(
       P, S, A, xcd_block(blockIdx.x, gridDim.x), &F);
